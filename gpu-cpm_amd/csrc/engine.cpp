@@ -1,0 +1,502 @@
+// engine.cpp -- host engine and C ABI of the MI355X CPM engine.
+//
+// Mirrors the reference host engine class Cpm (shabaz/gpu-cpm src/cpm.h,
+// src/cpm.cu) with per-instance parameters (the reference keeps one global
+// __constant__ Settings, kernels.cu:48-53), validated inputs, error codes
+// instead of printed CUDA errors (cpm.cu:52-56), and a deterministic,
+// resumable schedule keyed by (seed, MCS counter).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/gpucpm.h"
+#include "cpm_device.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(CPM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int ilog2(long v) {
+    int k = 0;
+    while ((1L << k) < v) k++;
+    return k;
+}
+
+constexpr int kMaxScaleLog2 = 8;
+constexpr int kMaxThr = 1 << 22;
+
+}  // namespace
+
+// Host mirror of the hot-path fields of the reference Settings
+// (settings.h:4-31); J and stickiness are int16 there.
+struct HostSettings {
+    int16_t J[CPM_NTYPES][CPM_NTYPES];
+    int16_t S[CPM_NTYPES][CPM_NTYPES];
+    double lam_area[CPM_NTYPES];
+    int32_t target_area[CPM_NTYPES];
+    double lam_perim[CPM_NTYPES];
+    int32_t target_perim[CPM_NTYPES];
+    double temperature;
+};
+
+struct cpm_engine {
+    int L = 0, d = 0, n = 0, lgL = 0;
+    size_t sites = 0;
+    int active = 0;
+    bool pushed = false;
+    bool params_dirty = true;
+    uint64_t seed = 0;
+    uint64_t mcs = 0;
+    HostSettings st{};
+
+    uint32_t* h_lat = nullptr;  // pinned host mirror
+    hipStream_t stream = nullptr;
+    uint32_t* d_lat = nullptr;
+    int64_t* d_tab[2] = {nullptr, nullptr};
+    int64_t* d_del[2] = {nullptr, nullptr};
+    int32_t* d_anchor = nullptr;
+    int64_t* d_csum = nullptr;
+    uint32_t* d_first = nullptr;
+    gcpm::DevParams* d_prm = nullptr;
+    uint32_t* d_thr = nullptr;
+    int thr_cap = 0;
+    int nthr = 0;
+    int scale = 1;
+
+    ~cpm_engine() {
+        // Teardown is best effort: errors here have no caller to report to.
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipFree(d_lat);
+        for (int i = 0; i < 2; i++) {
+            (void)hipFree(d_tab[i]);
+            (void)hipFree(d_del[i]);
+        }
+        (void)hipFree(d_anchor);
+        (void)hipFree(d_csum);
+        (void)hipFree(d_first);
+        (void)hipFree(d_prm);
+        (void)hipFree(d_thr);
+        if (h_lat) (void)hipHostFree(h_lat);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Smallest dyadic scale 2^k making every lambda integral (k <= 8).
+    int energy_scale() const {
+        for (int k = 0; k <= kMaxScaleLog2; k++) {
+            const double s = (double)(1 << k);
+            bool ok = true;
+            for (int t = 0; t < CPM_NTYPES && ok; t++) {
+                for (double v : {st.lam_area[t] * s, st.lam_perim[t] * s})
+                    if (v != std::floor(v) || std::fabs(v) >= 2147483648.0) ok = false;
+            }
+            if (ok) return 1 << k;
+        }
+        return -1;
+    }
+
+    int upload_params() {
+        if (!params_dirty) return CPM_OK;
+        const int sc = energy_scale();
+        if (sc < 0)
+            return fail(CPM_EINVAL,
+                        "lambda_area / lambda_perimeter must be multiples of 2^-8 (dyadic "
+                        "fixed-point energy)");
+        gcpm::DevParams p{};
+        for (int a = 0; a < CPM_NTYPES; a++) {
+            for (int b = 0; b < CPM_NTYPES; b++) {
+                p.J[a][b] = (int32_t)st.J[a][b] * sc;
+                p.JS[a][b] = ((int32_t)st.J[a][b] + st.S[a][b]) * sc;
+            }
+            p.la[a] = (int32_t)(st.lam_area[a] * sc);
+            p.lp[a] = (int32_t)(st.lam_perim[a] * sc);
+            p.a0[a] = st.target_area[a];
+            p.p0[a] = st.target_perim[a];
+        }
+        // Acceptance thresholds thr[k] = floor(exp(-k / (scale T)) 2^32).
+        std::vector<uint32_t> thr(1, 0xFFFFFFFFu);
+        if (st.temperature > 0.0) {
+            for (int k = 1;; k++) {
+                if (k >= kMaxThr)
+                    return fail(CPM_EINVAL, "temperature * energy scale too large for the acceptance table");
+                const double v = std::floor(std::exp(-(double)k / ((double)sc * st.temperature)) * 4294967296.0);
+                if (v <= 0.0) break;
+                thr.push_back((uint32_t)v);
+            }
+        }
+        if ((int)thr.size() > thr_cap) {
+            (void)hipFree(d_thr);
+            d_thr = nullptr;
+            HIP_TRY(hipMalloc(&d_thr, thr.size() * sizeof(uint32_t)));
+            thr_cap = (int)thr.size();
+        }
+        HIP_TRY(hipMemcpyAsync(d_thr, thr.data(), thr.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_prm, &p, sizeof(p), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        nthr = (int)thr.size();
+        scale = sc;
+        params_dirty = false;
+        return CPM_OK;
+    }
+};
+
+namespace {
+
+int check_handle(cpm_handle h) {
+    if (!h) return fail(CPM_EINVAL, "null handle");
+    return CPM_OK;
+}
+
+int check_type(int t, const char* what) {
+    if (t < 0 || t >= CPM_NTYPES)
+        return fail(CPM_EINVAL, std::string(what) + " must be in [0, 20)");
+    return CPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cpm_last_error(void) { return g_err.c_str(); }
+
+const char* cpm_version(void) { return "gpucpm-mi355x 0.1 (gfx950)"; }
+
+int cpm_create(int dimension, int dimensionality, int number_of_cells, int temperature,
+               int has_field, int history, int has_act, cpm_handle* out) {
+    (void)has_field;
+    (void)has_act;
+    if (!out) return fail(CPM_EINVAL, "null output handle");
+    *out = nullptr;
+    if (dimensionality != 2 && dimensionality != 3)
+        return fail(CPM_EINVAL, "dimensionality must be 2 or 3");
+    // getNeighbor masks with dimension-1 (random.cu:22-28): powers of two only.
+    if (!is_pow2(dimension) || dimension < 2)
+        return fail(CPM_EINVAL, "dimension must be a power of two >= 2");
+    if ((long)ilog2(dimension) * dimensionality > 30)
+        return fail(CPM_EINVAL, "lattice too large (at most 2^30 sites)");
+    if (number_of_cells < 0 || number_of_cells >= (1 << 24))
+        return fail(CPM_EINVAL, "number_of_cells must be in [0, 2^24)");
+    if (history < 1) return fail(CPM_EINVAL, "history must be >= 1");
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) return fail(CPM_EHIP, "no HIP device available");
+
+    cpm_engine* g = new (std::nothrow) cpm_engine();
+    if (!g) return fail(CPM_EHIP, "host allocation failed");
+    g->L = dimension;
+    g->d = dimensionality;
+    g->n = number_of_cells;
+    g->lgL = ilog2(dimension);
+    g->sites = (size_t)1 << (g->lgL * g->d);
+    g->st.temperature = (double)temperature;
+    std::random_device rd;  // like cpm.cu:347-350, but once per engine
+    g->seed = ((uint64_t)rd() << 32) ^ rd();
+
+    auto bail = [&](hipError_t err, const char* what) {
+        int rc = hip_fail(err, what);
+        delete g;
+        return rc;
+    };
+    const size_t rows = (size_t)g->n + 1;
+    hipError_t err;
+    if ((err = hipStreamCreate(&g->stream)) != hipSuccess) return bail(err, "hipStreamCreate");
+    if ((err = hipHostMalloc((void**)&g->h_lat, g->sites * 4, hipHostMallocDefault)) != hipSuccess)
+        return bail(err, "hipHostMalloc(lattice)");
+    std::memset(g->h_lat, 0, g->sites * 4);
+    if ((err = hipMalloc(&g->d_lat, g->sites * 4)) != hipSuccess) return bail(err, "hipMalloc(lattice)");
+    for (int i = 0; i < 2; i++) {
+        if ((err = hipMalloc(&g->d_tab[i], rows * 8)) != hipSuccess) return bail(err, "hipMalloc(tables)");
+        if ((err = hipMalloc(&g->d_del[i], rows * 8)) != hipSuccess) return bail(err, "hipMalloc(deltas)");
+    }
+    if ((err = hipMalloc(&g->d_anchor, rows * 3 * 4)) != hipSuccess) return bail(err, "hipMalloc(anchor)");
+    if ((err = hipMalloc(&g->d_csum, rows * 3 * 8)) != hipSuccess) return bail(err, "hipMalloc(csum)");
+    if ((err = hipMalloc(&g->d_first, rows * 4)) != hipSuccess) return bail(err, "hipMalloc(first)");
+    if ((err = hipMalloc(&g->d_prm, sizeof(gcpm::DevParams))) != hipSuccess) return bail(err, "hipMalloc(params)");
+    *out = g;
+    return CPM_OK;
+}
+
+int cpm_destroy(cpm_handle h) {
+    delete h;
+    return CPM_OK;
+}
+
+int cpm_set_temperature(cpm_handle h, double temperature) {
+    if (int rc = check_handle(h)) return rc;
+    if (!(temperature >= 0.0) || !std::isfinite(temperature))
+        return fail(CPM_EINVAL, "temperature must be finite and >= 0");
+    h->st.temperature = temperature;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_set_area(cpm_handle h, int type, int target, double lambda) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    h->st.lam_area[type] = lambda;
+    h->st.target_area[type] = target;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_set_perimeter(cpm_handle h, int type, int target, double lambda) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    h->st.lam_perim[type] = lambda;
+    h->st.target_perim[type] = target;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_set_adhesion(cpm_handle h, int type, int other_type, int adhesion, int set_both) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (int rc = check_type(other_type, "other_cell_type")) return rc;
+    h->st.J[type][other_type] = (int16_t)adhesion;  // short, settings.h:20
+    if (set_both) h->st.J[other_type][type] = (int16_t)adhesion;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (int rc = check_type(other_type, "other_cell_type")) return rc;
+    h->st.S[type][other_type] = (int16_t)stickiness;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_set_fixed(cpm_handle h, int type, int fixed) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (fixed) return fail(CPM_ENOTIMPL, "fixed cell types are not implemented");
+    return CPM_OK;
+}
+
+int cpm_add_cell(cpm_handle h, int type, int x, int y, int z) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell type")) return rc;
+    if (h->active >= h->n)
+        return fail(CPM_EINVAL, "add_cell: more cells than number_of_cells");
+    const int L = h->L;
+    if (x < 0 || x >= L || y < 0 || y >= L) return fail(CPM_EINVAL, "add_cell: position out of range");
+    size_t idx;
+    if (h->d == 3) {
+        if (z < 0 || z >= L) return fail(CPM_EINVAL, "add_cell: z out of range");
+        idx = ((size_t)z * L + y) * L + x;
+    } else {
+        idx = (size_t)y * L + x;
+    }
+    h->active++;
+    h->h_lat[idx] = (uint32_t)h->active | ((uint32_t)type << 24);
+    return CPM_OK;
+}
+
+int cpm_host_lattice(cpm_handle h, uint32_t** out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(CPM_EINVAL, "null output");
+    *out = h->h_lat;
+    return CPM_OK;
+}
+
+int cpm_active_cells(cpm_handle h, int* out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(CPM_EINVAL, "null output");
+    *out = h->active;
+    return CPM_OK;
+}
+
+int cpm_push(cpm_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    for (size_t i = 0; i < h->sites; i++) {
+        const uint32_t v = h->h_lat[i];
+        if ((v & gcpm::kSpinMask) > (uint32_t)h->n)
+            return fail(CPM_EINVAL, "lattice holds a cell number above number_of_cells");
+        if ((v >> 24) >= (uint32_t)CPM_NTYPES)
+            return fail(CPM_EINVAL, "lattice holds a cell type >= 20");
+    }
+    const size_t rows = (size_t)h->n + 1;
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(h->d_lat, h->h_lat, h->sites * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
+    HIP_TRY(hipMemsetAsync(h->d_del[0], 0, rows * 8, s));
+    HIP_TRY(hipMemsetAsync(h->d_del[1], 0, rows * 8, s));
+    HIP_TRY(hipMemsetAsync(h->d_csum, 0, rows * 3 * 8, s));
+    HIP_TRY(hipMemsetAsync(h->d_first, 0xFF, rows * 4, s));
+    HIP_TRY(gcpm::launch_tally(h->d, h->d_lat, h->lgL, h->n, h->d_tab[0], h->d_first, h->d_anchor,
+                               h->d_csum, s));
+    HIP_TRY(hipMemcpyAsync(h->d_tab[1], h->d_tab[0], rows * 8, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    h->pushed = true;
+    return CPM_OK;
+}
+
+int cpm_run(cpm_handle h, const cpm_run_args* a) {
+    if (int rc = check_handle(h)) return rc;
+    if (!a) return fail(CPM_EINVAL, "null run arguments");
+    if (!h->pushed) return fail(CPM_ESTATE, "run before push_to_gpu");
+    if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync (writeAreas lock) is not implemented");
+    if (a->threads_per_block < 1 || a->positions_per_thread < 1 || a->updates_per_barrier < 1)
+        return fail(CPM_EINVAL, "threads_per_block, positions_per_thread and updates_per_barrier must be >= 1");
+    const int c = a->positions_per_checkerboard;
+    if (!is_pow2(c) || h->L % (2 * c))
+        return fail(CPM_EINVAL, "positions_per_checkerboard must be a power of two with 2*c dividing dimension");
+    const long cd = (long)c * c * (h->d == 3 ? c : 1);
+    const int u = a->updates_per_checkerboard_switch;
+    if (u < 1 || u > cd)
+        return fail(CPM_EINVAL, "updates_per_checkerboard_switch must be in [1, c^d]");
+    if (a->inner_iterations < 1) return fail(CPM_EINVAL, "inner_iterations must be >= 1");
+    if (a->iterations < 0) return fail(CPM_EINVAL, "iterations must be >= 0");
+    if (int rc = h->upload_params()) return rc;
+
+    // cpm.cu:405: iterations / inner_iterations launches of inner_iterations MCS.
+    const long total = (long)(a->iterations / a->inner_iterations) * a->inner_iterations;
+    const int rounds = (int)(cd / u);  // kernels2d.cu:95
+    const int ncol = 1 << h->d;
+    const int lc = ilog2(c);
+    const int lh = h->lgL - lc - 1;
+    gcpm::PhaseArgs pa{};
+    pa.lat = h->d_lat;
+    pa.anchor = h->d_anchor;
+    pa.csum = h->d_csum;
+    pa.prm = h->d_prm;
+    pa.thr = h->d_thr;
+    pa.nthr = h->nthr;
+    pa.ncells = h->n;
+    pa.lgL = h->lgL;
+    pa.lc = lc;
+    pa.lh = lh;
+    pa.u = u;
+    pa.k0 = (uint32_t)h->seed;
+    pa.k1 = (uint32_t)(h->seed >> 32);
+    pa.nsub = 1 << (lh * h->d);
+    uint64_t g = 0;  // phase counter since the tables were last settled
+    for (long m = 0; m < total; m++) {
+        pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
+        for (int r = 0; r < rounds; r++) {
+            for (int col = 0; col < ncol; col++, g++) {
+                pa.cx = h->d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
+                pa.cy = h->d == 3 ? (col >> 1) & 1 : col & 1;
+                pa.cz = h->d == 3 ? col & 1 : 0;
+                pa.phase = (uint32_t)(r * ncol + col);
+                pa.R = h->d_tab[g & 1];
+                pa.W = h->d_tab[(g + 1) & 1];
+                pa.Dcur = h->d_del[g & 1];
+                pa.Dprev = h->d_del[(g + 1) & 1];
+                HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream));
+            }
+        }
+    }
+    if (g > 0) {
+        // Fold the last phase's deltas into the lagging buffer: both table
+        // buffers then hold the current state and both delta buffers are zero.
+        HIP_TRY(gcpm::launch_settle(h->d_tab[(g + 1) & 1], h->d_del[(g + 1) & 1], h->n, h->stream));
+    }
+    h->mcs += (uint64_t)total;
+    return CPM_OK;
+}
+
+int cpm_sync(cpm_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CPM_OK;
+}
+
+int cpm_pull(cpm_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipMemcpyAsync(h->h_lat, h->d_lat, h->sites * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CPM_OK;
+}
+
+int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anchor, int64_t* csum) {
+    if (int rc = check_handle(h)) return rc;
+    if (!h->pushed) return fail(CPM_ESTATE, "cell tables exist only after push_to_gpu");
+    const size_t rows = (size_t)h->n + 1;
+    std::vector<int64_t> tab(rows);
+    HIP_TRY(hipMemcpyAsync(tab.data(), h->d_tab[0], rows * 8, hipMemcpyDeviceToHost, h->stream));
+    if (anchor) HIP_TRY(hipMemcpyAsync(anchor, h->d_anchor, rows * 3 * 4, hipMemcpyDeviceToHost, h->stream));
+    if (csum) HIP_TRY(hipMemcpyAsync(csum, h->d_csum, rows * 3 * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t c = 0; c < rows; c++) {
+        if (area) area[c] = (int32_t)(uint32_t)tab[c];
+        if (perim) perim[c] = (int32_t)(tab[c] >> 32);
+    }
+    return CPM_OK;
+}
+
+int cpm_get_centroids(cpm_handle h, double* out, int capacity, int* rows) {
+    if (int rc = check_handle(h)) return rc;
+    if (rows) *rows = h->active;
+    if (!h->pushed) return fail(CPM_ESTATE, "centroids exist only after push_to_gpu");
+    if (capacity < h->active || (!out && h->active)) return fail(CPM_EINVAL, "output too small");
+    const size_t nr = (size_t)h->n + 1;
+    std::vector<int32_t> area(nr), anchor(nr * 3);
+    std::vector<int64_t> csum(nr * 3);
+    if (int rc = cpm_get_cell_table(h, area.data(), nullptr, anchor.data(), csum.data())) return rc;
+    // cpm.cu:68-110 reports sum/count per axis for cells 1..activeCells.
+    for (int c = 1; c <= h->active; c++) {
+        for (int ax = 0; ax < h->d; ax++) {
+            double v;
+            if (!area[c]) {
+                v = NAN;
+            } else {
+                v = (double)anchor[3 * c + ax] + (double)csum[3 * c + ax] / (double)area[c];
+                if (v < 0.0) v += h->L;
+                if (v >= h->L) v -= h->L;
+            }
+            out[(size_t)(c - 1) * h->d + ax] = v;
+        }
+    }
+    return CPM_OK;
+}
+
+int cpm_set_seed(cpm_handle h, uint64_t seed) {
+    if (int rc = check_handle(h)) return rc;
+    h->seed = seed;
+    return CPM_OK;
+}
+
+int cpm_get_seed(cpm_handle h, uint64_t* seed) {
+    if (int rc = check_handle(h)) return rc;
+    if (!seed) return fail(CPM_EINVAL, "null output");
+    *seed = h->seed;
+    return CPM_OK;
+}
+
+int cpm_set_mcs(cpm_handle h, uint64_t mcs) {
+    if (int rc = check_handle(h)) return rc;
+    h->mcs = mcs;
+    return CPM_OK;
+}
+
+int cpm_get_mcs(cpm_handle h, uint64_t* mcs) {
+    if (int rc = check_handle(h)) return rc;
+    if (!mcs) return fail(CPM_EINVAL, "null output");
+    *mcs = h->mcs;
+    return CPM_OK;
+}
+
+}  // extern "C"

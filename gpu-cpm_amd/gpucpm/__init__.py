@@ -1,0 +1,262 @@
+"""gpucpm -- MI355X Cellular Potts engine, drop-in for the reference module.
+
+``gpucpm.Cpm`` keeps the reference's Python surface (shabaz/gpu-cpm
+src/python_wrapper.cpp:27-380): constructor, ``set_constraints``,
+``add_cell``, ``push_to_gpu``, ``run``, ``synchronize``, ``pull_from_gpu``,
+``get_state``, ``get_centroids`` -- same names, positional/keyword rules,
+defaults and dtypes.  Every call goes through the C ABI of libgpucpm.so
+(include/gpucpm.h), whose sweep runs as HIP kernels on the GPU.
+
+Extensions: ``run(..., seed=...)``, ``set_seed``, ``seed``, ``mcs``,
+``get_cell_table``.
+"""
+from __future__ import annotations
+
+import ctypes
+import operator
+
+import numpy as np
+
+from . import _lib
+from ._lib import RunArgs, check, lib
+
+__all__ = ["Cpm", "RunArgs"]
+
+_UNSET = -1
+
+
+def _int(v, name):
+    """PyArg 'i' conversion: integers (anything with __index__) only."""
+    try:
+        return operator.index(v)
+    except TypeError:
+        raise TypeError(f"{name}: an integer is required (got {type(v).__name__})") from None
+
+
+def _float(v, name):
+    """PyArg 'd' conversion."""
+    try:
+        return float(v)
+    except (TypeError, ValueError):
+        raise TypeError(f"{name}: must be real number, not {type(v).__name__}") from None
+
+
+class Cpm:
+    """One Cellular Potts simulation (reference: class Cpm, src/cpm.h:78-112).
+
+    ``Cpm(dimension, dimensionality, number_of_cells, temperature, has_field
+    [, history=1, has_act=True])`` -- positional only, as PyCpm_init parses
+    "iiiip|ip" (python_wrapper.cpp:35).
+    """
+
+    def __init__(self, dimension, dimensionality, number_of_cells, temperature, has_field,
+                 history=1, has_act=True, /):
+        self._h = None
+        self.dimension = _int(dimension, "dimension")
+        self.dimensionality = _int(dimensionality, "dimensionality")
+        self.number_of_cells = _int(number_of_cells, "number_of_cells")
+        temperature = _int(temperature, "temperature")
+        self._has_field = bool(has_field)
+        history = _int(history, "history")
+        self._has_act = bool(has_act)
+        h = ctypes.c_void_p()
+        check(lib.cpm_create(self.dimension, self.dimensionality, self.number_of_cells,
+                             temperature, int(self._has_field), history, int(self._has_act),
+                             ctypes.byref(h)))
+        self._h = h
+        self._field = None
+        if self._has_field:
+            shape = (self.dimensionality,) + (self.dimension,) * self.dimensionality
+            self._field = np.zeros(shape, np.float64)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h is not None and h.value and lib is not None:
+            lib.cpm_destroy(h)
+
+    # -------------------------------------------------------------- setup --
+    def set_constraints(self, cell_type, *, other_cell_type=_UNSET, lambda_perimeter=-1.0,
+                        target_perimeter=_UNSET, lambda_area=-1.0, target_area=_UNSET,
+                        lambda_act=-1.0, max_act=_UNSET, adhesion=_UNSET, fixed=_UNSET,
+                        lambda_chemotaxis=-1.0, lambda_persistence=-1.0,
+                        persistence_diffusion=-1.0, persistence_time=_UNSET,
+                        set_adhesion_symmetric=1, stickiness=_UNSET):
+        """Set Hamiltonian constraints of one cell type (python_wrapper.cpp:55-129).
+
+        -1 means "not given"; area, perimeter, act, adhesion and stickiness
+        are applied only when both halves of their pair are given.  Adhesion
+        is always written symmetrically: the reference parses
+        ``set_adhesion_symmetric`` as an int into a double that is never 0
+        (python_wrapper.cpp:81-82), so J[a][b] and J[b][a] are both set.
+        """
+        ct = _int(cell_type, "cell_type")
+        oct_ = _int(other_cell_type, "other_cell_type")
+        lp, tp = _float(lambda_perimeter, "lambda_perimeter"), _int(target_perimeter, "target_perimeter")
+        la, ta = _float(lambda_area, "lambda_area"), _int(target_area, "target_area")
+        lact, mact = _float(lambda_act, "lambda_act"), _int(max_act, "max_act")
+        adh, fx = _int(adhesion, "adhesion"), _int(fixed, "fixed")
+        lchem = _float(lambda_chemotaxis, "lambda_chemotaxis")
+        lpers = _float(lambda_persistence, "lambda_persistence")
+        pdiff = _float(persistence_diffusion, "persistence_diffusion")
+        ptime = _int(persistence_time, "persistence_time")
+        _int(set_adhesion_symmetric, "set_adhesion_symmetric")
+        stick = _int(stickiness, "stickiness")
+
+        # Terms outside the implemented Hamiltonian are refused, not ignored.
+        if fx > 0:
+            raise NotImplementedError("fixed cell types are not implemented")
+        if lpers != -1 and pdiff != -1 and ptime != -1 and lpers != 0:
+            raise NotImplementedError("the persistence term is not implemented")
+        if lchem != -1 and lchem != 0:
+            raise NotImplementedError("the chemotaxis term is not implemented")
+        if lact != -1 and mact != -1 and lact != 0:
+            raise NotImplementedError("the act term is not implemented")
+
+        if fx >= 0:
+            check(lib.cpm_set_fixed(self._h, ct, fx))
+        if lp != -1 and tp != -1:
+            check(lib.cpm_set_perimeter(self._h, ct, tp, lp))
+        if la != -1 and ta != -1:
+            check(lib.cpm_set_area(self._h, ct, ta, la))
+        if oct_ != -1 and adh != -1:
+            check(lib.cpm_set_adhesion(self._h, ct, oct_, adh, 1))
+        if oct_ != -1 and stick != -1:
+            check(lib.cpm_set_stickiness(self._h, ct, oct_, stick))
+        return None
+
+    def add_cell(self, type, x, y, z=-1, /):
+        """Place a one-site cell of ``type`` at (x, y[, z]) in the host lattice
+        (python_wrapper.cpp:131-148, cpm.cu:160-174)."""
+        t, x, y, z = _int(type, "type"), _int(x, "x"), _int(y, "y"), _int(z, "z")
+        if self.dimensionality == 3 and z == -1:
+            z = 0  # the 2D form indexes y*L + x, i.e. the z = 0 plane
+        if self.dimensionality == 2:
+            if z not in (-1, 0):
+                raise ValueError("add_cell: z given for a 2D lattice")
+            z = -1
+        check(lib.cpm_add_cell(self._h, t, x, y, z))
+        return None
+
+    # ------------------------------------------------------------ transfer --
+    def push_to_gpu(self):
+        """Copy the host lattice to the GPU and rebuild the cell tables
+        (cpm.cu:541-597)."""
+        check(lib.cpm_push(self._h))
+
+    def pull_from_gpu(self):
+        """Copy the GPU lattice into the host lattice (cpm.cu:531-540)."""
+        check(lib.cpm_pull(self._h))
+
+    def synchronize(self):
+        """Wait for queued GPU work (cpm.cu:653-655)."""
+        check(lib.cpm_sync(self._h))
+
+    # ----------------------------------------------------------------- run --
+    def run(self, cell_sync, block_sync, global_sync, threads_per_block, positions_per_thread,
+            positions_per_checkerboard, updates_per_checkerboard_switch, updates_per_barrier,
+            iterations, inner_iterations, shared=0, partial_dispatch=0, centroid_tracking=False,
+            *, seed=None):
+        """Run (iterations // inner_iterations) * inner_iterations MCS
+        (python_wrapper.cpp:182-227, cpm.cu:291-517).
+
+        The checkerboard sub-block edge is ``positions_per_checkerboard`` and
+        each sub-block makes ``updates_per_checkerboard_switch`` attempts per
+        colour phase.  The other schedule arguments choose a thread mapping
+        in the reference and do not change results here.  ``seed`` (an
+        extension) sets the RNG key before running.
+        """
+        a = RunArgs()
+        vals = dict(cell_sync=cell_sync, block_sync=block_sync, global_sync=global_sync,
+                    threads_per_block=threads_per_block, positions_per_thread=positions_per_thread,
+                    positions_per_checkerboard=positions_per_checkerboard,
+                    updates_per_checkerboard_switch=updates_per_checkerboard_switch,
+                    updates_per_barrier=updates_per_barrier, iterations=iterations,
+                    inner_iterations=inner_iterations, shared=shared,
+                    partial_dispatch=partial_dispatch)
+        for k, v in vals.items():
+            setattr(a, k, _int(v, k))
+        a.centroid_tracking = int(bool(centroid_tracking))
+        if seed is not None:
+            self.set_seed(seed)
+        check(lib.cpm_run(self._h, ctypes.byref(a)))
+        return None
+
+    # ------------------------------------------------------------ readback --
+    def get_state(self):
+        """Writable int32 view of the host lattice, shape (L,)*d, indexed
+        [y][x] / [z][y][x] (python_wrapper.cpp:292-308).  The view keeps this
+        Cpm alive."""
+        p = ctypes.POINTER(ctypes.c_uint32)()
+        check(lib.cpm_host_lattice(self._h, ctypes.byref(p)))
+        n = self.dimension ** self.dimensionality
+        buf = (ctypes.c_int32 * n).from_address(ctypes.addressof(p.contents))
+        buf._owner = self
+        return np.ctypeslib.as_array(buf).reshape((self.dimension,) * self.dimensionality)
+
+    def get_centroids(self):
+        """float64 (active_cells, d) centroids [x, y(, z)] of the cells added
+        with add_cell (python_wrapper.cpp:151-177, cpm.cu:68-110)."""
+        n = ctypes.c_int()
+        check(lib.cpm_active_cells(self._h, ctypes.byref(n)))
+        out = np.zeros((n.value, self.dimensionality), np.float64)
+        rows = ctypes.c_int()
+        check(lib.cpm_get_centroids(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    n.value, ctypes.byref(rows)))
+        return out
+
+    def get_field(self):
+        """float64 (d, L, ...) host field array (python_wrapper.cpp:274-290).
+        It is host data only: the chemotaxis term that reads it is not
+        implemented."""
+        if self._field is None:
+            raise ValueError("get_field: this Cpm was created with has_field=False")
+        return self._field
+
+    def get_act_state(self):
+        """Not implemented: the act lattice belongs to the act term."""
+        raise NotImplementedError("the act lattice (act term) is not implemented")
+
+    # ---------------------------------------------------------- extensions --
+    def get_cell_table(self):
+        """All n+1 rows of the cell tables: area, perimeter (Moore/26 count
+        of unlike neighbours), anchor and csum (integer centroid state) and
+        centroid (float64 [x, y(, z)], NaN for empty rows)."""
+        rows = self.number_of_cells + 1
+        area = np.zeros(rows, np.int32)
+        perim = np.zeros(rows, np.int32)
+        anchor = np.zeros(rows * 3, np.int32)
+        csum = np.zeros(rows * 3, np.int64)
+        P = ctypes.POINTER
+        check(lib.cpm_get_cell_table(self._h, area.ctypes.data_as(P(ctypes.c_int32)),
+                                     perim.ctypes.data_as(P(ctypes.c_int32)),
+                                     anchor.ctypes.data_as(P(ctypes.c_int32)),
+                                     csum.ctypes.data_as(P(ctypes.c_int64))))
+        d, L = self.dimensionality, self.dimension
+        cen = np.full((rows, d), np.nan)
+        alive = area > 0
+        for ax in range(d):
+            v = anchor[ax::3][alive] + csum[ax::3][alive] / area[alive]
+            v = np.where(v < 0, v + L, v)
+            v = np.where(v >= L, v - L, v)
+            cen[alive, ax] = v
+        return {"area": area, "perimeter": perim, "anchor": anchor.reshape(rows, 3),
+                "csum": csum.reshape(rows, 3), "centroid": cen}
+
+    def set_seed(self, seed):
+        check(lib.cpm_set_seed(self._h, _int(seed, "seed") & (2**64 - 1)))
+
+    @property
+    def seed(self):
+        v = ctypes.c_uint64()
+        check(lib.cpm_get_seed(self._h, ctypes.byref(v)))
+        return v.value
+
+    @property
+    def mcs(self):
+        v = ctypes.c_uint64()
+        check(lib.cpm_get_mcs(self._h, ctypes.byref(v)))
+        return v.value
+
+    @mcs.setter
+    def mcs(self, value):
+        check(lib.cpm_set_mcs(self._h, _int(value, "mcs")))

@@ -1,0 +1,84 @@
+"""ctypes binding of libgpucpm.so (the C ABI declared in include/gpucpm.h).
+
+The library is required: importing this module raises ImportError when the
+HIP build is missing, so no CPU fallback can stand in for it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libgpucpm.so")
+
+CPM_OK = 0
+CPM_EINVAL = -1
+CPM_EHIP = -2
+CPM_ESTATE = -3
+CPM_ENOTIMPL = -4
+NTYPES = 20
+
+
+class RunArgs(ctypes.Structure):
+    """Mirror of cpm_run_args (keyword arguments of Cpm.run)."""
+
+    _fields_ = [(name, ctypes.c_int) for name in (
+        "cell_sync", "block_sync", "global_sync", "threads_per_block",
+        "positions_per_thread", "positions_per_checkerboard",
+        "updates_per_checkerboard_switch", "updates_per_barrier", "iterations",
+        "inner_iterations", "shared", "partial_dispatch", "centroid_tracking")]
+
+
+# (name, restype, argtypes) of every entry point in include/gpucpm.h
+_H = ctypes.c_void_p
+_I = ctypes.c_int
+_P = ctypes.POINTER
+SIGNATURES = [
+    ("cpm_last_error", ctypes.c_char_p, []),
+    ("cpm_version", ctypes.c_char_p, []),
+    ("cpm_create", _I, [_I, _I, _I, _I, _I, _I, _I, _P(_H)]),
+    ("cpm_destroy", _I, [_H]),
+    ("cpm_set_temperature", _I, [_H, ctypes.c_double]),
+    ("cpm_set_area", _I, [_H, _I, _I, ctypes.c_double]),
+    ("cpm_set_perimeter", _I, [_H, _I, _I, ctypes.c_double]),
+    ("cpm_set_adhesion", _I, [_H, _I, _I, _I, _I]),
+    ("cpm_set_stickiness", _I, [_H, _I, _I, _I]),
+    ("cpm_set_fixed", _I, [_H, _I, _I]),
+    ("cpm_add_cell", _I, [_H, _I, _I, _I, _I]),
+    ("cpm_host_lattice", _I, [_H, _P(_P(ctypes.c_uint32))]),
+    ("cpm_active_cells", _I, [_H, _P(_I)]),
+    ("cpm_push", _I, [_H]),
+    ("cpm_run", _I, [_H, _P(RunArgs)]),
+    ("cpm_sync", _I, [_H]),
+    ("cpm_pull", _I, [_H]),
+    ("cpm_get_centroids", _I, [_H, _P(ctypes.c_double), _I, _P(_I)]),
+    ("cpm_get_cell_table", _I, [_H, _P(ctypes.c_int32), _P(ctypes.c_int32), _P(ctypes.c_int32),
+                                _P(ctypes.c_int64)]),
+    ("cpm_set_seed", _I, [_H, ctypes.c_uint64]),
+    ("cpm_get_seed", _I, [_H, _P(ctypes.c_uint64)]),
+    ("cpm_set_mcs", _I, [_H, ctypes.c_uint64]),
+    ("cpm_get_mcs", _I, [_H, _P(ctypes.c_uint64)]),
+]
+
+if not os.path.exists(LIBPATH):
+    raise ImportError(
+        f"gpucpm: {LIBPATH} is missing; build it with `python gpu-cpm_amd/build.py` "
+        "(HIP for gfx950).  There is no CPU fallback.")
+
+lib = ctypes.CDLL(LIBPATH)
+for _name, _res, _args in SIGNATURES:
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def check(rc: int) -> None:
+    """Raise the Python exception matching a negative CPM_E* code."""
+    if rc == CPM_OK:
+        return
+    msg = (lib.cpm_last_error() or b"").decode()
+    if rc == CPM_EINVAL:
+        raise ValueError(msg)
+    if rc == CPM_ENOTIMPL:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)

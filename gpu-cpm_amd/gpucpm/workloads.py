@@ -1,0 +1,129 @@
+"""Synthetic workloads: the benchmark configurations as lattices + constraints.
+
+Each builder returns a ``Workload``: the initial lattice (uint32, ids
+``cell | type << 24``), the number of cells, the temperature and the list of
+``set_constraints`` keyword sets that define the Hamiltonian.  Placement uses
+``numpy.random.default_rng(seed)``.
+
+- ``plain2d``    256^2, 100 cells (5x5 squares) of one type, area constraint.
+- ``sorting2d``  1024^2, 63x63 squares of 10x10 tiling the centre, two types,
+                 area + perimeter constraints, differential adhesion.
+- ``plain3d``    256^3, 20000 cubes of 5^3 at random non-overlapping positions,
+                 volume + surface constraints, 26-neighbourhood.
+The shapes may be scaled down (``L`` / ``ncells`` arguments) for tests.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    L: int
+    d: int
+    ncells: int
+    temperature: int
+    lattice: np.ndarray
+    constraints: list = field(default_factory=list)
+
+    def make_cpm(self, cls=None):
+        """Build a Cpm with this workload's lattice and constraints (not pushed)."""
+        if cls is None:
+            from . import Cpm as cls
+        sim = cls(self.L, self.d, self.ncells, self.temperature, False)
+        for kw in self.constraints:
+            kw = dict(kw)
+            sim.set_constraints(kw.pop("cell_type"), **kw)
+        sim.get_state()[...] = self.lattice.view(np.int32)
+        return sim
+
+    def params(self):
+        """Dense parameter arrays (J, lam_area, target_area, lam_perim, target_perim)
+        as the constraints define them (adhesion written symmetrically)."""
+        J = np.zeros((20, 20), np.int64)
+        la = np.zeros(20)
+        a0 = np.zeros(20, np.int64)
+        lp = np.zeros(20)
+        p0 = np.zeros(20, np.int64)
+        S = np.zeros((20, 20), np.int64)
+        for kw in self.constraints:
+            t = kw["cell_type"]
+            if "adhesion" in kw:
+                o = kw["other_cell_type"]
+                J[t, o] = J[o, t] = kw["adhesion"]
+            if "stickiness" in kw:
+                S[t, kw["other_cell_type"]] = kw["stickiness"]
+            if "lambda_area" in kw:
+                la[t], a0[t] = kw["lambda_area"], kw["target_area"]
+            if "lambda_perimeter" in kw:
+                lp[t], p0[t] = kw["lambda_perimeter"], kw["target_perimeter"]
+        return dict(J=J, S=S, lam_area=la, target_area=a0, lam_perim=lp, target_perim=p0,
+                    temperature=float(self.temperature))
+
+
+def _place_boxes(L, d, n, edge, rng, types):
+    lat = np.zeros((L,) * d, np.uint32)
+    placed = 0
+    tries = 0
+    while placed < n:
+        tries += 1
+        if tries > 1000 * n + 1000:
+            raise RuntimeError("could not place all boxes")
+        corner = rng.integers(0, L - edge + 1, size=d)
+        sl = tuple(slice(int(c), int(c) + edge) for c in corner[::-1])  # [z][y][x]
+        if lat[sl].any():
+            continue
+        placed += 1
+        lat[sl] = (np.uint32(types[placed - 1]) << np.uint32(24)) | np.uint32(placed)
+    return lat
+
+
+def plain2d(L=256, ncells=100, seed=1):
+    """256^2, 100 type-1 cells as 5x5 squares; A0=50, lam_A=50, J(1,0)=20,
+    J(1,1)=40, T=20."""
+    rng = np.random.default_rng(seed)
+    lat = _place_boxes(L, 2, ncells, 5, rng, [1] * ncells)
+    cons = [dict(cell_type=1, lambda_area=50, target_area=50),
+            dict(cell_type=1, other_cell_type=0, adhesion=20),
+            dict(cell_type=1, other_cell_type=1, adhesion=40)]
+    return Workload("plain2d", L, 2, ncells, 20, lat, cons)
+
+
+def sorting2d(L=1024, grid=63, edge=10, seed=2, J_AB=11, temperature=10):
+    """Two-type cell sorting: grid x grid squares of edge x edge tiling the
+    centre; types 1/2 Bernoulli(1/2); A0=100, lam_A=25, P0=40, lam_P=2;
+    J_m1=J_m2=16, J_11=2, J_22=14, J_12=J_AB, T=10."""
+    rng = np.random.default_rng(seed)
+    n = grid * grid
+    types = rng.integers(1, 3, size=n)
+    lat = np.zeros((L, L), np.uint32)
+    off = (L - grid * edge) // 2
+    for gy in range(grid):
+        for gx in range(grid):
+            k = gy * grid + gx
+            y0, x0 = off + gy * edge, off + gx * edge
+            lat[y0:y0 + edge, x0:x0 + edge] = (np.uint32(types[k]) << np.uint32(24)) | np.uint32(k + 1)
+    cons = []
+    for t in (1, 2):
+        cons.append(dict(cell_type=t, lambda_area=25, target_area=100))
+        cons.append(dict(cell_type=t, lambda_perimeter=2, target_perimeter=40))
+        cons.append(dict(cell_type=t, other_cell_type=0, adhesion=16))
+    cons.append(dict(cell_type=1, other_cell_type=1, adhesion=2))
+    cons.append(dict(cell_type=2, other_cell_type=2, adhesion=14))
+    cons.append(dict(cell_type=1, other_cell_type=2, adhesion=int(J_AB)))
+    return Workload("sorting2d", L, 2, n, int(temperature), lat, cons)
+
+
+def plain3d(L=256, ncells=20000, seed=3):
+    """256^3, 20000 type-1 cubes of 5^3; V0=150, lam_V=25, S0=150, lam_S=1,
+    J(1,0)=10, J(1,1)=20, T=20."""
+    rng = np.random.default_rng(seed)
+    lat = _place_boxes(L, 3, ncells, 5, rng, [1] * ncells)
+    cons = [dict(cell_type=1, lambda_area=25, target_area=150),
+            dict(cell_type=1, lambda_perimeter=1, target_perimeter=150),
+            dict(cell_type=1, other_cell_type=0, adhesion=10),
+            dict(cell_type=1, other_cell_type=1, adhesion=20)]
+    return Workload("plain3d", L, 3, ncells, 20, lat, cons)

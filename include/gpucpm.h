@@ -1,0 +1,135 @@
+/*
+ * gpucpm.h -- C ABI of the MI355X Cellular Potts engine (libgpucpm.so).
+ *
+ * Each entry point replaces one method of the reference's host engine
+ * (shabaz/gpu-cpm src/cpm.h:78-112, src/cpm.cu) as reached through its
+ * CPython binding (src/python_wrapper.cpp).  Plain C types only: no torch,
+ * no HIP types.  Every call returns 0 on success or a negative CPM_E* code;
+ * cpm_last_error() then holds a message (per thread).  One engine per
+ * handle; a handle is not thread-safe.  The engine owns all memory;
+ * outputs go to caller buffers.
+ */
+#ifndef GPUCPM_H
+#define GPUCPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPM_OK 0
+#define CPM_EINVAL -1   /* bad argument (shape, type, schedule, ...) */
+#define CPM_EHIP -2     /* HIP runtime error (no device, out of memory, ...) */
+#define CPM_ESTATE -3   /* call out of order (e.g. run before push) */
+#define CPM_ENOTIMPL -4 /* accepted by the reference, not implemented here */
+
+#define CPM_NTYPES 20   /* settings.h:4 numTypes */
+
+typedef struct cpm_engine* cpm_handle;
+
+/* Keyword arguments of Cpm.run (python_wrapper.cpp:186-216, cpm.cu:291-296).
+ * Only positions_per_checkerboard, updates_per_checkerboard_switch,
+ * iterations and inner_iterations change results; the rest select a thread
+ * mapping in the reference and are validated but do not change results
+ * here.  seed_set/seed and the schedule fields are extensions. */
+typedef struct {
+    int cell_sync;
+    int block_sync;
+    int global_sync;
+    int threads_per_block;
+    int positions_per_thread;
+    int positions_per_checkerboard;
+    int updates_per_checkerboard_switch;
+    int updates_per_barrier;
+    int iterations;
+    int inner_iterations;
+    int shared;
+    int partial_dispatch;
+    int centroid_tracking;
+} cpm_run_args;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* cpm_last_error(void);
+
+/* Library build string (for diagnostics). */
+const char* cpm_version(void);
+
+/* Cpm::Cpm (cpm.cu:176-261), argument order of PyCpm_init
+ * (python_wrapper.cpp:27-41): dimension L (power of two), dimensionality d
+ * (2 or 3), number_of_cells n (tables hold n+1 rows, row 0 = medium),
+ * temperature, has_field, history, has_act. */
+int cpm_create(int dimension, int dimensionality, int number_of_cells, int temperature,
+               int has_field, int history, int has_act, cpm_handle* out);
+
+/* Cpm::~Cpm (cpm.cu:263-285). */
+int cpm_destroy(cpm_handle h);
+
+/* Cpm::setTemperature (cpm.cu:57-60); real temperatures are an extension. */
+int cpm_set_temperature(cpm_handle h, double temperature);
+
+/* Cpm::setArea (cpm.cu:125-129): area (volume in 3D) target and lambda. */
+int cpm_set_area(cpm_handle h, int type, int target, double lambda);
+
+/* Cpm::setPerimeter (cpm.cu:119-123): perimeter (surface in 3D). */
+int cpm_set_perimeter(cpm_handle h, int type, int target, double lambda);
+
+/* Cpm::setAdhesion (cpm.cu:137-143); J is stored as int16 (settings.h:20). */
+int cpm_set_adhesion(cpm_handle h, int type, int other_type, int adhesion, int set_both);
+
+/* Cpm::setStickiness (cpm.cu:145-148); int16 (settings.h:22). */
+int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness);
+
+/* Cpm::setFixed (cpm.cu:150-153).  Only fixed == 0 is accepted. */
+int cpm_set_fixed(cpm_handle h, int type, int fixed);
+
+/* Cpm::addCell (cpm.cu:160-174): writes id (k | type<<24) for the k-th
+ * added cell into the host lattice.  z < 0 selects the 2D form. */
+int cpm_add_cell(cpm_handle h, int type, int x, int y, int z);
+
+/* Cpm::getCellIds (cpm.cu:519-521): the engine-owned host lattice,
+ * L^d uint32 in [z][y][x] order, valid until cpm_destroy. */
+int cpm_host_lattice(cpm_handle h, uint32_t** out);
+
+/* Number of cells registered with cpm_add_cell (Cpm::_activeCells). */
+int cpm_active_cells(cpm_handle h, int* out);
+
+/* Cpm::moveCellIdsToGpu (cpm.cu:541-597): copy the host lattice to the
+ * device and rebuild every cell table from it. */
+int cpm_push(cpm_handle h);
+
+/* Cpm::run (cpm.cu:291-517): (iterations / inner_iterations) *
+ * inner_iterations Monte Carlo steps.  Asynchronous (like the 2D reference
+ * run): returns once the work is queued. */
+int cpm_run(cpm_handle h, const cpm_run_args* args);
+
+/* Cpm::synchronize (cpm.cu:653-655). */
+int cpm_sync(cpm_handle h);
+
+/* Cpm::retreiveCellIdsFromGpu (cpm.cu:531-540): device lattice -> host. */
+int cpm_pull(cpm_handle h);
+
+/* Cpm::get2dCentroids / get3dCentroids (cpm.cu:68-110): rows for cells
+ * 1..active_cells, d doubles each ([x, y(, z)]).  capacity is the row
+ * capacity of out; *rows receives active_cells. */
+int cpm_get_centroids(cpm_handle h, double* out, int capacity, int* rows);
+
+/* Extension (the reference keeps these tables device-only): raw integer
+ * cell tables for all n+1 rows.  area/perim: [n+1]; anchor: [(n+1)*3]
+ * coordinates of each cell's first site in raster order at the last push;
+ * csum: [(n+1)*3] sums of site coordinates unwrapped about the anchor.
+ * Any pointer may be NULL. */
+int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anchor,
+                       int64_t* csum);
+
+/* Extension: RNG seed (Philox4x32-10 key) and the MCS counter that keys the
+ * stream; together they make every run reproducible and resumable. */
+int cpm_set_seed(cpm_handle h, uint64_t seed);
+int cpm_get_seed(cpm_handle h, uint64_t* seed);
+int cpm_set_mcs(cpm_handle h, uint64_t mcs);
+int cpm_get_mcs(cpm_handle h, uint64_t* mcs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,455 @@
+/*
+ * cpm_oracle.c -- CPU restatement of the reference Cellular Potts sweep.
+ *
+ * TEST INFRASTRUCTURE ONLY (see cpm_oracle.h): the checker for the HIP
+ * product and the CPU baseline timed beside it.  Plain C99 + optional OpenMP.
+ */
+#include "cpm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------- RNG -- */
+
+/* random.cu:5-12: x <- 16807 x mod (2^31 - 1), evaluated in double. */
+int32_t orc_minstd_next_f64(int32_t seed) {
+    const double a = 16807.0;
+    const double m = 2147483647.0;
+    double temp = (double)seed * a;
+    return (int32_t)(temp - m * floor(temp * (1.0 / m)));
+}
+
+uint32_t orc_minstd_next_int(uint32_t x) {
+    return (uint32_t)(((uint64_t)x * 16807u) % 2147483647u);
+}
+
+/* Philox4x32 with 10 rounds (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel
+ * random numbers: as easy as 1, 2, 3").  Round: two 32x32->64 products of
+ * words 0 and 2 with the multipliers; the key is bumped by the Weyl
+ * constants before every round after the first. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+/* ------------------------------------------------------- parameters -- */
+
+int orc_energy_scale(const orc_params* p) {
+    for (int k = 0; k <= ORC_MAX_SCALE_LOG2; k++) {
+        double s = (double)(1 << k);
+        int ok = 1;
+        for (int t = 0; t < ORC_NTYPES && ok; t++) {
+            double v[2] = {p->lam_area[t] * s, p->lam_perim[t] * s};
+            for (int j = 0; j < 2; j++) {
+                if (v[j] != floor(v[j]) || fabs(v[j]) >= 2147483648.0) ok = 0;
+            }
+        }
+        if (ok) return 1 << k;
+    }
+    return -1;
+}
+
+int orc_threshold_table(double temperature, int scale, uint32_t* out, int cap) {
+    /* thr[0] is never consulted (dH <= 0 is always accepted). */
+    if (cap < 1) return -1;
+    out[0] = 0xFFFFFFFFu;
+    if (!(temperature > 0.0)) return 1;
+    int n = 1;
+    for (int k = 1; k < cap; k++) {
+        double v = floor(exp(-(double)k / ((double)scale * temperature)) * 4294967296.0);
+        if (v <= 0.0) return n;
+        out[k] = (uint32_t)v;
+        n = k + 1;
+    }
+    return -2; /* table would exceed cap */
+}
+
+/* ------------------------------------------------------- geometry -- */
+
+typedef struct {
+    int n;
+    int off[26][3];
+} orc_nbhd;
+
+/* Offsets in lexicographic (dz, dy, dx) order, dx fastest, origin skipped. */
+static void make_nbhd(int d, orc_nbhd* nb) {
+    nb->n = 0;
+    int zlo = d == 3 ? -1 : 0, zhi = d == 3 ? 1 : 0;
+    for (int dz = zlo; dz <= zhi; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                if (!dx && !dy && !dz) continue;
+                nb->off[nb->n][0] = dx;
+                nb->off[nb->n][1] = dy;
+                nb->off[nb->n][2] = dz;
+                nb->n++;
+            }
+}
+
+static inline int64_t site(int L, int x, int y, int z) {
+    return ((int64_t)z * L + y) * L + x;
+}
+
+static inline int32_t wrapc(int L, int v) {
+    /* signed minimum image in [-L/2, L/2) */
+    return ((v + L / 2) & (L - 1)) - L / 2;
+}
+
+static int log2i(int v) {
+    int k = 0;
+    while ((1 << k) < v) k++;
+    return k;
+}
+
+/* --------------------------------------------------------- tallies -- */
+
+int orc_tally(int L, int d, int ncells, const uint32_t* lat, int32_t* area,
+              int32_t* perim, int32_t* anchor, int64_t* csum) {
+    orc_nbhd nb;
+    make_nbhd(d, &nb);
+    int Lz = d == 3 ? L : 1;
+    memset(area, 0, sizeof(int32_t) * (size_t)(ncells + 1));
+    memset(perim, 0, sizeof(int32_t) * (size_t)(ncells + 1));
+    memset(csum, 0, sizeof(int64_t) * 3 * (size_t)(ncells + 1));
+    int64_t* first = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ncells + 1));
+    for (int c = 0; c <= ncells; c++) first[c] = -1;
+    for (int z = 0; z < Lz; z++)
+        for (int y = 0; y < L; y++)
+            for (int x = 0; x < L; x++) {
+                uint32_t v = lat[site(L, x, y, z)];
+                uint32_t c = v & 0xFFFFFFu;
+                if ((v >> 24) >= ORC_NTYPES || c > (uint32_t)ncells) {
+                    free(first);
+                    return -1;
+                }
+                if (!c) continue;
+                area[c]++;
+                if (first[c] < 0) first[c] = site(L, x, y, z);
+                int cnt = 0;
+                for (int k = 0; k < nb.n; k++) {
+                    int nx = (x + nb.off[k][0]) & (L - 1);
+                    int ny = (y + nb.off[k][1]) & (L - 1);
+                    int nz = d == 3 ? (z + nb.off[k][2]) & (L - 1) : 0;
+                    if ((lat[site(L, nx, ny, nz)] & 0xFFFFFFu) != c) cnt++;
+                }
+                perim[c] += cnt;
+            }
+    for (int c = 0; c <= ncells; c++) {
+        int64_t f = first[c] < 0 ? 0 : first[c];
+        anchor[3 * c + 0] = (int32_t)(f % L);
+        anchor[3 * c + 1] = (int32_t)((f / L) % L);
+        anchor[3 * c + 2] = d == 3 ? (int32_t)(f / ((int64_t)L * L)) : 0;
+    }
+    for (int z = 0; z < Lz; z++)
+        for (int y = 0; y < L; y++)
+            for (int x = 0; x < L; x++) {
+                uint32_t c = lat[site(L, x, y, z)] & 0xFFFFFFu;
+                if (!c) continue;
+                csum[3 * c + 0] += wrapc(L, x - anchor[3 * c + 0]);
+                csum[3 * c + 1] += wrapc(L, y - anchor[3 * c + 1]);
+                if (d == 3) csum[3 * c + 2] += wrapc(L, z - anchor[3 * c + 2]);
+            }
+    free(first);
+    return 0;
+}
+
+/* ------------------------------------------------------------ sweep -- */
+
+typedef struct {
+    int64_t J[ORC_NTYPES][ORC_NTYPES];   /* J * scale */
+    int64_t JS[ORC_NTYPES][ORC_NTYPES];  /* (J + S) * scale */
+    int64_t la[ORC_NTYPES], lp[ORC_NTYPES];
+    int64_t a0[ORC_NTYPES], p0[ORC_NTYPES];
+} orc_scaled;
+
+static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
+    for (int a = 0; a < ORC_NTYPES; a++) {
+        for (int b = 0; b < ORC_NTYPES; b++) {
+            q->J[a][b] = (int64_t)p->J[a][b] * scale;
+            q->JS[a][b] = ((int64_t)p->J[a][b] + p->S[a][b]) * scale;
+        }
+        q->la[a] = (int64_t)(p->lam_area[a] * scale);
+        q->lp[a] = (int64_t)(p->lam_perim[a] * scale);
+        q->a0[a] = p->target_area[a];
+        q->p0[a] = p->target_perim[a];
+    }
+}
+
+/* dH of writing value vs into target t, given its neighbourhood (the energy
+ * part of kernels2d.cu:346-519).  as_/at_ return the like-neighbour counts
+ * of source and target spins. */
+static int64_t delta_h_core(int L, int d, const orc_nbhd* nb, const uint32_t* lat,
+                            const int32_t* areaR, const int32_t* perimR,
+                            const orc_scaled* q, int tx, int ty, int tz, uint32_t vs,
+                            uint32_t vt, int* as_, int* at_) {
+    uint32_t ts = vs >> 24, tt = vt >> 24;
+    int64_t dh = 0;
+    int as = 0, at = 0;
+    for (int k = 0; k < nb->n; k++) {
+        int nx = (tx + nb->off[k][0]) & (L - 1);
+        int ny = (ty + nb->off[k][1]) & (L - 1);
+        int nz = d == 3 ? (tz + nb->off[k][2]) & (L - 1) : 0;
+        uint32_t vn = lat[site(L, nx, ny, nz)];
+        uint32_t tn = vn >> 24;
+        if (vn != vs) dh += q->J[ts][tn]; else as++;
+        if (vn != vt) dh -= q->JS[tt][tn]; else at++;
+    }
+    int N = nb->n;
+    uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
+    if (cs) {
+        int64_t A = areaR[cs], P = perimR[cs];
+        int64_t dP = N - 2 * as;
+        dh += q->la[ts] * (1 + 2 * A - 2 * q->a0[ts]);
+        dh += q->lp[ts] * dP * (dP + 2 * (P - q->p0[ts]));
+    }
+    if (ct) {
+        int64_t A = areaR[ct], P = perimR[ct];
+        int64_t dP = 2 * at - N;
+        dh += q->la[tt] * (1 - 2 * A + 2 * q->a0[tt]);
+        dh += q->lp[tt] * dP * (dP + 2 * (P - q->p0[tt]));
+    }
+    *as_ = as;
+    *at_ = at;
+    return dh;
+}
+
+int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,
+                    const int32_t* perim, const orc_params* p, int scale, int64_t t,
+                    int64_t s) {
+    orc_nbhd nb;
+    make_nbhd(d, &nb);
+    orc_scaled q;
+    scale_params(p, scale, &q);
+    int tx = (int)(t % L), ty = (int)((t / L) % L), tz = d == 3 ? (int)(t / ((int64_t)L * L)) : 0;
+    int as, at;
+    return delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, lat[s], lat[t], &as, &at);
+}
+
+int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+              const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+              uint64_t seed, uint32_t mcs0, int nmcs, int nthreads) {
+    if (d != 2 && d != 3) return -1;
+    if (L < 2 || (L & (L - 1))) return -2;
+    if (c < 1 || (c & (c - 1)) || L % (2 * c)) return -3;
+    int cd = d == 3 ? c * c * c : c * c;
+    if (u < 1 || u > cd) return -4;
+    int scale = orc_energy_scale(p);
+    if (scale < 0) return -5;
+    int cap = 1 << 22;
+    uint32_t* thr = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)cap);
+    int nthr = orc_threshold_table(p->temperature, scale, thr, cap);
+    if (nthr < 0) {
+        free(thr);
+        return -6;
+    }
+    orc_scaled q;
+    scale_params(p, scale, &q);
+    orc_nbhd nb;
+    make_nbhd(d, &nb);
+    const int N = nb.n;
+    const int lc = log2i(c);
+    const uint32_t cm = (uint32_t)c - 1;
+    const int nsb = L / c;       /* sub-blocks per axis */
+    const int h = nsb / 2;       /* same-colour sub-blocks per axis */
+    const int64_t nq = d == 3 ? (int64_t)h * h * h : (int64_t)h * h;
+    const int rounds = cd / u;   /* kernels2d.cu:95 */
+    const int ncol = 1 << d;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+
+    int32_t* areaR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
+    int32_t* perimR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#else
+    (void)nthreads;
+#endif
+
+    for (int m = 0; m < nmcs; m++) {
+        uint32_t mcs = mcs0 + (uint32_t)m;
+        for (int r = 0; r < rounds; r++) {
+            for (int col = 0; col < ncol; col++) {
+                /* col = cx*2 + cy (2D), cx*4 + cy*2 + cz (3D): cbX outermost */
+                int cx = d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
+                int cy = d == 3 ? (col >> 1) & 1 : col & 1;
+                int cz = d == 3 ? col & 1 : 0;
+                uint32_t phase = (uint32_t)(r * ncol + col);
+                memcpy(areaR, area, sizeof(int32_t) * (size_t)(ncells + 1));
+                memcpy(perimR, perim, sizeof(int32_t) * (size_t)(ncells + 1));
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+                for (int64_t qi = 0; qi < nq; qi++) {
+                    int bx = 2 * (int)(qi % h) + cx;
+                    int by = 2 * (int)((qi / h) % h) + cy;
+                    int bz = d == 3 ? 2 * (int)(qi / ((int64_t)h * h)) + cz : 0;
+                    uint32_t sb = (uint32_t)(((int64_t)bz * nsb + by) * nsb + bx);
+                    for (int i = 0; i < u; i++) {
+                        uint32_t ctr[4] = {sb, mcs, phase, (uint32_t)i};
+                        uint32_t w[4];
+                        orc_philox4x32_10(ctr, key, w);
+                        int tx = bx * c + (int)(w[0] & cm);
+                        int ty = by * c + (int)((w[0] >> lc) & cm);
+                        int tz = d == 3 ? bz * c + (int)((w[0] >> (2 * lc)) & cm) : 0;
+                        int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+                        int sx = (tx + nb.off[dir][0]) & (L - 1);
+                        int sy = (ty + nb.off[dir][1]) & (L - 1);
+                        int sz = d == 3 ? (tz + nb.off[dir][2]) & (L - 1) : 0;
+                        int64_t ti = site(L, tx, ty, tz);
+                        uint32_t vs = lat[site(L, sx, sy, sz)];
+                        uint32_t vt = lat[ti];
+                        if (vs == vt) continue; /* kernels2d.cu:276 */
+                        int as, at;
+                        int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
+                                                  tz, vs, vt, &as, &at);
+                        int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
+                        if (!accept) continue;
+                        lat[ti] = vs; /* kernels2d.cu:548 */
+                        uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
+                        int pos[3] = {tx, ty, tz};
+                        if (cs) { /* kernels2d.cu:552-560 */
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            area[cs] += 1;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            perim[cs] += N - 2 * as;
+                            for (int a = 0; a < d; a++) {
+                                int64_t w_ = wrapc(L, pos[a] - anchor[3 * cs + a]);
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                                csum[3 * cs + a] += w_;
+                            }
+                        }
+                        if (ct) { /* kernels2d.cu:561-569 */
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            area[ct] -= 1;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            perim[ct] += 2 * at - N;
+                            for (int a = 0; a < d; a++) {
+                                int64_t w_ = wrapc(L, pos[a] - anchor[3 * ct + a]);
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                                csum[3 * ct + a] -= w_;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free(areaR);
+    free(perimR);
+    free(thr);
+    return 0;
+}
+
+/* ----------------------------------------------------------- energy -- */
+
+int orc_energy_stats(int L, int d, int ncells, const uint32_t* lat, const orc_params* p,
+                     int64_t* pairs, int64_t* sa2, int64_t* sa1, int64_t* sp2,
+                     int64_t* sp1) {
+    orc_nbhd nb;
+    make_nbhd(d, &nb);
+    int Lz = d == 3 ? L : 1;
+    memset(pairs, 0, sizeof(int64_t) * ORC_NTYPES * ORC_NTYPES);
+    memset(sa2, 0, sizeof(int64_t) * ORC_NTYPES);
+    memset(sa1, 0, sizeof(int64_t) * ORC_NTYPES);
+    memset(sp2, 0, sizeof(int64_t) * ORC_NTYPES);
+    memset(sp1, 0, sizeof(int64_t) * ORC_NTYPES);
+    /* every unordered pair is met twice over all (site, offset) pairs */
+    int64_t* twice = (int64_t*)calloc(ORC_NTYPES * ORC_NTYPES, sizeof(int64_t));
+    int32_t* area = (int32_t*)calloc((size_t)ncells + 1, sizeof(int32_t));
+    int32_t* perim = (int32_t*)calloc((size_t)ncells + 1, sizeof(int32_t));
+    int32_t* ctype = (int32_t*)calloc((size_t)ncells + 1, sizeof(int32_t));
+    for (int z = 0; z < Lz; z++)
+        for (int y = 0; y < L; y++)
+            for (int x = 0; x < L; x++) {
+                uint32_t v = lat[site(L, x, y, z)];
+                uint32_t c = v & 0xFFFFFFu, t = v >> 24;
+                if (c > (uint32_t)ncells || t >= ORC_NTYPES) {
+                    free(twice); free(area); free(perim); free(ctype);
+                    return -1;
+                }
+                if (c) { area[c]++; ctype[c] = (int32_t)t; }
+                for (int k = 0; k < nb.n; k++) {
+                    int nx = (x + nb.off[k][0]) & (L - 1);
+                    int ny = (y + nb.off[k][1]) & (L - 1);
+                    int nz = d == 3 ? (z + nb.off[k][2]) & (L - 1) : 0;
+                    uint32_t vn = lat[site(L, nx, ny, nz)];
+                    if (vn == v) continue;
+                    uint32_t tn = vn >> 24;
+                    uint32_t a = t < tn ? t : tn, b = t < tn ? tn : t;
+                    twice[a * ORC_NTYPES + b]++;
+                    if (c && (vn & 0xFFFFFFu) != c) perim[c]++;
+                }
+            }
+    for (int i = 0; i < ORC_NTYPES * ORC_NTYPES; i++) pairs[i] = twice[i] / 2;
+    for (int c = 1; c <= ncells; c++) {
+        if (!area[c]) continue;
+        int t = ctype[c];
+        int64_t da = (int64_t)area[c] - p->target_area[t];
+        int64_t dp = (int64_t)perim[c] - p->target_perim[t];
+        sa2[t] += da * da;
+        sa1[t] += da;
+        sp2[t] += dp * dp;
+        sp1[t] += dp;
+    }
+    free(twice); free(area); free(perim); free(ctype);
+    return 0;
+}
+
+double orc_hamiltonian(int L, int d, int ncells, const uint32_t* lat, const orc_params* p) {
+    int64_t pairs[ORC_NTYPES * ORC_NTYPES], sa2[ORC_NTYPES], sa1[ORC_NTYPES],
+        sp2[ORC_NTYPES], sp1[ORC_NTYPES];
+    if (orc_energy_stats(L, d, ncells, lat, p, pairs, sa2, sa1, sp2, sp1)) return NAN;
+    double H = 0.0;
+    for (int a = 0; a < ORC_NTYPES; a++)
+        for (int b = a; b < ORC_NTYPES; b++)
+            H += (double)p->J[a][b] * (double)pairs[a * ORC_NTYPES + b];
+    for (int t = 0; t < ORC_NTYPES; t++)
+        H += p->lam_area[t] * (double)sa2[t] + p->lam_perim[t] * (double)sp2[t];
+    return H;
+}
+
+/* -------------------------------------------------------- readback -- */
+
+void orc_centroids(int L, int d, int ncells, const int32_t* area, const int32_t* anchor,
+                   const int64_t* csum, double* out) {
+    for (int c = 0; c <= ncells; c++)
+        for (int a = 0; a < d; a++) {
+            if (!area[c]) {
+                out[c * d + a] = NAN;
+                continue;
+            }
+            double v = (double)anchor[3 * c + a] + (double)csum[3 * c + a] / (double)area[c];
+            if (v < 0.0) v += L;
+            if (v >= L) v -= L;
+            out[c * d + a] = v;
+        }
+}

@@ -1,0 +1,106 @@
+/*
+ * cpm_oracle.h -- CPU restatement of the reference Cellular Potts sweep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this code, and only as the checker
+ * (or as the timed CPU baseline).  The product library never links it.
+ *
+ * Semantics follow the reference (shabaz/gpu-cpm, src/):
+ *   - lattice encoding  id = cell | type << 24           kernels.cu:44-46, cpm.cu:163
+ *   - row-major index, x fastest                          kernels.cu:20-42
+ *   - Moore 8 / 26 periodic neighbourhood                 kernels2d.cu:359-415, kernels3d.cu:368-429
+ *   - adhesion + stickiness, area, perimeter dH           kernels2d.cu:399-412,477-519
+ *   - tally recount (area, perimeter)                     kernels2d.cu:5-41, kernels3d.cu:3-47
+ *   - shared-mode roles: target drawn inside the
+ *     checkerboard sub-block, source a random neighbour  kernels2d.cu:204-226
+ *   - colour order cbX outer, cbY (, cbZ) inner,
+ *     (c^d / u) rounds per MCS                            kernels2d.cu:95,104,179-182
+ *   - MINSTD generator (compat / known-answer only)       random.cu:5-12
+ * and make it deterministic (the reference races on its cell tables):
+ *   - counter-based Philox4x32-10 (Salmon et al., SC'11) keyed by
+ *     (seed; sub-block, MCS, phase, attempt);
+ *   - cell tables read from a phase-start snapshot, updates applied as
+ *     commutative integer sums;
+ *   - integer dH in a dyadic fixed-point scale and a host-built acceptance
+ *     threshold table thr[k] = floor(exp(-k/(scale*T)) * 2^32).
+ */
+#ifndef CPM_ORACLE_H
+#define CPM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NTYPES 20
+#define ORC_MAX_SCALE_LOG2 8
+
+/* Parameter block: settings.h:4-31 restricted to the hot-path fields. */
+typedef struct {
+    int32_t J[ORC_NTYPES][ORC_NTYPES];   /* adhesion (int16 range), settings.h:20 */
+    int32_t S[ORC_NTYPES][ORC_NTYPES];   /* stickiness, settings.h:22 */
+    double lam_area[ORC_NTYPES];         /* settings.h:11 */
+    int32_t target_area[ORC_NTYPES];     /* settings.h:12 */
+    double lam_perim[ORC_NTYPES];        /* settings.h:17 */
+    int32_t target_perim[ORC_NTYPES];    /* settings.h:18 */
+    double temperature;                  /* settings.h:30 */
+} orc_params;
+
+/* Generators. */
+int32_t orc_minstd_next_f64(int32_t seed);      /* random.cu:5-12, double formula */
+uint32_t orc_minstd_next_int(uint32_t x);       /* 16807 x mod (2^31-1), integer */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Fixed-point scale: smallest 2^k (k <= 8) making every lambda integral, or -1. */
+int orc_energy_scale(const orc_params* p);
+/* Acceptance table; returns its length n (thr[k] for 0 <= k < n; thr[k>=n] = 0). */
+int orc_threshold_table(double temperature, int scale, uint32_t* out, int cap);
+
+/* Full recount of the cell tables from a lattice (kernels2d.cu:5-41).
+ * area/perim: [ncells+1]; anchor: [(ncells+1)*3] site coordinates of each
+ * cell's first site in raster order; csum: [(ncells+1)*3] sums of
+ * coordinates unwrapped about the anchor.  Returns 0, or -1 if a spin
+ * exceeds ncells or a type exceeds 19. */
+int orc_tally(int L, int d, int ncells, const uint32_t* lat, int32_t* area,
+              int32_t* perim, int32_t* anchor, int64_t* csum);
+
+/* Run nmcs Monte Carlo steps starting at MCS counter mcs0.
+ * c = positions_per_checkerboard, u = updates_per_checkerboard_switch.
+ * nthreads > 1 uses OpenMP over the sub-blocks of one colour phase (results
+ * are identical for every thread count).  Tables must hold a valid tally on
+ * entry and are kept exact.  Returns 0, or a negative error code. */
+int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+              const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+              uint64_t seed, uint32_t mcs0, int nmcs, int nthreads);
+
+/* Scaled integer dH of copying site s into site t (both linear indices) with
+ * the given tables (used to pin dH against H(after)-H(before)). */
+int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,
+                    const int32_t* perim, const orc_params* p, int scale, int64_t t,
+                    int64_t s);
+
+/* Full Hamiltonian in FP64 (stickiness excluded; J taken as given, so use a
+ * symmetric J for an energy):
+ *   H = sum_{unordered neighbour pairs, s_i != s_j} J[t_i][t_j]
+ *     + sum_{c>0} lam_A (A_c - A0)^2 + sum_{c>0} lam_P (P_c - P0)^2 */
+double orc_hamiltonian(int L, int d, int ncells, const uint32_t* lat, const orc_params* p);
+
+/* Sufficient statistics of H used by dH/dtheta (FP64-exact integers):
+ *   pairs[a][b] (a <= b, upper triangle of a 20x20 int64 array): number of
+ *       unordered unlike-cell neighbour pairs with types {a, b};
+ *   per type t: sa2[t] = sum (A-A0)^2, sa1[t] = sum (A-A0),
+ *               sp2[t] = sum (P-P0)^2, sp1[t] = sum (P-P0)  over cells c>0 of type t.
+ * ctype[c] gives the type of cell c (cells with area 0 are skipped). */
+int orc_energy_stats(int L, int d, int ncells, const uint32_t* lat, const orc_params* p,
+                     int64_t* pairs, int64_t* sa2, int64_t* sa1, int64_t* sp2,
+                     int64_t* sp1);
+
+/* Centroid readback: anchor + csum/area wrapped into [0, L); NaN for area 0. */
+void orc_centroids(int L, int d, int ncells, const int32_t* area, const int32_t* anchor,
+                   const int64_t* csum, double* out /* [(ncells+1)*d] */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

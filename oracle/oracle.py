@@ -1,0 +1,215 @@
+"""ctypes wrapper over the CPU oracle (libcpm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (the ``gpucpm``
+package) never imports it.
+
+The oracle restates the reference sweep (shabaz/gpu-cpm src/kernels2d.cu,
+src/kernels3d.cu, src/kernels.cu, src/random.cu); see cpm_oracle.h for the
+file:line map and the deterministic phase rules.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+NTYPES = 20
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libcpm_oracle.so")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIBPATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+    return _LIBPATH
+
+
+class Params(ctypes.Structure):
+    """Mirror of orc_params (settings.h:4-31, hot-path fields only)."""
+
+    _fields_ = [
+        ("J", (ctypes.c_int32 * NTYPES) * NTYPES),
+        ("S", (ctypes.c_int32 * NTYPES) * NTYPES),
+        ("lam_area", ctypes.c_double * NTYPES),
+        ("target_area", ctypes.c_int32 * NTYPES),
+        ("lam_perim", ctypes.c_double * NTYPES),
+        ("target_perim", ctypes.c_int32 * NTYPES),
+        ("temperature", ctypes.c_double),
+    ]
+
+    @classmethod
+    def make(cls, temperature, J=None, S=None, lam_area=None, target_area=None,
+             lam_perim=None, target_perim=None):
+        p = cls()
+        p.temperature = float(temperature)
+        for name, val in (("J", J), ("S", S)):
+            if val is not None:
+                arr = np.zeros((NTYPES, NTYPES), np.int32)
+                v = np.asarray(val, np.int32)
+                arr[: v.shape[0], : v.shape[1]] = v
+                for a in range(NTYPES):
+                    for b in range(NTYPES):
+                        getattr(p, name)[a][b] = int(arr[a, b])
+        for name, val in (("lam_area", lam_area), ("target_area", target_area),
+                          ("lam_perim", lam_perim), ("target_perim", target_perim)):
+            if val is not None:
+                for t, x in enumerate(val):
+                    getattr(p, name)[t] = x
+        return p
+
+    def J_array(self):
+        return np.array([[self.J[a][b] for b in range(NTYPES)] for a in range(NTYPES)])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIBPATH)
+        P = ctypes.POINTER
+        u32p, i32p, i64p, f64p = P(ctypes.c_uint32), P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_double)
+        L.orc_minstd_next_f64.argtypes = [ctypes.c_int32]
+        L.orc_minstd_next_f64.restype = ctypes.c_int32
+        L.orc_minstd_next_int.argtypes = [ctypes.c_uint32]
+        L.orc_minstd_next_int.restype = ctypes.c_uint32
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_philox4x32_10.restype = None
+        L.orc_energy_scale.argtypes = [P(Params)]
+        L.orc_energy_scale.restype = ctypes.c_int
+        L.orc_threshold_table.argtypes = [ctypes.c_double, ctypes.c_int, u32p, ctypes.c_int]
+        L.orc_threshold_table.restype = ctypes.c_int
+        L.orc_tally.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, i32p, i64p]
+        L.orc_tally.restype = ctypes.c_int
+        L.orc_sweep.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, i32p, i64p,
+                                P(Params), ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+        L.orc_sweep.restype = ctypes.c_int
+        L.orc_delta_h.argtypes = [ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, P(Params), ctypes.c_int,
+                                  ctypes.c_int64, ctypes.c_int64]
+        L.orc_delta_h.restype = ctypes.c_int64
+        L.orc_hamiltonian.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, P(Params)]
+        L.orc_hamiltonian.restype = ctypes.c_double
+        L.orc_energy_stats.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, P(Params),
+                                       i64p, i64p, i64p, i64p, i64p]
+        L.orc_energy_stats.restype = ctypes.c_int
+        L.orc_centroids.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i64p, f64p]
+        L.orc_centroids.restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def minstd_f64(x: int) -> int:
+    return lib().orc_minstd_next_f64(x)
+
+
+def minstd_int(x: int) -> int:
+    return lib().orc_minstd_next_int(x)
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32).copy()
+    k = np.asarray(key, np.uint32).copy()
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_ptr(c, ctypes.c_uint32), _ptr(k, ctypes.c_uint32), _ptr(out, ctypes.c_uint32))
+    return out
+
+
+def energy_scale(p: Params) -> int:
+    return lib().orc_energy_scale(ctypes.byref(p))
+
+
+def threshold_table(temperature: float, scale: int, cap: int = 1 << 22):
+    out = np.zeros(cap, np.uint32)
+    n = lib().orc_threshold_table(float(temperature), int(scale), _ptr(out, ctypes.c_uint32), cap)
+    if n < 0:
+        raise ValueError("threshold table does not fit")
+    return out[:n].copy()
+
+
+class Tables:
+    """Cell tables in the oracle's layout (area, perimeter, anchor, csum)."""
+
+    def __init__(self, ncells):
+        self.ncells = ncells
+        self.area = np.zeros(ncells + 1, np.int32)
+        self.perim = np.zeros(ncells + 1, np.int32)
+        self.anchor = np.zeros((ncells + 1) * 3, np.int32)
+        self.csum = np.zeros((ncells + 1) * 3, np.int64)
+
+    def copy(self):
+        t = Tables(self.ncells)
+        for k in ("area", "perim", "anchor", "csum"):
+            setattr(t, k, getattr(self, k).copy())
+        return t
+
+
+def tally(lat: np.ndarray, d: int, ncells: int) -> Tables:
+    lat = np.ascontiguousarray(lat).view(np.uint32)
+    L = lat.shape[0]
+    t = Tables(ncells)
+    rc = lib().orc_tally(L, d, ncells, _ptr(lat, ctypes.c_uint32), _ptr(t.area, ctypes.c_int32),
+                         _ptr(t.perim, ctypes.c_int32), _ptr(t.anchor, ctypes.c_int32),
+                         _ptr(t.csum, ctypes.c_int64))
+    if rc:
+        raise ValueError("lattice holds a spin above ncells or a type above 19")
+    return t
+
+
+def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
+          mcs0: int, nmcs: int, nthreads: int = 1) -> None:
+    """Run nmcs MCS in place on lat (uint32 view) and tables."""
+    assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
+    lat32 = lat.view(np.uint32)
+    L = lat.shape[0]
+    rc = lib().orc_sweep(L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
+                         _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+                         _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                         ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
+                         int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads))
+    if rc:
+        raise ValueError(f"orc_sweep failed with code {rc}")
+
+
+def delta_h(lat, d, tables: Tables, p: Params, scale: int, t: int, s: int) -> int:
+    lat32 = np.ascontiguousarray(lat).view(np.uint32)
+    return lib().orc_delta_h(lat32.shape[0], d, _ptr(lat32, ctypes.c_uint32),
+                             _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+                             ctypes.byref(p), scale, t, s)
+
+
+def hamiltonian(lat, d, ncells, p: Params) -> float:
+    lat32 = np.ascontiguousarray(lat).view(np.uint32)
+    return lib().orc_hamiltonian(lat32.shape[0], d, ncells, _ptr(lat32, ctypes.c_uint32), ctypes.byref(p))
+
+
+def energy_stats(lat, d, ncells, p: Params):
+    lat32 = np.ascontiguousarray(lat).view(np.uint32)
+    pairs = np.zeros((NTYPES, NTYPES), np.int64)
+    sa2, sa1, sp2, sp1 = (np.zeros(NTYPES, np.int64) for _ in range(4))
+    rc = lib().orc_energy_stats(lat32.shape[0], d, ncells, _ptr(lat32, ctypes.c_uint32), ctypes.byref(p),
+                                _ptr(pairs, ctypes.c_int64), _ptr(sa2, ctypes.c_int64),
+                                _ptr(sa1, ctypes.c_int64), _ptr(sp2, ctypes.c_int64),
+                                _ptr(sp1, ctypes.c_int64))
+    if rc:
+        raise ValueError("bad lattice")
+    return {"pairs": pairs, "sa2": sa2, "sa1": sa1, "sp2": sp2, "sp1": sp1}
+
+
+def centroids(L, d, tables: Tables):
+    out = np.zeros((tables.ncells + 1) * d, np.float64)
+    lib().orc_centroids(L, d, tables.ncells, _ptr(tables.area, ctypes.c_int32),
+                        _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                        _ptr(out, ctypes.c_double))
+    return out.reshape(tables.ncells + 1, d)

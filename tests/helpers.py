@@ -1,0 +1,44 @@
+"""Shared test helpers: oracle runs of a workload and GPU-vs-oracle comparisons."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+
+
+def oracle_params(wl) -> "O.Params":
+    p = wl.params()
+    return O.Params.make(p["temperature"], J=p["J"], S=p["S"], lam_area=p["lam_area"],
+                         target_area=p["target_area"], lam_perim=p["lam_perim"],
+                         target_perim=p["target_perim"])
+
+
+def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None):
+    """Tally + sweep in the oracle; returns (lattice, Tables)."""
+    lat = np.ascontiguousarray(wl.lattice if lattice is None else lattice).astype(np.uint32).copy()
+    tab = O.tally(lat, wl.d, wl.ncells)
+    if nmcs:
+        O.sweep(lat, wl.d, tab, oracle_params(wl), c, u, seed, mcs0, nmcs, nthreads)
+    return lat, tab
+
+
+def run_kwargs(c, u, iterations, inner=None, **extra):
+    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
+              positions_per_thread=2 * c, positions_per_checkerboard=c,
+              updates_per_checkerboard_switch=u, updates_per_barrier=1,
+              iterations=iterations, inner_iterations=inner or iterations)
+    kw.update(extra)
+    return kw
+
+
+def assert_tables_equal(gpu_table: dict, tab: "O.Tables", d: int):
+    n = tab.ncells + 1
+    np.testing.assert_array_equal(gpu_table["area"], tab.area)
+    np.testing.assert_array_equal(gpu_table["perimeter"], tab.perim)
+    np.testing.assert_array_equal(gpu_table["anchor"].reshape(-1)[: 3 * n], tab.anchor)
+    np.testing.assert_array_equal(gpu_table["csum"].reshape(-1)[: 3 * n], tab.csum)
+
+
+def lattice_digest(lat: np.ndarray) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(lat).view(np.uint32).tobytes()).hexdigest()

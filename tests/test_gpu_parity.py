@@ -1,0 +1,131 @@
+"""GPU parity: the HIP sweep through the C ABI vs the CPU oracle, bit-exact.
+
+Every case prints its name, shape and schedule before touching the GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, run_kwargs, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    import gpucpm
+    from gpucpm import workloads
+    return gpucpm, workloads
+
+
+def _run_gpu(wl, c, u, seed, nmcs, inner=None, splits=None):
+    gpucpm, _ = _gpu()
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(seed)
+    for n in (splits or [nmcs]):
+        sim.run(**run_kwargs(c, u, n, inner or n))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    return sim, sim.get_state().view(np.uint32).copy(), sim.get_cell_table()
+
+
+def _announce(*a):
+    print("CASE", *a, flush=True)
+
+
+def test_tally_after_push_matches_oracle():
+    _, W = _gpu()
+    for wl in (W.plain2d(L=64, ncells=20, seed=5), W.sorting2d(L=128, grid=9, edge=10, seed=6),
+               W.plain3d(L=32, ncells=40, seed=7)):
+        _announce("tally", wl.name, wl.L, wl.d)
+        sim = wl.make_cpm()
+        sim.push_to_gpu()
+        _, tab = run_oracle(wl, 1, 1, 0, 0)
+        assert_tables_equal(sim.get_cell_table(), tab, wl.d)
+
+
+@pytest.mark.parametrize("c,u", [(1, 1), (2, 1), (2, 4), (4, 3), (4, 16)])
+def test_sorting2d_small_bit_exact(c, u):
+    _, W = _gpu()
+    wl = W.sorting2d(L=128, grid=9, edge=10, seed=11)
+    _announce("sorting2d", wl.L, c, u)
+    nmcs = 30
+    _, lat, table = _run_gpu(wl, c, u, seed=1234, nmcs=nmcs)
+    lat_o, tab_o = run_oracle(wl, c, u, 1234, nmcs)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 2)
+
+
+@pytest.mark.parametrize("c,u", [(1, 1), (2, 8), (2, 3)])
+def test_plain3d_small_bit_exact(c, u):
+    _, W = _gpu()
+    wl = W.plain3d(L=32, ncells=60, seed=12)
+    _announce("plain3d", wl.L, c, u)
+    nmcs = 10
+    _, lat, table = _run_gpu(wl, c, u, seed=99, nmcs=nmcs)
+    lat_o, tab_o = run_oracle(wl, c, u, 99, nmcs)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 3)
+
+
+def test_plain2d_full_config_bit_exact():
+    """256^2, 100 cells, 500 MCS -- the full plain 2D configuration."""
+    _, W = _gpu()
+    wl = W.plain2d()
+    _announce("plain2d-full", wl.L, 2, 4)
+    _, lat, table = _run_gpu(wl, 2, 4, seed=1, nmcs=500)
+    lat_o, tab_o = run_oracle(wl, 2, 4, 1, 500)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 2)
+
+
+def test_sorting2d_full_size_prefix_bit_exact():
+    """1024^2 sorting lattice, first 20 MCS."""
+    _, W = _gpu()
+    wl = W.sorting2d()
+    _announce("sorting2d-full", wl.L, 1, 1)
+    _, lat, table = _run_gpu(wl, 1, 1, seed=2, nmcs=20)
+    lat_o, tab_o = run_oracle(wl, 1, 1, 2, 20)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 2)
+
+
+def test_plain3d_full_size_prefix_bit_exact():
+    """256^3 lattice with 20000 cubes, first 2 MCS."""
+    _, W = _gpu()
+    wl = W.plain3d()
+    _announce("plain3d-full", wl.L, 1, 1)
+    _, lat, table = _run_gpu(wl, 1, 1, seed=3, nmcs=2)
+    lat_o, tab_o = run_oracle(wl, 1, 1, 3, 2)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 3)
+
+
+def test_split_runs_equal_one_run():
+    """run(a) + run(b) == run(a + b): the stream is keyed by the MCS counter."""
+    _, W = _gpu()
+    wl = W.sorting2d(L=128, grid=9, edge=10, seed=13)
+    _announce("split", wl.L)
+    _, lat1, t1 = _run_gpu(wl, 2, 4, seed=5, nmcs=40)
+    _, lat2, t2 = _run_gpu(wl, 2, 4, seed=5, nmcs=40, splits=[15, 25])
+    _, lat3, t3 = _run_gpu(wl, 2, 4, seed=5, nmcs=40, inner=8)
+    np.testing.assert_array_equal(lat1, lat2)
+    np.testing.assert_array_equal(lat1, lat3)
+    for k in ("area", "perimeter", "csum"):
+        np.testing.assert_array_equal(t1[k], t2[k])
+        np.testing.assert_array_equal(t1[k], t3[k])
+
+
+def test_full_size_tables_equal_recount():
+    """Size-independent property at the 1024^2 shape: after 200 MCS the
+    incrementally kept tables equal a full recount of the final lattice."""
+    import oracle as O
+    _, W = _gpu()
+    wl = W.sorting2d()
+    _announce("recount", wl.L)
+    _, lat, table = _run_gpu(wl, 2, 4, seed=77, nmcs=200)
+    tab = O.tally(lat, 2, wl.ncells)
+    np.testing.assert_array_equal(table["area"], tab.area)
+    np.testing.assert_array_equal(table["perimeter"], tab.perim)
+    assert table["area"].sum() == np.count_nonzero(lat & 0xFFFFFF)
