@@ -1,0 +1,148 @@
+"""Benchmark: lattice site-updates/s of the 2D cell-sorting sweep on one GPU.
+
+Workload: 1024^2 lattice, 63x63 = 3969 cells of two types seeded as 10x10
+squares tiling the centre; area target 100 (lambda 25), perimeter target 40
+(lambda 2), J_mA = J_mB = 16, J_AA = 2, J_BB = 14, J_AB = 11, T = 10.
+A step is one ``run()`` of ``--mcs-per-step`` Monte Carlo steps (MCS); one
+MCS is L^2 copy attempts (site-updates).  The lattice and cell tables are
+resident in HBM before timing starts.
+
+Prints one JSON line with the throughput, the HBM roofline of the sweep
+kernel (8 algorithmic bytes per site-update: one int32 cell-id read and one
+write) from HIP events on the engine's stream, and the CPU oracle (same
+sweep, same schedule, OpenMP over the host cores) timed on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-cpm_amd"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mcs-per-step", type=int, default=200)
+    ap.add_argument("--workload", default="sorting2d", choices=["sorting2d", "plain3d", "plain2d"])
+    ap.add_argument("--checkerboard", type=int, default=1, help="positions_per_checkerboard c")
+    ap.add_argument("--updates", type=int, default=1, help="updates_per_checkerboard_switch u")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(wl, c, u, seconds):
+    """Time the CPU oracle (OpenMP over all host cores) on a bounded sample of
+    the same workload, from the same initial lattice."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as O
+    p = wl.params()
+    prm = O.Params.make(p["temperature"], J=p["J"], S=p["S"], lam_area=p["lam_area"],
+                        target_area=p["target_area"], lam_perim=p["lam_perim"],
+                        target_perim=p["target_perim"])
+    cores = len(os.sched_getaffinity(0))
+    lat = wl.lattice.copy()
+    tab = O.tally(lat, wl.d, wl.ncells)
+    t = time.perf_counter()
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 0, 2, cores)  # calibration
+    per_mcs = max((time.perf_counter() - t) / 2, 1e-6)
+    n = max(2, int(seconds / per_mcs))
+    t = time.perf_counter()
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 2, n, cores)
+    dt = time.perf_counter() - t
+    sites = wl.L ** wl.d
+    return {"value": n * sites / dt, "unit": "site-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{n} MCS of the same {wl.L}^{wl.d} workload (MCS 2..{n + 1}) from its "
+                      f"initial lattice, oracle/libcpm_oracle.so with OpenMP, {dt:.1f} s",
+            "mcs_per_s": n / dt}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(replicas shard across GPUs without exchange; see the sharding design).",
+              file=sys.stderr)
+        sys.exit(2)
+    import gpucpm
+    from gpucpm import workloads
+
+    wl = {"sorting2d": workloads.sorting2d, "plain3d": workloads.plain3d,
+          "plain2d": workloads.plain2d}[a.workload]()
+    c, u = a.checkerboard, a.updates
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(20240601)
+    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
+              positions_per_thread=2 * c, positions_per_checkerboard=c,
+              updates_per_checkerboard_switch=u, updates_per_barrier=1,
+              iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+    for _ in range(a.warmup):
+        sim.run(**kw)
+    sim.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sim.run(**kw)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+
+    # Kernel durations of the same number of steps, bracketed per launch.
+    sim.timing_begin()
+    for _ in range(a.steps):
+        sim.run(**kw)
+    kernel_ms, launches, updates = sim.timing_end()
+
+    sites = wl.L ** wl.d
+    mcs = a.steps * a.mcs_per_step
+    value = mcs * sites / wall
+    avg_launch_s = kernel_ms / 1e3 / launches
+    bytes_per_launch = updates / launches * BYTES_PER_UPDATE
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    workload_desc = {
+        "sorting2d": "2D cell sorting 1024^2, 3969 cells (10x10 squares), two types, 10000-MCS config",
+        "plain3d": "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood",
+        "plain2d": "plain 2D CPM 256^2, 100 cells",
+    }[a.workload]
+    out = {
+        "metric": "lattice site-updates/sec",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": workload_desc, "L": wl.L, "d": wl.d, "cells": wl.ncells,
+                   "mcs_per_step": a.mcs_per_step, "positions_per_checkerboard": c,
+                   "updates_per_checkerboard_switch": u, "seed": 20240601},
+        "mcs_per_s": mcs / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sweep_phase", "avg_launch_us": avg_launch_s * 1e6,
+                     "bytes_per_launch": bytes_per_launch, "launches": launches},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, c, u, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,7 +56,7 @@ struct PhaseArgs {
 };
 
 // Host-side launchers (kernels.hip).
-hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s);
+hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,
                         uint32_t* first, int32_t* anchor, int64_t* csum, hipStream_t s);
 hipError_t launch_settle(int64_t* tab, int64_t* del, int ncells, hipStream_t s);

@@ -84,6 +84,27 @@ struct cpm_engine {
     int nthr = 0;
     int scale = 1;
 
+    // Kernel timing (cpm_timing_begin/end): one event pair per sweep launch.
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    int64_t t_launches = 0;
+    int64_t t_updates = 0;
+
+    int next_events(hipEvent_t* e0, hipEvent_t* e1) {
+        *e0 = *e1 = nullptr;
+        if (!timing) return CPM_OK;
+        while (ev.size() < ev_used + 2) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        *e0 = ev[ev_used];
+        *e1 = ev[ev_used + 1];
+        ev_used += 2;
+        return CPM_OK;
+    }
+
     ~cpm_engine() {
         // Teardown is best effort: errors here have no caller to report to.
         if (stream) (void)hipStreamSynchronize(stream);
@@ -98,6 +119,7 @@ struct cpm_engine {
         (void)hipFree(d_prm);
         (void)hipFree(d_thr);
         if (h_lat) (void)hipHostFree(h_lat);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -405,7 +427,13 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                 pa.W = h->d_tab[(g + 1) & 1];
                 pa.Dcur = h->d_del[g & 1];
                 pa.Dprev = h->d_del[(g + 1) & 1];
-                HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream));
+                hipEvent_t e0, e1;
+                if (int rc = h->next_events(&e0, &e1)) return rc;
+                HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1));
+                if (h->timing) {
+                    h->t_launches++;
+                    h->t_updates += (int64_t)pa.nsub * u;
+                }
             }
         }
     }
@@ -470,6 +498,32 @@ int cpm_get_centroids(cpm_handle h, double* out, int capacity, int* rows) {
             out[(size_t)(c - 1) * h->d + ax] = v;
         }
     }
+    return CPM_OK;
+}
+
+int cpm_timing_begin(cpm_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    h->timing = true;
+    h->ev_used = 0;
+    h->t_launches = 0;
+    h->t_updates = 0;
+    return CPM_OK;
+}
+
+int cpm_timing_end(cpm_handle h, double* kernel_ms, int64_t* launches, int64_t* site_updates) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double total = 0.0;
+    for (size_t i = 0; i + 1 < h->ev_used; i += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+        total += ms;
+    }
+    if (kernel_ms) *kernel_ms = total;
+    if (launches) *launches = h->t_launches;
+    if (site_updates) *site_updates = h->t_updates;
+    h->timing = false;
+    h->ev_used = 0;
     return CPM_OK;
 }
 

@@ -4,6 +4,8 @@
 // (reference: src/kernels2d.cu:44-583, src/kernels3d.cu:51-594), the tally
 // rebuild run at push (kernels2d.cu:5-41, kernels3d.cu:3-47) and the
 // end-of-run settle of the double-buffered cell tables.
+#include <hip/hip_ext.h>
+
 #include "cpm_device.h"
 
 namespace gcpm {
@@ -213,13 +215,15 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
     del[c] = 0;
 }
 
-hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s) {
+hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const int work = a.nsub > a.ncells + 1 ? a.nsub : a.ncells + 1;
-    int blocks = (work + 255) / 256;
+    const int blocks = (work + 255) / 256;
+    // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
+    // (null events: plain launch).
     if (d == 2)
-        hipLaunchKernelGGL(k_sweep_phase<2>, dim3(blocks), dim3(256), 0, s, a);
+        hipExtLaunchKernelGGL(k_sweep_phase<2>, dim3(blocks), dim3(256), 0, s, e0, e1, 0, a);
     else
-        hipLaunchKernelGGL(k_sweep_phase<3>, dim3(blocks), dim3(256), 0, s, a);
+        hipExtLaunchKernelGGL(k_sweep_phase<3>, dim3(blocks), dim3(256), 0, s, e0, e1, 0, a);
     return hipGetLastError();
 }
 

@@ -242,6 +242,16 @@ class Cpm:
         return {"area": area, "perimeter": perim, "anchor": anchor.reshape(rows, 3),
                 "csum": csum.reshape(rows, 3), "centroid": cen}
 
+    def timing_begin(self):
+        """Start bracketing every sweep launch with HIP events."""
+        check(lib.cpm_timing_begin(self._h))
+
+    def timing_end(self):
+        """Synchronize; return (kernel_ms, launches, site_updates) since timing_begin."""
+        ms, n, upd = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib.cpm_timing_end(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(upd)))
+        return ms.value, n.value, upd.value
+
     def set_seed(self, seed):
         check(lib.cpm_set_seed(self._h, _int(seed, "seed") & (2**64 - 1)))
 

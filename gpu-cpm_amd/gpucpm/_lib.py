@@ -58,6 +58,8 @@ SIGNATURES = [
     ("cpm_get_seed", _I, [_H, _P(ctypes.c_uint64)]),
     ("cpm_set_mcs", _I, [_H, ctypes.c_uint64]),
     ("cpm_get_mcs", _I, [_H, _P(ctypes.c_uint64)]),
+    ("cpm_timing_begin", _I, [_H]),
+    ("cpm_timing_end", _I, [_H, _P(ctypes.c_double), _P(ctypes.c_int64), _P(ctypes.c_int64)]),
 ]
 
 if not os.path.exists(LIBPATH):

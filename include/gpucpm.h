@@ -32,7 +32,7 @@ typedef struct cpm_engine* cpm_handle;
  * Only positions_per_checkerboard, updates_per_checkerboard_switch,
  * iterations and inner_iterations change results; the rest select a thread
  * mapping in the reference and are validated but do not change results
- * here.  seed_set/seed and the schedule fields are extensions. */
+ * here.  The RNG seed is set separately (cpm_set_seed). */
 typedef struct {
     int cell_sync;
     int block_sync;
@@ -128,6 +128,14 @@ int cpm_set_seed(cpm_handle h, uint64_t seed);
 int cpm_get_seed(cpm_handle h, uint64_t* seed);
 int cpm_set_mcs(cpm_handle h, uint64_t mcs);
 int cpm_get_mcs(cpm_handle h, uint64_t* mcs);
+
+/* Extension (no counterpart in the reference): kernel timing.  Between
+ * begin and end every sweep launch is bracketed by HIP events recorded at
+ * the dispatch's start and end on the engine's stream; end synchronizes and
+ * returns the summed kernel time, the launch count and the number of copy
+ * attempts (site-updates) those launches made. */
+int cpm_timing_begin(cpm_handle h);
+int cpm_timing_end(cpm_handle h, double* kernel_ms, int64_t* launches, int64_t* site_updates);
 
 #ifdef __cplusplus
 }

@@ -123,6 +123,7 @@ static int log2i(int v) {
 
 int orc_tally(int L, int d, int ncells, const uint32_t* lat, int32_t* area,
               int32_t* perim, int32_t* anchor, int64_t* csum) {
+    if (L < 2 || (L & (L - 1)) || (d != 2 && d != 3)) return -2;
     orc_nbhd nb;
     make_nbhd(d, &nb);
     int Lz = d == 3 ? L : 1;
@@ -374,6 +375,7 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
 int orc_energy_stats(int L, int d, int ncells, const uint32_t* lat, const orc_params* p,
                      int64_t* pairs, int64_t* sa2, int64_t* sa1, int64_t* sp2,
                      int64_t* sp1) {
+    if (L < 2 || (L & (L - 1)) || (d != 2 && d != 3)) return -2;
     orc_nbhd nb;
     make_nbhd(d, &nb);
     int Lz = d == 3 ? L : 1;

@@ -60,8 +60,9 @@ int orc_threshold_table(double temperature, int scale, uint32_t* out, int cap);
 /* Full recount of the cell tables from a lattice (kernels2d.cu:5-41).
  * area/perim: [ncells+1]; anchor: [(ncells+1)*3] site coordinates of each
  * cell's first site in raster order; csum: [(ncells+1)*3] sums of
- * coordinates unwrapped about the anchor.  Returns 0, or -1 if a spin
- * exceeds ncells or a type exceeds 19. */
+ * coordinates unwrapped about the anchor.  L must be a power of two (the
+ * periodic wrap is a mask, as in random.cu:25).  Returns 0, -1 if a spin
+ * exceeds ncells or a type exceeds 19, -2 for a bad L or d. */
 int orc_tally(int L, int d, int ncells, const uint32_t* lat, int32_t* area,
               int32_t* perim, int32_t* anchor, int64_t* csum);
 

@@ -24,7 +24,7 @@ _LIBPATH = os.path.join(_HERE, "libcpm_oracle.so")
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, OpenMP)."""
     if force or not os.path.exists(_LIBPATH):
-        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
     return _LIBPATH
 
 
@@ -163,7 +163,7 @@ def tally(lat: np.ndarray, d: int, ncells: int) -> Tables:
                          _ptr(t.perim, ctypes.c_int32), _ptr(t.anchor, ctypes.c_int32),
                          _ptr(t.csum, ctypes.c_int64))
     if rc:
-        raise ValueError("lattice holds a spin above ncells or a type above 19")
+        raise ValueError("bad lattice: spin above ncells, type above 19, or L not a power of two")
     return t
 
 

@@ -1,0 +1,293 @@
+"""Pins for the CPU oracle (runs without a GPU).
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+published known-answer vectors for the generators, H(after) - H(before)
+for dH, an independent numpy recount for the tallies, the arithmetic mean
+for centroids, exp() for the acceptance table, and thread-count invariance
+for the phase independence of the checkerboard schedule.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _json(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+# ------------------------------------------------------------- generators --
+
+def test_philox_random123_kat():
+    kat = _json("philox4x32_10_kat.json")
+    for v in kat["vectors"]:
+        ctr = [int(x, 16) for x in v["ctr"]]
+        key = [int(x, 16) for x in v["key"]]
+        out = [int(x) for x in O.philox(ctr, key)]
+        assert out == [int(x, 16) for x in v["out"]]
+
+
+def test_minstd_kat_and_double_formula():
+    kat = _json("minstd_kat.json")
+    x = kat["seed"]
+    for _ in range(kat["steps"]):
+        x = O.minstd_int(x)
+    assert x == kat["value"]
+    # random.cu:5-12 evaluates the step in double; it equals the integer map.
+    rng = np.random.default_rng(0)
+    for s in rng.integers(1, 2**31 - 1, size=2000):
+        assert O.minstd_f64(int(s)) == O.minstd_int(int(s))
+
+
+# ------------------------------------------------------------- parameters --
+
+def test_energy_scale_is_smallest_dyadic():
+    p = O.Params.make(10, lam_area=[0, 25, 0.5], lam_perim=[0, 0.25])
+    assert O.energy_scale(p) == 4
+    p = O.Params.make(10, lam_area=[0, 3])
+    assert O.energy_scale(p) == 1
+    p = O.Params.make(10, lam_area=[0, 0.1])
+    assert O.energy_scale(p) == -1
+
+
+@pytest.mark.parametrize("T,scale", [(10, 1), (20, 1), (7.5, 4), (1, 2)])
+def test_threshold_table_matches_exp(T, scale):
+    thr = O.threshold_table(T, scale)
+    k = np.arange(1, len(thr))
+    p = np.exp(-k / (scale * T))
+    # floor(p * 2^32): error below one unit, monotone, ends where it hits zero
+    assert np.all(thr[1:] <= p * 2.0**32)
+    assert np.all(thr[1:] > p * 2.0**32 - 1)
+    assert np.all(np.diff(thr[1:].astype(np.int64)) <= 0)
+    assert math.floor(math.exp(-len(thr) / (scale * T)) * 2.0**32) == 0
+
+
+def test_zero_temperature_table():
+    thr = O.threshold_table(0.0, 1)
+    assert len(thr) == 1
+
+
+# ------------------------------------------------------------- energy -----
+
+def _random_lattice(rng, L, d, ncells, ntypes=2, fill=0.7):
+    shape = (L,) * d
+    ids = rng.integers(1, ncells + 1, size=shape)
+    ids[rng.random(shape) > fill] = 0
+    types = np.concatenate([[0], rng.integers(1, ntypes + 1, size=ncells)])
+    return ((types[ids].astype(np.uint32) << np.uint32(24)) | ids.astype(np.uint32)).astype(np.uint32)
+
+
+def _sym(rng, n, lo, hi):
+    a = rng.integers(lo, hi, size=(n, n))
+    return np.triu(a) + np.triu(a, 1).T
+
+
+@pytest.mark.parametrize("d,L", [(2, 4), (2, 8), (3, 4)])
+def test_delta_h_equals_energy_difference(d, L):
+    """dH for every (target, neighbour source) pair equals H(after) - H(before),
+    with dyadic lambdas (scale 4) and a symmetric J.  H sums over live cells,
+    so moves that delete a cell's last site are left out: there the
+    incremental dH (like kernels2d.cu:515-518) still charges the dead cell's
+    (0 - A0)^2 and (0 - P0)^2 terms."""
+    rng = np.random.default_rng(L * 10 + d)
+    ncells = 5
+    lat = _random_lattice(rng, L, d, ncells)
+    p = O.Params.make(3, J=_sym(rng, 3, 0, 30), lam_area=[0, 2.25, 3.5], target_area=[0, 4, 3],
+                      lam_perim=[0, 0.75, 1.5], target_perim=[0, 10, 6])
+    scale = O.energy_scale(p)
+    assert scale == 4
+    tab = O.tally(lat, d, ncells)
+    H0 = O.hamiltonian(lat, d, ncells, p)
+    flat = lat.reshape(-1)
+    offs = [o for o in np.ndindex(*(3,) * d) if any(x != 1 for x in o)]
+    checked = 0
+    for t in range(flat.size):
+        tc = np.unravel_index(t, lat.shape)  # (z, y, x) order
+        for o in offs:
+            sc = tuple((tc[i] + o[i] - 1) % L for i in range(d))
+            s = int(np.ravel_multi_index(sc, lat.shape))
+            if flat[s] == flat[t]:
+                continue
+            if (flat[t] & 0xFFFFFF) and tab.area[flat[t] & 0xFFFFFF] == 1:
+                continue
+            dh = O.delta_h(lat, d, tab, p, scale, t, s)
+            after = lat.copy()
+            after.reshape(-1)[t] = flat[s]
+            H1 = O.hamiltonian(after, d, ncells, p)
+            assert dh == round((H1 - H0) * scale), (t, s)
+            checked += 1
+    assert checked > 20
+
+
+def test_stickiness_closed_form():
+    """Stickiness adds -S[t_t][t_n] for every neighbour n unlike the target
+    (kernels2d.cu:400)."""
+    rng = np.random.default_rng(3)
+    L, d, ncells = 8, 2, 4
+    lat = _random_lattice(rng, L, d, ncells)
+    J = _sym(rng, 3, 0, 20)
+    S = rng.integers(0, 9, size=(3, 3))
+    p0 = O.Params.make(5, J=J)
+    p1 = O.Params.make(5, J=J, S=S)
+    tab = O.tally(lat, d, ncells)
+    flat = lat.reshape(-1)
+    for t in range(flat.size):
+        y, x = divmod(t, L)
+        s = ((y + 1) % L) * L + x
+        if flat[s] == flat[t]:
+            continue
+        nb = [lat[(y + dy) % L, (x + dx) % L] for dy in (-1, 0, 1) for dx in (-1, 0, 1) if dx or dy]
+        extra = sum(int(S[flat[t] >> 24, n >> 24]) for n in nb if n != flat[t])
+        assert O.delta_h(lat, d, tab, p1, 1, t, s) == O.delta_h(lat, d, tab, p0, 1, t, s) - extra
+
+
+# ------------------------------------------------------------- tallies ----
+
+def _numpy_recount(lat, d, ncells):
+    """Independent recount with np.roll: area and Moore/26 unlike-neighbour count."""
+    spin = (lat & 0xFFFFFF).astype(np.int64)
+    area = np.bincount(spin.reshape(-1), minlength=ncells + 1)
+    unlike = np.zeros(lat.shape, np.int64)
+    for o in np.ndindex(*(3,) * d):
+        if all(x == 1 for x in o):
+            continue
+        unlike += np.roll(spin, shift=[1 - x for x in o], axis=tuple(range(d))) != spin
+    perim = np.bincount(spin.reshape(-1), weights=unlike.reshape(-1), minlength=ncells + 1)
+    area[0] = perim[0] = 0
+    return area, perim.astype(np.int64)
+
+
+@pytest.mark.parametrize("d,L", [(2, 32), (3, 16)])
+def test_tally_matches_numpy_recount(d, L):
+    rng = np.random.default_rng(d)
+    lat = _random_lattice(rng, L, d, 30)
+    tab = O.tally(lat, d, 30)
+    area, perim = _numpy_recount(lat, d, 30)
+    np.testing.assert_array_equal(tab.area, area)
+    np.testing.assert_array_equal(tab.perim, perim)
+
+
+def test_tally_rejects_bad_ids():
+    with pytest.raises(ValueError):
+        O.tally(np.zeros((6, 6), np.uint32), 2, 1)  # L must be a power of two
+    lat = np.zeros((8, 8), np.uint32)
+    lat[0, 0] = 9
+    with pytest.raises(ValueError):
+        O.tally(lat, 2, 8)
+    lat[0, 0] = (20 << 24) | 1
+    with pytest.raises(ValueError):
+        O.tally(lat, 2, 8)
+
+
+def test_centroid_is_arithmetic_mean_for_compact_cells():
+    L = 32
+    lat = np.zeros((L, L), np.uint32)
+    lat[3:8, 10:14] = (1 << 24) | 1
+    lat[20:23, 2:9] = (1 << 24) | 2
+    lat[0:2, 30:32] = (1 << 24) | 3
+    lat[30:32, 30:32] = (1 << 24) | 3  # wraps across the y boundary
+    tab = O.tally(lat, 2, 3)
+    cen = O.centroids(L, 2, tab)
+    for c in (1, 2):
+        ys, xs = np.nonzero((lat & 0xFFFFFF) == c)
+        assert cen[c, 0] == pytest.approx(xs.mean())
+        assert cen[c, 1] == pytest.approx(ys.mean())
+    # the wrapped cell covers rows 30, 31, 0, 1: unwrapped mean 31.5
+    assert cen[3, 0] == pytest.approx(30.5)
+    assert cen[3, 1] == pytest.approx(31.5)
+    assert np.all(np.isnan(cen[0]))
+
+
+# ------------------------------------------------------------- sweep ------
+
+def _sorting_small(L=64, seed=0):
+    rng = np.random.default_rng(seed)
+    lat = np.zeros((L, L), np.uint32)
+    n = 0
+    for y0 in range(8, L - 8, 6):
+        for x0 in range(8, L - 8, 6):
+            n += 1
+            lat[y0:y0 + 6, x0:x0 + 6] = (np.uint32(rng.integers(1, 3)) << np.uint32(24)) | np.uint32(n)
+    J = [[0, 16, 16], [16, 2, 11], [16, 11, 14]]
+    p = O.Params.make(10, J=J, lam_area=[0, 2, 2], target_area=[0, 36, 36],
+                      lam_perim=[0, 0.5, 0.5], target_perim=[0, 60, 60])
+    return lat, n, p
+
+
+@pytest.mark.parametrize("c,u", [(1, 1), (2, 4), (4, 5), (8, 64)])
+def test_sweep_keeps_tables_exact_and_is_thread_invariant(c, u):
+    lat, n, p = _sorting_small()
+    tab = O.tally(lat, 2, n)
+    lat1, tab1 = lat.copy(), tab.copy()
+    O.sweep(lat, 2, tab, p, c, u, seed=11, mcs0=0, nmcs=25, nthreads=1)
+    O.sweep(lat1, 2, tab1, p, c, u, seed=11, mcs0=0, nmcs=25, nthreads=8)
+    np.testing.assert_array_equal(lat, lat1)
+    np.testing.assert_array_equal(tab.csum, tab1.csum)
+    area, perim = _numpy_recount(lat, 2, n)
+    np.testing.assert_array_equal(tab.area, area)
+    np.testing.assert_array_equal(tab.perim, perim)
+    assert np.count_nonzero(lat != _sorting_small()[0]) > 50
+
+
+def test_sweep_3d_tables_exact():
+    rng = np.random.default_rng(5)
+    L = 16
+    lat = np.zeros((L, L, L), np.uint32)
+    for k in range(1, 9):
+        z, y, x = rng.integers(0, L - 4, size=3)
+        lat[z:z + 4, y:y + 4, x:x + 4] = (1 << 24) | k
+    p = O.Params.make(20, J=[[0, 10], [10, 20]], lam_area=[0, 5], target_area=[0, 80],
+                      lam_perim=[0, 0.5], target_perim=[0, 400])
+    tab = O.tally(lat, 3, 8)
+    O.sweep(lat, 3, tab, p, 2, 8, seed=3, mcs0=0, nmcs=10, nthreads=4)
+    area, perim = _numpy_recount(lat, 3, 8)
+    np.testing.assert_array_equal(tab.area, area)
+    np.testing.assert_array_equal(tab.perim, perim)
+
+
+def test_sweep_split_equals_single_run():
+    lat, n, p = _sorting_small(seed=2)
+    a, ta = lat.copy(), O.tally(lat, 2, n)
+    b, tb = lat.copy(), O.tally(lat, 2, n)
+    O.sweep(a, 2, ta, p, 2, 4, 9, 0, 30)
+    O.sweep(b, 2, tb, p, 2, 4, 9, 0, 12)
+    O.sweep(b, 2, tb, p, 2, 4, 9, 12, 18)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_sweep_copies_stay_local():
+    """With c = 1, u = 1 one MCS is four colour phases and every site is a
+    target exactly once.  A target copies a neighbour of another colour,
+    which itself may have copied in an earlier phase, so after one MCS every
+    site holds a value found within distance 4 of it in the initial lattice,
+    and at distance 1 for sites of the first colour phase (even x, even y)."""
+    lat, n, p = _sorting_small(seed=4)
+    p.temperature = 1000.0  # accept almost everything
+    before = lat.copy()
+    tab = O.tally(lat, 2, n)
+    O.sweep(lat, 2, tab, p, 1, 1, 5, 0, 1)
+    L = lat.shape[0]
+    changed = np.argwhere(lat != before)
+    assert len(changed) > 100
+    for y, x in changed:
+        r = 1 if (x % 2 == 0 and y % 2 == 0) else 4
+        win = before[np.ix_([(y + k) % L for k in range(-r, r + 1)],
+                            [(x + k) % L for k in range(-r, r + 1)])]
+        assert lat[y, x] in win
+
+
+def test_sweep_rejects_bad_schedule():
+    lat, n, p = _sorting_small()
+    tab = O.tally(lat, 2, n)
+    for c, u in [(3, 1), (128, 1), (2, 0), (2, 5)]:
+        with pytest.raises(ValueError):
+            O.sweep(lat, 2, tab, p, c, u, 1, 0, 1)
