@@ -25,19 +25,19 @@ struct DevParams {
     int32_t p0[kNTypes];           // target perimeter            (settings.h:18)
 };
 
-// Arguments of one colour phase of the checkerboard sweep.
+// Constants of one sweep (one run() call).
 //
-// Cell tables are packed int64 rows: perimeter * 2^32 + area.  The phase
-// reads R (its start-of-phase snapshot), adds every accepted flip's delta to
-// W and to Dcur, and first folds the previous phase's deltas Dprev into W
-// (then clears Dprev).  With R/W and Dcur/Dprev swapping each phase, W
-// equals the end-of-phase state without any extra launch or barrier.
-struct PhaseArgs {
+// Cell tables are packed int64 rows: perimeter * 2^32 + area.  Colour phase
+// g reads tab[g & 1] (its start-of-phase snapshot), adds every accepted
+// flip's delta to W = tab[(g + 1) & 1] and to del[g & 1], and first folds the
+// previous phase's deltas del[(g + 1) & 1] into W (then clears them).  W is
+// never read during the phase, so after it W holds the end-of-phase state
+// with no extra launch or barrier.  A final settle folds the last deltas
+// into the lagging buffer, leaving both buffers equal and both deltas zero.
+struct SweepConsts {
     uint32_t* lat;
-    const int64_t* R;
-    int64_t* W;
-    int64_t* Dcur;
-    int64_t* Dprev;
+    int64_t* tab[2];
+    int64_t* del[2];
     const int32_t* anchor;  // [(n+1)*3]
     int64_t* csum;          // [(n+1)*3]
     const DevParams* prm;
@@ -46,19 +46,44 @@ struct PhaseArgs {
     int ncells;
     int lgL;                // log2 L
     int lc;                 // log2 positions_per_checkerboard
-    int lh;                 // log2 (same-colour sub-blocks per axis) = lgL - lc - 1
     int u;                  // updates_per_checkerboard_switch
+    int rounds;             // c^d / u colour rounds per MCS
+    uint32_t k0, k1;        // Philox key (seed)
+};
+
+// One colour phase per launch.
+struct PhaseArgs {
+    SweepConsts k;
     int cx, cy, cz;         // colour parity
     uint32_t phase;         // round * 2^d + colour
     uint32_t mcs;           // Monte Carlo step counter
-    uint32_t k0, k1;        // Philox key (seed)
+    int g;                  // phase counter since the last settle (parity selects buffers)
+    int lh;                 // log2 (same-colour sub-blocks per axis)
     int nsub;               // active sub-blocks in this phase
+};
+
+// A whole run in one cooperative launch, phases separated by grid barriers.
+struct PersistArgs {
+    SweepConsts k;
+    uint32_t mcs0;          // first MCS counter
+    int nmcs;               // MCS in this launch
+    int lh;                 // log2 (same-colour sub-blocks per axis)
+    unsigned int* bar;      // grid barrier {arrivals, generation}
+    int ltx, lty;           // log2 LDS tile edge (2D tiled kernel)
+    int lntx;               // log2 tiles per row
 };
 
 // Host-side launchers (kernels.hip).
 hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+// Cooperative persistent launches.  lds2d selects the 2D LDS-tiled kernel.
+// max_blocks returns the co-resident block limit for the chosen kernel.
+hipError_t persist_max_blocks(int d, bool lds2d, size_t lds_bytes, int* max_blocks);
+hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, size_t lds_bytes,
+                          hipStream_t s);
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,
                         uint32_t* first, int32_t* anchor, int64_t* csum, hipStream_t s);
 hipError_t launch_settle(int64_t* tab, int64_t* del, int ncells, hipStream_t s);
+
+constexpr int kBlock = 256;  // threads per workgroup of the sweep kernels
 
 }  // namespace gcpm

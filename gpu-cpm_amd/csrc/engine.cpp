@@ -5,8 +5,10 @@
 // __constant__ Settings, kernels.cu:48-53), validated inputs, error codes
 // instead of printed CUDA errors (cpm.cu:52-56), and a deterministic,
 // resumable schedule keyed by (seed, MCS counter).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <random>
@@ -44,6 +46,14 @@ int ilog2(long v) {
 }
 
 constexpr int kMaxScaleLog2 = 8;
+
+// How one run is launched.  All three forms give identical results.
+struct Plan {
+    enum Mode { kPhase, kGlobal, kLds2d } mode = kPhase;
+    int blocks = 0;        // cooperative grid size
+    int ltile = 0;         // log2 tile edge (kLds2d)
+    size_t lds_bytes = 0;  // dynamic LDS per block (kLds2d)
+};
 constexpr int kMaxThr = 1 << 22;
 
 }  // namespace
@@ -79,6 +89,7 @@ struct cpm_engine {
     int64_t* d_csum = nullptr;
     uint32_t* d_first = nullptr;
     gcpm::DevParams* d_prm = nullptr;
+    unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation}
     uint32_t* d_thr = nullptr;
     int thr_cap = 0;
     int nthr = 0;
@@ -117,6 +128,7 @@ struct cpm_engine {
         (void)hipFree(d_csum);
         (void)hipFree(d_first);
         (void)hipFree(d_prm);
+        (void)hipFree(d_bar);
         (void)hipFree(d_thr);
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -135,6 +147,44 @@ struct cpm_engine {
             if (ok) return 1 << k;
         }
         return -1;
+    }
+
+    // Launch form for a run with checkerboard edge c.  GPUCPM_MODE=phase |
+    // global | lds forces a form; GPUCPM_TILE sets the 2D tile edge.
+    int plan(int c, Plan* p) {
+        const char* env = std::getenv("GPUCPM_MODE");
+        const std::string want = env ? env : "phase";
+        if (want == "phase") {
+            p->mode = Plan::kPhase;
+            return CPM_OK;
+        }
+        if (d == 2 && want != "global") {
+            int t = std::max(2 * c, std::min(64, L / 16));
+            if (const char* te = std::getenv("GPUCPM_TILE")) t = std::atoi(te);
+            if (t > L) t = L;
+            if (is_pow2(t) && t >= 2 * c) {
+                const int lt = ilog2(t);
+                const int blocks = 1 << (2 * (lgL - lt));
+                const size_t bytes = (size_t)(t + 2) * (t + 2) * 4;
+                int maxb = 0;
+                HIP_TRY(gcpm::persist_max_blocks(2, true, bytes, &maxb));
+                if (blocks <= maxb) {
+                    p->mode = Plan::kLds2d;
+                    p->blocks = blocks;
+                    p->ltile = lt;
+                    p->lds_bytes = bytes;
+                    return CPM_OK;
+                }
+            }
+            if (want == "lds") return fail(CPM_EINVAL, "GPUCPM_MODE=lds: tiles do not fit co-resident");
+        }
+        int maxb = 0;
+        HIP_TRY(gcpm::persist_max_blocks(d, false, 0, &maxb));
+        const long nsub = 1L << ((lgL - ilog2(c) - 1) * d);
+        const long work = std::max(nsub, (long)n + 1);
+        p->mode = Plan::kGlobal;
+        p->blocks = (int)std::max(1L, std::min((long)maxb, (work + gcpm::kBlock - 1) / gcpm::kBlock));
+        return CPM_OK;
     }
 
     int upload_params() {
@@ -255,6 +305,8 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     if ((err = hipMalloc(&g->d_csum, rows * 3 * 8)) != hipSuccess) return bail(err, "hipMalloc(csum)");
     if ((err = hipMalloc(&g->d_first, rows * 4)) != hipSuccess) return bail(err, "hipMalloc(first)");
     if ((err = hipMalloc(&g->d_prm, sizeof(gcpm::DevParams))) != hipSuccess) return bail(err, "hipMalloc(params)");
+    if ((err = hipMalloc(&g->d_bar, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
+    if ((err = hipMemset(g->d_bar, 0, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
     *out = g;
     return CPM_OK;
 }
@@ -394,53 +446,85 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
     if (int rc = h->upload_params()) return rc;
 
     // cpm.cu:405: iterations / inner_iterations launches of inner_iterations MCS.
-    const long total = (long)(a->iterations / a->inner_iterations) * a->inner_iterations;
-    const int rounds = (int)(cd / u);  // kernels2d.cu:95
-    const int ncol = 1 << h->d;
-    const int lc = ilog2(c);
-    const int lh = h->lgL - lc - 1;
-    gcpm::PhaseArgs pa{};
-    pa.lat = h->d_lat;
-    pa.anchor = h->d_anchor;
-    pa.csum = h->d_csum;
-    pa.prm = h->d_prm;
-    pa.thr = h->d_thr;
-    pa.nthr = h->nthr;
-    pa.ncells = h->n;
-    pa.lgL = h->lgL;
-    pa.lc = lc;
-    pa.lh = lh;
-    pa.u = u;
-    pa.k0 = (uint32_t)h->seed;
-    pa.k1 = (uint32_t)(h->seed >> 32);
-    pa.nsub = 1 << (lh * h->d);
-    uint64_t g = 0;  // phase counter since the tables were last settled
-    for (long m = 0; m < total; m++) {
-        pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
-        for (int r = 0; r < rounds; r++) {
-            for (int col = 0; col < ncol; col++, g++) {
-                pa.cx = h->d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
-                pa.cy = h->d == 3 ? (col >> 1) & 1 : col & 1;
-                pa.cz = h->d == 3 ? col & 1 : 0;
-                pa.phase = (uint32_t)(r * ncol + col);
-                pa.R = h->d_tab[g & 1];
-                pa.W = h->d_tab[(g + 1) & 1];
-                pa.Dcur = h->d_del[g & 1];
-                pa.Dprev = h->d_del[(g + 1) & 1];
-                hipEvent_t e0, e1;
-                if (int rc = h->next_events(&e0, &e1)) return rc;
-                HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1));
-                if (h->timing) {
-                    h->t_launches++;
-                    h->t_updates += (int64_t)pa.nsub * u;
+    const long chunks = a->iterations / a->inner_iterations;
+    const long total = chunks * a->inner_iterations;
+    gcpm::SweepConsts k{};
+    k.lat = h->d_lat;
+    for (int i = 0; i < 2; i++) {
+        k.tab[i] = h->d_tab[i];
+        k.del[i] = h->d_del[i];
+    }
+    k.anchor = h->d_anchor;
+    k.csum = h->d_csum;
+    k.prm = h->d_prm;
+    k.thr = h->d_thr;
+    k.nthr = h->nthr;
+    k.ncells = h->n;
+    k.lgL = h->lgL;
+    k.lc = ilog2(c);
+    k.u = u;
+    k.rounds = (int)(cd / u);  // kernels2d.cu:95
+    k.k0 = (uint32_t)h->seed;
+    k.k1 = (uint32_t)(h->seed >> 32);
+    const int lh = h->lgL - k.lc - 1;
+    if (total == 0) return CPM_OK;
+
+    Plan plan;
+    if (int rc = h->plan(c, &plan)) return rc;
+    const int64_t updates_per_mcs = (int64_t)k.rounds * (1 << h->d) * ((int64_t)1 << (lh * h->d)) * u;
+
+    if (plan.mode == Plan::kPhase) {
+        gcpm::PhaseArgs pa{};
+        pa.k = k;
+        pa.lh = lh;
+        pa.nsub = 1 << (lh * h->d);
+        const int ncol = 1 << h->d;
+        int g = 0;  // phase counter since the tables were last settled
+        for (long m = 0; m < total; m++) {
+            pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
+            for (int r = 0; r < k.rounds; r++) {
+                for (int col = 0; col < ncol; col++, g++) {
+                    pa.cx = h->d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
+                    pa.cy = h->d == 3 ? (col >> 1) & 1 : col & 1;
+                    pa.cz = h->d == 3 ? col & 1 : 0;
+                    pa.phase = (uint32_t)(r * ncol + col);
+                    pa.g = g;
+                    hipEvent_t e0, e1;
+                    if (int rc = h->next_events(&e0, &e1)) return rc;
+                    HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1));
+                    if (h->timing) {
+                        h->t_launches++;
+                        h->t_updates += (int64_t)pa.nsub * u;
+                    }
                 }
             }
+            g &= 1;  // only the parity matters
         }
-    }
-    if (g > 0) {
         // Fold the last phase's deltas into the lagging buffer: both table
         // buffers then hold the current state and both delta buffers are zero.
         HIP_TRY(gcpm::launch_settle(h->d_tab[(g + 1) & 1], h->d_del[(g + 1) & 1], h->n, h->stream));
+    } else {
+        gcpm::PersistArgs pa{};
+        pa.k = k;
+        pa.lh = lh;
+        pa.bar = h->d_bar;
+        pa.ltx = plan.ltile;
+        pa.lty = plan.ltile;
+        pa.lntx = h->lgL - plan.ltile;
+        pa.nmcs = a->inner_iterations;
+        const bool lds = plan.mode == Plan::kLds2d;
+        for (long ch = 0; ch < chunks; ch++) {
+            pa.mcs0 = (uint32_t)(h->mcs + (uint64_t)ch * a->inner_iterations);
+            hipEvent_t e0, e1;
+            if (int rc = h->next_events(&e0, &e1)) return rc;
+            if (e0) HIP_TRY(hipEventRecord(e0, h->stream));
+            HIP_TRY(gcpm::launch_persist(h->d, lds, pa, plan.blocks, plan.lds_bytes, h->stream));
+            if (e1) HIP_TRY(hipEventRecord(e1, h->stream));
+            if (h->timing) {
+                h->t_launches++;
+                h->t_updates += updates_per_mcs * a->inner_iterations;
+            }
+        }
     }
     h->mcs += (uint64_t)total;
     return CPM_OK;

@@ -1,9 +1,12 @@
 // kernels.hip -- HIP kernels of the MI355X CPM engine (gfx950).
 //
-// One colour phase of the checkerboard copy-attempt sweep per launch
-// (reference: src/kernels2d.cu:44-583, src/kernels3d.cu:51-594), the tally
-// rebuild run at push (kernels2d.cu:5-41, kernels3d.cu:3-47) and the
-// end-of-run settle of the double-buffered cell tables.
+// The checkerboard copy-attempt sweep (reference: src/kernels2d.cu:44-583,
+// src/kernels3d.cu:51-594) in three launch forms that compute identical
+// results: one colour phase per launch, one cooperative launch per run with
+// the lattice in global memory, and (2D) one cooperative launch per run with
+// each block's tile resident in LDS.  Also the tally rebuild run at push
+// (kernels2d.cu:5-41, kernels3d.cu:3-47) and the end-of-run settle of the
+// double-buffered cell tables.
 #include <hip/hip_ext.h>
 
 #include "cpm_device.h"
@@ -51,64 +54,93 @@ __device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }
 
-// One colour phase.  Thread q owns same-colour sub-block q and makes its u
-// copy attempts in order; sub-blocks of one colour are two sub-blocks apart,
-// so no attempt reads a site another thread writes in this phase.
+// Lattice held in global memory (HBM / L2 / MALL).
 template <int D>
-__global__ __launch_bounds__(256) void k_sweep_phase(PhaseArgs a) {
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gsz = gridDim.x * blockDim.x;
-    // Fold the previous phase's deltas into W (W is never read in a phase).
-    for (int ci = gtid; ci <= a.ncells; ci += gsz) {
-        const int64_t dv = a.Dprev[ci];
-        if (dv != 0) {
-            atomic_add_i64(a.W + ci, dv);
-            a.Dprev[ci] = 0;
-        }
+struct GlobalLat {
+    uint32_t* p;
+    int lgL, Lm;
+    struct Pos {
+        int x, y, z;
+        uint32_t i;
+    };
+    __device__ __forceinline__ Pos at(int x, int y, int z) const {
+        return {x, y, z, site_index<D>(x, y, z, lgL)};
     }
-    if (gtid >= a.nsub) return;
+    __device__ __forceinline__ uint32_t self(const Pos& q) const { return p[q.i]; }
+    __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int dz) const {
+        return p[site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL)];
+    }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { p[q.i] = v; }
+};
 
+// 2D tile of the lattice resident in LDS with a one-site halo ring; row
+// stride S = tile width + 2.  Writes go to LDS and through to global memory,
+// where neighbouring tiles read them as their halo in the next phase.
+struct LdsLat2 {
+    uint32_t* s;
+    uint32_t* g;
+    int S, x0, y0, lgL;
+    struct Pos {
+        int li;
+        uint32_t gi;
+    };
+    __device__ __forceinline__ Pos at(int x, int y, int) const {
+        return {(y - y0 + 1) * S + (x - x0 + 1), (uint32_t)x | ((uint32_t)y << lgL)};
+    }
+    __device__ __forceinline__ uint32_t self(const Pos& q) const { return s[q.li]; }
+    __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int) const {
+        return s[q.li + dy * S + dx];
+    }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const {
+        s[q.li] = v;
+        g[q.gi] = v;
+    }
+};
+
+// The u copy attempts of sub-block (bx, by, bz) in one colour phase: the
+// copy attempt of kernels2d.cu:197-570 / kernels3d.cu:165-583 in the
+// shared-mode roles (target drawn in the sub-block, source a uniformly drawn
+// neighbour), with the deterministic RNG, snapshot tables and integer
+// acceptance described in SweepConsts.
+template <int D, class Lat>
+__device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k, int bx,
+                                                   int by, int bz, uint32_t mcs, uint32_t phase,
+                                                   const int64_t* __restrict__ R, int64_t* W,
+                                                   int64_t* Dc) {
     constexpr int N = D == 3 ? 26 : 8;
     constexpr int NB = D == 3 ? 27 : 9;
     constexpr int CENTRE = NB / 2;
-    const int lgL = a.lgL;
-    const int Lm = (1 << lgL) - 1;
-    const int lc = a.lc;
+    const int lc = k.lc;
     const uint32_t cm = (1u << lc) - 1u;
-    const int hm = (1 << a.lh) - 1;
-    const int bx = 2 * (gtid & hm) + a.cx;
-    const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
-    const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
-    const int lsb = lgL - lc;  // log2 sub-blocks per axis
+    const int lsb = k.lgL - lc;  // log2 sub-blocks per axis
+    const int Lm = (1 << k.lgL) - 1;
     const uint32_t sb = (uint32_t)bx | ((uint32_t)by << lsb) |
                         (D == 3 ? ((uint32_t)bz << (2 * lsb)) : 0u);
-    uint32_t* __restrict__ lat = a.lat;
-    const DevParams* __restrict__ P = a.prm;
+    const DevParams* __restrict__ P = k.prm;
 
-    for (int i = 0; i < a.u; i++) {
+    for (int i = 0; i < k.u; i++) {
         uint32_t w[4];
-        philox4x32_10(sb, a.mcs, a.phase, (uint32_t)i, a.k0, a.k1, w);
+        philox4x32_10(sb, mcs, phase, (uint32_t)i, k.k0, k.k1, w);
         const int tx = (bx << lc) + (int)(w[0] & cm);
         const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
         const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
         const int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
         int ox, oy, oz;
         block_offset<D>(dir < CENTRE ? dir : dir + 1, ox, oy, oz);
-        const uint32_t ti = site_index<D>(tx, ty, tz, lgL);
-        const uint32_t si = site_index<D>((tx + ox) & Lm, (ty + oy) & Lm, (tz + oz) & Lm, lgL);
-        const uint32_t vs = lat[si];
-        const uint32_t vt = lat[ti];
+        const typename Lat::Pos t = lat.at(tx, ty, tz);
+        const uint32_t vs = lat.nb(t, ox, oy, oz);
+        const uint32_t vt = lat.self(t);
         if (vs == vt) continue;  // kernels2d.cu:276
 
         const uint32_t ts = vs >> 24, tt = vt >> 24;
         int32_t adh = 0;
         int as = 0, at = 0;
 #pragma unroll
-        for (int k = 0; k < NB; k++) {
-            if (k == CENTRE) continue;
+        for (int n = 0; n < NB; n++) {
+            if (n == CENTRE) continue;
             int nx, ny, nz;
-            block_offset<D>(k, nx, ny, nz);
-            const uint32_t vn = lat[site_index<D>((tx + nx) & Lm, (ty + ny) & Lm, (tz + nz) & Lm, lgL)];
+            block_offset<D>(n, nx, ny, nz);
+            const uint32_t vn = lat.nb(t, nx, ny, nz);
             const uint32_t tn = vn >> 24;
             if (vn != vs) adh += P->J[ts][tn]; else as++;
             if (vn != vt) adh -= P->JS[tt][tn]; else at++;
@@ -116,44 +148,224 @@ __global__ __launch_bounds__(256) void k_sweep_phase(PhaseArgs a) {
         int64_t dh = adh;
         const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
         if (cs) {
-            const int64_t row = a.R[cs];
+            const int64_t row = R[cs];
             const int64_t A = (int64_t)(uint32_t)row, Pm = row >> 32;
             const int64_t dP = N - 2 * as;
             dh += (int64_t)P->la[ts] * (1 + 2 * A - 2 * (int64_t)P->a0[ts]);
             dh += (int64_t)P->lp[ts] * dP * (dP + 2 * (Pm - P->p0[ts]));
         }
         if (ct) {
-            const int64_t row = a.R[ct];
+            const int64_t row = R[ct];
             const int64_t A = (int64_t)(uint32_t)row, Pm = row >> 32;
             const int64_t dP = 2 * at - N;
             dh += (int64_t)P->la[tt] * (1 - 2 * A + 2 * (int64_t)P->a0[tt]);
             dh += (int64_t)P->lp[tt] * dP * (dP + 2 * (Pm - P->p0[tt]));
         }
-        const bool accept = dh <= 0 || (dh < a.nthr && w[2] < a.thr[dh]);
+        const bool accept = dh <= 0 || (dh < k.nthr && w[2] < k.thr[dh]);
         if (!accept) continue;
 
-        lat[ti] = vs;  // kernels2d.cu:548
+        lat.put(t, vs);  // kernels2d.cu:548
         const int pos[3] = {tx, ty, tz};
+        const int half = (Lm + 1) / 2;
         if (cs) {
             const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
-            atomic_add_i64(a.W + cs, delta);
-            atomic_add_i64(a.Dcur + cs, delta);
+            atomic_add_i64(W + cs, delta);
+            atomic_add_i64(Dc + cs, delta);
 #pragma unroll
             for (int ax = 0; ax < D; ax++) {
-                const int v = pos[ax] - a.anchor[3 * cs + ax];
-                atomic_add_i64(a.csum + 3 * cs + ax, (int64_t)(((v + (Lm + 1) / 2) & Lm) - (Lm + 1) / 2));
+                const int v = pos[ax] - k.anchor[3 * cs + ax];
+                atomic_add_i64(k.csum + 3 * cs + ax, (int64_t)(((v + half) & Lm) - half));
             }
         }
         if (ct) {
             const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
-            atomic_add_i64(a.W + ct, delta);
-            atomic_add_i64(a.Dcur + ct, delta);
+            atomic_add_i64(W + ct, delta);
+            atomic_add_i64(Dc + ct, delta);
 #pragma unroll
             for (int ax = 0; ax < D; ax++) {
-                const int v = pos[ax] - a.anchor[3 * ct + ax];
-                atomic_add_i64(a.csum + 3 * ct + ax, -(int64_t)(((v + (Lm + 1) / 2) & Lm) - (Lm + 1) / 2));
+                const int v = pos[ax] - k.anchor[3 * ct + ax];
+                atomic_add_i64(k.csum + 3 * ct + ax, -(int64_t)(((v + half) & Lm) - half));
             }
         }
+    }
+}
+
+// Fold the previous phase's deltas into W (W is never read in a phase).
+__device__ __forceinline__ void fold_deltas(int64_t* W, int64_t* Dp, int ncells, int gtid, int gsz) {
+    for (int ci = gtid; ci <= ncells; ci += gsz) {
+        const int64_t dv = Dp[ci];
+        if (dv != 0) {
+            atomic_add_i64(W + ci, dv);
+            Dp[ci] = 0;
+        }
+    }
+}
+
+// Colour of phase index col: cbX outermost, then cbY (, cbZ) (kernels2d.cu:179-180).
+template <int D>
+__device__ __forceinline__ void colour_of(int col, int& cx, int& cy, int& cz) {
+    cx = D == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
+    cy = D == 3 ? (col >> 1) & 1 : col & 1;
+    cz = D == 3 ? col & 1 : 0;
+}
+
+// One colour phase.  Thread q owns same-colour sub-block q and makes its u
+// copy attempts in order; sub-blocks of one colour are two sub-blocks apart,
+// so no attempt reads a site another thread writes in this phase.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gsz = gridDim.x * blockDim.x;
+    const SweepConsts& k = a.k;
+    int64_t* W = k.tab[(a.g + 1) & 1];
+    fold_deltas(W, k.del[(a.g + 1) & 1], k.ncells, gtid, gsz);
+    if (gtid >= a.nsub) return;
+    const int hm = (1 << a.lh) - 1;
+    const int bx = 2 * (gtid & hm) + a.cx;
+    const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
+    const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
+    const GlobalLat<D> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+    sub_block_attempts<D>(lat, k, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1], W, k.del[a.g & 1]);
+}
+
+// Grid-wide barrier of a cooperative launch (all blocks co-resident).
+// bar[0] counts arrivals, bar[1] is the generation; both persist in device
+// memory between launches (bar[0] returns to 0 at every release).  Every
+// thread fences at agent scope before arriving and after leaving, so plain
+// global stores of one phase are visible to plain loads of the next.
+__device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& gen) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int g = gen;
+        const unsigned int prev =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+                __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    gen++;
+    __syncthreads();
+    __threadfence();
+}
+
+// End of a persistent run: fold the last phase's deltas into the lagging
+// table buffer (after a barrier), leaving both buffers equal.
+__device__ __forceinline__ void settle_in_kernel(const SweepConsts& k, int g, int gtid, int gsz) {
+    int64_t* T = k.tab[(g + 1) & 1];
+    int64_t* Dl = k.del[(g + 1) & 1];
+    for (int ci = gtid; ci <= k.ncells; ci += gsz) {
+        T[ci] += Dl[ci];
+        Dl[ci] = 0;
+    }
+}
+
+// Whole run, lattice in global memory: grid-stride over the same-colour
+// sub-blocks of each phase, grid barrier between phases.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_persist_global(PersistArgs a) {
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gsz = gridDim.x * blockDim.x;
+    const SweepConsts& k = a.k;
+    unsigned int gen = __hip_atomic_load(&a.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const GlobalLat<D> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+    const int lh = a.lh, hm = (1 << lh) - 1;
+    const int nsub = 1 << (D * lh);
+    constexpr int NCOL = 1 << D;
+    int g = 0;
+    for (int m = 0; m < a.nmcs; m++) {
+        const uint32_t mcs = a.mcs0 + (uint32_t)m;
+        for (int r = 0; r < k.rounds; r++) {
+            for (int col = 0; col < NCOL; col++, g++) {
+                if (g) grid_barrier(a.bar, gen);
+                int cx, cy, cz;
+                colour_of<D>(col, cx, cy, cz);
+                int64_t* W = k.tab[(g + 1) & 1];
+                fold_deltas(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
+                const int64_t* R = k.tab[g & 1];
+                int64_t* Dc = k.del[g & 1];
+                const uint32_t phase = (uint32_t)(r * NCOL + col);
+                for (int q = gtid; q < nsub; q += gsz) {
+                    const int bx = 2 * (q & hm) + cx;
+                    const int by = 2 * ((q >> lh) & hm) + cy;
+                    const int bz = D == 3 ? 2 * (q >> (2 * lh)) + cz : 0;
+                    sub_block_attempts<D>(lat, k, bx, by, bz, mcs, phase, R, W, Dc);
+                }
+            }
+        }
+    }
+    if (g) {
+        grid_barrier(a.bar, gen);
+        settle_in_kernel(k, g, gtid, gsz);
+    }
+}
+
+// Whole 2D run with each block's tile resident in LDS.  Block b owns the
+// tile at ((b mod ntx) << ltx, (b / ntx) << lty); per phase it refreshes its
+// halo ring from global memory (neighbouring tiles write through), makes the
+// attempts of the tile's same-colour sub-blocks against LDS, and meets the
+// grid barrier.
+__global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
+    extern __shared__ uint32_t s[];
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gsz = gridDim.x * blockDim.x;
+    const SweepConsts& k = a.k;
+    unsigned int gen = __hip_atomic_load(&a.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int TX = 1 << a.ltx, TY = 1 << a.lty, S = TX + 2;
+    const int x0 = (int)(blockIdx.x & ((1u << a.lntx) - 1)) << a.ltx;
+    const int y0 = (int)(blockIdx.x >> a.lntx) << a.lty;
+    const int Lm = (1 << k.lgL) - 1;
+    const uint32_t* __restrict__ gl = k.lat;
+    // tile + halo
+    for (int i = threadIdx.x; i < S * (TY + 2); i += blockDim.x) {
+        const int ly = i / S, lx = i - ly * S;
+        s[i] = gl[(uint32_t)((x0 + lx - 1) & Lm) | ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
+    }
+    __syncthreads();
+    const LdsLat2 lat{s, k.lat, S, x0, y0, k.lgL};
+    const int lq = a.ltx - k.lc - 1;  // log2 same-colour sub-blocks per tile row
+    const int nq = 1 << (lq + a.lty - k.lc - 1);
+    const int ring = 2 * S + 2 * TY;
+    int g = 0;
+    for (int m = 0; m < a.nmcs; m++) {
+        const uint32_t mcs = a.mcs0 + (uint32_t)m;
+        for (int r = 0; r < k.rounds; r++) {
+            for (int col = 0; col < 4; col++, g++) {
+                if (g) {
+                    grid_barrier(a.bar, gen);
+                    for (int i = threadIdx.x; i < ring; i += blockDim.x) {
+                        int lx, ly;
+                        if (i < S) { lx = i; ly = 0; }
+                        else if (i < 2 * S) { lx = i - S; ly = TY + 1; }
+                        else if (i < 2 * S + TY) { lx = 0; ly = i - 2 * S + 1; }
+                        else { lx = TX + 1; ly = i - 2 * S - TY + 1; }
+                        s[ly * S + lx] = gl[(uint32_t)((x0 + lx - 1) & Lm) |
+                                            ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
+                    }
+                    __syncthreads();
+                }
+                int cx, cy, cz;
+                colour_of<2>(col, cx, cy, cz);
+                int64_t* W = k.tab[(g + 1) & 1];
+                fold_deltas(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
+                const int64_t* R = k.tab[g & 1];
+                int64_t* Dc = k.del[g & 1];
+                const uint32_t phase = (uint32_t)(r * 4 + col);
+                for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+                    const int bx = (x0 >> k.lc) + 2 * (q & ((1 << lq) - 1)) + cx;
+                    const int by = (y0 >> k.lc) + 2 * (q >> lq) + cy;
+                    sub_block_attempts<2>(lat, k, bx, by, 0, mcs, phase, R, W, Dc);
+                }
+            }
+        }
+    }
+    if (g) {
+        grid_barrier(a.bar, gen);
+        settle_in_kernel(k, g, gtid, gsz);
     }
 }
 
@@ -216,15 +428,42 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
 }
 
 hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    const int work = a.nsub > a.ncells + 1 ? a.nsub : a.ncells + 1;
-    const int blocks = (work + 255) / 256;
+    const int work = a.nsub > a.k.ncells + 1 ? a.nsub : a.k.ncells + 1;
+    const int blocks = (work + kBlock - 1) / kBlock;
     // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
     // (null events: plain launch).
     if (d == 2)
-        hipExtLaunchKernelGGL(k_sweep_phase<2>, dim3(blocks), dim3(256), 0, s, e0, e1, 0, a);
+        hipExtLaunchKernelGGL(k_sweep_phase<2>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     else
-        hipExtLaunchKernelGGL(k_sweep_phase<3>, dim3(blocks), dim3(256), 0, s, e0, e1, 0, a);
+        hipExtLaunchKernelGGL(k_sweep_phase<3>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     return hipGetLastError();
+}
+
+static const void* persist_kernel(int d, bool lds2d) {
+    if (lds2d) return reinterpret_cast<const void*>(&k_persist_lds2d);
+    return d == 2 ? reinterpret_cast<const void*>(&k_persist_global<2>)
+                  : reinterpret_cast<const void*>(&k_persist_global<3>);
+}
+
+hipError_t persist_max_blocks(int d, bool lds2d, size_t lds_bytes, int* max_blocks) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+        return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel(d, lds2d), kBlock,
+                                                          lds_bytes)) != hipSuccess)
+        return e;
+    *max_blocks = per_cu * cus;
+    return hipSuccess;
+}
+
+hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, size_t lds_bytes,
+                          hipStream_t s) {
+    PersistArgs copy = a;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(persist_kernel(d, lds2d), dim3(blocks), dim3(kBlock), args,
+                                      (unsigned int)lds_bytes, s);
 }
 
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,

@@ -129,3 +129,33 @@ def test_full_size_tables_equal_recount():
     np.testing.assert_array_equal(table["area"], tab.area)
     np.testing.assert_array_equal(table["perimeter"], tab.perim)
     assert table["area"].sum() == np.count_nonzero(lat & 0xFFFFFF)
+
+
+@pytest.mark.parametrize("mode", ["phase", "global", "lds"])
+def test_launch_forms_agree_with_oracle(mode, monkeypatch):
+    """The per-phase, persistent-global and persistent-LDS launch forms all
+    reproduce the oracle (2D), and phase/global do in 3D."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_MODE", mode)
+    cases = [(W.plain2d(L=64, ncells=20, seed=21), 2, 4, 40), (W.sorting2d(L=128, grid=9, edge=10, seed=22), 1, 1, 30)]
+    if mode != "lds":
+        cases.append((W.plain3d(L=32, ncells=60, seed=23), 1, 1, 8))
+    for wl, c, u, n in cases:
+        _announce("mode", mode, wl.name, wl.L, c, u)
+        _, lat, table = _run_gpu(wl, c, u, seed=31, nmcs=n, inner=max(1, n // 3))
+        lat_o, tab_o = run_oracle(wl, c, u, 31, (n // max(1, n // 3)) * max(1, n // 3))
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(table, tab_o, wl.d)
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_lds_tile_size_does_not_change_results(tile, monkeypatch):
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_MODE", "lds")
+    monkeypatch.setenv("GPUCPM_TILE", str(tile))
+    wl = W.plain2d(L=128, ncells=40, seed=24)
+    _announce("tile", tile)
+    _, lat, table = _run_gpu(wl, 2, 3, seed=8, nmcs=25)
+    lat_o, tab_o = run_oracle(wl, 2, 3, 8, 25)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 2)
