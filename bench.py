@@ -89,9 +89,13 @@ def main():
               positions_per_thread=2 * c, positions_per_checkerboard=c,
               updates_per_checkerboard_switch=u, updates_per_barrier=1,
               iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    log(f"{a.workload} L={wl.L} d={wl.d} cells={wl.ncells} c={c} u={u} "
+        f"mode={os.environ.get('GPUCPM_MODE', 'default')}: warmup {a.warmup} x {a.mcs_per_step} MCS")
     for _ in range(a.warmup):
         sim.run(**kw)
     sim.synchronize()
+    log(f"timed: {a.steps} x {a.mcs_per_step} MCS")
 
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -99,11 +103,13 @@ def main():
     sim.synchronize()
     wall = time.perf_counter() - t0
 
+    log(f"timed region {wall:.3f} s; kernel timing pass")
     # Kernel durations of the same number of steps, bracketed per launch.
     sim.timing_begin()
     for _ in range(a.steps):
         sim.run(**kw)
     kernel_ms, launches, updates = sim.timing_end()
+    launch = sim.last_launch()
 
     sites = wl.L ** wl.d
     mcs = a.steps * a.mcs_per_step
@@ -135,11 +141,13 @@ def main():
         "mcs_per_s": mcs / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweep_phase", "avg_launch_us": avg_launch_s * 1e6,
-                     "bytes_per_launch": bytes_per_launch, "launches": launches},
+                     "kernel": launch["kernel"], "avg_launch_us": avg_launch_s * 1e6,
+                     "bytes_per_launch": bytes_per_launch, "launches": launches,
+                     "grid_blocks": launch["blocks"], "lds_tile": launch["tile"]},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
+        log(f"CPU oracle sample (~{a.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(wl, c, u, a.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -73,6 +73,22 @@ struct PersistArgs {
     int lntx;               // log2 tiles per row
 };
 
+// One replica of a batched run.
+struct ReplicaDesc {
+    SweepConsts k;
+    uint32_t mcs0;          // this replica's MCS counter at the start
+    int lh;                 // log2 (same-colour sub-blocks per axis)
+    int block0, nblocks;    // its block group in the cooperative grid
+};
+
+// Batched replicas in one cooperative launch (device-resident descriptors).
+struct BatchArgs {
+    const ReplicaDesc* reps;
+    const int* rep_of;      // block -> replica
+    unsigned int* bars;     // 2 words per replica
+    int nmcs;               // MCS per replica in this launch
+};
+
 // Host-side launchers (kernels.hip).
 hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 // Cooperative persistent launches.  lds2d selects the 2D LDS-tiled kernel.
@@ -80,9 +96,15 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
 hipError_t persist_max_blocks(int d, bool lds2d, size_t lds_bytes, int* max_blocks);
 hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, size_t lds_bytes,
                           hipStream_t s);
+hipError_t batch_max_blocks(int d, int* max_blocks);
+hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s);
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,
                         uint32_t* first, int32_t* anchor, int64_t* csum, hipStream_t s);
 hipError_t launch_settle(int64_t* tab, int64_t* del, int ncells, hipStream_t s);
+// pairs[20*20] and mom[20*5] must be zero on entry; ctype[n+1] is scratch.
+hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab, int ncells,
+                        unsigned long long* pairs, unsigned long long* mom, int32_t* ctype,
+                        hipStream_t s);
 
 constexpr int kBlock = 256;  // threads per workgroup of the sweep kernels
 

@@ -61,7 +61,7 @@ constexpr int kMaxThr = 1 << 22;
 // Host mirror of the hot-path fields of the reference Settings
 // (settings.h:4-31); J and stickiness are int16 there.
 struct HostSettings {
-    int16_t J[CPM_NTYPES][CPM_NTYPES];
+    double J[CPM_NTYPES][CPM_NTYPES];  // int16 values (or dyadic reals via cpm_set_adhesion_real)
     int16_t S[CPM_NTYPES][CPM_NTYPES];
     double lam_area[CPM_NTYPES];
     int32_t target_area[CPM_NTYPES];
@@ -90,6 +90,8 @@ struct cpm_engine {
     uint32_t* d_first = nullptr;
     gcpm::DevParams* d_prm = nullptr;
     unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation}
+    unsigned long long* d_stats = nullptr;  // 400 pair counts + 100 moments
+    int32_t* d_ctype = nullptr;             // [n+1] cell types (stats scratch)
     uint32_t* d_thr = nullptr;
     int thr_cap = 0;
     int nthr = 0;
@@ -101,6 +103,7 @@ struct cpm_engine {
     size_t ev_used = 0;
     int64_t t_launches = 0;
     int64_t t_updates = 0;
+    Plan last_plan;  // launch form of the last run
 
     int next_events(hipEvent_t* e0, hipEvent_t* e1) {
         *e0 = *e1 = nullptr;
@@ -129,6 +132,8 @@ struct cpm_engine {
         (void)hipFree(d_first);
         (void)hipFree(d_prm);
         (void)hipFree(d_bar);
+        (void)hipFree(d_stats);
+        (void)hipFree(d_ctype);
         (void)hipFree(d_thr);
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -143,6 +148,10 @@ struct cpm_engine {
             for (int t = 0; t < CPM_NTYPES && ok; t++) {
                 for (double v : {st.lam_area[t] * s, st.lam_perim[t] * s})
                     if (v != std::floor(v) || std::fabs(v) >= 2147483648.0) ok = false;
+                for (int o = 0; o < CPM_NTYPES; o++) {
+                    const double v = st.J[t][o] * s;
+                    if (v != std::floor(v) || std::fabs(v) >= 1073741824.0) ok = false;
+                }
             }
             if (ok) return 1 << k;
         }
@@ -192,13 +201,13 @@ struct cpm_engine {
         const int sc = energy_scale();
         if (sc < 0)
             return fail(CPM_EINVAL,
-                        "lambda_area / lambda_perimeter must be multiples of 2^-8 (dyadic "
-                        "fixed-point energy)");
+                        "lambda_area / lambda_perimeter / adhesion must be multiples of 2^-8 "
+                        "(dyadic fixed-point energy)");
         gcpm::DevParams p{};
         for (int a = 0; a < CPM_NTYPES; a++) {
             for (int b = 0; b < CPM_NTYPES; b++) {
-                p.J[a][b] = (int32_t)st.J[a][b] * sc;
-                p.JS[a][b] = ((int32_t)st.J[a][b] + st.S[a][b]) * sc;
+                p.J[a][b] = (int32_t)(st.J[a][b] * sc);
+                p.JS[a][b] = (int32_t)(st.J[a][b] * sc) + (int32_t)st.S[a][b] * sc;
             }
             p.la[a] = (int32_t)(st.lam_area[a] * sc);
             p.lp[a] = (int32_t)(st.lam_perim[a] * sc);
@@ -229,6 +238,30 @@ struct cpm_engine {
         scale = sc;
         params_dirty = false;
         return CPM_OK;
+    }
+};
+
+// A fixed list of engines run together in one cooperative launch.
+struct cpm_batch {
+    std::vector<cpm_handle> sims;
+    int d = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> ready;       // one per replica stream
+    unsigned int* d_bars = nullptr;      // 2 words per replica
+    gcpm::ReplicaDesc* d_reps = nullptr;
+    int* d_rep_of = nullptr;
+    int rep_of_cap = 0;
+    int last_blocks = 0;
+
+    ~cpm_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipFree(d_bars);
+        (void)hipFree(d_reps);
+        (void)hipFree(d_rep_of);
+        for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -353,6 +386,18 @@ int cpm_set_adhesion(cpm_handle h, int type, int other_type, int adhesion, int s
     return CPM_OK;
 }
 
+int cpm_set_adhesion_real(cpm_handle h, int type, int other_type, double adhesion, int set_both) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (int rc = check_type(other_type, "other_cell_type")) return rc;
+    if (!std::isfinite(adhesion) || std::fabs(adhesion) > 32767.0)
+        return fail(CPM_EINVAL, "adhesion must be finite with |J| <= 32767");
+    h->st.J[type][other_type] = adhesion;
+    if (set_both) h->st.J[other_type][type] = adhesion;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
 int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness) {
     if (int rc = check_handle(h)) return rc;
     if (int rc = check_type(type, "cell_type")) return rc;
@@ -427,7 +472,11 @@ int cpm_push(cpm_handle h) {
     return CPM_OK;
 }
 
-int cpm_run(cpm_handle h, const cpm_run_args* a) {
+namespace {
+
+// Validate run arguments against engine h and fill the sweep constants
+// (cpm.cu:291-405 semantics; see cpm_run).
+int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, int* lh_out) {
     if (int rc = check_handle(h)) return rc;
     if (!a) return fail(CPM_EINVAL, "null run arguments");
     if (!h->pushed) return fail(CPM_ESTATE, "run before push_to_gpu");
@@ -444,10 +493,6 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
     if (a->inner_iterations < 1) return fail(CPM_EINVAL, "inner_iterations must be >= 1");
     if (a->iterations < 0) return fail(CPM_EINVAL, "iterations must be >= 0");
     if (int rc = h->upload_params()) return rc;
-
-    // cpm.cu:405: iterations / inner_iterations launches of inner_iterations MCS.
-    const long chunks = a->iterations / a->inner_iterations;
-    const long total = chunks * a->inner_iterations;
     gcpm::SweepConsts k{};
     k.lat = h->d_lat;
     for (int i = 0; i < 2; i++) {
@@ -466,11 +511,27 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
     k.rounds = (int)(cd / u);  // kernels2d.cu:95
     k.k0 = (uint32_t)h->seed;
     k.k1 = (uint32_t)(h->seed >> 32);
-    const int lh = h->lgL - k.lc - 1;
+    *kout = k;
+    *lh_out = h->lgL - k.lc - 1;
+    return CPM_OK;
+}
+
+}  // namespace
+
+int cpm_run(cpm_handle h, const cpm_run_args* a) {
+    gcpm::SweepConsts k{};
+    int lh = 0;
+    if (int rc = prepare_run(h, a, &k, &lh)) return rc;
+    const int c = a->positions_per_checkerboard;
+    const int u = a->updates_per_checkerboard_switch;
+    // cpm.cu:405: iterations / inner_iterations launches of inner_iterations MCS.
+    const long chunks = a->iterations / a->inner_iterations;
+    const long total = chunks * a->inner_iterations;
     if (total == 0) return CPM_OK;
 
     Plan plan;
     if (int rc = h->plan(c, &plan)) return rc;
+    h->last_plan = plan;
     const int64_t updates_per_mcs = (int64_t)k.rounds * (1 << h->d) * ((int64_t)1 << (lh * h->d)) * u;
 
     if (plan.mode == Plan::kPhase) {
@@ -611,6 +672,14 @@ int cpm_timing_end(cpm_handle h, double* kernel_ms, int64_t* launches, int64_t* 
     return CPM_OK;
 }
 
+int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile) {
+    if (int rc = check_handle(h)) return rc;
+    if (form) *form = (int)h->last_plan.mode;
+    if (blocks) *blocks = h->last_plan.blocks;
+    if (tile) *tile = h->last_plan.ltile ? 1 << h->last_plan.ltile : 0;
+    return CPM_OK;
+}
+
 int cpm_set_seed(cpm_handle h, uint64_t seed) {
     if (int rc = check_handle(h)) return rc;
     h->seed = seed;
@@ -634,6 +703,153 @@ int cpm_get_mcs(cpm_handle h, uint64_t* mcs) {
     if (int rc = check_handle(h)) return rc;
     if (!mcs) return fail(CPM_EINVAL, "null output");
     *mcs = h->mcs;
+    return CPM_OK;
+}
+
+int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
+    if (!out) return fail(CPM_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!sims || n < 1) return fail(CPM_EINVAL, "a batch needs at least one engine");
+    for (int i = 0; i < n; i++) {
+        if (!sims[i]) return fail(CPM_EINVAL, "null engine in batch");
+        if (sims[i]->d != sims[0]->d) return fail(CPM_EINVAL, "all engines of a batch must have the same dimensionality");
+        for (int j = 0; j < i; j++)
+            if (sims[j] == sims[i]) return fail(CPM_EINVAL, "an engine appears twice in the batch");
+    }
+    cpm_batch* b = new (std::nothrow) cpm_batch();
+    if (!b) return fail(CPM_EHIP, "host allocation failed");
+    b->sims.assign(sims, sims + n);
+    b->d = sims[0]->d;
+    auto bail = [&](hipError_t err, const char* what) {
+        int rc = hip_fail(err, what);
+        delete b;
+        return rc;
+    };
+    hipError_t err;
+    if ((err = hipStreamCreate(&b->stream)) != hipSuccess) return bail(err, "hipStreamCreate");
+    if ((err = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess) return bail(err, "hipEventCreate");
+    b->ready.resize(n, nullptr);
+    for (int i = 0; i < n; i++)
+        if ((err = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming)) != hipSuccess)
+            return bail(err, "hipEventCreate");
+    if ((err = hipMalloc(&b->d_bars, 2 * sizeof(unsigned int) * n)) != hipSuccess) return bail(err, "hipMalloc(barriers)");
+    if ((err = hipMemset(b->d_bars, 0, 2 * sizeof(unsigned int) * n)) != hipSuccess) return bail(err, "hipMemset(barriers)");
+    if ((err = hipMalloc(&b->d_reps, sizeof(gcpm::ReplicaDesc) * n)) != hipSuccess) return bail(err, "hipMalloc(replicas)");
+    *out = b;
+    return CPM_OK;
+}
+
+int cpm_batch_destroy(cpm_batch_handle b) {
+    delete b;
+    return CPM_OK;
+}
+
+int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    const int n = (int)b->sims.size();
+    std::vector<gcpm::ReplicaDesc> reps(n);
+    std::vector<long> need(n);
+    for (int i = 0; i < n; i++) {
+        if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
+        const long nsub = 1L << (reps[i].lh * b->d);
+        need[i] = std::max(1L, (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock);
+    }
+    const long chunks = a->iterations / a->inner_iterations;
+    if (chunks == 0) return CPM_OK;
+
+    // Block groups: each replica gets blocks in proportion to its work, at
+    // least one, within the co-resident limit of the cooperative grid.
+    int maxb = 0;
+    HIP_TRY(gcpm::batch_max_blocks(b->d, &maxb));
+    if (maxb < n) return fail(CPM_EINVAL, "more replicas than co-resident blocks");
+    long want = 0;
+    for (long v : need) want += v;
+    std::vector<int> nb(n);
+    int total = 0;
+    for (int i = 0; i < n; i++) {
+        long v = need[i];
+        if (want > maxb) v = std::max(1L, (long)((double)need[i] * (maxb - n) / (double)want) + 1);
+        nb[i] = (int)v;
+        total += nb[i];
+    }
+    while (total > maxb) {  // trim rounding excess from the largest groups
+        int j = (int)(std::max_element(nb.begin(), nb.end()) - nb.begin());
+        nb[j]--;
+        total--;
+    }
+    std::vector<int> rep_of(total);
+    for (int i = 0, b0 = 0; i < n; i++) {
+        reps[i].block0 = b0;
+        reps[i].nblocks = nb[i];
+        for (int j = 0; j < nb[i]; j++) rep_of[b0 + j] = i;
+        b0 += nb[i];
+    }
+    if (total > b->rep_of_cap) {
+        (void)hipFree(b->d_rep_of);
+        b->d_rep_of = nullptr;
+        HIP_TRY(hipMalloc(&b->d_rep_of, sizeof(int) * total));
+        b->rep_of_cap = total;
+    }
+    b->last_blocks = total;
+    // The batch stream waits for each replica's queued work.
+    for (int i = 0; i < n; i++) {
+        HIP_TRY(hipEventRecord(b->ready[i], b->sims[i]->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->ready[i], 0));
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int) * total, hipMemcpyHostToDevice, b->stream));
+    gcpm::BatchArgs ba{};
+    ba.reps = b->d_reps;
+    ba.rep_of = b->d_rep_of;
+    ba.bars = b->d_bars;
+    ba.nmcs = a->inner_iterations;
+    for (long ch = 0; ch < chunks; ch++) {
+        for (int i = 0; i < n; i++)
+            reps[i].mcs0 = (uint32_t)(b->sims[i]->mcs + (uint64_t)ch * a->inner_iterations);
+        // Wait for the copy (and so the previous chunk) before reps is
+        // rewritten for the next chunk.
+        HIP_TRY(hipMemcpyAsync(b->d_reps, reps.data(), sizeof(gcpm::ReplicaDesc) * n, hipMemcpyHostToDevice,
+                               b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        HIP_TRY(gcpm::launch_batch(b->d, ba, total, b->stream));
+    }
+    // Each replica's later work waits for the batch.
+    HIP_TRY(hipEventRecord(b->done, b->stream));
+    for (int i = 0; i < n; i++) {
+        HIP_TRY(hipStreamWaitEvent(b->sims[i]->stream, b->done, 0));
+        b->sims[i]->mcs += (uint64_t)(chunks * a->inner_iterations);
+    }
+    return CPM_OK;
+}
+
+int cpm_batch_sync(cpm_batch_handle b) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return CPM_OK;
+}
+
+int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    if (replicas) *replicas = (int)b->sims.size();
+    if (blocks) *blocks = b->last_blocks;
+    return CPM_OK;
+}
+
+int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments) {
+    if (int rc = check_handle(h)) return rc;
+    if (!h->pushed) return fail(CPM_ESTATE, "energy statistics exist only after push_to_gpu");
+    if (!pairs || !moments) return fail(CPM_EINVAL, "null output");
+    const int np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5;
+    if (!h->d_stats) HIP_TRY(hipMalloc(&h->d_stats, sizeof(unsigned long long) * (np + nm)));
+    if (!h->d_ctype) HIP_TRY(hipMalloc(&h->d_ctype, sizeof(int32_t) * ((size_t)h->n + 1)));
+    HIP_TRY(hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * (np + nm), h->stream));
+    HIP_TRY(gcpm::launch_stats(h->d, h->d_lat, h->lgL, h->d_tab[0], h->n, h->d_stats, h->d_stats + np,
+                               h->d_ctype, h->stream));
+    std::vector<unsigned long long> host(np + nm);
+    HIP_TRY(hipMemcpyAsync(host.data(), h->d_stats, sizeof(unsigned long long) * (np + nm),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < np; i++) pairs[i] = (int64_t)host[i];
+    for (int i = 0; i < nm; i++) moments[i] = (int64_t)host[np + i];
     return CPM_OK;
 }
 

@@ -9,6 +9,8 @@
 // double-buffered cell tables.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "cpm_device.h"
 
 namespace gcpm {
@@ -103,7 +105,9 @@ struct LdsLat2 {
 // neighbour), with the deterministic RNG, snapshot tables and integer
 // acceptance described in SweepConsts.
 template <int D, class Lat>
-__device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k, int bx,
+__device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k,
+                                                   const DevParams* __restrict__ P,
+                                                   const uint32_t* __restrict__ thr, int bx,
                                                    int by, int bz, uint32_t mcs, uint32_t phase,
                                                    const int64_t* __restrict__ R, int64_t* W,
                                                    int64_t* Dc) {
@@ -116,7 +120,6 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
     const int Lm = (1 << k.lgL) - 1;
     const uint32_t sb = (uint32_t)bx | ((uint32_t)by << lsb) |
                         (D == 3 ? ((uint32_t)bz << (2 * lsb)) : 0u);
-    const DevParams* __restrict__ P = k.prm;
 
     for (int i = 0; i < k.u; i++) {
         uint32_t w[4];
@@ -133,6 +136,11 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         if (vs == vt) continue;  // kernels2d.cu:276
 
         const uint32_t ts = vs >> 24, tt = vt >> 24;
+        const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
+        // Issue the snapshot-table loads first: their latency overlaps the
+        // neighbourhood scan.
+        const int64_t row_s = cs ? R[cs] : 0;
+        const int64_t row_t = ct ? R[ct] : 0;
         int32_t adh = 0;
         int as = 0, at = 0;
 #pragma unroll
@@ -146,22 +154,19 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
             if (vn != vt) adh -= P->JS[tt][tn]; else at++;
         }
         int64_t dh = adh;
-        const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
         if (cs) {
-            const int64_t row = R[cs];
-            const int64_t A = (int64_t)(uint32_t)row, Pm = row >> 32;
+            const int64_t A = (int64_t)(uint32_t)row_s, Pm = row_s >> 32;
             const int64_t dP = N - 2 * as;
             dh += (int64_t)P->la[ts] * (1 + 2 * A - 2 * (int64_t)P->a0[ts]);
             dh += (int64_t)P->lp[ts] * dP * (dP + 2 * (Pm - P->p0[ts]));
         }
         if (ct) {
-            const int64_t row = R[ct];
-            const int64_t A = (int64_t)(uint32_t)row, Pm = row >> 32;
+            const int64_t A = (int64_t)(uint32_t)row_t, Pm = row_t >> 32;
             const int64_t dP = 2 * at - N;
             dh += (int64_t)P->la[tt] * (1 - 2 * A + 2 * (int64_t)P->a0[tt]);
             dh += (int64_t)P->lp[tt] * dP * (dP + 2 * (Pm - P->p0[tt]));
         }
-        const bool accept = dh <= 0 || (dh < k.nthr && w[2] < k.thr[dh]);
+        const bool accept = dh <= 0 || (dh < k.nthr && w[2] < thr[dh]);
         if (!accept) continue;
 
         lat.put(t, vs);  // kernels2d.cu:548
@@ -188,6 +193,24 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
             }
         }
     }
+}
+
+// Parameters and (when short) the acceptance table staged in LDS.
+constexpr int kThrLds = 1024;
+struct LdsParams {
+    DevParams p;
+    uint32_t thr[kThrLds];
+};
+
+__device__ __forceinline__ const uint32_t* stage_params(LdsParams& sp, const SweepConsts& k) {
+    const int* src = reinterpret_cast<const int*>(k.prm);
+    int* dst = reinterpret_cast<int*>(&sp.p);
+    for (int i = threadIdx.x; i < (int)(sizeof(DevParams) / 4); i += blockDim.x) dst[i] = src[i];
+    const bool fits = k.nthr <= kThrLds;
+    if (fits)
+        for (int i = threadIdx.x; i < k.nthr; i += blockDim.x) sp.thr[i] = k.thr[i];
+    __syncthreads();
+    return fits ? sp.thr : k.thr;
 }
 
 // Fold the previous phase's deltas into W (W is never read in a phase).
@@ -217,6 +240,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
     const SweepConsts& k = a.k;
+    __shared__ LdsParams sp;
+    const uint32_t* thr = stage_params(sp, k);
     int64_t* W = k.tab[(a.g + 1) & 1];
     fold_deltas(W, k.del[(a.g + 1) & 1], k.ncells, gtid, gsz);
     if (gtid >= a.nsub) return;
@@ -225,22 +250,25 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
     const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
     const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
     const GlobalLat<D> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-    sub_block_attempts<D>(lat, k, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1], W, k.del[a.g & 1]);
+    sub_block_attempts<D>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1], W,
+                          k.del[a.g & 1]);
 }
 
-// Grid-wide barrier of a cooperative launch (all blocks co-resident).
+// Barrier over nblocks co-resident blocks of a cooperative launch (the
+// whole grid, or the blocks of one replica in a batch).
 // bar[0] counts arrivals, bar[1] is the generation; both persist in device
 // memory between launches (bar[0] returns to 0 at every release).  Every
 // thread fences at agent scope before arriving and after leaving, so plain
 // global stores of one phase are visible to plain loads of the next.
-__device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& gen) {
+__device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& gen,
+                                             unsigned int nblocks) {
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned int g = gen;
         const unsigned int prev =
             __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == gridDim.x - 1) {
+        if (prev == nblocks - 1) {
             __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&bar[1], g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -264,24 +292,27 @@ __device__ __forceinline__ void settle_in_kernel(const SweepConsts& k, int g, in
     }
 }
 
-// Whole run, lattice in global memory: grid-stride over the same-colour
-// sub-blocks of each phase, grid barrier between phases.
+// Whole run, lattice in global memory: the nblocks blocks of a group
+// (lb = this block's index in it) stride over the same-colour sub-blocks of
+// each phase and meet a group barrier between phases.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_persist_global(PersistArgs a) {
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gsz = gridDim.x * blockDim.x;
-    const SweepConsts& k = a.k;
-    unsigned int gen = __hip_atomic_load(&a.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_t mcs0, int nmcs,
+                                                   int lh, unsigned int* bar, int lb, int nblocks,
+                                                   LdsParams& sp) {
+    const int gtid = lb * blockDim.x + threadIdx.x;
+    const int gsz = nblocks * blockDim.x;
+    unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t* thr = stage_params(sp, k);
     const GlobalLat<D> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-    const int lh = a.lh, hm = (1 << lh) - 1;
+    const int hm = (1 << lh) - 1;
     const int nsub = 1 << (D * lh);
     constexpr int NCOL = 1 << D;
     int g = 0;
-    for (int m = 0; m < a.nmcs; m++) {
-        const uint32_t mcs = a.mcs0 + (uint32_t)m;
+    for (int m = 0; m < nmcs; m++) {
+        const uint32_t mcs = mcs0 + (uint32_t)m;
         for (int r = 0; r < k.rounds; r++) {
             for (int col = 0; col < NCOL; col++, g++) {
-                if (g) grid_barrier(a.bar, gen);
+                if (g) grid_barrier(bar, gen, nblocks);
                 int cx, cy, cz;
                 colour_of<D>(col, cx, cy, cz);
                 int64_t* W = k.tab[(g + 1) & 1];
@@ -293,15 +324,33 @@ __global__ __launch_bounds__(kBlock) void k_persist_global(PersistArgs a) {
                     const int bx = 2 * (q & hm) + cx;
                     const int by = 2 * ((q >> lh) & hm) + cy;
                     const int bz = D == 3 ? 2 * (q >> (2 * lh)) + cz : 0;
-                    sub_block_attempts<D>(lat, k, bx, by, bz, mcs, phase, R, W, Dc);
+                    sub_block_attempts<D>(lat, k, &sp.p, thr, bx, by, bz, mcs, phase, R, W, Dc);
                 }
             }
         }
     }
     if (g) {
-        grid_barrier(a.bar, gen);
+        grid_barrier(bar, gen, nblocks);
         settle_in_kernel(k, g, gtid, gsz);
     }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_persist_global(PersistArgs a) {
+    __shared__ LdsParams sp;
+    persist_global_run<D>(a.k, a.mcs0, a.nmcs, a.lh, a.bar, blockIdx.x, gridDim.x, sp);
+}
+
+// Batched replicas in one cooperative launch: block b works for replica
+// rep_of[b]; the blocks of one replica form a group with its own barrier,
+// so replicas of different sizes advance independently.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_global(BatchArgs a) {
+    __shared__ LdsParams sp;
+    const int r = a.rep_of[blockIdx.x];
+    const ReplicaDesc& rd = a.reps[r];
+    persist_global_run<D>(rd.k, rd.mcs0, a.nmcs, rd.lh, a.bars + 2 * r, blockIdx.x - rd.block0,
+                          rd.nblocks, sp);
 }
 
 // Whole 2D run with each block's tile resident in LDS.  Block b owns the
@@ -320,6 +369,8 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
     const int y0 = (int)(blockIdx.x >> a.lntx) << a.lty;
     const int Lm = (1 << k.lgL) - 1;
     const uint32_t* __restrict__ gl = k.lat;
+    __shared__ LdsParams sp;
+    const uint32_t* thr = stage_params(sp, k);
     // tile + halo
     for (int i = threadIdx.x; i < S * (TY + 2); i += blockDim.x) {
         const int ly = i / S, lx = i - ly * S;
@@ -336,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
         for (int r = 0; r < k.rounds; r++) {
             for (int col = 0; col < 4; col++, g++) {
                 if (g) {
-                    grid_barrier(a.bar, gen);
+                    grid_barrier(a.bar, gen, gridDim.x);
                     for (int i = threadIdx.x; i < ring; i += blockDim.x) {
                         int lx, ly;
                         if (i < S) { lx = i; ly = 0; }
@@ -358,13 +409,13 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
                 for (int q = threadIdx.x; q < nq; q += blockDim.x) {
                     const int bx = (x0 >> k.lc) + 2 * (q & ((1 << lq) - 1)) + cx;
                     const int by = (y0 >> k.lc) + 2 * (q >> lq) + cy;
-                    sub_block_attempts<2>(lat, k, bx, by, 0, mcs, phase, R, W, Dc);
+                    sub_block_attempts<2>(lat, k, &sp.p, thr, bx, by, 0, mcs, phase, R, W, Dc);
                 }
             }
         }
     }
     if (g) {
-        grid_barrier(a.bar, gen);
+        grid_barrier(a.bar, gen, gridDim.x);
         settle_in_kernel(k, g, gtid, gsz);
     }
 }
@@ -427,6 +478,82 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
     del[c] = 0;
 }
 
+// Sufficient statistics of H(sigma; theta) (the backward pass of dH/dtheta).
+// Pair counts: every unordered unlike-cell neighbour pair is seen once from
+// the site whose offset to the other is lexicographically positive
+// ((dz, dy, dx) > 0: 4 offsets in 2D, 13 in 3D), counted by the type pair
+// (min, max) in an LDS histogram and added to pairs[20*20] once per block.
+// Each site also records its cell's type.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_stats_pairs(const uint32_t* __restrict__ lat, int lgL,
+                                                        unsigned long long* pairs, int32_t* ctype) {
+    __shared__ unsigned int hist[kNTypes * kNTypes];
+    for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const uint32_t nsites = 1u << (D * lgL);
+    const int Lm = (1 << lgL) - 1;
+    constexpr int NB = D == 3 ? 27 : 9;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nsites; i += gridDim.x * blockDim.x) {
+        const uint32_t v = lat[i];
+        const uint32_t t = v >> 24;
+        if (v & kSpinMask) ctype[v & kSpinMask] = (int32_t)t;
+        const int x = (int)(i & Lm), y = (int)((i >> lgL) & Lm), z = D == 3 ? (int)(i >> (2 * lgL)) : 0;
+#pragma unroll
+        for (int n = NB / 2 + 1; n < NB; n++) {  // offsets after the centre: the positive half
+            int ox, oy, oz;
+            block_offset<D>(n, ox, oy, oz);
+            const uint32_t vn = lat[site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL)];
+            if (vn != v) {
+                const uint32_t tn = vn >> 24;
+                const uint32_t a = t < tn ? t : tn, b = t < tn ? tn : t;
+                atomicAdd(&hist[a * kNTypes + b], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x)
+        if (hist[i]) atomicAdd(&pairs[i], (unsigned long long)hist[i]);
+}
+
+// Per-type moments of the cell tables: n, sum A, sum A^2, sum P, sum P^2
+// over live cells (area > 0), mom[5 * type + k].
+__global__ __launch_bounds__(kBlock) void k_stats_cells(const int64_t* __restrict__ tab,
+                                                        const int32_t* __restrict__ ctype,
+                                                        int ncells, unsigned long long* mom) {
+    __shared__ unsigned long long acc[kNTypes * 5];
+    for (int i = threadIdx.x; i < kNTypes * 5; i += blockDim.x) acc[i] = 0;
+    __syncthreads();
+    for (int c = 1 + blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += gridDim.x * blockDim.x) {
+        const int64_t row = tab[c];
+        const uint64_t A = (uint32_t)row;
+        if (!A) continue;
+        const uint64_t P = (uint64_t)(row >> 32);
+        unsigned long long* m = acc + 5 * ctype[c];
+        atomicAdd(&m[0], 1ull);
+        atomicAdd(&m[1], (unsigned long long)A);
+        atomicAdd(&m[2], (unsigned long long)(A * A));
+        atomicAdd(&m[3], (unsigned long long)P);
+        atomicAdd(&m[4], (unsigned long long)(P * P));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNTypes * 5; i += blockDim.x)
+        if (acc[i]) atomicAdd(&mom[i], acc[i]);
+}
+
+hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab, int ncells,
+                        unsigned long long* pairs, unsigned long long* mom, int32_t* ctype,
+                        hipStream_t s) {
+    const uint32_t nsites = 1u << (d * lgL);
+    const int blocks = (int)std::min<uint32_t>((nsites + kBlock - 1) / kBlock, 4096u);
+    if (d == 2)
+        hipLaunchKernelGGL(k_stats_pairs<2>, dim3(blocks), dim3(kBlock), 0, s, lat, lgL, pairs, ctype);
+    else
+        hipLaunchKernelGGL(k_stats_pairs<3>, dim3(blocks), dim3(kBlock), 0, s, lat, lgL, pairs, ctype);
+    const int cb = std::max(1, std::min(1024, (ncells + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_stats_cells, dim3(cb), dim3(kBlock), 0, s, tab, ctype, ncells, mom);
+    return hipGetLastError();
+}
+
 hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const int work = a.nsub > a.k.ncells + 1 ? a.nsub : a.k.ncells + 1;
     const int blocks = (work + kBlock - 1) / kBlock;
@@ -437,6 +564,28 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
     else
         hipExtLaunchKernelGGL(k_sweep_phase<3>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     return hipGetLastError();
+}
+
+hipError_t batch_max_blocks(int d, int* max_blocks) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+        return e;
+    const void* f = d == 2 ? reinterpret_cast<const void*>(&k_batch_global<2>)
+                           : reinterpret_cast<const void*>(&k_batch_global<3>);
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, 0)) != hipSuccess)
+        return e;
+    *max_blocks = per_cu * cus;
+    return hipSuccess;
+}
+
+hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s) {
+    BatchArgs copy = a;
+    void* args[] = {&copy};
+    const void* f = d == 2 ? reinterpret_cast<const void*>(&k_batch_global<2>)
+                           : reinterpret_cast<const void*>(&k_batch_global<3>);
+    return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(kBlock), args, 0, s);
 }
 
 static const void* persist_kernel(int d, bool lds2d) {

@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib
 from ._lib import RunArgs, check, lib
 
-__all__ = ["Cpm", "RunArgs"]
+__all__ = ["Cpm", "Batch", "RunArgs"]
 
 _UNSET = -1
 
@@ -39,6 +39,22 @@ def _float(v, name):
         return float(v)
     except (TypeError, ValueError):
         raise TypeError(f"{name}: must be real number, not {type(v).__name__}") from None
+
+
+def _run_args(cell_sync, block_sync, global_sync, threads_per_block, positions_per_thread,
+              positions_per_checkerboard, updates_per_checkerboard_switch, updates_per_barrier,
+              iterations, inner_iterations, shared=0, partial_dispatch=0, centroid_tracking=False):
+    a = RunArgs()
+    vals = dict(cell_sync=cell_sync, block_sync=block_sync, global_sync=global_sync,
+                threads_per_block=threads_per_block, positions_per_thread=positions_per_thread,
+                positions_per_checkerboard=positions_per_checkerboard,
+                updates_per_checkerboard_switch=updates_per_checkerboard_switch,
+                updates_per_barrier=updates_per_barrier, iterations=iterations,
+                inner_iterations=inner_iterations, shared=shared, partial_dispatch=partial_dispatch)
+    for k, v in vals.items():
+        setattr(a, k, _int(v, k))
+    a.centroid_tracking = int(bool(centroid_tracking))
+    return a
 
 
 class Cpm:
@@ -165,17 +181,10 @@ class Cpm:
         in the reference and do not change results here.  ``seed`` (an
         extension) sets the RNG key before running.
         """
-        a = RunArgs()
-        vals = dict(cell_sync=cell_sync, block_sync=block_sync, global_sync=global_sync,
-                    threads_per_block=threads_per_block, positions_per_thread=positions_per_thread,
-                    positions_per_checkerboard=positions_per_checkerboard,
-                    updates_per_checkerboard_switch=updates_per_checkerboard_switch,
-                    updates_per_barrier=updates_per_barrier, iterations=iterations,
-                    inner_iterations=inner_iterations, shared=shared,
-                    partial_dispatch=partial_dispatch)
-        for k, v in vals.items():
-            setattr(a, k, _int(v, k))
-        a.centroid_tracking = int(bool(centroid_tracking))
+        a = _run_args(cell_sync, block_sync, global_sync, threads_per_block, positions_per_thread,
+                      positions_per_checkerboard, updates_per_checkerboard_switch,
+                      updates_per_barrier, iterations, inner_iterations, shared, partial_dispatch,
+                      centroid_tracking)
         if seed is not None:
             self.set_seed(seed)
         check(lib.cpm_run(self._h, ctypes.byref(a)))
@@ -242,6 +251,23 @@ class Cpm:
         return {"area": area, "perimeter": perim, "anchor": anchor.reshape(rows, 3),
                 "csum": csum.reshape(rows, 3), "centroid": cen}
 
+    def energy_stats(self):
+        """Sufficient statistics of H on the device lattice (exact integers):
+        pairs (20, 20) int64, upper triangle: unlike-cell neighbour pairs by
+        type pair; moments (20, 5) int64: per type count, sum A, sum A^2,
+        sum P, sum P^2 over live cells."""
+        pairs = np.zeros((20, 20), np.int64)
+        mom = np.zeros((20, 5), np.int64)
+        P = ctypes.POINTER(ctypes.c_int64)
+        check(lib.cpm_energy_stats(self._h, pairs.ctypes.data_as(P), mom.ctypes.data_as(P)))
+        return pairs, mom
+
+    def set_adhesion_real(self, cell_type, other_cell_type, adhesion, symmetric=True):
+        """Adhesion as a multiple of 2^-8 (extension used by parameter fitting)."""
+        check(lib.cpm_set_adhesion_real(self._h, _int(cell_type, "cell_type"),
+                                        _int(other_cell_type, "other_cell_type"),
+                                        float(adhesion), int(bool(symmetric))))
+
     def timing_begin(self):
         """Start bracketing every sweep launch with HIP events."""
         check(lib.cpm_timing_begin(self._h))
@@ -251,6 +277,13 @@ class Cpm:
         ms, n, upd = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
         check(lib.cpm_timing_end(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(upd)))
         return ms.value, n.value, upd.value
+
+    def last_launch(self):
+        """Launch form of the last run: dict(form, kernel, blocks, tile)."""
+        f, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib.cpm_last_launch(self._h, ctypes.byref(f), ctypes.byref(b), ctypes.byref(t)))
+        kernel = ["k_sweep_phase", "k_persist_global", "k_persist_lds2d"][f.value]
+        return {"form": f.value, "kernel": kernel, "blocks": b.value, "tile": t.value}
 
     def set_seed(self, seed):
         check(lib.cpm_set_seed(self._h, _int(seed, "seed") & (2**64 - 1)))
@@ -270,3 +303,41 @@ class Cpm:
     @mcs.setter
     def mcs(self, value):
         check(lib.cpm_set_mcs(self._h, _int(value, "mcs")))
+
+
+class Batch:
+    """Batched replicas: several pushed Cpm engines (same dimensionality; any
+    sizes, parameters, temperatures and seeds) advanced together in one
+    cooperative launch per inner_iterations chunk.  Each replica ends
+    bit-identical to calling its own ``run`` with the same arguments."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (ctypes.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = ctypes.c_void_p()
+        check(lib.cpm_batch_create(arr, len(self.sims), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h is not None and h.value and lib is not None:
+            lib.cpm_batch_destroy(h)
+
+    def run(self, cell_sync, block_sync, global_sync, threads_per_block, positions_per_thread,
+            positions_per_checkerboard, updates_per_checkerboard_switch, updates_per_barrier,
+            iterations, inner_iterations, shared=0, partial_dispatch=0, centroid_tracking=False):
+        """Same arguments as Cpm.run, applied to every replica."""
+        a = _run_args(cell_sync, block_sync, global_sync, threads_per_block, positions_per_thread,
+                      positions_per_checkerboard, updates_per_checkerboard_switch,
+                      updates_per_barrier, iterations, inner_iterations, shared, partial_dispatch,
+                      centroid_tracking)
+        check(lib.cpm_batch_run(self._h, ctypes.byref(a)))
+
+    def synchronize(self):
+        check(lib.cpm_batch_sync(self._h))
+
+    @property
+    def blocks(self):
+        n, b = ctypes.c_int(), ctypes.c_int()
+        check(lib.cpm_batch_info(self._h, ctypes.byref(n), ctypes.byref(b)))
+        return b.value

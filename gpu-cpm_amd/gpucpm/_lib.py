@@ -60,6 +60,14 @@ SIGNATURES = [
     ("cpm_get_mcs", _I, [_H, _P(ctypes.c_uint64)]),
     ("cpm_timing_begin", _I, [_H]),
     ("cpm_timing_end", _I, [_H, _P(ctypes.c_double), _P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    ("cpm_last_launch", _I, [_H, _P(_I), _P(_I), _P(_I)]),
+    ("cpm_batch_create", _I, [_P(_H), _I, _P(_H)]),
+    ("cpm_batch_run", _I, [_H, _P(RunArgs)]),
+    ("cpm_batch_sync", _I, [_H]),
+    ("cpm_batch_info", _I, [_H, _P(_I), _P(_I)]),
+    ("cpm_batch_destroy", _I, [_H]),
+    ("cpm_set_adhesion_real", _I, [_H, _I, _I, ctypes.c_double, _I]),
+    ("cpm_energy_stats", _I, [_H, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
 ]
 
 if not os.path.exists(LIBPATH):

@@ -10,6 +10,7 @@ Each builder returns a ``Workload``: the initial lattice (uint32, ids
                  area + perimeter constraints, differential adhesion.
 - ``plain3d``    256^3, 20000 cubes of 5^3 at random non-overlapping positions,
                  volume + surface constraints, 26-neighbourhood.
+- ``batched_sorting``  64 sorting replicas of mixed sizes and parameters.
 The shapes may be scaled down (``L`` / ``ncells`` arguments) for tests.
 """
 from __future__ import annotations
@@ -92,20 +93,20 @@ def plain2d(L=256, ncells=100, seed=1):
     return Workload("plain2d", L, 2, ncells, 20, lat, cons)
 
 
-def sorting2d(L=1024, grid=63, edge=10, seed=2, J_AB=11, temperature=10):
+def sorting2d(L=1024, grid=63, edge=10, seed=2, J_AB=11, temperature=10, ncells=None):
     """Two-type cell sorting: grid x grid squares of edge x edge tiling the
-    centre; types 1/2 Bernoulli(1/2); A0=100, lam_A=25, P0=40, lam_P=2;
-    J_m1=J_m2=16, J_11=2, J_22=14, J_12=J_AB, T=10."""
+    centre (the first ``ncells`` slots in row-major order when given); types
+    1/2 Bernoulli(1/2); A0=100, lam_A=25, P0=40, lam_P=2; J_m1=J_m2=16,
+    J_11=2, J_22=14, J_12=J_AB, T=10."""
     rng = np.random.default_rng(seed)
-    n = grid * grid
+    n = grid * grid if ncells is None else int(ncells)
     types = rng.integers(1, 3, size=n)
     lat = np.zeros((L, L), np.uint32)
     off = (L - grid * edge) // 2
-    for gy in range(grid):
-        for gx in range(grid):
-            k = gy * grid + gx
-            y0, x0 = off + gy * edge, off + gx * edge
-            lat[y0:y0 + edge, x0:x0 + edge] = (np.uint32(types[k]) << np.uint32(24)) | np.uint32(k + 1)
+    for k in range(n):
+        gy, gx = divmod(k, grid)
+        y0, x0 = off + gy * edge, off + gx * edge
+        lat[y0:y0 + edge, x0:x0 + edge] = (np.uint32(types[k]) << np.uint32(24)) | np.uint32(k + 1)
     cons = []
     for t in (1, 2):
         cons.append(dict(cell_type=t, lambda_area=25, target_area=100))
@@ -115,6 +116,25 @@ def sorting2d(L=1024, grid=63, edge=10, seed=2, J_AB=11, temperature=10):
     cons.append(dict(cell_type=2, other_cell_type=2, adhesion=14))
     cons.append(dict(cell_type=1, other_cell_type=2, adhesion=int(J_AB)))
     return Workload("sorting2d", L, 2, n, int(temperature), lat, cons)
+
+
+def batched_sorting(nrep=64, seed=4, sizes=(128, 256, 512, 1024)):
+    """Batched replicas: nrep sorting lattices with L drawn uniformly from
+    ``sizes``, L^2/250 cells (10x10 squares tiling the centre), integer
+    J_AB uniform in [6, 14] and integer T uniform in [5, 20]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nrep):
+        L = int(rng.choice(sizes))
+        n = L * L // 250
+        grid = int(np.ceil(np.sqrt(n)))
+        J_AB = int(rng.integers(6, 15))
+        T = int(rng.integers(5, 21))
+        wl = sorting2d(L=L, grid=grid, edge=10, seed=int(rng.integers(0, 2**31)), J_AB=J_AB,
+                       temperature=T, ncells=n)
+        wl.name = f"replica{i}"
+        out.append(wl)
+    return out
 
 
 def plain3d(L=256, ncells=20000, seed=3):

@@ -27,6 +27,7 @@ extern "C" {
 #define CPM_NTYPES 20   /* settings.h:4 numTypes */
 
 typedef struct cpm_engine* cpm_handle;
+typedef struct cpm_batch* cpm_batch_handle;
 
 /* Keyword arguments of Cpm.run (python_wrapper.cpp:186-216, cpm.cu:291-296).
  * Only positions_per_checkerboard, updates_per_checkerboard_switch,
@@ -76,6 +77,10 @@ int cpm_set_perimeter(cpm_handle h, int type, int target, double lambda);
 
 /* Cpm::setAdhesion (cpm.cu:137-143); J is stored as int16 (settings.h:20). */
 int cpm_set_adhesion(cpm_handle h, int type, int other_type, int adhesion, int set_both);
+
+/* Extension: adhesion as a real number; it must be a multiple of 2^-8 (the
+ * energy is kept in dyadic fixed point).  Used by parameter fitting. */
+int cpm_set_adhesion_real(cpm_handle h, int type, int other_type, double adhesion, int set_both);
 
 /* Cpm::setStickiness (cpm.cu:145-148); int16 (settings.h:22). */
 int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness);
@@ -136,6 +141,39 @@ int cpm_get_mcs(cpm_handle h, uint64_t* mcs);
  * attempts (site-updates) those launches made. */
 int cpm_timing_begin(cpm_handle h);
 int cpm_timing_end(cpm_handle h, double* kernel_ms, int64_t* launches, int64_t* site_updates);
+
+/* Extension (no counterpart in the reference): launch form of the last run.
+ * form: 0 = one launch per colour phase, 1 = one cooperative launch per
+ * inner_iterations chunk with the lattice in global memory, 2 = the same
+ * with 2D tiles resident in LDS; blocks = cooperative grid size; tile = LDS
+ * tile edge (0 if none).  Results never depend on the form. */
+int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile);
+
+/* Extension (no counterpart in the reference, whose parameters are one
+ * global __constant__ block, kernels.cu:48-53): batched replicas.  A batch
+ * is a fixed list of pushed engines of one dimensionality, each with its own
+ * lattice size, parameters, temperature, seed and MCS counter.
+ * cpm_batch_run applies one set of run arguments to all of them in one
+ * cooperative launch per inner_iterations chunk: each replica's blocks form
+ * a group with its own barrier.  Every replica ends bit-identical to running
+ * cpm_run on it alone.  The engines must outlive the batch; their streams
+ * are ordered after the batch launch. */
+int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out);
+int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* args);
+int cpm_batch_sync(cpm_batch_handle b);
+int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks);
+int cpm_batch_destroy(cpm_batch_handle b);
+
+/* Extension (no counterpart in the reference): sufficient statistics of
+ *   H(sigma; theta) = sum_{unordered neighbour pairs, s_i != s_j} J[t_i][t_j]
+ *                   + sum_{cells c>0} lam_A[t_c] (A_c - A0[t_c])^2
+ *                   + sum_{cells c>0} lam_P[t_c] (P_c - P0[t_c])^2
+ * on the current device lattice, as exact integers that do not depend on
+ * theta: pairs[a*20 + b] (a <= b) counts unlike-cell neighbour pairs of
+ * types {a, b}; moments[5*t + k] holds, over live cells of type t,
+ * k = 0: count, 1: sum A, 2: sum A^2, 3: sum P, 4: sum P^2.  H and dH/dtheta
+ * for every theta follow in closed form (the backward pass of the energy). */
+int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments);
 
 #ifdef __cplusplus
 }

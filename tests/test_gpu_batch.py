@@ -1,0 +1,66 @@
+"""Batched replicas on the GPU: one cooperative launch for many engines of
+different sizes and parameters; each replica must end bit-identical to its
+own single run and to the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, run_kwargs, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch():
+    from gpucpm import workloads as W
+    return W.batched_sorting(nrep=6, seed=40, sizes=(32, 64, 128))
+
+
+def test_batch_equals_single_runs_and_oracle():
+    import gpucpm
+    wls = _small_batch()
+    print("CASE batch", [w.L for w in wls], flush=True)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(100 + i)
+        sims.append(s)
+    batch = gpucpm.Batch(sims)
+    kw = run_kwargs(1, 1, 30, 10)
+    batch.run(**kw)
+    batch.synchronize()
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.pull_from_gpu()
+        lat = s.get_state().view(np.uint32).copy()
+        # the same engine alone
+        t = wl.make_cpm()
+        t.push_to_gpu()
+        t.run(**kw, seed=100 + i)
+        t.synchronize()
+        t.pull_from_gpu()
+        np.testing.assert_array_equal(lat, t.get_state().view(np.uint32))
+        lat_o, tab_o = run_oracle(wl, 1, 1, 100 + i, 30)
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 2)
+        assert s.mcs == 30
+
+
+def test_batch_continues_single_runs():
+    """A batch run after single runs continues each replica's stream."""
+    import gpucpm
+    wls = _small_batch()[:3]
+    print("CASE batch-continue", flush=True)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(7 + i)
+        s.run(**run_kwargs(2, 4, 10))
+        sims.append(s)
+    gpucpm.Batch(sims).run(**run_kwargs(2, 4, 15))
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        lat_o, _ = run_oracle(wl, 2, 4, 7 + i, 25)
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
