@@ -104,6 +104,8 @@ struct cpm_engine {
     int64_t t_launches = 0;
     int64_t t_updates = 0;
     Plan last_plan;  // launch form of the last run
+    int want_form = -1;  // cpm_set_launch: -1 auto, else Plan::Mode
+    int want_tile = 0;   // cpm_set_launch: LDS tile edge (0 = heuristic)
 
     int next_events(hipEvent_t* e0, hipEvent_t* e1) {
         *e0 = *e1 = nullptr;
@@ -162,7 +164,10 @@ struct cpm_engine {
     // global | lds forces a form; GPUCPM_TILE sets the 2D tile edge.
     int plan(int c, Plan* p) {
         const char* env = std::getenv("GPUCPM_MODE");
-        const std::string want = env ? env : "phase";
+        std::string want = env ? env : "phase";
+        if (want_form == Plan::kPhase) want = "phase";
+        if (want_form == Plan::kGlobal) want = "global";
+        if (want_form == Plan::kLds2d) want = "lds";
         if (want == "phase") {
             p->mode = Plan::kPhase;
             return CPM_OK;
@@ -170,6 +175,7 @@ struct cpm_engine {
         if (d == 2 && want != "global") {
             int t = std::max(2 * c, std::min(64, L / 16));
             if (const char* te = std::getenv("GPUCPM_TILE")) t = std::atoi(te);
+            if (want_tile > 0) t = want_tile;
             if (t > L) t = L;
             if (is_pow2(t) && t >= 2 * c) {
                 const int lt = ilog2(t);
@@ -186,6 +192,15 @@ struct cpm_engine {
                 }
             }
             if (want == "lds") return fail(CPM_EINVAL, "GPUCPM_MODE=lds: tiles do not fit co-resident");
+        }
+        if (d == 3) {
+            // The persistent global-memory form is switched off in 3D: at
+            // 256^3 (2048 co-resident blocks polling one barrier word) a run
+            // did not finish within its time limit.  3D runs one launch per
+            // colour phase.
+            if (want == "global") return fail(CPM_ENOTIMPL, "GPUCPM_MODE=global is disabled in 3D");
+            p->mode = Plan::kPhase;
+            return CPM_OK;
         }
         int maxb = 0;
         HIP_TRY(gcpm::persist_max_blocks(d, false, 0, &maxb));
@@ -249,6 +264,7 @@ struct cpm_batch {
     hipEvent_t done = nullptr;
     std::vector<hipEvent_t> ready;       // one per replica stream
     unsigned int* d_bars = nullptr;      // 2 words per replica
+    double block_scale = 1.0;            // cpm_batch_set_blocks: blocks per unit of work
     gcpm::ReplicaDesc* d_reps = nullptr;
     int* d_rep_of = nullptr;
     int rep_of_cap = 0;
@@ -680,6 +696,15 @@ int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile) {
     return CPM_OK;
 }
 
+int cpm_set_launch(cpm_handle h, int form, int tile) {
+    if (int rc = check_handle(h)) return rc;
+    if (form < -1 || form > 2) return fail(CPM_EINVAL, "form must be -1 (auto), 0, 1 or 2");
+    if (tile < 0 || (tile && !is_pow2(tile))) return fail(CPM_EINVAL, "tile must be 0 or a power of two");
+    h->want_form = form;
+    h->want_tile = tile;
+    return CPM_OK;
+}
+
 int cpm_set_seed(cpm_handle h, uint64_t seed) {
     if (int rc = check_handle(h)) return rc;
     h->seed = seed;
@@ -720,6 +745,10 @@ int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
     if (!b) return fail(CPM_EHIP, "host allocation failed");
     b->sims.assign(sims, sims + n);
     b->d = sims[0]->d;
+    if (b->d == 3) {
+        delete b;
+        return fail(CPM_ENOTIMPL, "batched 3D replicas are not enabled (the persistent global form is off in 3D)");
+    }
     auto bail = [&](hipError_t err, const char* what) {
         int rc = hip_fail(err, what);
         delete b;
@@ -752,7 +781,8 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     for (int i = 0; i < n; i++) {
         if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
         const long nsub = 1L << (reps[i].lh * b->d);
-        need[i] = std::max(1L, (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock);
+        const long w = (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
+        need[i] = std::max(1L, (long)std::ceil(w * b->block_scale));
     }
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
@@ -818,6 +848,14 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         HIP_TRY(hipStreamWaitEvent(b->sims[i]->stream, b->done, 0));
         b->sims[i]->mcs += (uint64_t)(chunks * a->inner_iterations);
     }
+    return CPM_OK;
+}
+
+int cpm_batch_set_blocks(cpm_batch_handle b, double blocks_per_work) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    if (!(blocks_per_work > 0.0) || blocks_per_work > 16.0)
+        return fail(CPM_EINVAL, "blocks_per_work must be in (0, 16]");
+    b->block_scale = blocks_per_work;
     return CPM_OK;
 }
 

@@ -56,8 +56,30 @@ __device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }
 
-// Lattice held in global memory (HBM / L2 / MALL).
-template <int D>
+// Device-coherent loads and stores (agent-scope relaxed atomics; on gfx950
+// they carry the sc1 bit and bypass the per-XCD caches).  The persistent
+// kernels use them for every word that one block writes and another block
+// reads in a later phase, so a phase boundary needs no L2 writeback or
+// invalidate, only completion of the stores (grid_barrier).
+template <bool CO, class T>
+__device__ __forceinline__ T ld(const T* p) {
+    if constexpr (CO)
+        return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+
+template <bool CO, class T>
+__device__ __forceinline__ void st(T* p, T v) {
+    if constexpr (CO)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
+// Lattice held in global memory (HBM / L2 / MALL); CO selects
+// device-coherent accesses (persistent kernels).
+template <int D, bool CO>
 struct GlobalLat {
     uint32_t* p;
     int lgL, Lm;
@@ -68,11 +90,11 @@ struct GlobalLat {
     __device__ __forceinline__ Pos at(int x, int y, int z) const {
         return {x, y, z, site_index<D>(x, y, z, lgL)};
     }
-    __device__ __forceinline__ uint32_t self(const Pos& q) const { return p[q.i]; }
+    __device__ __forceinline__ uint32_t self(const Pos& q) const { return ld<CO>(p + q.i); }
     __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int dz) const {
-        return p[site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL)];
+        return ld<CO>(p + site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL));
     }
-    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { p[q.i] = v; }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { st<CO>(p + q.i, v); }
 };
 
 // 2D tile of the lattice resident in LDS with a one-site halo ring; row
@@ -95,7 +117,7 @@ struct LdsLat2 {
     }
     __device__ __forceinline__ void put(const Pos& q, uint32_t v) const {
         s[q.li] = v;
-        g[q.gi] = v;
+        st<true>(g + q.gi, v);
     }
 };
 
@@ -104,7 +126,7 @@ struct LdsLat2 {
 // shared-mode roles (target drawn in the sub-block, source a uniformly drawn
 // neighbour), with the deterministic RNG, snapshot tables and integer
 // acceptance described in SweepConsts.
-template <int D, class Lat>
+template <int D, bool CO, class Lat>
 __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k,
                                                    const DevParams* __restrict__ P,
                                                    const uint32_t* __restrict__ thr, int bx,
@@ -139,8 +161,8 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
         // Issue the snapshot-table loads first: their latency overlaps the
         // neighbourhood scan.
-        const int64_t row_s = cs ? R[cs] : 0;
-        const int64_t row_t = ct ? R[ct] : 0;
+        const int64_t row_s = cs ? ld<CO>(R + cs) : 0;
+        const int64_t row_t = ct ? ld<CO>(R + ct) : 0;
         int32_t adh = 0;
         int as = 0, at = 0;
 #pragma unroll
@@ -214,12 +236,13 @@ __device__ __forceinline__ const uint32_t* stage_params(LdsParams& sp, const Swe
 }
 
 // Fold the previous phase's deltas into W (W is never read in a phase).
+template <bool CO>
 __device__ __forceinline__ void fold_deltas(int64_t* W, int64_t* Dp, int ncells, int gtid, int gsz) {
     for (int ci = gtid; ci <= ncells; ci += gsz) {
-        const int64_t dv = Dp[ci];
+        const int64_t dv = ld<CO>(Dp + ci);
         if (dv != 0) {
             atomic_add_i64(W + ci, dv);
-            Dp[ci] = 0;
+            st<CO>(Dp + ci, (int64_t)0);
         }
     }
 }
@@ -243,42 +266,50 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
     __shared__ LdsParams sp;
     const uint32_t* thr = stage_params(sp, k);
     int64_t* W = k.tab[(a.g + 1) & 1];
-    fold_deltas(W, k.del[(a.g + 1) & 1], k.ncells, gtid, gsz);
+    fold_deltas<false>(W, k.del[(a.g + 1) & 1], k.ncells, gtid, gsz);
     if (gtid >= a.nsub) return;
     const int hm = (1 << a.lh) - 1;
     const int bx = 2 * (gtid & hm) + a.cx;
     const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
     const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
-    const GlobalLat<D> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-    sub_block_attempts<D>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1], W,
-                          k.del[a.g & 1]);
+    const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+    sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1], W,
+                                 k.del[a.g & 1]);
 }
 
 // Barrier over nblocks co-resident blocks of a cooperative launch (the
-// whole grid, or the blocks of one replica in a batch).
-// bar[0] counts arrivals, bar[1] is the generation; both persist in device
-// memory between launches (bar[0] returns to 0 at every release).  Every
-// thread fences at agent scope before arriving and after leaving, so plain
-// global stores of one phase are visible to plain loads of the next.
+// whole grid, or the blocks of one replica in a batch).  bar[0] counts
+// arrivals, bar[1] is the generation; both persist in device memory between
+// launches (bar[0] returns to 0 at every release).  Every word that crosses
+// blocks goes through device-coherent accesses (ld/st<true>, atomics), so
+// the barrier only has to make each wave's outstanding memory operations
+// complete before its block arrives (s_waitcnt), with no cache maintenance.
+__device__ __forceinline__ void wait_memory() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 __device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& gen,
                                              unsigned int nblocks) {
-    __threadfence();
+    wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned int g = gen;
         const unsigned int prev =
-            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == nblocks - 1) {
             __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar[1], g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            wait_memory();
+            __hip_atomic_store(&bar[1], g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g)
                 __builtin_amdgcn_s_sleep(1);
         }
+        wait_memory();
     }
     gen++;
     __syncthreads();
-    __threadfence();
 }
 
 // End of a persistent run: fold the last phase's deltas into the lagging
@@ -287,8 +318,8 @@ __device__ __forceinline__ void settle_in_kernel(const SweepConsts& k, int g, in
     int64_t* T = k.tab[(g + 1) & 1];
     int64_t* Dl = k.del[(g + 1) & 1];
     for (int ci = gtid; ci <= k.ncells; ci += gsz) {
-        T[ci] += Dl[ci];
-        Dl[ci] = 0;
+        st<true>(T + ci, ld<true>(T + ci) + ld<true>(Dl + ci));
+        st<true>(Dl + ci, (int64_t)0);
     }
 }
 
@@ -303,7 +334,7 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
     const int gsz = nblocks * blockDim.x;
     unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t* thr = stage_params(sp, k);
-    const GlobalLat<D> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+    const GlobalLat<D, true> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
     const int hm = (1 << lh) - 1;
     const int nsub = 1 << (D * lh);
     constexpr int NCOL = 1 << D;
@@ -316,7 +347,7 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
                 int cx, cy, cz;
                 colour_of<D>(col, cx, cy, cz);
                 int64_t* W = k.tab[(g + 1) & 1];
-                fold_deltas(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
+                fold_deltas<true>(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
                 const int64_t* R = k.tab[g & 1];
                 int64_t* Dc = k.del[g & 1];
                 const uint32_t phase = (uint32_t)(r * NCOL + col);
@@ -324,7 +355,7 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
                     const int bx = 2 * (q & hm) + cx;
                     const int by = 2 * ((q >> lh) & hm) + cy;
                     const int bz = D == 3 ? 2 * (q >> (2 * lh)) + cz : 0;
-                    sub_block_attempts<D>(lat, k, &sp.p, thr, bx, by, bz, mcs, phase, R, W, Dc);
+                    sub_block_attempts<D, true>(lat, k, &sp.p, thr, bx, by, bz, mcs, phase, R, W, Dc);
                 }
             }
         }
@@ -348,7 +379,7 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_global(BatchArgs a) {
     __shared__ LdsParams sp;
     const int r = a.rep_of[blockIdx.x];
-    const ReplicaDesc& rd = a.reps[r];
+    const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
     persist_global_run<D>(rd.k, rd.mcs0, a.nmcs, rd.lh, a.bars + 2 * r, blockIdx.x - rd.block0,
                           rd.nblocks, sp);
 }
@@ -394,22 +425,22 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
                         else if (i < 2 * S) { lx = i - S; ly = TY + 1; }
                         else if (i < 2 * S + TY) { lx = 0; ly = i - 2 * S + 1; }
                         else { lx = TX + 1; ly = i - 2 * S - TY + 1; }
-                        s[ly * S + lx] = gl[(uint32_t)((x0 + lx - 1) & Lm) |
-                                            ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
+                        s[ly * S + lx] = ld<true>(gl + ((uint32_t)((x0 + lx - 1) & Lm) |
+                                                        ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)));
                     }
                     __syncthreads();
                 }
                 int cx, cy, cz;
                 colour_of<2>(col, cx, cy, cz);
                 int64_t* W = k.tab[(g + 1) & 1];
-                fold_deltas(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
+                fold_deltas<true>(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
                 const int64_t* R = k.tab[g & 1];
                 int64_t* Dc = k.del[g & 1];
                 const uint32_t phase = (uint32_t)(r * 4 + col);
                 for (int q = threadIdx.x; q < nq; q += blockDim.x) {
                     const int bx = (x0 >> k.lc) + 2 * (q & ((1 << lq) - 1)) + cx;
                     const int by = (y0 >> k.lc) + 2 * (q >> lq) + cy;
-                    sub_block_attempts<2>(lat, k, &sp.p, thr, bx, by, 0, mcs, phase, R, W, Dc);
+                    sub_block_attempts<2, true>(lat, k, &sp.p, thr, bx, by, 0, mcs, phase, R, W, Dc);
                 }
             }
         }

@@ -278,6 +278,11 @@ class Cpm:
         check(lib.cpm_timing_end(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(upd)))
         return ms.value, n.value, upd.value
 
+    def set_launch(self, form=-1, tile=0):
+        """Launch form (-1 auto, 0 per phase, 1 persistent global, 2 persistent
+        LDS tiles) and LDS tile edge (0 auto).  Results never depend on them."""
+        check(lib.cpm_set_launch(self._h, _int(form, "form"), _int(tile, "tile")))
+
     def last_launch(self):
         """Launch form of the last run: dict(form, kernel, blocks, tile)."""
         f, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -335,6 +340,10 @@ class Batch:
 
     def synchronize(self):
         check(lib.cpm_batch_sync(self._h))
+
+    def set_blocks(self, blocks_per_work):
+        """Blocks per unit of work in each replica's group (default 1.0)."""
+        check(lib.cpm_batch_set_blocks(self._h, float(blocks_per_work)))
 
     @property
     def blocks(self):

@@ -149,6 +149,11 @@ int cpm_timing_end(cpm_handle h, double* kernel_ms, int64_t* launches, int64_t* 
  * tile edge (0 if none).  Results never depend on the form. */
 int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile);
 
+/* Extension: choose the launch form (-1 = automatic, else a form number as
+ * above) and the LDS tile edge (0 = automatic) for this engine's runs; the
+ * autotuner sets these.  Results never depend on them. */
+int cpm_set_launch(cpm_handle h, int form, int tile);
+
 /* Extension (no counterpart in the reference, whose parameters are one
  * global __constant__ block, kernels.cu:48-53): batched replicas.  A batch
  * is a fixed list of pushed engines of one dimensionality, each with its own
@@ -161,6 +166,10 @@ int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile);
 int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out);
 int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* args);
 int cpm_batch_sync(cpm_batch_handle b);
+/* Blocks per unit of work (one block-width of sub-blocks) given to each
+ * replica's group, default 1; the autotuner sets it.  Results never depend
+ * on it. */
+int cpm_batch_set_blocks(cpm_batch_handle b, double blocks_per_work);
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks);
 int cpm_batch_destroy(cpm_batch_handle b);
 

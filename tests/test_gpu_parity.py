@@ -134,11 +134,13 @@ def test_full_size_tables_equal_recount():
 @pytest.mark.parametrize("mode", ["phase", "global", "lds"])
 def test_launch_forms_agree_with_oracle(mode, monkeypatch):
     """The per-phase, persistent-global and persistent-LDS launch forms all
-    reproduce the oracle (2D), and phase/global do in 3D."""
+    reproduce the oracle in 2D; 3D runs per phase."""
     _, W = _gpu()
     monkeypatch.setenv("GPUCPM_MODE", mode)
     cases = [(W.plain2d(L=64, ncells=20, seed=21), 2, 4, 40), (W.sorting2d(L=128, grid=9, edge=10, seed=22), 1, 1, 30)]
-    if mode != "lds":
+    if mode == "phase":
+        # the persistent global form is switched off in 3D (a 256^3 run
+        # exceeded its time limit); 3D always launches per phase
         cases.append((W.plain3d(L=32, ncells=60, seed=23), 1, 1, 8))
     for wl, c, u, n in cases:
         _announce("mode", mode, wl.name, wl.L, c, u)
