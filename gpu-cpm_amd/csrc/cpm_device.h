@@ -60,6 +60,7 @@ struct PhaseArgs {
     int g;                  // phase counter since the last settle (parity selects buffers)
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int nsub;               // active sub-blocks in this phase
+    int box3;               // 3D, c = 1: stage each block's box in LDS
 };
 
 // A whole run in one cooperative launch, phases separated by grid barriers.

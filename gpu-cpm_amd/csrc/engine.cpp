@@ -555,6 +555,10 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         pa.k = k;
         pa.lh = lh;
         pa.nsub = 1 << (lh * h->d);
+        // GPUCPM_BOX3=1 stages 3D boxes in LDS (c = 1).  Measured slower than
+        // global reads at 256^3 (61 vs 44 us per phase), so it is opt-in.
+        const char* bx = std::getenv("GPUCPM_BOX3");
+        pa.box3 = bx ? std::atoi(bx) : 0;
         const int ncol = 1 << h->d;
         int g = 0;  // phase counter since the tables were last settled
         for (long m = 0; m < total; m++) {

@@ -121,6 +121,31 @@ struct LdsLat2 {
     }
 };
 
+// A 3D box of the lattice staged in LDS for one colour phase, with a
+// one-site halo; strides SX (x) and SXY (z).  Writes go to LDS and to global
+// memory (plain stores: the next launch sees them).
+struct LdsLat3 {
+    uint32_t* s;
+    uint32_t* g;
+    int SX, SXY, x0, y0, z0, lgL;
+    struct Pos {
+        int li;
+        uint32_t gi;
+    };
+    __device__ __forceinline__ Pos at(int x, int y, int z) const {
+        return {(z - z0 + 1) * SXY + (y - y0 + 1) * SX + (x - x0 + 1),
+                (uint32_t)x | ((uint32_t)y << lgL) | ((uint32_t)z << (2 * lgL))};
+    }
+    __device__ __forceinline__ uint32_t self(const Pos& q) const { return s[q.li]; }
+    __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int dz) const {
+        return s[q.li + dz * SXY + dy * SX + dx];
+    }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const {
+        s[q.li] = v;
+        g[q.gi] = v;
+    }
+};
+
 // The u copy attempts of sub-block (bx, by, bz) in one colour phase: the
 // copy attempt of kernels2d.cu:197-570 / kernels3d.cu:165-583 in the
 // shared-mode roles (target drawn in the sub-block, source a uniformly drawn
@@ -149,31 +174,50 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         const int tx = (bx << lc) + (int)(w[0] & cm);
         const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
         const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
-        const int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
-        int ox, oy, oz;
-        block_offset<D>(dir < CENTRE ? dir : dir + 1, ox, oy, oz);
+        int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+        dir += dir >= CENTRE;  // index in the 3^D block, skipping the centre
         const typename Lat::Pos t = lat.at(tx, ty, tz);
-        const uint32_t vs = lat.nb(t, ox, oy, oz);
-        const uint32_t vt = lat.self(t);
+        // The whole neighbourhood in one round of independent loads; the
+        // source is the neighbour picked by dir.
+        uint32_t nv[NB];
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            int nx, ny, nz;
+            block_offset<D>(n, nx, ny, nz);
+            nv[n] = n == CENTRE ? lat.self(t) : lat.nb(t, nx, ny, nz);
+        }
+        const uint32_t vt = nv[CENTRE];
+        uint32_t vs = 0;
+#pragma unroll
+        for (int n = 0; n < NB; n++) vs = n == dir ? nv[n] : vs;
         if (vs == vt) continue;  // kernels2d.cu:276
 
         const uint32_t ts = vs >> 24, tt = vt >> 24;
         const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
         // Issue the snapshot-table loads first: their latency overlaps the
-        // neighbourhood scan.
+        // neighbourhood sums.
         const int64_t row_s = cs ? ld<CO>(R + cs) : 0;
         const int64_t row_t = ct ? ld<CO>(R + ct) : 0;
+        int anc_s[D], anc_t[D];  // centroid anchors (read-only during a run)
+#pragma unroll
+        for (int ax = 0; ax < D; ax++) {
+            anc_s[ax] = k.anchor[3 * cs + ax];
+            anc_t[ax] = k.anchor[3 * ct + ax];
+        }
+        const int32_t* __restrict__ Js = P->J[ts];
+        const int32_t* __restrict__ Jt = P->JS[tt];
         int32_t adh = 0;
         int as = 0, at = 0;
 #pragma unroll
         for (int n = 0; n < NB; n++) {
             if (n == CENTRE) continue;
-            int nx, ny, nz;
-            block_offset<D>(n, nx, ny, nz);
-            const uint32_t vn = lat.nb(t, nx, ny, nz);
+            const uint32_t vn = nv[n];
             const uint32_t tn = vn >> 24;
-            if (vn != vs) adh += P->J[ts][tn]; else as++;
-            if (vn != vt) adh -= P->JS[tt][tn]; else at++;
+            const int32_t js = Js[tn], jt = Jt[tn];
+            const bool eqs = vn == vs, eqt = vn == vt;
+            adh += (eqs ? 0 : js) - (eqt ? 0 : jt);
+            as += eqs;
+            at += eqt;
         }
         int64_t dh = adh;
         if (cs) {
@@ -200,7 +244,7 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
             atomic_add_i64(Dc + cs, delta);
 #pragma unroll
             for (int ax = 0; ax < D; ax++) {
-                const int v = pos[ax] - k.anchor[3 * cs + ax];
+                const int v = pos[ax] - anc_s[ax];
                 atomic_add_i64(k.csum + 3 * cs + ax, (int64_t)(((v + half) & Lm) - half));
             }
         }
@@ -210,7 +254,7 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
             atomic_add_i64(Dc + ct, delta);
 #pragma unroll
             for (int ax = 0; ax < D; ax++) {
-                const int v = pos[ax] - k.anchor[3 * ct + ax];
+                const int v = pos[ax] - anc_t[ax];
                 atomic_add_i64(k.csum + 3 * ct + ax, -(int64_t)(((v + half) & Lm) - half));
             }
         }
@@ -225,12 +269,33 @@ struct LdsParams {
 };
 
 __device__ __forceinline__ const uint32_t* stage_params(LdsParams& sp, const SweepConsts& k) {
+    // All global loads are issued before any LDS store, so staging costs
+    // one memory round trip.
+    constexpr int NP = (int)(sizeof(DevParams) / 4);
+    constexpr int RP = (NP + kBlock - 1) / kBlock;
+    constexpr int RT = kThrLds / kBlock;
     const int* src = reinterpret_cast<const int*>(k.prm);
     int* dst = reinterpret_cast<int*>(&sp.p);
-    for (int i = threadIdx.x; i < (int)(sizeof(DevParams) / 4); i += blockDim.x) dst[i] = src[i];
     const bool fits = k.nthr <= kThrLds;
-    if (fits)
-        for (int i = threadIdx.x; i < k.nthr; i += blockDim.x) sp.thr[i] = k.thr[i];
+    int pv[RP];
+    uint32_t tv[RT];
+#pragma unroll
+    for (int j = 0; j < RP; j++) {
+        const int i = threadIdx.x + j * kBlock;
+        pv[j] = i < NP ? src[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < RT; j++) {
+        const int i = threadIdx.x + j * kBlock;
+        tv[j] = fits && i < k.nthr ? k.thr[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < RP; j++) {
+        const int i = threadIdx.x + j * kBlock;
+        if (i < NP) dst[i] = pv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < RT; j++) sp.thr[threadIdx.x + j * kBlock] = tv[j];
     __syncthreads();
     return fits ? sp.thr : k.thr;
 }
@@ -266,16 +331,65 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
     __shared__ LdsParams sp;
     const uint32_t* thr = stage_params(sp, k);
     int64_t* W = k.tab[(a.g + 1) & 1];
+    if (gtid < a.nsub) {
+        const int hm = (1 << a.lh) - 1;
+        const int bx = 2 * (gtid & hm) + a.cx;
+        const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
+        const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
+        const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1],
+                                     W, k.del[a.g & 1]);
+    }
+    // W is not read in this phase, so the fold may come after the attempts.
     fold_deltas<false>(W, k.del[(a.g + 1) & 1], k.ncells, gtid, gsz);
-    if (gtid >= a.nsub) return;
-    const int hm = (1 << a.lh) - 1;
-    const int bx = 2 * (gtid & hm) + a.cx;
-    const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
-    const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
-    const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-    sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1], W,
-                                 k.del[a.g & 1]);
 }
+
+// One 3D colour phase with positions_per_checkerboard = 1 (every sub-block
+// is one site): block b stages the box of kBX x kBY x kBZ sites plus halo in
+// LDS with coalesced loads and makes the attempts of the box's active sites
+// (one per thread) against LDS.
+constexpr int kBX = 16, kBY = 16, kBZ = 8;
+
+__global__ __launch_bounds__(kBlock) void k_sweep_phase3_box(PhaseArgs a) {
+    constexpr int SX = kBX + 2, SY = kBY + 2, SZ = kBZ + 2, SXY = SX * SY, NS = SXY * SZ;
+    __shared__ LdsParams sp;
+    __shared__ uint32_t box[NS];
+    const SweepConsts& k = a.k;
+    const int lgL = k.lgL, Lm = (1 << lgL) - 1;
+    const int nbx = (Lm + 1) / kBX, nby = (Lm + 1) / kBY;
+    const int b = blockIdx.x;
+    const int x0 = (b % nbx) * kBX, y0 = ((b / nbx) % nby) * kBY, z0 = (b / (nbx * nby)) * kBZ;
+    const uint32_t* __restrict__ gl = k.lat;
+    // Box + halo: all loads issued before the LDS stores.
+    constexpr int R = (NS + kBlock - 1) / kBlock;
+    uint32_t v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const int i = threadIdx.x + j * kBlock;
+        if (i < NS) {
+            const int lz = i / SXY, rem = i - lz * SXY, ly = rem / SX, lx = rem - ly * SX;
+            v[j] = gl[(uint32_t)((x0 + lx - 1) & Lm) | ((uint32_t)((y0 + ly - 1) & Lm) << lgL) |
+                      ((uint32_t)((z0 + lz - 1) & Lm) << (2 * lgL))];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const int i = threadIdx.x + j * kBlock;
+        if (i < NS) box[i] = v[j];
+    }
+    const uint32_t* thr = stage_params(sp, k);  // ends with __syncthreads
+    // kBX/2 * kBY/2 * kBZ/2 = 256 active sites of this colour per box
+    const int t = threadIdx.x;
+    const int qx = t % (kBX / 2), qy = (t / (kBX / 2)) % (kBY / 2), qz = t / ((kBX / 2) * (kBY / 2));
+    const LdsLat3 lat{box, k.lat, SX, SXY, x0, y0, z0, lgL};
+    int64_t* W = k.tab[(a.g + 1) & 1];
+    sub_block_attempts<3, false>(lat, k, &sp.p, thr, x0 + 2 * qx + a.cx, y0 + 2 * qy + a.cy,
+                                 z0 + 2 * qz + a.cz, a.mcs, a.phase, k.tab[a.g & 1], W,
+                                 k.del[a.g & 1]);
+    fold_deltas<false>(W, k.del[(a.g + 1) & 1], k.ncells, b * blockDim.x + threadIdx.x,
+                       gridDim.x * blockDim.x);
+}
+static_assert((kBX / 2) * (kBY / 2) * (kBZ / 2) == kBlock, "one active site per thread");
 
 // Barrier over nblocks co-resident blocks of a cooperative launch (the
 // whole grid, or the blocks of one replica in a batch).  bar[0] counts
@@ -590,6 +704,14 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
     const int blocks = (work + kBlock - 1) / kBlock;
     // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
     // (null events: plain launch).
+    const int L = 1 << a.k.lgL;
+    if (d == 3 && a.k.lc == 0 && L >= kBX && L >= kBY && L >= kBZ && a.box3) {
+        const int nbox = (L / kBX) * (L / kBY) * (L / kBZ);
+        if ((long)nbox * kBlock >= a.k.ncells + 1) {
+            hipExtLaunchKernelGGL(k_sweep_phase3_box, dim3(nbox), dim3(kBlock), 0, s, e0, e1, 0, a);
+            return hipGetLastError();
+        }
+    }
     if (d == 2)
         hipExtLaunchKernelGGL(k_sweep_phase<2>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     else

@@ -161,3 +161,17 @@ def test_lds_tile_size_does_not_change_results(tile, monkeypatch):
     lat_o, tab_o = run_oracle(wl, 2, 3, 8, 25)
     np.testing.assert_array_equal(lat, lat_o)
     assert_tables_equal(table, tab_o, 2)
+
+
+@pytest.mark.parametrize("box", ["0", "1"])
+def test_3d_box_staging_matches_oracle(box, monkeypatch):
+    """3D with c = 1 reads global memory (GPUCPM_BOX3=0, default) or stages
+    16x16x8 boxes in LDS (1); both reproduce the oracle."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_BOX3", box)
+    wl = W.plain3d(L=64, ncells=300, seed=25)
+    _announce("box3", box, wl.L)
+    _, lat, table = _run_gpu(wl, 1, 1, seed=17, nmcs=6)
+    lat_o, tab_o = run_oracle(wl, 1, 1, 17, 6)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 3)
