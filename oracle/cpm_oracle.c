@@ -370,6 +370,73 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
     return 0;
 }
 
+/* Serial random-sequential CPM (the dynamics the reference's parallel
+ * sweep approximates): each MCS makes L^d attempts, each at a uniformly
+ * drawn target with a uniformly drawn neighbour as source, evaluated and
+ * committed one at a time against the live tables.  Same Hamiltonian and
+ * acceptance table as orc_sweep; its own stream (Philox keyed by seed and
+ * the attempt counter).  Used only for statistical comparisons. */
+int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                     const int32_t* anchor, int64_t* csum, const orc_params* p, uint64_t seed,
+                     uint32_t mcs0, int nmcs) {
+    if (d != 2 && d != 3) return -1;
+    if (L < 2 || (L & (L - 1))) return -2;
+    int scale = orc_energy_scale(p);
+    if (scale < 0) return -5;
+    int cap = 1 << 22;
+    uint32_t* thr = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)cap);
+    int nthr = orc_threshold_table(p->temperature, scale, thr, cap);
+    if (nthr < 0) {
+        free(thr);
+        return -6;
+    }
+    orc_scaled q;
+    scale_params(p, scale, &q);
+    orc_nbhd nb;
+    make_nbhd(d, &nb);
+    const int N = nb.n;
+    const int lgL = log2i(L);
+    const uint32_t key[2] = {(uint32_t)seed ^ 0x5E71A1u, (uint32_t)(seed >> 32)};
+    const int64_t sites = d == 3 ? (int64_t)L * L * L : (int64_t)L * L;
+    for (int m = 0; m < nmcs; m++) {
+        uint32_t mcs = mcs0 + (uint32_t)m;
+        for (int64_t a = 0; a < sites; a++) {
+            uint32_t ctr[4] = {(uint32_t)a, (uint32_t)(a >> 32), mcs, 0x5E71A1u};
+            uint32_t w[4];
+            orc_philox4x32_10(ctr, key, w);
+            int tx = (int)(w[0] & (uint32_t)(L - 1));
+            int ty = (int)((w[0] >> lgL) & (uint32_t)(L - 1));
+            int tz = d == 3 ? (int)(w[3] & (uint32_t)(L - 1)) : 0;
+            int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+            int sx = (tx + nb.off[dir][0]) & (L - 1);
+            int sy = (ty + nb.off[dir][1]) & (L - 1);
+            int sz = d == 3 ? (tz + nb.off[dir][2]) & (L - 1) : 0;
+            int64_t ti = site(L, tx, ty, tz);
+            uint32_t vs = lat[site(L, sx, sy, sz)];
+            uint32_t vt = lat[ti];
+            if (vs == vt) continue;
+            int as, at;
+            int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, vs, vt, &as, &at);
+            if (!(dh <= 0 || (dh < nthr && w[2] < thr[dh]))) continue;
+            lat[ti] = vs;
+            uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
+            int pos[3] = {tx, ty, tz};
+            if (cs) {
+                area[cs] += 1;
+                perim[cs] += N - 2 * as;
+                for (int k = 0; k < d; k++) csum[3 * cs + k] += wrapc(L, pos[k] - anchor[3 * cs + k]);
+            }
+            if (ct) {
+                area[ct] -= 1;
+                perim[ct] += 2 * at - N;
+                for (int k = 0; k < d; k++) csum[3 * ct + k] -= wrapc(L, pos[k] - anchor[3 * ct + k]);
+            }
+        }
+    }
+    free(thr);
+    return 0;
+}
+
 /* ----------------------------------------------------------- energy -- */
 
 int orc_energy_stats(int L, int d, int ncells, const uint32_t* lat, const orc_params* p,

@@ -75,6 +75,12 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
               const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
               uint64_t seed, uint32_t mcs0, int nmcs, int nthreads);
 
+/* Serial random-sequential CPM with live tables (statistical comparisons
+ * only; not bit-comparable with orc_sweep). */
+int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                     const int32_t* anchor, int64_t* csum, const orc_params* p, uint64_t seed,
+                     uint32_t mcs0, int nmcs);
+
 /* Scaled integer dH of copying site s into site t (both linear indices) with
  * the given tables (used to pin dH against H(after)-H(before)). */
 int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,

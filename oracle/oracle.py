@@ -91,6 +91,9 @@ def lib():
                                 P(Params), ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                 ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.orc_sweep.restype = ctypes.c_int
+        L.orc_serial_sweep.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, i32p,
+                                       i64p, P(Params), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int]
+        L.orc_serial_sweep.restype = ctypes.c_int
         L.orc_delta_h.argtypes = [ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, P(Params), ctypes.c_int,
                                   ctypes.c_int64, ctypes.c_int64]
         L.orc_delta_h.restype = ctypes.c_int64
@@ -180,6 +183,19 @@ def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, se
                          int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads))
     if rc:
         raise ValueError(f"orc_sweep failed with code {rc}")
+
+
+def serial_sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, seed: int, mcs0: int,
+                 nmcs: int) -> None:
+    """Serial random-sequential CPM (statistical comparisons only)."""
+    lat32 = lat.view(np.uint32)
+    rc = lib().orc_serial_sweep(lat.shape[0], d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
+                                _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+                                _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                                ctypes.byref(p), int(seed) & (2**64 - 1), int(mcs0) & 0xFFFFFFFF,
+                                int(nmcs))
+    if rc:
+        raise ValueError(f"orc_serial_sweep failed with code {rc}")
 
 
 def delta_h(lat, d, tables: Tables, p: Params, scale: int, t: int, s: int) -> int:
