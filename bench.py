@@ -33,11 +33,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mcs-per-step", type=int, default=200)
-    ap.add_argument("--workload", default="sorting2d", choices=["sorting2d", "plain3d", "plain2d"])
+    ap.add_argument("--workload", default="sorting2d",
+                    choices=["sorting2d", "plain3d", "plain2d", "batched"])
+    ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--checkerboard", type=int, default=1, help="positions_per_checkerboard c")
     ap.add_argument("--updates", type=int, default=1, help="updates_per_checkerboard_switch u")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (hipMemcpy), the known-size "
+                         "reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -69,13 +74,113 @@ def cpu_baseline(wl, c, u, seconds):
             "mcs_per_s": n / dt}
 
 
+def bench_batched(a, log):
+    """Batched replicas: 64 sorting lattices of mixed sizes and parameters in
+    one batch (one launch per colour phase for all of them) vs the same
+    replicas run one after another; both must end bit-identical."""
+    import hashlib
+
+    import numpy as np
+
+    import gpucpm
+    from gpucpm import workloads
+
+    wls = workloads.batched_sorting(nrep=a.replicas, seed=4)
+    c, u = a.checkerboard, a.updates
+    mcs = a.steps * a.mcs_per_step
+    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
+              positions_per_thread=2 * c, positions_per_checkerboard=c,
+              updates_per_checkerboard_switch=u, updates_per_barrier=1)
+    sites = sum(w.L ** 2 for w in wls)
+
+    def fresh():
+        sims = []
+        for i, wl in enumerate(wls):
+            s = wl.make_cpm()
+            s.push_to_gpu()
+            s.set_seed(9000 + i)
+            sims.append(s)
+        return sims
+
+    def digest(sims):
+        h = hashlib.sha256()
+        for s in sims:
+            s.pull_from_gpu()
+            h.update(s.get_state().tobytes())
+        return h.hexdigest()
+
+    log(f"batched: {len(wls)} replicas, {sites / 1e6:.1f} M sites, {mcs} MCS; batch")
+    sims = fresh()
+    batch = gpucpm.Batch(sims)
+    batch.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)  # warmup
+    batch.synchronize()
+    sims = fresh()
+    batch = gpucpm.Batch(sims)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        batch.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+    batch.synchronize()
+    t_batch = time.perf_counter() - t0
+    d_batch = digest(sims)
+    log(f"batch {t_batch:.3f} s; sequential")
+    sims = fresh()
+    t0 = time.perf_counter()
+    for s in sims:
+        for _ in range(a.steps):
+            s.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+        s.synchronize()
+    t_seq = time.perf_counter() - t0
+    d_seq = digest(sims)
+    log(f"sequential {t_seq:.3f} s")
+    out = {
+        "metric": "lattice site-updates/sec (batched replicas)",
+        "value": mcs * sites / t_batch,
+        "unit": "site-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": 1,
+        "ms_per_step": t_batch / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{len(wls)} sorting replicas, L in {{128,256,512,1024}}, "
+                               "J_AB in [6,14], T in [5,20], L^2/250 cells",
+                   "replicas": len(wls), "sites": sites, "mcs": mcs,
+                   "positions_per_checkerboard": c, "updates_per_checkerboard_switch": u},
+        "sequential_value": mcs * sites / t_seq,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "bit_identical_to_sequential": d_batch == d_seq,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """Three device-to-device copies of a known size through the HIP runtime
+    (its copy kernel reads and writes exactly mib MiB per copy)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(replicas shard across GPUs without exchange; see the sharding design).",
               file=sys.stderr)
         sys.exit(2)
+    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    if a.workload == "batched":
+        return bench_batched(a, log)
     import gpucpm
     from gpucpm import workloads
 
@@ -89,7 +194,6 @@ def main():
               positions_per_thread=2 * c, positions_per_checkerboard=c,
               updates_per_checkerboard_switch=u, updates_per_barrier=1,
               iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
-    log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
     log(f"{a.workload} L={wl.L} d={wl.d} cells={wl.ncells} c={c} u={u} "
         f"mode={os.environ.get('GPUCPM_MODE', 'default')}: warmup {a.warmup} x {a.mcs_per_step} MCS")
     for _ in range(a.warmup):

@@ -15,31 +15,33 @@ constexpr int kNTypes = 20;  // settings.h:4
 constexpr uint32_t kSpinMask = 0xFFFFFFu;
 
 // Hamiltonian parameters in fixed point: every entry is the reference value
-// multiplied by the engine's dyadic scale, so dH is an exact integer.
+// multiplied by the engine's dyadic scale, so dH is an exact integer.  Only
+// the nt types present in the lattice are stored (nt = 1 + largest type at
+// push), pair tables row-major with stride nt, so kernels stage
+// 2 nt^2 + 4 nt words.
 struct DevParams {
-    int32_t J[kNTypes][kNTypes];   // adhesion * scale            (settings.h:20)
-    int32_t JS[kNTypes][kNTypes];  // (adhesion + stickiness) * scale (settings.h:22)
-    int32_t la[kNTypes];           // lambda_area * scale         (settings.h:11)
-    int32_t a0[kNTypes];           // target area                 (settings.h:12)
-    int32_t lp[kNTypes];           // lambda_perimeter * scale    (settings.h:17)
-    int32_t p0[kNTypes];           // target perimeter            (settings.h:18)
+    int32_t J[kNTypes * kNTypes];   // adhesion * scale, [a * nt + b]            (settings.h:20)
+    int32_t JS[kNTypes * kNTypes];  // (adhesion + stickiness) * scale, [a * nt + b] (settings.h:22)
+    int32_t la[kNTypes];            // lambda_area * scale                       (settings.h:11)
+    int32_t a0[kNTypes];            // target area                               (settings.h:12)
+    int32_t lp[kNTypes];            // lambda_perimeter * scale                  (settings.h:17)
+    int32_t p0[kNTypes];            // target perimeter                          (settings.h:18)
 };
 
 // Constants of one sweep (one run() call).
 //
 // Cell tables are packed int64 rows: perimeter * 2^32 + area.  Colour phase
-// g reads tab[g & 1] (its start-of-phase snapshot), adds every accepted
-// flip's delta to W = tab[(g + 1) & 1] and to del[g & 1], and first folds the
-// previous phase's deltas del[(g + 1) & 1] into W (then clears them).  W is
-// never read during the phase, so after it W holds the end-of-phase state
-// with no extra launch or barrier.  A final settle folds the last deltas
-// into the lagging buffer, leaving both buffers equal and both deltas zero.
+// g (counted from the last settle) reads tab[g & 1], its start-of-phase
+// snapshot; every accepted flip adds its delta to W = tab[(g + 1) & 1] and
+// to del[g & 1]; and the phase also folds the previous phase's deltas
+// del[(g + 1) & 1] into W (then clears them).  W is never read in a phase,
+// so after it W holds the end-of-phase state with no extra launch or
+// barrier.  A settle after the last phase folds the last deltas into the
+// lagging base, leaving both bases equal and both delta buffers zero.
 struct SweepConsts {
     uint32_t* lat;
     int64_t* tab[2];
     int64_t* del[2];
-    const int32_t* anchor;  // [(n+1)*3]
-    int64_t* csum;          // [(n+1)*3]
     const DevParams* prm;
     const uint32_t* thr;    // acceptance thresholds, thr[k] for 0 <= k < nthr
     int nthr;
@@ -49,6 +51,7 @@ struct SweepConsts {
     int u;                  // updates_per_checkerboard_switch
     int rounds;             // c^d / u colour rounds per MCS
     uint32_t k0, k1;        // Philox key (seed)
+    int nt;                 // types present (pair-table stride)
 };
 
 // One colour phase per launch.
@@ -57,7 +60,7 @@ struct PhaseArgs {
     int cx, cy, cz;         // colour parity
     uint32_t phase;         // round * 2^d + colour
     uint32_t mcs;           // Monte Carlo step counter
-    int g;                  // phase counter since the last settle (parity selects buffers)
+    int g;                  // phase counter since the last settle (its parity selects buffers)
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int nsub;               // active sub-blocks in this phase
     int box3;               // 3D, c = 1: stage each block's box in LDS
@@ -90,6 +93,17 @@ struct BatchArgs {
     int nmcs;               // MCS per replica in this launch
 };
 
+// Batched replicas, one launch per colour phase: block b serves replica
+// rep_of[b]; every replica has the same c and u, so phases stay in step.
+struct BatchPhaseArgs {
+    const ReplicaDesc* reps;
+    const int* rep_of;
+    int cx, cy, cz;
+    uint32_t phase;         // round * 2^d + colour
+    uint32_t m;             // MCS offset within this run (replica MCS = mcs0 + m)
+    int g;                  // phase counter since the last settle (parity)
+};
+
 // Host-side launchers (kernels.hip).
 hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 // Cooperative persistent launches.  lds2d selects the 2D LDS-tiled kernel.
@@ -99,9 +113,16 @@ hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, s
                           hipStream_t s);
 hipError_t batch_max_blocks(int d, int* max_blocks);
 hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s);
+hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s);
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,
-                        uint32_t* first, int32_t* anchor, int64_t* csum, hipStream_t s);
-hipError_t launch_settle(int64_t* tab, int64_t* del, int ncells, hipStream_t s);
+                        uint32_t* first, int32_t* anchor, hipStream_t s);
+// Settle after G phases of a run (only the parity of G matters).
+hipError_t launch_settle(int64_t* const tab[2], int64_t* const del[2], int G, int ncells,
+                         hipStream_t s);
+// Centroid sums of the current lattice about the push-time anchors
+// (csum must be zero on entry).
+hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* anchor, int64_t* csum,
+                       hipStream_t s);
 // pairs[20*20] and mom[20*5] must be zero on entry; ctype[n+1] is scratch.
 hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab, int ncells,
                         unsigned long long* pairs, unsigned long long* mom, int32_t* ctype,

@@ -96,6 +96,7 @@ struct cpm_engine {
     int thr_cap = 0;
     int nthr = 0;
     int scale = 1;
+    int nt = 1;  // types present in the lattice at the last push
 
     // Kernel timing (cpm_timing_begin/end): one event pair per sweep launch.
     bool timing = false;
@@ -125,10 +126,8 @@ struct cpm_engine {
         // Teardown is best effort: errors here have no caller to report to.
         if (stream) (void)hipStreamSynchronize(stream);
         (void)hipFree(d_lat);
-        for (int i = 0; i < 2; i++) {
-            (void)hipFree(d_tab[i]);
-            (void)hipFree(d_del[i]);
-        }
+        for (int i = 0; i < 2; i++) (void)hipFree(d_tab[i]);
+        for (int i = 0; i < 2; i++) (void)hipFree(d_del[i]);
         (void)hipFree(d_anchor);
         (void)hipFree(d_csum);
         (void)hipFree(d_first);
@@ -219,11 +218,12 @@ struct cpm_engine {
                         "lambda_area / lambda_perimeter / adhesion must be multiples of 2^-8 "
                         "(dyadic fixed-point energy)");
         gcpm::DevParams p{};
-        for (int a = 0; a < CPM_NTYPES; a++) {
-            for (int b = 0; b < CPM_NTYPES; b++) {
-                p.J[a][b] = (int32_t)(st.J[a][b] * sc);
-                p.JS[a][b] = (int32_t)(st.J[a][b] * sc) + (int32_t)st.S[a][b] * sc;
+        for (int a = 0; a < nt; a++)
+            for (int b = 0; b < nt; b++) {
+                p.J[a * nt + b] = (int32_t)(st.J[a][b] * sc);
+                p.JS[a * nt + b] = (int32_t)(st.J[a][b] * sc) + (int32_t)st.S[a][b] * sc;
             }
+        for (int a = 0; a < CPM_NTYPES; a++) {
             p.la[a] = (int32_t)(st.lam_area[a] * sc);
             p.lp[a] = (int32_t)(st.lam_perim[a] * sc);
             p.a0[a] = st.target_area[a];
@@ -265,6 +265,7 @@ struct cpm_batch {
     std::vector<hipEvent_t> ready;       // one per replica stream
     unsigned int* d_bars = nullptr;      // 2 words per replica
     double block_scale = 1.0;            // cpm_batch_set_blocks: blocks per unit of work
+    int form = 0;                        // cpm_batch_set_form: 0 per phase, 1 persistent
     gcpm::ReplicaDesc* d_reps = nullptr;
     int* d_rep_of = nullptr;
     int rep_of_cap = 0;
@@ -346,10 +347,10 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
         return bail(err, "hipHostMalloc(lattice)");
     std::memset(g->h_lat, 0, g->sites * 4);
     if ((err = hipMalloc(&g->d_lat, g->sites * 4)) != hipSuccess) return bail(err, "hipMalloc(lattice)");
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < 2; i++)
         if ((err = hipMalloc(&g->d_tab[i], rows * 8)) != hipSuccess) return bail(err, "hipMalloc(tables)");
+    for (int i = 0; i < 2; i++)
         if ((err = hipMalloc(&g->d_del[i], rows * 8)) != hipSuccess) return bail(err, "hipMalloc(deltas)");
-    }
     if ((err = hipMalloc(&g->d_anchor, rows * 3 * 4)) != hipSuccess) return bail(err, "hipMalloc(anchor)");
     if ((err = hipMalloc(&g->d_csum, rows * 3 * 8)) != hipSuccess) return bail(err, "hipMalloc(csum)");
     if ((err = hipMalloc(&g->d_first, rows * 4)) != hipSuccess) return bail(err, "hipMalloc(first)");
@@ -465,23 +466,26 @@ int cpm_active_cells(cpm_handle h, int* out) {
 
 int cpm_push(cpm_handle h) {
     if (int rc = check_handle(h)) return rc;
+    uint32_t tmax = 0;
     for (size_t i = 0; i < h->sites; i++) {
         const uint32_t v = h->h_lat[i];
         if ((v & gcpm::kSpinMask) > (uint32_t)h->n)
             return fail(CPM_EINVAL, "lattice holds a cell number above number_of_cells");
         if ((v >> 24) >= (uint32_t)CPM_NTYPES)
             return fail(CPM_EINVAL, "lattice holds a cell type >= 20");
+        tmax = std::max(tmax, v >> 24);
+    }
+    if ((int)tmax + 1 != h->nt) {
+        h->nt = (int)tmax + 1;  // flips only copy existing values: no new types appear
+        h->params_dirty = true;
     }
     const size_t rows = (size_t)h->n + 1;
     hipStream_t s = h->stream;
     HIP_TRY(hipMemcpyAsync(h->d_lat, h->h_lat, h->sites * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
-    HIP_TRY(hipMemsetAsync(h->d_del[0], 0, rows * 8, s));
-    HIP_TRY(hipMemsetAsync(h->d_del[1], 0, rows * 8, s));
-    HIP_TRY(hipMemsetAsync(h->d_csum, 0, rows * 3 * 8, s));
+    for (int i = 0; i < 2; i++) HIP_TRY(hipMemsetAsync(h->d_del[i], 0, rows * 8, s));
     HIP_TRY(hipMemsetAsync(h->d_first, 0xFF, rows * 4, s));
-    HIP_TRY(gcpm::launch_tally(h->d, h->d_lat, h->lgL, h->n, h->d_tab[0], h->d_first, h->d_anchor,
-                               h->d_csum, s));
+    HIP_TRY(gcpm::launch_tally(h->d, h->d_lat, h->lgL, h->n, h->d_tab[0], h->d_first, h->d_anchor, s));
     HIP_TRY(hipMemcpyAsync(h->d_tab[1], h->d_tab[0], rows * 8, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     h->pushed = true;
@@ -511,12 +515,8 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (int rc = h->upload_params()) return rc;
     gcpm::SweepConsts k{};
     k.lat = h->d_lat;
-    for (int i = 0; i < 2; i++) {
-        k.tab[i] = h->d_tab[i];
-        k.del[i] = h->d_del[i];
-    }
-    k.anchor = h->d_anchor;
-    k.csum = h->d_csum;
+    for (int i = 0; i < 2; i++) k.tab[i] = h->d_tab[i];
+    for (int i = 0; i < 2; i++) k.del[i] = h->d_del[i];
     k.prm = h->d_prm;
     k.thr = h->d_thr;
     k.nthr = h->nthr;
@@ -527,6 +527,7 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     k.rounds = (int)(cd / u);  // kernels2d.cu:95
     k.k0 = (uint32_t)h->seed;
     k.k1 = (uint32_t)(h->seed >> 32);
+    k.nt = h->nt;
     *kout = k;
     *lh_out = h->lgL - k.lc - 1;
     return CPM_OK;
@@ -581,9 +582,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             }
             g &= 1;  // only the parity matters
         }
-        // Fold the last phase's deltas into the lagging buffer: both table
-        // buffers then hold the current state and both delta buffers are zero.
-        HIP_TRY(gcpm::launch_settle(h->d_tab[(g + 1) & 1], h->d_del[(g + 1) & 1], h->n, h->stream));
+        // Fold the last phase's deltas into the lagging base: both bases
+        // then hold the current state and both delta buffers are zero.
+        HIP_TRY(gcpm::launch_settle(h->d_tab, h->d_del, g, h->n, h->stream));
     } else {
         gcpm::PersistArgs pa{};
         pa.k = k;
@@ -631,7 +632,14 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
     std::vector<int64_t> tab(rows);
     HIP_TRY(hipMemcpyAsync(tab.data(), h->d_tab[0], rows * 8, hipMemcpyDeviceToHost, h->stream));
     if (anchor) HIP_TRY(hipMemcpyAsync(anchor, h->d_anchor, rows * 3 * 4, hipMemcpyDeviceToHost, h->stream));
-    if (csum) HIP_TRY(hipMemcpyAsync(csum, h->d_csum, rows * 3 * 8, hipMemcpyDeviceToHost, h->stream));
+    if (csum) {
+        // Centroid sums about the push-time anchors, rebuilt from the current
+        // lattice: equal to sums kept per accepted flip (each site adds
+        // wrap(x - anchor) while it belongs to the cell).
+        HIP_TRY(hipMemsetAsync(h->d_csum, 0, rows * 3 * 8, h->stream));
+        HIP_TRY(gcpm::launch_csum(h->d, h->d_lat, h->lgL, h->d_anchor, h->d_csum, h->stream));
+        HIP_TRY(hipMemcpyAsync(csum, h->d_csum, rows * 3 * 8, hipMemcpyDeviceToHost, h->stream));
+    }
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (size_t c = 0; c < rows; c++) {
         if (area) area[c] = (int32_t)(uint32_t)tab[c];
@@ -735,6 +743,75 @@ int cpm_get_mcs(cpm_handle h, uint64_t* mcs) {
     return CPM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Batched run, one launch per colour phase for all replicas (every replica
+// has the same c and u, hence the same phase sequence).
+int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::ReplicaDesc>& reps,
+                    long total) {
+    const int n = (int)b->sims.size();
+    int nblocks = 0;
+    for (int i = 0; i < n; i++) {
+        const long nsub = 1L << (reps[i].lh * b->d);
+        const long work = std::max(nsub, (long)b->sims[i]->n + 1);
+        reps[i].block0 = nblocks;
+        reps[i].nblocks = (int)((work + gcpm::kBlock - 1) / gcpm::kBlock);
+        reps[i].mcs0 = (uint32_t)b->sims[i]->mcs;
+        nblocks += reps[i].nblocks;
+    }
+    std::vector<int> rep_of(nblocks);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < reps[i].nblocks; j++) rep_of[reps[i].block0 + j] = i;
+    if (nblocks > b->rep_of_cap) {
+        (void)hipFree(b->d_rep_of);
+        b->d_rep_of = nullptr;
+        HIP_TRY(hipMalloc(&b->d_rep_of, sizeof(int) * nblocks));
+        b->rep_of_cap = nblocks;
+    }
+    b->last_blocks = nblocks;
+    for (int i = 0; i < n; i++) {
+        HIP_TRY(hipEventRecord(b->ready[i], b->sims[i]->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->ready[i], 0));
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_reps, reps.data(), sizeof(gcpm::ReplicaDesc) * n, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));  // reps and rep_of are host temporaries
+    gcpm::BatchPhaseArgs pa{};
+    pa.reps = b->d_reps;
+    pa.rep_of = b->d_rep_of;
+    const int ncol = 1 << b->d;
+    const int rounds = reps[0].k.rounds;
+    int g = 0;
+    for (long m = 0; m < total; m++) {
+        pa.m = (uint32_t)m;
+        for (int r = 0; r < rounds; r++) {
+            for (int col = 0; col < ncol; col++, g++) {
+                pa.cx = b->d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
+                pa.cy = b->d == 3 ? (col >> 1) & 1 : col & 1;
+                pa.cz = b->d == 3 ? col & 1 : 0;
+                pa.phase = (uint32_t)(r * ncol + col);
+                pa.g = g;
+                HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
+            }
+        }
+        g &= 1;
+    }
+    for (int i = 0; i < n; i++)
+        HIP_TRY(gcpm::launch_settle(b->sims[i]->d_tab, b->sims[i]->d_del, g, b->sims[i]->n, b->stream));
+    HIP_TRY(hipEventRecord(b->done, b->stream));
+    for (int i = 0; i < n; i++) {
+        HIP_TRY(hipStreamWaitEvent(b->sims[i]->stream, b->done, 0));
+        b->sims[i]->mcs += (uint64_t)total;
+    }
+    return CPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
     if (!out) return fail(CPM_EINVAL, "null output handle");
     *out = nullptr;
@@ -749,10 +826,7 @@ int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
     if (!b) return fail(CPM_EHIP, "host allocation failed");
     b->sims.assign(sims, sims + n);
     b->d = sims[0]->d;
-    if (b->d == 3) {
-        delete b;
-        return fail(CPM_ENOTIMPL, "batched 3D replicas are not enabled (the persistent global form is off in 3D)");
-    }
+
     auto bail = [&](hipError_t err, const char* what) {
         int rc = hip_fail(err, what);
         delete b;
@@ -790,6 +864,8 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     }
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
+    if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations);
+    if (b->d == 3) return fail(CPM_ENOTIMPL, "the persistent batch form is off in 3D");
 
     // Block groups: each replica gets blocks in proportion to its work, at
     // least one, within the co-resident limit of the cooperative grid.
@@ -852,6 +928,13 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         HIP_TRY(hipStreamWaitEvent(b->sims[i]->stream, b->done, 0));
         b->sims[i]->mcs += (uint64_t)(chunks * a->inner_iterations);
     }
+    return CPM_OK;
+}
+
+int cpm_batch_set_form(cpm_batch_handle b, int form) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    if (form != 0 && form != 1) return fail(CPM_EINVAL, "form must be 0 (per phase) or 1 (persistent)");
+    b->form = form;
     return CPM_OK;
 }
 

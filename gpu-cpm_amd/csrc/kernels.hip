@@ -146,6 +146,18 @@ struct LdsLat3 {
     }
 };
 
+// Cell-table buffers of colour phase g (see SweepConsts).
+struct PhaseTables {
+    const int64_t* R;   // tab[g & 1]: the phase-start snapshot
+    int64_t* W;         // tab[(g + 1) & 1]: becomes the state after phase g
+    int64_t* Dc;        // del[g & 1]: deltas of this phase
+    int64_t* Dp;        // del[(g + 1) & 1]: deltas of phase g - 1, folded into W
+};
+
+__device__ __forceinline__ PhaseTables tables_of(const SweepConsts& k, int g) {
+    return {k.tab[g & 1], k.tab[(g + 1) & 1], k.del[g & 1], k.del[(g + 1) & 1]};
+}
+
 // The u copy attempts of sub-block (bx, by, bz) in one colour phase: the
 // copy attempt of kernels2d.cu:197-570 / kernels3d.cu:165-583 in the
 // shared-mode roles (target drawn in the sub-block, source a uniformly drawn
@@ -156,8 +168,7 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
                                                    const DevParams* __restrict__ P,
                                                    const uint32_t* __restrict__ thr, int bx,
                                                    int by, int bz, uint32_t mcs, uint32_t phase,
-                                                   const int64_t* __restrict__ R, int64_t* W,
-                                                   int64_t* Dc) {
+                                                   const PhaseTables& T) {
     constexpr int N = D == 3 ? 26 : 8;
     constexpr int NB = D == 3 ? 27 : 9;
     constexpr int CENTRE = NB / 2;
@@ -196,16 +207,10 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
         // Issue the snapshot-table loads first: their latency overlaps the
         // neighbourhood sums.
-        const int64_t row_s = cs ? ld<CO>(R + cs) : 0;
-        const int64_t row_t = ct ? ld<CO>(R + ct) : 0;
-        int anc_s[D], anc_t[D];  // centroid anchors (read-only during a run)
-#pragma unroll
-        for (int ax = 0; ax < D; ax++) {
-            anc_s[ax] = k.anchor[3 * cs + ax];
-            anc_t[ax] = k.anchor[3 * ct + ax];
-        }
-        const int32_t* __restrict__ Js = P->J[ts];
-        const int32_t* __restrict__ Jt = P->JS[tt];
+        const int64_t row_s = cs ? ld<CO>(T.R + cs) : 0;
+        const int64_t row_t = ct ? ld<CO>(T.R + ct) : 0;
+        const int32_t* __restrict__ Js = P->J + ts * k.nt;
+        const int32_t* __restrict__ Jt = P->JS + tt * k.nt;
         int32_t adh = 0;
         int as = 0, at = 0;
 #pragma unroll
@@ -236,27 +241,17 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         if (!accept) continue;
 
         lat.put(t, vs);  // kernels2d.cu:548
-        const int pos[3] = {tx, ty, tz};
-        const int half = (Lm + 1) / 2;
+        // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums
+        // are rebuilt from the lattice when read.
         if (cs) {
             const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
-            atomic_add_i64(W + cs, delta);
-            atomic_add_i64(Dc + cs, delta);
-#pragma unroll
-            for (int ax = 0; ax < D; ax++) {
-                const int v = pos[ax] - anc_s[ax];
-                atomic_add_i64(k.csum + 3 * cs + ax, (int64_t)(((v + half) & Lm) - half));
-            }
+            atomic_add_i64(T.W + cs, delta);
+            atomic_add_i64(T.Dc + cs, delta);
         }
         if (ct) {
             const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
-            atomic_add_i64(W + ct, delta);
-            atomic_add_i64(Dc + ct, delta);
-#pragma unroll
-            for (int ax = 0; ax < D; ax++) {
-                const int v = pos[ax] - anc_t[ax];
-                atomic_add_i64(k.csum + 3 * ct + ax, -(int64_t)(((v + half) & Lm) - half));
-            }
+            atomic_add_i64(T.W + ct, delta);
+            atomic_add_i64(T.Dc + ct, delta);
         }
     }
 }
@@ -269,45 +264,59 @@ struct LdsParams {
 };
 
 __device__ __forceinline__ const uint32_t* stage_params(LdsParams& sp, const SweepConsts& k) {
-    // All global loads are issued before any LDS store, so staging costs
-    // one memory round trip.
-    constexpr int NP = (int)(sizeof(DevParams) / 4);
-    constexpr int RP = (NP + kBlock - 1) / kBlock;
-    constexpr int RT = kThrLds / kBlock;
+    // Only the nt types present are staged: 2 nt^2 pair entries and 4 nt
+    // per-type entries.  All global loads are issued before any LDS store.
+    const int nt = k.nt, n2 = nt * nt;
+    const int npar = 2 * n2 + 4 * nt;
+    const int i = threadIdx.x;
+    const bool fits = k.nthr <= kThrLds;
+    // word i of the compact list -> offset in DevParams
+    auto src_of = [&](int w) -> int {
+        if (w < n2) return w;                                   // J
+        if (w < 2 * n2) return kNTypes * kNTypes + (w - n2);    // JS
+        const int r = w - 2 * n2, f = r / nt, t = r - f * nt;   // la, a0, lp, p0
+        return 2 * kNTypes * kNTypes + f * kNTypes + t;
+    };
     const int* src = reinterpret_cast<const int*>(k.prm);
     int* dst = reinterpret_cast<int*>(&sp.p);
-    const bool fits = k.nthr <= kThrLds;
+    constexpr int RP = (2 * kNTypes * kNTypes + 4 * kNTypes + kBlock - 1) / kBlock;
+    constexpr int RT = kThrLds / kBlock;
     int pv[RP];
     uint32_t tv[RT];
 #pragma unroll
     for (int j = 0; j < RP; j++) {
-        const int i = threadIdx.x + j * kBlock;
-        pv[j] = i < NP ? src[i] : 0;
+        const int w = i + j * kBlock;
+        pv[j] = w < npar ? src[src_of(w)] : 0;
     }
 #pragma unroll
     for (int j = 0; j < RT; j++) {
-        const int i = threadIdx.x + j * kBlock;
-        tv[j] = fits && i < k.nthr ? k.thr[i] : 0u;
+        const int w = i + j * kBlock;
+        tv[j] = fits && w < k.nthr ? k.thr[w] : 0u;
     }
 #pragma unroll
     for (int j = 0; j < RP; j++) {
-        const int i = threadIdx.x + j * kBlock;
-        if (i < NP) dst[i] = pv[j];
+        const int w = i + j * kBlock;
+        if (w < npar) dst[src_of(w)] = pv[j];
     }
 #pragma unroll
-    for (int j = 0; j < RT; j++) sp.thr[threadIdx.x + j * kBlock] = tv[j];
+    for (int j = 0; j < RT; j++) {
+        const int w = i + j * kBlock;
+        if (fits && w < k.nthr) sp.thr[w] = tv[j];
+    }
     __syncthreads();
     return fits ? sp.thr : k.thr;
 }
 
-// Fold the previous phase's deltas into W (W is never read in a phase).
+// Fold of phase g: add the previous phase's deltas to W and clear them.  W
+// is never read in a phase and the fold commutes with the flips' own adds,
+// so it may run at any time during the phase.
 template <bool CO>
-__device__ __forceinline__ void fold_deltas(int64_t* W, int64_t* Dp, int ncells, int gtid, int gsz) {
+__device__ __forceinline__ void fold_tables(const PhaseTables& T, int ncells, int gtid, int gsz) {
     for (int ci = gtid; ci <= ncells; ci += gsz) {
-        const int64_t dv = ld<CO>(Dp + ci);
+        const int64_t dv = ld<CO>(T.Dp + ci);
         if (dv != 0) {
-            atomic_add_i64(W + ci, dv);
-            st<CO>(Dp + ci, (int64_t)0);
+            atomic_add_i64(T.W + ci, dv);
+            st<CO>(T.Dp + ci, (int64_t)0);
         }
     }
 }
@@ -330,18 +339,40 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
     const SweepConsts& k = a.k;
     __shared__ LdsParams sp;
     const uint32_t* thr = stage_params(sp, k);
-    int64_t* W = k.tab[(a.g + 1) & 1];
+    const PhaseTables T = tables_of(k, a.g);
     if (gtid < a.nsub) {
         const int hm = (1 << a.lh) - 1;
         const int bx = 2 * (gtid & hm) + a.cx;
         const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
         const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, k.tab[a.g & 1],
-                                     W, k.del[a.g & 1]);
+        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, T);
     }
-    // W is not read in this phase, so the fold may come after the attempts.
-    fold_deltas<false>(W, k.del[(a.g + 1) & 1], k.ncells, gtid, gsz);
+    fold_tables<false>(T, k.ncells, gtid, gsz);
+}
+
+// Batched replicas, one colour phase: block b works for replica rep_of[b]
+// on its same-colour sub-block (b - block0) * blockDim + t.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
+    const int r = a.rep_of[blockIdx.x];
+    const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
+    const SweepConsts& k = rd.k;
+    __shared__ LdsParams sp;
+    const uint32_t* thr = stage_params(sp, k);
+    const int gtid = (blockIdx.x - rd.block0) * blockDim.x + threadIdx.x;
+    const int gsz = rd.nblocks * blockDim.x;
+    const PhaseTables T = tables_of(k, a.g);
+    const int lh = rd.lh;
+    if (gtid < (1 << (D * lh))) {
+        const int hm = (1 << lh) - 1;
+        const int bx = 2 * (gtid & hm) + a.cx;
+        const int by = 2 * ((gtid >> lh) & hm) + a.cy;
+        const int bz = D == 3 ? 2 * (gtid >> (2 * lh)) + a.cz : 0;
+        const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T);
+    }
+    fold_tables<false>(T, k.ncells, gtid, gsz);
 }
 
 // One 3D colour phase with positions_per_checkerboard = 1 (every sub-block
@@ -382,12 +413,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase3_box(PhaseArgs a) {
     const int t = threadIdx.x;
     const int qx = t % (kBX / 2), qy = (t / (kBX / 2)) % (kBY / 2), qz = t / ((kBX / 2) * (kBY / 2));
     const LdsLat3 lat{box, k.lat, SX, SXY, x0, y0, z0, lgL};
-    int64_t* W = k.tab[(a.g + 1) & 1];
+    const PhaseTables T = tables_of(k, a.g);
     sub_block_attempts<3, false>(lat, k, &sp.p, thr, x0 + 2 * qx + a.cx, y0 + 2 * qy + a.cy,
-                                 z0 + 2 * qz + a.cz, a.mcs, a.phase, k.tab[a.g & 1], W,
-                                 k.del[a.g & 1]);
-    fold_deltas<false>(W, k.del[(a.g + 1) & 1], k.ncells, b * blockDim.x + threadIdx.x,
-                       gridDim.x * blockDim.x);
+                                 z0 + 2 * qz + a.cz, a.mcs, a.phase, T);
+    fold_tables<false>(T, k.ncells, b * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 static_assert((kBX / 2) * (kBY / 2) * (kBZ / 2) == kBlock, "one active site per thread");
 
@@ -428,9 +457,11 @@ __device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& ge
 
 // End of a persistent run: fold the last phase's deltas into the lagging
 // table buffer (after a barrier), leaving both buffers equal.
-__device__ __forceinline__ void settle_in_kernel(const SweepConsts& k, int g, int gtid, int gsz) {
-    int64_t* T = k.tab[(g + 1) & 1];
-    int64_t* Dl = k.del[(g + 1) & 1];
+__device__ __forceinline__ void settle_in_kernel(const SweepConsts& k, int G, int gtid, int gsz) {
+    // After G phases tab[G & 1] is current and tab[(G + 1) & 1] lags by the
+    // last phase's deltas del[(G + 1) & 1].
+    int64_t* T = k.tab[(G + 1) & 1];
+    int64_t* Dl = k.del[(G + 1) & 1];
     for (int ci = gtid; ci <= k.ncells; ci += gsz) {
         st<true>(T + ci, ld<true>(T + ci) + ld<true>(Dl + ci));
         st<true>(Dl + ci, (int64_t)0);
@@ -460,16 +491,14 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
                 if (g) grid_barrier(bar, gen, nblocks);
                 int cx, cy, cz;
                 colour_of<D>(col, cx, cy, cz);
-                int64_t* W = k.tab[(g + 1) & 1];
-                fold_deltas<true>(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
-                const int64_t* R = k.tab[g & 1];
-                int64_t* Dc = k.del[g & 1];
+                const PhaseTables T = tables_of(k, g);
+                fold_tables<true>(T, k.ncells, gtid, gsz);
                 const uint32_t phase = (uint32_t)(r * NCOL + col);
                 for (int q = gtid; q < nsub; q += gsz) {
                     const int bx = 2 * (q & hm) + cx;
                     const int by = 2 * ((q >> lh) & hm) + cy;
                     const int bz = D == 3 ? 2 * (q >> (2 * lh)) + cz : 0;
-                    sub_block_attempts<D, true>(lat, k, &sp.p, thr, bx, by, bz, mcs, phase, R, W, Dc);
+                    sub_block_attempts<D, true>(lat, k, &sp.p, thr, bx, by, bz, mcs, phase, T);
                 }
             }
         }
@@ -546,15 +575,13 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
                 }
                 int cx, cy, cz;
                 colour_of<2>(col, cx, cy, cz);
-                int64_t* W = k.tab[(g + 1) & 1];
-                fold_deltas<true>(W, k.del[(g + 1) & 1], k.ncells, gtid, gsz);
-                const int64_t* R = k.tab[g & 1];
-                int64_t* Dc = k.del[g & 1];
+                const PhaseTables T = tables_of(k, g);
+                fold_tables<true>(T, k.ncells, gtid, gsz);
                 const uint32_t phase = (uint32_t)(r * 4 + col);
                 for (int q = threadIdx.x; q < nq; q += blockDim.x) {
                     const int bx = (x0 >> k.lc) + 2 * (q & ((1 << lq) - 1)) + cx;
                     const int by = (y0 >> k.lc) + 2 * (q >> lq) + cy;
-                    sub_block_attempts<2, true>(lat, k, &sp.p, thr, bx, by, 0, mcs, phase, R, W, Dc);
+                    sub_block_attempts<2, true>(lat, k, &sp.p, thr, bx, by, 0, mcs, phase, T);
                 }
             }
         }
@@ -616,6 +643,8 @@ __global__ __launch_bounds__(256) void k_tally_csum(const uint32_t* __restrict__
     }
 }
 
+// Settle after G phases: fold the last phase's deltas into the lagging
+// base, leaving both bases current and both delta buffers zero.
 __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > ncells) return;
@@ -741,6 +770,14 @@ hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s) {
     return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(kBlock), args, 0, s);
 }
 
+hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
+    if (d == 2)
+        hipLaunchKernelGGL(k_batch_phase<2>, dim3(blocks), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_batch_phase<3>, dim3(blocks), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
 static const void* persist_kernel(int d, bool lds2d) {
     if (lds2d) return reinterpret_cast<const void*>(&k_persist_lds2d);
     return d == 2 ? reinterpret_cast<const void*>(&k_persist_global<2>)
@@ -769,7 +806,7 @@ hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, s
 }
 
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,
-                        uint32_t* first, int32_t* anchor, int64_t* csum, hipStream_t s) {
+                        uint32_t* first, int32_t* anchor, hipStream_t s) {
     const uint32_t nsites = 1u << (d * lgL);
     const int blocks = (int)((nsites + 255) / 256);
     if (d == 2)
@@ -777,15 +814,24 @@ hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t
     else
         hipLaunchKernelGGL(k_tally_area_perim<3>, dim3(blocks), dim3(256), 0, s, lat, lgL, tab, first);
     hipLaunchKernelGGL(k_anchor, dim3((ncells + 1 + 255) / 256), dim3(256), 0, s, first, anchor, ncells, lgL, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_settle(int64_t* const tab[2], int64_t* const del[2], int G, int ncells,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_settle, dim3((ncells + 1 + 255) / 256), dim3(256), 0, s, tab[(G + 1) & 1],
+                       del[(G + 1) & 1], ncells);
+    return hipGetLastError();
+}
+
+hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* anchor, int64_t* csum,
+                       hipStream_t s) {
+    const uint32_t nsites = 1u << (d * lgL);
+    const int blocks = (int)((nsites + 255) / 256);
     if (d == 2)
         hipLaunchKernelGGL(k_tally_csum<2>, dim3(blocks), dim3(256), 0, s, lat, lgL, anchor, csum);
     else
         hipLaunchKernelGGL(k_tally_csum<3>, dim3(blocks), dim3(256), 0, s, lat, lgL, anchor, csum);
-    return hipGetLastError();
-}
-
-hipError_t launch_settle(int64_t* tab, int64_t* del, int ncells, hipStream_t s) {
-    hipLaunchKernelGGL(k_settle, dim3((ncells + 1 + 255) / 256), dim3(256), 0, s, tab, del, ncells);
     return hipGetLastError();
 }
 

@@ -312,9 +312,9 @@ class Cpm:
 
 class Batch:
     """Batched replicas: several pushed Cpm engines (same dimensionality; any
-    sizes, parameters, temperatures and seeds) advanced together in one
-    cooperative launch per inner_iterations chunk.  Each replica ends
-    bit-identical to calling its own ``run`` with the same arguments."""
+    sizes, parameters, temperatures and seeds) advanced together, by default
+    with one launch per colour phase covering every replica.  Each replica
+    ends bit-identical to calling its own ``run`` with the same arguments."""
 
     def __init__(self, sims):
         self.sims = list(sims)
@@ -340,6 +340,11 @@ class Batch:
 
     def synchronize(self):
         check(lib.cpm_batch_sync(self._h))
+
+    def set_form(self, form):
+        """0: one launch per colour phase for all replicas (default);
+        1: one persistent cooperative launch per chunk (2D)."""
+        check(lib.cpm_batch_set_form(self._h, _int(form, "form")))
 
     def set_blocks(self, blocks_per_work):
         """Blocks per unit of work in each replica's group (default 1.0)."""

@@ -70,6 +70,7 @@ SIGNATURES = [
     ("cpm_energy_stats", _I, [_H, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("cpm_set_launch", _I, [_H, _I, _I]),
     ("cpm_batch_set_blocks", _I, [_H, ctypes.c_double]),
+    ("cpm_batch_set_form", _I, [_H, _I]),
 ]
 
 if not os.path.exists(LIBPATH):

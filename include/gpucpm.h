@@ -158,18 +158,22 @@ int cpm_set_launch(cpm_handle h, int form, int tile);
  * global __constant__ block, kernels.cu:48-53): batched replicas.  A batch
  * is a fixed list of pushed engines of one dimensionality, each with its own
  * lattice size, parameters, temperature, seed and MCS counter.
- * cpm_batch_run applies one set of run arguments to all of them in one
- * cooperative launch per inner_iterations chunk: each replica's blocks form
- * a group with its own barrier.  Every replica ends bit-identical to running
- * cpm_run on it alone.  The engines must outlive the batch; their streams
- * are ordered after the batch launch. */
+ * cpm_batch_run applies one set of run arguments to all of them: by
+ * default with one launch per colour phase covering every replica (form 0),
+ * or with one cooperative launch per inner_iterations chunk in which each
+ * replica's blocks form a group with its own barrier (form 1, 2D only).
+ * Every replica ends bit-identical to running cpm_run on it alone.  The
+ * engines must outlive the batch; their streams are ordered after the
+ * batch's work. */
 int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out);
 int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* args);
 int cpm_batch_sync(cpm_batch_handle b);
-/* Blocks per unit of work (one block-width of sub-blocks) given to each
- * replica's group, default 1; the autotuner sets it.  Results never depend
- * on it. */
+/* Persistent form: blocks per unit of work (one block-width of
+ * sub-blocks) given to each replica's group, default 1; the autotuner sets
+ * it.  Results never depend on it. */
 int cpm_batch_set_blocks(cpm_batch_handle b, double blocks_per_work);
+/* Batch launch form: 0 = one launch per colour phase, 1 = persistent. */
+int cpm_batch_set_form(cpm_batch_handle b, int form);
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks);
 int cpm_batch_destroy(cpm_batch_handle b);
 

