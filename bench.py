@@ -34,12 +34,17 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mcs-per-step", type=int, default=200)
     ap.add_argument("--workload", default="sorting2d",
-                    choices=["sorting2d", "plain3d", "plain2d", "batched"])
+                    choices=["sorting2d", "plain3d", "plain2d", "batched", "gradient"])
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--checkerboard", type=int, default=1, help="positions_per_checkerboard c")
     ap.add_argument("--updates", type=int, default=1, help="updates_per_checkerboard_switch u")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="skip the dH/dtheta check")
+    ap.add_argument("--pmc-summary", nargs=6, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
+                                                       "COPY_WRITE_DIR", "KERNEL", "OUT_JSON"),
+                    help="summarize rocprofv3 FETCH_SIZE / WRITE_SIZE runs of KERNEL, scaled by the "
+                         "same counters on a --copy-calibration run, into OUT_JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (hipMemcpy), the known-size "
                          "reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -151,6 +156,157 @@ def bench_batched(a, log):
     print(json.dumps(out), flush=True)
 
 
+def _oracle_energy_fn(lats, wls, layout):
+    """FP64 H(theta) averaged over lattices, from the oracle's own reduction."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as O
+    stats = []
+    for lat, wl in zip(lats, wls):
+        st = O.energy_stats(lat, 2, wl.ncells, O.Params.make(1.0))  # targets 0: raw moments
+        spin, typ = lat & 0xFFFFFF, lat >> 24
+        ncell = [len(np.unique(spin[(typ == t) & (spin > 0)])) for t in range(layout.nt)]
+        stats.append((st, ncell))
+
+    def H(th):
+        nt, nj = layout.nt, layout.nj
+        J = np.zeros((20, 20))
+        for k, (a, b) in enumerate(layout.pairs):
+            J[a, b] = th[k]
+        tot = 0.0
+        for st, ncell in stats:
+            tot += float(np.sum(J * st["pairs"]))
+            for t in range(nt):
+                la, a0 = th[nj + t], th[nj + nt + t]
+                lp, p0 = th[nj + 2 * nt + t], th[nj + 3 * nt + t]
+                tot += la * (st["sa2"][t] - 2 * a0 * st["sa1"][t] + ncell[t] * a0 * a0)
+                tot += lp * (st["sp2"][t] - 2 * p0 * st["sp1"][t] + ncell[t] * p0 * p0)
+        return tot / len(stats)
+    return H
+
+
+def gradient_check(nrep, L, mcs, log, J_AB=11, seed0=0, contrast=False):
+    """Replicas of the sorting lattice (C2 parameters) run forward mcs MCS as
+    one batch; then the backward statistics kernel gives dH/dtheta (FP32)
+    for J, lambda_A, A0, lambda_P, P0, checked against central differences
+    (step 1e-3) of the oracle's FP64 Hamiltonian on the same lattices."""
+    import numpy as np
+    import torch
+
+    import gpucpm
+    from gpucpm import workloads
+    from gpucpm.autograd import ThetaLayout, contrastive_gradient, energy
+
+    grid = (L - 112) // 10
+    layout = ThetaLayout(3)
+
+    def make(J, seed):
+        wls = [workloads.sorting2d(L=L, grid=grid, edge=10, seed=seed + i, J_AB=J) for i in range(nrep)]
+        sims = []
+        for i, wl in enumerate(wls):
+            s = wl.make_cpm()
+            s.push_to_gpu()
+            s.set_seed(7000 + seed + i)
+            sims.append(s)
+        return wls, sims
+
+    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4, positions_per_thread=2,
+              positions_per_checkerboard=1, updates_per_checkerboard_switch=1, updates_per_barrier=1,
+              iterations=mcs, inner_iterations=mcs)
+    wls, sims = make(J_AB, seed0)
+    log(f"gradient: {nrep} x {L}^2 forward {mcs} MCS")
+    t0 = time.perf_counter()
+    gpucpm.Batch(sims).run(**kw)
+    for s in sims:
+        s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    p = wls[0].params()
+    theta = layout.pack(p["J"], p["lam_area"], p["target_area"], p["lam_perim"], p["target_perim"],
+                        dtype=torch.float32).requires_grad_(True)
+    t0 = time.perf_counter()
+    H = energy(theta, sims, layout)
+    H.backward()
+    t_bwd = time.perf_counter() - t0
+    g = theta.grad.double().numpy()
+    lats = []
+    for s in sims:
+        s.pull_from_gpu()
+        lats.append(s.get_state().view(np.uint32).copy())
+    H64 = _oracle_energy_fn(lats, wls, layout)
+    th = theta.detach().double().numpy()
+    h = 1e-3
+    fd = np.array([(H64(th + h * e) - H64(th - h * e)) / (2 * h) for e in np.eye(th.size)])
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1.0)
+    out = {"gradient_rel_error": float(rel.max()),
+           "energy_rel_error": abs(float(H.detach()) - H64(th)) / abs(H64(th)),
+           "replicas": nrep, "L": L, "mcs": mcs, "forward_s": t_fwd,
+           "backward_s": t_bwd, "backward_sites_per_s": nrep * L * L / t_bwd}
+    if contrast:
+        log(f"contrastive: data sampled at J_AB = 8")
+        _, data = make(8, seed0 + 1000)
+        gpucpm.Batch(data).run(**kw)
+        for s in data:
+            s.synchronize()
+        gc = contrastive_gradient(theta, data, sims, layout).numpy()
+        k = layout.pairs.index((1, 2))
+        out["contrastive_grad_J_AB"] = float(gc[k])
+        out["contrastive_step_moves_J_AB_toward_8"] = bool(gc[k] > 0)  # theta -= lr * grad lowers J_AB
+    return out
+
+
+def bench_gradient(a, log):
+    out = gradient_check(32, 512, a.steps * a.mcs_per_step, log, contrast=True)
+    line = {
+        "metric": "gradient rel. error (dH/dtheta FP32 vs FP64 central differences)",
+        "value": out["gradient_rel_error"], "unit": "relative", "n_gpus": 1,
+        "steps": a.steps, "warmup": 0, "ms_per_step": out["forward_s"] / a.steps * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32 sweep, int64 statistics, fp32 gradient", "data": "synthetic",
+        "config": {"workload": "32 replicas of 512^2 two-type sorting (C2 parameters), forward then "
+                               "dH/dtheta for J (3x3), lambda_A, A0, lambda_P, P0; FD step 1e-3; "
+                               "contrastive step toward J_AB = 8"},
+        "roofline": {"bound": "hbm", "kernel": "k_stats_pairs + k_stats_cells",
+                     "achieved": out["backward_sites_per_s"] * 4 / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": out["backward_sites_per_s"] * 4 / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "note": "backward time includes the host round trip per replica"},
+        "cpu_baseline": None,
+    }
+    line.update({k: v for k, v in out.items() if k != "gradient_rel_error"})
+    print(json.dumps(line), flush=True)
+
+
+def _pmc_mean(d, kernel):
+    import csv
+    import glob
+    vals = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if kernel in r["Kernel_Name"]:
+                vals.append(float(r["Counter_Value"]))
+    if not vals:
+        raise SystemExit(f"no {kernel} dispatches in {d}")
+    return sum(vals) / len(vals), len(vals)
+
+
+def pmc_summary(fetch_dir, write_dir, cfetch_dir, cwrite_dir, kernel, out):
+    """HBM bytes per dispatch of KERNEL from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled so that the runtime's copy kernel of a known size
+    reads and writes exactly that size."""
+    f, nf = _pmc_mean(fetch_dir, kernel)
+    w, nw = _pmc_mean(write_dir, kernel)
+    cf, _ = _pmc_mean(cfetch_dir, "copyBuffer")
+    cw, _ = _pmc_mean(cwrite_dir, "copyBuffer")
+    true_kib = 256 * 1024  # --copy-calibration 256
+    kf, kw = true_kib / cf, true_kib / cw
+    res = {"kernel": kernel, "fetch_kib": f, "write_kib": w, "dispatches": [nf, nw],
+           "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_scale": kf, "write_scale": kw,
+           "bytes_per_dispatch": (f * kf + w * kw) * 1024}
+    with open(out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res))
+
+
 def copy_calibration(mib):
     """Three device-to-device copies of a known size through the HIP runtime
     (its copy kernel reads and writes exactly mib MiB per copy)."""
@@ -173,6 +329,8 @@ def main():
     a = parse()
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration)
+    if a.pmc_summary:
+        return pmc_summary(*a.pmc_summary)
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(replicas shard across GPUs without exchange; see the sharding design).",
@@ -181,6 +339,8 @@ def main():
     log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
     if a.workload == "batched":
         return bench_batched(a, log)
+    if a.workload == "gradient":
+        return bench_gradient(a, log)
     import gpucpm
     from gpucpm import workloads
 
@@ -221,6 +381,10 @@ def main():
     avg_launch_s = kernel_ms / 1e3 / launches
     bytes_per_launch = updates / launches * BYTES_PER_UPDATE
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{launch['kernel']}_{a.workload}.json")
+    if os.path.exists(tfile) and c == 1 and u == 1:
+        traffic = json.load(open(tfile))["bytes_per_dispatch"]
     workload_desc = {
         "sorting2d": "2D cell sorting 1024^2, 3969 cells (10x10 squares), two types, 10000-MCS config",
         "plain3d": "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood",
@@ -244,7 +408,7 @@ def main():
                    "updates_per_checkerboard_switch": u, "seed": 20240601},
         "mcs_per_s": mcs / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": launch["kernel"], "avg_launch_us": avg_launch_s * 1e6,
                      "bytes_per_launch": bytes_per_launch, "launches": launches,
                      "grid_blocks": launch["blocks"], "lds_tile": launch["tile"]},
@@ -253,6 +417,10 @@ def main():
     if not a.no_cpu:
         log(f"CPU oracle sample (~{a.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(wl, c, u, a.cpu_seconds)
+    if a.workload == "sorting2d" and not a.no_grad:
+        g = gradient_check(4, 512, 200, log)
+        out["gradient_rel_error"] = g["gradient_rel_error"]
+        out["gradient_check"] = {k: g[k] for k in ("replicas", "L", "mcs", "energy_rel_error")}
     print(json.dumps(out), flush=True)
 
 

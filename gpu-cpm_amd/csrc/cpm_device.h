@@ -64,6 +64,9 @@ struct PhaseArgs {
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int nsub;               // active sub-blocks in this phase
     int box3;               // 3D, c = 1: stage each block's box in LDS
+    int sync_pass;          // cell_sync: 0 off, 1 claim pass, 2 commit pass
+    uint64_t* claims;       // cell_sync: per-cell claim keys
+    uint32_t sync_tag;      // cell_sync: phase tag (grows across runs)
 };
 
 // A whole run in one cooperative launch, phases separated by grid barriers.

@@ -91,6 +91,8 @@ struct cpm_engine {
     gcpm::DevParams* d_prm = nullptr;
     unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation}
     unsigned long long* d_stats = nullptr;  // 400 pair counts + 100 moments
+    uint64_t* d_claims = nullptr;           // cell_sync claim keys [n+1]
+    uint32_t sync_tag = 0;                  // cell_sync phase tag
     int32_t* d_ctype = nullptr;             // [n+1] cell types (stats scratch)
     uint32_t* d_thr = nullptr;
     int thr_cap = 0;
@@ -134,6 +136,7 @@ struct cpm_engine {
         (void)hipFree(d_prm);
         (void)hipFree(d_bar);
         (void)hipFree(d_stats);
+        (void)hipFree(d_claims);
         (void)hipFree(d_ctype);
         (void)hipFree(d_thr);
         if (h_lat) (void)hipHostFree(h_lat);
@@ -500,7 +503,8 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (int rc = check_handle(h)) return rc;
     if (!a) return fail(CPM_EINVAL, "null run arguments");
     if (!h->pushed) return fail(CPM_ESTATE, "run before push_to_gpu");
-    if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync (writeAreas lock) is not implemented");
+    if (a->cell_sync && a->updates_per_checkerboard_switch != 1)
+        return fail(CPM_ENOTIMPL, "cell_sync is implemented for updates_per_checkerboard_switch = 1");
     if (a->threads_per_block < 1 || a->positions_per_thread < 1 || a->updates_per_barrier < 1)
         return fail(CPM_EINVAL, "threads_per_block, positions_per_thread and updates_per_barrier must be >= 1");
     const int c = a->positions_per_checkerboard;
@@ -547,7 +551,15 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
     if (total == 0) return CPM_OK;
 
     Plan plan;
-    if (int rc = h->plan(c, &plan)) return rc;
+    if (a->cell_sync) {
+        plan.mode = Plan::kPhase;  // two launches per phase: claim, commit
+        if (!h->d_claims) {
+            HIP_TRY(hipMalloc(&h->d_claims, sizeof(uint64_t) * ((size_t)h->n + 1)));
+            HIP_TRY(hipMemsetAsync(h->d_claims, 0, sizeof(uint64_t) * ((size_t)h->n + 1), h->stream));
+        }
+    } else if (int rc = h->plan(c, &plan)) {
+        return rc;
+    }
     h->last_plan = plan;
     const int64_t updates_per_mcs = (int64_t)k.rounds * (1 << h->d) * ((int64_t)1 << (lh * h->d)) * u;
 
@@ -571,6 +583,15 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                     pa.cz = h->d == 3 ? col & 1 : 0;
                     pa.phase = (uint32_t)(r * ncol + col);
                     pa.g = g;
+                    if (a->cell_sync) {
+                        // the tag only grows: claims never need clearing
+                        if (++h->sync_tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
+                        pa.sync_tag = h->sync_tag;
+                        pa.claims = h->d_claims;
+                        pa.sync_pass = 1;
+                        HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, nullptr, nullptr));
+                        pa.sync_pass = 2;
+                    }
                     hipEvent_t e0, e1;
                     if (int rc = h->next_events(&e0, &e1)) return rc;
                     HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1));
@@ -864,6 +885,7 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     }
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
+    if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync is not available for batched runs");
     if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations);
     if (b->d == 3) return fail(CPM_ENOTIMPL, "the persistent batch form is off in 3D");
 

@@ -163,12 +163,24 @@ __device__ __forceinline__ PhaseTables tables_of(const SweepConsts& k, int g) {
 // shared-mode roles (target drawn in the sub-block, source a uniformly drawn
 // neighbour), with the deterministic RNG, snapshot tables and integer
 // acceptance described in SweepConsts.
+// cell_sync (kernels.cu:123-141 restated deterministically, u = 1): in the
+// claim pass every accepted proposal raises claims[c] to
+// (tag << 32 | ~sb) for its non-medium cells, so the lowest sub-block of the
+// phase wins; the commit pass recomputes the attempt and commits only if it
+// holds both claims.
+struct CellSync {
+    int pass;  // 0 off, 1 claim, 2 commit
+    uint64_t* claims;
+    uint32_t tag;
+};
+
 template <int D, bool CO, class Lat>
 __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k,
                                                    const DevParams* __restrict__ P,
                                                    const uint32_t* __restrict__ thr, int bx,
                                                    int by, int bz, uint32_t mcs, uint32_t phase,
-                                                   const PhaseTables& T) {
+                                                   const PhaseTables& T,
+                                                   const CellSync cy = CellSync{0, nullptr, 0}) {
     constexpr int N = D == 3 ? 26 : 8;
     constexpr int NB = D == 3 ? 27 : 9;
     constexpr int CENTRE = NB / 2;
@@ -239,6 +251,15 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         }
         const bool accept = dh <= 0 || (dh < k.nthr && w[2] < thr[dh]);
         if (!accept) continue;
+        if (cy.pass) {
+            const uint64_t key = ((uint64_t)cy.tag << 32) | (uint64_t)(0xFFFFFFFFu - sb);
+            if (cy.pass == 1) {
+                if (cs) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + cs), key);
+                if (ct) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + ct), key);
+                continue;
+            }
+            if ((cs && cy.claims[cs] != key) || (ct && cy.claims[ct] != key)) continue;
+        }
 
         lat.put(t, vs);  // kernels2d.cu:548
         // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums
@@ -346,9 +367,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
         const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
         const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, T);
+        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, T,
+                                     CellSync{a.sync_pass, a.claims, a.sync_tag});
     }
-    fold_tables<false>(T, k.ncells, gtid, gsz);
+    if (a.sync_pass != 1) fold_tables<false>(T, k.ncells, gtid, gsz);
 }
 
 // Batched replicas, one colour phase: block b works for replica rep_of[b]
@@ -734,7 +756,7 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
     // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
     // (null events: plain launch).
     const int L = 1 << a.k.lgL;
-    if (d == 3 && a.k.lc == 0 && L >= kBX && L >= kBY && L >= kBZ && a.box3) {
+    if (d == 3 && a.k.lc == 0 && L >= kBX && L >= kBY && L >= kBZ && a.box3 && !a.sync_pass) {
         const int nbox = (L / kBX) * (L / kBY) * (L / kBZ);
         if ((long)nbox * kBlock >= a.k.ncells + 1) {
             hipExtLaunchKernelGGL(k_sweep_phase3_box, dim3(nbox), dim3(kBlock), 0, s, e0, e1, 0, a);

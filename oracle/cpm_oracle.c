@@ -244,10 +244,25 @@ int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,
     return delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, lat[s], lat[t], &as, &at);
 }
 
+static void claim_min(uint32_t* owner, uint32_t c, uint32_t sb) {
+    uint32_t cur = __atomic_load_n(&owner[c], __ATOMIC_RELAXED);
+    while (sb < cur &&
+           !__atomic_compare_exchange_n(&owner[c], &cur, sb, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+
 int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
               const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
               uint64_t seed, uint32_t mcs0, int nmcs, int nthreads) {
+    return orc_sweep_sync(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
+                          nthreads, 0);
+}
+
+int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                   const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                   uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync) {
     if (d != 2 && d != 3) return -1;
+    if (cell_sync && u != 1) return -7;
     if (L < 2 || (L & (L - 1))) return -2;
     if (c < 1 || (c & (c - 1)) || L % (2 * c)) return -3;
     int cd = d == 3 ? c * c * c : c * c;
@@ -277,6 +292,7 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
 
     int32_t* areaR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
     int32_t* perimR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
+    uint32_t* owner = cell_sync ? (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ncells + 1)) : NULL;
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
 #else
@@ -294,6 +310,13 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
                 uint32_t phase = (uint32_t)(r * ncol + col);
                 memcpy(areaR, area, sizeof(int32_t) * (size_t)(ncells + 1));
                 memcpy(perimR, perim, sizeof(int32_t) * (size_t)(ncells + 1));
+                if (cell_sync) memset(owner, 0xFF, sizeof(uint32_t) * (size_t)(ncells + 1));
+                /* cell_sync (kernels.cu:123-141, as a deterministic rule): pass 0
+                 * claims the cells of every accepted proposal for the lowest
+                 * sub-block index; pass 1 recomputes each attempt (nothing has
+                 * been written) and commits it only if its sub-block holds the
+                 * claim on both of its non-medium cells. */
+                for (int pass = cell_sync ? 0 : 1; pass < 2; pass++)
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
@@ -322,8 +345,14 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
                                                   tz, vs, vt, &as, &at);
                         int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
                         if (!accept) continue;
-                        lat[ti] = vs; /* kernels2d.cu:548 */
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
+                        if (cell_sync && pass == 0) {
+                            if (cs) claim_min(owner, cs, sb);
+                            if (ct) claim_min(owner, ct, sb);
+                            continue;
+                        }
+                        if (cell_sync && ((cs && owner[cs] != sb) || (ct && owner[ct] != sb))) continue;
+                        lat[ti] = vs; /* kernels2d.cu:548 */
                         int pos[3] = {tx, ty, tz};
                         if (cs) { /* kernels2d.cu:552-560 */
 #ifdef _OPENMP
@@ -366,6 +395,7 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
     }
     free(areaR);
     free(perimR);
+    free(owner);
     free(thr);
     return 0;
 }

@@ -75,6 +75,15 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
               const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
               uint64_t seed, uint32_t mcs0, int nmcs, int nthreads);
 
+/* orc_sweep with cell_sync (kernels.cu:123-141 restated deterministically,
+ * u = 1 only): in each colour phase every accepted proposal claims its
+ * non-medium cells for the lowest sub-block index, and a flip commits only
+ * if its sub-block holds both claims (at most one flip per cell per phase).
+ * Returns -7 for u != 1. */
+int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                   const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                   uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync);
+
 /* Serial random-sequential CPM with live tables (statistical comparisons
  * only; not bit-comparable with orc_sweep). */
 int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,

@@ -91,6 +91,8 @@ def lib():
                                 P(Params), ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                 ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.orc_sweep.restype = ctypes.c_int
+        L.orc_sweep_sync.argtypes = L.orc_sweep.argtypes + [ctypes.c_int]
+        L.orc_sweep_sync.restype = ctypes.c_int
         L.orc_serial_sweep.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, i32p,
                                        i64p, P(Params), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int]
         L.orc_serial_sweep.restype = ctypes.c_int
@@ -171,16 +173,16 @@ def tally(lat: np.ndarray, d: int, ncells: int) -> Tables:
 
 
 def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
-          mcs0: int, nmcs: int, nthreads: int = 1) -> None:
+          mcs0: int, nmcs: int, nthreads: int = 1, cell_sync: bool = False) -> None:
     """Run nmcs MCS in place on lat (uint32 view) and tables."""
     assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
     lat32 = lat.view(np.uint32)
     L = lat.shape[0]
-    rc = lib().orc_sweep(L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
-                         _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
-                         _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
-                         ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
-                         int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads))
+    rc = lib().orc_sweep_sync(L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
+                              _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+                              _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                              ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
+                              int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads), int(bool(cell_sync)))
     if rc:
         raise ValueError(f"orc_sweep failed with code {rc}")
 

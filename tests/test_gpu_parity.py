@@ -175,3 +175,25 @@ def test_3d_box_staging_matches_oracle(box, monkeypatch):
     lat_o, tab_o = run_oracle(wl, 1, 1, 17, 6)
     np.testing.assert_array_equal(lat, lat_o)
     assert_tables_equal(table, tab_o, 3)
+
+
+def test_cell_sync_matches_oracle():
+    """cell_sync = 1 (at most one flip per cell per colour phase, lowest
+    sub-block wins) in 2D and 3D, bit-exact against the oracle."""
+    _, W = _gpu()
+    for wl, n in ((W.plain2d(L=64, ncells=20, seed=26), 30), (W.plain3d(L=32, ncells=60, seed=27), 6)):
+        _announce("cell_sync", wl.name, wl.L)
+        gpucpm, _ = _gpu()
+        sim = wl.make_cpm()
+        sim.push_to_gpu()
+        sim.set_seed(41)
+        sim.run(**run_kwargs(1, 1, n, cell_sync=1))
+        sim.synchronize()
+        sim.pull_from_gpu()
+        import oracle as O
+        from helpers import oracle_params
+        lat_o = wl.lattice.copy()
+        tab_o = O.tally(lat_o, wl.d, wl.ncells)
+        O.sweep(lat_o, wl.d, tab_o, oracle_params(wl), 1, 1, 41, 0, n, 8, cell_sync=True)
+        np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(sim.get_cell_table(), tab_o, wl.d)

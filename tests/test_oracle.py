@@ -291,3 +291,33 @@ def test_sweep_rejects_bad_schedule():
     for c, u in [(3, 1), (128, 1), (2, 0), (2, 5)]:
         with pytest.raises(ValueError):
             O.sweep(lat, 2, tab, p, c, u, 1, 0, 1)
+
+
+def test_cell_sync_is_deterministic_and_stricter():
+    """cell_sync (kernels.cu:123-141, restated as: at most one committed flip
+    per cell per colour phase, lowest sub-block index wins) is thread-count
+    invariant, keeps exact tables and commits no more flips than the plain
+    sweep from the same state."""
+    lat, n, p = _sorting_small(seed=3)
+    p.temperature = 1000.0
+    res = []
+    for cs in (False, True):
+        for nt in (1, 8):
+            l = lat.copy()
+            t = O.tally(l, 2, n)
+            O.sweep(l, 2, t, p, 1, 1, 5, 0, 1, nt, cell_sync=cs)
+            area, perim = _numpy_recount(l, 2, n)
+            np.testing.assert_array_equal(t.area, area)
+            np.testing.assert_array_equal(t.perim, perim)
+            res.append(l)
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[2], res[3])
+    plain, sync = res[0], res[2]
+    assert 0 < np.count_nonzero(sync != lat) < np.count_nonzero(plain != lat)
+    # at most one flip per cell per colour phase: over one MCS (four
+    # phases) no cell's area moves by more than 4
+    t0 = O.tally(lat, 2, n)
+    t1 = O.tally(sync, 2, n)
+    assert np.max(np.abs(t1.area.astype(int) - t0.area)) <= 4
+    with pytest.raises(ValueError):
+        O.sweep(lat.copy(), 2, O.tally(lat, 2, n), p, 2, 4, 5, 0, 1, cell_sync=True)
