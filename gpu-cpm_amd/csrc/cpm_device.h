@@ -52,6 +52,7 @@ struct SweepConsts {
     int rounds;             // c^d / u colour rounds per MCS
     uint32_t k0, k1;        // Philox key (seed)
     int nt;                 // types present (pair-table stride)
+    uint32_t fixed_mask;    // bit t set: type t is fixed (settings.h:24)
 };
 
 // One colour phase per launch.

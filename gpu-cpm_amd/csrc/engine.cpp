@@ -68,6 +68,7 @@ struct HostSettings {
     double lam_perim[CPM_NTYPES];
     int32_t target_perim[CPM_NTYPES];
     double temperature;
+    int32_t fixed[CPM_NTYPES];  // settings.h:24
 };
 
 struct cpm_engine {
@@ -430,7 +431,7 @@ int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness) {
 int cpm_set_fixed(cpm_handle h, int type, int fixed) {
     if (int rc = check_handle(h)) return rc;
     if (int rc = check_type(type, "cell_type")) return rc;
-    if (fixed) return fail(CPM_ENOTIMPL, "fixed cell types are not implemented");
+    h->st.fixed[type] = fixed;
     return CPM_OK;
 }
 
@@ -532,6 +533,9 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     k.k0 = (uint32_t)h->seed;
     k.k1 = (uint32_t)(h->seed >> 32);
     k.nt = h->nt;
+    k.fixed_mask = 0;
+    for (int t = 0; t < CPM_NTYPES; t++)
+        if (h->st.fixed[t]) k.fixed_mask |= 1u << t;
     *kout = k;
     *lh_out = h->lgL - k.lc - 1;
     return CPM_OK;

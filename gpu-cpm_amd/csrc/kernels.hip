@@ -214,6 +214,13 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
 #pragma unroll
         for (int n = 0; n < NB; n++) vs = n == dir ? nv[n] : vs;
         if (vs == vt) continue;  // kernels2d.cu:276
+        if (k.fixed_mask) {      // fixed types (kernels2d.cu:281-290)
+            if ((k.fixed_mask >> (vt >> 24)) & 1u) continue;  // a fixed target stays
+            if ((k.fixed_mask >> (vs >> 24)) & 1u) {          // a fixed source copies medium
+                vs = 0;
+                if (vs == vt) continue;
+            }
+        }
 
         const uint32_t ts = vs >> 24, tt = vt >> 24;
         const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;

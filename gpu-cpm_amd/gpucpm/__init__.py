@@ -119,8 +119,6 @@ class Cpm:
         stick = _int(stickiness, "stickiness")
 
         # Terms outside the implemented Hamiltonian are refused, not ignored.
-        if fx > 0:
-            raise NotImplementedError("fixed cell types are not implemented")
         if lpers != -1 and pdiff != -1 and ptime != -1 and lpers != 0:
             raise NotImplementedError("the persistence term is not implemented")
         if lchem != -1 and lchem != 0:

@@ -11,6 +11,7 @@ Each builder returns a ``Workload``: the initial lattice (uint32, ids
 - ``plain3d``    256^3, 20000 cubes of 5^3 at random non-overlapping positions,
                  volume + surface constraints, 26-neighbourhood.
 - ``batched_sorting``  64 sorting replicas of mixed sizes and parameters.
+- ``walled``     any of these inside a one-site frame of a fixed cell type.
 The shapes may be scaled down (``L`` / ``ncells`` arguments) for tests.
 """
 from __future__ import annotations
@@ -50,6 +51,7 @@ class Workload:
         lp = np.zeros(20)
         p0 = np.zeros(20, np.int64)
         S = np.zeros((20, 20), np.int64)
+        fixed = np.zeros(20, np.int64)
         for kw in self.constraints:
             t = kw["cell_type"]
             if "adhesion" in kw:
@@ -61,8 +63,10 @@ class Workload:
                 la[t], a0[t] = kw["lambda_area"], kw["target_area"]
             if "lambda_perimeter" in kw:
                 lp[t], p0[t] = kw["lambda_perimeter"], kw["target_perimeter"]
+            if kw.get("fixed", -1) >= 0:
+                fixed[t] = kw["fixed"]
         return dict(J=J, S=S, lam_area=la, target_area=a0, lam_perim=lp, target_perim=p0,
-                    temperature=float(self.temperature))
+                    temperature=float(self.temperature), fixed=fixed)
 
 
 def _place_boxes(L, d, n, edge, rng, types):
@@ -135,6 +139,25 @@ def batched_sorting(nrep=64, seed=4, sizes=(128, 256, 512, 1024)):
         wl.name = f"replica{i}"
         out.append(wl)
     return out
+
+
+def walled(wl: Workload, wall_type=3, J_wall=(8, 4, 12)) -> Workload:
+    """``wl`` inside a fixed wall: the lattice sites with any coordinate 0
+    become one extra cell (id ``ncells + 1``) of ``wall_type``, which
+    ``set_constraints(fixed=1)`` makes unmovable.  ``J_wall`` holds the
+    adhesion of the wall with medium, type 1 and type 2."""
+    lat = wl.lattice.copy()
+    wid = wl.ncells + 1
+    edge = np.zeros(lat.shape, bool)
+    for ax in range(wl.d):
+        sl = [slice(None)] * wl.d
+        sl[ax] = 0
+        edge[tuple(sl)] = True
+    lat[edge] = (np.uint32(wall_type) << np.uint32(24)) | np.uint32(wid)
+    cons = list(wl.constraints) + [dict(cell_type=wall_type, fixed=1)]
+    for other, j in zip((0, 1, 2), J_wall):
+        cons.append(dict(cell_type=wall_type, other_cell_type=other, adhesion=int(j)))
+    return Workload(wl.name + "_walled", wl.L, wl.d, wid, wl.temperature, lat, cons)
 
 
 def plain3d(L=256, ncells=20000, seed=3):

@@ -85,7 +85,8 @@ int cpm_set_adhesion_real(cpm_handle h, int type, int other_type, double adhesio
 /* Cpm::setStickiness (cpm.cu:145-148); int16 (settings.h:22). */
 int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness);
 
-/* Cpm::setFixed (cpm.cu:150-153).  Only fixed == 0 is accepted. */
+/* Cpm::setFixed (cpm.cu:150-153): a fixed type is never overwritten, and a
+ * copy from a fixed source writes medium instead (kernels2d.cu:281-290). */
 int cpm_set_fixed(cpm_handle h, int type, int fixed);
 
 /* Cpm::addCell (cpm.cu:160-174): writes id (k | type<<24) for the k-th

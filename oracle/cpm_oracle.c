@@ -340,6 +340,13 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                         uint32_t vs = lat[site(L, sx, sy, sz)];
                         uint32_t vt = lat[ti];
                         if (vs == vt) continue; /* kernels2d.cu:276 */
+                        /* fixed types (kernels2d.cu:281-290): a fixed target is
+                         * never overwritten; a fixed source copies medium */
+                        if (p->fixed[vt >> 24]) continue;
+                        if (p->fixed[vs >> 24]) {
+                            vs = 0;
+                            if (vs == vt) continue;
+                        }
                         int as, at;
                         int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
                                                   tz, vs, vt, &as, &at);
@@ -445,6 +452,11 @@ int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int
             uint32_t vs = lat[site(L, sx, sy, sz)];
             uint32_t vt = lat[ti];
             if (vs == vt) continue;
+            if (p->fixed[vt >> 24]) continue;
+            if (p->fixed[vs >> 24]) {
+                vs = 0;
+                if (vs == vt) continue;
+            }
             int as, at;
             int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, vs, vt, &as, &at);
             if (!(dh <= 0 || (dh < nthr && w[2] < thr[dh]))) continue;

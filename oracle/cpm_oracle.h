@@ -45,6 +45,7 @@ typedef struct {
     double lam_perim[ORC_NTYPES];        /* settings.h:17 */
     int32_t target_perim[ORC_NTYPES];    /* settings.h:18 */
     double temperature;                  /* settings.h:30 */
+    int32_t fixed[ORC_NTYPES];           /* settings.h:24: unmovable types */
 } orc_params;
 
 /* Generators. */

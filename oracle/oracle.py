@@ -39,11 +39,12 @@ class Params(ctypes.Structure):
         ("lam_perim", ctypes.c_double * NTYPES),
         ("target_perim", ctypes.c_int32 * NTYPES),
         ("temperature", ctypes.c_double),
+        ("fixed", ctypes.c_int32 * NTYPES),
     ]
 
     @classmethod
     def make(cls, temperature, J=None, S=None, lam_area=None, target_area=None,
-             lam_perim=None, target_perim=None):
+             lam_perim=None, target_perim=None, fixed=None):
         p = cls()
         p.temperature = float(temperature)
         for name, val in (("J", J), ("S", S)):
@@ -55,10 +56,11 @@ class Params(ctypes.Structure):
                     for b in range(NTYPES):
                         getattr(p, name)[a][b] = int(arr[a, b])
         for name, val in (("lam_area", lam_area), ("target_area", target_area),
-                          ("lam_perim", lam_perim), ("target_perim", target_perim)):
+                          ("lam_perim", lam_perim), ("target_perim", target_perim),
+                          ("fixed", fixed)):
             if val is not None:
                 for t, x in enumerate(val):
-                    getattr(p, name)[t] = x
+                    getattr(p, name)[t] = float(x) if name.startswith("lam") else int(x)
         return p
 
     def J_array(self):

@@ -197,3 +197,24 @@ def test_cell_sync_matches_oracle():
         O.sweep(lat_o, wl.d, tab_o, oracle_params(wl), 1, 1, 41, 0, n, 8, cell_sync=True)
         np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
         assert_tables_equal(sim.get_cell_table(), tab_o, wl.d)
+
+
+@pytest.mark.parametrize("mode,d,c,u", [("phase", 2, 1, 1), ("phase", 2, 2, 4), ("global", 2, 2, 3),
+                                        ("lds", 2, 1, 1), ("phase", 3, 1, 1), ("phase", 3, 2, 8)])
+def test_fixed_wall_bit_exact(mode, d, c, u, monkeypatch):
+    """A fixed cell type (Cpm::setFixed, kernels2d.cu:281-290) as a frame
+    around the cells: bit-exact against the oracle in every launch form, and
+    the wall is unchanged."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_MODE", mode)
+    base = W.sorting2d(L=128, grid=9, edge=10, seed=28) if d == 2 else W.plain3d(L=32, ncells=60, seed=29)
+    wl = W.walled(base)
+    _announce("fixed", mode, wl.name, wl.L, c, u)
+    n = 30 if d == 2 else 8
+    _, lat, table = _run_gpu(wl, c, u, seed=51, nmcs=n)
+    lat_o, tab_o = run_oracle(wl, c, u, 51, n)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, d)
+    wall = (wl.lattice >> 24) == 3
+    np.testing.assert_array_equal(lat[wall], wl.lattice[wall])
+    assert np.count_nonzero(lat != wl.lattice) > 0

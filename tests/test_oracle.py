@@ -321,3 +321,60 @@ def test_cell_sync_is_deterministic_and_stricter():
     assert np.max(np.abs(t1.area.astype(int) - t0.area)) <= 4
     with pytest.raises(ValueError):
         O.sweep(lat.copy(), 2, O.tally(lat, 2, n), p, 2, 4, 5, 0, 1, cell_sync=True)
+
+
+# ------------------------------------------------------- fixed types ------
+
+def _walled_blocks(L=32, fixed=1):
+    """No medium at all: a one-site frame (cell 5, type 3) and four type-1
+    cells filling the inside.  The wall adheres badly to cells and well to
+    medium, so copies of medium from the wall are favoured."""
+    lat = np.zeros((L, L), np.uint32)
+    h = L // 2
+    for k, (y0, x0) in enumerate([(0, 0), (0, h), (h, 0), (h, h)]):
+        lat[y0:y0 + h, x0:x0 + h] = (1 << 24) | (k + 1)
+    lat[0, :] = lat[:, 0] = (3 << 24) | 5
+    J = np.zeros((20, 20))
+    J[1, 1] = 6
+    J[0, 1] = J[1, 0] = 2
+    J[1, 3] = J[3, 1] = 30
+    fx = np.zeros(20)
+    fx[3] = fixed
+    p = O.Params.make(10, J=J, lam_area=[0, 1, 0, 0], target_area=[0, 225, 0, 0],
+                      fixed=fx)
+    return lat, 5, p
+
+
+@pytest.mark.parametrize("serial", [False, True])
+def test_fixed_type_never_moves_and_copies_medium(serial):
+    """settings.h:24 / kernels2d.cu:281-290: a fixed target keeps its value
+    and a copy from a fixed source writes medium.  The lattice starts with
+    no medium, so every medium site after the run was created by the second
+    rule; with the wall not fixed no medium can ever appear and the wall
+    moves."""
+    for fixed in (1, 0):
+        lat, n, p = _walled_blocks(fixed=fixed)
+        before = lat.copy()
+        tab = O.tally(lat, 2, n)
+        if serial:
+            O.serial_sweep(lat, 2, tab, p, seed=3, mcs0=0, nmcs=20)
+        else:
+            O.sweep(lat, 2, tab, p, 2, 4, seed=3, mcs0=0, nmcs=20)
+        wall_before = (before >> 24) == 3
+        wall_after = (lat >> 24) == 3
+        area, perim = _numpy_recount(lat, 2, n)
+        np.testing.assert_array_equal(tab.area, area)
+        np.testing.assert_array_equal(tab.perim, perim)
+        if fixed:
+            np.testing.assert_array_equal(wall_after, wall_before)
+            assert np.all(lat[wall_before] == before[wall_before])
+            nmed = int(np.count_nonzero(lat == 0))
+            assert nmed > 10
+            # medium first appears next to the wall; after 20 MCS it is still
+            # within a few sites of it (x or y small, or wrapped to L-1)
+            ys, xs = np.nonzero(lat == 0)
+            d = np.minimum(np.minimum(xs, 32 - xs), np.minimum(ys, 32 - ys))
+            assert d.max() <= 10
+        else:
+            assert np.count_nonzero(lat == 0) == 0
+            assert np.count_nonzero(wall_after != wall_before) > 0
