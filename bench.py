@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -276,6 +277,12 @@ def bench_gradient(a, log):
     print(json.dumps(line), flush=True)
 
 
+def kernel_token(name):
+    """File-name form of a kernel name: 'k_sweep_phase_compact<2, true, false>'
+    -> 'k_sweep_phase_compact_2_true_false'."""
+    return re.sub(r"[^A-Za-z0-9]+", "_", name).strip("_")
+
+
 def _pmc_mean(d, kernel):
     import csv
     import glob
@@ -382,7 +389,7 @@ def main():
     bytes_per_launch = updates / launches * BYTES_PER_UPDATE
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{launch['kernel']}_{a.workload}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel_token(launch['kernel'])}_{a.workload}.json")
     if os.path.exists(tfile) and c == 1 and u == 1:
         traffic = json.load(open(tfile))["bytes_per_dispatch"]
     workload_desc = {

@@ -17,15 +17,36 @@ constexpr uint32_t kSpinMask = 0xFFFFFFu;
 // Hamiltonian parameters in fixed point: every entry is the reference value
 // multiplied by the engine's dyadic scale, so dH is an exact integer.  Only
 // the nt types present in the lattice are stored (nt = 1 + largest type at
-// push), pair tables row-major with stride nt, so kernels stage
-// 2 nt^2 + 4 nt words.
+// push), as 2 nt^2 + 4 nt int32 words that kernels stage verbatim:
+//   J  [a * nt + b]  adhesion * scale                      (settings.h:20)
+//   JS [a * nt + b]  (adhesion + stickiness) * scale        (settings.h:22)
+//   la [a]           lambda_area * scale                    (settings.h:11)
+//   a0 [a]           target area                            (settings.h:12)
+//   lp [a]           lambda_perimeter * scale               (settings.h:17)
+//   p0 [a]           target perimeter                       (settings.h:18)
+// When at most four types are present the block is also kept padded to
+// four types (s4: J[4][4], JS[4][4], la[4], a0[4], lp[4], p0[4]), which the
+// sweep reads with uniform (scalar) loads instead of staging it.
+constexpr int kParamWords = 2 * kNTypes * kNTypes + 4 * kNTypes;
+constexpr int kSmallTypes = 4;
+constexpr int kSmallWords = 2 * kSmallTypes * kSmallTypes + 4 * kSmallTypes;
 struct DevParams {
-    int32_t J[kNTypes * kNTypes];   // adhesion * scale, [a * nt + b]            (settings.h:20)
-    int32_t JS[kNTypes * kNTypes];  // (adhesion + stickiness) * scale, [a * nt + b] (settings.h:22)
-    int32_t la[kNTypes];            // lambda_area * scale                       (settings.h:11)
-    int32_t a0[kNTypes];            // target area                               (settings.h:12)
-    int32_t lp[kNTypes];            // lambda_perimeter * scale                  (settings.h:17)
-    int32_t p0[kNTypes];            // target perimeter                          (settings.h:18)
+    int32_t w[kParamWords];
+    int32_t s4[kSmallWords];
+};
+
+__host__ __device__ inline int param_words(int nt) { return 2 * nt * nt + 4 * nt; }
+
+// Indexing of a compact parameter block of nt types.
+struct ParamView {
+    const int32_t* w;
+    int nt, n2;
+    __device__ __forceinline__ int32_t J(int a, int b) const { return w[a * nt + b]; }
+    __device__ __forceinline__ int32_t JS(int a, int b) const { return w[n2 + a * nt + b]; }
+    __device__ __forceinline__ int32_t la(int a) const { return w[2 * n2 + a]; }
+    __device__ __forceinline__ int32_t a0(int a) const { return w[2 * n2 + nt + a]; }
+    __device__ __forceinline__ int32_t lp(int a) const { return w[2 * n2 + 2 * nt + a]; }
+    __device__ __forceinline__ int32_t p0(int a) const { return w[2 * n2 + 3 * nt + a]; }
 };
 
 // Constants of one sweep (one run() call).
@@ -68,7 +89,13 @@ struct PhaseArgs {
     int sync_pass;          // cell_sync: 0 off, 1 claim pass, 2 commit pass
     uint64_t* claims;       // cell_sync: per-cell claim keys
     uint32_t sync_tag;      // cell_sync: phase tag (grows across runs)
+    int form;               // kForm* bits: which per-phase kernel runs
 };
+
+// Per-phase kernel forms (results never depend on them).
+constexpr int kFormSmall = 1;    // read at most four types' parameters without LDS staging
+constexpr int kFormCompact = 2;  // u = 1: queue the attempts whose source and target differ
+constexpr int kFormFull = 4;     // compact: the draw loads (and queues) the whole neighbourhood
 
 // A whole run in one cooperative launch, phases separated by grid barriers.
 struct PersistArgs {
@@ -109,7 +136,10 @@ struct BatchPhaseArgs {
 };
 
 // Host-side launchers (kernels.hip).
-hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+// kernel_id (optional) receives the kernel used; phase_kernel_name names it.
+hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        int* kernel_id = nullptr);
+const char* phase_kernel_name(int id);
 // Cooperative persistent launches.  lds2d selects the 2D LDS-tiled kernel.
 // max_blocks returns the co-resident block limit for the chosen kernel.
 hipError_t persist_max_blocks(int d, bool lds2d, size_t lds_bytes, int* max_blocks);

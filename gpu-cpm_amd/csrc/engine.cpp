@@ -100,6 +100,7 @@ struct cpm_engine {
     int nthr = 0;
     int scale = 1;
     int nt = 1;  // types present in the lattice at the last push
+    int last_kernel = -1;  // per-phase kernel of the last run (gcpm::phase_kernel_name)
 
     // Kernel timing (cpm_timing_begin/end): one event pair per sweep launch.
     bool timing = false;
@@ -221,17 +222,28 @@ struct cpm_engine {
             return fail(CPM_EINVAL,
                         "lambda_area / lambda_perimeter / adhesion must be multiples of 2^-8 "
                         "(dyadic fixed-point energy)");
+        // Compact block of the nt types present (layout in cpm_device.h).
         gcpm::DevParams p{};
-        for (int a = 0; a < nt; a++)
+        const int n2 = nt * nt;
+        for (int a = 0; a < nt; a++) {
             for (int b = 0; b < nt; b++) {
-                p.J[a * nt + b] = (int32_t)(st.J[a][b] * sc);
-                p.JS[a * nt + b] = (int32_t)(st.J[a][b] * sc) + (int32_t)st.S[a][b] * sc;
+                p.w[a * nt + b] = (int32_t)(st.J[a][b] * sc);
+                p.w[n2 + a * nt + b] = (int32_t)(st.J[a][b] * sc) + (int32_t)st.S[a][b] * sc;
             }
-        for (int a = 0; a < CPM_NTYPES; a++) {
-            p.la[a] = (int32_t)(st.lam_area[a] * sc);
-            p.lp[a] = (int32_t)(st.lam_perim[a] * sc);
-            p.a0[a] = st.target_area[a];
-            p.p0[a] = st.target_perim[a];
+            p.w[2 * n2 + a] = (int32_t)(st.lam_area[a] * sc);
+            p.w[2 * n2 + nt + a] = st.target_area[a];
+            p.w[2 * n2 + 2 * nt + a] = (int32_t)(st.lam_perim[a] * sc);
+            p.w[2 * n2 + 3 * nt + a] = st.target_perim[a];
+        }
+        if (nt <= gcpm::kSmallTypes) {
+            constexpr int S = gcpm::kSmallTypes, S2 = S * S;
+            for (int a = 0; a < nt; a++) {
+                for (int b = 0; b < nt; b++) {
+                    p.s4[a * S + b] = p.w[a * nt + b];
+                    p.s4[S2 + a * S + b] = p.w[n2 + a * nt + b];
+                }
+                for (int f = 0; f < 4; f++) p.s4[2 * S2 + f * S + a] = p.w[2 * n2 + f * nt + a];
+            }
         }
         // Acceptance thresholds thr[k] = floor(exp(-k / (scale T)) 2^32).
         std::vector<uint32_t> thr(1, 0xFFFFFFFFu);
@@ -576,6 +588,17 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         // global reads at 256^3 (61 vs 44 us per phase), so it is opt-in.
         const char* bx = std::getenv("GPUCPM_BOX3");
         pa.box3 = bx ? std::atoi(bx) : 0;
+        // Kernel form (measured fastest by default): GPUCPM_SMALL=1 reads
+        // parameters without LDS staging (default in 3D),
+        // GPUCPM_COMPACT=0 finishes every attempt in its own thread,
+        // GPUCPM_FULL=1 queues whole neighbourhoods.
+        auto env_flag = [](const char* name, bool dflt) {
+            const char* e = std::getenv(name);
+            return e ? std::atoi(e) != 0 : dflt;
+        };
+        pa.form = (env_flag("GPUCPM_SMALL", h->d == 3) ? gcpm::kFormSmall : 0) |
+                  (env_flag("GPUCPM_COMPACT", true) ? gcpm::kFormCompact : 0) |
+                  (env_flag("GPUCPM_FULL", false) ? gcpm::kFormFull : 0);
         const int ncol = 1 << h->d;
         int g = 0;  // phase counter since the tables were last settled
         for (long m = 0; m < total; m++) {
@@ -598,7 +621,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                     }
                     hipEvent_t e0, e1;
                     if (int rc = h->next_events(&e0, &e1)) return rc;
-                    HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1));
+                    HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1, &h->last_kernel));
                     if (h->timing) {
                         h->t_launches++;
                         h->t_updates += (int64_t)pa.nsub * u;
@@ -731,6 +754,15 @@ int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile) {
     if (blocks) *blocks = h->last_plan.blocks;
     if (tile) *tile = h->last_plan.ltile ? 1 << h->last_plan.ltile : 0;
     return CPM_OK;
+}
+
+const char* cpm_last_kernel(cpm_handle h) {
+    if (check_handle(h)) return "";
+    switch (h->last_plan.mode) {
+        case Plan::kGlobal: return h->d == 3 ? "k_persist_global<3>" : "k_persist_global<2>";
+        case Plan::kLds2d: return "k_persist_lds2d";
+        default: return gcpm::phase_kernel_name(h->last_kernel);
+    }
 }
 
 int cpm_set_launch(cpm_handle h, int form, int tile) {

@@ -10,6 +10,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "cpm_device.h"
 
@@ -19,6 +20,13 @@ namespace gcpm {
 __device__ __forceinline__ void philox4x32_10(uint32_t x0, uint32_t x1, uint32_t x2,
                                               uint32_t x3, uint32_t k0, uint32_t k1,
                                               uint32_t w[4]) {
+    // Opaque to the optimiser, so the ten round keys are computed per call
+    // (scalar adds) rather than hoisted out of the caller's loop into twenty
+    // long-lived scalar registers.
+    // (The key is the same for every lane.)
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    k1 = __builtin_amdgcn_readfirstlane(k1);
+    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
@@ -95,7 +103,40 @@ struct GlobalLat {
         return ld<CO>(p + site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL));
     }
     __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { st<CO>(p + q.i, v); }
+    // The 3^D block around q (centre included), (dz, dy, dx) order with dx
+    // fastest.  Byte offsets from the lattice base fit in 32 bits (at most
+    // 2^30 sites), so every load is base (scalar) + one 32-bit offset.
+    __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
+        const char* base = reinterpret_cast<const char*>(p);
+        const uint32_t xo[3] = {(uint32_t)((q.x - 1) & Lm) << 2, (uint32_t)q.x << 2,
+                                (uint32_t)((q.x + 1) & Lm) << 2};
+        const uint32_t yo[3] = {(uint32_t)((q.y - 1) & Lm) << (lgL + 2), (uint32_t)q.y << (lgL + 2),
+                                (uint32_t)((q.y + 1) & Lm) << (lgL + 2)};
+        uint32_t zo[3] = {0u, 0u, 0u};
+        if constexpr (D == 3) {
+            zo[0] = (uint32_t)((q.z - 1) & Lm) << (2 * lgL + 2);
+            zo[1] = (uint32_t)q.z << (2 * lgL + 2);
+            zo[2] = (uint32_t)((q.z + 1) & Lm) << (2 * lgL + 2);
+        }
+        constexpr int NB = D == 3 ? 27 : 9;
+#pragma unroll
+        for (int n = 0; n < NB; n++)
+            nv[n] = ld<CO>(reinterpret_cast<const uint32_t*>(base + (xo[n % 3] | yo[(n / 3) % 3] |
+                                                                      zo[n / 9])));
+    }
 };
+
+// Neighbourhood loads of a lattice held in LDS.
+template <int D, class Lat, class Pos>
+__device__ __forceinline__ void lds_block(const Lat& lat, const Pos& q, uint32_t* nv) {
+    constexpr int NB = D == 3 ? 27 : 9;
+#pragma unroll
+    for (int n = 0; n < NB; n++) {
+        int nx, ny, nz;
+        block_offset<D>(n, nx, ny, nz);
+        nv[n] = lat.nb(q, nx, ny, nz);
+    }
+}
 
 // 2D tile of the lattice resident in LDS with a one-site halo ring; row
 // stride S = tile width + 2.  Writes go to LDS and through to global memory,
@@ -118,6 +159,9 @@ struct LdsLat2 {
     __device__ __forceinline__ void put(const Pos& q, uint32_t v) const {
         s[q.li] = v;
         st<true>(g + q.gi, v);
+    }
+    __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
+        lds_block<2>(*this, q, nv);
     }
 };
 
@@ -144,6 +188,9 @@ struct LdsLat3 {
         s[q.li] = v;
         g[q.gi] = v;
     }
+    __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
+        lds_block<3>(*this, q, nv);
+    }
 };
 
 // Cell-table buffers of colour phase g (see SweepConsts).
@@ -155,11 +202,65 @@ struct PhaseTables {
 };
 
 __device__ __forceinline__ PhaseTables tables_of(const SweepConsts& k, int g) {
-    return {k.tab[g & 1], k.tab[(g + 1) & 1], k.del[g & 1], k.del[(g + 1) & 1]};
+    // selects rather than an index: a dynamic index into a kernel's private
+    // copy of SweepConsts would place the copy in scratch memory
+    const bool odd = g & 1;
+    return {odd ? k.tab[1] : k.tab[0], odd ? k.tab[0] : k.tab[1], odd ? k.del[1] : k.del[0],
+            odd ? k.del[0] : k.del[1]};
 }
 
-// The u copy attempts of sub-block (bx, by, bz) in one colour phase: the
-// copy attempt of kernels2d.cu:197-570 / kernels3d.cu:165-583 in the
+// One copy attempt, split in two: draw_attempt makes the Philox draw and
+// issues the loads of the target's 3^D neighbourhood; finish_attempt
+// evaluates dH and commits.
+template <int D, class Lat>
+struct Draw {
+    static constexpr int NB = D == 3 ? 27 : 9;
+    uint32_t w[4];
+    typename Lat::Pos t;
+    uint32_t nv[NB];  // the 3^D block around the target, centre included
+    uint32_t vs;      // the source: the neighbour picked by w[1]
+};
+
+template <int D>
+__device__ __forceinline__ uint32_t sub_block_id(const SweepConsts& k, int bx, int by, int bz) {
+    const int lsb = k.lgL - k.lc;  // log2 sub-blocks per axis
+    return (uint32_t)bx | ((uint32_t)by << lsb) | (D == 3 ? ((uint32_t)bz << (2 * lsb)) : 0u);
+}
+
+// Source direction (kernels2d.cu:216-225 drawn in one step): floor(w1 N /
+// 2^32) over the N neighbours in block order, skipping the centre.
+// In 2D the source is selected from the nine loaded values, so all loads
+// are issued in one round.  In 3D it is loaded on its own: the compiler then
+// issues the target and source loads first and the other 25 only when they
+// differ, which skips most of the loads (and work) in sparse lattices.
+template <int D, class Lat>
+__device__ __forceinline__ void draw_attempt(const Lat& lat, const SweepConsts& k, uint32_t sb,
+                                             int bx, int by, int bz, uint32_t mcs,
+                                             uint32_t phase, uint32_t i, Draw<D, Lat>& d) {
+    const int lc = k.lc;
+    const uint32_t cm = (1u << lc) - 1u;
+    philox4x32_10(sb, mcs, phase, i, k.k0, k.k1, d.w);
+    const int tx = (bx << lc) + (int)(d.w[0] & cm);
+    const int ty = (by << lc) + (int)((d.w[0] >> lc) & cm);
+    const int tz = D == 3 ? (bz << lc) + (int)((d.w[0] >> (2 * lc)) & cm) : 0;
+    d.t = lat.at(tx, ty, tz);
+    lat.block(d.t, d.nv);
+    constexpr int N = D == 3 ? 26 : 8;
+    int dir = (int)(((uint64_t)d.w[1] * (uint64_t)N) >> 32);
+    dir += dir >= N / 2;
+    if constexpr (D == 2) {
+        uint32_t vs = 0;
+#pragma unroll
+        for (int n = 0; n < 9; n++) vs = n == dir ? d.nv[n] : vs;
+        d.vs = vs;
+    } else {
+        int sx, sy, sz;
+        block_offset<D>(dir, sx, sy, sz);
+        d.vs = lat.nb(d.t, sx, sy, sz);
+    }
+}
+
+// The attempt of kernels2d.cu:197-570 / kernels3d.cu:165-583 in the
 // shared-mode roles (target drawn in the sub-block, source a uniformly drawn
 // neighbour), with the deterministic RNG, snapshot tables and integer
 // acceptance described in SweepConsts.
@@ -174,164 +275,232 @@ struct CellSync {
     uint32_t tag;
 };
 
-template <int D, bool CO, class Lat>
-__device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k,
-                                                   const DevParams* __restrict__ P,
-                                                   const uint32_t* __restrict__ thr, int bx,
-                                                   int by, int bz, uint32_t mcs, uint32_t phase,
-                                                   const PhaseTables& T,
-                                                   const CellSync cy = CellSync{0, nullptr, 0}) {
-    constexpr int N = D == 3 ? 26 : 8;
-    constexpr int NB = D == 3 ? 27 : 9;
-    constexpr int CENTRE = NB / 2;
-    const int lc = k.lc;
-    const uint32_t cm = (1u << lc) - 1u;
-    const int lsb = k.lgL - lc;  // log2 sub-blocks per axis
-    const int Lm = (1 << k.lgL) - 1;
-    const uint32_t sb = (uint32_t)bx | ((uint32_t)by << lsb) |
-                        (D == 3 ? ((uint32_t)bz << (2 * lsb)) : 0u);
+// Adhesion over the neighbourhood (kernels2d.cu:359-402,477):
+//   sum_n [v_n != v_s] J(t_s, t_n) - sum_n [v_n != v_t] JS(t_t, t_n).
+// v_n == v_s implies t_n == t_s, so with c_t the number of neighbours of
+// type t this is sum_t (J(t_s, t) - JS(t_t, t)) c_t - a_s J(t_s, t_s) +
+// a_t JS(t_t, t_t), where a_s, a_t count the neighbours equal to source and
+// target.  For up to six types the c_t are packed as 5-bit fields of one
+// word (at most 26 neighbours), so the neighbour loop does no table lookups.
+constexpr int kHistTypes = 6;
 
-    for (int i = 0; i < k.u; i++) {
-        uint32_t w[4];
-        philox4x32_10(sb, mcs, phase, (uint32_t)i, k.k0, k.k1, w);
-        const int tx = (bx << lc) + (int)(w[0] & cm);
-        const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
-        const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
-        int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
-        dir += dir >= CENTRE;  // index in the 3^D block, skipping the centre
-        const typename Lat::Pos t = lat.at(tx, ty, tz);
-        // The whole neighbourhood in one round of independent loads; the
-        // source is the neighbour picked by dir.
-        uint32_t nv[NB];
-#pragma unroll
-        for (int n = 0; n < NB; n++) {
-            int nx, ny, nz;
-            block_offset<D>(n, nx, ny, nz);
-            nv[n] = n == CENTRE ? lat.self(t) : lat.nb(t, nx, ny, nz);
-        }
-        const uint32_t vt = nv[CENTRE];
-        uint32_t vs = 0;
-#pragma unroll
-        for (int n = 0; n < NB; n++) vs = n == dir ? nv[n] : vs;
-        if (vs == vt) continue;  // kernels2d.cu:276
-        if (k.fixed_mask) {      // fixed types (kernels2d.cu:281-290)
-            if ((k.fixed_mask >> (vt >> 24)) & 1u) continue;  // a fixed target stays
-            if ((k.fixed_mask >> (vs >> 24)) & 1u) {          // a fixed source copies medium
-                vs = 0;
-                if (vs == vt) continue;
-            }
-        }
-
-        const uint32_t ts = vs >> 24, tt = vt >> 24;
-        const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
-        // Issue the snapshot-table loads first: their latency overlaps the
-        // neighbourhood sums.
-        const int64_t row_s = cs ? ld<CO>(T.R + cs) : 0;
-        const int64_t row_t = ct ? ld<CO>(T.R + ct) : 0;
-        const int32_t* __restrict__ Js = P->J + ts * k.nt;
-        const int32_t* __restrict__ Jt = P->JS + tt * k.nt;
+// Parameters staged in LDS (any number of types).
+struct LdsParamView {
+    static constexpr bool kAlwaysHist = false;
+    const int32_t* w;
+    int nt, n2;
+    __device__ __forceinline__ bool hist() const { return nt <= kHistTypes; }
+    __device__ __forceinline__ int32_t J(int a, int b) const { return w[a * nt + b]; }
+    __device__ __forceinline__ int32_t JS(int a, int b) const { return w[n2 + a * nt + b]; }
+    __device__ __forceinline__ int32_t la(int a) const { return w[2 * n2 + a]; }
+    __device__ __forceinline__ int32_t a0(int a) const { return w[2 * n2 + nt + a]; }
+    __device__ __forceinline__ int32_t lp(int a) const { return w[2 * n2 + 2 * nt + a]; }
+    __device__ __forceinline__ int32_t p0(int a) const { return w[2 * n2 + 3 * nt + a]; }
+    __device__ __forceinline__ int32_t adhesion(int ts, int tt, uint32_t hist, int as,
+                                                int at) const {
         int32_t adh = 0;
-        int as = 0, at = 0;
+        for (int t = 0; t < nt; t++)
+            adh += (J(ts, t) - JS(tt, t)) * (int32_t)((hist >> (5 * t)) & 31u);
+        return adh + at * JS(tt, tt) - as * J(ts, ts);
+    }
+};
+
+__device__ __forceinline__ int32_t pick4(int i, int32_t v0, int32_t v1, int32_t v2, int32_t v3) {
+    return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
+}
+
+// At most four types: the padded block in registers (uniform loads, the
+// same for every lane), each lane's entries picked by selects.
+struct SmallParams {
+    static constexpr bool kAlwaysHist = true;
+    static constexpr int S = kSmallTypes;
+    int32_t v[kSmallWords];
+    __device__ __forceinline__ void load(const DevParams* prm) {
+#pragma unroll
+        for (int i = 0; i < kSmallWords; i++) v[i] = prm->s4[i];
+    }
+    __device__ __forceinline__ bool hist() const { return true; }
+    __device__ __forceinline__ int32_t f(int field, int a) const {
+        const int o = 2 * S * S + field * S;
+        return pick4(a, v[o], v[o + 1], v[o + 2], v[o + 3]);
+    }
+    __device__ __forceinline__ int32_t la(int a) const { return f(0, a); }
+    __device__ __forceinline__ int32_t a0(int a) const { return f(1, a); }
+    __device__ __forceinline__ int32_t lp(int a) const { return f(2, a); }
+    __device__ __forceinline__ int32_t p0(int a) const { return f(3, a); }
+    __device__ __forceinline__ int32_t J(int, int) const { return 0; }  // unused
+    __device__ __forceinline__ int32_t JS(int, int) const { return 0; }
+    __device__ __forceinline__ int32_t adhesion(int ts, int tt, uint32_t hist, int as,
+                                                int at) const {
+        int32_t adh = 0;
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            const int32_t js = pick4(ts, v[t], v[S + t], v[2 * S + t], v[3 * S + t]);
+            const int32_t jt = pick4(tt, v[S * S + t], v[S * S + S + t], v[S * S + 2 * S + t],
+                                     v[S * S + 3 * S + t]);
+            adh += (js - jt) * (int32_t)((hist >> (5 * t)) & 31u);
+        }
+        const int32_t jss = pick4(ts, v[0], v[S + 1], v[2 * S + 2], v[3 * S + 3]);
+        const int32_t jtt = pick4(tt, v[S * S], v[S * S + S + 1], v[S * S + 2 * S + 2],
+                                  v[S * S + 3 * S + 3]);
+        return adh + at * jtt - as * jss;
+    }
+};
+
+template <int D, bool CO, class Lat, class PV>
+__device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts& k, const PV& P,
+                                               const uint32_t* __restrict__ thr,
+                                               const PhaseTables& T, const CellSync& cy,
+                                               uint32_t sb, const Draw<D, Lat>& d) {
+    constexpr int N = D == 3 ? 26 : 8;
+    constexpr int NB = N + 1;
+    constexpr int CENTRE = NB / 2;
+    const uint32_t vt = d.nv[CENTRE];
+    uint32_t vs = d.vs;
+    if (vs == vt) return;  // kernels2d.cu:276
+    if (k.fixed_mask) {    // fixed types (kernels2d.cu:281-290)
+        if ((k.fixed_mask >> (vt >> 24)) & 1u) return;  // a fixed target stays
+        if ((k.fixed_mask >> (vs >> 24)) & 1u) {        // a fixed source copies medium
+            vs = 0;
+            if (vs == vt) return;
+        }
+    }
+
+    const int ts = (int)(vs >> 24), tt = (int)(vt >> 24);
+    const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
+    // Issue the snapshot-table loads first: their latency overlaps the
+    // neighbourhood sums.
+    const int64_t row_s = cs ? ld<CO>(T.R + cs) : 0;
+    const int64_t row_t = ct ? ld<CO>(T.R + ct) : 0;
+    int32_t adh = 0;
+    int as = 0, at = 0;
+    if (PV::kAlwaysHist || P.hist()) {
+        uint32_t hist = 0;
 #pragma unroll
         for (int n = 0; n < NB; n++) {
             if (n == CENTRE) continue;
-            const uint32_t vn = nv[n];
-            const uint32_t tn = vn >> 24;
-            const int32_t js = Js[tn], jt = Jt[tn];
+            const uint32_t vn = d.nv[n];
+            hist += 1u << ((vn >> 24) * 5u);
+            as += vn == vs;
+            at += vn == vt;
+        }
+        adh = P.adhesion(ts, tt, hist, as, at);
+    } else {
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            if (n == CENTRE) continue;
+            const uint32_t vn = d.nv[n];
+            const int tn = (int)(vn >> 24);
+            const int32_t js = P.J(ts, tn), jt = P.JS(tt, tn);
             const bool eqs = vn == vs, eqt = vn == vt;
             adh += (eqs ? 0 : js) - (eqt ? 0 : jt);
             as += eqs;
             at += eqt;
         }
-        int64_t dh = adh;
-        if (cs) {
-            const int64_t A = (int64_t)(uint32_t)row_s, Pm = row_s >> 32;
-            const int64_t dP = N - 2 * as;
-            dh += (int64_t)P->la[ts] * (1 + 2 * A - 2 * (int64_t)P->a0[ts]);
-            dh += (int64_t)P->lp[ts] * dP * (dP + 2 * (Pm - P->p0[ts]));
+    }
+    int64_t dh = adh;
+    if (cs) {
+        const int64_t A = (int64_t)(uint32_t)row_s, Pm = row_s >> 32;
+        const int64_t dP = N - 2 * as;
+        dh += (int64_t)P.la(ts) * (1 + 2 * A - 2 * (int64_t)P.a0(ts));
+        dh += (int64_t)P.lp(ts) * dP * (dP + 2 * (Pm - P.p0(ts)));
+    }
+    if (ct) {
+        const int64_t A = (int64_t)(uint32_t)row_t, Pm = row_t >> 32;
+        const int64_t dP = 2 * at - N;
+        dh += (int64_t)P.la(tt) * (1 - 2 * A + 2 * (int64_t)P.a0(tt));
+        dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
+    }
+    const bool accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
+    if (!accept) return;
+    if (cy.pass) {
+        const uint64_t key = ((uint64_t)cy.tag << 32) | (uint64_t)(0xFFFFFFFFu - sb);
+        if (cy.pass == 1) {
+            if (cs) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + cs), key);
+            if (ct) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + ct), key);
+            return;
         }
-        if (ct) {
-            const int64_t A = (int64_t)(uint32_t)row_t, Pm = row_t >> 32;
-            const int64_t dP = 2 * at - N;
-            dh += (int64_t)P->la[tt] * (1 - 2 * A + 2 * (int64_t)P->a0[tt]);
-            dh += (int64_t)P->lp[tt] * dP * (dP + 2 * (Pm - P->p0[tt]));
-        }
-        const bool accept = dh <= 0 || (dh < k.nthr && w[2] < thr[dh]);
-        if (!accept) continue;
-        if (cy.pass) {
-            const uint64_t key = ((uint64_t)cy.tag << 32) | (uint64_t)(0xFFFFFFFFu - sb);
-            if (cy.pass == 1) {
-                if (cs) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + cs), key);
-                if (ct) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + ct), key);
-                continue;
-            }
-            if ((cs && cy.claims[cs] != key) || (ct && cy.claims[ct] != key)) continue;
-        }
+        if ((cs && cy.claims[cs] != key) || (ct && cy.claims[ct] != key)) return;
+    }
 
-        lat.put(t, vs);  // kernels2d.cu:548
-        // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums
-        // are rebuilt from the lattice when read.
-        if (cs) {
-            const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
-            atomic_add_i64(T.W + cs, delta);
-            atomic_add_i64(T.Dc + cs, delta);
-        }
-        if (ct) {
-            const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
-            atomic_add_i64(T.W + ct, delta);
-            atomic_add_i64(T.Dc + ct, delta);
-        }
+    lat.put(d.t, vs);  // kernels2d.cu:548
+    // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums are
+    // rebuilt from the lattice when read.
+    if (cs) {
+        const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
+        atomic_add_i64(T.W + cs, delta);
+        atomic_add_i64(T.Dc + cs, delta);
+    }
+    if (ct) {
+        const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
+        atomic_add_i64(T.W + ct, delta);
+        atomic_add_i64(T.Dc + ct, delta);
+    }
+}
+
+// The u copy attempts of sub-block (bx, by, bz) in one colour phase, in
+// order.
+template <int D, bool CO, class Lat, class PV>
+__device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k,
+                                                   const PV& P, const uint32_t* __restrict__ thr,
+                                                   int bx, int by, int bz, uint32_t mcs,
+                                                   uint32_t phase, const PhaseTables& T,
+                                                   const CellSync cy = CellSync{0, nullptr, 0}) {
+    const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
+    if (k.u == 1) {
+        // One attempt, straight-line: with no loop the compiler keeps no
+        // loop-invariant values alive (they would spill scalar registers).
+        Draw<D, Lat> d;
+        draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, 0u, d);
+        finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
+        return;
+    }
+    for (int i = 0; i < k.u; i++) {
+        Draw<D, Lat> d;
+        draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, (uint32_t)i, d);
+        finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
     }
 }
 
 // Parameters and (when short) the acceptance table staged in LDS.
 constexpr int kThrLds = 1024;
 struct LdsParams {
-    DevParams p;
+    int32_t p[kParamWords];
     uint32_t thr[kThrLds];
 };
 
-__device__ __forceinline__ const uint32_t* stage_params(LdsParams& sp, const SweepConsts& k) {
-    // Only the nt types present are staged: 2 nt^2 pair entries and 4 nt
-    // per-type entries.  All global loads are issued before any LDS store.
-    const int nt = k.nt, n2 = nt * nt;
-    const int npar = 2 * n2 + 4 * nt;
-    const int i = threadIdx.x;
-    const bool fits = k.nthr <= kThrLds;
-    // word i of the compact list -> offset in DevParams
-    auto src_of = [&](int w) -> int {
-        if (w < n2) return w;                                   // J
-        if (w < 2 * n2) return kNTypes * kNTypes + (w - n2);    // JS
-        const int r = w - 2 * n2, f = r / nt, t = r - f * nt;   // la, a0, lp, p0
-        return 2 * kNTypes * kNTypes + f * kNTypes + t;
-    };
-    const int* src = reinterpret_cast<const int*>(k.prm);
-    int* dst = reinterpret_cast<int*>(&sp.p);
-    constexpr int RP = (2 * kNTypes * kNTypes + 4 * kNTypes + kBlock - 1) / kBlock;
+// All global loads are issued before any LDS store.
+__device__ __forceinline__ const uint32_t* stage_params(LdsParams& sp, const SweepConsts& k,
+                                                        LdsParamView* view) {
+    constexpr int RP = (kParamWords + kBlock - 1) / kBlock;
     constexpr int RT = kThrLds / kBlock;
-    int pv[RP];
+    const int npar = param_words(k.nt);
+    const bool fits = k.nthr <= kThrLds;
+    const int32_t* src = k.prm->w;
+    int32_t pv[RP];
     uint32_t tv[RT];
 #pragma unroll
     for (int j = 0; j < RP; j++) {
-        const int w = i + j * kBlock;
-        pv[j] = w < npar ? src[src_of(w)] : 0;
+        const int w = threadIdx.x + j * kBlock;
+        pv[j] = w < npar ? src[w] : 0;
     }
 #pragma unroll
     for (int j = 0; j < RT; j++) {
-        const int w = i + j * kBlock;
+        const int w = threadIdx.x + j * kBlock;
         tv[j] = fits && w < k.nthr ? k.thr[w] : 0u;
     }
 #pragma unroll
     for (int j = 0; j < RP; j++) {
-        const int w = i + j * kBlock;
-        if (w < npar) dst[src_of(w)] = pv[j];
+        const int w = threadIdx.x + j * kBlock;
+        if (w < npar) sp.p[w] = pv[j];
     }
 #pragma unroll
     for (int j = 0; j < RT; j++) {
-        const int w = i + j * kBlock;
+        const int w = threadIdx.x + j * kBlock;
         if (fits && w < k.nthr) sp.thr[w] = tv[j];
     }
     __syncthreads();
+    *view = LdsParamView{sp.p, k.nt, k.nt * k.nt};
     return fits ? sp.thr : k.thr;
 }
 
@@ -360,24 +529,212 @@ __device__ __forceinline__ void colour_of(int col, int& cx, int& cy, int& cz) {
 // One colour phase.  Thread q owns same-colour sub-block q and makes its u
 // copy attempts in order; sub-blocks of one colour are two sub-blocks apart,
 // so no attempt reads a site another thread writes in this phase.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
+// One colour phase.  Thread q owns same-colour sub-block q and makes its u
+// copy attempts in order; sub-blocks of one colour are two sub-blocks apart,
+// so no attempt reads a site another thread writes in this phase.  The
+// first ncells + 1 threads also fold the previous phase's deltas (their
+// loads are issued before the attempts, the adds made after them).
+template <int D, class PV>
+__device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P,
+                                                 const uint32_t* __restrict__ thr) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
     const SweepConsts& k = a.k;
-    __shared__ LdsParams sp;
-    const uint32_t* thr = stage_params(sp, k);
     const PhaseTables T = tables_of(k, a.g);
+    const bool fold = a.sync_pass != 1;
+    const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
     if (gtid < a.nsub) {
         const int hm = (1 << a.lh) - 1;
         const int bx = 2 * (gtid & hm) + a.cx;
         const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
         const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, a.mcs, a.phase, T,
+        sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
                                      CellSync{a.sync_pass, a.claims, a.sync_tag});
     }
-    if (a.sync_pass != 1) fold_tables<false>(T, k.ncells, gtid, gsz);
+    if (fold) {
+        if (dv != 0) {
+            atomic_add_i64(T.W + gtid, dv);
+            T.Dp[gtid] = 0;
+        }
+        for (int ci = gtid + gsz; ci <= k.ncells; ci += gsz) {
+            const int64_t v = T.Dp[ci];
+            if (v != 0) {
+                atomic_add_i64(T.W + ci, v);
+                T.Dp[ci] = 0;
+            }
+        }
+    }
+}
+
+// At most four types: parameters from uniform (scalar) loads, thresholds
+// read from global memory; no staging and no block barrier.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
+    SmallParams P;
+    P.load(a.k.prm);
+    sweep_phase_body<D>(a, P, a.k.thr);
+}
+
+// One attempt per sub-block (u = 1), with the attempts compacted: every
+// thread draws its attempt and loads target and source (in 2D the whole
+// neighbourhood); only the attempts whose source and target differ (most do
+// not: cell interiors and medium) are queued in LDS, and the block's threads
+// then finish the queued attempts densely.  Parameters and thresholds are
+// staged in LDS while the draws' loads are in flight.  The result does not
+// depend on which thread finishes an attempt.
+template <int D, bool FULL>
+struct QueuedAttempt {
+    static constexpr int NQ = FULL ? (D == 2 ? 9 : 27) : 0;  // neighbourhood carried
+    uint32_t site;  // target site index
+    uint32_t vs;    // source value
+    uint32_t w2;    // acceptance draw
+    uint32_t sb;    // sub-block id
+    uint32_t nv[NQ > 0 ? NQ : 1];
+};
+
+template <int D, bool STAGE, bool FULL>
+__global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
+    __shared__ LdsParams sp;
+    __shared__ QueuedAttempt<D, FULL> queue[kBlock];
+    __shared__ uint32_t nqueued;
+    constexpr int NQ = QueuedAttempt<D, FULL>::NQ;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gsz = gridDim.x * blockDim.x;
+    const SweepConsts& k = a.k;
+    const PhaseTables T = tables_of(k, a.g);
+    // Loads in flight together: parameters, the fold's deltas, the draw.
+    constexpr int RP = (kParamWords + kBlock - 1) / kBlock;
+    constexpr int RT = kThrLds / kBlock;
+    const int npar = STAGE ? param_words(k.nt) : 0;
+    const bool thr_lds = STAGE && k.nthr <= kThrLds;
+    int32_t pv[RP];
+    uint32_t tv[RT];
+#pragma unroll
+    for (int j = 0; j < RP; j++) {
+        const int w = threadIdx.x + j * kBlock;
+        pv[j] = w < npar ? k.prm->w[w] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < RT; j++) {
+        const int w = threadIdx.x + j * kBlock;
+        tv[j] = thr_lds && w < k.nthr ? k.thr[w] : 0u;
+    }
+    const bool fold = a.sync_pass != 1;
+    const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
+    using Lat = GlobalLat<D, false>;
+    const Lat lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+    // Draw (kernels2d.cu:199-243 in the shared roles).
+    bool active = false;
+    QueuedAttempt<D, FULL> e;
+    if (gtid < a.nsub) {
+        const int hm = (1 << a.lh) - 1;
+        const int bx = 2 * (gtid & hm) + a.cx;
+        const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
+        const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
+        const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
+        uint32_t w[4];
+        philox4x32_10(sb, a.mcs, a.phase, 0u, k.k0, k.k1, w);
+        const int lc = k.lc;
+        const uint32_t cm = (1u << lc) - 1u;
+        const int tx = (bx << lc) + (int)(w[0] & cm);
+        const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
+        const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
+        const typename Lat::Pos t = lat.at(tx, ty, tz);
+        constexpr int N = D == 3 ? 26 : 8;
+        int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+        dir += dir >= N / 2;
+        uint32_t vs, vt;
+        if constexpr (NQ > 0) {
+            lat.block(t, e.nv);
+            vs = 0;
+#pragma unroll
+            for (int n = 0; n < NQ; n++) vs = n == dir ? e.nv[n] : vs;
+            vt = e.nv[NQ / 2];
+        } else {
+            int sx, sy, sz;
+            block_offset<D>(dir, sx, sy, sz);
+            vs = lat.nb(t, sx, sy, sz);
+            vt = lat.self(t);
+        }
+        active = vs != vt;  // kernels2d.cu:276
+        e.site = t.i;
+        e.vs = vs;
+        e.w2 = w[2];
+        e.sb = sb;
+    }
+#pragma unroll
+    for (int j = 0; j < RP; j++) {
+        const int w = threadIdx.x + j * kBlock;
+        if (w < npar) sp.p[w] = pv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < RT; j++) {
+        const int w = threadIdx.x + j * kBlock;
+        if (thr_lds && w < k.nthr) sp.thr[w] = tv[j];
+    }
+    if (threadIdx.x == 0) nqueued = 0;
+    __syncthreads();
+    // Queue the active attempts: one LDS atomic per wave.
+    const uint64_t mask = __ballot(active);
+    if (mask != 0) {
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)__lane_id() == leader) base = atomicAdd(&nqueued, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if (active) queue[base + before] = e;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < (int)nqueued) {
+        const QueuedAttempt<D, FULL>& q = queue[threadIdx.x];
+        const int lgL = k.lgL, Lm = (1 << lgL) - 1;
+        const typename Lat::Pos t{(int)(q.site & Lm), (int)((q.site >> lgL) & Lm),
+                                  D == 3 ? (int)(q.site >> (2 * lgL)) : 0, q.site};
+        Draw<D, Lat> d;
+        d.w[0] = d.w[1] = d.w[3] = 0u;
+        d.w[2] = q.w2;
+        d.t = t;
+        if constexpr (NQ > 0) {
+#pragma unroll
+            for (int n = 0; n < NQ; n++) d.nv[n] = q.nv[n];
+        } else {
+            lat.block(t, d.nv);
+        }
+        d.vs = q.vs;
+        const CellSync cy{a.sync_pass, a.claims, a.sync_tag};
+        if constexpr (STAGE) {
+            const LdsParamView P{sp.p, k.nt, k.nt * k.nt};
+            finish_attempt<D, false>(lat, k, P, thr_lds ? sp.thr : k.thr, T, cy, q.sb, d);
+        } else {
+            SmallParams P;
+            P.load(k.prm);
+            finish_attempt<D, false>(lat, k, P, k.thr, T, cy, q.sb, d);
+        }
+    }
+    if (fold) {
+        if (dv != 0) {
+            atomic_add_i64(T.W + gtid, dv);
+            T.Dp[gtid] = 0;
+        }
+        for (int ci = gtid + gsz; ci <= k.ncells; ci += gsz) {
+            const int64_t v = T.Dp[ci];
+            if (v != 0) {
+                atomic_add_i64(T.W + ci, v);
+                T.Dp[ci] = 0;
+            }
+        }
+    }
+}
+
+// Any number of types: parameters and thresholds staged in LDS.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_phase_lds(PhaseArgs a) {
+    __shared__ LdsParams sp;
+    LdsParamView P;
+    const uint32_t* thr = stage_params(sp, a.k, &P);
+    sweep_phase_body<D>(a, P, thr);
 }
 
 // Batched replicas, one colour phase: block b works for replica rep_of[b]
@@ -388,7 +745,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
     const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
     const SweepConsts& k = rd.k;
     __shared__ LdsParams sp;
-    const uint32_t* thr = stage_params(sp, k);
+    LdsParamView P;
+    const uint32_t* thr = stage_params(sp, k, &P);
     const int gtid = (blockIdx.x - rd.block0) * blockDim.x + threadIdx.x;
     const int gsz = rd.nblocks * blockDim.x;
     const PhaseTables T = tables_of(k, a.g);
@@ -399,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
         const int by = 2 * ((gtid >> lh) & hm) + a.cy;
         const int bz = D == 3 ? 2 * (gtid >> (2 * lh)) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, &sp.p, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T);
+        sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T);
     }
     fold_tables<false>(T, k.ncells, gtid, gsz);
 }
@@ -437,13 +795,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase3_box(PhaseArgs a) {
         const int i = threadIdx.x + j * kBlock;
         if (i < NS) box[i] = v[j];
     }
-    const uint32_t* thr = stage_params(sp, k);  // ends with __syncthreads
+    LdsParamView P;
+    const uint32_t* thr = stage_params(sp, k, &P);  // ends with __syncthreads
     // kBX/2 * kBY/2 * kBZ/2 = 256 active sites of this colour per box
     const int t = threadIdx.x;
     const int qx = t % (kBX / 2), qy = (t / (kBX / 2)) % (kBY / 2), qz = t / ((kBX / 2) * (kBY / 2));
     const LdsLat3 lat{box, k.lat, SX, SXY, x0, y0, z0, lgL};
     const PhaseTables T = tables_of(k, a.g);
-    sub_block_attempts<3, false>(lat, k, &sp.p, thr, x0 + 2 * qx + a.cx, y0 + 2 * qy + a.cy,
+    sub_block_attempts<3, false>(lat, k, P, thr, x0 + 2 * qx + a.cx, y0 + 2 * qy + a.cy,
                                  z0 + 2 * qz + a.cz, a.mcs, a.phase, T);
     fold_tables<false>(T, k.ncells, b * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
@@ -507,7 +866,8 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
     const int gtid = lb * blockDim.x + threadIdx.x;
     const int gsz = nblocks * blockDim.x;
     unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t* thr = stage_params(sp, k);
+    LdsParamView P;
+    const uint32_t* thr = stage_params(sp, k, &P);
     const GlobalLat<D, true> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
     const int hm = (1 << lh) - 1;
     const int nsub = 1 << (D * lh);
@@ -527,7 +887,7 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
                     const int bx = 2 * (q & hm) + cx;
                     const int by = 2 * ((q >> lh) & hm) + cy;
                     const int bz = D == 3 ? 2 * (q >> (2 * lh)) + cz : 0;
-                    sub_block_attempts<D, true>(lat, k, &sp.p, thr, bx, by, bz, mcs, phase, T);
+                    sub_block_attempts<D, true>(lat, k, P, thr, bx, by, bz, mcs, phase, T);
                 }
             }
         }
@@ -573,7 +933,8 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
     const int Lm = (1 << k.lgL) - 1;
     const uint32_t* __restrict__ gl = k.lat;
     __shared__ LdsParams sp;
-    const uint32_t* thr = stage_params(sp, k);
+    LdsParamView P;
+    const uint32_t* thr = stage_params(sp, k, &P);
     // tile + halo
     for (int i = threadIdx.x; i < S * (TY + 2); i += blockDim.x) {
         const int ly = i / S, lx = i - ly * S;
@@ -610,7 +971,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
                 for (int q = threadIdx.x; q < nq; q += blockDim.x) {
                     const int bx = (x0 >> k.lc) + 2 * (q & ((1 << lq) - 1)) + cx;
                     const int by = (y0 >> k.lc) + 2 * (q >> lq) + cy;
-                    sub_block_attempts<2, true>(lat, k, &sp.p, thr, bx, by, 0, mcs, phase, T);
+                    sub_block_attempts<2, true>(lat, k, P, thr, bx, by, 0, mcs, phase, T);
                 }
             }
         }
@@ -757,23 +1118,65 @@ hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab,
     return hipGetLastError();
 }
 
-hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    const int work = a.nsub > a.k.ncells + 1 ? a.nsub : a.k.ncells + 1;
-    const int blocks = (work + kBlock - 1) / kBlock;
-    // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
-    // (null events: plain launch).
+// The per-phase kernel a launch uses (results never depend on the choice).
+enum PhaseKernel {
+    kPkBox3, kPkCompact2Small, kPkCompact2SmallFull, kPkCompact2Lds, kPkCompact2LdsFull,
+    kPkCompact3Small, kPkCompact3Lds, kPkPhase2Small, kPkPhase2Lds, kPkPhase3Small, kPkPhase3Lds
+};
+
+const char* phase_kernel_name(int id) {
+    static const char* const names[] = {
+        "k_sweep_phase3_box",
+        "k_sweep_phase_compact<2, false, false>", "k_sweep_phase_compact<2, false, true>",
+        "k_sweep_phase_compact<2, true, false>", "k_sweep_phase_compact<2, true, true>",
+        "k_sweep_phase_compact<3, false, false>", "k_sweep_phase_compact<3, true, false>",
+        "k_sweep_phase<2>", "k_sweep_phase_lds<2>", "k_sweep_phase<3>", "k_sweep_phase_lds<3>"};
+    return id >= 0 && id <= kPkPhase3Lds ? names[id] : "";
+}
+
+static int choose_phase_kernel(int d, const PhaseArgs& a) {
     const int L = 1 << a.k.lgL;
     if (d == 3 && a.k.lc == 0 && L >= kBX && L >= kBY && L >= kBZ && a.box3 && !a.sync_pass) {
-        const int nbox = (L / kBX) * (L / kBY) * (L / kBZ);
-        if ((long)nbox * kBlock >= a.k.ncells + 1) {
-            hipExtLaunchKernelGGL(k_sweep_phase3_box, dim3(nbox), dim3(kBlock), 0, s, e0, e1, 0, a);
-            return hipGetLastError();
-        }
+        const long nbox = (long)(L / kBX) * (L / kBY) * (L / kBZ);
+        if (nbox * kBlock >= a.k.ncells + 1) return kPkBox3;
     }
-    if (d == 2)
-        hipExtLaunchKernelGGL(k_sweep_phase<2>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
-    else
-        hipExtLaunchKernelGGL(k_sweep_phase<3>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
+    const bool small = a.k.nt <= kSmallTypes && (a.form & kFormSmall);
+    const bool compact = a.k.u == 1 && (a.form & kFormCompact);
+    const bool full = a.form & kFormFull;
+    if (compact) {
+        if (d == 2) return small ? (full ? kPkCompact2SmallFull : kPkCompact2Small)
+                                 : (full ? kPkCompact2LdsFull : kPkCompact2Lds);
+        return small ? kPkCompact3Small : kPkCompact3Lds;
+    }
+    if (d == 2) return small ? kPkPhase2Small : kPkPhase2Lds;
+    return small ? kPkPhase3Small : kPkPhase3Lds;
+}
+
+hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        int* kernel_id) {
+    const int work = a.nsub > a.k.ncells + 1 ? a.nsub : a.k.ncells + 1;
+    const int blocks = (work + kBlock - 1) / kBlock;
+    const int L = 1 << a.k.lgL;
+    const int id = choose_phase_kernel(d, a);
+    if (kernel_id) *kernel_id = id;
+    // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
+    // (null events: plain launch).
+#define CPM_LAUNCH(KERNEL, GRID) \
+    hipExtLaunchKernelGGL(KERNEL, dim3(GRID), dim3(kBlock), 0, s, e0, e1, 0, a)
+    switch (id) {
+        case kPkBox3: CPM_LAUNCH(k_sweep_phase3_box, (L / kBX) * (L / kBY) * (L / kBZ)); break;
+        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false>), blocks); break;
+        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true>), blocks); break;
+        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false>), blocks); break;
+        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true>), blocks); break;
+        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false>), blocks); break;
+        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false>), blocks); break;
+        case kPkPhase2Small: CPM_LAUNCH(k_sweep_phase<2>, blocks); break;
+        case kPkPhase2Lds: CPM_LAUNCH(k_sweep_phase_lds<2>, blocks); break;
+        case kPkPhase3Small: CPM_LAUNCH(k_sweep_phase<3>, blocks); break;
+        default: CPM_LAUNCH(k_sweep_phase_lds<3>, blocks); break;
+    }
+#undef CPM_LAUNCH
     return hipGetLastError();
 }
 

@@ -285,7 +285,7 @@ class Cpm:
         """Launch form of the last run: dict(form, kernel, blocks, tile)."""
         f, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib.cpm_last_launch(self._h, ctypes.byref(f), ctypes.byref(b), ctypes.byref(t)))
-        kernel = ["k_sweep_phase", "k_persist_global", "k_persist_lds2d"][f.value]
+        kernel = lib.cpm_last_kernel(self._h).decode()
         return {"form": f.value, "kernel": kernel, "blocks": b.value, "tile": t.value}
 
     def set_seed(self, seed):

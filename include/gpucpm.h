@@ -149,6 +149,9 @@ int cpm_timing_end(cpm_handle h, double* kernel_ms, int64_t* launches, int64_t* 
  * with 2D tiles resident in LDS; blocks = cooperative grid size; tile = LDS
  * tile edge (0 if none).  Results never depend on the form. */
 int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile);
+/* Extension: name of the sweep kernel the last run launched, as profilers
+ * show it (e.g. "k_sweep_phase_compact<2, true, false>"); "" before any run. */
+const char* cpm_last_kernel(cpm_handle h);
 
 /* Extension: choose the launch form (-1 = automatic, else a form number as
  * above) and the LDS tile edge (0 = automatic) for this engine's runs; the
