@@ -96,6 +96,8 @@ struct PhaseArgs {
 constexpr int kFormSmall = 1;    // read at most four types' parameters without LDS staging
 constexpr int kFormCompact = 2;  // u = 1: queue the attempts whose source and target differ
 constexpr int kFormFull = 4;     // compact: the draw loads (and queues) the whole neighbourhood
+constexpr int kFormAtt2 = 8;     // compact: each thread draws 2 attempts
+constexpr int kFormAtt4 = 16;    // compact: each thread draws 4 attempts
 
 // A whole run in one cooperative launch, phases separated by grid barriers.
 struct PersistArgs {

@@ -599,6 +599,13 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         pa.form = (env_flag("GPUCPM_SMALL", h->d == 3) ? gcpm::kFormSmall : 0) |
                   (env_flag("GPUCPM_COMPACT", true) ? gcpm::kFormCompact : 0) |
                   (env_flag("GPUCPM_FULL", false) ? gcpm::kFormFull : 0);
+        {
+            // attempts drawn per thread by the compacted kernel (measured:
+            // 4 in 3D, 1 in 2D)
+            const char* e = std::getenv("GPUCPM_ATT");
+            const int att = e ? std::atoi(e) : (h->d == 3 ? 4 : 1);
+            pa.form |= att == 4 ? gcpm::kFormAtt4 : att == 2 ? gcpm::kFormAtt2 : 0;
+        }
         const int ncol = 1 << h->d;
         int g = 0;  // phase counter since the tables were last settled
         for (long m = 0; m < total; m++) {

@@ -593,17 +593,21 @@ struct QueuedAttempt {
     uint32_t nv[NQ > 0 ? NQ : 1];
 };
 
-template <int D, bool STAGE, bool FULL>
+// Thread t of block b draws the attempts of same-colour sub-blocks
+// (b ATT + r) blockDim + t for r < ATT (consecutive threads, consecutive
+// sub-blocks), so the block's queue collects up to ATT blockDim attempts.
+template <int D, bool STAGE, bool FULL, int ATT>
 __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
+    static_assert(!FULL || ATT == 1, "the full-neighbourhood queue holds one attempt per thread");
     __shared__ LdsParams sp;
-    __shared__ QueuedAttempt<D, FULL> queue[kBlock];
+    __shared__ QueuedAttempt<D, FULL> queue[kBlock * ATT];
     __shared__ uint32_t nqueued;
     constexpr int NQ = QueuedAttempt<D, FULL>::NQ;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
     const SweepConsts& k = a.k;
     const PhaseTables T = tables_of(k, a.g);
-    // Loads in flight together: parameters, the fold's deltas, the draw.
+    // Loads in flight together: parameters, the fold's deltas, the draws.
     constexpr int RP = (kParamWords + kBlock - 1) / kBlock;
     constexpr int RT = kThrLds / kBlock;
     const int npar = STAGE ? param_words(k.nt) : 0;
@@ -624,44 +628,49 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
     const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
     using Lat = GlobalLat<D, false>;
     const Lat lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-    // Draw (kernels2d.cu:199-243 in the shared roles).
-    bool active = false;
-    QueuedAttempt<D, FULL> e;
-    if (gtid < a.nsub) {
-        const int hm = (1 << a.lh) - 1;
-        const int bx = 2 * (gtid & hm) + a.cx;
-        const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
-        const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
-        const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
-        uint32_t w[4];
-        philox4x32_10(sb, a.mcs, a.phase, 0u, k.k0, k.k1, w);
-        const int lc = k.lc;
-        const uint32_t cm = (1u << lc) - 1u;
-        const int tx = (bx << lc) + (int)(w[0] & cm);
-        const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
-        const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
-        const typename Lat::Pos t = lat.at(tx, ty, tz);
-        constexpr int N = D == 3 ? 26 : 8;
-        int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
-        dir += dir >= N / 2;
-        uint32_t vs, vt;
-        if constexpr (NQ > 0) {
-            lat.block(t, e.nv);
-            vs = 0;
+    // Draws (kernels2d.cu:199-243 in the shared roles).
+    bool active[ATT];
+    QueuedAttempt<D, FULL> e[ATT];
 #pragma unroll
-            for (int n = 0; n < NQ; n++) vs = n == dir ? e.nv[n] : vs;
-            vt = e.nv[NQ / 2];
-        } else {
-            int sx, sy, sz;
-            block_offset<D>(dir, sx, sy, sz);
-            vs = lat.nb(t, sx, sy, sz);
-            vt = lat.self(t);
+    for (int r = 0; r < ATT; r++) {
+        const int q = ((int)blockIdx.x * ATT + r) * kBlock + (int)threadIdx.x;
+        active[r] = false;
+        if (q < a.nsub) {
+            const int hm = (1 << a.lh) - 1;
+            const int bx = 2 * (q & hm) + a.cx;
+            const int by = 2 * ((q >> a.lh) & hm) + a.cy;
+            const int bz = D == 3 ? 2 * (q >> (2 * a.lh)) + a.cz : 0;
+            const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
+            uint32_t w[4];
+            philox4x32_10(sb, a.mcs, a.phase, 0u, k.k0, k.k1, w);
+            const int lc = k.lc;
+            const uint32_t cm = (1u << lc) - 1u;
+            const int tx = (bx << lc) + (int)(w[0] & cm);
+            const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
+            const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
+            const typename Lat::Pos t = lat.at(tx, ty, tz);
+            constexpr int N = D == 3 ? 26 : 8;
+            int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+            dir += dir >= N / 2;
+            uint32_t vs, vt;
+            if constexpr (NQ > 0) {
+                lat.block(t, e[r].nv);
+                vs = 0;
+#pragma unroll
+                for (int n = 0; n < NQ; n++) vs = n == dir ? e[r].nv[n] : vs;
+                vt = e[r].nv[NQ / 2];
+            } else {
+                int sx, sy, sz;
+                block_offset<D>(dir, sx, sy, sz);
+                vs = lat.nb(t, sx, sy, sz);
+                vt = lat.self(t);
+            }
+            active[r] = vs != vt;  // kernels2d.cu:276
+            e[r].site = t.i;
+            e[r].vs = vs;
+            e[r].w2 = w[2];
+            e[r].sb = sb;
         }
-        active = vs != vt;  // kernels2d.cu:276
-        e.site = t.i;
-        e.vs = vs;
-        e.w2 = w[2];
-        e.sb = sb;
     }
 #pragma unroll
     for (int j = 0; j < RP; j++) {
@@ -675,20 +684,24 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
     }
     if (threadIdx.x == 0) nqueued = 0;
     __syncthreads();
-    // Queue the active attempts: one LDS atomic per wave.
-    const uint64_t mask = __ballot(active);
-    if (mask != 0) {
-        const uint32_t before = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if ((int)__lane_id() == leader) base = atomicAdd(&nqueued, (uint32_t)__popcll(mask));
-        base = __shfl(base, leader);
-        if (active) queue[base + before] = e;
+    // Queue the active attempts: one LDS atomic per wave and round.
+#pragma unroll
+    for (int r = 0; r < ATT; r++) {
+        const uint64_t mask = __ballot(active[r]);
+        if (mask != 0) {
+            const uint32_t before = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&nqueued, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader);
+            if (active[r]) queue[base + before] = e[r];
+        }
     }
     __syncthreads();
-    if ((int)threadIdx.x < (int)nqueued) {
-        const QueuedAttempt<D, FULL>& q = queue[threadIdx.x];
+    const int nq = (int)nqueued;
+    for (int j = threadIdx.x; j < nq; j += kBlock) {
+        const QueuedAttempt<D, FULL>& q = queue[j];
         const int lgL = k.lgL, Lm = (1 << lgL) - 1;
         const typename Lat::Pos t{(int)(q.site & Lm), (int)((q.site >> lgL) & Lm),
                                   D == 3 ? (int)(q.site >> (2 * lgL)) : 0, q.site};
@@ -1121,17 +1134,20 @@ hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab,
 // The per-phase kernel a launch uses (results never depend on the choice).
 enum PhaseKernel {
     kPkBox3, kPkCompact2Small, kPkCompact2SmallFull, kPkCompact2Lds, kPkCompact2LdsFull,
-    kPkCompact3Small, kPkCompact3Lds, kPkPhase2Small, kPkPhase2Lds, kPkPhase3Small, kPkPhase3Lds
+    kPkCompact3Small, kPkCompact3Lds, kPkPhase2Small, kPkPhase2Lds, kPkPhase3Small, kPkPhase3Lds,
+    kPkCompact2Lds2, kPkCompact2Lds4, kPkCompact3Small2, kPkCompact3Small4, kPkCount
 };
 
 const char* phase_kernel_name(int id) {
     static const char* const names[] = {
         "k_sweep_phase3_box",
-        "k_sweep_phase_compact<2, false, false>", "k_sweep_phase_compact<2, false, true>",
-        "k_sweep_phase_compact<2, true, false>", "k_sweep_phase_compact<2, true, true>",
-        "k_sweep_phase_compact<3, false, false>", "k_sweep_phase_compact<3, true, false>",
-        "k_sweep_phase<2>", "k_sweep_phase_lds<2>", "k_sweep_phase<3>", "k_sweep_phase_lds<3>"};
-    return id >= 0 && id <= kPkPhase3Lds ? names[id] : "";
+        "k_sweep_phase_compact<2, false, false, 1>", "k_sweep_phase_compact<2, false, true, 1>",
+        "k_sweep_phase_compact<2, true, false, 1>", "k_sweep_phase_compact<2, true, true, 1>",
+        "k_sweep_phase_compact<3, false, false, 1>", "k_sweep_phase_compact<3, true, false, 1>",
+        "k_sweep_phase<2>", "k_sweep_phase_lds<2>", "k_sweep_phase<3>", "k_sweep_phase_lds<3>",
+        "k_sweep_phase_compact<2, true, false, 2>", "k_sweep_phase_compact<2, true, false, 4>",
+        "k_sweep_phase_compact<3, false, false, 2>", "k_sweep_phase_compact<3, false, false, 4>"};
+    return id >= 0 && id < kPkCount ? names[id] : "";
 }
 
 static int choose_phase_kernel(int d, const PhaseArgs& a) {
@@ -1143,7 +1159,10 @@ static int choose_phase_kernel(int d, const PhaseArgs& a) {
     const bool small = a.k.nt <= kSmallTypes && (a.form & kFormSmall);
     const bool compact = a.k.u == 1 && (a.form & kFormCompact);
     const bool full = a.form & kFormFull;
+    const int att = (a.form & kFormAtt4) ? 4 : (a.form & kFormAtt2) ? 2 : 1;
     if (compact) {
+        if (d == 2 && !small && !full && att > 1) return att == 4 ? kPkCompact2Lds4 : kPkCompact2Lds2;
+        if (d == 3 && small && att > 1) return att == 4 ? kPkCompact3Small4 : kPkCompact3Small2;
         if (d == 2) return small ? (full ? kPkCompact2SmallFull : kPkCompact2Small)
                                  : (full ? kPkCompact2LdsFull : kPkCompact2Lds);
         return small ? kPkCompact3Small : kPkCompact3Lds;
@@ -1163,14 +1182,24 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
     // (null events: plain launch).
 #define CPM_LAUNCH(KERNEL, GRID) \
     hipExtLaunchKernelGGL(KERNEL, dim3(GRID), dim3(kBlock), 0, s, e0, e1, 0, a)
+    // blocks of kBlock threads drawing att attempts each
+    auto cblocks = [&](int att) {
+        const int per = att * kBlock;
+        const int nb = (a.nsub + per - 1) / per;
+        return nb > 0 ? nb : 1;
+    };
     switch (id) {
         case kPkBox3: CPM_LAUNCH(k_sweep_phase3_box, (L / kBX) * (L / kBY) * (L / kBZ)); break;
-        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false>), blocks); break;
-        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true>), blocks); break;
-        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false>), blocks); break;
-        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true>), blocks); break;
-        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false>), blocks); break;
-        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false>), blocks); break;
+        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false, 1>), blocks); break;
+        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true, 1>), blocks); break;
+        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 1>), blocks); break;
+        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true, 1>), blocks); break;
+        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 1>), blocks); break;
+        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false, 1>), blocks); break;
+        case kPkCompact2Lds2: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 2>), cblocks(2)); break;
+        case kPkCompact2Lds4: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 4>), cblocks(4)); break;
+        case kPkCompact3Small2: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 2>), cblocks(2)); break;
+        case kPkCompact3Small4: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4>), cblocks(4)); break;
         case kPkPhase2Small: CPM_LAUNCH(k_sweep_phase<2>, blocks); break;
         case kPkPhase2Lds: CPM_LAUNCH(k_sweep_phase_lds<2>, blocks); break;
         case kPkPhase3Small: CPM_LAUNCH(k_sweep_phase<3>, blocks); break;

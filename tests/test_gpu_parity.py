@@ -220,9 +220,11 @@ def test_fixed_wall_bit_exact(mode, d, c, u, monkeypatch):
     assert np.count_nonzero(lat != wl.lattice) > 0
 
 
-@pytest.mark.parametrize("small,compact,full", [("0", "0", "0"), ("1", "0", "0"), ("1", "1", "0"),
-                                                ("1", "1", "1"), ("0", "1", "0"), ("0", "1", "1")])
-def test_phase_kernel_forms_agree_with_oracle(small, compact, full, monkeypatch):
+@pytest.mark.parametrize("small,compact,full,att", [
+    ("0", "0", "0", "1"), ("1", "0", "0", "1"), ("1", "1", "0", "1"), ("1", "1", "1", "1"),
+    ("0", "1", "0", "1"), ("0", "1", "1", "1"), ("0", "1", "0", "2"), ("0", "1", "0", "4"),
+    ("1", "1", "0", "2"), ("1", "1", "0", "4")])
+def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, monkeypatch):
     """The per-phase kernel forms (parameters staged in LDS or read
     directly; attempts finished in place or queued and finished densely)
     reproduce the oracle in 2D and 3D, with and without a fixed wall."""
@@ -230,13 +232,14 @@ def test_phase_kernel_forms_agree_with_oracle(small, compact, full, monkeypatch)
     monkeypatch.setenv("GPUCPM_SMALL", small)
     monkeypatch.setenv("GPUCPM_COMPACT", compact)
     monkeypatch.setenv("GPUCPM_FULL", full)
+    monkeypatch.setenv("GPUCPM_ATT", att)
     cases = [(W.sorting2d(L=128, grid=9, edge=10, seed=61), 1, 1, 30),
              (W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=62)), 1, 1, 30),
              (W.sorting2d(L=128, grid=9, edge=10, seed=63), 2, 4, 20),
              (W.plain3d(L=32, ncells=60, seed=64), 1, 1, 8),
              (W.walled(W.plain3d(L=32, ncells=60, seed=65)), 1, 1, 8)]
     for wl, c, u, n in cases:
-        _announce("forms", small, compact, full, wl.name, wl.L, c, u)
+        _announce("forms", small, compact, full, att, wl.name, wl.L, c, u)
         _, lat, table = _run_gpu(wl, c, u, seed=71, nmcs=n)
         lat_o, tab_o = run_oracle(wl, c, u, 71, n)
         np.testing.assert_array_equal(lat, lat_o)
