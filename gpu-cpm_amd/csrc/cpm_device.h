@@ -33,6 +33,9 @@ constexpr int kSmallWords = 2 * kSmallTypes * kSmallTypes + 4 * kSmallTypes;
 struct DevParams {
     int32_t w[kParamWords];
     int32_t s4[kSmallWords];
+    // Chemotaxis multipliers llround(lambda * scale * 2^8 / sqrt(k)) for the
+    // neighbour classes k = |o|^2 = 1, 2, 3 (index 0 unused).
+    int64_t chem_m[kNTypes][4];
 };
 
 __host__ __device__ inline int param_words(int nt) { return 2 * nt * nt + 4 * nt; }
@@ -74,6 +77,8 @@ struct SweepConsts {
     uint32_t k0, k1;        // Philox key (seed)
     int nt;                 // types present (pair-table stride)
     uint32_t fixed_mask;    // bit t set: type t is fixed (settings.h:24)
+    int chem;               // chemotaxis term on
+    const int32_t* field;   // field [d][L^d], quantized to 2^-16 (or null)
 };
 
 // One colour phase per launch.

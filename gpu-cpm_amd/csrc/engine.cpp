@@ -69,6 +69,7 @@ struct HostSettings {
     int32_t target_perim[CPM_NTYPES];
     double temperature;
     int32_t fixed[CPM_NTYPES];  // settings.h:24
+    double chem[CPM_NTYPES];    // chemotaxis lambda (cpm.cu:155)
 };
 
 struct cpm_engine {
@@ -82,6 +83,10 @@ struct cpm_engine {
     HostSettings st{};
 
     uint32_t* h_lat = nullptr;  // pinned host mirror
+    bool has_field = false;
+    std::vector<double> h_field;         // host field [d][sites] (cpm.cu:189-190)
+    std::vector<double> h_field_pushed;  // the field as last pushed (pull restores it)
+    int32_t* d_field = nullptr;          // device field, quantized to 2^-16
     hipStream_t stream = nullptr;
     uint32_t* d_lat = nullptr;
     int64_t* d_tab[2] = {nullptr, nullptr};
@@ -141,14 +146,22 @@ struct cpm_engine {
         (void)hipFree(d_claims);
         (void)hipFree(d_ctype);
         (void)hipFree(d_thr);
+        (void)hipFree(d_field);
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // Smallest dyadic scale 2^k making every lambda integral (k <= 8).
+    bool chem_on() const {
+        for (int t = 0; t < CPM_NTYPES; t++)
+            if (st.chem[t] != 0.0) return true;
+        return false;
+    }
+
+    // Smallest dyadic scale 2^k making every lambda integral (k <= 8); with a
+    // chemotaxis term 2^8, which resolves it to 1/256 energy units.
     int energy_scale() const {
-        for (int k = 0; k <= kMaxScaleLog2; k++) {
+        for (int k = chem_on() ? kMaxScaleLog2 : 0; k <= kMaxScaleLog2; k++) {
             const double s = (double)(1 << k);
             bool ok = true;
             for (int t = 0; t < CPM_NTYPES && ok; t++) {
@@ -235,6 +248,11 @@ struct cpm_engine {
             p.w[2 * n2 + 2 * nt + a] = (int32_t)(st.lam_perim[a] * sc);
             p.w[2 * n2 + 3 * nt + a] = st.target_perim[a];
         }
+        for (int a = 0; a < CPM_NTYPES; a++)
+            for (int kk = 0; kk < 4; kk++)
+                p.chem_m[a][kk] = kk ? (int64_t)std::llround(st.chem[a] * (double)sc * 256.0 /
+                                                             std::sqrt((double)kk))
+                                     : 0;
         if (nt <= gcpm::kSmallTypes) {
             constexpr int S = gcpm::kSmallTypes, S2 = S * S;
             for (int a = 0; a < nt; a++) {
@@ -321,7 +339,6 @@ const char* cpm_version(void) { return "gpucpm-mi355x 0.1 (gfx950)"; }
 
 int cpm_create(int dimension, int dimensionality, int number_of_cells, int temperature,
                int has_field, int history, int has_act, cpm_handle* out) {
-    (void)has_field;
     (void)has_act;
     if (!out) return fail(CPM_EINVAL, "null output handle");
     *out = nullptr;
@@ -373,6 +390,13 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     if ((err = hipMalloc(&g->d_prm, sizeof(gcpm::DevParams))) != hipSuccess) return bail(err, "hipMalloc(params)");
     if ((err = hipMalloc(&g->d_bar, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
     if ((err = hipMemset(g->d_bar, 0, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
+    if (has_field) {
+        g->has_field = true;
+        g->h_field.assign(g->sites * g->d, 0.0);
+        g->h_field_pushed.assign(g->sites * g->d, 0.0);
+        if ((err = hipMalloc(&g->d_field, g->sites * g->d * 4)) != hipSuccess) return bail(err, "hipMalloc(field)");
+        if ((err = hipMemset(g->d_field, 0, g->sites * g->d * 4)) != hipSuccess) return bail(err, "hipMemset(field)");
+    }
     *out = g;
     return CPM_OK;
 }
@@ -447,6 +471,24 @@ int cpm_set_fixed(cpm_handle h, int type, int fixed) {
     return CPM_OK;
 }
 
+int cpm_set_chemotaxis(cpm_handle h, int type, double lambda) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (!std::isfinite(lambda) || std::fabs(lambda) > 8192.0)
+        return fail(CPM_EINVAL, "lambda_chemotaxis must be finite with |lambda| <= 8192");
+    h->st.chem[type] = lambda;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_host_field(cpm_handle h, double** out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(CPM_EINVAL, "null output");
+    if (!h->has_field) return fail(CPM_ESTATE, "this engine was created with has_field = 0");
+    *out = h->h_field.data();
+    return CPM_OK;
+}
+
 int cpm_add_cell(cpm_handle h, int type, int x, int y, int z) {
     if (int rc = check_handle(h)) return rc;
     if (int rc = check_type(type, "cell type")) return rc;
@@ -495,8 +537,22 @@ int cpm_push(cpm_handle h) {
         h->nt = (int)tmax + 1;  // flips only copy existing values: no new types appear
         h->params_dirty = true;
     }
+    // The field goes with the lattice (cpm.cu:542-543), quantized to 2^-16.
+    std::vector<int32_t> fq;
+    if (h->has_field) {
+        fq.resize(h->h_field.size());
+        for (size_t i = 0; i < fq.size(); i++) {
+            const double f = h->h_field[i];
+            if (!(std::fabs(f) < 32768.0)) return fail(CPM_EINVAL, "field values must satisfy |f| < 2^15");
+            fq[i] = (int32_t)std::llround(f * 65536.0);
+        }
+    }
     const size_t rows = (size_t)h->n + 1;
     hipStream_t s = h->stream;
+    if (h->has_field) {
+        HIP_TRY(hipMemcpyAsync(h->d_field, fq.data(), fq.size() * 4, hipMemcpyHostToDevice, s));
+        h->h_field_pushed = h->h_field;
+    }
     HIP_TRY(hipMemcpyAsync(h->d_lat, h->h_lat, h->sites * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
     for (int i = 0; i < 2; i++) HIP_TRY(hipMemsetAsync(h->d_del[i], 0, rows * 8, s));
@@ -545,6 +601,10 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     k.k0 = (uint32_t)h->seed;
     k.k1 = (uint32_t)(h->seed >> 32);
     k.nt = h->nt;
+    k.chem = h->chem_on() ? 1 : 0;
+    if (k.chem && !h->has_field)
+        return fail(CPM_EINVAL, "lambda_chemotaxis is set but the engine has no field (has_field = 0)");
+    k.field = h->d_field;
     k.fixed_mask = 0;
     for (int t = 0; t < CPM_NTYPES; t++)
         if (h->st.fixed[t]) k.fixed_mask |= 1u << t;
@@ -677,6 +737,9 @@ int cpm_pull(cpm_handle h) {
     if (int rc = check_handle(h)) return rc;
     HIP_TRY(hipMemcpyAsync(h->h_lat, h->d_lat, h->sites * 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    // cpm.cu:537-538 copies the device field back: the sweep never writes
+    // it, so that is the field as last pushed.
+    if (h->has_field) h->h_field = h->h_field_pushed;
     return CPM_OK;
 }
 

@@ -103,6 +103,7 @@ struct GlobalLat {
         return ld<CO>(p + site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL));
     }
     __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { st<CO>(p + q.i, v); }
+    __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.i; }
     // The 3^D block around q (centre included), (dz, dy, dx) order with dx
     // fastest.  Byte offsets from the lattice base fit in 32 bits (at most
     // 2^30 sites), so every load is base (scalar) + one 32-bit offset.
@@ -163,6 +164,7 @@ struct LdsLat2 {
     __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
         lds_block<2>(*this, q, nv);
     }
+    __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.gi; }
 };
 
 // A 3D box of the lattice staged in LDS for one colour phase, with a
@@ -191,6 +193,7 @@ struct LdsLat3 {
     __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
         lds_block<3>(*this, q, nv);
     }
+    __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.gi; }
 };
 
 // Cell-table buffers of colour phase g (see SweepConsts).
@@ -219,6 +222,7 @@ struct Draw {
     typename Lat::Pos t;
     uint32_t nv[NB];  // the 3^D block around the target, centre included
     uint32_t vs;      // the source: the neighbour picked by w[1]
+    int dir;          // the source's index in the 3^D block
 };
 
 template <int D>
@@ -248,6 +252,7 @@ __device__ __forceinline__ void draw_attempt(const Lat& lat, const SweepConsts& 
     constexpr int N = D == 3 ? 26 : 8;
     int dir = (int)(((uint64_t)d.w[1] * (uint64_t)N) >> 32);
     dir += dir >= N / 2;
+    d.dir = dir;
     if constexpr (D == 2) {
         uint32_t vs = 0;
 #pragma unroll
@@ -347,6 +352,32 @@ struct SmallParams {
     }
 };
 
+// Chemotaxis (kernels2d.cu:325-334, kernels3d.cu:314-325) in fixed point:
+// -lambda (d . f(s)) with d = (t - s) / |t - s| = -o / |o|, o = s - t the
+// source's block offset, is +lambda (o . f(s)) / sqrt(k), k = |o|^2.  lambda
+// is that of the source type, or of the target type when the source is
+// medium.  With m = llround(lambda scale 2^8 / sqrt(k)) and the field held
+// as round(f 2^16), the term is (m g + 2^23) >> 24, g = sum_i o_i f_i(s).
+template <int D, class Lat>
+__device__ __forceinline__ int64_t chem_energy(const SweepConsts& k, const Lat& lat,
+                                               const typename Lat::Pos& t, int dir, int ts,
+                                               int tt) {
+    int ox, oy, oz;
+    block_offset<D>(dir, ox, oy, oz);
+    const int tl = ts ? ts : tt;
+    const int kk = (ox != 0) + (oy != 0) + (oz != 0);
+    const int64_t m = k.prm->chem_m[tl][kk];
+    if (m == 0) return 0;
+    const int lgL = k.lgL, Lm = (1 << lgL) - 1;
+    const uint32_t gi = lat.gindex(t);
+    const int x = (int)(gi & Lm), y = (int)((gi >> lgL) & Lm), z = D == 3 ? (int)(gi >> (2 * lgL)) : 0;
+    const uint32_t src = site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL);
+    const uint32_t n = 1u << (D * lgL);
+    int64_t g = (int64_t)ox * k.field[src] + (int64_t)oy * k.field[n + src];
+    if constexpr (D == 3) g += (int64_t)oz * k.field[2 * n + src];
+    return (m * g + ((int64_t)1 << 23)) >> 24;
+}
+
 template <int D, bool CO, class Lat, class PV>
 __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts& k, const PV& P,
                                                const uint32_t* __restrict__ thr,
@@ -411,6 +442,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         dh += (int64_t)P.la(tt) * (1 - 2 * A + 2 * (int64_t)P.a0(tt));
         dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
     }
+    if (k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
     const bool accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
     if (!accept) return;
     if (cy.pass) {
@@ -590,6 +622,7 @@ struct QueuedAttempt {
     uint32_t vs;    // source value
     uint32_t w2;    // acceptance draw
     uint32_t sb;    // sub-block id
+    uint32_t dir;   // source index in the 3^D block
     uint32_t nv[NQ > 0 ? NQ : 1];
 };
 
@@ -670,6 +703,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
             e[r].vs = vs;
             e[r].w2 = w[2];
             e[r].sb = sb;
+            e[r].dir = (uint32_t)dir;
         }
     }
 #pragma unroll
@@ -716,6 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
             lat.block(t, d.nv);
         }
         d.vs = q.vs;
+        d.dir = (int)q.dir;
         const CellSync cy{a.sync_pass, a.claims, a.sync_tag};
         if constexpr (STAGE) {
             const LdsParamView P{sp.p, k.nt, k.nt * k.nt};

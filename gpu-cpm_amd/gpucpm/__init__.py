@@ -82,8 +82,14 @@ class Cpm:
         self._h = h
         self._field = None
         if self._has_field:
+            # engine-owned host field (cpm.cu:189-190), viewed in place
+            p = ctypes.POINTER(ctypes.c_double)()
+            check(lib.cpm_host_field(self._h, ctypes.byref(p)))
             shape = (self.dimensionality,) + (self.dimension,) * self.dimensionality
-            self._field = np.zeros(shape, np.float64)
+            n = int(np.prod(shape))
+            buf = (ctypes.c_double * n).from_address(ctypes.addressof(p.contents))
+            buf._owner = self
+            self._field = np.ctypeslib.as_array(buf).reshape(shape)
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
@@ -121,13 +127,13 @@ class Cpm:
         # Terms outside the implemented Hamiltonian are refused, not ignored.
         if lpers != -1 and pdiff != -1 and ptime != -1 and lpers != 0:
             raise NotImplementedError("the persistence term is not implemented")
-        if lchem != -1 and lchem != 0:
-            raise NotImplementedError("the chemotaxis term is not implemented")
         if lact != -1 and mact != -1 and lact != 0:
             raise NotImplementedError("the act term is not implemented")
 
         if fx >= 0:
             check(lib.cpm_set_fixed(self._h, ct, fx))
+        if lchem != -1:
+            check(lib.cpm_set_chemotaxis(self._h, ct, lchem))
         if lp != -1 and tp != -1:
             check(lib.cpm_set_perimeter(self._h, ct, tp, lp))
         if la != -1 and ta != -1:
@@ -212,9 +218,9 @@ class Cpm:
         return out
 
     def get_field(self):
-        """float64 (d, L, ...) host field array (python_wrapper.cpp:274-290).
-        It is host data only: the chemotaxis term that reads it is not
-        implemented."""
+        """float64 (d, L, ...) view of the host field (python_wrapper.cpp:274-290):
+        fill it, then push_to_gpu; the chemotaxis term reads it at the source
+        site of each copy.  The view keeps this Cpm alive."""
         if self._field is None:
             raise ValueError("get_field: this Cpm was created with has_field=False")
         return self._field

@@ -62,6 +62,8 @@ SIGNATURES = [
     ("cpm_timing_end", _I, [_H, _P(ctypes.c_double), _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("cpm_last_launch", _I, [_H, _P(_I), _P(_I), _P(_I)]),
     ("cpm_last_kernel", ctypes.c_char_p, [_H]),
+    ("cpm_set_chemotaxis", _I, [_H, _I, ctypes.c_double]),
+    ("cpm_host_field", _I, [_H, _P(_P(ctypes.c_double))]),
     ("cpm_batch_create", _I, [_P(_H), _I, _P(_H)]),
     ("cpm_batch_run", _I, [_H, _P(RunArgs)]),
     ("cpm_batch_sync", _I, [_H]),

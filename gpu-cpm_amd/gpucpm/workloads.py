@@ -12,6 +12,7 @@ Each builder returns a ``Workload``: the initial lattice (uint32, ids
                  volume + surface constraints, 26-neighbourhood.
 - ``batched_sorting``  64 sorting replicas of mixed sizes and parameters.
 - ``walled``     any of these inside a one-site frame of a fixed cell type.
+- ``chemotactic`` any of these with a chemotaxis term and a smooth field.
 The shapes may be scaled down (``L`` / ``ncells`` arguments) for tests.
 """
 from __future__ import annotations
@@ -30,16 +31,19 @@ class Workload:
     temperature: int
     lattice: np.ndarray
     constraints: list = field(default_factory=list)
+    field: np.ndarray = None  # chemotaxis field [d][L]^d (float64), or None
 
     def make_cpm(self, cls=None):
         """Build a Cpm with this workload's lattice and constraints (not pushed)."""
         if cls is None:
             from . import Cpm as cls
-        sim = cls(self.L, self.d, self.ncells, self.temperature, False)
+        sim = cls(self.L, self.d, self.ncells, self.temperature, self.field is not None)
         for kw in self.constraints:
             kw = dict(kw)
             sim.set_constraints(kw.pop("cell_type"), **kw)
         sim.get_state()[...] = self.lattice.view(np.int32)
+        if self.field is not None:
+            sim.get_field()[...] = self.field
         return sim
 
     def params(self):
@@ -52,6 +56,7 @@ class Workload:
         p0 = np.zeros(20, np.int64)
         S = np.zeros((20, 20), np.int64)
         fixed = np.zeros(20, np.int64)
+        chem = np.zeros(20)
         for kw in self.constraints:
             t = kw["cell_type"]
             if "adhesion" in kw:
@@ -65,8 +70,11 @@ class Workload:
                 lp[t], p0[t] = kw["lambda_perimeter"], kw["target_perimeter"]
             if kw.get("fixed", -1) >= 0:
                 fixed[t] = kw["fixed"]
+            if kw.get("lambda_chemotaxis", -1) != -1:
+                chem[t] = kw["lambda_chemotaxis"]
         return dict(J=J, S=S, lam_area=la, target_area=a0, lam_perim=lp, target_perim=p0,
-                    temperature=float(self.temperature), fixed=fixed)
+                    temperature=float(self.temperature), fixed=fixed, lam_chem=chem,
+                    field=self.field)
 
 
 def _place_boxes(L, d, n, edge, rng, types):
@@ -157,7 +165,26 @@ def walled(wl: Workload, wall_type=3, J_wall=(8, 4, 12)) -> Workload:
     cons = list(wl.constraints) + [dict(cell_type=wall_type, fixed=1)]
     for other, j in zip((0, 1, 2), J_wall):
         cons.append(dict(cell_type=wall_type, other_cell_type=other, adhesion=int(j)))
-    return Workload(wl.name + "_walled", wl.L, wl.d, wid, wl.temperature, lat, cons)
+    return Workload(wl.name + "_walled", wl.L, wl.d, wid, wl.temperature, lat, cons, wl.field)
+
+
+def chemotactic(wl: Workload, lam=(0.0, 20.0, 5.0), seed=7, amplitude=2.0) -> Workload:
+    """``wl`` with a chemotaxis term: lambda_chemotaxis ``lam[t]`` for types
+    0, 1, 2 and a smooth random field (a sum of a few periodic modes, values
+    within +-amplitude)."""
+    rng = np.random.default_rng(seed)
+    L, d = wl.L, wl.d
+    grids = np.meshgrid(*([np.arange(L) * 2 * np.pi / L] * d), indexing="ij")[::-1]  # x, y(, z)
+    f = np.zeros((d,) + (L,) * d)
+    for a in range(d):
+        for _ in range(3):
+            k = rng.integers(-2, 3, size=d)
+            ph = rng.uniform(0, 2 * np.pi)
+            f[a] += np.cos(sum(int(k[i]) * grids[i] for i in range(d)) + ph)
+    f *= amplitude / max(1e-12, np.abs(f).max())
+    cons = list(wl.constraints) + [dict(cell_type=t, lambda_chemotaxis=float(v))
+                                   for t, v in enumerate(lam) if v != 0.0]
+    return Workload(wl.name + "_chemo", L, d, wl.ncells, wl.temperature, wl.lattice.copy(), cons, f)
 
 
 def plain3d(L=256, ncells=20000, seed=3):

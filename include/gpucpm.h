@@ -85,6 +85,14 @@ int cpm_set_adhesion_real(cpm_handle h, int type, int other_type, double adhesio
 /* Cpm::setStickiness (cpm.cu:145-148); int16 (settings.h:22). */
 int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness);
 
+/* Cpm::setChemotaxis (cpm.cu:155-156): lambda of the chemotaxis term
+ * -lambda (d . f) (kernels2d.cu:325-334, kernels3d.cu:314-325), d the unit
+ * vector of the copy, f the field at the source site.  |lambda| <= 8192. */
+int cpm_set_chemotaxis(cpm_handle h, int type, double lambda);
+/* Cpm::getField (cpm.cu:527-528): host field, d * L^d doubles, axis-major
+ * ([axis][site]); pushed with the lattice, quantized to 2^-16 on the device
+ * (|f| < 2^15).  CPM_ESTATE if the engine was created with has_field = 0. */
+int cpm_host_field(cpm_handle h, double** out);
 /* Cpm::setFixed (cpm.cu:150-153): a fixed type is never overwritten, and a
  * copy from a fixed source writes medium instead (kernels2d.cu:281-290). */
 int cpm_set_fixed(cpm_handle h, int type, int fixed);

@@ -52,8 +52,27 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* ------------------------------------------------------- parameters -- */
 
+static int chem_on(const orc_params* p) {
+    for (int t = 0; t < ORC_NTYPES; t++)
+        if (p->lam_chem[t] != 0.0) return 1;
+    return 0;
+}
+
+int64_t orc_chem_multiplier(double lambda, int scale, int k) {
+    return llround(lambda * (double)scale * (double)(1 << ORC_CHEM_FRAC) / sqrt((double)k));
+}
+
+int orc_quantize_field(int64_t n, const double* f, int32_t* out) {
+    for (int64_t i = 0; i < n; i++) {
+        if (!(fabs(f[i]) < 32768.0)) return -1;
+        out[i] = (int32_t)llround(f[i] * (double)(1 << ORC_FIELD_FRAC));
+    }
+    return 0;
+}
+
 int orc_energy_scale(const orc_params* p) {
-    for (int k = 0; k <= ORC_MAX_SCALE_LOG2; k++) {
+    /* a chemotaxis term is resolved at 1/256 energy units */
+    for (int k = chem_on(p) ? ORC_MAX_SCALE_LOG2 : 0; k <= ORC_MAX_SCALE_LOG2; k++) {
         double s = (double)(1 << k);
         int ok = 1;
         for (int t = 0; t < ORC_NTYPES && ok; t++) {
@@ -179,6 +198,9 @@ typedef struct {
     int64_t JS[ORC_NTYPES][ORC_NTYPES];  /* (J + S) * scale */
     int64_t la[ORC_NTYPES], lp[ORC_NTYPES];
     int64_t a0[ORC_NTYPES], p0[ORC_NTYPES];
+    int chem;                            /* chemotaxis term on */
+    int64_t chem_m[ORC_NTYPES][4];       /* orc_chem_multiplier, k = 1..3 */
+    const int32_t* field_q;
 } orc_scaled;
 
 static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
@@ -191,7 +213,25 @@ static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
         q->lp[a] = (int64_t)(p->lam_perim[a] * scale);
         q->a0[a] = p->target_area[a];
         q->p0[a] = p->target_perim[a];
+        for (int k = 0; k < 4; k++)
+            q->chem_m[a][k] = k ? orc_chem_multiplier(p->lam_chem[a], scale, k) : 0;
     }
+    q->chem = chem_on(p) && p->field_q != NULL;
+    q->field_q = p->field_q;
+}
+
+/* Chemotaxis energy of copying from the neighbour at offset o of target t
+ * (see cpm_oracle.h), in units of 1/scale. */
+static int64_t chem_energy(int L, int d, const orc_scaled* q, uint32_t ts, uint32_t tt,
+                           const int o[3], int64_t src) {
+    uint32_t tl = ts ? ts : tt;
+    int k = (o[0] != 0) + (o[1] != 0) + (d == 3 && o[2] != 0);
+    int64_t m = q->chem_m[tl][k];
+    if (m == 0) return 0;
+    int64_t n = d == 3 ? (int64_t)L * L * L : (int64_t)L * L;
+    int64_t g = 0;
+    for (int a = 0; a < d; a++) g += (int64_t)o[a] * q->field_q[a * n + src];
+    return (m * g + ((int64_t)1 << 23)) >> 24;
 }
 
 /* dH of writing value vs into target t, given its neighbourhood (the energy
@@ -240,8 +280,15 @@ int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,
     orc_scaled q;
     scale_params(p, scale, &q);
     int tx = (int)(t % L), ty = (int)((t / L) % L), tz = d == 3 ? (int)(t / ((int64_t)L * L)) : 0;
+    int sx = (int)(s % L), sy = (int)((s / L) % L), sz = d == 3 ? (int)(s / ((int64_t)L * L)) : 0;
     int as, at;
-    return delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, lat[s], lat[t], &as, &at);
+    int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, lat[s], lat[t], &as, &at);
+    if (q.chem) {
+        /* o = s - t, wrapped into {-1, 0, 1} */
+        int o[3] = {wrapc(L, sx - tx), wrapc(L, sy - ty), d == 3 ? wrapc(L, sz - tz) : 0};
+        dh += chem_energy(L, d, &q, lat[s] >> 24, lat[t] >> 24, o, s);
+    }
+    return dh;
 }
 
 static void claim_min(uint32_t* owner, uint32_t c, uint32_t sb) {
@@ -350,6 +397,9 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                         int as, at;
                         int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
                                                   tz, vs, vt, &as, &at);
+                        if (q.chem)
+                            dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, nb.off[dir],
+                                              site(L, sx, sy, sz));
                         int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
                         if (!accept) continue;
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
@@ -459,6 +509,8 @@ int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int
             }
             int as, at;
             int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, vs, vt, &as, &at);
+            if (q.chem)
+                dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, nb.off[dir], site(L, sx, sy, sz));
             if (!(dh <= 0 || (dh < nthr && w[2] < thr[dh]))) continue;
             lat[ti] = vs;
             uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;

@@ -46,7 +46,25 @@ typedef struct {
     int32_t target_perim[ORC_NTYPES];    /* settings.h:18 */
     double temperature;                  /* settings.h:30 */
     int32_t fixed[ORC_NTYPES];           /* settings.h:24: unmovable types */
+    double lam_chem[ORC_NTYPES];         /* settings.h: chemotaxis (cpm.cu:155) */
+    const int32_t* field_q;              /* field [d][L^d] quantized to 2^-16, or NULL */
 } orc_params;
+
+/* Chemotaxis (kernels2d.cu:325-334, kernels3d.cu:314-325) in fixed point.
+ * A copy from source s to target t = s - o (o = source - target, one of the
+ * 8 / 26 neighbour offsets) adds -lambda (d . f(s)) with d = (t - s) / |t - s|
+ * = -o / |o|, i.e. +lambda (o . f(s)) / sqrt(k), k = |o|^2 in {1, 2, 3}.
+ * lambda is that of the source type, or of the target type when the source
+ * is medium.  In units of 1/scale:
+ *   m[type][k] = llround(lambda * scale * 2^8 / sqrt(k)),
+ *   g = sum_i o_i f_q,i(s)   (f_q = llround(f * 2^16)),
+ *   E = (m g + 2^23) >> 24   (arithmetic shift: rounds half up).
+ * With a chemotaxis term the scale is 2^8.  Fields must satisfy |f| < 2^15
+ * and |lambda| <= 2^13 so that m g fits in 63 bits. */
+#define ORC_FIELD_FRAC 16
+#define ORC_CHEM_FRAC 8
+int64_t orc_chem_multiplier(double lambda, int scale, int k);
+int orc_quantize_field(int64_t n, const double* f, int32_t* out);
 
 /* Generators. */
 int32_t orc_minstd_next_f64(int32_t seed);      /* random.cu:5-12, double formula */

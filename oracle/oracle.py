@@ -40,11 +40,19 @@ class Params(ctypes.Structure):
         ("target_perim", ctypes.c_int32 * NTYPES),
         ("temperature", ctypes.c_double),
         ("fixed", ctypes.c_int32 * NTYPES),
+        ("lam_chem", ctypes.c_double * NTYPES),
+        ("field_q", ctypes.POINTER(ctypes.c_int32)),
     ]
+
+    def set_field(self, field):
+        """Attach a field [d][L]^d (float64): quantized to 2^-16 as the engine
+        does at push.  The quantized copy is kept alive on this object."""
+        self._field_q = quantize_field(field)
+        self.field_q = _ptr(self._field_q, ctypes.c_int32)
 
     @classmethod
     def make(cls, temperature, J=None, S=None, lam_area=None, target_area=None,
-             lam_perim=None, target_perim=None, fixed=None):
+             lam_perim=None, target_perim=None, fixed=None, lam_chem=None, field=None):
         p = cls()
         p.temperature = float(temperature)
         for name, val in (("J", J), ("S", S)):
@@ -57,10 +65,12 @@ class Params(ctypes.Structure):
                         getattr(p, name)[a][b] = int(arr[a, b])
         for name, val in (("lam_area", lam_area), ("target_area", target_area),
                           ("lam_perim", lam_perim), ("target_perim", target_perim),
-                          ("fixed", fixed)):
+                          ("fixed", fixed), ("lam_chem", lam_chem)):
             if val is not None:
                 for t, x in enumerate(val):
                     getattr(p, name)[t] = float(x) if name.startswith("lam") else int(x)
+        if field is not None:
+            p.set_field(field)
         return p
 
     def J_array(self):
@@ -108,6 +118,10 @@ def lib():
         L.orc_energy_stats.restype = ctypes.c_int
         L.orc_centroids.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i64p, f64p]
         L.orc_centroids.restype = None
+        L.orc_quantize_field.argtypes = [ctypes.c_int64, f64p, i32p]
+        L.orc_quantize_field.restype = ctypes.c_int
+        L.orc_chem_multiplier.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_chem_multiplier.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -131,6 +145,19 @@ def philox(ctr, key):
     out = np.zeros(4, np.uint32)
     lib().orc_philox4x32_10(_ptr(c, ctypes.c_uint32), _ptr(k, ctypes.c_uint32), _ptr(out, ctypes.c_uint32))
     return out
+
+
+def quantize_field(field) -> np.ndarray:
+    """float64 field -> int32 round(f * 2^16) (orc_quantize_field)."""
+    f = np.ascontiguousarray(field, np.float64)
+    out = np.zeros(f.shape, np.int32)
+    if lib().orc_quantize_field(f.size, _ptr(f, ctypes.c_double), _ptr(out, ctypes.c_int32)):
+        raise ValueError("field values must satisfy |f| < 2^15")
+    return out
+
+
+def chem_multiplier(lam: float, scale: int, k: int) -> int:
+    return lib().orc_chem_multiplier(lam, scale, k)
 
 
 def energy_scale(p: Params) -> int:

@@ -10,7 +10,8 @@ def oracle_params(wl) -> "O.Params":
     p = wl.params()
     return O.Params.make(p["temperature"], J=p["J"], S=p["S"], lam_area=p["lam_area"],
                          target_area=p["target_area"], lam_perim=p["lam_perim"],
-                         target_perim=p["target_perim"], fixed=p["fixed"])
+                         target_perim=p["target_perim"], fixed=p["fixed"], lam_chem=p["lam_chem"],
+                         field=p["field"])
 
 
 def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None):

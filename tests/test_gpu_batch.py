@@ -13,8 +13,9 @@ pytestmark = pytest.mark.gpu
 
 def _small_batch():
     from gpucpm import workloads as W
-    # every other replica inside a fixed wall: per-replica fixed masks
-    return [W.walled(w) if i % 2 else w
+    # every other replica inside a fixed wall (per-replica fixed masks), one
+    # with a chemotaxis field
+    return [W.walled(w) if i % 2 else (W.chemotactic(w) if i == 2 else w)
             for i, w in enumerate(W.batched_sorting(nrep=6, seed=40, sizes=(32, 64, 128)))]
 
 

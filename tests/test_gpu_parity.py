@@ -244,3 +244,39 @@ def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, monkeyp
         lat_o, tab_o = run_oracle(wl, c, u, 71, n)
         np.testing.assert_array_equal(lat, lat_o)
         assert_tables_equal(table, tab_o, wl.d)
+
+
+@pytest.mark.parametrize("small,compact,att", [("0", "1", "1"), ("1", "1", "4"), ("0", "0", "1"),
+                                               ("1", "0", "1")])
+def test_chemotaxis_bit_exact(small, compact, att, monkeypatch):
+    """The chemotaxis term (Cpm::setChemotaxis, kernels2d.cu:325-334,
+    kernels3d.cu:314-325) with a smooth field, in 2D and 3D, every kernel
+    form, bit-exact against the oracle."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_SMALL", small)
+    monkeypatch.setenv("GPUCPM_COMPACT", compact)
+    monkeypatch.setenv("GPUCPM_ATT", att)
+    cases = [(W.chemotactic(W.sorting2d(L=128, grid=9, edge=10, seed=81)), 1, 1, 30),
+             (W.chemotactic(W.sorting2d(L=64, grid=5, edge=10, seed=82)), 2, 4, 20),
+             (W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=83))), 1, 1, 20),
+             (W.chemotactic(W.plain3d(L=32, ncells=60, seed=84), lam=(0.0, 15.0)), 1, 1, 8)]
+    for wl, c, u, n in cases:
+        _announce("chemo", small, compact, att, wl.name, wl.L, c, u)
+        sim, lat, table = _run_gpu(wl, c, u, seed=91, nmcs=n)
+        lat_o, tab_o = run_oracle(wl, c, u, 91, n)
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(table, tab_o, wl.d)
+        # pull restores the pushed field
+        np.testing.assert_array_equal(sim.get_field(), wl.field)
+
+
+def test_chemotaxis_needs_a_field():
+    """A chemotaxis lambda on an engine made with has_field = 0 is refused at
+    run (the reference would read an unallocated field)."""
+    gpucpm, W = _gpu()
+    wl = W.sorting2d(L=64, grid=5, edge=10, seed=85)
+    sim = wl.make_cpm()
+    sim.set_constraints(1, lambda_chemotaxis=2.0)
+    sim.push_to_gpu()
+    with pytest.raises(ValueError):
+        sim.run(**run_kwargs(1, 1, 1))

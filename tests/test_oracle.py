@@ -378,3 +378,82 @@ def test_fixed_type_never_moves_and_copies_medium(serial):
         else:
             assert np.count_nonzero(lat == 0) == 0
             assert np.count_nonzero(wall_after != wall_before) > 0
+
+
+# --------------------------------------------------------- chemotaxis ------
+
+def _one_cell(L=16, d=2, t=1):
+    lat = np.zeros((L,) * d, np.uint32)
+    sl = tuple(slice(L // 2 - 2, L // 2 + 2) for _ in range(d))
+    lat[sl] = (np.uint32(t) << np.uint32(24)) | np.uint32(1)
+    return lat
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_chemotaxis_energy_closed_form(d):
+    """kernels2d.cu:325-334 / kernels3d.cu:314-325: a copy from source s to
+    target t adds -lambda (d . f(s)) with d the unit vector from s to t;
+    lambda of the source type, or of the target type when the source is
+    medium.  The expected values come from the continuous formula; the
+    oracle's fixed point (scale 2^8) must agree to within its rounding."""
+    L = 16
+    lat = _one_cell(L, d)
+    n = lat.size
+    rng = np.random.default_rng(3)
+    field = rng.uniform(-3, 3, size=(d,) + lat.shape)
+    lam = np.zeros(20)
+    lam[1] = 2.5
+    p0 = O.Params.make(10, J=[[0, 4], [4, 8]], lam_area=[0, 1], target_area=[0, 16])
+    p1 = O.Params.make(10, J=[[0, 4], [4, 8]], lam_area=[0, 1], target_area=[0, 16],
+                       lam_chem=lam, field=field)
+    scale = O.energy_scale(p1)
+    assert scale == 256
+    tab = O.tally(lat, d, 1)
+    flat = lat.reshape(-1)
+    coords = np.stack(np.unravel_index(np.arange(n), lat.shape)[::-1], 1)  # x, y(, z)
+    fq = field.reshape(d, -1)
+    checked = 0
+    for t in range(n):
+        for off in np.ndindex(*(3,) * d):
+            o = np.array(off) - 1
+            if not o.any():
+                continue
+            sc = (coords[t] + o) % L
+            s = int(np.ravel_multi_index(tuple(sc[::-1]), lat.shape))
+            if flat[s] == flat[t]:
+                continue
+            e = O.delta_h(lat, d, tab, p1, scale, t, s) - O.delta_h(lat, d, tab, p0, scale, t, s)
+            lam_used = 2.5 if (flat[s] >> 24) == 1 or (flat[t] >> 24) == 1 else 0.0
+            dvec = -o / np.linalg.norm(o)
+            expect = -lam_used * float(dvec @ fq[:, s]) * scale
+            # final rounding (0.5) plus the field's 2^-16 and the
+            # multiplier's 2^-8 quantization (below 0.05 at these sizes)
+            assert abs(e - expect) <= 0.55, (t, s, e, expect)
+            checked += 1
+    assert checked > 50
+
+
+def test_chemotaxis_moves_cell_up_the_field():
+    """A cell in a uniform field f = (f, 0) with lambda > 0 extends along +x
+    (the copy direction aligned with f lowers the energy) and its centroid
+    drifts in +x; with lambda = 0 it does not."""
+    L = 64
+    lat0 = np.zeros((L, L), np.uint32)
+    lat0[28:36, 28:36] = (1 << 24) | 1
+    field = np.zeros((2, L, L))
+    field[0] = 1.0
+    drift = {}
+    for lam in (0.0, 4.0):
+        lat = lat0.copy()
+        p = O.Params.make(10, J=[[0, 8], [8, 16]], lam_area=[0, 2], target_area=[0, 64],
+                          lam_chem=[0, lam], field=field)
+        tab = O.tally(lat, 2, 1)
+        x0 = O.centroids(L, 2, tab)[1, 0]
+        O.sweep(lat, 2, tab, p, 1, 1, seed=5, mcs0=0, nmcs=300)
+        area, perim = _numpy_recount(lat, 2, 1)
+        np.testing.assert_array_equal(tab.area, area)
+        np.testing.assert_array_equal(tab.perim, perim)
+        c = O.centroids(L, 2, tab)[1]
+        drift[lam] = ((c[0] - x0 + L / 2) % L - L / 2, c[1] - 31.5)
+    assert drift[4.0][0] > 5.0, drift
+    assert abs(drift[0.0][0]) < 4.0, drift
