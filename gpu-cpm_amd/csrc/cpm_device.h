@@ -36,6 +36,9 @@ struct DevParams {
     // Chemotaxis multipliers llround(lambda * scale * 2^8 / sqrt(k)) for the
     // neighbour classes k = |o|^2 = 1, 2, 3 (index 0 unused).
     int64_t chem_m[kNTypes][4];
+    // Act: llround(lambda scale / actMax 2^16) and actMax per type.
+    int64_t act_m[kNTypes];
+    int32_t act_max[kNTypes];
 };
 
 __host__ __device__ inline int param_words(int nt) { return 2 * nt * nt + 4 * nt; }
@@ -79,6 +82,10 @@ struct SweepConsts {
     uint32_t fixed_mask;    // bit t set: type t is fixed (settings.h:24)
     int chem;               // chemotaxis term on
     const int32_t* field;   // field [d][L^d], quantized to 2^-16 (or null)
+    int act;                // act term on
+    uint32_t* act_lat;      // act lattice [L^d]: MCS of the last copy into each site
+    const int64_t* act_tab; // 2^(j/256) 2^30 [256], then llround(log2(v) 2^24) [v]
+    int64_t act_ln2q;       // llround(ln 2 2^30)
 };
 
 // One colour phase per launch.

@@ -70,6 +70,8 @@ struct HostSettings {
     double temperature;
     int32_t fixed[CPM_NTYPES];  // settings.h:24
     double chem[CPM_NTYPES];    // chemotaxis lambda (cpm.cu:155)
+    double lam_act[CPM_NTYPES];  // actLambda (cpm.cu:131-135)
+    int32_t act_max[CPM_NTYPES]; // actMax
 };
 
 struct cpm_engine {
@@ -87,6 +89,11 @@ struct cpm_engine {
     std::vector<double> h_field;         // host field [d][sites] (cpm.cu:189-190)
     std::vector<double> h_field_pushed;  // the field as last pushed (pull restores it)
     int32_t* d_field = nullptr;          // device field, quantized to 2^-16
+    bool has_act = false;
+    uint32_t* h_act = nullptr;           // host act lattice (cpm.cu:193), pinned
+    uint32_t* d_act = nullptr;
+    int64_t* d_act_tab = nullptr;        // act 2^x table [256] + log2 table [amax + 1]
+    int act_tab_cap = 0;
     hipStream_t stream = nullptr;
     uint32_t* d_lat = nullptr;
     int64_t* d_tab[2] = {nullptr, nullptr};
@@ -147,9 +154,18 @@ struct cpm_engine {
         (void)hipFree(d_ctype);
         (void)hipFree(d_thr);
         (void)hipFree(d_field);
+        (void)hipFree(d_act);
+        (void)hipFree(d_act_tab);
+        if (h_act) (void)hipHostFree(h_act);
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    bool act_on() const {
+        for (int t = 0; t < CPM_NTYPES; t++)
+            if (st.lam_act[t] != 0.0) return true;
+        return false;
     }
 
     bool chem_on() const {
@@ -161,7 +177,7 @@ struct cpm_engine {
     // Smallest dyadic scale 2^k making every lambda integral (k <= 8); with a
     // chemotaxis term 2^8, which resolves it to 1/256 energy units.
     int energy_scale() const {
-        for (int k = chem_on() ? kMaxScaleLog2 : 0; k <= kMaxScaleLog2; k++) {
+        for (int k = chem_on() || act_on() ? kMaxScaleLog2 : 0; k <= kMaxScaleLog2; k++) {
             const double s = (double)(1 << k);
             bool ok = true;
             for (int t = 0; t < CPM_NTYPES && ok; t++) {
@@ -253,6 +269,30 @@ struct cpm_engine {
                 p.chem_m[a][kk] = kk ? (int64_t)std::llround(st.chem[a] * (double)sc * 256.0 /
                                                              std::sqrt((double)kk))
                                      : 0;
+        // Act: multipliers llround(lambda scale / actMax 2^16), the 2^x table
+        // llround(2^(j/256) 2^30) and the log2 table llround(log2(v) 2^24).
+        int amax = 0;
+        for (int a = 0; a < CPM_NTYPES; a++) {
+            p.act_max[a] = st.act_max[a];
+            p.act_m[a] = st.act_max[a] >= 1 ? (int64_t)std::llround(st.lam_act[a] * (double)sc /
+                                                                   (double)st.act_max[a] * 65536.0)
+                                             : 0;
+            amax = std::max(amax, st.act_max[a]);  // any type's G may be evaluated
+        }
+        if (act_on()) {
+            std::vector<int64_t> tab(256 + (size_t)amax + 1);
+            for (int j = 0; j < 256; j++) tab[j] = (int64_t)std::llround(std::exp2(j / 256.0) * 1073741824.0);
+            tab[256] = 0;
+            for (int v = 1; v <= amax; v++) tab[256 + v] = (int64_t)std::llround(std::log2((double)v) * 16777216.0);
+            if ((int)tab.size() > act_tab_cap) {
+                (void)hipFree(d_act_tab);
+                d_act_tab = nullptr;
+                HIP_TRY(hipMalloc(&d_act_tab, tab.size() * 8));
+                act_tab_cap = (int)tab.size();
+            }
+            HIP_TRY(hipMemcpyAsync(d_act_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
         if (nt <= gcpm::kSmallTypes) {
             constexpr int S = gcpm::kSmallTypes, S2 = S * S;
             for (int a = 0; a < nt; a++) {
@@ -339,7 +379,6 @@ const char* cpm_version(void) { return "gpucpm-mi355x 0.1 (gfx950)"; }
 
 int cpm_create(int dimension, int dimensionality, int number_of_cells, int temperature,
                int has_field, int history, int has_act, cpm_handle* out) {
-    (void)has_act;
     if (!out) return fail(CPM_EINVAL, "null output handle");
     *out = nullptr;
     if (dimensionality != 2 && dimensionality != 3)
@@ -390,6 +429,14 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     if ((err = hipMalloc(&g->d_prm, sizeof(gcpm::DevParams))) != hipSuccess) return bail(err, "hipMalloc(params)");
     if ((err = hipMalloc(&g->d_bar, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
     if ((err = hipMemset(g->d_bar, 0, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
+    if (has_act) {  // cpm.cu:192-193,204-205
+        g->has_act = true;
+        if ((err = hipHostMalloc((void**)&g->h_act, g->sites * 4, hipHostMallocDefault)) != hipSuccess)
+            return bail(err, "hipHostMalloc(act)");
+        std::memset(g->h_act, 0, g->sites * 4);
+        if ((err = hipMalloc(&g->d_act, g->sites * 4)) != hipSuccess) return bail(err, "hipMalloc(act)");
+        if ((err = hipMemset(g->d_act, 0, g->sites * 4)) != hipSuccess) return bail(err, "hipMemset(act)");
+    }
     if (has_field) {
         g->has_field = true;
         g->h_field.assign(g->sites * g->d, 0.0);
@@ -481,6 +528,25 @@ int cpm_set_chemotaxis(cpm_handle h, int type, double lambda) {
     return CPM_OK;
 }
 
+int cpm_set_act(cpm_handle h, int type, int max_act, double lambda) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (!std::isfinite(lambda)) return fail(CPM_EINVAL, "lambda_act must be finite");
+    if (max_act < 0 || max_act > 65535) return fail(CPM_EINVAL, "max_act must be in [0, 65535]");
+    h->st.lam_act[type] = lambda;
+    h->st.act_max[type] = max_act;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_host_act(cpm_handle h, uint32_t** out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(CPM_EINVAL, "null output");
+    if (!h->has_act) return fail(CPM_ESTATE, "this engine was created with has_act = 0");
+    *out = h->h_act;
+    return CPM_OK;
+}
+
 int cpm_host_field(cpm_handle h, double** out) {
     if (int rc = check_handle(h)) return rc;
     if (!out) return fail(CPM_EINVAL, "null output");
@@ -553,6 +619,8 @@ int cpm_push(cpm_handle h) {
         HIP_TRY(hipMemcpyAsync(h->d_field, fq.data(), fq.size() * 4, hipMemcpyHostToDevice, s));
         h->h_field_pushed = h->h_field;
     }
+    if (h->has_act)  // cpm.cu:548
+        HIP_TRY(hipMemcpyAsync(h->d_act, h->h_act, h->sites * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(h->d_lat, h->h_lat, h->sites * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
     for (int i = 0; i < 2; i++) HIP_TRY(hipMemsetAsync(h->d_del[i], 0, rows * 8, s));
@@ -605,6 +673,24 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (k.chem && !h->has_field)
         return fail(CPM_EINVAL, "lambda_chemotaxis is set but the engine has no field (has_field = 0)");
     k.field = h->d_field;
+    k.act = h->act_on() ? 1 : 0;
+    if (k.act) {
+        if (!h->has_act)
+            return fail(CPM_EINVAL, "lambda_act is set but the engine has no act lattice (has_act = 0)");
+        if (c < 2)
+            return fail(CPM_ENOTIMPL,
+                        "the act term needs positions_per_checkerboard >= 2 (with 1 the source "
+                        "neighbourhood reaches targets of the same colour phase)");
+        for (int t = 0; t < CPM_NTYPES; t++) {
+            if (h->st.lam_act[t] == 0.0) continue;
+            if (h->st.act_max[t] < 1) return fail(CPM_EINVAL, "max_act must be >= 1 where lambda_act != 0");
+            if (std::fabs(h->st.lam_act[t] * h->scale / h->st.act_max[t]) >= 16384.0)
+                return fail(CPM_EINVAL, "|lambda_act / max_act| too large (at most 63)");
+        }
+    }
+    k.act_lat = h->d_act;
+    k.act_tab = h->d_act_tab;
+    k.act_ln2q = (int64_t)std::llround(std::log(2.0) * 1073741824.0);
     k.fixed_mask = 0;
     for (int t = 0; t < CPM_NTYPES; t++)
         if (h->st.fixed[t]) k.fixed_mask |= 1u << t;
@@ -736,6 +822,8 @@ int cpm_sync(cpm_handle h) {
 int cpm_pull(cpm_handle h) {
     if (int rc = check_handle(h)) return rc;
     HIP_TRY(hipMemcpyAsync(h->h_lat, h->d_lat, h->sites * 4, hipMemcpyDeviceToHost, h->stream));
+    if (h->has_act)  // cpm.cu:535
+        HIP_TRY(hipMemcpyAsync(h->h_act, h->d_act, h->sites * 4, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     // cpm.cu:537-538 copies the device field back: the sweep never writes
     // it, so that is the field as last pushed.

@@ -223,6 +223,7 @@ struct Draw {
     uint32_t nv[NB];  // the 3^D block around the target, centre included
     uint32_t vs;      // the source: the neighbour picked by w[1]
     int dir;          // the source's index in the 3^D block
+    uint32_t tick;    // MCS counter (act term)
 };
 
 template <int D>
@@ -244,6 +245,7 @@ __device__ __forceinline__ void draw_attempt(const Lat& lat, const SweepConsts& 
     const int lc = k.lc;
     const uint32_t cm = (1u << lc) - 1u;
     philox4x32_10(sb, mcs, phase, i, k.k0, k.k1, d.w);
+    d.tick = mcs;
     const int tx = (bx << lc) + (int)(d.w[0] & cm);
     const int ty = (by << lc) + (int)((d.w[0] >> lc) & cm);
     const int tz = D == 3 ? (bz << lc) + (int)((d.w[0] >> (2 * lc)) & cm) : 0;
@@ -378,6 +380,85 @@ __device__ __forceinline__ int64_t chem_energy(const SweepConsts& k, const Lat& 
     return (m * g + ((int64_t)1 << 23)) >> 24;
 }
 
+// Act (kernels2d.cu:312-343,386-397,417-473; kernels3d.cu likewise) in
+// fixed point.  A site's act value is min(max(actMax + act - tick, 0),
+// actMax); G(site) is the geometric mean of the values of the site and its
+// neighbours holding the same cell value (0 for medium or when a value is 0),
+// 2^(mean log2) from integer tables, in 2^-16 units.  The term is
+// -lambda/actMax (G_source - G_target), lambda and actMax of the target type
+// (of the source type when the target is medium): (-mA dG + 2^31) >> 32.
+__device__ __forceinline__ int64_t act_geo_mean(const SweepConsts& k, int64_t sum_lg, int count) {
+    const int64_t mean = sum_lg / count;
+    const int64_t I = mean >> 24, f = mean & 0xFFFFFF;
+    const int64_t j = f >> 16, r = f & 0xFFFF;
+    const int64_t X = (r * k.act_ln2q) >> 24;
+    const int64_t P = ((int64_t)1 << 30) + X + ((X * X) >> 31);
+    const int64_t M = (k.act_tab[j] * P) >> 30;
+    return (M << I) >> 14;
+}
+
+// G of the site at global index gi holding value v, whose 3^D block values
+// are nv (the centre is the site itself).
+template <int D, bool CO>
+__device__ __forceinline__ int64_t act_site_mean(const SweepConsts& k, uint32_t gi, uint32_t v,
+                                                 const uint32_t* nv, uint32_t tick) {
+    if ((v & kSpinMask) == 0) return 0;
+    constexpr int NB = D == 3 ? 27 : 9;
+    const int lgL = k.lgL, Lm = (1 << lgL) - 1;
+    const int x = (int)(gi & Lm), y = (int)((gi >> lgL) & Lm), z = D == 3 ? (int)(gi >> (2 * lgL)) : 0;
+    const int64_t am = k.prm->act_max[v >> 24];
+    int64_t sum = 0;
+    int count = 0;
+    bool zero = false;
+#pragma unroll
+    for (int n = 0; n < NB; n++) {
+        if (nv[n] != v) continue;  // the centre always matches
+        int ox, oy, oz;
+        block_offset<D>(n, ox, oy, oz);
+        const uint32_t si = site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL);
+        int64_t a = am + (int64_t)ld<CO>(k.act_lat + si) - (int64_t)tick;
+        a = a > am ? am : a;
+        if (a <= 0)
+            zero = true;
+        else
+            sum += k.act_tab[256 + a];
+        count++;
+    }
+    return zero ? 0 : act_geo_mean(k, sum, count);
+}
+
+template <int D, bool CO, class Lat>
+__device__ __forceinline__ int64_t act_energy(const SweepConsts& k, const Lat& lat,
+                                              const typename Lat::Pos& t, const uint32_t* nvt,
+                                              int dir, uint32_t vs, uint32_t vt, uint32_t tick) {
+    const int tl = (vt >> 24) ? (int)(vt >> 24) : (int)(vs >> 24);
+    const int64_t m = k.prm->act_m[tl];
+    if (m == 0) return 0;
+    constexpr int NB = D == 3 ? 27 : 9;
+    const int lgL = k.lgL, Lm = (1 << lgL) - 1;
+    const uint32_t gt = lat.gindex(t);
+    const int x = (int)(gt & Lm), y = (int)((gt >> lgL) & Lm), z = D == 3 ? (int)(gt >> (2 * lgL)) : 0;
+    int ox, oy, oz;
+    block_offset<D>(dir, ox, oy, oz);
+    const int sx = (x + ox) & Lm, sy = (y + oy) & Lm, sz = (z + oz) & Lm;
+    const uint32_t gs = site_index<D>(sx, sy, sz, lgL);
+    // the source's block from global memory (up to two sites from the
+    // target: untouched in this phase when positions_per_checkerboard >= 2)
+    int64_t g_s = 0;
+    if (vs & kSpinMask) {
+        uint32_t nvs[NB];
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            int px, py, pz;
+            block_offset<D>(n, px, py, pz);
+            nvs[n] = ld<CO>(k.lat + site_index<D>((sx + px) & Lm, (sy + py) & Lm, (sz + pz) & Lm, lgL));
+        }
+        g_s = act_site_mean<D, CO>(k, gs, vs, nvs, tick);
+    }
+    const int64_t g_t = act_site_mean<D, CO>(k, gt, vt, nvt, tick);
+    return (-m * (g_s - g_t) + ((int64_t)1 << 31)) >> 32;
+}
+
 template <int D, bool CO, class Lat, class PV>
 __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts& k, const PV& P,
                                                const uint32_t* __restrict__ thr,
@@ -443,6 +524,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
     }
     if (k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
+    if (k.act) dh += act_energy<D, CO>(k, lat, d.t, d.nv, d.dir, vs, vt, d.tick);
     const bool accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
     if (!accept) return;
     if (cy.pass) {
@@ -456,6 +538,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     }
 
     lat.put(d.t, vs);  // kernels2d.cu:548
+    if (k.act) st<CO>(k.act_lat + lat.gindex(d.t), d.tick);  // kernels2d.cu:532-534
     // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums are
     // rebuilt from the lattice when read.
     if (cs) {
@@ -751,6 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
         }
         d.vs = q.vs;
         d.dir = (int)q.dir;
+        d.tick = a.mcs;
         const CellSync cy{a.sync_pass, a.claims, a.sync_tag};
         if constexpr (STAGE) {
             const LdsParamView P{sp.p, k.nt, k.nt * k.nt};

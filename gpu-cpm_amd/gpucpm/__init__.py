@@ -127,8 +127,6 @@ class Cpm:
         # Terms outside the implemented Hamiltonian are refused, not ignored.
         if lpers != -1 and pdiff != -1 and ptime != -1 and lpers != 0:
             raise NotImplementedError("the persistence term is not implemented")
-        if lact != -1 and mact != -1 and lact != 0:
-            raise NotImplementedError("the act term is not implemented")
 
         if fx >= 0:
             check(lib.cpm_set_fixed(self._h, ct, fx))
@@ -138,6 +136,8 @@ class Cpm:
             check(lib.cpm_set_perimeter(self._h, ct, tp, lp))
         if la != -1 and ta != -1:
             check(lib.cpm_set_area(self._h, ct, ta, la))
+        if lact != -1 and mact != -1:
+            check(lib.cpm_set_act(self._h, ct, mact, lact))
         if oct_ != -1 and adh != -1:
             check(lib.cpm_set_adhesion(self._h, ct, oct_, adh, 1))
         if oct_ != -1 and stick != -1:
@@ -226,8 +226,15 @@ class Cpm:
         return self._field
 
     def get_act_state(self):
-        """Not implemented: the act lattice belongs to the act term."""
-        raise NotImplementedError("the act lattice (act term) is not implemented")
+        """int32 view of the host act lattice (python_wrapper.cpp:312-328):
+        the MCS of the last copy into each site, pushed and pulled with the
+        lattice.  The view keeps this Cpm alive."""
+        p = ctypes.POINTER(ctypes.c_uint32)()
+        check(lib.cpm_host_act(self._h, ctypes.byref(p)))
+        n = self.dimension ** self.dimensionality
+        buf = (ctypes.c_int32 * n).from_address(ctypes.addressof(p.contents))
+        buf._owner = self
+        return np.ctypeslib.as_array(buf).reshape((self.dimension,) * self.dimensionality)
 
     # ---------------------------------------------------------- extensions --
     def get_cell_table(self):

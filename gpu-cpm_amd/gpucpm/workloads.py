@@ -13,6 +13,7 @@ Each builder returns a ``Workload``: the initial lattice (uint32, ids
 - ``batched_sorting``  64 sorting replicas of mixed sizes and parameters.
 - ``walled``     any of these inside a one-site frame of a fixed cell type.
 - ``chemotactic`` any of these with a chemotaxis term and a smooth field.
+- ``active``     any of these with the act term and a random act lattice.
 The shapes may be scaled down (``L`` / ``ncells`` arguments) for tests.
 """
 from __future__ import annotations
@@ -32,6 +33,7 @@ class Workload:
     lattice: np.ndarray
     constraints: list = field(default_factory=list)
     field: np.ndarray = None  # chemotaxis field [d][L]^d (float64), or None
+    act: np.ndarray = None    # initial act lattice (uint32), or None
 
     def make_cpm(self, cls=None):
         """Build a Cpm with this workload's lattice and constraints (not pushed)."""
@@ -44,6 +46,8 @@ class Workload:
         sim.get_state()[...] = self.lattice.view(np.int32)
         if self.field is not None:
             sim.get_field()[...] = self.field
+        if self.act is not None:
+            sim.get_act_state()[...] = self.act.view(np.int32)
         return sim
 
     def params(self):
@@ -57,6 +61,8 @@ class Workload:
         S = np.zeros((20, 20), np.int64)
         fixed = np.zeros(20, np.int64)
         chem = np.zeros(20)
+        lam_act = np.zeros(20)
+        act_max = np.zeros(20, np.int64)
         for kw in self.constraints:
             t = kw["cell_type"]
             if "adhesion" in kw:
@@ -72,9 +78,11 @@ class Workload:
                 fixed[t] = kw["fixed"]
             if kw.get("lambda_chemotaxis", -1) != -1:
                 chem[t] = kw["lambda_chemotaxis"]
+            if kw.get("lambda_act", -1) != -1 and kw.get("max_act", -1) != -1:
+                lam_act[t], act_max[t] = kw["lambda_act"], kw["max_act"]
         return dict(J=J, S=S, lam_area=la, target_area=a0, lam_perim=lp, target_perim=p0,
                     temperature=float(self.temperature), fixed=fixed, lam_chem=chem,
-                    field=self.field)
+                    field=self.field, lam_act=lam_act, act_max=act_max)
 
 
 def _place_boxes(L, d, n, edge, rng, types):
@@ -165,7 +173,8 @@ def walled(wl: Workload, wall_type=3, J_wall=(8, 4, 12)) -> Workload:
     cons = list(wl.constraints) + [dict(cell_type=wall_type, fixed=1)]
     for other, j in zip((0, 1, 2), J_wall):
         cons.append(dict(cell_type=wall_type, other_cell_type=other, adhesion=int(j)))
-    return Workload(wl.name + "_walled", wl.L, wl.d, wid, wl.temperature, lat, cons, wl.field)
+    return Workload(wl.name + "_walled", wl.L, wl.d, wid, wl.temperature, lat, cons, wl.field,
+                    wl.act)
 
 
 def chemotactic(wl: Workload, lam=(0.0, 20.0, 5.0), seed=7, amplitude=2.0) -> Workload:
@@ -184,7 +193,20 @@ def chemotactic(wl: Workload, lam=(0.0, 20.0, 5.0), seed=7, amplitude=2.0) -> Wo
     f *= amplitude / max(1e-12, np.abs(f).max())
     cons = list(wl.constraints) + [dict(cell_type=t, lambda_chemotaxis=float(v))
                                    for t, v in enumerate(lam) if v != 0.0]
-    return Workload(wl.name + "_chemo", L, d, wl.ncells, wl.temperature, wl.lattice.copy(), cons, f)
+    return Workload(wl.name + "_chemo", L, d, wl.ncells, wl.temperature, wl.lattice.copy(), cons, f,
+                    wl.act)
+
+
+def active(wl: Workload, lam=(0.0, 300.0, 150.0), act_max=(0, 40, 30), seed=9) -> Workload:
+    """``wl`` with the act term: lambda_act ``lam[t]`` and max_act
+    ``act_max[t]`` for types 0, 1, 2, and an act lattice of recent random
+    stamps (MCS in [0, 40))."""
+    rng = np.random.default_rng(seed)
+    cons = list(wl.constraints) + [dict(cell_type=t, lambda_act=float(l), max_act=int(m))
+                                   for t, (l, m) in enumerate(zip(lam, act_max)) if l != 0.0]
+    act = rng.integers(0, 40, size=wl.lattice.shape).astype(np.uint32)
+    return Workload(wl.name + "_act", wl.L, wl.d, wl.ncells, wl.temperature, wl.lattice.copy(),
+                    cons, wl.field, act)
 
 
 def plain3d(L=256, ncells=20000, seed=3):

@@ -89,6 +89,16 @@ int cpm_set_stickiness(cpm_handle h, int type, int other_type, int stickiness);
  * -lambda (d . f) (kernels2d.cu:325-334, kernels3d.cu:314-325), d the unit
  * vector of the copy, f the field at the source site.  |lambda| <= 8192. */
 int cpm_set_chemotaxis(cpm_handle h, int type, double lambda);
+/* Cpm::setAct (cpm.cu:131-135): lambda and actMax of the act term
+ * -lambda/actMax (G_source - G_target) (kernels2d.cu:312-343,386-397,
+ * 417-473,532-534), G the geometric mean of site activities over a site and
+ * its same-cell neighbours.  max_act in [0, 65535].  Runs with the term need
+ * has_act and positions_per_checkerboard >= 2. */
+int cpm_set_act(cpm_handle h, int type, int max_act, double lambda);
+/* Cpm::getActData (cpm.cu:522-525): host act lattice, L^d values (the MCS of
+ * the last copy into each site); pushed and pulled with the lattice.
+ * CPM_ESTATE if the engine was created with has_act = 0. */
+int cpm_host_act(cpm_handle h, uint32_t** out);
 /* Cpm::getField (cpm.cu:527-528): host field, d * L^d doubles, axis-major
  * ([axis][site]); pushed with the lattice, quantized to 2^-16 on the device
  * (|f| < 2^15).  CPM_ESTATE if the engine was created with has_field = 0. */

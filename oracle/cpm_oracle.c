@@ -52,6 +52,41 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* ------------------------------------------------------- parameters -- */
 
+static int act_on(const orc_params* p) {
+    for (int t = 0; t < ORC_NTYPES; t++)
+        if (p->lam_act[t] != 0.0) return 1;
+    return 0;
+}
+
+int64_t orc_act_log2(int v) {
+    return v >= 1 ? llround(log2((double)v) * 16777216.0) : 0;
+}
+
+int64_t orc_act_multiplier(double lambda, int act_max, int scale) {
+    if (act_max < 1) return 0;
+    return llround(lambda * (double)scale / (double)act_max * 65536.0);
+}
+
+static int64_t act_exp_table(int j) { return llround(exp2((double)j / 256.0) * 1073741824.0); }
+
+int64_t orc_act_geo_mean(int64_t sum_lg, int count) {
+    static int64_t ex[256];
+    static int64_t ln2q;
+    static int init = 0;
+    if (!init) {
+        for (int j = 0; j < 256; j++) ex[j] = act_exp_table(j);
+        ln2q = llround(log(2.0) * 1073741824.0);
+        init = 1;
+    }
+    int64_t mean = sum_lg / count;
+    int64_t I = mean >> 24, f = mean & 0xFFFFFF;
+    int64_t j = f >> 16, r = f & 0xFFFF;
+    int64_t X = (r * ln2q) >> 24;                      /* r ln2 / 2^24 in 2^-30 */
+    int64_t P = ((int64_t)1 << 30) + X + ((X * X) >> 31);
+    int64_t M = (ex[j] * P) >> 30;                     /* 2^(f / 2^24) in 2^-30 */
+    return (M << I) >> 14;
+}
+
 static int chem_on(const orc_params* p) {
     for (int t = 0; t < ORC_NTYPES; t++)
         if (p->lam_chem[t] != 0.0) return 1;
@@ -71,8 +106,8 @@ int orc_quantize_field(int64_t n, const double* f, int32_t* out) {
 }
 
 int orc_energy_scale(const orc_params* p) {
-    /* a chemotaxis term is resolved at 1/256 energy units */
-    for (int k = chem_on(p) ? ORC_MAX_SCALE_LOG2 : 0; k <= ORC_MAX_SCALE_LOG2; k++) {
+    /* chemotaxis and act terms are resolved at 1/256 energy units */
+    for (int k = chem_on(p) || act_on(p) ? ORC_MAX_SCALE_LOG2 : 0; k <= ORC_MAX_SCALE_LOG2; k++) {
         double s = (double)(1 << k);
         int ok = 1;
         for (int t = 0; t < ORC_NTYPES && ok; t++) {
@@ -201,6 +236,10 @@ typedef struct {
     int chem;                            /* chemotaxis term on */
     int64_t chem_m[ORC_NTYPES][4];       /* orc_chem_multiplier, k = 1..3 */
     const int32_t* field_q;
+    int act;                             /* act term on */
+    int64_t act_m[ORC_NTYPES];           /* orc_act_multiplier */
+    int32_t act_max[ORC_NTYPES];
+    uint32_t* act_lat;
 } orc_scaled;
 
 static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
@@ -218,6 +257,53 @@ static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
     }
     q->chem = chem_on(p) && p->field_q != NULL;
     q->field_q = p->field_q;
+    q->act = act_on(p) && p->act != NULL;
+    q->act_lat = p->act;
+    for (int a = 0; a < ORC_NTYPES; a++) {
+        q->act_max[a] = p->act_max[a];
+        q->act_m[a] = orc_act_multiplier(p->lam_act[a], p->act_max[a], scale);
+    }
+}
+
+/* Geometric mean of act values over site (x, y, z) and its neighbours that
+ * hold value v (kernels2d.cu:386-397 / 430-460), in 2^-16 units; 0 for
+ * medium or when any value is 0. */
+static int64_t act_site_mean(int L, int d, const orc_nbhd* nb, const uint32_t* lat,
+                             const orc_scaled* q, int x, int y, int z, uint32_t v,
+                             uint32_t tick) {
+    if ((v & 0xFFFFFFu) == 0) return 0;
+    int64_t am = q->act_max[v >> 24];
+    int64_t sum = 0;
+    int count = 0, zero = 0;
+    for (int k = -1; k < nb->n; k++) {
+        int64_t si;
+        if (k < 0) {
+            si = site(L, x, y, z);
+        } else {
+            int nx = (x + nb->off[k][0]) & (L - 1);
+            int ny = (y + nb->off[k][1]) & (L - 1);
+            int nz = d == 3 ? (z + nb->off[k][2]) & (L - 1) : 0;
+            si = site(L, nx, ny, nz);
+            if (lat[si] != v) continue;
+        }
+        int64_t a = am + (int64_t)q->act_lat[si] - (int64_t)tick;
+        if (a > am) a = am;
+        if (a <= 0) zero = 1; else sum += orc_act_log2((int)a);
+        count++;
+    }
+    return zero ? 0 : orc_act_geo_mean(sum, count);
+}
+
+/* Act energy of copying vs (at source s) into target t holding vt. */
+static int64_t act_energy(int L, int d, const orc_nbhd* nb, const uint32_t* lat,
+                          const orc_scaled* q, int tx, int ty, int tz, int sx, int sy, int sz,
+                          uint32_t vs, uint32_t vt, uint32_t tick) {
+    uint32_t tl = (vt >> 24) ? vt >> 24 : vs >> 24;
+    int64_t m = q->act_m[tl];
+    if (m == 0) return 0;
+    int64_t gs = act_site_mean(L, d, nb, lat, q, sx, sy, sz, vs, tick);
+    int64_t gt = act_site_mean(L, d, nb, lat, q, tx, ty, tz, vt, tick);
+    return (-m * (gs - gt) + ((int64_t)1 << 31)) >> 32;
 }
 
 /* Chemotaxis energy of copying from the neighbour at offset o of target t
@@ -291,6 +377,18 @@ int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,
     return dh;
 }
 
+int64_t orc_act_energy(int L, int d, const uint32_t* lat, const orc_params* p, int scale,
+                       int64_t t, int64_t s, uint32_t tick) {
+    orc_nbhd nb;
+    make_nbhd(d, &nb);
+    orc_scaled q;
+    scale_params(p, scale, &q);
+    if (!q.act) return 0;
+    int tx = (int)(t % L), ty = (int)((t / L) % L), tz = d == 3 ? (int)(t / ((int64_t)L * L)) : 0;
+    int sx = (int)(s % L), sy = (int)((s / L) % L), sz = d == 3 ? (int)(s / ((int64_t)L * L)) : 0;
+    return act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, lat[s], lat[t], tick);
+}
+
 static void claim_min(uint32_t* owner, uint32_t c, uint32_t sb) {
     uint32_t cur = __atomic_load_n(&owner[c], __ATOMIC_RELAXED);
     while (sb < cur &&
@@ -310,6 +408,7 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                    uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync) {
     if (d != 2 && d != 3) return -1;
     if (cell_sync && u != 1) return -7;
+    if (act_on(p) && p->act != NULL && c < 2) return -8;
     if (L < 2 || (L & (L - 1))) return -2;
     if (c < 1 || (c & (c - 1)) || L % (2 * c)) return -3;
     int cd = d == 3 ? c * c * c : c * c;
@@ -400,6 +499,8 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                         if (q.chem)
                             dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, nb.off[dir],
                                               site(L, sx, sy, sz));
+                        if (q.act)
+                            dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
                         int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
                         if (!accept) continue;
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
@@ -410,6 +511,7 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                         }
                         if (cell_sync && ((cs && owner[cs] != sb) || (ct && owner[ct] != sb))) continue;
                         lat[ti] = vs; /* kernels2d.cu:548 */
+                        if (q.act) q.act_lat[ti] = mcs; /* kernels2d.cu:532-534 */
                         int pos[3] = {tx, ty, tz};
                         if (cs) { /* kernels2d.cu:552-560 */
 #ifdef _OPENMP
@@ -511,8 +613,10 @@ int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int
             int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, vs, vt, &as, &at);
             if (q.chem)
                 dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, nb.off[dir], site(L, sx, sy, sz));
+            if (q.act) dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
             if (!(dh <= 0 || (dh < nthr && w[2] < thr[dh]))) continue;
             lat[ti] = vs;
+            if (q.act) q.act_lat[ti] = mcs;
             uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
             int pos[3] = {tx, ty, tz};
             if (cs) {

@@ -48,7 +48,35 @@ typedef struct {
     int32_t fixed[ORC_NTYPES];           /* settings.h:24: unmovable types */
     double lam_chem[ORC_NTYPES];         /* settings.h: chemotaxis (cpm.cu:155) */
     const int32_t* field_q;              /* field [d][L^d] quantized to 2^-16, or NULL */
+    double lam_act[ORC_NTYPES];          /* settings.h: actLambda (cpm.cu:131-135) */
+    int32_t act_max[ORC_NTYPES];         /* settings.h: actMax */
+    uint32_t* act;                       /* act lattice [L^d] (MCS of the last copy into
+                                            each site), updated by sweeps; or NULL */
 } orc_params;
+
+/* Act term (kernels2d.cu:312-343,386-397,417-473,532-534) in fixed point.
+ * Each site's act value is v = min(max(actMax + act[site] - tick, 0), actMax)
+ * (tick = the MCS counter).  G(site, cell) is the geometric mean of v over
+ * the site and its neighbours holding the same cell value (0 for medium).
+ * A copy adds -lambda/actMax (G_source - G_target), lambda and actMax of the
+ * target type (of the source type when the target is medium); the source
+ * uses its type's actMax, the target its own.  An accepted copy sets
+ * act[target] = tick.  In fixed point:
+ *   LG[v] = llround(log2(v) 2^24) (v >= 1), mean = sum LG / count,
+ *   G = 2^(mean / 2^24) by a 256-entry table and a second-order term,
+ *   held in units of 2^-16 (orc_act_geo_mean);
+ *   E = (-mA (Gs - Gt) + 2^31) >> 32, mA = llround(lambda scale / actMax 2^16).
+ * With an act term the scale is 2^8.  It needs actMax in [1, 65535],
+ * |lambda scale / actMax| < 2^14, and positions_per_checkerboard >= 2: the
+ * source's neighbourhood reaches two sites from the target, where with
+ * c = 1 other targets of the same colour phase lie. */
+#define ORC_ACT_MAX 65535
+int64_t orc_act_log2(int v);                          /* LG[v] */
+int64_t orc_act_geo_mean(int64_t sum_lg, int count);  /* G in 2^-16 units */
+int64_t orc_act_multiplier(double lambda, int act_max, int scale);
+/* The act term of copying site s into site t at MCS tick, units of 1/scale. */
+int64_t orc_act_energy(int L, int d, const uint32_t* lat, const orc_params* p, int scale,
+                       int64_t t, int64_t s, uint32_t tick);
 
 /* Chemotaxis (kernels2d.cu:325-334, kernels3d.cu:314-325) in fixed point.
  * A copy from source s to target t = s - o (o = source - target, one of the

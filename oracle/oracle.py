@@ -42,7 +42,17 @@ class Params(ctypes.Structure):
         ("fixed", ctypes.c_int32 * NTYPES),
         ("lam_chem", ctypes.c_double * NTYPES),
         ("field_q", ctypes.POINTER(ctypes.c_int32)),
+        ("lam_act", ctypes.c_double * NTYPES),
+        ("act_max", ctypes.c_int32 * NTYPES),
+        ("act", ctypes.POINTER(ctypes.c_uint32)),
     ]
+
+    def set_act(self, act):
+        """Attach the act lattice (uint32, C-contiguous, the lattice's shape);
+        sweeps update it in place."""
+        assert act.dtype == np.uint32 and act.flags["C_CONTIGUOUS"]
+        self._act = act
+        self.act = _ptr(act, ctypes.c_uint32)
 
     def set_field(self, field):
         """Attach a field [d][L]^d (float64): quantized to 2^-16 as the engine
@@ -52,7 +62,8 @@ class Params(ctypes.Structure):
 
     @classmethod
     def make(cls, temperature, J=None, S=None, lam_area=None, target_area=None,
-             lam_perim=None, target_perim=None, fixed=None, lam_chem=None, field=None):
+             lam_perim=None, target_perim=None, fixed=None, lam_chem=None, field=None,
+             lam_act=None, act_max=None, act=None):
         p = cls()
         p.temperature = float(temperature)
         for name, val in (("J", J), ("S", S)):
@@ -65,12 +76,15 @@ class Params(ctypes.Structure):
                         getattr(p, name)[a][b] = int(arr[a, b])
         for name, val in (("lam_area", lam_area), ("target_area", target_area),
                           ("lam_perim", lam_perim), ("target_perim", target_perim),
-                          ("fixed", fixed), ("lam_chem", lam_chem)):
+                          ("fixed", fixed), ("lam_chem", lam_chem), ("lam_act", lam_act),
+                          ("act_max", act_max)):
             if val is not None:
                 for t, x in enumerate(val):
                     getattr(p, name)[t] = float(x) if name.startswith("lam") else int(x)
         if field is not None:
             p.set_field(field)
+        if act is not None:
+            p.set_act(act)
         return p
 
     def J_array(self):
@@ -122,6 +136,15 @@ def lib():
         L.orc_quantize_field.restype = ctypes.c_int
         L.orc_chem_multiplier.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_chem_multiplier.restype = ctypes.c_int64
+        L.orc_act_log2.argtypes = [ctypes.c_int]
+        L.orc_act_log2.restype = ctypes.c_int64
+        L.orc_act_geo_mean.argtypes = [ctypes.c_int64, ctypes.c_int]
+        L.orc_act_geo_mean.restype = ctypes.c_int64
+        L.orc_act_multiplier.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_act_multiplier.restype = ctypes.c_int64
+        L.orc_act_energy.argtypes = [ctypes.c_int, ctypes.c_int, u32p, P(Params), ctypes.c_int,
+                                     ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32]
+        L.orc_act_energy.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -154,6 +177,25 @@ def quantize_field(field) -> np.ndarray:
     if lib().orc_quantize_field(f.size, _ptr(f, ctypes.c_double), _ptr(out, ctypes.c_int32)):
         raise ValueError("field values must satisfy |f| < 2^15")
     return out
+
+
+def act_log2(v: int) -> int:
+    return lib().orc_act_log2(int(v))
+
+
+def act_geo_mean(sum_lg: int, count: int) -> int:
+    """Geometric mean in 2^-16 units from a sum of act_log2 values."""
+    return lib().orc_act_geo_mean(int(sum_lg), int(count))
+
+
+def act_energy(lat, d, p: Params, scale: int, t: int, s: int, tick: int) -> int:
+    lat32 = np.ascontiguousarray(lat).view(np.uint32)
+    return lib().orc_act_energy(lat32.shape[0], d, _ptr(lat32, ctypes.c_uint32), ctypes.byref(p),
+                                scale, t, s, tick & 0xFFFFFFFF)
+
+
+def act_multiplier(lam: float, act_max: int, scale: int) -> int:
+    return lib().orc_act_multiplier(lam, int(act_max), int(scale))
 
 
 def chem_multiplier(lam: float, scale: int, k: int) -> int:

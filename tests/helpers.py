@@ -11,15 +11,22 @@ def oracle_params(wl) -> "O.Params":
     return O.Params.make(p["temperature"], J=p["J"], S=p["S"], lam_area=p["lam_area"],
                          target_area=p["target_area"], lam_perim=p["lam_perim"],
                          target_perim=p["target_perim"], fixed=p["fixed"], lam_chem=p["lam_chem"],
-                         field=p["field"])
+                         field=p["field"], lam_act=p["lam_act"], act_max=p["act_max"])
 
 
 def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None):
     """Tally + sweep in the oracle; returns (lattice, Tables)."""
     lat = np.ascontiguousarray(wl.lattice if lattice is None else lattice).astype(np.uint32).copy()
     tab = O.tally(lat, wl.d, wl.ncells)
+    p = oracle_params(wl)
+    act = None
+    if getattr(wl, "act", None) is not None:
+        act = np.ascontiguousarray(wl.act, np.uint32).copy()
+        p.set_act(act)
     if nmcs:
-        O.sweep(lat, wl.d, tab, oracle_params(wl), c, u, seed, mcs0, nmcs, nthreads)
+        O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0, nmcs, nthreads)
+    if act is not None:
+        tab.act = act
     return lat, tab
 
 

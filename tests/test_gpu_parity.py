@@ -280,3 +280,42 @@ def test_chemotaxis_needs_a_field():
     sim.push_to_gpu()
     with pytest.raises(ValueError):
         sim.run(**run_kwargs(1, 1, 1))
+
+
+@pytest.mark.parametrize("small,compact,att,mode", [("0", "1", "1", "phase"), ("1", "1", "4", "phase"),
+                                                    ("0", "0", "1", "phase"), ("1", "0", "1", "phase"),
+                                                    ("0", "1", "1", "global"), ("0", "1", "1", "lds")])
+def test_act_bit_exact(small, compact, att, mode, monkeypatch):
+    """The act term (Cpm::setAct, kernels2d.cu:312-343,386-397,417-473,
+    532-534) in 2D and 3D, every kernel and launch form, bit-exact against
+    the oracle, act lattice included; combined with chemotaxis and a fixed
+    wall."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_SMALL", small)
+    monkeypatch.setenv("GPUCPM_COMPACT", compact)
+    monkeypatch.setenv("GPUCPM_ATT", att)
+    monkeypatch.setenv("GPUCPM_MODE", mode)
+    cases = [(W.active(W.sorting2d(L=128, grid=9, edge=10, seed=101)), 2, 1, 30),
+             (W.active(W.sorting2d(L=64, grid=5, edge=10, seed=102)), 2, 4, 20),
+             (W.active(W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=103)))), 4, 3, 20)]
+    if mode == "phase":
+        cases.append((W.active(W.plain3d(L=32, ncells=60, seed=104), lam=(0.0, 200.0),
+                               act_max=(0, 30)), 2, 1, 6))
+    for wl, c, u, n in cases:
+        _announce("act", small, compact, att, mode, wl.name, wl.L, c, u)
+        sim, lat, table = _run_gpu(wl, c, u, seed=111, nmcs=n)
+        lat_o, tab_o = run_oracle(wl, c, u, 111, n)
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(table, tab_o, wl.d)
+        np.testing.assert_array_equal(sim.get_act_state().view(np.uint32), tab_o.act)
+
+
+def test_act_refuses_single_site_checkerboard():
+    """With positions_per_checkerboard = 1 the act term's source neighbourhood
+    reaches same-phase targets; the run is refused, not raced."""
+    _, W = _gpu()
+    wl = W.active(W.sorting2d(L=64, grid=5, edge=10, seed=105))
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    with pytest.raises(NotImplementedError):
+        sim.run(**run_kwargs(1, 1, 1))

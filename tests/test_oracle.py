@@ -457,3 +457,144 @@ def test_chemotaxis_moves_cell_up_the_field():
         drift[lam] = ((c[0] - x0 + L / 2) % L - L / 2, c[1] - 31.5)
     assert drift[4.0][0] > 5.0, drift
     assert abs(drift[0.0][0]) < 4.0, drift
+
+
+# ---------------------------------------------------------------- act ------
+
+def test_act_geometric_mean_fixed_point():
+    """The fixed-point geometric mean (log2 table, 2^x table + second-order
+    term) against numpy in double, over random value sets of 1..27 values."""
+    rng = np.random.default_rng(8)
+    for _ in range(400):
+        n = int(rng.integers(1, 28))
+        v = rng.integers(1, 200, size=n)
+        g = O.act_geo_mean(sum(O.act_log2(int(x)) for x in v), n) / 65536.0
+        ref = float(np.exp(np.mean(np.log(v.astype(np.float64)))))
+        assert abs(g - ref) <= 2e-6 * ref + 2.0 / 65536, (v, g, ref)
+    assert O.act_geo_mean(O.act_log2(1), 1) == 65536
+    assert O.act_geo_mean(3 * O.act_log2(8), 3) == 8 * 65536
+
+
+def _act_setup(L=16, d=2, lam=40.0, amax=20, seed=1):
+    rng = np.random.default_rng(seed)
+    lat = np.zeros((L,) * d, np.uint32)
+    sl = tuple(slice(4, 10) for _ in range(d))
+    lat[sl] = (1 << 24) | 1
+    sl2 = tuple(slice(9, 13) for _ in range(d))
+    lat[sl2] = (2 << 24) | 2
+    act = rng.integers(80, 101, size=lat.shape).astype(np.uint32)
+    la = np.zeros(20)
+    la[1], la[2] = lam, lam / 2
+    am = np.zeros(20, np.int64)
+    am[1], am[2] = amax, amax + 10
+    p = O.Params.make(10, J=[[0, 4, 4], [4, 8, 6], [4, 6, 8]], lam_area=[0, 1, 1],
+                      target_area=[0, 30, 30], lam_act=la, act_max=am, act=act)
+    return lat, act, p, la, am
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_act_energy_closed_form(d):
+    """kernels2d.cu:312-343,386-397,417-473: -lambda/actMax (G_s - G_t), G the
+    geometric mean of max(actMax + act - tick, 0) over the site and its
+    same-cell neighbours (0 for medium), lambda/actMax of the target type or
+    of the source type when the target is medium."""
+    L = 16
+    lat, act, p, la, am = _act_setup(L, d)
+    scale = O.energy_scale(p)
+    assert scale == 256
+    tick = 100
+    flat, aflat = lat.reshape(-1), act.reshape(-1)
+    coords = np.stack(np.unravel_index(np.arange(lat.size), lat.shape)[::-1], 1)
+
+    def gmean(si):
+        v = flat[si]
+        if v & 0xFFFFFF == 0:
+            return 0.0
+        amax = am[v >> 24]
+        vals = [min(max(amax + int(aflat[si]) - tick, 0), amax)]
+        for off in np.ndindex(*(3,) * d):
+            o = np.array(off) - 1
+            if not o.any():
+                continue
+            ni = int(np.ravel_multi_index(tuple(((coords[si] + o) % L)[::-1]), lat.shape))
+            if flat[ni] == v:
+                vals.append(min(max(amax + int(aflat[ni]) - tick, 0), amax))
+        vals = np.array(vals, np.float64)
+        return 0.0 if (vals == 0).any() else float(np.exp(np.log(vals).mean()))
+
+    checked = 0
+    for t in range(0, lat.size, 3):
+        for off in np.ndindex(*(3,) * d):
+            o = np.array(off) - 1
+            if not o.any():
+                continue
+            s = int(np.ravel_multi_index(tuple(((coords[t] + o) % L)[::-1]), lat.shape))
+            if flat[s] == flat[t]:
+                continue
+            tl = flat[t] >> 24 if flat[t] >> 24 else flat[s] >> 24
+            if la[tl] == 0:
+                continue
+            e = O.act_energy(lat, d, p, scale, t, s, tick)
+            expect = -la[tl] / am[tl] * (gmean(s) - gmean(t)) * scale
+            assert abs(e - expect) <= 0.5 + 1e-3 * abs(expect) + 0.01, (t, s, e, expect)
+            checked += 1
+    assert checked > 20
+
+
+def test_act_sweep_keeps_tables_exact_and_stamps_copies():
+    """An accepted copy sets act[target] = tick (kernels2d.cu:532-534): after
+    one MCS at tick m every site that changed holds act m; the tables stay
+    equal to a recount; c = 1 is refused (the source neighbourhood reaches
+    same-phase targets)."""
+    lat, act, p, _, _ = _act_setup(32, 2)
+    before, act0 = lat.copy(), act.copy()
+    tab = O.tally(lat, 2, 2)
+    O.sweep(lat, 2, tab, p, 2, 1, seed=4, mcs0=123, nmcs=1, nthreads=4)
+    changed = lat != before
+    assert changed.sum() > 5
+    assert (act[changed] == 123).all()
+    assert (act[~changed & (act != act0)] == 123).all()
+    area, perim = _numpy_recount(lat, 2, 2)
+    np.testing.assert_array_equal(tab.area, area)
+    np.testing.assert_array_equal(tab.perim, perim)
+    lat2, act2, p2, _, _ = _act_setup(32, 2)
+    tab2 = O.tally(lat2, 2, 2)
+    with pytest.raises(ValueError):
+        O.sweep(lat2, 2, tab2, p2, 1, 1, seed=4, mcs0=0, nmcs=1)
+    # thread-count invariance with act (phase independence at c >= 2)
+    la3, ac3, p3, _, _ = _act_setup(32, 2, seed=2)
+    lb3, ab3 = la3.copy(), ac3.copy()
+    p3b = O.Params.make(10, J=[[0, 4, 4], [4, 8, 6], [4, 6, 8]], lam_area=[0, 1, 1],
+                        target_area=[0, 30, 30], lam_act=np.array(p3.lam_act),
+                        act_max=np.array(p3.act_max), act=ab3)
+    ta, tb = O.tally(la3, 2, 2), O.tally(lb3, 2, 2)
+    O.sweep(la3, 2, ta, p3, 2, 3, seed=9, mcs0=50, nmcs=20, nthreads=1)
+    O.sweep(lb3, 2, tb, p3b, 2, 3, seed=9, mcs0=50, nmcs=20, nthreads=8)
+    np.testing.assert_array_equal(la3, lb3)
+    np.testing.assert_array_equal(ac3, ab3)
+
+
+def test_act_cell_migrates():
+    """Act cells (lambda_act > 0) move persistently: over 400 MCS the
+    centroid of a single act cell travels farther than without the act
+    term."""
+    L = 64
+    dist = {}
+    for lam in (0.0, 300.0):
+        lat = np.zeros((L, L), np.uint32)
+        lat[28:36, 28:36] = (1 << 24) | 1
+        act = np.zeros((L, L), np.uint32)
+        p = O.Params.make(20, J=[[0, 10], [10, 0]], lam_area=[0, 5], target_area=[0, 64],
+                          lam_act=[0, lam], act_max=[0, 40], act=act)
+        tab = O.tally(lat, 2, 1)
+        c0 = O.centroids(L, 2, tab)[1].copy()
+        path = 0.0
+        prev = c0
+        for k in range(8):
+            O.sweep(lat, 2, tab, p, 2, 1, seed=11, mcs0=1 + 50 * k, nmcs=50, nthreads=4)
+            c = O.centroids(L, 2, tab)[1].copy()
+            dv = (c - prev + L / 2) % L - L / 2
+            path += float(np.hypot(*dv))
+            prev = c
+        dist[lam] = path
+    assert dist[300.0] > 4.0 * dist[0.0], dist
