@@ -22,6 +22,9 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     # packed FP32 instructions stay disabled in all HIP code built here
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no contraction of floating-point multiply-adds: the preferred-direction
+    # update is the same double arithmetic as the oracle's
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 

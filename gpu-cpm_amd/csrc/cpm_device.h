@@ -39,6 +39,8 @@ struct DevParams {
     // Act: llround(lambda scale / actMax 2^16) and actMax per type.
     int64_t act_m[kNTypes];
     int32_t act_max[kNTypes];
+    // Persistence multipliers, the chemotaxis formula with lambda_persistence.
+    int64_t pers_m[kNTypes][4];
 };
 
 __host__ __device__ inline int param_words(int nt) { return 2 * nt * nt + 4 * nt; }
@@ -82,6 +84,8 @@ struct SweepConsts {
     uint32_t fixed_mask;    // bit t set: type t is fixed (settings.h:24)
     int chem;               // chemotaxis term on
     const int32_t* field;   // field [d][L^d], quantized to 2^-16 (or null)
+    int pers;               // persistence term on (3D)
+    const int32_t* pref;    // preferred directions [(n+1)*3], llround(p 2^16)
     int act;                // act term on
     uint32_t* act_lat;      // act lattice [L^d]: MCS of the last copy into each site
     const int64_t* act_tab; // 2^(j/256) 2^30 [256], then llround(log2(v) 2^24) [v]

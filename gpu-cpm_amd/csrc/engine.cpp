@@ -56,6 +56,35 @@ struct Plan {
 };
 constexpr int kMaxThr = 1 << 22;
 
+// Philox4x32-10 on the host (Salmon et al., SC'11), for draws made outside
+// the sweep (initial preferred directions).
+void host_philox(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t k0, uint32_t k1,
+                 uint32_t w[4]) {
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        x1 = (uint32_t)p1;
+        x3 = (uint32_t)p0;
+        x0 = y0;
+        x2 = y2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    w[0] = x0;
+    w[1] = x1;
+    w[2] = x2;
+    w[3] = x3;
+}
+
+// kernels.cu:150-156
+double wrap_half(double x, int L) {
+    if (x > L / 2) x -= L;
+    if (x < -L / 2) x += L;
+    return x;
+}
+
 }  // namespace
 
 // Host mirror of the hot-path fields of the reference Settings
@@ -72,6 +101,9 @@ struct HostSettings {
     double chem[CPM_NTYPES];    // chemotaxis lambda (cpm.cu:155)
     double lam_act[CPM_NTYPES];  // actLambda (cpm.cu:131-135)
     int32_t act_max[CPM_NTYPES]; // actMax
+    double lam_pers[CPM_NTYPES];   // persistenceLambda (cpm.cu:112-115)
+    double pers_diff[CPM_NTYPES];  // persistenceDiffusion
+    int32_t pers_time[CPM_NTYPES]; // persistenceTime (stored; the device path does not read it)
 };
 
 struct cpm_engine {
@@ -94,6 +126,16 @@ struct cpm_engine {
     uint32_t* d_act = nullptr;
     int64_t* d_act_tab = nullptr;        // act 2^x table [256] + log2 table [amax + 1]
     int act_tab_cap = 0;
+    // Persistence (3D): preferred directions, the centroid history ring of
+    // `history` entries (cpm.cu:220-223), cell types.
+    int history = 1;
+    std::vector<double> pref;            // [(n+1)*3]
+    std::vector<double> ring;            // [history][(n+1)*3]
+    int ring_active = 0;
+    bool ring_valid = false;             // filled at the first persistence run after a push
+    bool pref_dirty = true;              // d_pref needs an upload
+    int32_t* d_pref = nullptr;           // [(n+1)*3] llround(p 2^16)
+    std::vector<int32_t> ctype;          // [n+1] type of each cell at push
     hipStream_t stream = nullptr;
     uint32_t* d_lat = nullptr;
     int64_t* d_tab[2] = {nullptr, nullptr};
@@ -155,11 +197,117 @@ struct cpm_engine {
         (void)hipFree(d_thr);
         (void)hipFree(d_field);
         (void)hipFree(d_act);
+        (void)hipFree(d_pref);
         (void)hipFree(d_act_tab);
         if (h_act) (void)hipHostFree(h_act);
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    bool pers_on() const {
+        for (int t = 0; t < CPM_NTYPES; t++)
+            if (st.lam_pers[t] != 0.0) return true;
+        return false;
+    }
+
+    // Initial preferred directions from the seed (cpm.cu:225-246 draws unit
+    // vectors from random_device): uniform on the sphere by rejection, basic
+    // arithmetic only.  Cell c, attempt j: Philox4x32-10 counter
+    // (c, 0x50524546, j, 0) gives x, y, z = w / 2^31 - 1; the first attempt with
+    // 0 < s = |(x, y, z)|^2 <= 1 gives p = (x, y, z) / sqrt(s).
+    void init_pref_dirs() {
+        for (int c = 0; c <= n; c++) {
+            for (uint32_t j = 0;; j++) {
+                uint32_t w[4];
+                host_philox(c, 0x50524546u, j, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), w);
+                const double x = (double)w[0] / 2147483648.0 - 1.0;
+                const double y = (double)w[1] / 2147483648.0 - 1.0;
+                const double z = (double)w[2] / 2147483648.0 - 1.0;
+                const double s2 = x * x + y * y + z * z;
+                if (s2 > 0.0 && s2 <= 1.0) {
+                    const double nn = std::sqrt(s2);
+                    pref[3 * c + 0] = x / nn;
+                    pref[3 * c + 1] = y / nn;
+                    pref[3 * c + 2] = z / nn;
+                    break;
+                }
+            }
+        }
+        pref_dirty = true;
+    }
+
+    // Centroids c = anchor + csum / area of the current lattice (settled
+    // tables), into out [(n+1)*3]; dead cells get 0.
+    int centroids_now(std::vector<double>& out, std::vector<int32_t>& areas) {
+        const size_t rows = (size_t)n + 1;
+        std::vector<int64_t> tab(rows), csum(rows * 3);
+        std::vector<int32_t> anchor(rows * 3);
+        HIP_TRY(hipMemsetAsync(d_csum, 0, rows * 3 * 8, stream));
+        HIP_TRY(gcpm::launch_csum(d, d_lat, lgL, d_anchor, d_csum, stream));
+        HIP_TRY(hipMemcpyAsync(tab.data(), d_tab[0], rows * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(csum.data(), d_csum, rows * 3 * 8, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(anchor.data(), d_anchor, rows * 3 * 4, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        out.assign(rows * 3, 0.0);
+        areas.assign(rows, 0);
+        for (size_t c = 0; c < rows; c++) {
+            const int32_t area = (int32_t)(uint32_t)tab[c];
+            areas[c] = area;
+            for (int a = 0; a < 3; a++)
+                out[3 * c + a] = (a < d && area > 0)
+                                     ? (double)anchor[3 * c + a] + (double)csum[3 * c + a] / (double)area
+                                     : 0.0;
+        }
+        return CPM_OK;
+    }
+
+    int upload_pref() {
+        std::vector<int32_t> q(pref.size());
+        for (size_t i = 0; i < q.size(); i++) q[i] = (int32_t)std::llround(pref[i] * 65536.0);
+        HIP_TRY(hipMemcpyAsync(d_pref, q.data(), q.size() * 4, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        pref_dirty = false;
+        return CPM_OK;
+    }
+
+    // Fill the history ring with the current centroids (first persistence run
+    // after a push).
+    int init_ring() {
+        std::vector<double> c;
+        std::vector<int32_t> areas;
+        if (int rc = centroids_now(c, areas)) return rc;
+        ring.resize((size_t)history * c.size());
+        for (int i = 0; i < history; i++) std::copy(c.begin(), c.end(), ring.begin() + (size_t)i * c.size());
+        ring_active = 0;
+        ring_valid = true;
+        return CPM_OK;
+    }
+
+    // Preferred-direction update after a run chunk (kernels.cu:158-212,
+    // cpm.cu:487-514; displacement wrapped before its length is taken).
+    int update_pref_dirs() {
+        std::vector<double> now;
+        std::vector<int32_t> areas;
+        if (int rc = centroids_now(now, areas)) return rc;
+        const size_t m = now.size();
+        std::copy(now.begin(), now.end(), ring.begin() + (size_t)ring_active * m);
+        ring_active = (ring_active + 1) % history;
+        const double* prev = ring.data() + (size_t)ring_active * m;
+        for (int c = 1; c <= n; c++) {
+            if (areas[c] <= 0) continue;
+            double u[3];
+            for (int a = 0; a < 3; a++) u[a] = wrap_half(now[3 * c + a] - prev[3 * c + a], L);
+            const double len = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            if (len == 0.0) continue;
+            const double D = st.pers_diff[ctype[c]];
+            double np[3];
+            for (int a = 0; a < 3; a++) np[a] = u[a] / len * (1.0 - D) + pref[3 * c + a] * D;
+            const double nl = std::sqrt(np[0] * np[0] + np[1] * np[1] + np[2] * np[2]);
+            if (nl == 0.0) continue;
+            for (int a = 0; a < 3; a++) pref[3 * c + a] = np[a] / nl;
+        }
+        return upload_pref();
     }
 
     bool act_on() const {
@@ -177,7 +325,7 @@ struct cpm_engine {
     // Smallest dyadic scale 2^k making every lambda integral (k <= 8); with a
     // chemotaxis term 2^8, which resolves it to 1/256 energy units.
     int energy_scale() const {
-        for (int k = chem_on() || act_on() ? kMaxScaleLog2 : 0; k <= kMaxScaleLog2; k++) {
+        for (int k = chem_on() || act_on() || pers_on() ? kMaxScaleLog2 : 0; k <= kMaxScaleLog2; k++) {
             const double s = (double)(1 << k);
             bool ok = true;
             for (int t = 0; t < CPM_NTYPES && ok; t++) {
@@ -265,10 +413,14 @@ struct cpm_engine {
             p.w[2 * n2 + 3 * nt + a] = st.target_perim[a];
         }
         for (int a = 0; a < CPM_NTYPES; a++)
-            for (int kk = 0; kk < 4; kk++)
+            for (int kk = 0; kk < 4; kk++) {
                 p.chem_m[a][kk] = kk ? (int64_t)std::llround(st.chem[a] * (double)sc * 256.0 /
                                                              std::sqrt((double)kk))
                                      : 0;
+                p.pers_m[a][kk] = kk ? (int64_t)std::llround(st.lam_pers[a] * (double)sc * 256.0 /
+                                                             std::sqrt((double)kk))
+                                     : 0;
+            }
         // Act: multipliers llround(lambda scale / actMax 2^16), the 2^x table
         // llround(2^(j/256) 2^30) and the log2 table llround(log2(v) 2^24).
         int amax = 0;
@@ -406,6 +558,10 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     g->st.temperature = (double)temperature;
     std::random_device rd;  // like cpm.cu:347-350, but once per engine
     g->seed = ((uint64_t)rd() << 32) ^ rd();
+    g->history = history;
+    g->pref.assign(((size_t)number_of_cells + 1) * 3, 0.0);
+    g->ctype.assign((size_t)number_of_cells + 1, 0);
+    g->init_pref_dirs();
 
     auto bail = [&](hipError_t err, const char* what) {
         int rc = hip_fail(err, what);
@@ -429,6 +585,7 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     if ((err = hipMalloc(&g->d_prm, sizeof(gcpm::DevParams))) != hipSuccess) return bail(err, "hipMalloc(params)");
     if ((err = hipMalloc(&g->d_bar, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
     if ((err = hipMemset(g->d_bar, 0, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
+    if ((err = hipMalloc(&g->d_pref, rows * 3 * 4)) != hipSuccess) return bail(err, "hipMalloc(pref)");
     if (has_act) {  // cpm.cu:192-193,204-205
         g->has_act = true;
         if ((err = hipHostMalloc((void**)&g->h_act, g->sites * 4, hipHostMallocDefault)) != hipSuccess)
@@ -547,6 +704,36 @@ int cpm_host_act(cpm_handle h, uint32_t** out) {
     return CPM_OK;
 }
 
+int cpm_set_persistence(cpm_handle h, int type, int time, double diffusion, double lambda) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = check_type(type, "cell_type")) return rc;
+    if (!std::isfinite(lambda) || std::fabs(lambda) > 8192.0)
+        return fail(CPM_EINVAL, "lambda_persistence must be finite with |lambda| <= 8192");
+    if (!std::isfinite(diffusion)) return fail(CPM_EINVAL, "persistence_diffusion must be finite");
+    h->st.pers_time[type] = time;
+    h->st.pers_diff[type] = diffusion;
+    h->st.lam_pers[type] = lambda;
+    h->params_dirty = true;
+    return CPM_OK;
+}
+
+int cpm_get_pref_dirs(cpm_handle h, double* out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(CPM_EINVAL, "null output");
+    std::memcpy(out, h->pref.data(), h->pref.size() * sizeof(double));
+    return CPM_OK;
+}
+
+int cpm_set_pref_dirs(cpm_handle h, const double* in) {
+    if (int rc = check_handle(h)) return rc;
+    if (!in) return fail(CPM_EINVAL, "null input");
+    for (size_t i = 0; i < h->pref.size(); i++)
+        if (!(std::fabs(in[i]) < 32768.0)) return fail(CPM_EINVAL, "preferred directions must satisfy |p| < 2^15");
+    std::memcpy(h->pref.data(), in, h->pref.size() * sizeof(double));
+    h->pref_dirty = true;
+    return CPM_OK;
+}
+
 int cpm_host_field(cpm_handle h, double** out) {
     if (int rc = check_handle(h)) return rc;
     if (!out) return fail(CPM_EINVAL, "null output");
@@ -603,6 +790,11 @@ int cpm_push(cpm_handle h) {
         h->nt = (int)tmax + 1;  // flips only copy existing values: no new types appear
         h->params_dirty = true;
     }
+    for (size_t i = 0; i < h->sites; i++) {  // a cell keeps its type
+        const uint32_t v = h->h_lat[i];
+        if (v & gcpm::kSpinMask) h->ctype[v & gcpm::kSpinMask] = (int32_t)(v >> 24);
+    }
+    h->ring_valid = false;
     // The field goes with the lattice (cpm.cu:542-543), quantized to 2^-16.
     std::vector<int32_t> fq;
     if (h->has_field) {
@@ -688,6 +880,12 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
                 return fail(CPM_EINVAL, "|lambda_act / max_act| too large (at most 63)");
         }
     }
+    // Persistence acts in 3D only, as in the reference (kernels3d.cu:327-344;
+    // the prefDir update runs for dimensionality 3, cpm.cu:487).
+    k.pers = h->d == 3 && h->pers_on() ? 1 : 0;
+    k.pref = h->d_pref;
+    if (k.pers && h->pref_dirty)
+        if (int rc = h->upload_pref()) return rc;
     k.act_lat = h->d_act;
     k.act_tab = h->d_act_tab;
     k.act_ln2q = (int64_t)std::llround(std::log(2.0) * 1073741824.0);
@@ -754,6 +952,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         }
         const int ncol = 1 << h->d;
         int g = 0;  // phase counter since the tables were last settled
+        const bool pers = k.pers != 0;
+        if (pers && !h->ring_valid)
+            if (int rc = h->init_ring()) return rc;
         for (long m = 0; m < total; m++) {
             pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
             for (int r = 0; r < k.rounds; r++) {
@@ -782,6 +983,13 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                 }
             }
             g &= 1;  // only the parity matters
+            if (pers && (m + 1) % a->inner_iterations == 0) {
+                // end of a chunk (one reference launch): settle, then update
+                // the preferred directions from the centroid history
+                HIP_TRY(gcpm::launch_settle(h->d_tab, h->d_del, g, h->n, h->stream));
+                g = 0;
+                if (int rc = h->update_pref_dirs()) return rc;
+            }
         }
         // Fold the last phase's deltas into the lagging base: both bases
         // then hold the current state and both delta buffers are zero.
@@ -935,6 +1143,7 @@ int cpm_set_launch(cpm_handle h, int form, int tile) {
 int cpm_set_seed(cpm_handle h, uint64_t seed) {
     if (int rc = check_handle(h)) return rc;
     h->seed = seed;
+    h->init_pref_dirs();  // the initial preferred directions are drawn from the seed
     return CPM_OK;
 }
 
@@ -1073,6 +1282,8 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     std::vector<long> need(n);
     for (int i = 0; i < n; i++) {
         if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
+        if (reps[i].k.pers)
+            return fail(CPM_ENOTIMPL, "the persistence term is not available for batched runs");
         const long nsub = 1L << (reps[i].lh * b->d);
         const long w = (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
         need[i] = std::max(1L, (long)std::ceil(w * b->block_scale));

@@ -459,6 +459,26 @@ __device__ __forceinline__ int64_t act_energy(const SweepConsts& k, const Lat& l
     return (-m * (g_s - g_t) + ((int64_t)1 << 31)) >> 32;
 }
 
+// Persistence (kernels3d.cu:327-344, 3D): -lambda (d . p) with p the
+// preferred direction of the source cell (of the target cell when the source
+// is medium) and lambda of the source type (target type likewise); fixed
+// point as for chemotaxis, p held as llround(p 2^16).
+template <int D>
+__device__ __forceinline__ int64_t pers_energy(const SweepConsts& k, int dir, uint32_t vs,
+                                               uint32_t vt) {
+    const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
+    const int tl = cs ? (int)(vs >> 24) : (int)(vt >> 24);
+    const uint32_t cell = cs ? cs : ct;
+    int ox, oy, oz;
+    block_offset<D>(dir, ox, oy, oz);
+    const int kk = (ox != 0) + (oy != 0) + (oz != 0);
+    const int64_t m = k.prm->pers_m[tl][kk];
+    if (m == 0 || cell == 0) return 0;
+    const int32_t* p = k.pref + 3 * cell;
+    const int64_t g = (int64_t)ox * p[0] + (int64_t)oy * p[1] + (int64_t)oz * p[2];
+    return (m * g + ((int64_t)1 << 23)) >> 24;
+}
+
 template <int D, bool CO, class Lat, class PV>
 __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts& k, const PV& P,
                                                const uint32_t* __restrict__ thr,
@@ -525,6 +545,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     }
     if (k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
     if (k.act) dh += act_energy<D, CO>(k, lat, d.t, d.nv, d.dir, vs, vt, d.tick);
+    if (D == 3 && k.pers) dh += pers_energy<D>(k, d.dir, vs, vt);
     const bool accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
     if (!accept) return;
     if (cy.pass) {

@@ -125,11 +125,11 @@ class Cpm:
         stick = _int(stickiness, "stickiness")
 
         # Terms outside the implemented Hamiltonian are refused, not ignored.
-        if lpers != -1 and pdiff != -1 and ptime != -1 and lpers != 0:
-            raise NotImplementedError("the persistence term is not implemented")
 
         if fx >= 0:
             check(lib.cpm_set_fixed(self._h, ct, fx))
+        if lpers != -1 and pdiff != -1 and ptime != -1:
+            check(lib.cpm_set_persistence(self._h, ct, ptime, pdiff, lpers))
         if lchem != -1:
             check(lib.cpm_set_chemotaxis(self._h, ct, lchem))
         if lp != -1 and tp != -1:
@@ -224,6 +224,19 @@ class Cpm:
         if self._field is None:
             raise ValueError("get_field: this Cpm was created with has_field=False")
         return self._field
+
+    def get_pref_dirs(self):
+        """Extension: float64 (number_of_cells + 1, 3) preferred directions of
+        the persistence term (row 0 unused); drawn from the seed, then
+        updated after every inner_iterations chunk (3D)."""
+        out = np.zeros((self.number_of_cells + 1, 3), np.float64)
+        check(lib.cpm_get_pref_dirs(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def set_pref_dirs(self, dirs):
+        """Extension: set the preferred directions ((number_of_cells + 1, 3))."""
+        a = np.ascontiguousarray(dirs, np.float64).reshape(self.number_of_cells + 1, 3)
+        check(lib.cpm_set_pref_dirs(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
     def get_act_state(self):
         """int32 view of the host act lattice (python_wrapper.cpp:312-328):

@@ -66,6 +66,9 @@ SIGNATURES = [
     ("cpm_host_field", _I, [_H, _P(_P(ctypes.c_double))]),
     ("cpm_set_act", _I, [_H, _I, _I, ctypes.c_double]),
     ("cpm_host_act", _I, [_H, _P(_P(ctypes.c_uint32))]),
+    ("cpm_set_persistence", _I, [_H, _I, _I, ctypes.c_double, ctypes.c_double]),
+    ("cpm_get_pref_dirs", _I, [_H, _P(ctypes.c_double)]),
+    ("cpm_set_pref_dirs", _I, [_H, _P(ctypes.c_double)]),
     ("cpm_batch_create", _I, [_P(_H), _I, _P(_H)]),
     ("cpm_batch_run", _I, [_H, _P(RunArgs)]),
     ("cpm_batch_sync", _I, [_H]),

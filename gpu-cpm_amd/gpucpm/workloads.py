@@ -14,6 +14,7 @@ Each builder returns a ``Workload``: the initial lattice (uint32, ids
 - ``walled``     any of these inside a one-site frame of a fixed cell type.
 - ``chemotactic`` any of these with a chemotaxis term and a smooth field.
 - ``active``     any of these with the act term and a random act lattice.
+- ``persistent`` any 3D one with the persistence term.
 The shapes may be scaled down (``L`` / ``ncells`` arguments) for tests.
 """
 from __future__ import annotations
@@ -34,12 +35,14 @@ class Workload:
     constraints: list = field(default_factory=list)
     field: np.ndarray = None  # chemotaxis field [d][L]^d (float64), or None
     act: np.ndarray = None    # initial act lattice (uint32), or None
+    history: int = 1          # centroid history length of the persistence term
 
     def make_cpm(self, cls=None):
         """Build a Cpm with this workload's lattice and constraints (not pushed)."""
         if cls is None:
             from . import Cpm as cls
-        sim = cls(self.L, self.d, self.ncells, self.temperature, self.field is not None)
+        sim = cls(self.L, self.d, self.ncells, self.temperature, self.field is not None,
+                  self.history)
         for kw in self.constraints:
             kw = dict(kw)
             sim.set_constraints(kw.pop("cell_type"), **kw)
@@ -63,6 +66,8 @@ class Workload:
         chem = np.zeros(20)
         lam_act = np.zeros(20)
         act_max = np.zeros(20, np.int64)
+        lam_pers = np.zeros(20)
+        pers_diff = np.zeros(20)
         for kw in self.constraints:
             t = kw["cell_type"]
             if "adhesion" in kw:
@@ -80,9 +85,13 @@ class Workload:
                 chem[t] = kw["lambda_chemotaxis"]
             if kw.get("lambda_act", -1) != -1 and kw.get("max_act", -1) != -1:
                 lam_act[t], act_max[t] = kw["lambda_act"], kw["max_act"]
+            if all(kw.get(k, -1) != -1 for k in ("lambda_persistence", "persistence_diffusion",
+                                                  "persistence_time")):
+                lam_pers[t], pers_diff[t] = kw["lambda_persistence"], kw["persistence_diffusion"]
         return dict(J=J, S=S, lam_area=la, target_area=a0, lam_perim=lp, target_perim=p0,
                     temperature=float(self.temperature), fixed=fixed, lam_chem=chem,
-                    field=self.field, lam_act=lam_act, act_max=act_max)
+                    field=self.field, lam_act=lam_act, act_max=act_max, lam_pers=lam_pers,
+                    pers_diff=pers_diff)
 
 
 def _place_boxes(L, d, n, edge, rng, types):
@@ -174,7 +183,7 @@ def walled(wl: Workload, wall_type=3, J_wall=(8, 4, 12)) -> Workload:
     for other, j in zip((0, 1, 2), J_wall):
         cons.append(dict(cell_type=wall_type, other_cell_type=other, adhesion=int(j)))
     return Workload(wl.name + "_walled", wl.L, wl.d, wid, wl.temperature, lat, cons, wl.field,
-                    wl.act)
+                    wl.act, wl.history)
 
 
 def chemotactic(wl: Workload, lam=(0.0, 20.0, 5.0), seed=7, amplitude=2.0) -> Workload:
@@ -194,7 +203,7 @@ def chemotactic(wl: Workload, lam=(0.0, 20.0, 5.0), seed=7, amplitude=2.0) -> Wo
     cons = list(wl.constraints) + [dict(cell_type=t, lambda_chemotaxis=float(v))
                                    for t, v in enumerate(lam) if v != 0.0]
     return Workload(wl.name + "_chemo", L, d, wl.ncells, wl.temperature, wl.lattice.copy(), cons, f,
-                    wl.act)
+                    wl.act, wl.history)
 
 
 def active(wl: Workload, lam=(0.0, 300.0, 150.0), act_max=(0, 40, 30), seed=9) -> Workload:
@@ -206,7 +215,19 @@ def active(wl: Workload, lam=(0.0, 300.0, 150.0), act_max=(0, 40, 30), seed=9) -
                                    for t, (l, m) in enumerate(zip(lam, act_max)) if l != 0.0]
     act = rng.integers(0, 40, size=wl.lattice.shape).astype(np.uint32)
     return Workload(wl.name + "_act", wl.L, wl.d, wl.ncells, wl.temperature, wl.lattice.copy(),
-                    cons, wl.field, act)
+                    cons, wl.field, act, wl.history)
+
+
+def persistent(wl: Workload, lam=(0.0, 200.0, 100.0), diffusion=(0.0, 0.8, 0.5), history=3,
+               time=10) -> Workload:
+    """``wl`` with the persistence term (3D): lambda_persistence ``lam[t]``
+    and persistence_diffusion ``diffusion[t]`` for types 0, 1, 2, and a
+    centroid history of ``history`` chunks."""
+    cons = list(wl.constraints) + [dict(cell_type=t, lambda_persistence=float(l),
+                                        persistence_diffusion=float(D), persistence_time=int(time))
+                                   for t, (l, D) in enumerate(zip(lam, diffusion)) if l != 0.0]
+    return Workload(wl.name + "_pers", wl.L, wl.d, wl.ncells, wl.temperature, wl.lattice.copy(),
+                    cons, wl.field, wl.act, history)
 
 
 def plain3d(L=256, ncells=20000, seed=3):

@@ -95,6 +95,17 @@ int cpm_set_chemotaxis(cpm_handle h, int type, double lambda);
  * its same-cell neighbours.  max_act in [0, 65535].  Runs with the term need
  * has_act and positions_per_checkerboard >= 2. */
 int cpm_set_act(cpm_handle h, int type, int max_act, double lambda);
+/* Cpm::setPersistence (cpm.cu:112-115): the persistence term -lambda (d . p)
+ * (kernels3d.cu:327-344, 3D only as in the reference), p the cell's preferred
+ * direction, updated after every inner_iterations chunk from the centroid
+ * history of `history` chunks with the type's diffusion (kernels.cu:158-212,
+ * cpm.cu:487-514).  time is stored (the reference's device path ignores it).
+ * |lambda| <= 8192. */
+int cpm_set_persistence(cpm_handle h, int type, int time, double diffusion, double lambda);
+/* Extension: preferred directions, (n+1)*3 doubles (row 0 unused).  They
+ * start as unit vectors drawn from the seed (cpm_set_seed re-draws them). */
+int cpm_get_pref_dirs(cpm_handle h, double* out);
+int cpm_set_pref_dirs(cpm_handle h, const double* in);
 /* Cpm::getActData (cpm.cu:522-525): host act lattice, L^d values (the MCS of
  * the last copy into each site); pushed and pulled with the lattice.
  * CPM_ESTATE if the engine was created with has_act = 0. */

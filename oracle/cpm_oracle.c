@@ -87,6 +87,12 @@ int64_t orc_act_geo_mean(int64_t sum_lg, int count) {
     return (M << I) >> 14;
 }
 
+static int pers_on(const orc_params* p) {
+    for (int t = 0; t < ORC_NTYPES; t++)
+        if (p->lam_pers[t] != 0.0) return 1;
+    return 0;
+}
+
 static int chem_on(const orc_params* p) {
     for (int t = 0; t < ORC_NTYPES; t++)
         if (p->lam_chem[t] != 0.0) return 1;
@@ -107,7 +113,8 @@ int orc_quantize_field(int64_t n, const double* f, int32_t* out) {
 
 int orc_energy_scale(const orc_params* p) {
     /* chemotaxis and act terms are resolved at 1/256 energy units */
-    for (int k = chem_on(p) || act_on(p) ? ORC_MAX_SCALE_LOG2 : 0; k <= ORC_MAX_SCALE_LOG2; k++) {
+    for (int k = chem_on(p) || act_on(p) || pers_on(p) ? ORC_MAX_SCALE_LOG2 : 0;
+         k <= ORC_MAX_SCALE_LOG2; k++) {
         double s = (double)(1 << k);
         int ok = 1;
         for (int t = 0; t < ORC_NTYPES && ok; t++) {
@@ -236,6 +243,9 @@ typedef struct {
     int chem;                            /* chemotaxis term on */
     int64_t chem_m[ORC_NTYPES][4];       /* orc_chem_multiplier, k = 1..3 */
     const int32_t* field_q;
+    int pers;                            /* persistence term on (3D) */
+    int64_t pers_m[ORC_NTYPES][4];
+    const int32_t* pref_q;
     int act;                             /* act term on */
     int64_t act_m[ORC_NTYPES];           /* orc_act_multiplier */
     int32_t act_max[ORC_NTYPES];
@@ -252,9 +262,13 @@ static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
         q->lp[a] = (int64_t)(p->lam_perim[a] * scale);
         q->a0[a] = p->target_area[a];
         q->p0[a] = p->target_perim[a];
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 4; k++) {
             q->chem_m[a][k] = k ? orc_chem_multiplier(p->lam_chem[a], scale, k) : 0;
+            q->pers_m[a][k] = k ? orc_chem_multiplier(p->lam_pers[a], scale, k) : 0;
+        }
     }
+    q->pers = pers_on(p) && p->pref_q != NULL;
+    q->pref_q = p->pref_q;
     q->chem = chem_on(p) && p->field_q != NULL;
     q->field_q = p->field_q;
     q->act = act_on(p) && p->act != NULL;
@@ -263,6 +277,19 @@ static void scale_params(const orc_params* p, int scale, orc_scaled* q) {
         q->act_max[a] = p->act_max[a];
         q->act_m[a] = orc_act_multiplier(p->lam_act[a], p->act_max[a], scale);
     }
+}
+
+/* Persistence energy (3D) of copying vs into vt through offset o. */
+static int64_t pers_energy(const orc_scaled* q, uint32_t vs, uint32_t vt, const int o[3]) {
+    uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
+    uint32_t tl = cs ? vs >> 24 : vt >> 24;
+    uint32_t cell = cs ? cs : ct;
+    int k = (o[0] != 0) + (o[1] != 0) + (o[2] != 0);
+    int64_t m = q->pers_m[tl][k];
+    if (m == 0 || cell == 0) return 0;
+    int64_t g = 0;
+    for (int a = 0; a < 3; a++) g += (int64_t)o[a] * q->pref_q[3 * cell + a];
+    return (m * g + ((int64_t)1 << 23)) >> 24;
 }
 
 /* Geometric mean of act values over site (x, y, z) and its neighbours that
@@ -369,11 +396,10 @@ int64_t orc_delta_h(int L, int d, const uint32_t* lat, const int32_t* area,
     int sx = (int)(s % L), sy = (int)((s / L) % L), sz = d == 3 ? (int)(s / ((int64_t)L * L)) : 0;
     int as, at;
     int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, lat[s], lat[t], &as, &at);
-    if (q.chem) {
-        /* o = s - t, wrapped into {-1, 0, 1} */
-        int o[3] = {wrapc(L, sx - tx), wrapc(L, sy - ty), d == 3 ? wrapc(L, sz - tz) : 0};
-        dh += chem_energy(L, d, &q, lat[s] >> 24, lat[t] >> 24, o, s);
-    }
+    /* o = s - t, wrapped into {-1, 0, 1} */
+    int o[3] = {wrapc(L, sx - tx), wrapc(L, sy - ty), d == 3 ? wrapc(L, sz - tz) : 0};
+    if (q.chem) dh += chem_energy(L, d, &q, lat[s] >> 24, lat[t] >> 24, o, s);
+    if (q.pers && d == 3) dh += pers_energy(&q, lat[s], lat[t], o);
     return dh;
 }
 
@@ -501,6 +527,7 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                                               site(L, sx, sy, sz));
                         if (q.act)
                             dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
+                        if (q.pers && d == 3) dh += pers_energy(&q, vs, vt, nb.off[dir]);
                         int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
                         if (!accept) continue;
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
@@ -614,6 +641,7 @@ int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int
             if (q.chem)
                 dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, nb.off[dir], site(L, sx, sy, sz));
             if (q.act) dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
+            if (q.pers && d == 3) dh += pers_energy(&q, vs, vt, nb.off[dir]);
             if (!(dh <= 0 || (dh < nthr && w[2] < thr[dh]))) continue;
             lat[ti] = vs;
             if (q.act) q.act_lat[ti] = mcs;
@@ -719,4 +747,72 @@ void orc_centroids(int L, int d, int ncells, const int32_t* area, const int32_t*
             if (v >= L) v -= L;
             out[c * d + a] = v;
         }
+}
+
+/* ------------------------------------------------------ persistence -- */
+
+void orc_pref_centroids(int L, int d, int ncells, const int32_t* area, const int32_t* anchor,
+                        const int64_t* csum, double* out) {
+    (void)L;
+    for (int c = 0; c <= ncells; c++)
+        for (int a = 0; a < 3; a++)
+            out[3 * c + a] = (a < d && area[c] > 0)
+                                 ? (double)anchor[3 * c + a] + (double)csum[3 * c + a] / (double)area[c]
+                                 : 0.0;
+}
+
+static double wrap_half(double x, int L) {
+    /* kernels.cu:150-156 */
+    if (x > L / 2) x -= L;
+    if (x < -L / 2) x += L;
+    return x;
+}
+
+void orc_update_pref_dirs(int L, int d, int ncells, const int32_t* area, const int32_t* anchor,
+                          const int64_t* csum, const int32_t* ctype, const double* diffusion,
+                          double* pref, double* ring, int H, int* active, int32_t* pref_q) {
+    const size_t rows = (size_t)ncells + 1;
+    double* cur = ring + (size_t)(*active) * rows * 3;
+    orc_pref_centroids(L, d, ncells, area, anchor, csum, cur);
+    double* now = (double*)malloc(sizeof(double) * rows * 3);
+    memcpy(now, cur, sizeof(double) * rows * 3);
+    *active = (*active + 1) % H;
+    const double* prev = ring + (size_t)(*active) * rows * 3;
+    for (int c = 1; c <= ncells; c++) {
+        if (area[c] <= 0) continue;
+        double u[3];
+        for (int a = 0; a < 3; a++) u[a] = wrap_half(now[3 * c + a] - prev[3 * c + a], L);
+        double len = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (len == 0.0) continue;
+        double D = diffusion[ctype[c]];
+        double np[3];
+        for (int a = 0; a < 3; a++) np[a] = u[a] / len * (1.0 - D) + pref[3 * c + a] * D;
+        double nl = sqrt(np[0] * np[0] + np[1] * np[1] + np[2] * np[2]);
+        if (nl == 0.0) continue;
+        for (int a = 0; a < 3; a++) pref[3 * c + a] = np[a] / nl;
+    }
+    free(now);
+    for (size_t i = 0; i < rows * 3; i++) pref_q[i] = (int32_t)llround(pref[i] * 65536.0);
+}
+
+void orc_init_pref_dirs(int ncells, uint64_t seed, double* pref) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int c = 0; c <= ncells; c++) {
+        for (uint32_t j = 0;; j++) {
+            uint32_t ctr[4] = {(uint32_t)c, 0x50524546u, j, 0u};
+            uint32_t w[4];
+            orc_philox4x32_10(ctr, key, w);
+            double x = (double)w[0] / 2147483648.0 - 1.0;
+            double y = (double)w[1] / 2147483648.0 - 1.0;
+            double z = (double)w[2] / 2147483648.0 - 1.0;
+            double s2 = x * x + y * y + z * z;
+            if (s2 > 0.0 && s2 <= 1.0) {
+                double n = sqrt(s2);
+                pref[3 * c + 0] = x / n;
+                pref[3 * c + 1] = y / n;
+                pref[3 * c + 2] = z / n;
+                break;
+            }
+        }
+    }
 }

@@ -52,7 +52,38 @@ typedef struct {
     int32_t act_max[ORC_NTYPES];         /* settings.h: actMax */
     uint32_t* act;                       /* act lattice [L^d] (MCS of the last copy into
                                             each site), updated by sweeps; or NULL */
+    double lam_pers[ORC_NTYPES];         /* settings.h: persistenceLambda (cpm.cu:112-115) */
+    const int32_t* pref_q;               /* preferred directions [(ncells+1)*3] in 2^-16, or NULL */
 } orc_params;
+
+/* Persistence (kernels3d.cu:327-344; 3D only, as in the reference): a copy
+ * adds -lambda (d . p), d the unit vector from source to target, p the
+ * preferred direction of the source cell (of the target cell when the source
+ * is medium), lambda of the source type (of the target type when the source
+ * is medium).  In fixed point exactly as chemotaxis, with the field replaced
+ * by p quantized to 2^-16: E = (m sum_i o_i p_q,i + 2^23) >> 24.  The scale
+ * is 2^8. */
+
+/* Preferred-direction update after each run chunk (kernels.cu:158-212,
+ * cpm.cu:487-514): centroids c = anchor + csum / area; the ring of H past
+ * centroids takes the current ones at *active, *active advances, and the
+ * entry now at *active (H chunks old) is the previous position; for each
+ * live cell, u = wrap(c - prev) / |wrap(c - prev)| (no update when zero),
+ * p = normalize(u (1 - D) + p D) with D the persistence diffusion of the
+ * cell's type.  ctype[cell] is the cell's type.  pref and ring are doubles
+ * [(ncells+1)*3] and [H*(ncells+1)*3]; pref_q receives llround(p 2^16). */
+void orc_pref_centroids(int L, int d, int ncells, const int32_t* area, const int32_t* anchor,
+                        const int64_t* csum, double* out);
+void orc_update_pref_dirs(int L, int d, int ncells, const int32_t* area, const int32_t* anchor,
+                          const int64_t* csum, const int32_t* ctype, const double* diffusion,
+                          double* pref, double* ring, int H, int* active, int32_t* pref_q);
+/* Initial preferred directions drawn from the seed (cpm.cu:225-246 draws
+ * unit vectors from random_device): uniform on the sphere by rejection, with
+ * basic arithmetic only so that every compiler gives the same bits.  For
+ * cell c, attempt j = 0, 1, ...: Philox4x32-10 with counter
+ * (c, 0x50524546, j, 0) gives x, y, z = w / 2^31 - 1 for w0..w2; the first
+ * attempt with 0 < s = x^2 + y^2 + z^2 <= 1 gives p = (x, y, z) / sqrt(s). */
+void orc_init_pref_dirs(int ncells, uint64_t seed, double* pref);
 
 /* Act term (kernels2d.cu:312-343,386-397,417-473,532-534) in fixed point.
  * Each site's act value is v = min(max(actMax + act[site] - tick, 0), actMax)

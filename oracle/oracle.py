@@ -45,7 +45,15 @@ class Params(ctypes.Structure):
         ("lam_act", ctypes.c_double * NTYPES),
         ("act_max", ctypes.c_int32 * NTYPES),
         ("act", ctypes.POINTER(ctypes.c_uint32)),
+        ("lam_pers", ctypes.c_double * NTYPES),
+        ("pref_q", ctypes.POINTER(ctypes.c_int32)),
     ]
+
+    def set_pref(self, pref_q):
+        """Attach quantized preferred directions (int32 [(ncells+1)*3])."""
+        assert pref_q.dtype == np.int32 and pref_q.flags["C_CONTIGUOUS"]
+        self._pref_q = pref_q
+        self.pref_q = _ptr(pref_q, ctypes.c_int32)
 
     def set_act(self, act):
         """Attach the act lattice (uint32, C-contiguous, the lattice's shape);
@@ -63,7 +71,7 @@ class Params(ctypes.Structure):
     @classmethod
     def make(cls, temperature, J=None, S=None, lam_area=None, target_area=None,
              lam_perim=None, target_perim=None, fixed=None, lam_chem=None, field=None,
-             lam_act=None, act_max=None, act=None):
+             lam_act=None, act_max=None, act=None, lam_pers=None):
         p = cls()
         p.temperature = float(temperature)
         for name, val in (("J", J), ("S", S)):
@@ -77,7 +85,7 @@ class Params(ctypes.Structure):
         for name, val in (("lam_area", lam_area), ("target_area", target_area),
                           ("lam_perim", lam_perim), ("target_perim", target_perim),
                           ("fixed", fixed), ("lam_chem", lam_chem), ("lam_act", lam_act),
-                          ("act_max", act_max)):
+                          ("act_max", act_max), ("lam_pers", lam_pers)):
             if val is not None:
                 for t, x in enumerate(val):
                     getattr(p, name)[t] = float(x) if name.startswith("lam") else int(x)
@@ -145,6 +153,13 @@ def lib():
         L.orc_act_energy.argtypes = [ctypes.c_int, ctypes.c_int, u32p, P(Params), ctypes.c_int,
                                      ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32]
         L.orc_act_energy.restype = ctypes.c_int64
+        L.orc_pref_centroids.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i64p, f64p]
+        L.orc_pref_centroids.restype = None
+        L.orc_update_pref_dirs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i64p,
+                                           i32p, f64p, f64p, f64p, ctypes.c_int, P(ctypes.c_int), i32p]
+        L.orc_update_pref_dirs.restype = None
+        L.orc_init_pref_dirs.argtypes = [ctypes.c_int, ctypes.c_uint64, f64p]
+        L.orc_init_pref_dirs.restype = None
         _lib = L
     return _lib
 
@@ -192,6 +207,43 @@ def act_energy(lat, d, p: Params, scale: int, t: int, s: int, tick: int) -> int:
     lat32 = np.ascontiguousarray(lat).view(np.uint32)
     return lib().orc_act_energy(lat32.shape[0], d, _ptr(lat32, ctypes.c_uint32), ctypes.byref(p),
                                 scale, t, s, tick & 0xFFFFFFFF)
+
+
+def init_pref_dirs(ncells: int, seed: int) -> np.ndarray:
+    out = np.zeros((ncells + 1) * 3, np.float64)
+    lib().orc_init_pref_dirs(int(ncells), int(seed) & (2**64 - 1), _ptr(out, ctypes.c_double))
+    return out
+
+
+def pref_centroids(L, d, tables: Tables) -> np.ndarray:
+    out = np.zeros((tables.ncells + 1) * 3, np.float64)
+    lib().orc_pref_centroids(L, d, tables.ncells, _ptr(tables.area, ctypes.c_int32),
+                             _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                             _ptr(out, ctypes.c_double))
+    return out
+
+
+class PrefState:
+    """Preferred directions and the centroid history ring (H entries, all
+    holding the centroids when the state is made)."""
+
+    def __init__(self, L, d, tables: Tables, H: int, seed: int):
+        self.L, self.d, self.H = L, d, H
+        self.pref = init_pref_dirs(tables.ncells, seed)
+        self.ring = np.tile(pref_centroids(L, d, tables), H).copy()
+        self.active = 0
+        self.pref_q = quantize_field(self.pref)  # llround(p 2^16), as after each update
+
+    def update(self, tables: Tables, ctype, diffusion):
+        act = ctypes.c_int(self.active)
+        ct = np.ascontiguousarray(ctype, np.int32)
+        df = np.ascontiguousarray(diffusion, np.float64)
+        lib().orc_update_pref_dirs(self.L, self.d, tables.ncells, _ptr(tables.area, ctypes.c_int32),
+                                   _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                                   _ptr(ct, ctypes.c_int32), _ptr(df, ctypes.c_double),
+                                   _ptr(self.pref, ctypes.c_double), _ptr(self.ring, ctypes.c_double),
+                                   self.H, ctypes.byref(act), _ptr(self.pref_q, ctypes.c_int32))
+        self.active = act.value
 
 
 def act_multiplier(lam: float, act_max: int, scale: int) -> int:

@@ -11,11 +11,14 @@ def oracle_params(wl) -> "O.Params":
     return O.Params.make(p["temperature"], J=p["J"], S=p["S"], lam_area=p["lam_area"],
                          target_area=p["target_area"], lam_perim=p["lam_perim"],
                          target_perim=p["target_perim"], fixed=p["fixed"], lam_chem=p["lam_chem"],
-                         field=p["field"], lam_act=p["lam_act"], act_max=p["act_max"])
+                         field=p["field"], lam_act=p["lam_act"], act_max=p["act_max"],
+                         lam_pers=p["lam_pers"])
 
 
-def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None):
-    """Tally + sweep in the oracle; returns (lattice, Tables)."""
+def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=None):
+    """Tally + sweep in the oracle; returns (lattice, Tables).  With a 3D
+    persistence term the run is cut into chunks of ``inner`` MCS with a
+    preferred-direction update after each (tables.pref holds the result)."""
     lat = np.ascontiguousarray(wl.lattice if lattice is None else lattice).astype(np.uint32).copy()
     tab = O.tally(lat, wl.d, wl.ncells)
     p = oracle_params(wl)
@@ -23,7 +26,21 @@ def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None):
     if getattr(wl, "act", None) is not None:
         act = np.ascontiguousarray(wl.act, np.uint32).copy()
         p.set_act(act)
-    if nmcs:
+    pw = wl.params()
+    pers = wl.d == 3 and bool(np.any(pw["lam_pers"] != 0))
+    if pers:
+        ps = O.PrefState(wl.L, wl.d, tab, wl.history, seed)
+        p.set_pref(ps.pref_q)
+        ctype = np.zeros(wl.ncells + 1, np.int32)
+        v = lat.reshape(-1)
+        ctype[v & 0xFFFFFF] = (v >> 24).astype(np.int32)
+        ctype[0] = 0
+        inner = inner or nmcs
+        for m in range(0, nmcs, inner):
+            O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0 + m, inner, nthreads)
+            ps.update(tab, ctype, pw["pers_diff"])
+        tab.pref = ps.pref
+    elif nmcs:
         O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0, nmcs, nthreads)
     if act is not None:
         tab.act = act

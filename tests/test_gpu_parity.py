@@ -319,3 +319,40 @@ def test_act_refuses_single_site_checkerboard():
     sim.push_to_gpu()
     with pytest.raises(NotImplementedError):
         sim.run(**run_kwargs(1, 1, 1))
+
+
+@pytest.mark.parametrize("small,compact,att", [("1", "1", "4"), ("0", "1", "1"), ("1", "0", "1")])
+def test_persistence_bit_exact(small, compact, att, monkeypatch):
+    """The 3D persistence term (Cpm::setPersistence, kernels3d.cu:327-344)
+    with preferred directions updated from the centroid history after every
+    inner_iterations chunk (kernels.cu:158-212, cpm.cu:487-514): lattice,
+    tables and preferred directions bit-exact against the oracle."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_SMALL", small)
+    monkeypatch.setenv("GPUCPM_COMPACT", compact)
+    monkeypatch.setenv("GPUCPM_ATT", att)
+    cases = [(W.persistent(W.plain3d(L=32, ncells=60, seed=121), lam=(0.0, 300.0), diffusion=(0.0, 0.7)),
+              1, 1, 12, 2),
+             (W.persistent(W.plain3d(L=32, ncells=40, seed=122), lam=(0.0, 150.0), diffusion=(0.0, 0.0),
+                           history=1), 2, 4, 6, 3),
+             (W.persistent(W.chemotactic(W.plain3d(L=32, ncells=40, seed=123), lam=(0.0, 10.0)),
+                           lam=(0.0, 200.0), diffusion=(0.0, 0.5), history=4), 1, 1, 12, 3)]
+    for wl, c, u, n, inner in cases:
+        _announce("pers", small, compact, att, wl.name, wl.L, c, u, inner)
+        sim, lat, table = _run_gpu(wl, c, u, seed=131, nmcs=n, inner=inner)
+        lat_o, tab_o = run_oracle(wl, c, u, 131, n, inner=inner)
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(table, tab_o, wl.d)
+        np.testing.assert_array_equal(sim.get_pref_dirs().reshape(-1), tab_o.pref)
+
+
+def test_persistence_has_no_effect_in_2d():
+    """The reference has no 2D persistence term (kernels2d.cu has none and
+    the prefDir update runs for dimensionality 3 only, cpm.cu:487)."""
+    _, W = _gpu()
+    base = W.sorting2d(L=64, grid=5, edge=10, seed=124)
+    wl = W.persistent(base)
+    _announce("pers2d")
+    _, lat_a, _ = _run_gpu(base, 1, 1, seed=5, nmcs=10)
+    _, lat_b, _ = _run_gpu(wl, 1, 1, seed=5, nmcs=10)
+    np.testing.assert_array_equal(lat_a, lat_b)

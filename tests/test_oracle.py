@@ -598,3 +598,102 @@ def test_act_cell_migrates():
             prev = c
         dist[lam] = path
     assert dist[300.0] > 4.0 * dist[0.0], dist
+
+
+# -------------------------------------------------------- persistence ------
+
+def _pref_tables(shift):
+    """One 4^3 cell, then the same cell translated by shift (periodic)."""
+    L = 16
+    a = np.zeros((L, L, L), np.uint32)
+    a[6:10, 6:10, 6:10] = (1 << 24) | 1
+    b = np.roll(a, shift=tuple(shift[::-1]), axis=(0, 1, 2))
+    return L, O.tally(a, 3, 1), O.tally(b, 3, 1)
+
+
+def test_pref_dir_update_closed_forms():
+    """kernels.cu:158-212 / cpm.cu:487-514: with diffusion 0 the preferred
+    direction becomes the unit displacement of the centroid over the history
+    window; with diffusion 1 it stays; with a history of 1 there is no
+    displacement and it stays; displacements wrap across the boundary."""
+    for shift, expect in (((2, 0, 0), (1, 0, 0)), ((0, -3, 4), (0, -0.6, 0.8)),
+                          ((9, 0, 0), (-1, 0, 0))):  # 9 on L = 16 wraps to -7
+        L, ta, tb = _pref_tables(shift)
+        for D, H in ((0.0, 2), (1.0, 2), (0.0, 1)):
+            ps = O.PrefState(L, 3, ta, H, seed=5)
+            p0 = ps.pref.copy()
+            ps.update(tb, [0, 1], np.array([0.0, D] + [0.0] * 18))
+            p1 = ps.pref.reshape(-1, 3)[1]
+            if D == 0.0 and H == 2:
+                np.testing.assert_allclose(p1, expect, atol=1e-12)
+            else:
+                np.testing.assert_array_equal(ps.pref, p0)
+            np.testing.assert_allclose(np.linalg.norm(ps.pref.reshape(-1, 3)[1:], axis=1), 1.0, atol=1e-12)
+            np.testing.assert_array_equal(ps.pref_q, np.round(ps.pref * 65536).astype(np.int32))
+    # diffusion 0.5 blends the old direction and the displacement
+    L, ta, tb = _pref_tables((3, 0, 0))
+    ps = O.PrefState(L, 3, ta, 2, seed=5)
+    old = ps.pref.reshape(-1, 3)[1].copy()
+    ps.update(tb, [0, 1], np.array([0.0, 0.5] + [0.0] * 18))
+    mix = 0.5 * np.array([1.0, 0, 0]) + 0.5 * old
+    np.testing.assert_allclose(ps.pref.reshape(-1, 3)[1], mix / np.linalg.norm(mix), atol=1e-12)
+
+
+def test_persistence_energy_closed_form():
+    """kernels3d.cu:327-344: -lambda (d . p), d the unit vector from source to
+    target, p the source cell's preferred direction (the target cell's when the
+    source is medium), lambda of the source type (target type likewise)."""
+    L = 16
+    lat = _one_cell(L, 3)
+    n = lat.size
+    tab = O.tally(lat, 3, 1)
+    ps = O.PrefState(L, 3, tab, 2, seed=9)
+    p0 = O.Params.make(10, J=[[0, 4], [4, 8]], lam_area=[0, 1], target_area=[0, 64])
+    p1 = O.Params.make(10, J=[[0, 4], [4, 8]], lam_area=[0, 1], target_area=[0, 64],
+                       lam_pers=[0, 3.0])
+    p1.set_pref(ps.pref_q)
+    scale = O.energy_scale(p1)
+    assert scale == 256
+    flat = lat.reshape(-1)
+    coords = np.stack(np.unravel_index(np.arange(n), lat.shape)[::-1], 1)
+    pref = ps.pref.reshape(-1, 3)
+    checked = 0
+    for t in range(0, n, 5):
+        for off in np.ndindex(3, 3, 3):
+            o = np.array(off) - 1
+            if not o.any():
+                continue
+            s = int(np.ravel_multi_index(tuple(((coords[t] + o) % L)[::-1]), lat.shape))
+            if flat[s] == flat[t]:
+                continue
+            cell = (flat[s] & 0xFFFFFF) or (flat[t] & 0xFFFFFF)
+            e = O.delta_h(lat, 3, tab, p1, scale, t, s) - O.delta_h(lat, 3, tab, p0, scale, t, s)
+            expect = -3.0 * float((-o / np.linalg.norm(o)) @ pref[cell]) * scale
+            assert abs(e - expect) <= 0.55, (t, s, e, expect)
+            checked += 1
+    assert checked > 30
+
+
+def test_persistence_makes_cells_travel():
+    """A 3D cell with a persistence term, its preferred direction updated
+    from the centroid history after every chunk, travels far more than one
+    without."""
+    L = 32
+    path = {}
+    for lam in (0.0, 200.0):
+        lat = np.zeros((L, L, L), np.uint32)
+        lat[13:19, 13:19, 13:19] = (1 << 24) | 1
+        tab = O.tally(lat, 3, 1)
+        ps = O.PrefState(L, 3, tab, 3, seed=3)
+        p = O.Params.make(20, J=[[0, 10], [10, 0]], lam_area=[0, 5], target_area=[0, 216],
+                          lam_pers=[0, lam])
+        p.set_pref(ps.pref_q)
+        prev, total = O.pref_centroids(L, 3, tab).reshape(-1, 3)[1].copy(), 0.0
+        for k in range(20):
+            O.sweep(lat, 3, tab, p, 2, 1, seed=3, mcs0=10 * k, nmcs=10, nthreads=8)
+            ps.update(tab, [0, 1], np.array([0.0, 0.8] + [0.0] * 18))
+            c = O.pref_centroids(L, 3, tab).reshape(-1, 3)[1].copy()
+            total += float(np.linalg.norm((c - prev + L / 2) % L - L / 2))
+            prev = c
+        path[lam] = total
+    assert path[200.0] > 5 * path[0.0], path
