@@ -94,9 +94,10 @@ def bench_batched(a, log):
     wls = workloads.batched_sorting(nrep=a.replicas, seed=4)
     c, u = a.checkerboard, a.updates
     mcs = a.steps * a.mcs_per_step
+    # shared = 1: the reference's shared-memory schedule (K = 2 colours)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
               positions_per_thread=2 * c, positions_per_checkerboard=c,
-              updates_per_checkerboard_switch=u, updates_per_barrier=1)
+              updates_per_checkerboard_switch=u, updates_per_barrier=1, shared=1)
     sites = sum(w.L ** 2 for w in wls)
 
     def fresh():
@@ -214,7 +215,7 @@ def gradient_check(nrep, L, mcs, log, J_AB=11, seed0=0, contrast=False):
 
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4, positions_per_thread=2,
               positions_per_checkerboard=1, updates_per_checkerboard_switch=1, updates_per_barrier=1,
-              iterations=mcs, inner_iterations=mcs)
+              iterations=mcs, inner_iterations=mcs, shared=1)
     wls, sims = make(J_AB, seed0)
     log(f"gradient: {nrep} x {L}^2 forward {mcs} MCS")
     t0 = time.perf_counter()
@@ -357,10 +358,11 @@ def main():
     sim = wl.make_cpm()
     sim.push_to_gpu()
     sim.set_seed(20240601)
+    # shared = 1: the reference's shared-memory schedule (K = 2 colours)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
               positions_per_thread=2 * c, positions_per_checkerboard=c,
               updates_per_checkerboard_switch=u, updates_per_barrier=1,
-              iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+              iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step, shared=1)
     log(f"{a.workload} L={wl.L} d={wl.d} cells={wl.ncells} c={c} u={u} "
         f"mode={os.environ.get('GPUCPM_MODE', 'default')}: warmup {a.warmup} x {a.mcs_per_step} MCS")
     for _ in range(a.warmup):

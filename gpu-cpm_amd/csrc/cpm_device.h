@@ -82,6 +82,10 @@ struct SweepConsts {
     uint32_t k0, k1;        // Philox key (seed)
     int nt;                 // types present (pair-table stride)
     uint32_t fixed_mask;    // bit t set: type t is fixed (settings.h:24)
+    int roles;              // 0 shared: target drawn in the sub-block; 1: source drawn
+    int K, S;               // colours and groups per axis
+    int lstride;            // log2 (S K)
+    int phases_per_round;   // K^d S^d
     int chem;               // chemotaxis term on
     const int32_t* field;   // field [d][L^d], quantized to 2^-16 (or null)
     int pers;               // persistence term on (3D)
@@ -95,8 +99,11 @@ struct SweepConsts {
 // One colour phase per launch.
 struct PhaseArgs {
     SweepConsts k;
-    int cx, cy, cz;         // colour parity
-    uint32_t phase;         // round * 2^d + colour
+    // Active sub-blocks: ((q mod 2^lh) << lstride) + cx per axis (sub-block
+    // units; q the thread's index per axis): group * K + colour.
+    int cx, cy, cz;
+    int lstride;            // log2 (S K): sub-blocks between active ones
+    uint32_t phase;         // Philox phase index (round K^d + colour) S^d + group
     uint32_t mcs;           // Monte Carlo step counter
     int g;                  // phase counter since the last settle (its parity selects buffers)
     int lh;                 // log2 (same-colour sub-blocks per axis)
@@ -147,7 +154,8 @@ struct BatchArgs {
 struct BatchPhaseArgs {
     const ReplicaDesc* reps;
     const int* rep_of;
-    int cx, cy, cz;
+    int cx, cy, cz;         // as PhaseArgs
+    int lstride;
     uint32_t phase;         // round * 2^d + colour
     uint32_t m;             // MCS offset within this run (replica MCS = mcs0 + m)
     int g;                  // phase counter since the last settle (parity)

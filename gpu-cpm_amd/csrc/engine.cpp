@@ -826,6 +826,23 @@ int cpm_push(cpm_handle h) {
 
 namespace {
 
+// Active sub-block offsets of phase pc of a round: colour (cbX outermost,
+// kernels2d.cu:179-180) then group, offset = group K + colour per axis.
+void phase_offsets(int d, const gcpm::SweepConsts& k, int pc, int* ox, int* oy, int* oz) {
+    const int K = k.K, S = k.S;
+    const int ngrp = d == 3 ? S * S * S : S * S;
+    const int col = pc / ngrp, grp = pc % ngrp;
+    const int cx = d == 3 ? col / (K * K) : col / K;
+    const int cy = d == 3 ? (col / K) % K : col % K;
+    const int cz = d == 3 ? col % K : 0;
+    const int gx = d == 3 ? grp / (S * S) : grp / S;
+    const int gy = d == 3 ? (grp / S) % S : grp % S;
+    const int gz = d == 3 ? grp % S : 0;
+    *ox = gx * K + cx;
+    *oy = gy * K + cy;
+    *oz = gz * K + cz;
+}
+
 // Validate run arguments against engine h and fill the sweep constants
 // (cpm.cu:291-405 semantics; see cpm_run).
 int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, int* lh_out) {
@@ -839,6 +856,23 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     const int c = a->positions_per_checkerboard;
     if (!is_pow2(c) || h->L % (2 * c))
         return fail(CPM_EINVAL, "positions_per_checkerboard must be a power of two with 2*c dividing dimension");
+    // Schedule (kernels2d.cu:72-106,179-243): shared roles use regions of 2c
+    // sites and K = 2 colours per axis; non-shared roles (the reference's
+    // default, shared = 0) regions of positions_per_thread sites, K = ppt / c,
+    // with the regions of a colour in S^d groups so that concurrent attempts
+    // stay >= 3 sites apart (S ppt - c >= 3, S <= L / ppt).
+    const bool shared = a->shared != 0;
+    const int R = shared ? 2 * c : a->positions_per_thread;
+    if (!shared && (R < c || R % c || h->L % R || !is_pow2(R)))
+        return fail(CPM_EINVAL,
+                    "with shared = 0, positions_per_thread must be a multiple of "
+                    "positions_per_checkerboard dividing dimension");
+    const int K = R / c;
+    int S = 1;
+    if (!shared) {
+        while (S * R - c < 3) S *= 2;
+        if (S > h->L / R) S = h->L / R;
+    }
     const long cd = (long)c * c * (h->d == 3 ? c : 1);
     const int u = a->updates_per_checkerboard_switch;
     if (u < 1 || u > cd)
@@ -869,10 +903,10 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (k.act) {
         if (!h->has_act)
             return fail(CPM_EINVAL, "lambda_act is set but the engine has no act lattice (has_act = 0)");
-        if (c < 2)
+        if (shared && c < 2)
             return fail(CPM_ENOTIMPL,
-                        "the act term needs positions_per_checkerboard >= 2 (with 1 the source "
-                        "neighbourhood reaches targets of the same colour phase)");
+                        "with shared roles the act term needs positions_per_checkerboard >= 2 (with 1 "
+                        "the source neighbourhood reaches targets of the same colour phase)");
         for (int t = 0; t < CPM_NTYPES; t++) {
             if (h->st.lam_act[t] == 0.0) continue;
             if (h->st.act_max[t] < 1) return fail(CPM_EINVAL, "max_act must be >= 1 where lambda_act != 0");
@@ -892,8 +926,14 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     k.fixed_mask = 0;
     for (int t = 0; t < CPM_NTYPES; t++)
         if (h->st.fixed[t]) k.fixed_mask |= 1u << t;
+    k.roles = shared ? 0 : 1;
+    k.lstride = ilog2((long)S * K);
+    k.K = K;
+    k.S = S;
+    k.phases_per_round = 1;
+    for (int i = 0; i < h->d; i++) k.phases_per_round *= K * S;
     *kout = k;
-    *lh_out = h->lgL - k.lc - 1;
+    *lh_out = h->lgL - k.lc - k.lstride;
     return CPM_OK;
 }
 
@@ -917,11 +957,13 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             HIP_TRY(hipMalloc(&h->d_claims, sizeof(uint64_t) * ((size_t)h->n + 1)));
             HIP_TRY(hipMemsetAsync(h->d_claims, 0, sizeof(uint64_t) * ((size_t)h->n + 1), h->stream));
         }
+    } else if (k.roles) {
+        plan.mode = Plan::kPhase;  // the persistent forms implement shared roles only
     } else if (int rc = h->plan(c, &plan)) {
         return rc;
     }
     h->last_plan = plan;
-    const int64_t updates_per_mcs = (int64_t)k.rounds * (1 << h->d) * ((int64_t)1 << (lh * h->d)) * u;
+    const int64_t updates_per_mcs = (int64_t)k.rounds * k.phases_per_round * ((int64_t)1 << (lh * h->d)) * u;
 
     if (plan.mode == Plan::kPhase) {
         gcpm::PhaseArgs pa{};
@@ -950,7 +992,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             const int att = e ? std::atoi(e) : (h->d == 3 ? 4 : 1);
             pa.form |= att == 4 ? gcpm::kFormAtt4 : att == 2 ? gcpm::kFormAtt2 : 0;
         }
-        const int ncol = 1 << h->d;
+        pa.lstride = k.lstride;
         int g = 0;  // phase counter since the tables were last settled
         const bool pers = k.pers != 0;
         if (pers && !h->ring_valid)
@@ -958,11 +1000,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         for (long m = 0; m < total; m++) {
             pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
             for (int r = 0; r < k.rounds; r++) {
-                for (int col = 0; col < ncol; col++, g++) {
-                    pa.cx = h->d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
-                    pa.cy = h->d == 3 ? (col >> 1) & 1 : col & 1;
-                    pa.cz = h->d == 3 ? col & 1 : 0;
-                    pa.phase = (uint32_t)(r * ncol + col);
+                for (int pc = 0; pc < k.phases_per_round; pc++, g++) {
+                    phase_offsets(h->d, k, pc, &pa.cx, &pa.cy, &pa.cz);
+                    pa.phase = (uint32_t)(r * k.phases_per_round + pc);
                     pa.g = g;
                     if (a->cell_sync) {
                         // the tag only grows: claims never need clearing
@@ -1205,17 +1245,16 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     gcpm::BatchPhaseArgs pa{};
     pa.reps = b->d_reps;
     pa.rep_of = b->d_rep_of;
-    const int ncol = 1 << b->d;
-    const int rounds = reps[0].k.rounds;
+    const gcpm::SweepConsts& k0 = reps[0].k;
+    const int rounds = k0.rounds;
+    pa.lstride = k0.lstride;
     int g = 0;
     for (long m = 0; m < total; m++) {
         pa.m = (uint32_t)m;
         for (int r = 0; r < rounds; r++) {
-            for (int col = 0; col < ncol; col++, g++) {
-                pa.cx = b->d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
-                pa.cy = b->d == 3 ? (col >> 1) & 1 : col & 1;
-                pa.cz = b->d == 3 ? col & 1 : 0;
-                pa.phase = (uint32_t)(r * ncol + col);
+            for (int pc = 0; pc < k0.phases_per_round; pc++, g++) {
+                phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
+                pa.phase = (uint32_t)(r * k0.phases_per_round + pc);
                 pa.g = g;
                 HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
             }
@@ -1284,6 +1323,8 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
         if (reps[i].k.pers)
             return fail(CPM_ENOTIMPL, "the persistence term is not available for batched runs");
+        if (reps[i].k.K != reps[0].k.K || reps[i].k.S != reps[0].k.S)
+            return fail(CPM_EINVAL, "batched replicas need the same colour schedule (K, S)");
         const long nsub = 1L << (reps[i].lh * b->d);
         const long w = (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
         need[i] = std::max(1L, (long)std::ceil(w * b->block_scale));
@@ -1292,6 +1333,8 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     if (chunks == 0) return CPM_OK;
     if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync is not available for batched runs");
     if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations);
+    if (reps[0].k.roles)
+        return fail(CPM_ENOTIMPL, "the persistent batch form implements shared roles only (shared = 1)");
     if (b->d == 3) return fail(CPM_ENOTIMPL, "the persistent batch form is off in 3D");
 
     // Block groups: each replica gets blocks in proportion to its work, at

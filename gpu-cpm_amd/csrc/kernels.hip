@@ -246,14 +246,27 @@ __device__ __forceinline__ void draw_attempt(const Lat& lat, const SweepConsts& 
     const uint32_t cm = (1u << lc) - 1u;
     philox4x32_10(sb, mcs, phase, i, k.k0, k.k1, d.w);
     d.tick = mcs;
-    const int tx = (bx << lc) + (int)(d.w[0] & cm);
-    const int ty = (by << lc) + (int)((d.w[0] >> lc) & cm);
-    const int tz = D == 3 ? (bz << lc) + (int)((d.w[0] >> (2 * lc)) & cm) : 0;
-    d.t = lat.at(tx, ty, tz);
-    lat.block(d.t, d.nv);
     constexpr int N = D == 3 ? 26 : 8;
+    constexpr int NB = N + 1;
+    int px = (bx << lc) + (int)(d.w[0] & cm);
+    int py = (by << lc) + (int)((d.w[0] >> lc) & cm);
+    int pz = D == 3 ? (bz << lc) + (int)((d.w[0] >> (2 * lc)) & cm) : 0;
     int dir = (int)(((uint64_t)d.w[1] * (uint64_t)N) >> 32);
     dir += dir >= N / 2;
+    if (k.roles) {
+        // non-shared roles (kernels2d.cu:227-243): the drawn position is the
+        // source and the target its neighbour; the source is then the
+        // target's mirrored neighbour
+        int ox, oy, oz;
+        block_offset<D>(dir, ox, oy, oz);
+        const int Lm = (1 << k.lgL) - 1;
+        px = (px + ox) & Lm;
+        py = (py + oy) & Lm;
+        pz = D == 3 ? (pz + oz) & Lm : 0;
+        dir = NB - 1 - dir;
+    }
+    d.t = lat.at(px, py, pz);
+    lat.block(d.t, d.nv);
     d.dir = dir;
     if constexpr (D == 2) {
         uint32_t vs = 0;
@@ -681,9 +694,9 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
     const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
     if (gtid < a.nsub) {
         const int hm = (1 << a.lh) - 1;
-        const int bx = 2 * (gtid & hm) + a.cx;
-        const int by = 2 * ((gtid >> a.lh) & hm) + a.cy;
-        const int bz = D == 3 ? 2 * (gtid >> (2 * a.lh)) + a.cz : 0;
+        const int bx = ((gtid & hm) << a.lstride) + a.cx;
+        const int by = (((gtid >> a.lh) & hm) << a.lstride) + a.cy;
+        const int bz = D == 3 ? ((gtid >> (2 * a.lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
         sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
                                      CellSync{a.sync_pass, a.claims, a.sync_tag});
@@ -774,21 +787,30 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
         active[r] = false;
         if (q < a.nsub) {
             const int hm = (1 << a.lh) - 1;
-            const int bx = 2 * (q & hm) + a.cx;
-            const int by = 2 * ((q >> a.lh) & hm) + a.cy;
-            const int bz = D == 3 ? 2 * (q >> (2 * a.lh)) + a.cz : 0;
+            const int bx = ((q & hm) << a.lstride) + a.cx;
+            const int by = (((q >> a.lh) & hm) << a.lstride) + a.cy;
+            const int bz = D == 3 ? ((q >> (2 * a.lh)) << a.lstride) + a.cz : 0;
             const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
             uint32_t w[4];
             philox4x32_10(sb, a.mcs, a.phase, 0u, k.k0, k.k1, w);
             const int lc = k.lc;
             const uint32_t cm = (1u << lc) - 1u;
-            const int tx = (bx << lc) + (int)(w[0] & cm);
-            const int ty = (by << lc) + (int)((w[0] >> lc) & cm);
-            const int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
-            const typename Lat::Pos t = lat.at(tx, ty, tz);
             constexpr int N = D == 3 ? 26 : 8;
+            int tx = (bx << lc) + (int)(w[0] & cm);
+            int ty = (by << lc) + (int)((w[0] >> lc) & cm);
+            int tz = D == 3 ? (bz << lc) + (int)((w[0] >> (2 * lc)) & cm) : 0;
             int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
             dir += dir >= N / 2;
+            if (k.roles) {  // non-shared roles: see draw_attempt
+                int ox, oy, oz;
+                block_offset<D>(dir, ox, oy, oz);
+                const int Lm = (1 << k.lgL) - 1;
+                tx = (tx + ox) & Lm;
+                ty = (ty + oy) & Lm;
+                tz = D == 3 ? (tz + oz) & Lm : 0;
+                dir = N - dir;  // NB - 1 - dir
+            }
+            const typename Lat::Pos t = lat.at(tx, ty, tz);
             uint32_t vs, vt;
             if constexpr (NQ > 0) {
                 lat.block(t, e[r].nv);
@@ -906,9 +928,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
     const int lh = rd.lh;
     if (gtid < (1 << (D * lh))) {
         const int hm = (1 << lh) - 1;
-        const int bx = 2 * (gtid & hm) + a.cx;
-        const int by = 2 * ((gtid >> lh) & hm) + a.cy;
-        const int bz = D == 3 ? 2 * (gtid >> (2 * lh)) + a.cz : 0;
+        const int bx = ((gtid & hm) << a.lstride) + a.cx;
+        const int by = (((gtid >> lh) & hm) << a.lstride) + a.cy;
+        const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
         sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T);
     }
@@ -1292,7 +1314,8 @@ const char* phase_kernel_name(int id) {
 
 static int choose_phase_kernel(int d, const PhaseArgs& a) {
     const int L = 1 << a.k.lgL;
-    if (d == 3 && a.k.lc == 0 && L >= kBX && L >= kBY && L >= kBZ && a.box3 && !a.sync_pass) {
+    if (d == 3 && a.k.lc == 0 && !a.k.roles && a.lstride == 1 && L >= kBX && L >= kBY && L >= kBZ &&
+        a.box3 && !a.sync_pass) {
         const long nbox = (long)(L / kBX) * (L / kBY) * (L / kBZ);
         if (nbox * kBlock >= a.k.ncells + 1) return kPkBox3;
     }

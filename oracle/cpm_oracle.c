@@ -432,11 +432,39 @@ int orc_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* p
 int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                    const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                    uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync) {
+    return orc_sweep_ex(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
+                        nthreads, cell_sync, 1, 2 * c);
+}
+
+int orc_schedule(int L, int d, int c, int shared, int ppt, int* K, int* S) {
+    (void)d;
+    if (c < 1 || (c & (c - 1))) return -3;
+    int R = shared ? 2 * c : ppt;
+    if (R < c || R % c || L % R) return -3;
+    *K = R / c;
+    int s = 1;
+    if (!shared) {
+        /* same-group sub-blocks must be >= 3 sites apart: a non-shared
+         * attempt writes up to one site outside its sub-block and reads up
+         * to two */
+        while (s * R - c < 3) s *= 2;
+        if (s > L / R) s = L / R;
+    }
+    *S = s;
+    return 0;
+}
+
+int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                 const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                 uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
+                 int ppt) {
     if (d != 2 && d != 3) return -1;
     if (cell_sync && u != 1) return -7;
-    if (act_on(p) && p->act != NULL && c < 2) return -8;
+    if (shared && act_on(p) && p->act != NULL && c < 2) return -8;
     if (L < 2 || (L & (L - 1))) return -2;
-    if (c < 1 || (c & (c - 1)) || L % (2 * c)) return -3;
+    int K, S;
+    if (orc_schedule(L, d, c, shared, ppt, &K, &S)) return -3;
+    const int R = K * c;
     int cd = d == 3 ? c * c * c : c * c;
     if (u < 1 || u > cd) return -4;
     int scale = orc_energy_scale(p);
@@ -456,10 +484,11 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
     const int lc = log2i(c);
     const uint32_t cm = (uint32_t)c - 1;
     const int nsb = L / c;       /* sub-blocks per axis */
-    const int h = nsb / 2;       /* same-colour sub-blocks per axis */
+    const int h = L / R / S;     /* same-phase sub-blocks per axis */
     const int64_t nq = d == 3 ? (int64_t)h * h * h : (int64_t)h * h;
     const int rounds = cd / u;   /* kernels2d.cu:95 */
-    const int ncol = 1 << d;
+    const int ncol = d == 3 ? K * K * K : K * K;   /* colours: K^d (kernels2d.cu:91) */
+    const int ngrp = d == 3 ? S * S * S : S * S;
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
 
     int32_t* areaR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
@@ -474,12 +503,16 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
     for (int m = 0; m < nmcs; m++) {
         uint32_t mcs = mcs0 + (uint32_t)m;
         for (int r = 0; r < rounds; r++) {
-            for (int col = 0; col < ncol; col++) {
-                /* col = cx*2 + cy (2D), cx*4 + cy*2 + cz (3D): cbX outermost */
-                int cx = d == 3 ? (col >> 2) & 1 : (col >> 1) & 1;
-                int cy = d == 3 ? (col >> 1) & 1 : col & 1;
-                int cz = d == 3 ? col & 1 : 0;
-                uint32_t phase = (uint32_t)(r * ncol + col);
+            for (int pc = 0; pc < ncol * ngrp; pc++) {
+                /* colour (cbX outermost, kernels2d.cu:179-180), then group */
+                int col = pc / ngrp, grp = pc % ngrp;
+                int cx = d == 3 ? col / (K * K) : col / K;
+                int cy = d == 3 ? (col / K) % K : col % K;
+                int cz = d == 3 ? col % K : 0;
+                int gx = d == 3 ? grp / (S * S) : grp / S;
+                int gy = d == 3 ? (grp / S) % S : grp % S;
+                int gz = d == 3 ? grp % S : 0;
+                uint32_t phase = (uint32_t)(r * ncol * ngrp + pc);
                 memcpy(areaR, area, sizeof(int32_t) * (size_t)(ncells + 1));
                 memcpy(perimR, perim, sizeof(int32_t) * (size_t)(ncells + 1));
                 if (cell_sync) memset(owner, 0xFF, sizeof(uint32_t) * (size_t)(ncells + 1));
@@ -493,21 +526,30 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
                 for (int64_t qi = 0; qi < nq; qi++) {
-                    int bx = 2 * (int)(qi % h) + cx;
-                    int by = 2 * (int)((qi / h) % h) + cy;
-                    int bz = d == 3 ? 2 * (int)(qi / ((int64_t)h * h)) + cz : 0;
+                    /* sub-block of region (a S + g) at colour offset c */
+                    int bx = ((int)(qi % h) * S + gx) * K + cx;
+                    int by = ((int)((qi / h) % h) * S + gy) * K + cy;
+                    int bz = d == 3 ? ((int)(qi / ((int64_t)h * h)) * S + gz) * K + cz : 0;
                     uint32_t sb = (uint32_t)(((int64_t)bz * nsb + by) * nsb + bx);
                     for (int i = 0; i < u; i++) {
                         uint32_t ctr[4] = {sb, mcs, phase, (uint32_t)i};
                         uint32_t w[4];
                         orc_philox4x32_10(ctr, key, w);
-                        int tx = bx * c + (int)(w[0] & cm);
-                        int ty = by * c + (int)((w[0] >> lc) & cm);
-                        int tz = d == 3 ? bz * c + (int)((w[0] >> (2 * lc)) & cm) : 0;
-                        int dir = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
-                        int sx = (tx + nb.off[dir][0]) & (L - 1);
-                        int sy = (ty + nb.off[dir][1]) & (L - 1);
-                        int sz = d == 3 ? (tz + nb.off[dir][2]) & (L - 1) : 0;
+                        /* the position drawn in the sub-block and a neighbour:
+                         * shared roles (kernels2d.cu:204-226): target, source;
+                         * non-shared (:227-243): source, target */
+                        int px = bx * c + (int)(w[0] & cm);
+                        int py = by * c + (int)((w[0] >> lc) & cm);
+                        int pz = d == 3 ? bz * c + (int)((w[0] >> (2 * lc)) & cm) : 0;
+                        int dn = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+                        int qx = (px + nb.off[dn][0]) & (L - 1);
+                        int qy = (py + nb.off[dn][1]) & (L - 1);
+                        int qz = d == 3 ? (pz + nb.off[dn][2]) & (L - 1) : 0;
+                        int tx = shared ? px : qx, ty = shared ? py : qy, tz = shared ? pz : qz;
+                        int sx = shared ? qx : px, sy = shared ? qy : py, sz = shared ? qz : pz;
+                        /* the source's offset from the target (the list is
+                         * symmetric: entry N-1-n is minus entry n) */
+                        int dir = shared ? dn : N - 1 - dn;
                         int64_t ti = site(L, tx, ty, tz);
                         uint32_t vs = lat[site(L, sx, sy, sz)];
                         uint32_t vt = lat[ti];

@@ -162,6 +162,25 @@ int orc_sweep_sync(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                    const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                    uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync);
 
+/* The run() schedule (kernels2d.cu:72-106,179-243, cpm.cu:405):
+ * shared = 1: regions of 2c sites, K = 2 colours per axis, the target drawn in
+ * the sub-block and the source a neighbour; shared = 0 (the reference's
+ * default): regions of ppt = positions_per_thread sites, K = ppt / c colours
+ * per axis, the source drawn in the sub-block and the target a neighbour.
+ * One MCS is c^d / u rounds of K^d colour phases (cbX outermost); a colour
+ * phase covers sub-block (region r) K + colour of every region.  Without
+ * shared roles an attempt writes one site outside its sub-block and reads
+ * two, so the regions of a colour are processed in S^d groups (r mod S), S
+ * the smallest power of two with S ppt - c >= 3 (at most L / ppt): the
+ * reference runs them together and races when ppt - c < 3.  The Philox
+ * phase index is (round K^d + colour) S^d + group.  orc_schedule returns K
+ * and S (or -3 for a bad c / ppt). */
+int orc_schedule(int L, int d, int c, int shared, int ppt, int* K, int* S);
+int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                 const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                 uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
+                 int ppt);
+
 /* Serial random-sequential CPM with live tables (statistical comparisons
  * only; not bit-comparable with orc_sweep). */
 int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,

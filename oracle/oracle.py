@@ -127,6 +127,10 @@ def lib():
         L.orc_sweep.restype = ctypes.c_int
         L.orc_sweep_sync.argtypes = L.orc_sweep.argtypes + [ctypes.c_int]
         L.orc_sweep_sync.restype = ctypes.c_int
+        L.orc_sweep_ex.argtypes = L.orc_sweep.argtypes + [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_sweep_ex.restype = ctypes.c_int
+        L.orc_schedule.argtypes = [ctypes.c_int] * 5 + [P(ctypes.c_int), P(ctypes.c_int)]
+        L.orc_schedule.restype = ctypes.c_int
         L.orc_serial_sweep.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, i32p,
                                        i64p, P(Params), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int]
         L.orc_serial_sweep.restype = ctypes.c_int
@@ -295,17 +299,29 @@ def tally(lat: np.ndarray, d: int, ncells: int) -> Tables:
     return t
 
 
+def schedule(L: int, d: int, c: int, shared: bool, ppt: int):
+    """(K colours per axis, S groups per axis) of a run() schedule."""
+    K, S = ctypes.c_int(), ctypes.c_int()
+    if lib().orc_schedule(L, d, c, int(bool(shared)), int(ppt), ctypes.byref(K), ctypes.byref(S)):
+        raise ValueError("bad positions_per_checkerboard / positions_per_thread")
+    return K.value, S.value
+
+
 def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
-          mcs0: int, nmcs: int, nthreads: int = 1, cell_sync: bool = False) -> None:
-    """Run nmcs MCS in place on lat (uint32 view) and tables."""
+          mcs0: int, nmcs: int, nthreads: int = 1, cell_sync: bool = False, shared: bool = True,
+          ppt: int = None) -> None:
+    """Run nmcs MCS in place on lat (uint32 view) and tables.  shared selects
+    the roles of the run() kwarg of that name; ppt is positions_per_thread
+    (used without shared roles)."""
     assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
     lat32 = lat.view(np.uint32)
     L = lat.shape[0]
-    rc = lib().orc_sweep_sync(L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
-                              _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
-                              _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
-                              ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
-                              int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads), int(bool(cell_sync)))
+    rc = lib().orc_sweep_ex(L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
+                            _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+                            _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                            ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
+                            int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads), int(bool(cell_sync)),
+                            int(bool(shared)), int(ppt if ppt is not None else 2 * c))
     if rc:
         raise ValueError(f"orc_sweep failed with code {rc}")
 

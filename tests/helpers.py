@@ -15,7 +15,8 @@ def oracle_params(wl) -> "O.Params":
                          lam_pers=p["lam_pers"])
 
 
-def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=None):
+def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=None, shared=True,
+               ppt=None):
     """Tally + sweep in the oracle; returns (lattice, Tables).  With a 3D
     persistence term the run is cut into chunks of ``inner`` MCS with a
     preferred-direction update after each (tables.pref holds the result)."""
@@ -37,21 +38,22 @@ def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=Non
         ctype[0] = 0
         inner = inner or nmcs
         for m in range(0, nmcs, inner):
-            O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0 + m, inner, nthreads)
+            O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0 + m, inner, nthreads, shared=shared, ppt=ppt)
             ps.update(tab, ctype, pw["pers_diff"])
         tab.pref = ps.pref
     elif nmcs:
-        O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0, nmcs, nthreads)
+        O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0, nmcs, nthreads, shared=shared, ppt=ppt)
     if act is not None:
         tab.act = act
     return lat, tab
 
 
 def run_kwargs(c, u, iterations, inner=None, **extra):
+    """run() kwargs; shared roles unless shared=0 is passed."""
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
               positions_per_thread=2 * c, positions_per_checkerboard=c,
               updates_per_checkerboard_switch=u, updates_per_barrier=1,
-              iterations=iterations, inner_iterations=inner or iterations)
+              iterations=iterations, inner_iterations=inner or iterations, shared=1)
     kw.update(extra)
     return kw
 

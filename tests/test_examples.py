@@ -14,12 +14,15 @@ from helpers import assert_tables_equal
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_from_sim(sim, L, d, ncells, T, J, lam_area, target_area, c, u, seed, nmcs, lattice):
+def _oracle_from_sim(sim, L, d, ncells, T, J, lam_area, target_area, c, u, seed, nmcs, lattice,
+                     ppt):
+    """The scripts pass no ``shared``: the reference's default, non-shared
+    roles with positions_per_thread ppt."""
     import oracle as O
     p = O.Params.make(T, J=J, lam_area=lam_area, target_area=target_area)
     lat = lattice.astype(np.uint32).copy()
     tab = O.tally(lat, d, ncells)
-    O.sweep(lat, d, tab, p, c, u, seed, 0, nmcs, 8)
+    O.sweep(lat, d, tab, p, c, u, seed, 0, nmcs, 8, shared=False, ppt=ppt)
     return lat, tab
 
 
@@ -47,7 +50,7 @@ def test_example2d_replay():
     J = np.zeros((2, 2), int)
     J[0, 1] = J[1, 0] = 10
     lat_o, tab_o = _oracle_from_sim(sim, dimension, 2, 10, 20, J, [0, 10], [0, 200], 4, 1, 7, 100,
-                                    initial)
+                                    initial, ppt=8)
     np.testing.assert_array_equal(state, lat_o)
     assert_tables_equal(sim.get_cell_table(), tab_o, 2)
     # continue to the example's 2000 MCS and check what the example prints
@@ -90,7 +93,7 @@ def test_example3d_replay():
     J = np.zeros((2, 2), int)
     J[0, 1] = J[1, 0] = 10
     lat_o, tab_o = _oracle_from_sim(sim, dimension, 3, 10, 20, J, [0, 10], [0, 2000], 4, 1, 11, 60,
-                                    initial)
+                                    initial, ppt=8)
     np.testing.assert_array_equal(state, lat_o)
     assert_tables_equal(sim.get_cell_table(), tab_o, 3)
     assert sim.get_centroids().shape == (3, 3)
@@ -133,7 +136,7 @@ def test_sorting_example_replay():
     got = sim.get_state().view(np.uint32).copy()
     J = np.array([[0, 16, 16], [16, 14, 11], [16, 11, 2]])
     lat_o, tab_o = _oracle_from_sim(sim, dimension, 2, ncells, 10, J, [0, 1, 1], [0, 40, 40], 2, 1,
-                                    3, 200, newstate.view(np.uint32))
+                                    3, 200, newstate.view(np.uint32), ppt=4)
     np.testing.assert_array_equal(got, lat_o)
     assert_tables_equal(sim.get_cell_table(), tab_o, 2)
     sim.run(**kw, iterations=800, inner_iterations=800)

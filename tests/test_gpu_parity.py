@@ -356,3 +356,68 @@ def test_persistence_has_no_effect_in_2d():
     _, lat_a, _ = _run_gpu(base, 1, 1, seed=5, nmcs=10)
     _, lat_b, _ = _run_gpu(wl, 1, 1, seed=5, nmcs=10)
     np.testing.assert_array_equal(lat_a, lat_b)
+
+
+@pytest.mark.parametrize("compact,att", [("1", "1"), ("1", "4"), ("0", "1")])
+def test_nonshared_roles_bit_exact(compact, att, monkeypatch):
+    """run(shared=0), the reference's default (kernels2d.cu:227-243): the
+    source drawn in the sub-block, the target its neighbour, K = ppt / c
+    colours and groups that keep concurrent attempts apart; bit-exact against
+    the oracle in 2D and 3D, with every Hamiltonian term."""
+    _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_COMPACT", compact)
+    monkeypatch.setenv("GPUCPM_ATT", att)
+    cases = [(W.sorting2d(L=128, grid=9, edge=10, seed=141), 1, 2, 1, 20),
+             (W.sorting2d(L=128, grid=9, edge=10, seed=142), 2, 4, 1, 10),
+             (W.sorting2d(L=64, grid=5, edge=10, seed=143), 2, 4, 4, 10),
+             (W.sorting2d(L=128, grid=9, edge=10, seed=144), 4, 8, 1, 6),
+             (W.sorting2d(L=64, grid=5, edge=10, seed=145), 1, 8, 1, 10),
+             (W.active(W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=146)))), 1, 2, 1, 10),
+             (W.plain3d(L=32, ncells=60, seed=147), 1, 2, 1, 4),
+             (W.active(W.plain3d(L=32, ncells=40, seed=148), lam=(0.0, 200.0), act_max=(0, 30)), 2, 4, 1, 3)]
+    for wl, c, ppt, u, n in cases:
+        _announce("nonshared", compact, att, wl.name, wl.L, c, ppt, u)
+        gpucpm, _ = _gpu()
+        sim = wl.make_cpm()
+        sim.push_to_gpu()
+        sim.set_seed(151)
+        sim.run(**run_kwargs(c, u, n, shared=0, positions_per_thread=ppt))
+        sim.synchronize()
+        sim.pull_from_gpu()
+        lat_o, tab_o = run_oracle(wl, c, u, 151, n, shared=False, ppt=ppt)
+        np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(sim.get_cell_table(), tab_o, wl.d)
+
+
+def test_nonshared_cell_sync_and_batch():
+    """Non-shared roles with cell_sync, and in a batch, against the oracle."""
+    gpucpm, W = _gpu()
+    import oracle as O
+    from helpers import oracle_params
+    wl = W.sorting2d(L=64, grid=5, edge=10, seed=152)
+    _announce("nonshared cell_sync")
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(3)
+    sim.run(**run_kwargs(1, 1, 10, cell_sync=1, shared=0, positions_per_thread=2))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    lat_o = wl.lattice.copy()
+    tab_o = O.tally(lat_o, 2, wl.ncells)
+    O.sweep(lat_o, 2, tab_o, oracle_params(wl), 1, 1, 3, 0, 10, 8, cell_sync=True, shared=False, ppt=2)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    _announce("nonshared batch")
+    wls = W.batched_sorting(nrep=4, seed=153, sizes=(64, 128))
+    sims = []
+    for i, w in enumerate(wls):
+        s = w.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(20 + i)
+        sims.append(s)
+    gpucpm.Batch(sims).run(**run_kwargs(2, 1, 8, shared=0, positions_per_thread=4))
+    for i, (w, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        lat_o, tab_o = run_oracle(w, 2, 1, 20 + i, 8, shared=False, ppt=4)
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 2)

@@ -697,3 +697,113 @@ def test_persistence_makes_cells_travel():
             prev = c
         path[lam] = total
     assert path[200.0] > 5 * path[0.0], path
+
+
+# ------------------------------------------------- run() roles/schedule ------
+
+def test_schedule_colours_and_groups():
+    """Shared roles: K = 2 colours, one group.  Non-shared (the reference's
+    default): K = ppt / c colours; groups S with S ppt - c >= 3."""
+    assert O.schedule(64, 2, 1, True, 2) == (2, 1)
+    assert O.schedule(64, 2, 4, True, 99) == (2, 1)
+    assert O.schedule(64, 2, 4, False, 8) == (2, 1)   # example2d.py
+    assert O.schedule(64, 2, 2, False, 4) == (2, 2)   # sorting.py: ppt - c = 2
+    assert O.schedule(64, 2, 1, False, 2) == (2, 2)
+    assert O.schedule(64, 2, 1, False, 8) == (8, 1)
+    assert O.schedule(64, 2, 1, False, 1) == (1, 4)
+    assert O.schedule(4, 2, 1, False, 2) == (2, 2)
+    with pytest.raises(ValueError):
+        O.schedule(64, 2, 2, False, 3)
+
+
+def _python_sweep(lat, d, ncells, p, c, u, seed, mcs0, nmcs, shared, ppt):
+    """Plain-Python restatement of the run() phase loop (kernels2d.cu:72-243
+    in the deterministic form of cpm_oracle.h): the schedule, the roles and
+    the Philox counters; the energy comes from the oracle's pinned delta_h."""
+    L = lat.shape[0]
+    K, S = O.schedule(L, d, c, shared, ppt)
+    R = K * c
+    N = 3 ** d - 1
+    offs = [o for o in np.ndindex(*(3,) * d) if any(x != 1 for x in o)]
+    offs = [tuple(np.array(o[::-1]) - 1) for o in offs]  # (dx, dy[, dz]), dx fastest
+    scale = O.energy_scale(p)
+    thr = O.threshold_table(p.temperature, scale)
+    flat = lat.reshape(-1)
+    tab = O.tally(lat, d, ncells)
+    rounds = c ** d // u
+    h = L // R // S
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    nsb = L // c
+    lc = int(np.log2(c))
+
+    def idx(x):
+        return int(sum(int(x[a]) % L * L ** a for a in range(d)))
+
+    for m in range(nmcs):
+        mcs = mcs0 + m
+        for r in range(rounds):
+            for pc in range(K ** d * S ** d):
+                col, grp = divmod(pc, S ** d)
+                cc = [(col // K ** (d - 1 - a)) % K for a in range(d)]   # cbX outermost
+                gg = [(grp // S ** (d - 1 - a)) % S for a in range(d)]
+                phase = r * K ** d * S ** d + pc
+                snap = tab.copy()
+                for qi in range(h ** d):
+                    qa = [(qi // h ** a) % h for a in range(d)]           # x fastest
+                    b = [(qa[a] * S + gg[a]) * K + cc[a] for a in range(d)]
+                    sb = sum(b[a] * nsb ** a for a in range(d))
+                    for i in range(u):
+                        w = O.philox([sb, mcs, phase, i], key)
+                        pos = [b[a] * c + ((int(w[0]) >> (a * lc)) & (c - 1)) for a in range(d)]
+                        dn = (int(w[1]) * N) >> 32
+                        nbr = [(pos[a] + offs[dn][a]) % L for a in range(d)]
+                        t, s_ = (pos, nbr) if shared else (nbr, pos)
+                        ti, si = idx(t), idx(s_)
+                        vs, vt = int(flat[si]), int(flat[ti])
+                        if vs == vt:
+                            continue
+                        dh = O.delta_h(lat, d, snap, p, scale, ti, si)
+                        if not (dh <= 0 or (dh < len(thr) and int(w[2]) < int(thr[dh]))):
+                            continue
+                        flat[ti] = vs
+                        tab = O.tally(lat, d, ncells)  # exact tables (live)
+    return tab
+
+
+@pytest.mark.parametrize("d,L,c,u,shared,ppt", [(2, 8, 1, 1, False, 2), (2, 8, 2, 2, False, 4),
+                                                (2, 16, 1, 1, False, 8), (2, 8, 2, 3, True, 4),
+                                                (3, 8, 1, 1, False, 2)])
+def test_sweep_matches_python_restatement(d, L, c, u, shared, ppt):
+    """The C oracle's schedule and roles equal a plain-Python loop over the
+    phases (shared and non-shared roles)."""
+    rng = np.random.default_rng(L + c + u)
+    lat = np.zeros((L,) * d, np.uint32)
+    for k in range(1, 4):
+        sl = tuple(slice(int(x), int(x) + 3) for x in rng.integers(0, L - 3, size=d))
+        lat[sl] = (np.uint32(1 + k % 2) << np.uint32(24)) | np.uint32(k)
+    p = O.Params.make(20, J=[[0, 6, 6], [6, 8, 4], [6, 4, 8]], lam_area=[0, 1, 1],
+                      target_area=[0, 9, 9])
+    a = lat.copy()
+    tab_py = _python_sweep(a, d, 3, p, c, u, 77, 5, 2, shared, ppt)
+    b = lat.copy()
+    tb = O.tally(b, d, 3)
+    O.sweep(b, d, tb, p, c, u, 77, 5, 2, nthreads=4, shared=shared, ppt=ppt)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(tab_py.area, tb.area)
+    np.testing.assert_array_equal(tab_py.perim, tb.perim)
+    assert np.count_nonzero(a != lat) > 0
+
+
+def test_nonshared_sweep_is_thread_invariant_and_exact():
+    """Non-shared roles with groups (ppt - c < 3): parallel = serial, exact
+    tables (the groups keep concurrent attempts apart)."""
+    lat, n, p = _sorting_small(seed=7)
+    for c, ppt, u in ((1, 2, 1), (2, 4, 1), (2, 4, 4), (4, 8, 2)):
+        a, b = lat.copy(), lat.copy()
+        ta, tb = O.tally(a, 2, n), O.tally(b, 2, n)
+        O.sweep(a, 2, ta, p, c, u, 3, 0, 10, nthreads=1, shared=False, ppt=ppt)
+        O.sweep(b, 2, tb, p, c, u, 3, 0, 10, nthreads=8, shared=False, ppt=ppt)
+        np.testing.assert_array_equal(a, b)
+        area, perim = _numpy_recount(a, 2, n)
+        np.testing.assert_array_equal(ta.area, area)
+        np.testing.assert_array_equal(ta.perim, perim)
