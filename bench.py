@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mcs-per-step", type=int, default=200)
     ap.add_argument("--workload", default="sorting2d",
-                    choices=["sorting2d", "plain3d", "plain2d", "batched", "gradient"])
+                    choices=["sorting2d", "plain3d", "plain2d", "batched", "gradient", "interactive"])
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--checkerboard", type=int, default=1, help="positions_per_checkerboard c")
     ap.add_argument("--updates", type=int, default=1, help="updates_per_checkerboard_switch u")
@@ -257,6 +257,62 @@ def gradient_check(nrep, L, mcs, log, J_AB=11, seed0=0, contrast=False):
     return out
 
 
+def bench_interactive(a, log):
+    """The notebook pattern (examples/gpucpm_sorting_example.ipynb: one
+    run(iterations=ticks, inner_iterations=1) per frame, then a readback):
+    frames per second on the 256^2 sorting lattice with one MCS and one
+    pull_from_gpu per frame, the default shared=0 roles of the notebook
+    (positions_per_thread 4, c 2)."""
+    import gpucpm
+    from gpucpm import workloads
+    wl = workloads.sorting2d(L=256, grid=20, edge=10, seed=2)
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(20240601)
+    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
+              positions_per_thread=4, positions_per_checkerboard=2,
+              updates_per_checkerboard_switch=1, updates_per_barrier=1,
+              iterations=1, inner_iterations=1, shared=0, partial_dispatch=1)
+    frames = a.steps * a.mcs_per_step
+    for _ in range(a.warmup * 10):
+        sim.run(**kw)
+        sim.pull_from_gpu()
+    log(f"interactive: {frames} frames of 1 MCS + readback at 256^2")
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.run(**kw)
+        sim.pull_from_gpu()
+    wall = time.perf_counter() - t0
+    sim.timing_begin()
+    for _ in range(a.steps):
+        sim.run(**kw)
+    kernel_ms, launches, updates = sim.timing_end()
+    launch = sim.last_launch()
+    return {
+        "metric": "frames/s (1 MCS + lattice readback per frame)",
+        "value": frames / wall,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "interactive notebook pattern: 2D sorting 256^2, 400 cells, "
+                               "run(iterations=1) + pull per frame, shared=0, ppt 4, c 2",
+                   "frames": frames},
+        "ms_per_frame": wall / frames * 1e3,
+        "kernel_ms_per_mcs": kernel_ms / a.steps,
+        "launches_per_mcs": launches / a.steps,
+        "kernel": launch["kernel"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def bench_gradient(a, log):
     out = gradient_check(32, 512, a.steps * a.mcs_per_step, log, contrast=True)
     line = {
@@ -347,6 +403,10 @@ def main():
     log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
     if a.workload == "batched":
         return bench_batched(a, log)
+    if a.workload == "interactive":
+        out = bench_interactive(a, log)
+        print(json.dumps(out), flush=True)
+        return 0
     if a.workload == "gradient":
         return bench_gradient(a, log)
     import gpucpm
