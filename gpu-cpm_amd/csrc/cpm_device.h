@@ -113,6 +113,7 @@ struct PhaseArgs {
     uint64_t* claims;       // cell_sync: per-cell claim keys
     uint32_t sync_tag;      // cell_sync: phase tag (grows across runs)
     int form;               // kForm* bits: which per-phase kernel runs
+    int att0, natt;         // attempt indices att0 .. att0 + natt - 1 of each sub-block
 };
 
 // Per-phase kernel forms (results never depend on them).

@@ -849,8 +849,6 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (int rc = check_handle(h)) return rc;
     if (!a) return fail(CPM_EINVAL, "null run arguments");
     if (!h->pushed) return fail(CPM_ESTATE, "run before push_to_gpu");
-    if (a->cell_sync && a->updates_per_checkerboard_switch != 1)
-        return fail(CPM_ENOTIMPL, "cell_sync is implemented for updates_per_checkerboard_switch = 1");
     if (a->threads_per_block < 1 || a->positions_per_thread < 1 || a->updates_per_barrier < 1)
         return fail(CPM_EINVAL, "threads_per_block, positions_per_thread and updates_per_barrier must be >= 1");
     const int c = a->positions_per_checkerboard;
@@ -1004,21 +1002,28 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                     phase_offsets(h->d, k, pc, &pa.cx, &pa.cy, &pa.cz);
                     pa.phase = (uint32_t)(r * k.phases_per_round + pc);
                     pa.g = g;
-                    if (a->cell_sync) {
-                        // the tag only grows: claims never need clearing
-                        if (++h->sync_tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
-                        pa.sync_tag = h->sync_tag;
-                        pa.claims = h->d_claims;
-                        pa.sync_pass = 1;
-                        HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, nullptr, nullptr));
-                        pa.sync_pass = 2;
-                    }
-                    hipEvent_t e0, e1;
-                    if (int rc = h->next_events(&e0, &e1)) return rc;
-                    HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1, &h->last_kernel));
-                    if (h->timing) {
-                        h->t_launches++;
-                        h->t_updates += (int64_t)pa.nsub * u;
+                    // cell_sync: per attempt index, a claim launch and a commit
+                    // launch; otherwise one launch makes all u attempts
+                    const int nidx = a->cell_sync ? u : 1;
+                    for (int i = 0; i < nidx; i++) {
+                        pa.att0 = a->cell_sync ? i : 0;
+                        pa.natt = a->cell_sync ? 1 : u;
+                        if (a->cell_sync) {
+                            // the tag only grows: claims never need clearing
+                            if (++h->sync_tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
+                            pa.sync_tag = h->sync_tag;
+                            pa.claims = h->d_claims;
+                            pa.sync_pass = 1;
+                            HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, nullptr, nullptr));
+                            pa.sync_pass = 2;
+                        }
+                        hipEvent_t e0, e1;
+                        if (int rc = h->next_events(&e0, &e1)) return rc;
+                        HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1, &h->last_kernel));
+                        if (h->timing) {
+                            h->t_launches++;
+                            h->t_updates += (int64_t)pa.nsub * pa.natt;
+                        }
                     }
                 }
             }

@@ -594,17 +594,19 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
                                                    const PV& P, const uint32_t* __restrict__ thr,
                                                    int bx, int by, int bz, uint32_t mcs,
                                                    uint32_t phase, const PhaseTables& T,
-                                                   const CellSync cy = CellSync{0, nullptr, 0}) {
+                                                   const CellSync cy = CellSync{0, nullptr, 0},
+                                                   int att0 = 0, int natt = -1) {
     const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
-    if (k.u == 1) {
+    if (natt < 0) natt = k.u;
+    if (natt == 1) {
         // One attempt, straight-line: with no loop the compiler keeps no
         // loop-invariant values alive (they would spill scalar registers).
         Draw<D, Lat> d;
-        draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, 0u, d);
+        draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, (uint32_t)att0, d);
         finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
         return;
     }
-    for (int i = 0; i < k.u; i++) {
+    for (int i = att0; i < att0 + natt; i++) {
         Draw<D, Lat> d;
         draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, (uint32_t)i, d);
         finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
@@ -699,7 +701,7 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
         const int bz = D == 3 ? ((gtid >> (2 * a.lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
         sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
-                                     CellSync{a.sync_pass, a.claims, a.sync_tag});
+                                     CellSync{a.sync_pass, a.claims, a.sync_tag}, a.att0, a.natt);
     }
     if (fold) {
         if (dv != 0) {

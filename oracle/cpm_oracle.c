@@ -459,7 +459,6 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt) {
     if (d != 2 && d != 3) return -1;
-    if (cell_sync && u != 1) return -7;
     if (shared && act_on(p) && p->act != NULL && c < 2) return -8;
     if (L < 2 || (L & (L - 1))) return -2;
     int K, S;
@@ -515,13 +514,15 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                 uint32_t phase = (uint32_t)(r * ncol * ngrp + pc);
                 memcpy(areaR, area, sizeof(int32_t) * (size_t)(ncells + 1));
                 memcpy(perimR, perim, sizeof(int32_t) * (size_t)(ncells + 1));
-                if (cell_sync) memset(owner, 0xFF, sizeof(uint32_t) * (size_t)(ncells + 1));
-                /* cell_sync (kernels.cu:123-141, as a deterministic rule): pass 0
-                 * claims the cells of every accepted proposal for the lowest
-                 * sub-block index; pass 1 recomputes each attempt (nothing has
-                 * been written) and commits it only if its sub-block holds the
-                 * claim on both of its non-medium cells. */
-                for (int pass = cell_sync ? 0 : 1; pass < 2; pass++)
+                /* cell_sync (kernels.cu:123-141, as a deterministic rule), per
+                 * attempt index i: pass 0 claims the cells of every accepted
+                 * proposal i for the lowest sub-block index; pass 1 recomputes
+                 * each attempt i (nothing has been written) and commits it only
+                 * if its sub-block holds the claim on both of its non-medium
+                 * cells.  Without cell_sync one pass makes all u attempts. */
+                for (int i0 = 0; i0 < u; i0 += cell_sync ? 1 : u)
+                for (int pass = cell_sync ? 0 : 1; pass < 2; pass++) {
+                if (cell_sync && pass == 0) memset(owner, 0xFF, sizeof(uint32_t) * (size_t)(ncells + 1));
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
@@ -531,7 +532,7 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                     int by = ((int)((qi / h) % h) * S + gy) * K + cy;
                     int bz = d == 3 ? ((int)(qi / ((int64_t)h * h)) * S + gz) * K + cz : 0;
                     uint32_t sb = (uint32_t)(((int64_t)bz * nsb + by) * nsb + bx);
-                    for (int i = 0; i < u; i++) {
+                    for (int i = i0; i < (cell_sync ? i0 + 1 : u); i++) {
                         uint32_t ctr[4] = {sb, mcs, phase, (uint32_t)i};
                         uint32_t w[4];
                         orc_philox4x32_10(ctr, key, w);
@@ -617,6 +618,7 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                             }
                         }
                     }
+                }
                 }
             }
         }

@@ -421,3 +421,27 @@ def test_nonshared_cell_sync_and_batch():
         lat_o, tab_o = run_oracle(w, 2, 1, 20 + i, 8, shared=False, ppt=4)
         np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
         assert_tables_equal(s.get_cell_table(), tab_o, 2)
+
+
+@pytest.mark.parametrize("d,c,u,shared", [(2, 2, 4, 1), (2, 2, 3, 0), (3, 2, 8, 1)])
+def test_cell_sync_several_updates_matches_oracle(d, c, u, shared):
+    """cell_sync with u > 1: a claim and a commit launch per attempt index
+    (at most one flip per cell per index), bit-exact against the oracle."""
+    gpucpm, W = _gpu()
+    import oracle as O
+    from helpers import oracle_params
+    wl = W.sorting2d(L=64, grid=5, edge=10, seed=161) if d == 2 else W.plain3d(L=32, ncells=60, seed=162)
+    _announce("cell_sync u", d, c, u, shared)
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(17)
+    n = 6 if d == 2 else 2
+    sim.run(**run_kwargs(c, u, n, cell_sync=1, shared=shared, positions_per_thread=2 * c))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    lat_o = wl.lattice.copy()
+    tab_o = O.tally(lat_o, d, wl.ncells)
+    O.sweep(lat_o, d, tab_o, oracle_params(wl), c, u, 17, 0, n, 8, cell_sync=True, shared=bool(shared),
+            ppt=2 * c)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, d)

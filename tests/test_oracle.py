@@ -319,8 +319,23 @@ def test_cell_sync_is_deterministic_and_stricter():
     t0 = O.tally(lat, 2, n)
     t1 = O.tally(sync, 2, n)
     assert np.max(np.abs(t1.area.astype(int) - t0.area)) <= 4
-    with pytest.raises(ValueError):
-        O.sweep(lat.copy(), 2, O.tally(lat, 2, n), p, 2, 4, 5, 0, 1, cell_sync=True)
+    # u > 1: one claim/commit round per attempt index, so per MCS (4 colour
+    # phases x u indices, one round) a cell changes at most 4 u times
+    res = []
+    for nth in (1, 8):
+        a = lat.copy()
+        t = O.tally(a, 2, n)
+        O.sweep(a, 2, t, p, 2, 4, 5, 0, 1, nthreads=nth, cell_sync=True)
+        res.append((a, t))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    area, perim = _numpy_recount(res[0][0], 2, n)
+    np.testing.assert_array_equal(res[0][1].area, area)
+    np.testing.assert_array_equal(res[0][1].perim, perim)
+    t0 = O.tally(lat, 2, n)
+    assert np.max(np.abs(res[0][1].area.astype(int) - t0.area)) <= 16
+    plain4 = lat.copy()
+    O.sweep(plain4, 2, O.tally(plain4, 2, n), p, 2, 4, 5, 0, 1)
+    assert 0 < np.count_nonzero(res[0][0] != lat) <= np.count_nonzero(plain4 != lat)
 
 
 # ------------------------------------------------------- fixed types ------
