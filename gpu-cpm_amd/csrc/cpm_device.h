@@ -137,6 +137,7 @@ struct PersistArgs {
 // One replica of a batched run.
 struct ReplicaDesc {
     SweepConsts k;
+    uint64_t* claims;       // cell_sync claim keys of this replica
     uint32_t mcs0;          // this replica's MCS counter at the start
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int block0, nblocks;    // its block group in the cooperative grid
@@ -160,6 +161,9 @@ struct BatchPhaseArgs {
     uint32_t phase;         // round * 2^d + colour
     uint32_t m;             // MCS offset within this run (replica MCS = mcs0 + m)
     int g;                  // phase counter since the last settle (parity)
+    int sync_pass;          // cell_sync: 0 off, 1 claim, 2 commit
+    uint32_t sync_tag;      // cell_sync: tag (above every replica's earlier tags)
+    int att0, natt;         // attempt indices of this launch
 };
 
 // Host-side launchers (kernels.hip).

@@ -1221,6 +1221,22 @@ namespace {
 int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::ReplicaDesc>& reps,
                     long total) {
     const int n = (int)b->sims.size();
+    uint32_t tag = 0;  // cell_sync: one tag for all replicas, above each one's earlier tags
+    bool pers = false;
+    for (int i = 0; i < n; i++) {
+        cpm_handle h = b->sims[i];
+        if (a->cell_sync && !h->d_claims) {
+            HIP_TRY(hipMalloc(&h->d_claims, sizeof(uint64_t) * ((size_t)h->n + 1)));
+            HIP_TRY(hipMemsetAsync(h->d_claims, 0, sizeof(uint64_t) * ((size_t)h->n + 1), h->stream));
+        }
+        reps[i].claims = h->d_claims;
+        tag = std::max(tag, h->sync_tag);
+        if (reps[i].k.pers) {
+            pers = true;
+            if (!h->ring_valid)
+                if (int rc = h->init_ring()) return rc;
+        }
+    }
     int nblocks = 0;
     for (int i = 0; i < n; i++) {
         const long nsub = 1L << (reps[i].lh * b->d);
@@ -1253,6 +1269,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     const gcpm::SweepConsts& k0 = reps[0].k;
     const int rounds = k0.rounds;
     pa.lstride = k0.lstride;
+    const int u = a->updates_per_checkerboard_switch;
     int g = 0;
     for (long m = 0; m < total; m++) {
         pa.m = (uint32_t)m;
@@ -1261,11 +1278,38 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                 phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
                 pa.phase = (uint32_t)(r * k0.phases_per_round + pc);
                 pa.g = g;
-                HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
+                // as cpm_run: cell_sync makes a claim and a commit launch per
+                // attempt index
+                const int nidx = a->cell_sync ? u : 1;
+                for (int i = 0; i < nidx; i++) {
+                    pa.att0 = a->cell_sync ? i : 0;
+                    pa.natt = a->cell_sync ? 1 : u;
+                    pa.sync_pass = 0;
+                    if (a->cell_sync) {
+                        if (++tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
+                        pa.sync_tag = tag;
+                        pa.sync_pass = 1;
+                        HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
+                        pa.sync_pass = 2;
+                    }
+                    HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
+                }
             }
         }
         g &= 1;
+        if (pers && (m + 1) % a->inner_iterations == 0) {
+            // end of a chunk: settle every replica, then update the preferred
+            // directions of those with a persistence term (host, per replica)
+            for (int i = 0; i < n; i++)
+                HIP_TRY(gcpm::launch_settle(b->sims[i]->d_tab, b->sims[i]->d_del, g, b->sims[i]->n, b->stream));
+            g = 0;
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            for (int i = 0; i < n; i++)
+                if (reps[i].k.pers)
+                    if (int rc = b->sims[i]->update_pref_dirs()) return rc;
+        }
     }
+    for (int i = 0; i < n; i++) b->sims[i]->sync_tag = tag;
     for (int i = 0; i < n; i++)
         HIP_TRY(gcpm::launch_settle(b->sims[i]->d_tab, b->sims[i]->d_del, g, b->sims[i]->n, b->stream));
     HIP_TRY(hipEventRecord(b->done, b->stream));
@@ -1326,8 +1370,6 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     std::vector<long> need(n);
     for (int i = 0; i < n; i++) {
         if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
-        if (reps[i].k.pers)
-            return fail(CPM_ENOTIMPL, "the persistence term is not available for batched runs");
         if (reps[i].k.K != reps[0].k.K || reps[i].k.S != reps[0].k.S)
             return fail(CPM_EINVAL, "batched replicas need the same colour schedule (K, S)");
         const long nsub = 1L << (reps[i].lh * b->d);
@@ -1336,10 +1378,12 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     }
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
-    if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync is not available for batched runs");
     if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations);
     if (reps[0].k.roles)
         return fail(CPM_ENOTIMPL, "the persistent batch form implements shared roles only (shared = 1)");
+    if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync needs the per-phase batch form");
+    for (int i = 0; i < n; i++)
+        if (reps[i].k.pers) return fail(CPM_ENOTIMPL, "persistence needs the per-phase batch form");
     if (b->d == 3) return fail(CPM_ENOTIMPL, "the persistent batch form is off in 3D");
 
     // Block groups: each replica gets blocks in proportion to its work, at

@@ -934,9 +934,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
         const int by = (((gtid >> lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T);
+        sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T,
+                                     CellSync{a.sync_pass, rd.claims, a.sync_tag}, a.att0, a.natt);
     }
-    fold_tables<false>(T, k.ncells, gtid, gsz);
+    if (a.sync_pass != 1) fold_tables<false>(T, k.ncells, gtid, gsz);
 }
 
 // One 3D colour phase with positions_per_checkerboard = 1 (every sub-block

@@ -89,3 +89,56 @@ def test_batch_3d_per_phase():
         lat_o, tab_o = run_oracle(wl, 2, 8, 300 + i, 6)
         np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
         assert_tables_equal(s.get_cell_table(), tab_o, 3)
+
+
+@pytest.mark.parametrize("c,u,shared", [(1, 1, 1), (2, 4, 0)])
+def test_batch_cell_sync_equals_single_runs(c, u, shared):
+    """cell_sync in a batch: each replica bit-identical to its own single run
+    (which the single-run tests pin to the oracle)."""
+    import gpucpm
+    wls = _small_batch()[:4]
+    print("CASE batch cell_sync", c, u, shared, flush=True)
+    kw = run_kwargs(c, u, 6, cell_sync=1, shared=shared, positions_per_thread=2 * c)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(300 + i)
+        sims.append(s)
+    gpucpm.Batch(sims).run(**kw)
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        t = wl.make_cpm()
+        t.push_to_gpu()
+        t.set_seed(300 + i)
+        t.run(**kw)
+        t.synchronize()
+        t.pull_from_gpu()
+        np.testing.assert_array_equal(s.get_state(), t.get_state())
+
+
+def test_batch_persistence_equals_single_runs():
+    """3D replicas with a persistence term in a batch, chunked by
+    inner_iterations: lattice and preferred directions bit-identical to
+    single runs and to the oracle."""
+    import gpucpm
+    from gpucpm import workloads as W
+    wls = [W.persistent(W.plain3d(L=32, ncells=30, seed=310 + i), lam=(0.0, 200.0 + 50 * i),
+                        diffusion=(0.0, 0.6), history=2) for i in range(3)]
+    print("CASE batch persistence", flush=True)
+    kw = run_kwargs(1, 1, 6, inner=2)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(320 + i)
+        sims.append(s)
+    gpucpm.Batch(sims).run(**kw)
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        lat_o, tab_o = run_oracle(wl, 1, 1, 320 + i, 6, inner=2)
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 3)
+        np.testing.assert_array_equal(s.get_pref_dirs().reshape(-1), tab_o.pref)
