@@ -164,6 +164,7 @@ struct BatchPhaseArgs {
     int sync_pass;          // cell_sync: 0 off, 1 claim, 2 commit
     uint32_t sync_tag;      // cell_sync: tag (above every replica's earlier tags)
     int att0, natt;         // attempt indices of this launch
+    int form;               // kForm* bits (compact, small, attempts per thread)
 };
 
 // Host-side launchers (kernels.hip).

@@ -1237,12 +1237,29 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                 if (int rc = h->init_ring()) return rc;
         }
     }
+    // Kernel form, as for single runs: compacted when u = 1 (GPUCPM_COMPACT),
+    // parameters without staging when every replica has at most four types
+    // (GPUCPM_SMALL; default in 3D), GPUCPM_ATT attempts drawn per thread.
+    auto env_int = [](const char* name, int dflt) {
+        const char* e = std::getenv(name);
+        return e ? std::atoi(e) : dflt;
+    };
+    const int u1 = a->updates_per_checkerboard_switch;
+    const bool compact = u1 == 1 && env_int("GPUCPM_COMPACT", 1) != 0;
+    bool small = env_int("GPUCPM_SMALL", b->d == 3) != 0;
+    for (int i = 0; i < n; i++) small = small && reps[i].k.nt <= gcpm::kSmallTypes;
+    const int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 4 : 1) : 1;
+    const int form = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
+                     (att == 4 ? gcpm::kFormAtt4 : att == 2 ? gcpm::kFormAtt2 : 0);
+    const long per_block = (long)gcpm::kBlock * (compact ? att : 1);
     int nblocks = 0;
     for (int i = 0; i < n; i++) {
         const long nsub = 1L << (reps[i].lh * b->d);
-        const long work = std::max(nsub, (long)b->sims[i]->n + 1);
+        // compacted: the fold strides over the cells, so blocks follow the draws
+        const long nb = compact ? std::max(1L, (nsub + per_block - 1) / per_block)
+                                : (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
         reps[i].block0 = nblocks;
-        reps[i].nblocks = (int)((work + gcpm::kBlock - 1) / gcpm::kBlock);
+        reps[i].nblocks = (int)nb;
         reps[i].mcs0 = (uint32_t)b->sims[i]->mcs;
         nblocks += reps[i].nblocks;
     }
@@ -1266,6 +1283,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     gcpm::BatchPhaseArgs pa{};
     pa.reps = b->d_reps;
     pa.rep_of = b->d_rep_of;
+    pa.form = form;
     const gcpm::SweepConsts& k0 = reps[0].k;
     const int rounds = k0.rounds;
     pa.lstride = k0.lstride;

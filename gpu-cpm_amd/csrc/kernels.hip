@@ -745,19 +745,30 @@ struct QueuedAttempt {
     uint32_t nv[NQ > 0 ? NQ : 1];
 };
 
+// One colour phase of one lattice as seen by a block group (the whole grid,
+// or the blocks of one replica in a batch).
+struct CompactPhase {
+    int nsub, lh, lstride, cx, cy, cz, g;
+    uint32_t phase, mcs;
+    int sync_pass;
+    uint64_t* claims;
+    uint32_t sync_tag;
+    int lblock;     // this block's index in its group
+    int gtid, gsz;  // thread index and count in the group (fold)
+};
+
 // Thread t of block b draws the attempts of same-colour sub-blocks
 // (b ATT + r) blockDim + t for r < ATT (consecutive threads, consecutive
 // sub-blocks), so the block's queue collects up to ATT blockDim attempts.
 template <int D, bool STAGE, bool FULL, int ATT>
-__global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
+__device__ __forceinline__ void compact_phase(const SweepConsts& k, const CompactPhase& a) {
     static_assert(!FULL || ATT == 1, "the full-neighbourhood queue holds one attempt per thread");
     __shared__ LdsParams sp;
     __shared__ QueuedAttempt<D, FULL> queue[kBlock * ATT];
     __shared__ uint32_t nqueued;
     constexpr int NQ = QueuedAttempt<D, FULL>::NQ;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gsz = gridDim.x * blockDim.x;
-    const SweepConsts& k = a.k;
+    const int gtid = a.gtid;
+    const int gsz = a.gsz;
     const PhaseTables T = tables_of(k, a.g);
     // Loads in flight together: parameters, the fold's deltas, the draws.
     constexpr int RP = (kParamWords + kBlock - 1) / kBlock;
@@ -785,7 +796,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
     QueuedAttempt<D, FULL> e[ATT];
 #pragma unroll
     for (int r = 0; r < ATT; r++) {
-        const int q = ((int)blockIdx.x * ATT + r) * kBlock + (int)threadIdx.x;
+        const int q = (a.lblock * ATT + r) * kBlock + (int)threadIdx.x;
         active[r] = false;
         if (q < a.nsub) {
             const int hm = (1 << a.lh) - 1;
@@ -903,6 +914,29 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
             }
         }
     }
+}
+
+
+template <int D, bool STAGE, bool FULL, int ATT>
+__global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
+    const CompactPhase cp{a.nsub, a.lh, a.lstride, a.cx, a.cy, a.cz, a.g, a.phase, a.mcs,
+                          a.sync_pass, a.claims, a.sync_tag, (int)blockIdx.x,
+                          (int)(blockIdx.x * blockDim.x + threadIdx.x),
+                          (int)(gridDim.x * blockDim.x)};
+    compact_phase<D, STAGE, FULL, ATT>(a.k, cp);
+}
+
+// Batched replicas, one colour phase, compacted: block b serves replica
+// rep_of[b] as block b - block0 of its group.
+template <int D, bool STAGE, int ATT>
+__global__ __launch_bounds__(kBlock) void k_batch_phase_compact(BatchPhaseArgs a) {
+    const int r = a.rep_of[blockIdx.x];
+    const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
+    const int lb = (int)blockIdx.x - rd.block0;
+    const CompactPhase cp{1 << (D * rd.lh), rd.lh, a.lstride, a.cx, a.cy, a.cz, a.g, a.phase,
+                          rd.mcs0 + a.m, a.sync_pass, rd.claims, a.sync_tag, lb,
+                          lb * (int)blockDim.x + (int)threadIdx.x, rd.nblocks * (int)blockDim.x};
+    compact_phase<D, STAGE, false, ATT>(rd.k, cp);
 }
 
 // Any number of types: parameters and thresholds staged in LDS.
@@ -1398,6 +1432,28 @@ hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s) {
 }
 
 hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
+    const bool compact = a.form & kFormCompact;
+    const bool small = a.form & kFormSmall;
+    const int att = (a.form & kFormAtt4) ? 4 : (a.form & kFormAtt2) ? 2 : 1;
+#define CPM_BATCH(DD, ST, AT) \
+    hipLaunchKernelGGL((k_batch_phase_compact<DD, ST, AT>), dim3(blocks), dim3(kBlock), 0, s, a)
+    if (compact) {
+        if (d == 2) {
+            if (small) {
+                if (att == 4) CPM_BATCH(2, false, 4); else if (att == 2) CPM_BATCH(2, false, 2); else CPM_BATCH(2, false, 1);
+            } else {
+                if (att == 4) CPM_BATCH(2, true, 4); else if (att == 2) CPM_BATCH(2, true, 2); else CPM_BATCH(2, true, 1);
+            }
+        } else {
+            if (small) {
+                if (att == 4) CPM_BATCH(3, false, 4); else if (att == 2) CPM_BATCH(3, false, 2); else CPM_BATCH(3, false, 1);
+            } else {
+                if (att == 4) CPM_BATCH(3, true, 4); else if (att == 2) CPM_BATCH(3, true, 2); else CPM_BATCH(3, true, 1);
+            }
+        }
+        return hipGetLastError();
+    }
+#undef CPM_BATCH
     if (d == 2)
         hipLaunchKernelGGL(k_batch_phase<2>, dim3(blocks), dim3(kBlock), 0, s, a);
     else
