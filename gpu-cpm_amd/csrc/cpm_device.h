@@ -67,8 +67,18 @@ struct ParamView {
 // so after it W holds the end-of-phase state with no extra launch or
 // barrier.  A settle after the last phase folds the last deltas into the
 // lagging base, leaving both bases equal and both delta buffers zero.
+//
+// The per-phase kernels may run on a 16-bit copy of the lattice (lat16),
+// made when every site fits: cell | type << idbits with idbits = 16 - the
+// bits of the largest type, cells up to 2^idbits - 1.  The map is one to
+// one on the sites present, so equal sites stay equal and every result is
+// that of the 32-bit lattice (idbits = 24); it halves the bytes each phase
+// moves.
 struct SweepConsts {
     uint32_t* lat;
+    uint16_t* lat16;        // the 16-bit lattice, or null
+    uint32_t idbits;        // a site holds cell | type << idbits
+    uint32_t idmask;        // (1 << idbits) - 1
     int64_t* tab[2];
     int64_t* del[2];
     const DevParams* prm;
@@ -165,6 +175,7 @@ struct BatchPhaseArgs {
     uint32_t sync_tag;      // cell_sync: tag (above every replica's earlier tags)
     int att0, natt;         // attempt indices of this launch
     int form;               // kForm* bits (compact, small, attempts per thread)
+    int narrow;             // every replica on its 16-bit lattice (else all on 32-bit)
 };
 
 // Host-side launchers (kernels.hip).
@@ -187,6 +198,9 @@ hipError_t launch_settle(int64_t* const tab[2], int64_t* const del[2], int G, in
                          hipStream_t s);
 // Centroid sums of the current lattice about the push-time anchors
 // (csum must be zero on entry).
+// Copy the 32-bit lattice into its 16-bit form (to_narrow) or back; sites % 4 == 0.
+hipError_t launch_convert(uint32_t* wide, uint16_t* narrow, size_t sites, int idbits, bool to_narrow,
+                          hipStream_t s);
 hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* anchor, int64_t* csum,
                        hipStream_t s);
 // pairs[20*20] and mom[20*5] must be zero on entry; ctype[n+1] is scratch.

@@ -138,6 +138,13 @@ struct cpm_engine {
     std::vector<int32_t> ctype;          // [n+1] type of each cell at push
     hipStream_t stream = nullptr;
     uint32_t* d_lat = nullptr;
+    // 16-bit form of the lattice (cell | type << lat16_bits) for the per-phase
+    // kernels; wide_valid / narrow_valid say which copy holds the current
+    // state (at least one does).
+    uint16_t* d_lat16 = nullptr;
+    int lat16_bits = 0;
+    bool wide_valid = true;
+    bool narrow_valid = false;
     int64_t* d_tab[2] = {nullptr, nullptr};
     int64_t* d_del[2] = {nullptr, nullptr};
     int32_t* d_anchor = nullptr;
@@ -184,6 +191,7 @@ struct cpm_engine {
         // Teardown is best effort: errors here have no caller to report to.
         if (stream) (void)hipStreamSynchronize(stream);
         (void)hipFree(d_lat);
+        (void)hipFree(d_lat16);
         for (int i = 0; i < 2; i++) (void)hipFree(d_tab[i]);
         for (int i = 0; i < 2; i++) (void)hipFree(d_del[i]);
         (void)hipFree(d_anchor);
@@ -237,12 +245,44 @@ struct cpm_engine {
         pref_dirty = true;
     }
 
+    // Cell bits of the 16-bit lattice for the current cells and types, or 0
+    // when the sites do not fit in 16 bits (or GPUCPM_NARROW=0).
+    int narrow_bits() const {
+        if (const char* e = std::getenv("GPUCPM_NARROW"))
+            if (std::atoi(e) == 0) return 0;
+        int tb = 0;
+        while ((1 << tb) < nt) tb++;
+        const int bits = 16 - tb;
+        return n <= (1 << bits) - 1 && sites % 4 == 0 ? bits : 0;
+    }
+
+    // Make the 32-bit lattice current (after runs on the 16-bit one).
+    int ensure_wide() {
+        if (!wide_valid) {
+            HIP_TRY(gcpm::launch_convert(d_lat, d_lat16, sites, lat16_bits, false, stream));
+            wide_valid = true;
+        }
+        return CPM_OK;
+    }
+
+    // Make the 16-bit lattice with `bits` cell bits current.
+    int ensure_narrow(int bits) {
+        if (narrow_valid && lat16_bits == bits) return CPM_OK;
+        if (int rc = ensure_wide()) return rc;
+        if (!d_lat16) HIP_TRY(hipMalloc(&d_lat16, sites * 2));
+        HIP_TRY(gcpm::launch_convert(d_lat, d_lat16, sites, bits, true, stream));
+        lat16_bits = bits;
+        narrow_valid = true;
+        return CPM_OK;
+    }
+
     // Centroids c = anchor + csum / area of the current lattice (settled
     // tables), into out [(n+1)*3]; dead cells get 0.
     int centroids_now(std::vector<double>& out, std::vector<int32_t>& areas) {
         const size_t rows = (size_t)n + 1;
         std::vector<int64_t> tab(rows), csum(rows * 3);
         std::vector<int32_t> anchor(rows * 3);
+        if (int rc = ensure_wide()) return rc;
         HIP_TRY(hipMemsetAsync(d_csum, 0, rows * 3 * 8, stream));
         HIP_TRY(gcpm::launch_csum(d, d_lat, lgL, d_anchor, d_csum, stream));
         HIP_TRY(hipMemcpyAsync(tab.data(), d_tab[0], rows * 8, hipMemcpyDeviceToHost, stream));
@@ -814,6 +854,8 @@ int cpm_push(cpm_handle h) {
     if (h->has_act)  // cpm.cu:548
         HIP_TRY(hipMemcpyAsync(h->d_act, h->h_act, h->sites * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(h->d_lat, h->h_lat, h->sites * 4, hipMemcpyHostToDevice, s));
+    h->wide_valid = true;
+    h->narrow_valid = false;
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
     for (int i = 0; i < 2; i++) HIP_TRY(hipMemsetAsync(h->d_del[i], 0, rows * 8, s));
     HIP_TRY(hipMemsetAsync(h->d_first, 0xFF, rows * 4, s));
@@ -880,6 +922,9 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (int rc = h->upload_params()) return rc;
     gcpm::SweepConsts k{};
     k.lat = h->d_lat;
+    k.lat16 = nullptr;  // set by the caller when it runs on the 16-bit lattice
+    k.idbits = 24;
+    k.idmask = gcpm::kSpinMask;
     for (int i = 0; i < 2; i++) k.tab[i] = h->d_tab[i];
     for (int i = 0; i < 2; i++) k.del[i] = h->d_del[i];
     k.prm = h->d_prm;
@@ -972,6 +1017,18 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         // global reads at 256^3 (61 vs 44 us per phase), so it is opt-in.
         const char* bx = std::getenv("GPUCPM_BOX3");
         pa.box3 = bx ? std::atoi(bx) : 0;
+        // The per-phase kernels run on the 16-bit lattice when every site
+        // fits (the LDS box form reads 32-bit sites only).
+        const int bits = pa.box3 ? 0 : h->narrow_bits();
+        if (bits) {
+            if (int rc = h->ensure_narrow(bits)) return rc;
+            pa.k.lat16 = h->d_lat16;
+            pa.k.idbits = (uint32_t)bits;
+            pa.k.idmask = (1u << bits) - 1u;
+        } else {
+            if (int rc = h->ensure_wide()) return rc;
+            h->narrow_valid = false;
+        }
         // Kernel form (measured fastest by default): GPUCPM_SMALL=1 reads
         // parameters without LDS staging (default in 3D),
         // GPUCPM_COMPACT=0 finishes every attempt in its own thread,
@@ -997,6 +1054,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             if (int rc = h->init_ring()) return rc;
         for (long m = 0; m < total; m++) {
             pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
+            if (bits) h->wide_valid = false;  // (a chunk end may have refreshed it)
             for (int r = 0; r < k.rounds; r++) {
                 for (int pc = 0; pc < k.phases_per_round; pc++, g++) {
                     phase_offsets(h->d, k, pc, &pa.cx, &pa.cy, &pa.cz);
@@ -1040,6 +1098,8 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         // then hold the current state and both delta buffers are zero.
         HIP_TRY(gcpm::launch_settle(h->d_tab, h->d_del, g, h->n, h->stream));
     } else {
+        if (int rc = h->ensure_wide()) return rc;
+        h->narrow_valid = false;
         gcpm::PersistArgs pa{};
         pa.k = k;
         pa.lh = lh;
@@ -1074,6 +1134,7 @@ int cpm_sync(cpm_handle h) {
 
 int cpm_pull(cpm_handle h) {
     if (int rc = check_handle(h)) return rc;
+    if (int rc = h->ensure_wide()) return rc;
     HIP_TRY(hipMemcpyAsync(h->h_lat, h->d_lat, h->sites * 4, hipMemcpyDeviceToHost, h->stream));
     if (h->has_act)  // cpm.cu:535
         HIP_TRY(hipMemcpyAsync(h->h_act, h->d_act, h->sites * 4, hipMemcpyDeviceToHost, h->stream));
@@ -1095,6 +1156,7 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
         // Centroid sums about the push-time anchors, rebuilt from the current
         // lattice: equal to sums kept per accepted flip (each site adds
         // wrap(x - anchor) while it belongs to the cell).
+        if (int rc = h->ensure_wide()) return rc;
         HIP_TRY(hipMemsetAsync(h->d_csum, 0, rows * 3 * 8, h->stream));
         HIP_TRY(gcpm::launch_csum(h->d, h->d_lat, h->lgL, h->d_anchor, h->d_csum, h->stream));
         HIP_TRY(hipMemcpyAsync(csum, h->d_csum, rows * 3 * 8, hipMemcpyDeviceToHost, h->stream));
@@ -1251,6 +1313,22 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     const int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 4 : 1) : 1;
     const int form = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
                      (att == 4 ? gcpm::kFormAtt4 : att == 2 ? gcpm::kFormAtt2 : 0);
+    // 16-bit lattices when every replica's sites fit, else all 32-bit.
+    bool narrow = true;
+    for (int i = 0; i < n; i++) narrow = narrow && b->sims[i]->narrow_bits() != 0;
+    for (int i = 0; i < n; i++) {
+        cpm_handle h = b->sims[i];
+        if (narrow) {
+            const int bits = h->narrow_bits();
+            if (int rc = h->ensure_narrow(bits)) return rc;
+            reps[i].k.lat16 = h->d_lat16;
+            reps[i].k.idbits = (uint32_t)bits;
+            reps[i].k.idmask = (1u << bits) - 1u;
+        } else {
+            if (int rc = h->ensure_wide()) return rc;
+            h->narrow_valid = false;
+        }
+    }
     const long per_block = (long)gcpm::kBlock * (compact ? att : 1);
     int nblocks = 0;
     for (int i = 0; i < n; i++) {
@@ -1284,6 +1362,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     pa.reps = b->d_reps;
     pa.rep_of = b->d_rep_of;
     pa.form = form;
+    pa.narrow = narrow ? 1 : 0;
     const gcpm::SweepConsts& k0 = reps[0].k;
     const int rounds = k0.rounds;
     pa.lstride = k0.lstride;
@@ -1291,6 +1370,8 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     int g = 0;
     for (long m = 0; m < total; m++) {
         pa.m = (uint32_t)m;
+        if (narrow)  // (a chunk end may have refreshed them)
+            for (int i = 0; i < n; i++) b->sims[i]->wide_valid = false;
         for (int r = 0; r < rounds; r++) {
             for (int pc = 0; pc < k0.phases_per_round; pc++, g++) {
                 phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
@@ -1403,6 +1484,10 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     for (int i = 0; i < n; i++)
         if (reps[i].k.pers) return fail(CPM_ENOTIMPL, "persistence needs the per-phase batch form");
     if (b->d == 3) return fail(CPM_ENOTIMPL, "the persistent batch form is off in 3D");
+    for (int i = 0; i < n; i++) {  // it reads 32-bit sites
+        if (int rc = b->sims[i]->ensure_wide()) return rc;
+        b->sims[i]->narrow_valid = false;
+    }
 
     // Block groups: each replica gets blocks in proportion to its work, at
     // least one, within the co-resident limit of the cooperative grid.
@@ -1503,6 +1588,7 @@ int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments) {
     const int np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5;
     if (!h->d_stats) HIP_TRY(hipMalloc(&h->d_stats, sizeof(unsigned long long) * (np + nm)));
     if (!h->d_ctype) HIP_TRY(hipMalloc(&h->d_ctype, sizeof(int32_t) * ((size_t)h->n + 1)));
+    if (int rc = h->ensure_wide()) return rc;
     HIP_TRY(hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * (np + nm), h->stream));
     HIP_TRY(gcpm::launch_stats(h->d, h->d_lat, h->lgL, h->d_tab[0], h->n, h->d_stats, h->d_stats + np,
                                h->d_ctype, h->stream));

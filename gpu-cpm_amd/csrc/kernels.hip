@@ -87,9 +87,10 @@ __device__ __forceinline__ void st(T* p, T v) {
 
 // Lattice held in global memory (HBM / L2 / MALL); CO selects
 // device-coherent accesses (persistent kernels).
-template <int D, bool CO>
+template <int D, bool CO, class T = uint32_t>
 struct GlobalLat {
-    uint32_t* p;
+    using Site = T;
+    T* p;
     int lgL, Lm;
     struct Pos {
         int x, y, z;
@@ -102,30 +103,40 @@ struct GlobalLat {
     __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int dz) const {
         return ld<CO>(p + site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL));
     }
-    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { st<CO>(p + q.i, v); }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { st<CO>(p + q.i, (T)v); }
     __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.i; }
     // The 3^D block around q (centre included), (dz, dy, dx) order with dx
     // fastest.  Byte offsets from the lattice base fit in 32 bits (at most
     // 2^30 sites), so every load is base (scalar) + one 32-bit offset.
     __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
+        constexpr int SH = sizeof(T) == 2 ? 1 : 2;  // log2 bytes per site
         const char* base = reinterpret_cast<const char*>(p);
-        const uint32_t xo[3] = {(uint32_t)((q.x - 1) & Lm) << 2, (uint32_t)q.x << 2,
-                                (uint32_t)((q.x + 1) & Lm) << 2};
-        const uint32_t yo[3] = {(uint32_t)((q.y - 1) & Lm) << (lgL + 2), (uint32_t)q.y << (lgL + 2),
-                                (uint32_t)((q.y + 1) & Lm) << (lgL + 2)};
+        const uint32_t xo[3] = {(uint32_t)((q.x - 1) & Lm) << SH, (uint32_t)q.x << SH,
+                                (uint32_t)((q.x + 1) & Lm) << SH};
+        const uint32_t yo[3] = {(uint32_t)((q.y - 1) & Lm) << (lgL + SH), (uint32_t)q.y << (lgL + SH),
+                                (uint32_t)((q.y + 1) & Lm) << (lgL + SH)};
         uint32_t zo[3] = {0u, 0u, 0u};
         if constexpr (D == 3) {
-            zo[0] = (uint32_t)((q.z - 1) & Lm) << (2 * lgL + 2);
-            zo[1] = (uint32_t)q.z << (2 * lgL + 2);
-            zo[2] = (uint32_t)((q.z + 1) & Lm) << (2 * lgL + 2);
+            zo[0] = (uint32_t)((q.z - 1) & Lm) << (2 * lgL + SH);
+            zo[1] = (uint32_t)q.z << (2 * lgL + SH);
+            zo[2] = (uint32_t)((q.z + 1) & Lm) << (2 * lgL + SH);
         }
         constexpr int NB = D == 3 ? 27 : 9;
 #pragma unroll
         for (int n = 0; n < NB; n++)
-            nv[n] = ld<CO>(reinterpret_cast<const uint32_t*>(base + (xo[n % 3] | yo[(n / 3) % 3] |
-                                                                      zo[n / 9])));
+            nv[n] = ld<CO>(reinterpret_cast<const T*>(base + (xo[n % 3] | yo[(n / 3) % 3] | zo[n / 9])));
     }
 };
+
+// The global lattice of a sweep in site type T: the 32-bit lattice, or its
+// 16-bit form (SweepConsts::lat16).
+template <class T>
+__device__ __forceinline__ T* lat_base(const SweepConsts& k) {
+    if constexpr (sizeof(T) == 2)
+        return k.lat16;
+    else
+        return k.lat;
+}
 
 // Neighbourhood loads of a lattice held in LDS.
 template <int D, class Lat, class Pos>
@@ -143,6 +154,7 @@ __device__ __forceinline__ void lds_block(const Lat& lat, const Pos& q, uint32_t
 // stride S = tile width + 2.  Writes go to LDS and through to global memory,
 // where neighbouring tiles read them as their halo in the next phase.
 struct LdsLat2 {
+    using Site = uint32_t;
     uint32_t* s;
     uint32_t* g;
     int S, x0, y0, lgL;
@@ -171,6 +183,7 @@ struct LdsLat2 {
 // one-site halo; strides SX (x) and SXY (z).  Writes go to LDS and to global
 // memory (plain stores: the next launch sees them).
 struct LdsLat3 {
+    using Site = uint32_t;
     uint32_t* s;
     uint32_t* g;
     int SX, SXY, x0, y0, z0, lgL;
@@ -415,11 +428,11 @@ __device__ __forceinline__ int64_t act_geo_mean(const SweepConsts& k, int64_t su
 template <int D, bool CO>
 __device__ __forceinline__ int64_t act_site_mean(const SweepConsts& k, uint32_t gi, uint32_t v,
                                                  const uint32_t* nv, uint32_t tick) {
-    if ((v & kSpinMask) == 0) return 0;
+    if ((v & k.idmask) == 0) return 0;
     constexpr int NB = D == 3 ? 27 : 9;
     const int lgL = k.lgL, Lm = (1 << lgL) - 1;
     const int x = (int)(gi & Lm), y = (int)((gi >> lgL) & Lm), z = D == 3 ? (int)(gi >> (2 * lgL)) : 0;
-    const int64_t am = k.prm->act_max[v >> 24];
+    const int64_t am = k.prm->act_max[v >> k.idbits];
     int64_t sum = 0;
     int count = 0;
     bool zero = false;
@@ -444,7 +457,7 @@ template <int D, bool CO, class Lat>
 __device__ __forceinline__ int64_t act_energy(const SweepConsts& k, const Lat& lat,
                                               const typename Lat::Pos& t, const uint32_t* nvt,
                                               int dir, uint32_t vs, uint32_t vt, uint32_t tick) {
-    const int tl = (vt >> 24) ? (int)(vt >> 24) : (int)(vs >> 24);
+    const int tl = (vt >> k.idbits) ? (int)(vt >> k.idbits) : (int)(vs >> k.idbits);
     const int64_t m = k.prm->act_m[tl];
     if (m == 0) return 0;
     constexpr int NB = D == 3 ? 27 : 9;
@@ -458,13 +471,14 @@ __device__ __forceinline__ int64_t act_energy(const SweepConsts& k, const Lat& l
     // the source's block from global memory (up to two sites from the
     // target: untouched in this phase when positions_per_checkerboard >= 2)
     int64_t g_s = 0;
-    if (vs & kSpinMask) {
+    if (vs & k.idmask) {
         uint32_t nvs[NB];
 #pragma unroll
         for (int n = 0; n < NB; n++) {
             int px, py, pz;
             block_offset<D>(n, px, py, pz);
-            nvs[n] = ld<CO>(k.lat + site_index<D>((sx + px) & Lm, (sy + py) & Lm, (sz + pz) & Lm, lgL));
+            nvs[n] = ld<CO>(lat_base<typename Lat::Site>(k) +
+                             site_index<D>((sx + px) & Lm, (sy + py) & Lm, (sz + pz) & Lm, lgL));
         }
         g_s = act_site_mean<D, CO>(k, gs, vs, nvs, tick);
     }
@@ -479,8 +493,8 @@ __device__ __forceinline__ int64_t act_energy(const SweepConsts& k, const Lat& l
 template <int D>
 __device__ __forceinline__ int64_t pers_energy(const SweepConsts& k, int dir, uint32_t vs,
                                                uint32_t vt) {
-    const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
-    const int tl = cs ? (int)(vs >> 24) : (int)(vt >> 24);
+    const uint32_t cs = vs & k.idmask, ct = vt & k.idmask;
+    const int tl = cs ? (int)(vs >> k.idbits) : (int)(vt >> k.idbits);
     const uint32_t cell = cs ? cs : ct;
     int ox, oy, oz;
     block_offset<D>(dir, ox, oy, oz);
@@ -504,15 +518,15 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     uint32_t vs = d.vs;
     if (vs == vt) return;  // kernels2d.cu:276
     if (k.fixed_mask) {    // fixed types (kernels2d.cu:281-290)
-        if ((k.fixed_mask >> (vt >> 24)) & 1u) return;  // a fixed target stays
-        if ((k.fixed_mask >> (vs >> 24)) & 1u) {        // a fixed source copies medium
+        if ((k.fixed_mask >> (vt >> k.idbits)) & 1u) return;  // a fixed target stays
+        if ((k.fixed_mask >> (vs >> k.idbits)) & 1u) {        // a fixed source copies medium
             vs = 0;
             if (vs == vt) return;
         }
     }
 
-    const int ts = (int)(vs >> 24), tt = (int)(vt >> 24);
-    const uint32_t cs = vs & kSpinMask, ct = vt & kSpinMask;
+    const int ts = (int)(vs >> k.idbits), tt = (int)(vt >> k.idbits);
+    const uint32_t cs = vs & k.idmask, ct = vt & k.idmask;
     // Issue the snapshot-table loads first: their latency overlaps the
     // neighbourhood sums.
     const int64_t row_s = cs ? ld<CO>(T.R + cs) : 0;
@@ -525,7 +539,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         for (int n = 0; n < NB; n++) {
             if (n == CENTRE) continue;
             const uint32_t vn = d.nv[n];
-            hist += 1u << ((vn >> 24) * 5u);
+            hist += 1u << ((vn >> k.idbits) * 5u);
             as += vn == vs;
             at += vn == vt;
         }
@@ -535,7 +549,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         for (int n = 0; n < NB; n++) {
             if (n == CENTRE) continue;
             const uint32_t vn = d.nv[n];
-            const int tn = (int)(vn >> 24);
+            const int tn = (int)(vn >> k.idbits);
             const int32_t js = P.J(ts, tn), jt = P.JS(tt, tn);
             const bool eqs = vn == vs, eqt = vn == vt;
             adh += (eqs ? 0 : js) - (eqt ? 0 : jt);
@@ -685,7 +699,7 @@ __device__ __forceinline__ void colour_of(int col, int& cx, int& cy, int& cz) {
 // so no attempt reads a site another thread writes in this phase.  The
 // first ncells + 1 threads also fold the previous phase's deltas (their
 // loads are issued before the attempts, the adds made after them).
-template <int D, class PV>
+template <int D, class ST, class PV>
 __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P,
                                                  const uint32_t* __restrict__ thr) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -699,7 +713,7 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
         const int bx = ((gtid & hm) << a.lstride) + a.cx;
         const int by = (((gtid >> a.lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * a.lh)) << a.lstride) + a.cz : 0;
-        const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+        const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
         sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
                                      CellSync{a.sync_pass, a.claims, a.sync_tag}, a.att0, a.natt);
     }
@@ -720,11 +734,11 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
 
 // At most four types: parameters from uniform (scalar) loads, thresholds
 // read from global memory; no staging and no block barrier.
-template <int D>
+template <int D, class ST>
 __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
     SmallParams P;
     P.load(a.k.prm);
-    sweep_phase_body<D>(a, P, a.k.thr);
+    sweep_phase_body<D, ST>(a, P, a.k.thr);
 }
 
 // One attempt per sub-block (u = 1), with the attempts compacted: every
@@ -760,7 +774,7 @@ struct CompactPhase {
 // Thread t of block b draws the attempts of same-colour sub-blocks
 // (b ATT + r) blockDim + t for r < ATT (consecutive threads, consecutive
 // sub-blocks), so the block's queue collects up to ATT blockDim attempts.
-template <int D, bool STAGE, bool FULL, int ATT>
+template <int D, bool STAGE, bool FULL, int ATT, class ST>
 __device__ __forceinline__ void compact_phase(const SweepConsts& k, const CompactPhase& a) {
     static_assert(!FULL || ATT == 1, "the full-neighbourhood queue holds one attempt per thread");
     __shared__ LdsParams sp;
@@ -789,8 +803,8 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
     }
     const bool fold = a.sync_pass != 1;
     const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
-    using Lat = GlobalLat<D, false>;
-    const Lat lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+    using Lat = GlobalLat<D, false, ST>;
+    const Lat lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
     // Draws (kernels2d.cu:199-243 in the shared roles).
     bool active[ATT];
     QueuedAttempt<D, FULL> e[ATT];
@@ -917,18 +931,18 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
 }
 
 
-template <int D, bool STAGE, bool FULL, int ATT>
+template <int D, bool STAGE, bool FULL, int ATT, class ST>
 __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
     const CompactPhase cp{a.nsub, a.lh, a.lstride, a.cx, a.cy, a.cz, a.g, a.phase, a.mcs,
                           a.sync_pass, a.claims, a.sync_tag, (int)blockIdx.x,
                           (int)(blockIdx.x * blockDim.x + threadIdx.x),
                           (int)(gridDim.x * blockDim.x)};
-    compact_phase<D, STAGE, FULL, ATT>(a.k, cp);
+    compact_phase<D, STAGE, FULL, ATT, ST>(a.k, cp);
 }
 
 // Batched replicas, one colour phase, compacted: block b serves replica
 // rep_of[b] as block b - block0 of its group.
-template <int D, bool STAGE, int ATT>
+template <int D, bool STAGE, int ATT, class ST>
 __global__ __launch_bounds__(kBlock) void k_batch_phase_compact(BatchPhaseArgs a) {
     const int r = a.rep_of[blockIdx.x];
     const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
@@ -936,21 +950,21 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase_compact(BatchPhaseArgs a
     const CompactPhase cp{1 << (D * rd.lh), rd.lh, a.lstride, a.cx, a.cy, a.cz, a.g, a.phase,
                           rd.mcs0 + a.m, a.sync_pass, rd.claims, a.sync_tag, lb,
                           lb * (int)blockDim.x + (int)threadIdx.x, rd.nblocks * (int)blockDim.x};
-    compact_phase<D, STAGE, false, ATT>(rd.k, cp);
+    compact_phase<D, STAGE, false, ATT, ST>(rd.k, cp);
 }
 
 // Any number of types: parameters and thresholds staged in LDS.
-template <int D>
+template <int D, class ST>
 __global__ __launch_bounds__(kBlock) void k_sweep_phase_lds(PhaseArgs a) {
     __shared__ LdsParams sp;
     LdsParamView P;
     const uint32_t* thr = stage_params(sp, a.k, &P);
-    sweep_phase_body<D>(a, P, thr);
+    sweep_phase_body<D, ST>(a, P, thr);
 }
 
 // Batched replicas, one colour phase: block b works for replica rep_of[b]
 // on its same-colour sub-block (b - block0) * blockDim + t.
-template <int D>
+template <int D, class ST>
 __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
     const int r = a.rep_of[blockIdx.x];
     const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
@@ -967,7 +981,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
         const int bx = ((gtid & hm) << a.lstride) + a.cx;
         const int by = (((gtid >> lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
-        const GlobalLat<D, false> lat{k.lat, k.lgL, (1 << k.lgL) - 1};
+        const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
         sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T,
                                      CellSync{a.sync_pass, rd.claims, a.sync_tag}, a.att0, a.natt);
     }
@@ -1245,6 +1259,41 @@ __global__ __launch_bounds__(256) void k_tally_csum(const uint32_t* __restrict__
     }
 }
 
+// 32-bit <-> 16-bit lattice (cell | type << 24 <-> cell | type << idbits),
+// four sites per thread.
+__global__ __launch_bounds__(256) void k_to_narrow(const uint4* __restrict__ w, uint2* n, uint32_t nq,
+                                                   uint32_t idbits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint4 v = w[i];
+    auto f = [idbits](uint32_t x) { return (x & kSpinMask) | ((x >> 24) << idbits); };
+    n[i] = make_uint2(f(v.x) | (f(v.y) << 16), f(v.z) | (f(v.w) << 16));
+}
+
+__global__ __launch_bounds__(256) void k_to_wide(const uint2* __restrict__ n, uint4* w, uint32_t nq,
+                                                 uint32_t idbits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint2 v = n[i];
+    const uint32_t m = (1u << idbits) - 1u;
+    auto f = [idbits, m](uint32_t x) { return (x & m) | ((x >> idbits) << 24); };
+    w[i] = make_uint4(f(v.x & 0xFFFFu), f(v.x >> 16), f(v.y & 0xFFFFu), f(v.y >> 16));
+}
+
+hipError_t launch_convert(uint32_t* wide, uint16_t* narrow, size_t sites, int idbits, bool to_narrow,
+                          hipStream_t s) {
+    if (sites % 4 != 0 || sites / 4 > 0xFFFFFFFFu) return hipErrorInvalidValue;
+    const uint32_t nq = (uint32_t)(sites / 4);
+    const uint32_t blocks = (nq + 255) / 256;
+    if (to_narrow)
+        hipLaunchKernelGGL(k_to_narrow, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const uint4*>(wide),
+                           reinterpret_cast<uint2*>(narrow), nq, (uint32_t)idbits);
+    else
+        hipLaunchKernelGGL(k_to_wide, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const uint2*>(narrow),
+                           reinterpret_cast<uint4*>(wide), nq, (uint32_t)idbits);
+    return hipGetLastError();
+}
+
 // Settle after G phases: fold the last phase's deltas into the lagging
 // base, leaving both bases current and both delta buffers zero.
 __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
@@ -1337,22 +1386,38 @@ enum PhaseKernel {
     kPkCompact2Lds2, kPkCompact2Lds4, kPkCompact3Small2, kPkCompact3Small4, kPkCount
 };
 
+// Kernels on the 16-bit lattice have ids kPkNarrow + id.
+constexpr int kPkNarrow = 64;
+
 const char* phase_kernel_name(int id) {
-    static const char* const names[] = {
-        "k_sweep_phase3_box",
-        "k_sweep_phase_compact<2, false, false, 1>", "k_sweep_phase_compact<2, false, true, 1>",
-        "k_sweep_phase_compact<2, true, false, 1>", "k_sweep_phase_compact<2, true, true, 1>",
-        "k_sweep_phase_compact<3, false, false, 1>", "k_sweep_phase_compact<3, true, false, 1>",
-        "k_sweep_phase<2>", "k_sweep_phase_lds<2>", "k_sweep_phase<3>", "k_sweep_phase_lds<3>",
-        "k_sweep_phase_compact<2, true, false, 2>", "k_sweep_phase_compact<2, true, false, 4>",
-        "k_sweep_phase_compact<3, false, false, 2>", "k_sweep_phase_compact<3, false, false, 4>"};
-    return id >= 0 && id < kPkCount ? names[id] : "";
+#define CPM_NAMES(ST)                                                                              \
+    {"k_sweep_phase3_box",                                                                         \
+     "k_sweep_phase_compact<2, false, false, 1, " ST ">",                                         \
+     "k_sweep_phase_compact<2, false, true, 1, " ST ">",                                          \
+     "k_sweep_phase_compact<2, true, false, 1, " ST ">",                                          \
+     "k_sweep_phase_compact<2, true, true, 1, " ST ">",                                           \
+     "k_sweep_phase_compact<3, false, false, 1, " ST ">",                                         \
+     "k_sweep_phase_compact<3, true, false, 1, " ST ">",                                          \
+     "k_sweep_phase<2, " ST ">", "k_sweep_phase_lds<2, " ST ">",                                  \
+     "k_sweep_phase<3, " ST ">", "k_sweep_phase_lds<3, " ST ">",                                  \
+     "k_sweep_phase_compact<2, true, false, 2, " ST ">",                                          \
+     "k_sweep_phase_compact<2, true, false, 4, " ST ">",                                          \
+     "k_sweep_phase_compact<3, false, false, 2, " ST ">",                                         \
+     "k_sweep_phase_compact<3, false, false, 4, " ST ">"}
+    static const char* const wide[kPkCount] = CPM_NAMES("unsigned int");
+    static const char* const narrow[kPkCount] = CPM_NAMES("unsigned short");
+#undef CPM_NAMES
+    if (id >= kPkNarrow) {
+        id -= kPkNarrow;
+        return id > kPkBox3 && id < kPkCount ? narrow[id] : "";
+    }
+    return id >= 0 && id < kPkCount ? wide[id] : "";
 }
 
-static int choose_phase_kernel(int d, const PhaseArgs& a) {
+static int choose_phase_kernel(int d, const PhaseArgs& a, bool box_ok) {
     const int L = 1 << a.k.lgL;
-    if (d == 3 && a.k.lc == 0 && !a.k.roles && a.lstride == 1 && L >= kBX && L >= kBY && L >= kBZ &&
-        a.box3 && !a.sync_pass) {
+    if (box_ok && d == 3 && a.k.lc == 0 && !a.k.roles && a.lstride == 1 && L >= kBX && L >= kBY &&
+        L >= kBZ && a.box3 && !a.sync_pass) {
         const long nbox = (long)(L / kBX) * (L / kBY) * (L / kBZ);
         if (nbox * kBlock >= a.k.ncells + 1) return kPkBox3;
     }
@@ -1371,13 +1436,10 @@ static int choose_phase_kernel(int d, const PhaseArgs& a) {
     return small ? kPkPhase3Small : kPkPhase3Lds;
 }
 
-hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                        int* kernel_id) {
+template <class ST>
+static void launch_phase_st(int id, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const int work = a.nsub > a.k.ncells + 1 ? a.nsub : a.k.ncells + 1;
     const int blocks = (work + kBlock - 1) / kBlock;
-    const int L = 1 << a.k.lgL;
-    const int id = choose_phase_kernel(d, a);
-    if (kernel_id) *kernel_id = id;
     // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
     // (null events: plain launch).
 #define CPM_LAUNCH(KERNEL, GRID) \
@@ -1389,23 +1451,38 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
         return nb > 0 ? nb : 1;
     };
     switch (id) {
-        case kPkBox3: CPM_LAUNCH(k_sweep_phase3_box, (L / kBX) * (L / kBY) * (L / kBZ)); break;
-        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false, 1>), blocks); break;
-        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true, 1>), blocks); break;
-        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 1>), blocks); break;
-        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true, 1>), blocks); break;
-        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 1>), blocks); break;
-        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false, 1>), blocks); break;
-        case kPkCompact2Lds2: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 2>), cblocks(2)); break;
-        case kPkCompact2Lds4: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 4>), cblocks(4)); break;
-        case kPkCompact3Small2: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 2>), cblocks(2)); break;
-        case kPkCompact3Small4: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4>), cblocks(4)); break;
-        case kPkPhase2Small: CPM_LAUNCH(k_sweep_phase<2>, blocks); break;
-        case kPkPhase2Lds: CPM_LAUNCH(k_sweep_phase_lds<2>, blocks); break;
-        case kPkPhase3Small: CPM_LAUNCH(k_sweep_phase<3>, blocks); break;
-        default: CPM_LAUNCH(k_sweep_phase_lds<3>, blocks); break;
+        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false, 1, ST>), blocks); break;
+        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true, 1, ST>), blocks); break;
+        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 1, ST>), blocks); break;
+        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true, 1, ST>), blocks); break;
+        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 1, ST>), blocks); break;
+        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false, 1, ST>), blocks); break;
+        case kPkCompact2Lds2: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 2, ST>), cblocks(2)); break;
+        case kPkCompact2Lds4: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 4, ST>), cblocks(4)); break;
+        case kPkCompact3Small2: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 2, ST>), cblocks(2)); break;
+        case kPkCompact3Small4: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4, ST>), cblocks(4)); break;
+        case kPkPhase2Small: CPM_LAUNCH((k_sweep_phase<2, ST>), blocks); break;
+        case kPkPhase2Lds: CPM_LAUNCH((k_sweep_phase_lds<2, ST>), blocks); break;
+        case kPkPhase3Small: CPM_LAUNCH((k_sweep_phase<3, ST>), blocks); break;
+        default: CPM_LAUNCH((k_sweep_phase_lds<3, ST>), blocks); break;
     }
 #undef CPM_LAUNCH
+}
+
+hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        int* kernel_id) {
+    const int L = 1 << a.k.lgL;
+    const bool narrow = a.k.lat16 != nullptr;
+    // the LDS box form reads the 32-bit lattice only
+    const int id = choose_phase_kernel(d, a, !narrow);
+    if (kernel_id) *kernel_id = id + (narrow ? kPkNarrow : 0);
+    if (id == kPkBox3)
+        hipExtLaunchKernelGGL(k_sweep_phase3_box, dim3((L / kBX) * (L / kBY) * (L / kBZ)), dim3(kBlock),
+                              0, s, e0, e1, 0, a);
+    else if (narrow)
+        launch_phase_st<uint16_t>(id, a, s, e0, e1);
+    else
+        launch_phase_st<uint32_t>(id, a, s, e0, e1);
     return hipGetLastError();
 }
 
@@ -1431,12 +1508,13 @@ hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s) {
     return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(kBlock), args, 0, s);
 }
 
-hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
+template <class T>
+static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
     const bool compact = a.form & kFormCompact;
     const bool small = a.form & kFormSmall;
     const int att = (a.form & kFormAtt4) ? 4 : (a.form & kFormAtt2) ? 2 : 1;
 #define CPM_BATCH(DD, ST, AT) \
-    hipLaunchKernelGGL((k_batch_phase_compact<DD, ST, AT>), dim3(blocks), dim3(kBlock), 0, s, a)
+    hipLaunchKernelGGL((k_batch_phase_compact<DD, ST, AT, T>), dim3(blocks), dim3(kBlock), 0, s, a)
     if (compact) {
         if (d == 2) {
             if (small) {
@@ -1451,13 +1529,20 @@ hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStr
                 if (att == 4) CPM_BATCH(3, true, 4); else if (att == 2) CPM_BATCH(3, true, 2); else CPM_BATCH(3, true, 1);
             }
         }
-        return hipGetLastError();
+        return;
     }
 #undef CPM_BATCH
     if (d == 2)
-        hipLaunchKernelGGL(k_batch_phase<2>, dim3(blocks), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_batch_phase<2, T>), dim3(blocks), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL(k_batch_phase<3>, dim3(blocks), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_batch_phase<3, T>), dim3(blocks), dim3(kBlock), 0, s, a);
+}
+
+hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
+    if (a.narrow)
+        launch_batch_phase_st<uint16_t>(d, a, blocks, s);
+    else
+        launch_batch_phase_st<uint32_t>(d, a, blocks, s);
     return hipGetLastError();
 }
 

@@ -142,3 +142,31 @@ def test_batch_persistence_equals_single_runs():
         np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
         assert_tables_equal(s.get_cell_table(), tab_o, 3)
         np.testing.assert_array_equal(s.get_pref_dirs().reshape(-1), tab_o.pref)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_batch_site_width_mixed(form):
+    """Single runs on 16-bit sites, then a batch in which one replica's cells
+    do not fit in 16 bits (so every replica runs on 32-bit sites), or the
+    persistent batch form (32-bit sites): each replica continues its stream
+    exactly."""
+    import gpucpm
+    from gpucpm import workloads as W
+    wls = _small_batch()[:2] + [W.sorting2d(L=256, grid=128, edge=2, seed=41, ncells=16384)]
+    print("CASE batch-width", form, [w.L for w in wls], flush=True)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(20 + i)
+        s.run(**run_kwargs(1, 1, 3))
+        sims.append(s)
+    batch = gpucpm.Batch(sims)
+    batch.set_form(form)
+    batch.run(**run_kwargs(1, 1, 4))
+    batch.synchronize()
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.pull_from_gpu()
+        lat_o, tab_o = run_oracle(wl, 1, 1, 20 + i, 7)
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 2)

@@ -224,11 +224,14 @@ def test_fixed_wall_bit_exact(mode, d, c, u, monkeypatch):
     ("0", "0", "0", "1"), ("1", "0", "0", "1"), ("1", "1", "0", "1"), ("1", "1", "1", "1"),
     ("0", "1", "0", "1"), ("0", "1", "1", "1"), ("0", "1", "0", "2"), ("0", "1", "0", "4"),
     ("1", "1", "0", "2"), ("1", "1", "0", "4")])
-def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, monkeypatch):
+@pytest.mark.parametrize("narrow", ["1", "0"])
+def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, narrow, monkeypatch):
     """The per-phase kernel forms (parameters staged in LDS or read
-    directly; attempts finished in place or queued and finished densely)
-    reproduce the oracle in 2D and 3D, with and without a fixed wall."""
+    directly; attempts finished in place or queued and finished densely;
+    16-bit or 32-bit sites) reproduce the oracle in 2D and 3D, with and
+    without a fixed wall."""
     _, W = _gpu()
+    monkeypatch.setenv("GPUCPM_NARROW", narrow)
     monkeypatch.setenv("GPUCPM_SMALL", small)
     monkeypatch.setenv("GPUCPM_COMPACT", compact)
     monkeypatch.setenv("GPUCPM_FULL", full)
@@ -239,7 +242,7 @@ def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, monkeyp
              (W.plain3d(L=32, ncells=60, seed=64), 1, 1, 8),
              (W.walled(W.plain3d(L=32, ncells=60, seed=65)), 1, 1, 8)]
     for wl, c, u, n in cases:
-        _announce("forms", small, compact, full, att, wl.name, wl.L, c, u)
+        _announce("forms", small, compact, full, att, narrow, wl.name, wl.L, c, u)
         _, lat, table = _run_gpu(wl, c, u, seed=71, nmcs=n)
         lat_o, tab_o = run_oracle(wl, c, u, 71, n)
         np.testing.assert_array_equal(lat, lat_o)
