@@ -132,6 +132,13 @@ constexpr int kFormCompact = 2;  // u = 1: queue the attempts whose source and t
 constexpr int kFormFull = 4;     // compact: the draw loads (and queues) the whole neighbourhood
 constexpr int kFormAtt2 = 8;     // compact: each thread draws 2 attempts
 constexpr int kFormAtt4 = 16;    // compact: each thread draws 4 attempts
+constexpr int kFormAtt8 = 32;    // compact: each thread draws 8 attempts
+__host__ __device__ inline int form_att(int form) {
+    return (form & kFormAtt8) ? 8 : (form & kFormAtt4) ? 4 : (form & kFormAtt2) ? 2 : 1;
+}
+__host__ __device__ inline int att_form(int att) {
+    return att == 8 ? kFormAtt8 : att == 4 ? kFormAtt4 : att == 2 ? kFormAtt2 : 0;
+}
 
 // A whole run in one cooperative launch, phases separated by grid barriers.
 struct PersistArgs {

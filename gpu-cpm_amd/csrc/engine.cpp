@@ -1042,10 +1042,10 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                   (env_flag("GPUCPM_FULL", false) ? gcpm::kFormFull : 0);
         {
             // attempts drawn per thread by the compacted kernel (measured:
-            // 4 in 3D, 1 in 2D)
+            // 8 in 3D, 1 in 2D)
             const char* e = std::getenv("GPUCPM_ATT");
-            const int att = e ? std::atoi(e) : (h->d == 3 ? 4 : 1);
-            pa.form |= att == 4 ? gcpm::kFormAtt4 : att == 2 ? gcpm::kFormAtt2 : 0;
+            const int att = e ? std::atoi(e) : (h->d == 3 ? 8 : 1);
+            pa.form |= gcpm::att_form(att);
         }
         pa.lstride = k.lstride;
         int g = 0;  // phase counter since the tables were last settled
@@ -1312,7 +1312,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     for (int i = 0; i < n; i++) small = small && reps[i].k.nt <= gcpm::kSmallTypes;
     const int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 4 : 1) : 1;
     const int form = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
-                     (att == 4 ? gcpm::kFormAtt4 : att == 2 ? gcpm::kFormAtt2 : 0);
+                     gcpm::att_form(std::min(att, 4));
     // 16-bit lattices when every replica's sites fit, else all 32-bit.
     bool narrow = true;
     for (int i = 0; i < n; i++) narrow = narrow && b->sims[i]->narrow_bits() != 0;
@@ -1329,7 +1329,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
             h->narrow_valid = false;
         }
     }
-    const long per_block = (long)gcpm::kBlock * (compact ? att : 1);
+    const long per_block = (long)gcpm::kBlock * (compact ? std::min(att, 4) : 1);
     int nblocks = 0;
     for (int i = 0; i < n; i++) {
         const long nsub = 1L << (reps[i].lh * b->d);

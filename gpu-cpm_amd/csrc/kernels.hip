@@ -748,15 +748,39 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase(PhaseArgs a) {
 // then finish the queued attempts densely.  Parameters and thresholds are
 // staged in LDS while the draws' loads are in flight.  The result does not
 // depend on which thread finishes an attempt.
-template <int D, bool FULL>
-struct QueuedAttempt {
+template <int N>
+struct QueueExtra {
+    uint32_t x[N];
+};
+template <>
+struct QueueExtra<0> {};
+
+// 16 bytes per entry with 16-bit sites: the target value rides in aux
+// beside the source index; 32-bit sites carry it in x[0]; FULL entries carry
+// the whole 3^D block in x.  (The sub-block id is recomputed when needed.)
+template <int D, bool FULL, class ST>
+struct QueuedAttempt : QueueExtra<FULL ? (D == 2 ? 9 : 27) : (sizeof(ST) == 2 ? 0 : 1)> {
     static constexpr int NQ = FULL ? (D == 2 ? 9 : 27) : 0;  // neighbourhood carried
+    static constexpr bool PACK = !FULL && sizeof(ST) == 2;
     uint32_t site;  // target site index
     uint32_t vs;    // source value
     uint32_t w2;    // acceptance draw
-    uint32_t sb;    // sub-block id
-    uint32_t dir;   // source index in the 3^D block
-    uint32_t nv[NQ > 0 ? NQ : 1];
+    uint32_t aux;   // source index in the 3^D block (| target value << 8 when PACK)
+    __device__ __forceinline__ void set_vt(uint32_t vt) {
+        if constexpr (PACK)
+            aux |= vt << 8;
+        else if constexpr (!FULL)
+            this->x[0] = vt;
+    }
+    __device__ __forceinline__ int dir() const { return PACK ? (int)(aux & 0xFFu) : (int)aux; }
+    __device__ __forceinline__ uint32_t vt() const {
+        if constexpr (PACK)
+            return aux >> 8;
+        else if constexpr (!FULL)
+            return this->x[0];
+        else
+            return this->x[NQ / 2];
+    }
 };
 
 // One colour phase of one lattice as seen by a block group (the whole grid,
@@ -778,9 +802,9 @@ template <int D, bool STAGE, bool FULL, int ATT, class ST>
 __device__ __forceinline__ void compact_phase(const SweepConsts& k, const CompactPhase& a) {
     static_assert(!FULL || ATT == 1, "the full-neighbourhood queue holds one attempt per thread");
     __shared__ LdsParams sp;
-    __shared__ QueuedAttempt<D, FULL> queue[kBlock * ATT];
+    __shared__ QueuedAttempt<D, FULL, ST> queue[kBlock * ATT];
     __shared__ uint32_t nqueued;
-    constexpr int NQ = QueuedAttempt<D, FULL>::NQ;
+    constexpr int NQ = QueuedAttempt<D, FULL, ST>::NQ;
     const int gtid = a.gtid;
     const int gsz = a.gsz;
     const PhaseTables T = tables_of(k, a.g);
@@ -807,7 +831,7 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
     const Lat lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
     // Draws (kernels2d.cu:199-243 in the shared roles).
     bool active[ATT];
-    QueuedAttempt<D, FULL> e[ATT];
+    QueuedAttempt<D, FULL, ST> e[ATT];
 #pragma unroll
     for (int r = 0; r < ATT; r++) {
         const int q = (a.lblock * ATT + r) * kBlock + (int)threadIdx.x;
@@ -840,11 +864,11 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
             const typename Lat::Pos t = lat.at(tx, ty, tz);
             uint32_t vs, vt;
             if constexpr (NQ > 0) {
-                lat.block(t, e[r].nv);
+                lat.block(t, e[r].x);
                 vs = 0;
 #pragma unroll
-                for (int n = 0; n < NQ; n++) vs = n == dir ? e[r].nv[n] : vs;
-                vt = e[r].nv[NQ / 2];
+                for (int n = 0; n < NQ; n++) vs = n == dir ? e[r].x[n] : vs;
+                vt = e[r].x[NQ / 2];
             } else {
                 int sx, sy, sz;
                 block_offset<D>(dir, sx, sy, sz);
@@ -855,8 +879,8 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
             e[r].site = t.i;
             e[r].vs = vs;
             e[r].w2 = w[2];
-            e[r].sb = sb;
-            e[r].dir = (uint32_t)dir;
+            e[r].aux = (uint32_t)dir;
+            e[r].set_vt(vt);
         }
     }
 #pragma unroll
@@ -888,7 +912,7 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
     __syncthreads();
     const int nq = (int)nqueued;
     for (int j = threadIdx.x; j < nq; j += kBlock) {
-        const QueuedAttempt<D, FULL>& q = queue[j];
+        const QueuedAttempt<D, FULL, ST>& q = queue[j];
         const int lgL = k.lgL, Lm = (1 << lgL) - 1;
         const typename Lat::Pos t{(int)(q.site & Lm), (int)((q.site >> lgL) & Lm),
                                   D == 3 ? (int)(q.site >> (2 * lgL)) : 0, q.site};
@@ -898,21 +922,34 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
         d.t = t;
         if constexpr (NQ > 0) {
 #pragma unroll
-            for (int n = 0; n < NQ; n++) d.nv[n] = q.nv[n];
+            for (int n = 0; n < NQ; n++) d.nv[n] = q.x[n];
         } else {
+            // The target holds the value the draw read (no other attempt of
+            // the phase writes it): taking it from the queue keeps the
+            // early exit on vs == vt from waiting for a reload.
             lat.block(t, d.nv);
+            d.nv[(D == 3 ? 27 : 9) / 2] = q.vt();
         }
         d.vs = q.vs;
-        d.dir = (int)q.dir;
+        d.dir = q.dir();
         d.tick = a.mcs;
+        // The sub-block (cell_sync's key) from the drawn site: the target,
+        // or with non-shared roles the source t + offset(dir).
+        uint32_t sb = 0;
+        if (a.sync_pass) {
+            int ox = 0, oy = 0, oz = 0;
+            if (k.roles) block_offset<D>(d.dir, ox, oy, oz);
+            sb = sub_block_id<D>(k, ((t.x + ox) & Lm) >> k.lc, ((t.y + oy) & Lm) >> k.lc,
+                                 D == 3 ? ((t.z + oz) & Lm) >> k.lc : 0);
+        }
         const CellSync cy{a.sync_pass, a.claims, a.sync_tag};
         if constexpr (STAGE) {
             const LdsParamView P{sp.p, k.nt, k.nt * k.nt};
-            finish_attempt<D, false>(lat, k, P, thr_lds ? sp.thr : k.thr, T, cy, q.sb, d);
+            finish_attempt<D, false>(lat, k, P, thr_lds ? sp.thr : k.thr, T, cy, sb, d);
         } else {
             SmallParams P;
             P.load(k.prm);
-            finish_attempt<D, false>(lat, k, P, k.thr, T, cy, q.sb, d);
+            finish_attempt<D, false>(lat, k, P, k.thr, T, cy, sb, d);
         }
     }
     if (fold) {
@@ -1383,7 +1420,8 @@ hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab,
 enum PhaseKernel {
     kPkBox3, kPkCompact2Small, kPkCompact2SmallFull, kPkCompact2Lds, kPkCompact2LdsFull,
     kPkCompact3Small, kPkCompact3Lds, kPkPhase2Small, kPkPhase2Lds, kPkPhase3Small, kPkPhase3Lds,
-    kPkCompact2Lds2, kPkCompact2Lds4, kPkCompact3Small2, kPkCompact3Small4, kPkCount
+    kPkCompact2Lds2, kPkCompact2Lds4, kPkCompact3Small2, kPkCompact3Small4, kPkCompact3Small8,
+    kPkCompact2Lds8, kPkCount
 };
 
 // Kernels on the 16-bit lattice have ids kPkNarrow + id.
@@ -1403,7 +1441,9 @@ const char* phase_kernel_name(int id) {
      "k_sweep_phase_compact<2, true, false, 2, " ST ">",                                          \
      "k_sweep_phase_compact<2, true, false, 4, " ST ">",                                          \
      "k_sweep_phase_compact<3, false, false, 2, " ST ">",                                         \
-     "k_sweep_phase_compact<3, false, false, 4, " ST ">"}
+     "k_sweep_phase_compact<3, false, false, 4, " ST ">",                                         \
+     "k_sweep_phase_compact<3, false, false, 8, " ST ">",                                         \
+     "k_sweep_phase_compact<2, true, false, 8, " ST ">"}
     static const char* const wide[kPkCount] = CPM_NAMES("unsigned int");
     static const char* const narrow[kPkCount] = CPM_NAMES("unsigned short");
 #undef CPM_NAMES
@@ -1424,10 +1464,12 @@ static int choose_phase_kernel(int d, const PhaseArgs& a, bool box_ok) {
     const bool small = a.k.nt <= kSmallTypes && (a.form & kFormSmall);
     const bool compact = a.k.u == 1 && (a.form & kFormCompact);
     const bool full = a.form & kFormFull;
-    const int att = (a.form & kFormAtt4) ? 4 : (a.form & kFormAtt2) ? 2 : 1;
+    const int att = form_att(a.form);
     if (compact) {
-        if (d == 2 && !small && !full && att > 1) return att == 4 ? kPkCompact2Lds4 : kPkCompact2Lds2;
-        if (d == 3 && small && att > 1) return att == 4 ? kPkCompact3Small4 : kPkCompact3Small2;
+        if (d == 2 && !small && !full && att > 1)
+            return att == 8 ? kPkCompact2Lds8 : att == 4 ? kPkCompact2Lds4 : kPkCompact2Lds2;
+        if (d == 3 && small && att > 1)
+            return att == 8 ? kPkCompact3Small8 : att == 4 ? kPkCompact3Small4 : kPkCompact3Small2;
         if (d == 2) return small ? (full ? kPkCompact2SmallFull : kPkCompact2Small)
                                  : (full ? kPkCompact2LdsFull : kPkCompact2Lds);
         return small ? kPkCompact3Small : kPkCompact3Lds;
@@ -1461,6 +1503,8 @@ static void launch_phase_st(int id, const PhaseArgs& a, hipStream_t s, hipEvent_
         case kPkCompact2Lds4: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 4, ST>), cblocks(4)); break;
         case kPkCompact3Small2: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 2, ST>), cblocks(2)); break;
         case kPkCompact3Small4: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4, ST>), cblocks(4)); break;
+        case kPkCompact3Small8: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 8, ST>), cblocks(8)); break;
+        case kPkCompact2Lds8: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 8, ST>), cblocks(8)); break;
         case kPkPhase2Small: CPM_LAUNCH((k_sweep_phase<2, ST>), blocks); break;
         case kPkPhase2Lds: CPM_LAUNCH((k_sweep_phase_lds<2, ST>), blocks); break;
         case kPkPhase3Small: CPM_LAUNCH((k_sweep_phase<3, ST>), blocks); break;
@@ -1512,7 +1556,7 @@ template <class T>
 static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
     const bool compact = a.form & kFormCompact;
     const bool small = a.form & kFormSmall;
-    const int att = (a.form & kFormAtt4) ? 4 : (a.form & kFormAtt2) ? 2 : 1;
+    const int att = std::min(form_att(a.form), 4);
 #define CPM_BATCH(DD, ST, AT) \
     hipLaunchKernelGGL((k_batch_phase_compact<DD, ST, AT, T>), dim3(blocks), dim3(kBlock), 0, s, a)
     if (compact) {

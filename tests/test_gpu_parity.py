@@ -223,7 +223,7 @@ def test_fixed_wall_bit_exact(mode, d, c, u, monkeypatch):
 @pytest.mark.parametrize("small,compact,full,att", [
     ("0", "0", "0", "1"), ("1", "0", "0", "1"), ("1", "1", "0", "1"), ("1", "1", "1", "1"),
     ("0", "1", "0", "1"), ("0", "1", "1", "1"), ("0", "1", "0", "2"), ("0", "1", "0", "4"),
-    ("1", "1", "0", "2"), ("1", "1", "0", "4")])
+    ("1", "1", "0", "2"), ("1", "1", "0", "4"), ("0", "1", "0", "8"), ("1", "1", "0", "8")])
 @pytest.mark.parametrize("narrow", ["1", "0"])
 def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, narrow, monkeypatch):
     """The per-phase kernel forms (parameters staged in LDS or read
