@@ -104,6 +104,12 @@ struct SweepConsts {
     uint32_t* act_lat;      // act lattice [L^d]: MCS of the last copy into each site
     const int64_t* act_tab; // 2^(j/256) 2^30 [256], then llround(log2(v) 2^24) [v]
     int64_t act_ln2q;       // llround(ln 2 2^30)
+    // Reference stream (cpm_set_rng): one MINSTD state per region of K
+    // sub-blocks per axis (a reference thread), rstate[(rz << 2 lr) | (ry <<
+    // lr) | rx]; thr then holds the 31-bit acceptance table.
+    int refstream;
+    int32_t* rstate;
+    int lk, lr;             // log2 K, log2 (regions per axis)
 };
 
 // One colour phase per launch.
@@ -206,6 +212,9 @@ hipError_t launch_settle(int64_t* const tab[2], int64_t* const del[2], int G, in
 // Centroid sums of the current lattice about the push-time anchors
 // (csum must be zero on entry).
 // Copy the 32-bit lattice into its 16-bit form (to_narrow) or back; sites % 4 == 0.
+// Seed every region's reference stream for one reference launch:
+// rstate[r] = MINSTD(index of the region's first site + seed32).
+hipError_t launch_ref_seed(int d, const SweepConsts& k, uint32_t seed32, hipStream_t s);
 hipError_t launch_convert(uint32_t* wide, uint16_t* narrow, size_t sites, int idbits, bool to_narrow,
                           hipStream_t s);
 hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* anchor, int64_t* csum,

@@ -143,6 +143,11 @@ struct cpm_engine {
     // state (at least one does).
     uint16_t* d_lat16 = nullptr;
     int lat16_bits = 0;
+    // Random stream: 0 the counter stream (Philox), 1 the reference's
+    // per-thread MINSTD stream (cpm_set_rng), whose states live in d_rstate.
+    int rng = 0;
+    int32_t* d_rstate = nullptr;
+    size_t rstate_cap = 0;
     bool wide_valid = true;
     bool narrow_valid = false;
     int64_t* d_tab[2] = {nullptr, nullptr};
@@ -192,6 +197,7 @@ struct cpm_engine {
         if (stream) (void)hipStreamSynchronize(stream);
         (void)hipFree(d_lat);
         (void)hipFree(d_lat16);
+        (void)hipFree(d_rstate);
         for (int i = 0; i < 2; i++) (void)hipFree(d_tab[i]);
         for (int i = 0; i < 2; i++) (void)hipFree(d_del[i]);
         (void)hipFree(d_anchor);
@@ -495,15 +501,25 @@ struct cpm_engine {
                 for (int f = 0; f < 4; f++) p.s4[2 * S2 + f * S + a] = p.w[2 * n2 + f * nt + a];
             }
         }
-        // Acceptance thresholds thr[k] = floor(exp(-k / (scale T)) 2^32).
+        // Acceptance thresholds thr[k] = floor(exp(-k / (scale T)) 2^32) for
+        // a 32-bit draw; with the reference stream the draw u is MINSTD's and
+        // the reference's r = u / (2^31 - 1) < exp(-E / T) is, exactly,
+        // u < thr[k] = ceil(exp(-k / (scale T)) (2^31 - 1)) (kept while > 1).
         std::vector<uint32_t> thr(1, 0xFFFFFFFFu);
         if (st.temperature > 0.0) {
             for (int k = 1;; k++) {
                 if (k >= kMaxThr)
                     return fail(CPM_EINVAL, "temperature * energy scale too large for the acceptance table");
-                const double v = std::floor(std::exp(-(double)k / ((double)sc * st.temperature)) * 4294967296.0);
-                if (v <= 0.0) break;
-                thr.push_back((uint32_t)v);
+                const double e = std::exp(-(double)k / ((double)sc * st.temperature));
+                if (rng == 1) {
+                    const double v = std::ceil(e * 2147483647.0);
+                    if (v <= 1.0) break;
+                    thr.push_back((uint32_t)v);
+                } else {
+                    const double v = std::floor(e * 4294967296.0);
+                    if (v <= 0.0) break;
+                    thr.push_back((uint32_t)v);
+                }
             }
         }
         if ((int)thr.size() > thr_cap) {
@@ -975,6 +991,10 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     k.S = S;
     k.phases_per_round = 1;
     for (int i = 0; i < h->d; i++) k.phases_per_round *= K * S;
+    k.refstream = 0;
+    k.rstate = nullptr;
+    k.lk = ilog2(K);
+    k.lr = h->lgL - ilog2(R);
     *kout = k;
     *lh_out = h->lgL - k.lc - k.lstride;
     return CPM_OK;
@@ -1004,6 +1024,26 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         plan.mode = Plan::kPhase;  // the persistent forms implement shared roles only
     } else if (int rc = h->plan(c, &plan)) {
         return rc;
+    }
+    if (h->rng == 1) {
+        // The reference stream: one launch per colour phase (each region's
+        // state is carried between phases in d_rstate).
+        if (!k.roles && a->positions_per_thread != 2 * c)
+            return fail(CPM_ENOTIMPL,
+                        "the reference stream with shared = 1 needs positions_per_thread = 2 * "
+                        "positions_per_checkerboard (with more, the reference's threads loop over "
+                        "several regions)");
+        plan = Plan{};
+        plan.mode = Plan::kPhase;
+        const size_t nreg = (size_t)1 << (h->d * k.lr);
+        if (nreg > h->rstate_cap) {
+            (void)hipFree(h->d_rstate);
+            h->d_rstate = nullptr;
+            HIP_TRY(hipMalloc(&h->d_rstate, nreg * sizeof(int32_t)));
+            h->rstate_cap = nreg;
+        }
+        k.refstream = 1;
+        k.rstate = h->d_rstate;
     }
     h->last_plan = plan;
     const int64_t updates_per_mcs = (int64_t)k.rounds * k.phases_per_round * ((int64_t)1 << (lh * h->d)) * u;
@@ -1055,6 +1095,10 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         for (long m = 0; m < total; m++) {
             pa.mcs = (uint32_t)(h->mcs + (uint64_t)m);
             if (bits) h->wide_valid = false;  // (a chunk end may have refreshed it)
+            // The reference seeds its threads at every launch of
+            // inner_iterations MCS, from the same seed (cpm.cu:347-360).
+            if (k.refstream && m % a->inner_iterations == 0)
+                HIP_TRY(gcpm::launch_ref_seed(h->d, k, (uint32_t)h->seed, h->stream));
             for (int r = 0; r < k.rounds; r++) {
                 for (int pc = 0; pc < k.phases_per_round; pc++, g++) {
                     phase_offsets(h->d, k, pc, &pa.cx, &pa.cy, &pa.cz);
@@ -1244,6 +1288,21 @@ int cpm_set_launch(cpm_handle h, int form, int tile) {
     if (tile < 0 || (tile && !is_pow2(tile))) return fail(CPM_EINVAL, "tile must be 0 or a power of two");
     h->want_form = form;
     h->want_tile = tile;
+    return CPM_OK;
+}
+
+int cpm_set_rng(cpm_handle h, int mode) {
+    if (int rc = check_handle(h)) return rc;
+    if (mode != 0 && mode != 1) return fail(CPM_EINVAL, "rng mode must be 0 (counter stream) or 1 (reference stream)");
+    if (mode != h->rng) h->params_dirty = true;  // the acceptance table differs
+    h->rng = mode;
+    return CPM_OK;
+}
+
+int cpm_get_rng(cpm_handle h, int* mode) {
+    if (int rc = check_handle(h)) return rc;
+    if (!mode) return fail(CPM_EINVAL, "null output");
+    *mode = h->rng;
     return CPM_OK;
 }
 
@@ -1468,6 +1527,8 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     std::vector<gcpm::ReplicaDesc> reps(n);
     std::vector<long> need(n);
     for (int i = 0; i < n; i++) {
+        if (b->sims[i]->rng != 0)
+            return fail(CPM_ENOTIMPL, "the reference stream (cpm_set_rng) runs single engines only");
         if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
         if (reps[i].k.K != reps[0].k.K || reps[i].k.S != reps[0].k.S)
             return fail(CPM_EINVAL, "batched replicas need the same colour schedule (K, S)");

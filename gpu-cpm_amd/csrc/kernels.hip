@@ -601,6 +601,88 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     }
 }
 
+// MINSTD (random.cu:5-12): 16807 x mod (2^31 - 1), in double as there.
+__device__ __forceinline__ int32_t minstd_next(int32_t x) {
+    const double temp = (double)x * 16807.0;
+    return (int32_t)(temp - 2147483647.0 * floor(temp * (1.0 / 2147483647.0)));
+}
+
+// At most this many direction triples per attempt (a stream stuck at one
+// value would otherwise never end; a real one needs more than a few with
+// probability 9^-n).
+constexpr int kRefTries = 256;
+
+// The attempts of sub-block (bx, by, bz) drawn from its region's reference
+// stream, in the reference's order (kernels2d.cu:199-243,276-290,522;
+// kernels3d.cu:160-215): d position draws mod c; direction triples
+// (each u mod 3 - 1) until one passes the reference's rejection test (all
+// coordinates equal; in 3D without shared roles it compares z with y,
+// kernels3d.cu:208); then, when the sites differ and the target type is not
+// fixed, the acceptance draw.  A claim pass (cell_sync) leaves the state
+// for its commit pass to redo.
+template <int D, bool CO, class Lat, class PV>
+__device__ __forceinline__ void ref_attempts(const Lat& lat, const SweepConsts& k, const PV& P,
+                                             const uint32_t* __restrict__ thr, int bx, int by,
+                                             int bz, uint32_t mcs, const PhaseTables& T,
+                                             const CellSync& cy, int natt) {
+    constexpr int NB = D == 3 ? 27 : 9;
+    const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
+    if (natt < 0) natt = k.u;
+    const uint32_t rid = (uint32_t)(bx >> k.lk) | ((uint32_t)(by >> k.lk) << k.lr) |
+                         (D == 3 ? (uint32_t)(bz >> k.lk) << (2 * k.lr) : 0u);
+    int32_t st = k.rstate[rid];
+    const int c = 1 << k.lc, Lm = (1 << k.lgL) - 1;
+    for (int i = 0; i < natt; i++) {
+        st = minstd_next(st);
+        const int px = (bx << k.lc) + st % c;
+        st = minstd_next(st);
+        const int py = (by << k.lc) + st % c;
+        int pz = 0;
+        if (D == 3) {
+            st = minstd_next(st);
+            pz = (bz << k.lc) + st % c;
+        }
+        int dx = 0, dy = 0, dz = 0, qx = px, qy = py, qz = pz;
+        for (int tries = 0; tries < kRefTries; tries++) {
+            st = minstd_next(st);
+            dx = st % 3 - 1;
+            st = minstd_next(st);
+            dy = st % 3 - 1;
+            if (D == 3) {
+                st = minstd_next(st);
+                dz = st % 3 - 1;
+            }
+            qx = (px + dx) & Lm;
+            qy = (py + dy) & Lm;
+            qz = D == 3 ? (pz + dz) & Lm : 0;
+            const bool rej = D == 2  ? (qx == px && qy == py)
+                             : k.roles ? (qx == px && qy == py && qz == py)
+                                       : (qx == px && qy == py && qz == pz);
+            if (!rej) break;
+        }
+        // shared roles: the drawn site is the target; otherwise the source
+        const int tx = k.roles ? qx : px, ty = k.roles ? qy : py, tz = k.roles ? qz : pz;
+        const int ox = k.roles ? -dx : dx, oy = k.roles ? -dy : dy, oz = k.roles ? -dz : dz;
+        Draw<D, Lat> d;
+        d.t = lat.at(tx, ty, tz);
+        lat.block(d.t, d.nv);
+        d.dir = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
+        uint32_t vs = 0;
+#pragma unroll
+        for (int n = 0; n < NB; n++) vs = n == d.dir ? d.nv[n] : vs;
+        d.vs = vs;
+        const uint32_t vt = d.nv[NB / 2];
+        d.w[0] = d.w[1] = d.w[2] = d.w[3] = 0u;
+        if (vs != vt && !((k.fixed_mask >> (vt >> k.idbits)) & 1u)) {
+            st = minstd_next(st);
+            d.w[2] = (uint32_t)st;
+        }
+        d.tick = mcs;
+        finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
+    }
+    if (cy.pass != 1) k.rstate[rid] = st;
+}
+
 // The u copy attempts of sub-block (bx, by, bz) in one colour phase, in
 // order.
 template <int D, bool CO, class Lat, class PV>
@@ -699,7 +781,7 @@ __device__ __forceinline__ void colour_of(int col, int& cx, int& cy, int& cz) {
 // so no attempt reads a site another thread writes in this phase.  The
 // first ncells + 1 threads also fold the previous phase's deltas (their
 // loads are issued before the attempts, the adds made after them).
-template <int D, class ST, class PV>
+template <int D, class ST, bool REF = false, class PV>
 __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P,
                                                  const uint32_t* __restrict__ thr) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -714,8 +796,13 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
         const int by = (((gtid >> a.lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * a.lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
-                                     CellSync{a.sync_pass, a.claims, a.sync_tag}, a.att0, a.natt);
+        if constexpr (REF)
+            ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, T,
+                                   CellSync{a.sync_pass, a.claims, a.sync_tag}, a.natt);
+        else
+            sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
+                                         CellSync{a.sync_pass, a.claims, a.sync_tag}, a.att0,
+                                         a.natt);
     }
     if (fold) {
         if (dv != 0) {
@@ -997,6 +1084,32 @@ __global__ __launch_bounds__(kBlock) void k_sweep_phase_lds(PhaseArgs a) {
     LdsParamView P;
     const uint32_t* thr = stage_params(sp, a.k, &P);
     sweep_phase_body<D, ST>(a, P, thr);
+}
+
+// The reference stream: one thread per sub-block, staged parameters.
+template <int D, class ST>
+__global__ __launch_bounds__(kBlock) void k_sweep_phase_ref(PhaseArgs a) {
+    __shared__ LdsParams sp;
+    LdsParamView P;
+    const uint32_t* thr = stage_params(sp, a.k, &P);
+    sweep_phase_body<D, ST, true>(a, P, thr);
+}
+
+__global__ void k_ref_seed(int32_t* st, uint32_t n, int lr, int lR, int lgL, int d, uint32_t seed32) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t rm = (1u << lr) - 1u;
+    const uint32_t x = (i & rm) << lR, y = ((i >> lr) & rm) << lR;
+    const uint32_t z = d == 3 ? (i >> (2 * lr)) << lR : 0u;
+    const uint32_t base = x | (y << lgL) | (z << (2 * lgL));  // kernels2d.cu:99
+    st[i] = minstd_next((int32_t)(base + seed32));
+}
+
+hipError_t launch_ref_seed(int d, const SweepConsts& k, uint32_t seed32, hipStream_t s) {
+    const uint32_t n = 1u << (d * k.lr);
+    hipLaunchKernelGGL(k_ref_seed, dim3((n + 255) / 256), dim3(256), 0, s, k.rstate, n, k.lr,
+                       k.lk + k.lc, k.lgL, d, seed32);
+    return hipGetLastError();
 }
 
 // Batched replicas, one colour phase: block b works for replica rep_of[b]
@@ -1421,7 +1534,7 @@ enum PhaseKernel {
     kPkBox3, kPkCompact2Small, kPkCompact2SmallFull, kPkCompact2Lds, kPkCompact2LdsFull,
     kPkCompact3Small, kPkCompact3Lds, kPkPhase2Small, kPkPhase2Lds, kPkPhase3Small, kPkPhase3Lds,
     kPkCompact2Lds2, kPkCompact2Lds4, kPkCompact3Small2, kPkCompact3Small4, kPkCompact3Small8,
-    kPkCompact2Lds8, kPkCount
+    kPkCompact2Lds8, kPkRef2, kPkRef3, kPkCount
 };
 
 // Kernels on the 16-bit lattice have ids kPkNarrow + id.
@@ -1443,7 +1556,8 @@ const char* phase_kernel_name(int id) {
      "k_sweep_phase_compact<3, false, false, 2, " ST ">",                                         \
      "k_sweep_phase_compact<3, false, false, 4, " ST ">",                                         \
      "k_sweep_phase_compact<3, false, false, 8, " ST ">",                                         \
-     "k_sweep_phase_compact<2, true, false, 8, " ST ">"}
+     "k_sweep_phase_compact<2, true, false, 8, " ST ">",                                          \
+     "k_sweep_phase_ref<2, " ST ">", "k_sweep_phase_ref<3, " ST ">"}
     static const char* const wide[kPkCount] = CPM_NAMES("unsigned int");
     static const char* const narrow[kPkCount] = CPM_NAMES("unsigned short");
 #undef CPM_NAMES
@@ -1456,6 +1570,7 @@ const char* phase_kernel_name(int id) {
 
 static int choose_phase_kernel(int d, const PhaseArgs& a, bool box_ok) {
     const int L = 1 << a.k.lgL;
+    if (a.k.refstream) return d == 2 ? kPkRef2 : kPkRef3;
     if (box_ok && d == 3 && a.k.lc == 0 && !a.k.roles && a.lstride == 1 && L >= kBX && L >= kBY &&
         L >= kBZ && a.box3 && !a.sync_pass) {
         const long nbox = (long)(L / kBX) * (L / kBY) * (L / kBZ);
@@ -1508,6 +1623,8 @@ static void launch_phase_st(int id, const PhaseArgs& a, hipStream_t s, hipEvent_
         case kPkPhase2Small: CPM_LAUNCH((k_sweep_phase<2, ST>), blocks); break;
         case kPkPhase2Lds: CPM_LAUNCH((k_sweep_phase_lds<2, ST>), blocks); break;
         case kPkPhase3Small: CPM_LAUNCH((k_sweep_phase<3, ST>), blocks); break;
+        case kPkRef2: CPM_LAUNCH((k_sweep_phase_ref<2, ST>), blocks); break;
+        case kPkRef3: CPM_LAUNCH((k_sweep_phase_ref<3, ST>), blocks); break;
         default: CPM_LAUNCH((k_sweep_phase_lds<3, ST>), blocks); break;
     }
 #undef CPM_LAUNCH

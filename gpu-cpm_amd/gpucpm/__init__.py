@@ -314,6 +314,26 @@ class Cpm:
         kernel = lib.cpm_last_kernel(self._h).decode()
         return {"form": f.value, "kernel": kernel, "blocks": b.value, "tile": t.value}
 
+    _RNG = {"philox": 0, "minstd": 1}
+
+    def set_rng(self, name):
+        """Random stream: "philox" (default; counter-based, keyed by sub-block,
+        MCS, phase and attempt) or "minstd", the reference's per-thread
+        stream (random.cu, kernels2d.cu:99): one MINSTD state per region of
+        positions_per_thread sites, reseeded every inner_iterations chunk
+        from the region's first site index + the low 32 bits of the seed, for
+        comparisons with the reference.  "minstd" runs single engines, and
+        with shared=1 needs positions_per_thread = 2 * positions_per_checkerboard."""
+        if name not in self._RNG:
+            raise ValueError("rng must be 'philox' or 'minstd'")
+        check(lib.cpm_set_rng(self._h, self._RNG[name]))
+
+    @property
+    def rng(self):
+        v = ctypes.c_int()
+        check(lib.cpm_get_rng(self._h, ctypes.byref(v)))
+        return {0: "philox", 1: "minstd"}[v.value]
+
     def set_seed(self, seed):
         check(lib.cpm_set_seed(self._h, _int(seed, "seed") & (2**64 - 1)))
 

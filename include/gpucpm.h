@@ -157,6 +157,18 @@ int cpm_get_centroids(cpm_handle h, double* out, int capacity, int* rows);
 int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anchor,
                        int64_t* csum);
 
+/* Extension: the random stream.  mode 0 (default): the counter stream, Philox
+ * keyed by (sub-block, MCS, phase, attempt).  mode 1: the reference's stream
+ * (random.cu:5-28 seeded per thread at kernels2d.cu:99 / kernels3d.cu:86),
+ * for comparisons with the reference: one MINSTD state per region of
+ * positions_per_thread sites per axis (2c with shared = 1, which then needs
+ * positions_per_thread = 2c), seeded at every inner_iterations chunk from
+ * the region's first site index + the low 32 bits of the seed, consumed in
+ * the reference's draw order; acceptance u < ceil(exp(-E/T) (2^31 - 1)).
+ * Single engines only (a batch returns CPM_ENOTIMPL). */
+int cpm_set_rng(cpm_handle h, int mode);
+int cpm_get_rng(cpm_handle h, int* mode);
+
 /* Extension: RNG seed (Philox4x32-10 key) and the MCS counter that keys the
  * stream; together they make every run reproducible and resumable. */
 int cpm_set_seed(cpm_handle h, uint64_t seed);

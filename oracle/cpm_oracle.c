@@ -143,6 +143,25 @@ int orc_threshold_table(double temperature, int scale, uint32_t* out, int cap) {
     return -2; /* table would exceed cap */
 }
 
+/* The reference accepts a positive energy E when r < exp(-E / T) with
+ * r = u / (2^31 - 1) and u the next MINSTD value (kernels2d.cu:522-525).  In
+ * exact arithmetic that is u < exp(-E/T) (2^31 - 1), i.e. u < ceil(...): with
+ * E = k / scale the table holds thr31[k] = ceil(exp(-k / (scale T)) (2^31 -
+ * 1)) while it exceeds 1 (u >= 1, so a smaller entry never accepts). */
+int orc_ref_threshold_table(double temperature, int scale, uint32_t* out, int cap) {
+    if (cap < 1) return -1;
+    out[0] = 0xFFFFFFFFu;
+    if (!(temperature > 0.0)) return 1;
+    int n = 1;
+    for (int k = 1; k < cap; k++) {
+        double v = ceil(exp(-(double)k / ((double)scale * temperature)) * 2147483647.0);
+        if (v <= 1.0) return n;
+        out[k] = (uint32_t)v;
+        n = k + 1;
+    }
+    return -2;
+}
+
 /* ------------------------------------------------------- geometry -- */
 
 typedef struct {
@@ -454,10 +473,50 @@ int orc_schedule(int L, int d, int c, int shared, int ppt, int* K, int* S) {
     return 0;
 }
 
+/* Reference stream (random.cu:5-12, kernels2d.cu:99,199-243,522,
+ * kernels3d.cu:86,160-215): one MINSTD state per region of R sites per axis
+ * (a reference thread), seeded at the start of every call (one reference
+ * launch) with x = MINSTD(index of the region's first site + seed32) and then
+ * consumed in the region's attempt order: the position in the sub-block (d
+ * draws mod c), then direction triples (each axis u mod 3 - 1) until one is
+ * not rejected, then -- when the two sites differ and the target type is not
+ * fixed -- the acceptance draw u.  The rejection test is the reference's:
+ * all drawn coordinates equal in 2D and in 3D with shared roles; in 3D
+ * without shared roles the reference compares the z coordinate with y
+ * (kernels3d.cu:208), which is kept.  At most kRefTries direction triples
+ * are drawn (a stream stuck at one value would otherwise never end).
+ * Acceptance: dH <= 0 or u < thr31[dH] (orc_ref_threshold_table). */
+#define REF_TRIES 256
+static int32_t ref_next(int32_t x) { return orc_minstd_next_f64(x); }
+
+static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                      const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                      uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
+                      int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
+                      int64_t* trace_n);
+
 int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                  const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt) {
+    return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
+                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL);
+}
+
+int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                        const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                        uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
+                        int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n) {
+    if (shared && ppt != 2 * c) return -9; /* the reference's shared mode with ppt > 2c loops regions per thread */
+    return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
+                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n);
+}
+
+static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                      const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                      uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
+                      int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
+                      int64_t* trace_n) {
     if (d != 2 && d != 3) return -1;
     if (shared && act_on(p) && p->act != NULL && c < 2) return -8;
     if (L < 2 || (L & (L - 1))) return -2;
@@ -470,7 +529,8 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
     if (scale < 0) return -5;
     int cap = 1 << 22;
     uint32_t* thr = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)cap);
-    int nthr = orc_threshold_table(p->temperature, scale, thr, cap);
+    int nthr = refstream ? orc_ref_threshold_table(p->temperature, scale, thr, cap)
+                         : orc_threshold_table(p->temperature, scale, thr, cap);
     if (nthr < 0) {
         free(thr);
         return -6;
@@ -481,6 +541,20 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
     make_nbhd(d, &nb);
     const int N = nb.n;
     const int lc = log2i(c);
+    /* reference stream: one state per region, seeded per call */
+    const int nreg = L / R;
+    int32_t* rstate = NULL;
+    if (refstream) {
+        int64_t nr = d == 3 ? (int64_t)nreg * nreg * nreg : (int64_t)nreg * nreg;
+        rstate = (int32_t*)malloc(sizeof(int32_t) * (size_t)nr);
+        for (int64_t ri = 0; ri < nr; ri++) {
+            int rx = (int)(ri % nreg), ry = (int)((ri / nreg) % nreg);
+            int rz = d == 3 ? (int)(ri / ((int64_t)nreg * nreg)) : 0;
+            uint32_t base = (uint32_t)site(L, rx * R, ry * R, rz * R); /* kernels2d.cu:99 */
+            rstate[ri] = ref_next((int32_t)(base + (uint32_t)seed));
+        }
+        if (trace_n) *trace_n = 0;
+    }
     const uint32_t cm = (uint32_t)c - 1;
     const int nsb = L / c;       /* sub-blocks per axis */
     const int h = L / R / S;     /* same-phase sub-blocks per axis */
@@ -532,28 +606,85 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                     int by = ((int)((qi / h) % h) * S + gy) * K + cy;
                     int bz = d == 3 ? ((int)(qi / ((int64_t)h * h)) * S + gz) * K + cz : 0;
                     uint32_t sb = (uint32_t)(((int64_t)bz * nsb + by) * nsb + bx);
+                    const int64_t rid = d == 3 ? ((int64_t)(bz / K) * nreg + by / K) * nreg + bx / K
+                                               : (int64_t)(by / K) * nreg + bx / K;
+                    int32_t rs = refstream ? rstate[rid] : 0;
                     for (int i = i0; i < (cell_sync ? i0 + 1 : u); i++) {
-                        uint32_t ctr[4] = {sb, mcs, phase, (uint32_t)i};
-                        uint32_t w[4];
-                        orc_philox4x32_10(ctr, key, w);
-                        /* the position drawn in the sub-block and a neighbour:
-                         * shared roles (kernels2d.cu:204-226): target, source;
-                         * non-shared (:227-243): source, target */
-                        int px = bx * c + (int)(w[0] & cm);
-                        int py = by * c + (int)((w[0] >> lc) & cm);
-                        int pz = d == 3 ? bz * c + (int)((w[0] >> (2 * lc)) & cm) : 0;
-                        int dn = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
-                        int qx = (px + nb.off[dn][0]) & (L - 1);
-                        int qy = (py + nb.off[dn][1]) & (L - 1);
-                        int qz = d == 3 ? (pz + nb.off[dn][2]) & (L - 1) : 0;
+                        uint32_t w[4] = {0, 0, 0, 0};
+                        int px, py, pz, qx, qy, qz;
+                        int o[3]; /* the source's offset from the target */
+                        if (!refstream) {
+                            uint32_t ctr[4] = {sb, mcs, phase, (uint32_t)i};
+                            orc_philox4x32_10(ctr, key, w);
+                            /* the position drawn in the sub-block and a
+                             * neighbour: shared roles (kernels2d.cu:204-226):
+                             * target, source; non-shared (:227-243): source,
+                             * target */
+                            px = bx * c + (int)(w[0] & cm);
+                            py = by * c + (int)((w[0] >> lc) & cm);
+                            pz = d == 3 ? bz * c + (int)((w[0] >> (2 * lc)) & cm) : 0;
+                            int dn = (int)(((uint64_t)w[1] * (uint64_t)N) >> 32);
+                            qx = (px + nb.off[dn][0]) & (L - 1);
+                            qy = (py + nb.off[dn][1]) & (L - 1);
+                            qz = d == 3 ? (pz + nb.off[dn][2]) & (L - 1) : 0;
+                            /* (the list is symmetric: entry N-1-n is minus entry n) */
+                            int dir = shared ? dn : N - 1 - dn;
+                            for (int a = 0; a < 3; a++) o[a] = nb.off[dir][a];
+                        } else {
+                            rs = ref_next(rs);
+                            px = bx * c + rs % c;
+                            rs = ref_next(rs);
+                            py = by * c + rs % c;
+                            pz = 0;
+                            if (d == 3) {
+                                rs = ref_next(rs);
+                                pz = bz * c + rs % c;
+                            }
+                            int dx = 0, dy = 0, dz = 0;
+                            for (int tries = 0; tries < REF_TRIES; tries++) {
+                                rs = ref_next(rs);
+                                dx = rs % 3 - 1;
+                                rs = ref_next(rs);
+                                dy = rs % 3 - 1;
+                                if (d == 3) {
+                                    rs = ref_next(rs);
+                                    dz = rs % 3 - 1;
+                                }
+                                qx = (px + dx) & (L - 1);
+                                qy = (py + dy) & (L - 1);
+                                qz = d == 3 ? (pz + dz) & (L - 1) : 0;
+                                int rej = d == 2   ? (qx == px && qy == py)
+                                          : shared ? (qx == px && qy == py && qz == pz)
+                                                   : (qx == px && qy == py && qz == py);
+                                if (!rej) break;
+                            }
+                            o[0] = shared ? dx : -dx;
+                            o[1] = shared ? dy : -dy;
+                            o[2] = shared ? dz : -dz;
+                        }
                         int tx = shared ? px : qx, ty = shared ? py : qy, tz = shared ? pz : qz;
                         int sx = shared ? qx : px, sy = shared ? qy : py, sz = shared ? qz : pz;
-                        /* the source's offset from the target (the list is
-                         * symmetric: entry N-1-n is minus entry n) */
-                        int dir = shared ? dn : N - 1 - dn;
                         int64_t ti = site(L, tx, ty, tz);
                         uint32_t vs = lat[site(L, sx, sy, sz)];
                         uint32_t vt = lat[ti];
+                        if (refstream) {
+                            /* the acceptance draw is taken when the sites
+                             * differ and the target is not fixed
+                             * (kernels2d.cu:276-284,522) */
+                            int take = vs != vt && !p->fixed[vt >> 24];
+                            if (take) {
+                                rs = ref_next(rs);
+                                w[2] = (uint32_t)rs;
+                            }
+                            if (trace && !(cell_sync && pass == 0) && *trace_n < trace_cap) {
+                                int64_t* e = trace + 4 * *trace_n;
+                                e[0] = rid;
+                                e[1] = ti;
+                                e[2] = site(L, sx, sy, sz);
+                                e[3] = take ? (int64_t)(uint32_t)rs : -1;
+                                ++*trace_n;
+                            }
+                        }
                         if (vs == vt) continue; /* kernels2d.cu:276 */
                         /* fixed types (kernels2d.cu:281-290): a fixed target is
                          * never overwritten; a fixed source copies medium */
@@ -566,11 +697,10 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                         int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
                                                   tz, vs, vt, &as, &at);
                         if (q.chem)
-                            dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, nb.off[dir],
-                                              site(L, sx, sy, sz));
+                            dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, o, site(L, sx, sy, sz));
                         if (q.act)
                             dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
-                        if (q.pers && d == 3) dh += pers_energy(&q, vs, vt, nb.off[dir]);
+                        if (q.pers && d == 3) dh += pers_energy(&q, vs, vt, o);
                         int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
                         if (!accept) continue;
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
@@ -618,6 +748,8 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                             }
                         }
                     }
+                    /* a claim pass recomputes the attempts its commit pass makes */
+                    if (refstream && !(cell_sync && pass == 0)) rstate[rid] = rs;
                 }
                 }
             }
@@ -627,6 +759,7 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
     free(perimR);
     free(owner);
     free(thr);
+    free(rstate);
     return 0;
 }
 

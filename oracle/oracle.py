@@ -129,6 +129,11 @@ def lib():
         L.orc_sweep_sync.restype = ctypes.c_int
         L.orc_sweep_ex.argtypes = L.orc_sweep.argtypes + [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_sweep_ex.restype = ctypes.c_int
+        L.orc_sweep_refstream.argtypes = L.orc_sweep_ex.argtypes + [
+            P(ctypes.c_int64), ctypes.c_int64, P(ctypes.c_int64)]
+        L.orc_sweep_refstream.restype = ctypes.c_int
+        L.orc_ref_threshold_table.argtypes = [ctypes.c_double, ctypes.c_int, u32p, ctypes.c_int]
+        L.orc_ref_threshold_table.restype = ctypes.c_int
         L.orc_schedule.argtypes = [ctypes.c_int] * 5 + [P(ctypes.c_int), P(ctypes.c_int)]
         L.orc_schedule.restype = ctypes.c_int
         L.orc_serial_sweep.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, i32p, i32p, i32p,
@@ -309,21 +314,44 @@ def schedule(L: int, d: int, c: int, shared: bool, ppt: int):
 
 def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
           mcs0: int, nmcs: int, nthreads: int = 1, cell_sync: bool = False, shared: bool = True,
-          ppt: int = None) -> None:
+          ppt: int = None, rng: str = "philox", trace_cap: int = 0):
     """Run nmcs MCS in place on lat (uint32 view) and tables.  shared selects
     the roles of the run() kwarg of that name; ppt is positions_per_thread
-    (used without shared roles)."""
+    (used without shared roles).  rng = "minstd" uses the reference's
+    per-thread MINSTD stream (one call = one reference launch: the streams
+    are seeded at entry); with trace_cap > 0 the attempts' (region, target,
+    source, acceptance draw or -1) rows are returned."""
     assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
     lat32 = lat.view(np.uint32)
     L = lat.shape[0]
-    rc = lib().orc_sweep_ex(L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
-                            _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
-                            _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
-                            ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
-                            int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads), int(bool(cell_sync)),
-                            int(bool(shared)), int(ppt if ppt is not None else 2 * c))
+    args = (L, d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
+            _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+            _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+            ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
+            int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads), int(bool(cell_sync)),
+            int(bool(shared)), int(ppt if ppt is not None else 2 * c))
+    if rng == "philox":
+        rc = lib().orc_sweep_ex(*args)
+        trace = None
+    elif rng == "minstd":
+        trace = np.zeros((max(trace_cap, 1), 4), np.int64)
+        n = ctypes.c_int64(0)
+        rc = lib().orc_sweep_refstream(*args, _ptr(trace, ctypes.c_int64) if trace_cap else None,
+                                       int(trace_cap), ctypes.byref(n))
+        trace = trace[:n.value] if trace_cap else None
+    else:
+        raise ValueError("rng must be 'philox' or 'minstd'")
     if rc:
         raise ValueError(f"orc_sweep failed with code {rc}")
+    return trace
+
+
+def ref_threshold_table(temperature: float, scale: int, cap: int = 1 << 16) -> np.ndarray:
+    out = np.zeros(cap, np.uint32)
+    n = lib().orc_ref_threshold_table(float(temperature), int(scale), _ptr(out, ctypes.c_uint32), cap)
+    if n < 0:
+        raise ValueError("table too long")
+    return out[:n]
 
 
 def serial_sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, seed: int, mcs0: int,

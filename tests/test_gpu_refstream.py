@@ -1,0 +1,95 @@
+"""GPU parity of the reference-stream mode (Cpm.set_rng("minstd")): the
+HIP sweep through the C ABI against the oracle's reference stream, which
+tests/test_refstream.py pins against a thread-major restatement of the
+reference kernel.  Every case prints its name before touching the GPU."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, run_kwargs, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(*a):
+    print("CASE", *a, flush=True)
+
+
+def _run(wl, c, u, seed, nmcs, inner, shared, ppt, cell_sync=0):
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(seed)
+    sim.set_rng("minstd")
+    assert sim.rng == "minstd"
+    sim.run(**run_kwargs(c, u, nmcs, inner, shared=int(shared), positions_per_thread=ppt,
+                         cell_sync=cell_sync))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    return sim, sim.get_state().view(np.uint32).copy(), sim.get_cell_table()
+
+
+def _cases():
+    from gpucpm import workloads as W
+    return [  # name, workload, c, u, shared, ppt, nmcs, inner
+        ("2d-ns", W.sorting2d(L=128, grid=9, edge=10, seed=81), 2, 1, False, 4, 20, 5),
+        ("2d-ns-u3", W.sorting2d(L=128, grid=9, edge=10, seed=82), 2, 3, False, 8, 12, 4),
+        ("2d-sh", W.sorting2d(L=128, grid=9, edge=10, seed=83), 1, 1, True, 2, 20, 10),
+        ("2d-sh-u4", W.sorting2d(L=64, grid=5, edge=10, seed=84), 2, 4, True, 4, 10, 10),
+        ("2d-wall", W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=85)), 2, 1, False, 4, 20, 7),
+        ("2d-chem", W.chemotactic(W.sorting2d(L=64, grid=5, edge=10, seed=86)), 2, 1, False, 4, 15, 5),
+        ("3d-ns", W.plain3d(L=32, ncells=60, seed=87), 1, 1, False, 2, 6, 3),
+        ("3d-ns-u3", W.plain3d(L=32, ncells=60, seed=88), 2, 3, False, 4, 4, 2),
+        ("3d-sh", W.plain3d(L=32, ncells=60, seed=89), 1, 1, True, 2, 6, 6),
+    ]
+
+
+@pytest.mark.parametrize("narrow", ["1", "0"])
+@pytest.mark.parametrize("idx", range(9))
+def test_refstream_matches_oracle(idx, narrow, monkeypatch):
+    monkeypatch.setenv("GPUCPM_NARROW", narrow)
+    name, wl, c, u, shared, ppt, nmcs, inner = _cases()[idx]
+    _announce("refstream", name, narrow, wl.L, wl.d, c, u, shared, ppt)
+    sim, lat, table = _run(wl, c, u, 0x9E3779B9 + idx, nmcs, inner, shared, ppt)
+    assert "k_sweep_phase_ref" in sim.last_launch()["kernel"]
+    lat_o, tab_o = run_oracle(wl, c, u, 0x9E3779B9 + idx, nmcs, inner=inner, shared=shared, ppt=ppt,
+                              rng="minstd")
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, wl.d)
+    assert np.count_nonzero(lat != wl.lattice) > 0
+
+
+def test_refstream_cell_sync_matches_oracle():
+    from gpucpm import workloads as W
+    wl = W.sorting2d(L=128, grid=9, edge=10, seed=90)
+    _announce("refstream cell_sync", wl.L)
+    _, lat, table = _run(wl, 2, 2, 31, 10, 5, False, 4, cell_sync=1)
+    lat_o, tab_o = run_oracle(wl, 2, 2, 31, 10, inner=5, shared=False, ppt=4, rng="minstd",
+                              cell_sync=True)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 2)
+
+
+def test_refstream_refusals_and_switching_back():
+    """shared=1 with positions_per_thread > 2c and batches are refused; the
+    counter stream is restored by set_rng("philox")."""
+    import gpucpm
+    from gpucpm import workloads as W
+    wl = W.sorting2d(L=64, grid=5, edge=10, seed=91)
+    _announce("refstream refusals", wl.L)
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_rng("minstd")
+    with pytest.raises(NotImplementedError):
+        sim.run(**run_kwargs(1, 1, 2, shared=1, positions_per_thread=4))
+    with pytest.raises(NotImplementedError):
+        gpucpm.Batch([sim]).run(**run_kwargs(2, 1, 2, shared=0, positions_per_thread=4))
+    with pytest.raises(ValueError):
+        sim.set_rng("mt19937")
+    sim.set_rng("philox")
+    sim.set_seed(5)
+    sim.run(**run_kwargs(1, 1, 5))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    lat_o, _ = run_oracle(wl, 1, 1, 5, 5)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
