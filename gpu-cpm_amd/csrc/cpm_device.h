@@ -125,9 +125,10 @@ struct PhaseArgs {
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int nsub;               // active sub-blocks in this phase
     int box3;               // 3D, c = 1: stage each block's box in LDS
-    int sync_pass;          // cell_sync: 0 off, 1 claim pass, 2 commit pass
+    int sync_pass;          // 0 off; cell_sync: 1 claim, 2 commit; split phase: 3 decide, 4 apply
     uint64_t* claims;       // cell_sync: per-cell claim keys
     uint32_t sync_tag;      // cell_sync: phase tag (grows across runs)
+    uint8_t* decide;        // split phase: one decision per sub-block of the phase
     int form;               // kForm* bits: which per-phase kernel runs
     int att0, natt;         // attempt indices att0 .. att0 + natt - 1 of each sub-block
 };

@@ -148,6 +148,8 @@ struct cpm_engine {
     int rng = 0;
     int32_t* d_rstate = nullptr;
     size_t rstate_cap = 0;
+    uint8_t* d_decide = nullptr;  // split phases (act, shared roles, c = 1): decisions
+    size_t decide_cap = 0;
     bool wide_valid = true;
     bool narrow_valid = false;
     int64_t* d_tab[2] = {nullptr, nullptr};
@@ -198,6 +200,7 @@ struct cpm_engine {
         (void)hipFree(d_lat);
         (void)hipFree(d_lat16);
         (void)hipFree(d_rstate);
+        (void)hipFree(d_decide);
         for (int i = 0; i < 2; i++) (void)hipFree(d_tab[i]);
         for (int i = 0; i < 2; i++) (void)hipFree(d_del[i]);
         (void)hipFree(d_anchor);
@@ -962,10 +965,6 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (k.act) {
         if (!h->has_act)
             return fail(CPM_EINVAL, "lambda_act is set but the engine has no act lattice (has_act = 0)");
-        if (shared && c < 2)
-            return fail(CPM_ENOTIMPL,
-                        "with shared roles the act term needs positions_per_checkerboard >= 2 (with 1 "
-                        "the source neighbourhood reaches targets of the same colour phase)");
         for (int t = 0; t < CPM_NTYPES; t++) {
             if (h->st.lam_act[t] == 0.0) continue;
             if (h->st.act_max[t] < 1) return fail(CPM_EINVAL, "max_act must be >= 1 where lambda_act != 0");
@@ -1024,6 +1023,26 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         plan.mode = Plan::kPhase;  // the persistent forms implement shared roles only
     } else if (int rc = h->plan(c, &plan)) {
         return rc;
+    }
+    // Act with shared roles and c = 1: an attempt's act term reads the
+    // source's neighbourhood, two sites from its target, where other targets
+    // of the phase lie.  Each phase is then a decide launch (every attempt
+    // against the phase-start lattice, decisions recorded) and an apply
+    // launch (the accepted ones committed; a target's own neighbourhood
+    // holds no other target, so the deltas are those decided).
+    const bool split = k.act && !k.roles && c == 1;
+    if (split) {
+        if (a->cell_sync)
+            return fail(CPM_ENOTIMPL, "cell_sync with the act term, shared roles and positions_per_checkerboard = 1");
+        plan = Plan{};
+        plan.mode = Plan::kPhase;
+        const size_t nsub = (size_t)1 << (lh * h->d);
+        if (nsub > h->decide_cap) {
+            (void)hipFree(h->d_decide);
+            h->d_decide = nullptr;
+            HIP_TRY(hipMalloc(&h->d_decide, nsub));
+            h->decide_cap = nsub;
+        }
     }
     if (h->rng == 1) {
         // The reference stream: one launch per colour phase (each region's
@@ -1110,6 +1129,12 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                     for (int i = 0; i < nidx; i++) {
                         pa.att0 = a->cell_sync ? i : 0;
                         pa.natt = a->cell_sync ? 1 : u;
+                        if (split) {
+                            pa.decide = h->d_decide;
+                            pa.sync_pass = 3;
+                            HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, nullptr, nullptr));
+                            pa.sync_pass = 4;
+                        }
                         if (a->cell_sync) {
                             // the tag only grows: claims never need clearing
                             if (++h->sync_tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
@@ -1530,6 +1555,9 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         if (b->sims[i]->rng != 0)
             return fail(CPM_ENOTIMPL, "the reference stream (cpm_set_rng) runs single engines only");
         if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
+        if (reps[i].k.act && !reps[i].k.roles && reps[i].k.lc == 0)
+            return fail(CPM_ENOTIMPL,
+                        "batched replicas with the act term, shared roles and positions_per_checkerboard = 1");
         if (reps[i].k.K != reps[0].k.K || reps[i].k.S != reps[0].k.S)
             return fail(CPM_EINVAL, "batched replicas need the same colour schedule (K, S)");
         const long nsub = 1L << (reps[i].lh * b->d);

@@ -303,9 +303,11 @@ __device__ __forceinline__ void draw_attempt(const Lat& lat, const SweepConsts& 
 // phase wins; the commit pass recomputes the attempt and commits only if it
 // holds both claims.
 struct CellSync {
-    int pass;  // 0 off, 1 claim, 2 commit
+    int pass;  // 0 off; cell_sync: 1 claim, 2 commit; split phase: 3 decide, 4 apply
     uint64_t* claims;
     uint32_t tag;
+    uint8_t* decide;  // split phase: decisions by phase-local sub-block index
+    int q;            // this attempt's sub-block index in the phase
 };
 
 // Adhesion over the neighbourhood (kernels2d.cu:359-402,477):
@@ -570,12 +572,23 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         dh += (int64_t)P.la(tt) * (1 - 2 * A + 2 * (int64_t)P.a0(tt));
         dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
     }
-    if (k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
-    if (k.act) dh += act_energy<D, CO>(k, lat, d.t, d.nv, d.dir, vs, vt, d.tick);
-    if (D == 3 && k.pers) dh += pers_energy<D>(k, d.dir, vs, vt);
-    const bool accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
+    bool accept;
+    if (cy.pass == 4) {
+        // split phase, apply: the decision made against the phase-start
+        // lattice (the act term reads sites that other attempts now write)
+        accept = cy.decide[cy.q] != 0;
+    } else {
+        if (k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
+        if (k.act) dh += act_energy<D, CO>(k, lat, d.t, d.nv, d.dir, vs, vt, d.tick);
+        if (D == 3 && k.pers) dh += pers_energy<D>(k, d.dir, vs, vt);
+        accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
+    }
+    if (cy.pass == 3) {  // split phase, decide: record, write nothing else
+        if (accept) cy.decide[cy.q] = 1;
+        return;
+    }
     if (!accept) return;
-    if (cy.pass) {
+    if (cy.pass == 1 || cy.pass == 2) {
         const uint64_t key = ((uint64_t)cy.tag << 32) | (uint64_t)(0xFFFFFFFFu - sb);
         if (cy.pass == 1) {
             if (cs) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + cs), key);
@@ -680,7 +693,7 @@ __device__ __forceinline__ void ref_attempts(const Lat& lat, const SweepConsts& 
         d.tick = mcs;
         finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
     }
-    if (cy.pass != 1) k.rstate[rid] = st;
+    if (cy.pass != 1 && cy.pass != 3) k.rstate[rid] = st;
 }
 
 // The u copy attempts of sub-block (bx, by, bz) in one colour phase, in
@@ -788,20 +801,20 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
     const int gsz = gridDim.x * blockDim.x;
     const SweepConsts& k = a.k;
     const PhaseTables T = tables_of(k, a.g);
-    const bool fold = a.sync_pass != 1;
+    const bool fold = a.sync_pass != 1 && a.sync_pass != 3;
     const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
     if (gtid < a.nsub) {
+        if (a.sync_pass == 3) a.decide[gtid] = 0;
         const int hm = (1 << a.lh) - 1;
         const int bx = ((gtid & hm) << a.lstride) + a.cx;
         const int by = (((gtid >> a.lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * a.lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
+        const CellSync cy{a.sync_pass, a.claims, a.sync_tag, a.decide, gtid};
         if constexpr (REF)
-            ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, T,
-                                   CellSync{a.sync_pass, a.claims, a.sync_tag}, a.natt);
+            ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, T, cy, a.natt);
         else
-            sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T,
-                                         CellSync{a.sync_pass, a.claims, a.sync_tag}, a.att0,
+            sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, a.mcs, a.phase, T, cy, a.att0,
                                          a.natt);
     }
     if (fold) {
@@ -1577,7 +1590,7 @@ static int choose_phase_kernel(int d, const PhaseArgs& a, bool box_ok) {
         if (nbox * kBlock >= a.k.ncells + 1) return kPkBox3;
     }
     const bool small = a.k.nt <= kSmallTypes && (a.form & kFormSmall);
-    const bool compact = a.k.u == 1 && (a.form & kFormCompact);
+    const bool compact = a.k.u == 1 && (a.form & kFormCompact) && a.sync_pass < 3;
     const bool full = a.form & kFormFull;
     const int att = form_att(a.form);
     if (compact) {

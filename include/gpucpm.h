@@ -93,7 +93,10 @@ int cpm_set_chemotaxis(cpm_handle h, int type, double lambda);
  * -lambda/actMax (G_source - G_target) (kernels2d.cu:312-343,386-397,
  * 417-473,532-534), G the geometric mean of site activities over a site and
  * its same-cell neighbours.  max_act in [0, 65535].  Runs with the term need
- * has_act and positions_per_checkerboard >= 2. */
+ * has_act.  With shared = 1 and positions_per_checkerboard = 1 (the term
+ * then reads sites other attempts of the phase write) each phase is decided
+ * against its start lattice, then applied: two launches per phase; not with
+ * cell_sync or in batches (CPM_ENOTIMPL). */
 int cpm_set_act(cpm_handle h, int type, int max_act, double lambda);
 /* Cpm::setPersistence (cpm.cu:112-115): the persistence term -lambda (d . p)
  * (kernels3d.cu:327-344, 3D only as in the reference), p the cell's preferred

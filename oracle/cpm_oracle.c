@@ -518,7 +518,14 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
                       int64_t* trace_n) {
     if (d != 2 && d != 3) return -1;
-    if (shared && act_on(p) && p->act != NULL && c < 2) return -8;
+    /* Act with shared roles and c = 1: an attempt's act term reads the
+     * source's neighbourhood, two sites from its target, where other targets
+     * of the phase lie.  Each phase is then made in two passes: every
+     * attempt is decided against the phase-start lattice (nothing written),
+     * then the accepted ones are applied (a target's own neighbourhood holds
+     * no other target, so the applied deltas are those decided). */
+    const int split = shared && act_on(p) && p->act != NULL && c < 2;
+    if (split && cell_sync) return -8;
     if (L < 2 || (L & (L - 1))) return -2;
     int K, S;
     if (orc_schedule(L, d, c, shared, ppt, &K, &S)) return -3;
@@ -567,6 +574,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
     int32_t* areaR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
     int32_t* perimR = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
     uint32_t* owner = cell_sync ? (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ncells + 1)) : NULL;
+    uint8_t* dec = split ? (uint8_t*)malloc((size_t)nq) : NULL;
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
 #else
@@ -595,7 +603,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                  * if its sub-block holds the claim on both of its non-medium
                  * cells.  Without cell_sync one pass makes all u attempts. */
                 for (int i0 = 0; i0 < u; i0 += cell_sync ? 1 : u)
-                for (int pass = cell_sync ? 0 : 1; pass < 2; pass++) {
+                for (int pass = cell_sync || split ? 0 : 1; pass < 2; pass++) {
                 if (cell_sync && pass == 0) memset(owner, 0xFF, sizeof(uint32_t) * (size_t)(ncells + 1));
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(static)
@@ -609,6 +617,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                     const int64_t rid = d == 3 ? ((int64_t)(bz / K) * nreg + by / K) * nreg + bx / K
                                                : (int64_t)(by / K) * nreg + bx / K;
                     int32_t rs = refstream ? rstate[rid] : 0;
+                    if (split && pass == 0) dec[qi] = 0;
                     for (int i = i0; i < (cell_sync ? i0 + 1 : u); i++) {
                         uint32_t w[4] = {0, 0, 0, 0};
                         int px, py, pz, qx, qy, qz;
@@ -676,7 +685,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                                 rs = ref_next(rs);
                                 w[2] = (uint32_t)rs;
                             }
-                            if (trace && !(cell_sync && pass == 0) && *trace_n < trace_cap) {
+                            if (trace && !((cell_sync || split) && pass == 0) && *trace_n < trace_cap) {
                                 int64_t* e = trace + 4 * *trace_n;
                                 e[0] = rid;
                                 e[1] = ti;
@@ -696,12 +705,21 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                         int as, at;
                         int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
                                                   tz, vs, vt, &as, &at);
-                        if (q.chem)
-                            dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, o, site(L, sx, sy, sz));
-                        if (q.act)
-                            dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
-                        if (q.pers && d == 3) dh += pers_energy(&q, vs, vt, o);
-                        int accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
+                        int accept;
+                        if (split && pass == 1) {
+                            accept = dec[qi]; /* decided against the phase-start lattice */
+                        } else {
+                            if (q.chem)
+                                dh += chem_energy(L, d, &q, vs >> 24, vt >> 24, o, site(L, sx, sy, sz));
+                            if (q.act)
+                                dh += act_energy(L, d, &nb, lat, &q, tx, ty, tz, sx, sy, sz, vs, vt, mcs);
+                            if (q.pers && d == 3) dh += pers_energy(&q, vs, vt, o);
+                            accept = dh <= 0 || (dh < nthr && w[2] < thr[dh]);
+                        }
+                        if (split && pass == 0) {
+                            dec[qi] = (uint8_t)accept;
+                            continue;
+                        }
                         if (!accept) continue;
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
                         if (cell_sync && pass == 0) {
@@ -749,7 +767,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                         }
                     }
                     /* a claim pass recomputes the attempts its commit pass makes */
-                    if (refstream && !(cell_sync && pass == 0)) rstate[rid] = rs;
+                    if (refstream && !((cell_sync || split) && pass == 0)) rstate[rid] = rs;
                 }
                 }
             }
@@ -758,6 +776,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
     free(areaR);
     free(perimR);
     free(owner);
+    free(dec);
     free(thr);
     free(rstate);
     return 0;

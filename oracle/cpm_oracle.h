@@ -98,9 +98,11 @@ void orc_init_pref_dirs(int ncells, uint64_t seed, double* pref);
  *   held in units of 2^-16 (orc_act_geo_mean);
  *   E = (-mA (Gs - Gt) + 2^31) >> 32, mA = llround(lambda scale / actMax 2^16).
  * With an act term the scale is 2^8.  It needs actMax in [1, 65535],
- * |lambda scale / actMax| < 2^14, and positions_per_checkerboard >= 2: the
- * source's neighbourhood reaches two sites from the target, where with
- * c = 1 other targets of the same colour phase lie. */
+ * |lambda scale / actMax| < 2^14.  The source's neighbourhood reaches two
+ * sites from the target, where with shared roles and c = 1 other targets of
+ * the same colour phase lie: such phases are made in two passes, every
+ * attempt decided against the phase-start lattice, then the accepted ones
+ * applied (not combined with cell_sync: -8). */
 #define ORC_ACT_MAX 65535
 int64_t orc_act_log2(int v);                          /* LG[v] */
 int64_t orc_act_geo_mean(int64_t sum_lg, int count);  /* G in 2^-16 units */

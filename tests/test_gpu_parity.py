@@ -313,15 +313,46 @@ def test_act_bit_exact(small, compact, att, mode, monkeypatch):
         np.testing.assert_array_equal(sim.get_act_state().view(np.uint32), tab_o.act)
 
 
-def test_act_refuses_single_site_checkerboard():
-    """With positions_per_checkerboard = 1 the act term's source neighbourhood
-    reaches same-phase targets; the run is refused, not raced."""
+@pytest.mark.parametrize("narrow", ["1", "0"])
+def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
+    """With shared roles and positions_per_checkerboard = 1 the act term's
+    source neighbourhood reaches same-phase targets: each phase is decided
+    against its start lattice and then applied (two launches), bit-exact
+    against the oracle's two passes in 2D and 3D (also with the reference
+    stream); with cell_sync, or in a batch, it is refused, not raced."""
+    import gpucpm
     _, W = _gpu()
-    wl = W.active(W.sorting2d(L=64, grid=5, edge=10, seed=105))
+    monkeypatch.setenv("GPUCPM_NARROW", narrow)
+    cases = [(W.active(W.sorting2d(L=128, grid=9, edge=10, seed=105)), 30, "philox"),
+             (W.active(W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=106)))), 20,
+              "philox"),
+             (W.active(W.plain3d(L=32, ncells=60, seed=107), lam=(0.0, 200.0), act_max=(0, 30)), 6,
+              "philox"),
+             (W.active(W.sorting2d(L=64, grid=5, edge=10, seed=108)), 12, "minstd")]
+    for wl, n, rng in cases:
+        _announce("act c=1", narrow, rng, wl.name, wl.L, wl.d)
+        sim = wl.make_cpm()
+        sim.push_to_gpu()
+        sim.set_seed(113)
+        sim.set_rng(rng)
+        sim.run(**run_kwargs(1, 1, n, n // 2 if rng == "minstd" else n, positions_per_thread=2))
+        sim.synchronize()
+        sim.pull_from_gpu()
+        lat = sim.get_state().view(np.uint32).copy()
+        lat_o, tab_o = run_oracle(wl, 1, 1, 113, n, inner=n // 2 if rng == "minstd" else None,
+                                  rng=rng)
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(sim.get_cell_table(), tab_o, wl.d)
+        np.testing.assert_array_equal(sim.get_act_state().view(np.uint32), tab_o.act)
+        assert np.count_nonzero(lat != wl.lattice) > 0
+    wl = W.active(W.sorting2d(L=64, grid=5, edge=10, seed=109))
+    _announce("act c=1 refusals", wl.L)
     sim = wl.make_cpm()
     sim.push_to_gpu()
     with pytest.raises(NotImplementedError):
-        sim.run(**run_kwargs(1, 1, 1))
+        sim.run(**run_kwargs(1, 1, 1, cell_sync=1))
+    with pytest.raises(NotImplementedError):
+        gpucpm.Batch([sim]).run(**run_kwargs(1, 1, 1))
 
 
 @pytest.mark.parametrize("small,compact,att", [("1", "1", "4"), ("0", "1", "1"), ("1", "0", "1")])

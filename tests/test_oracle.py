@@ -559,8 +559,8 @@ def test_act_energy_closed_form(d):
 def test_act_sweep_keeps_tables_exact_and_stamps_copies():
     """An accepted copy sets act[target] = tick (kernels2d.cu:532-534): after
     one MCS at tick m every site that changed holds act m; the tables stay
-    equal to a recount; c = 1 is refused (the source neighbourhood reaches
-    same-phase targets)."""
+    equal to a recount; with c = 1 (decided then applied per phase) likewise,
+    and the run is thread-count invariant; c = 1 with cell_sync is refused."""
     lat, act, p, _, _ = _act_setup(32, 2)
     before, act0 = lat.copy(), act.copy()
     tab = O.tally(lat, 2, 2)
@@ -575,7 +575,21 @@ def test_act_sweep_keeps_tables_exact_and_stamps_copies():
     lat2, act2, p2, _, _ = _act_setup(32, 2)
     tab2 = O.tally(lat2, 2, 2)
     with pytest.raises(ValueError):
-        O.sweep(lat2, 2, tab2, p2, 1, 1, seed=4, mcs0=0, nmcs=1)
+        O.sweep(lat2, 2, tab2, p2, 1, 1, seed=4, mcs0=0, nmcs=1, cell_sync=True)
+    for d in (2, 3):
+        outs = []
+        for nth in (1, 8):
+            l1, a1, p1, _, _ = _act_setup(16, d, seed=3)
+            before = l1.copy()
+            t1 = O.tally(l1, d, 2)
+            O.sweep(l1, d, t1, p1, 1, 1, seed=6, mcs0=40, nmcs=6, nthreads=nth)
+            area, perim = _numpy_recount(l1, d, 2)
+            np.testing.assert_array_equal(t1.area, area)
+            np.testing.assert_array_equal(t1.perim, perim)
+            assert np.count_nonzero(l1 != before) > 0
+            outs.append((l1, a1))
+        np.testing.assert_array_equal(outs[0][0], outs[1][0])
+        np.testing.assert_array_equal(outs[0][1], outs[1][1])
     # thread-count invariance with act (phase independence at c >= 2)
     la3, ac3, p3, _, _ = _act_setup(32, 2, seed=2)
     lb3, ab3 = la3.copy(), ac3.copy()
@@ -587,6 +601,35 @@ def test_act_sweep_keeps_tables_exact_and_stamps_copies():
     O.sweep(lb3, 2, tb, p3b, 2, 3, seed=9, mcs0=50, nmcs=20, nthreads=8)
     np.testing.assert_array_equal(la3, lb3)
     np.testing.assert_array_equal(ac3, ab3)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_act_single_site_checkerboard_decides_on_phase_start(d):
+    """Shared roles, c = 1, with an act term whose energy is exactly zero
+    (lambda_act = 2^-10, actMax = 1: |m dG| < 2^31, so E = 0): the two-pass
+    phases (decide on the phase-start lattice, then apply) accept exactly what
+    the act-free sweep accepts, and stamp act[t] = tick on every copy."""
+    lat, act, _, _, _ = _act_setup(16, d, seed=5)
+    lat0 = lat.copy()
+    la = np.zeros(20)
+    la[1] = la[2] = 2.0 ** -10
+    am = np.zeros(20, np.int64)
+    am[1] = am[2] = 1
+    J = [[0, 4, 4], [4, 8, 6], [4, 6, 8]]
+    p_act = O.Params.make(10, J=J, lam_area=[0, 1, 1], target_area=[0, 30, 30], lam_act=la,
+                          act_max=am, act=act)
+    p_off = O.Params.make(10, J=J, lam_area=[0, 1, 1], target_area=[0, 30, 30])
+    ta, tb = O.tally(lat, d, 2), O.tally(lat0, d, 2)
+    act0 = act.copy()
+    for m in range(4):
+        before = lat.copy()
+        O.sweep(lat, d, ta, p_act, 1, 1, seed=21, mcs0=100 + m, nmcs=1)
+        O.sweep(lat0, d, tb, p_off, 1, 1, seed=21, mcs0=100 + m, nmcs=1)
+        np.testing.assert_array_equal(lat, lat0)
+        assert (act[lat != before] == 100 + m).all()
+    np.testing.assert_array_equal(ta.area, tb.area)
+    np.testing.assert_array_equal(ta.perim, tb.perim)
+    assert np.count_nonzero(act != act0) > 0
 
 
 def test_act_cell_migrates():
