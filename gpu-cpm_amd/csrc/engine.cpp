@@ -1394,9 +1394,12 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     const bool compact = u1 == 1 && env_int("GPUCPM_COMPACT", 1) != 0;
     bool small = env_int("GPUCPM_SMALL", b->d == 3) != 0;
     for (int i = 0; i < n; i++) small = small && reps[i].k.nt <= gcpm::kSmallTypes;
-    const int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 4 : 1) : 1;
+    // Attempts drawn per thread: 1, 4 or 8 (a batch is throughput bound, so
+    // fewer, fuller blocks stage the parameters fewer times).
+    int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 8 : 4) : 1;
+    att = att >= 8 ? 8 : att >= 4 ? 4 : 1;
     const int form = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
-                     gcpm::att_form(std::min(att, 4));
+                     gcpm::att_form(att);
     // 16-bit lattices when every replica's sites fit, else all 32-bit.
     bool narrow = true;
     for (int i = 0; i < n; i++) narrow = narrow && b->sims[i]->narrow_bits() != 0;
@@ -1413,7 +1416,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
             h->narrow_valid = false;
         }
     }
-    const long per_block = (long)gcpm::kBlock * (compact ? std::min(att, 4) : 1);
+    const long per_block = (long)gcpm::kBlock * (compact ? att : 1);
     int nblocks = 0;
     for (int i = 0; i < n; i++) {
         const long nsub = 1L << (reps[i].lh * b->d);

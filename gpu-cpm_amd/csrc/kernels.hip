@@ -1686,21 +1686,21 @@ template <class T>
 static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
     const bool compact = a.form & kFormCompact;
     const bool small = a.form & kFormSmall;
-    const int att = std::min(form_att(a.form), 4);
+    const int att = form_att(a.form) >= 8 ? 8 : form_att(a.form) >= 4 ? 4 : 1;
 #define CPM_BATCH(DD, ST, AT) \
     hipLaunchKernelGGL((k_batch_phase_compact<DD, ST, AT, T>), dim3(blocks), dim3(kBlock), 0, s, a)
     if (compact) {
         if (d == 2) {
             if (small) {
-                if (att == 4) CPM_BATCH(2, false, 4); else if (att == 2) CPM_BATCH(2, false, 2); else CPM_BATCH(2, false, 1);
+                if (att == 8) CPM_BATCH(2, false, 8); else if (att == 4) CPM_BATCH(2, false, 4); else CPM_BATCH(2, false, 1);
             } else {
-                if (att == 4) CPM_BATCH(2, true, 4); else if (att == 2) CPM_BATCH(2, true, 2); else CPM_BATCH(2, true, 1);
+                if (att == 8) CPM_BATCH(2, true, 8); else if (att == 4) CPM_BATCH(2, true, 4); else CPM_BATCH(2, true, 1);
             }
         } else {
             if (small) {
-                if (att == 4) CPM_BATCH(3, false, 4); else if (att == 2) CPM_BATCH(3, false, 2); else CPM_BATCH(3, false, 1);
+                if (att == 8) CPM_BATCH(3, false, 8); else if (att == 4) CPM_BATCH(3, false, 4); else CPM_BATCH(3, false, 1);
             } else {
-                if (att == 4) CPM_BATCH(3, true, 4); else if (att == 2) CPM_BATCH(3, true, 2); else CPM_BATCH(3, true, 1);
+                if (att == 8) CPM_BATCH(3, true, 8); else if (att == 4) CPM_BATCH(3, true, 4); else CPM_BATCH(3, true, 1);
             }
         }
         return;

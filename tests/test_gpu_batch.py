@@ -170,3 +170,27 @@ def test_batch_site_width_mixed(form):
         lat_o, tab_o = run_oracle(wl, 1, 1, 20 + i, 7)
         np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
         assert_tables_equal(s.get_cell_table(), tab_o, 2)
+
+
+@pytest.mark.parametrize("att", ["1", "4", "8"])
+def test_batch_attempts_per_thread(att, monkeypatch):
+    """The compacted batch kernel with 1, 4 or 8 attempts drawn per thread
+    (each replica's block group covers its sub-blocks in chunks of att x 256)
+    equals each replica's oracle run."""
+    import gpucpm
+    monkeypatch.setenv("GPUCPM_ATT", att)
+    wls = _small_batch()
+    print("CASE batch att", att, [w.L for w in wls], flush=True)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(300 + i)
+        sims.append(s)
+    gpucpm.Batch(sims).run(**run_kwargs(1, 1, 12))
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        lat_o, tab_o = run_oracle(wl, 1, 1, 300 + i, 12)
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 2)
