@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--updates", type=int, default=1, help="updates_per_checkerboard_switch u")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the engine time its kernel forms during warmup and keep the fastest")
     ap.add_argument("--no-grad", action="store_true", help="skip the dH/dtheta check")
     ap.add_argument("--pmc-summary", nargs=6, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
                                                        "COPY_WRITE_DIR", "KERNEL", "OUT_JSON"),
@@ -119,10 +121,12 @@ def bench_batched(a, log):
     log(f"batched: {len(wls)} replicas, {sites / 1e6:.1f} M sites, {mcs} MCS; batch")
     sims = fresh()
     batch = gpucpm.Batch(sims)
+    batch.set_autotune(a.autotune)  # (its choices carry over to the timed batch)
     batch.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)  # warmup
     batch.synchronize()
     sims = fresh()
     batch = gpucpm.Batch(sims)
+    batch.set_autotune(a.autotune)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         batch.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
@@ -150,7 +154,8 @@ def bench_batched(a, log):
         "config": {"workload": f"{len(wls)} sorting replicas, L in {{128,256,512,1024}}, "
                                "J_AB in [6,14], T in [5,20], L^2/250 cells",
                    "replicas": len(wls), "sites": sites, "mcs": mcs,
-                   "positions_per_checkerboard": c, "updates_per_checkerboard_switch": u},
+                   "positions_per_checkerboard": c, "updates_per_checkerboard_switch": u,
+                   "autotune": bool(a.autotune)},
         "sequential_value": mcs * sites / t_seq,
         "speedup_vs_sequential": t_seq / t_batch,
         "bit_identical_to_sequential": d_batch == d_seq,
@@ -418,6 +423,7 @@ def main():
     sim = wl.make_cpm()
     sim.push_to_gpu()
     sim.set_seed(20240601)
+    sim.set_autotune(a.autotune)
     # shared = 1: the reference's shared-memory schedule (K = 2 colours)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
               positions_per_thread=2 * c, positions_per_checkerboard=c,
@@ -474,7 +480,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_desc, "L": wl.L, "d": wl.d, "cells": wl.ncells,
                    "mcs_per_step": a.mcs_per_step, "positions_per_checkerboard": c,
-                   "updates_per_checkerboard_switch": u, "seed": 20240601},
+                   "updates_per_checkerboard_switch": u, "seed": 20240601,
+                   "autotune": bool(a.autotune)},
         "mcs_per_s": mcs / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

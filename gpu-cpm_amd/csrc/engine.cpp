@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -55,6 +57,111 @@ struct Plan {
     size_t lds_bytes = 0;  // dynamic LDS per block (kLds2d)
 };
 constexpr int kMaxThr = 1 << 22;
+
+// Online autotuner of the per-phase kernel form (cpm_set_autotune).  Every
+// form computes the same results, so the tuner times a run's own launches:
+// while a workload key is unsettled each launch takes the candidate with the
+// fewest samples and is bracketed by events; the timings are read when the
+// run ends, and once every candidate has kSamples the fastest mean is kept
+// for that key.  Samples carry over between runs (interactive runs of one MCS
+// settle after a few calls) and between engines: the table is process-wide,
+// the pending timings belong to one engine or batch.
+struct FormTuner {
+    static constexpr int kSamples = 8;
+    struct Entry {
+        std::vector<int> forms;
+        std::vector<double> ms;
+        std::vector<int> n;
+        int best = -1;  // the chosen form, or -1 while sampling
+    };
+    struct Pending {
+        Entry* e;
+        int idx;
+        hipEvent_t e0, e1;
+    };
+    static std::map<uint64_t, Entry>& table() {
+        static std::map<uint64_t, Entry> t;
+        return t;
+    }
+    static std::mutex& lock() {
+        static std::mutex m;
+        return m;
+    }
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+
+    Entry& entry(uint64_t key, const std::vector<int>& forms) {
+        std::lock_guard<std::mutex> g(lock());
+        Entry& e = table()[key];
+        if (e.forms.empty()) {
+            e.forms = forms;
+            e.ms.assign(forms.size(), 0.0);
+            e.n.assign(forms.size(), 0);
+            if (forms.size() == 1) e.best = forms[0];
+        }
+        return e;
+    }
+    // Candidate for the next launch, or -1 when every candidate already has
+    // its samples (recorded or pending).
+    int pick(const Entry& e) const {
+        std::lock_guard<std::mutex> g(lock());
+        int idx = -1, least = kSamples;
+        for (size_t i = 0; i < e.forms.size(); i++) {
+            int c = e.n[i];
+            for (const Pending& p : pending)
+                if (p.e == &e && p.idx == (int)i) c++;
+            if (c < least) {
+                least = c;
+                idx = (int)i;
+            }
+        }
+        return idx;
+    }
+    hipError_t events(hipEvent_t* e0, hipEvent_t* e1) {
+        while (pool.size() < used + 2) {
+            hipEvent_t ev;
+            hipError_t err = hipEventCreate(&ev);
+            if (err != hipSuccess) return err;
+            pool.push_back(ev);
+        }
+        *e0 = pool[used];
+        *e1 = pool[used + 1];
+        used += 2;
+        return hipSuccess;
+    }
+    // Read the pending timings (waits for their launches) and settle the
+    // entries that have all their samples.
+    hipError_t settle() {
+        if (pending.empty()) return hipSuccess;
+        hipError_t err = hipEventSynchronize(pending.back().e1);
+        if (err != hipSuccess) return err;
+        std::lock_guard<std::mutex> g(lock());
+        for (const Pending& p : pending) {
+            float ms = 0.0f;
+            if ((err = hipEventElapsedTime(&ms, p.e0, p.e1)) != hipSuccess) return err;
+            p.e->ms[p.idx] += ms;
+            p.e->n[p.idx]++;
+        }
+        for (const Pending& p : pending) {
+            Entry& e = *p.e;
+            if (e.best >= 0) continue;
+            bool done = true;
+            for (int c : e.n) done = done && c >= kSamples;
+            if (!done) continue;
+            size_t bi = 0;
+            for (size_t i = 1; i < e.forms.size(); i++)
+                if (e.ms[i] / e.n[i] < e.ms[bi] / e.n[bi]) bi = i;
+            e.best = e.forms[bi];
+        }
+        pending.clear();
+        used = 0;
+        return hipSuccess;
+    }
+    ~FormTuner() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
 
 // Philox4x32-10 on the host (Salmon et al., SC'11), for draws made outside
 // the sweep (initial preferred directions).
@@ -169,6 +276,8 @@ struct cpm_engine {
     int scale = 1;
     int nt = 1;  // types present in the lattice at the last push
     int last_kernel = -1;  // per-phase kernel of the last run (gcpm::phase_kernel_name)
+    bool autotune = false;  // cpm_set_autotune
+    FormTuner tuner;
 
     // Kernel timing (cpm_timing_begin/end): one event pair per sweep launch.
     bool timing = false;
@@ -555,6 +664,8 @@ struct cpm_batch {
     int* d_rep_of = nullptr;
     int rep_of_cap = 0;
     int last_blocks = 0;
+    bool autotune = false;  // cpm_batch_set_autotune
+    FormTuner tuner;
 
     ~cpm_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -1107,6 +1218,35 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             pa.form |= gcpm::att_form(att);
         }
         pa.lstride = k.lstride;
+        // Autotuner: the candidate forms of this workload (all compute the
+        // same results; the special launches -- cell_sync, split phases, the
+        // reference stream, the LDS box form -- are not tuned).
+        FormTuner::Entry* tune = nullptr;
+        if (h->autotune && !a->cell_sync && !split && !k.refstream && !pa.box3 && !h->timing) {
+            const bool sm = h->nt <= gcpm::kSmallTypes;
+            std::vector<int> forms;
+            const int C = gcpm::kFormCompact, SM = gcpm::kFormSmall;
+            if (u == 1 && h->d == 2) {
+                forms = {C, C | gcpm::kFormAtt2, C | gcpm::kFormAtt4};
+                if (sm) forms.push_back(C | SM);
+            } else if (u == 1) {
+                forms = {C};
+                if (sm) forms.insert(forms.end(), {C | SM | gcpm::kFormAtt4, C | SM | gcpm::kFormAtt8});
+            } else {
+                forms = {0};
+                if (sm) forms.push_back(SM);
+            }
+            const uint64_t key = (uint64_t)1 << 63 | (uint64_t)h->d | (uint64_t)h->lgL << 2 | (uint64_t)ilog2(c) << 8 |
+                                 (uint64_t)u << 12 | (uint64_t)k.roles << 24 | (uint64_t)sm << 25 |
+                                 (uint64_t)(bits != 0) << 26 | (uint64_t)k.chem << 27 |
+                                 (uint64_t)k.act << 28 | (uint64_t)k.pers << 29 | (uint64_t)k.S << 32;
+            tune = &h->tuner.entry(key, forms);
+            if (tune->best >= 0) {
+                pa.form = tune->best;
+                tune = nullptr;
+            }
+        }
+        const int form0 = pa.form;
         int g = 0;  // phase counter since the tables were last settled
         const bool pers = k.pers != 0;
         if (pers && !h->ring_valid)
@@ -1146,7 +1286,14 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                         }
                         hipEvent_t e0, e1;
                         if (int rc = h->next_events(&e0, &e1)) return rc;
+                        const int ti = tune ? h->tuner.pick(*tune) : -1;
+                        if (ti >= 0) {
+                            pa.form = tune->forms[ti];
+                            HIP_TRY(h->tuner.events(&e0, &e1));
+                            h->tuner.pending.push_back({tune, ti, e0, e1});
+                        }
                         HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1, &h->last_kernel));
+                        pa.form = form0;
                         if (h->timing) {
                             h->t_launches++;
                             h->t_updates += (int64_t)pa.nsub * pa.natt;
@@ -1166,6 +1313,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         // Fold the last phase's deltas into the lagging base: both bases
         // then hold the current state and both delta buffers are zero.
         HIP_TRY(gcpm::launch_settle(h->d_tab, h->d_del, g, h->n, h->stream));
+        HIP_TRY(h->tuner.settle());
     } else {
         if (int rc = h->ensure_wide()) return rc;
         h->narrow_valid = false;
@@ -1316,6 +1464,12 @@ int cpm_set_launch(cpm_handle h, int form, int tile) {
     return CPM_OK;
 }
 
+int cpm_set_autotune(cpm_handle h, int on) {
+    if (int rc = check_handle(h)) return rc;
+    h->autotune = on != 0;
+    return CPM_OK;
+}
+
 int cpm_set_rng(cpm_handle h, int mode) {
     if (int rc = check_handle(h)) return rc;
     if (mode != 0 && mode != 1) return fail(CPM_EINVAL, "rng mode must be 0 (counter stream) or 1 (reference stream)");
@@ -1398,8 +1552,8 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     // fewer, fuller blocks stage the parameters fewer times).
     int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 8 : 4) : 1;
     att = att >= 8 ? 8 : att >= 4 ? 4 : 1;
-    const int form = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
-                     gcpm::att_form(att);
+    const int form_default = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
+                             gcpm::att_form(att);
     // 16-bit lattices when every replica's sites fit, else all 32-bit.
     bool narrow = true;
     for (int i = 0; i < n; i++) narrow = narrow && b->sims[i]->narrow_bits() != 0;
@@ -1416,38 +1570,85 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
             h->narrow_valid = false;
         }
     }
-    const long per_block = (long)gcpm::kBlock * (compact ? att : 1);
-    int nblocks = 0;
-    for (int i = 0; i < n; i++) {
-        const long nsub = 1L << (reps[i].lh * b->d);
-        // compacted: the fold strides over the cells, so blocks follow the draws
-        const long nb = compact ? std::max(1L, (nsub + per_block - 1) / per_block)
-                                : (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
-        reps[i].block0 = nblocks;
-        reps[i].nblocks = (int)nb;
-        reps[i].mcs0 = (uint32_t)b->sims[i]->mcs;
-        nblocks += reps[i].nblocks;
+    // Autotuner (cpm_batch_set_autotune): attempts per thread 1 / 4 / 8, with
+    // staged or (at most four types) scalar parameters; every form computes
+    // the same results.
+    FormTuner::Entry* tune = nullptr;
+    int form = form_default;
+    if (b->autotune && compact && !a->cell_sync) {
+        bool sm = true;
+        for (int i = 0; i < n; i++) sm = sm && reps[i].k.nt <= gcpm::kSmallTypes;
+        std::vector<int> forms;
+        for (int t : {1, 4, 8}) {
+            forms.push_back(gcpm::kFormCompact | gcpm::att_form(t));
+            if (sm) forms.push_back(gcpm::kFormCompact | gcpm::kFormSmall | gcpm::att_form(t));
+        }
+        uint64_t key = (uint64_t)b->d | (uint64_t)n << 2 | (uint64_t)reps[0].k.lc << 20 |
+                       (uint64_t)reps[0].k.roles << 24 | (uint64_t)narrow << 25 | (uint64_t)sm << 26;
+        for (int i = 0; i < n; i++) key = key * 1000003u + (uint64_t)reps[i].lh;
+        tune = &b->tuner.entry(key, forms);
+        if (tune->best >= 0) {
+            form = tune->best;
+            tune = nullptr;
+        }
     }
-    std::vector<int> rep_of(nblocks);
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < reps[i].nblocks; j++) rep_of[reps[i].block0 + j] = i;
-    if (nblocks > b->rep_of_cap) {
+    // Block layouts: each replica's blocks cover its sub-blocks, att x 256
+    // per block when compacted (one layout per attempts-per-thread value in
+    // use, stacked in d_reps / d_rep_of).
+    std::vector<int> atts;
+    if (tune) atts = {1, 4, 8};
+    else atts = {compact ? gcpm::form_att(form) : 1};
+    std::vector<gcpm::ReplicaDesc> all_reps;
+    std::vector<int> rep_of, lay_blocks, lay_rep_of0;
+    for (int t : atts) {
+        const long per_block = (long)gcpm::kBlock * t;
+        int nb_total = 0;
+        lay_rep_of0.push_back((int)rep_of.size());
+        for (int i = 0; i < n; i++) {
+            const long nsub = 1L << (reps[i].lh * b->d);
+            // compacted: the fold strides over the cells, so blocks follow the draws
+            const long nb = compact ? std::max(1L, (nsub + per_block - 1) / per_block)
+                                    : (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
+            reps[i].block0 = nb_total;
+            reps[i].nblocks = (int)nb;
+            reps[i].mcs0 = (uint32_t)b->sims[i]->mcs;
+            nb_total += reps[i].nblocks;
+            for (int j = 0; j < reps[i].nblocks; j++) rep_of.push_back(i);
+        }
+        lay_blocks.push_back(nb_total);
+        all_reps.insert(all_reps.end(), reps.begin(), reps.end());
+    }
+    const int total_blocks = (int)rep_of.size();
+    if (total_blocks > b->rep_of_cap) {
         (void)hipFree(b->d_rep_of);
         b->d_rep_of = nullptr;
-        HIP_TRY(hipMalloc(&b->d_rep_of, sizeof(int) * nblocks));
-        b->rep_of_cap = nblocks;
+        HIP_TRY(hipMalloc(&b->d_rep_of, sizeof(int) * total_blocks));
+        b->rep_of_cap = total_blocks;
     }
+    auto layout_of = [&](int f) {
+        const int t = compact ? gcpm::form_att(f) : 1;
+        for (size_t li = 0; li < atts.size(); li++)
+            if (atts[li] == t) return (int)li;
+        return 0;
+    };
+    const int lay0 = layout_of(form);
+    int nblocks = lay_blocks[lay0];
     b->last_blocks = nblocks;
     for (int i = 0; i < n; i++) {
         HIP_TRY(hipEventRecord(b->ready[i], b->sims[i]->stream));
         HIP_TRY(hipStreamWaitEvent(b->stream, b->ready[i], 0));
     }
-    HIP_TRY(hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipMemcpyAsync(b->d_reps, reps.data(), sizeof(gcpm::ReplicaDesc) * n, hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));  // reps and rep_of are host temporaries
+    HIP_TRY(hipMemcpyAsync(b->d_rep_of, rep_of.data(), sizeof(int) * total_blocks, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_reps, all_reps.data(), sizeof(gcpm::ReplicaDesc) * all_reps.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));  // the layouts are host temporaries
     gcpm::BatchPhaseArgs pa{};
-    pa.reps = b->d_reps;
-    pa.rep_of = b->d_rep_of;
+    auto set_layout = [&](int li) {
+        pa.reps = b->d_reps + (size_t)li * n;
+        pa.rep_of = b->d_rep_of + lay_rep_of0[li];
+        nblocks = lay_blocks[li];
+    };
+    set_layout(lay0);
     pa.form = form;
     pa.narrow = narrow ? 1 : 0;
     const gcpm::SweepConsts& k0 = reps[0].k;
@@ -1478,7 +1679,21 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                         HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
                         pa.sync_pass = 2;
                     }
+                    const int ti = tune ? b->tuner.pick(*tune) : -1;
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    if (ti >= 0) {
+                        pa.form = tune->forms[ti];
+                        set_layout(layout_of(pa.form));
+                        HIP_TRY(b->tuner.events(&e0, &e1));
+                        b->tuner.pending.push_back({tune, ti, e0, e1});
+                        HIP_TRY(hipEventRecord(e0, b->stream));
+                    }
                     HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
+                    if (ti >= 0) {
+                        HIP_TRY(hipEventRecord(e1, b->stream));
+                        pa.form = form;
+                        set_layout(lay0);
+                    }
                 }
             }
         }
@@ -1496,6 +1711,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
         }
     }
     for (int i = 0; i < n; i++) b->sims[i]->sync_tag = tag;
+    HIP_TRY(b->tuner.settle());
     for (int i = 0; i < n; i++)
         HIP_TRY(gcpm::launch_settle(b->sims[i]->d_tab, b->sims[i]->d_del, g, b->sims[i]->n, b->stream));
     HIP_TRY(hipEventRecord(b->done, b->stream));
@@ -1539,7 +1755,9 @@ int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
             return bail(err, "hipEventCreate");
     if ((err = hipMalloc(&b->d_bars, 2 * sizeof(unsigned int) * n)) != hipSuccess) return bail(err, "hipMalloc(barriers)");
     if ((err = hipMemset(b->d_bars, 0, 2 * sizeof(unsigned int) * n)) != hipSuccess) return bail(err, "hipMemset(barriers)");
-    if ((err = hipMalloc(&b->d_reps, sizeof(gcpm::ReplicaDesc) * n)) != hipSuccess) return bail(err, "hipMalloc(replicas)");
+    // three block layouts at most (the autotuner's attempts-per-thread values)
+    if ((err = hipMalloc(&b->d_reps, sizeof(gcpm::ReplicaDesc) * n * 3)) != hipSuccess)
+        return bail(err, "hipMalloc(replicas)");
     *out = b;
     return CPM_OK;
 }
@@ -1642,6 +1860,12 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         HIP_TRY(hipStreamWaitEvent(b->sims[i]->stream, b->done, 0));
         b->sims[i]->mcs += (uint64_t)(chunks * a->inner_iterations);
     }
+    return CPM_OK;
+}
+
+int cpm_batch_set_autotune(cpm_batch_handle b, int on) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    b->autotune = on != 0;
     return CPM_OK;
 }
 

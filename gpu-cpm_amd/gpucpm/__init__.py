@@ -314,6 +314,11 @@ class Cpm:
         kernel = lib.cpm_last_kernel(self._h).decode()
         return {"form": f.value, "kernel": kernel, "blocks": b.value, "tile": t.value}
 
+    def set_autotune(self, on=True):
+        """Time each candidate kernel form on this workload's own launches and
+        keep the fastest (all forms compute the same results)."""
+        check(lib.cpm_set_autotune(self._h, int(bool(on))))
+
     _RNG = {"philox": 0, "minstd": 1}
 
     def set_rng(self, name):
@@ -384,6 +389,11 @@ class Batch:
 
     def synchronize(self):
         check(lib.cpm_batch_sync(self._h))
+
+    def set_autotune(self, on=True):
+        """Time the candidate kernel forms (attempts per thread, staged or
+        scalar parameters) on this batch's own launches and keep the fastest."""
+        check(lib.cpm_batch_set_autotune(self._h, int(bool(on))))
 
     def set_form(self, form):
         """0: one launch per colour phase for all replicas (default);

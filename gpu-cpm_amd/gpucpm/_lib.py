@@ -54,6 +54,8 @@ SIGNATURES = [
     ("cpm_get_centroids", _I, [_H, _P(ctypes.c_double), _I, _P(_I)]),
     ("cpm_get_cell_table", _I, [_H, _P(ctypes.c_int32), _P(ctypes.c_int32), _P(ctypes.c_int32),
                                 _P(ctypes.c_int64)]),
+    ("cpm_set_autotune", _I, [_H, _I]),
+    ("cpm_batch_set_autotune", _I, [_H, _I]),
     ("cpm_set_rng", _I, [_H, _I]),
     ("cpm_get_rng", _I, [_H, _P(_I)]),
     ("cpm_set_seed", _I, [_H, ctypes.c_uint64]),

@@ -169,6 +169,14 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
  * the region's first site index + the low 32 bits of the seed, consumed in
  * the reference's draw order; acceptance u < ceil(exp(-E/T) (2^31 - 1)).
  * Single engines only (a batch returns CPM_ENOTIMPL). */
+/* Extension: online autotuning of the per-phase kernel form (no counterpart
+ * in the reference, whose kernel is fixed at build time).  Every form
+ * computes the same results, so while a workload (dimension, size, schedule,
+ * types, terms) is unsettled each launch takes the candidate form with the
+ * fewest samples, timed with events; after 8 samples of each the fastest is
+ * kept for later runs of that workload (the choices are shared by the
+ * engines of one process).  Off by default. */
+int cpm_set_autotune(cpm_handle h, int on);
 int cpm_set_rng(cpm_handle h, int mode);
 int cpm_get_rng(cpm_handle h, int* mode);
 
@@ -220,6 +228,9 @@ int cpm_batch_sync(cpm_batch_handle b);
  * sub-blocks) given to each replica's group, default 1; the autotuner sets
  * it.  Results never depend on it. */
 int cpm_batch_set_blocks(cpm_batch_handle b, double blocks_per_work);
+/* Extension: the autotuner of cpm_set_autotune for batched runs (attempts
+ * drawn per thread 1 / 4 / 8, staged or scalar parameters). */
+int cpm_batch_set_autotune(cpm_batch_handle b, int on);
 /* Batch launch form: 0 = one launch per colour phase, 1 = persistent. */
 int cpm_batch_set_form(cpm_batch_handle b, int form);
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks);
