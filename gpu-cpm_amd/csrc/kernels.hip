@@ -10,6 +10,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 
 #include "cpm_device.h"
@@ -508,7 +509,9 @@ __device__ __forceinline__ int64_t pers_energy(const SweepConsts& k, int dir, ui
     return (m * g + ((int64_t)1 << 23)) >> 24;
 }
 
-template <int D, bool CO, class Lat, class PV>
+// XT = false leaves out the chemotaxis, act and persistence terms (the
+// caller launches it only when none is on): fewer registers in 3D.
+template <int D, bool CO, bool XT = true, class Lat, class PV>
 __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts& k, const PV& P,
                                                const uint32_t* __restrict__ thr,
                                                const PhaseTables& T, const CellSync& cy,
@@ -578,9 +581,9 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         // lattice (the act term reads sites that other attempts now write)
         accept = cy.decide[cy.q] != 0;
     } else {
-        if (k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
-        if (k.act) dh += act_energy<D, CO>(k, lat, d.t, d.nv, d.dir, vs, vt, d.tick);
-        if (D == 3 && k.pers) dh += pers_energy<D>(k, d.dir, vs, vt);
+        if (XT && k.chem) dh += chem_energy<D>(k, lat, d.t, d.dir, ts, tt);
+        if (XT && k.act) dh += act_energy<D, CO>(k, lat, d.t, d.nv, d.dir, vs, vt, d.tick);
+        if (XT && D == 3 && k.pers) dh += pers_energy<D>(k, d.dir, vs, vt);
         accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
     }
     if (cy.pass == 3) {  // split phase, decide: record, write nothing else
@@ -599,7 +602,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     }
 
     lat.put(d.t, vs);  // kernels2d.cu:548
-    if (k.act) st<CO>(k.act_lat + lat.gindex(d.t), d.tick);  // kernels2d.cu:532-534
+    if (XT && k.act) st<CO>(k.act_lat + lat.gindex(d.t), d.tick);  // kernels2d.cu:532-534
     // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums are
     // rebuilt from the lattice when read.
     if (cs) {
@@ -898,7 +901,7 @@ struct CompactPhase {
 // Thread t of block b draws the attempts of same-colour sub-blocks
 // (b ATT + r) blockDim + t for r < ATT (consecutive threads, consecutive
 // sub-blocks), so the block's queue collects up to ATT blockDim attempts.
-template <int D, bool STAGE, bool FULL, int ATT, class ST>
+template <int D, bool STAGE, bool FULL, int ATT, class ST, bool XT = true>
 __device__ __forceinline__ void compact_phase(const SweepConsts& k, const CompactPhase& a) {
     static_assert(!FULL || ATT == 1, "the full-neighbourhood queue holds one attempt per thread");
     __shared__ LdsParams sp;
@@ -1045,11 +1048,11 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
         const CellSync cy{a.sync_pass, a.claims, a.sync_tag};
         if constexpr (STAGE) {
             const LdsParamView P{sp.p, k.nt, k.nt * k.nt};
-            finish_attempt<D, false>(lat, k, P, thr_lds ? sp.thr : k.thr, T, cy, sb, d);
+            finish_attempt<D, false, XT>(lat, k, P, thr_lds ? sp.thr : k.thr, T, cy, sb, d);
         } else {
             SmallParams P;
             P.load(k.prm);
-            finish_attempt<D, false>(lat, k, P, k.thr, T, cy, sb, d);
+            finish_attempt<D, false, XT>(lat, k, P, k.thr, T, cy, sb, d);
         }
     }
     if (fold) {
@@ -1068,13 +1071,13 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
 }
 
 
-template <int D, bool STAGE, bool FULL, int ATT, class ST>
+template <int D, bool STAGE, bool FULL, int ATT, class ST, bool XT>
 __global__ __launch_bounds__(kBlock) void k_sweep_phase_compact(PhaseArgs a) {
     const CompactPhase cp{a.nsub, a.lh, a.lstride, a.cx, a.cy, a.cz, a.g, a.phase, a.mcs,
                           a.sync_pass, a.claims, a.sync_tag, (int)blockIdx.x,
                           (int)(blockIdx.x * blockDim.x + threadIdx.x),
                           (int)(gridDim.x * blockDim.x)};
-    compact_phase<D, STAGE, FULL, ATT, ST>(a.k, cp);
+    compact_phase<D, STAGE, FULL, ATT, ST, XT>(a.k, cp);
 }
 
 // Batched replicas, one colour phase, compacted: block b serves replica
@@ -1550,35 +1553,43 @@ enum PhaseKernel {
     kPkCompact2Lds8, kPkRef2, kPkRef3, kPkCount
 };
 
-// Kernels on the 16-bit lattice have ids kPkNarrow + id.
+// Kernels on the 16-bit lattice have ids kPkNarrow + id; the 3D scalar
+// compacted kernels without the extra terms kPkLean + that.
 constexpr int kPkNarrow = 64;
+constexpr int kPkLean = 128;
 
 const char* phase_kernel_name(int id) {
-#define CPM_NAMES(ST)                                                                              \
-    {"k_sweep_phase3_box",                                                                         \
-     "k_sweep_phase_compact<2, false, false, 1, " ST ">",                                         \
-     "k_sweep_phase_compact<2, false, true, 1, " ST ">",                                          \
-     "k_sweep_phase_compact<2, true, false, 1, " ST ">",                                          \
-     "k_sweep_phase_compact<2, true, true, 1, " ST ">",                                           \
-     "k_sweep_phase_compact<3, false, false, 1, " ST ">",                                         \
-     "k_sweep_phase_compact<3, true, false, 1, " ST ">",                                          \
-     "k_sweep_phase<2, " ST ">", "k_sweep_phase_lds<2, " ST ">",                                  \
-     "k_sweep_phase<3, " ST ">", "k_sweep_phase_lds<3, " ST ">",                                  \
-     "k_sweep_phase_compact<2, true, false, 2, " ST ">",                                          \
-     "k_sweep_phase_compact<2, true, false, 4, " ST ">",                                          \
-     "k_sweep_phase_compact<3, false, false, 2, " ST ">",                                         \
-     "k_sweep_phase_compact<3, false, false, 4, " ST ">",                                         \
-     "k_sweep_phase_compact<3, false, false, 8, " ST ">",                                         \
-     "k_sweep_phase_compact<2, true, false, 8, " ST ">",                                          \
-     "k_sweep_phase_ref<2, " ST ">", "k_sweep_phase_ref<3, " ST ">"}
-    static const char* const wide[kPkCount] = CPM_NAMES("unsigned int");
-    static const char* const narrow[kPkCount] = CPM_NAMES("unsigned short");
-#undef CPM_NAMES
-    if (id >= kPkNarrow) {
-        id -= kPkNarrow;
-        return id > kPkBox3 && id < kPkCount ? narrow[id] : "";
+    static const char* const base[kPkCount] = {
+        "k_sweep_phase3_box",
+        "k_sweep_phase_compact<2, false, false, 1, %s, true>",
+        "k_sweep_phase_compact<2, false, true, 1, %s, true>",
+        "k_sweep_phase_compact<2, true, false, 1, %s, true>",
+        "k_sweep_phase_compact<2, true, true, 1, %s, true>",
+        "k_sweep_phase_compact<3, false, false, 1, %s, true>",
+        "k_sweep_phase_compact<3, true, false, 1, %s, true>",
+        "k_sweep_phase<2, %s>", "k_sweep_phase_lds<2, %s>",
+        "k_sweep_phase<3, %s>", "k_sweep_phase_lds<3, %s>",
+        "k_sweep_phase_compact<2, true, false, 2, %s, true>",
+        "k_sweep_phase_compact<2, true, false, 4, %s, true>",
+        "k_sweep_phase_compact<3, false, false, 2, %s, true>",
+        "k_sweep_phase_compact<3, false, false, 4, %s, %s>",
+        "k_sweep_phase_compact<3, false, false, 8, %s, %s>",
+        "k_sweep_phase_compact<2, true, false, 8, %s, true>",
+        "k_sweep_phase_ref<2, %s>", "k_sweep_phase_ref<3, %s>"};
+    static char names[4][kPkCount][96];
+    static bool made = false;
+    if (!made) {
+        for (int v = 0; v < 4; v++)
+            for (int i = 0; i < kPkCount; i++)
+                snprintf(names[v][i], sizeof(names[v][i]), base[i],
+                         (v & 1) ? "unsigned short" : "unsigned int", (v & 2) ? "false" : "true");
+        made = true;
     }
-    return id >= 0 && id < kPkCount ? wide[id] : "";
+    if (id < 0) return "";
+    const int v = ((id & kPkNarrow) ? 1 : 0) | ((id & kPkLean) ? 2 : 0);
+    id &= kPkNarrow - 1;
+    if (id >= kPkCount || (id == kPkBox3 && v)) return "";
+    return id == kPkBox3 ? base[kPkBox3] : names[v][id];
 }
 
 static int choose_phase_kernel(int d, const PhaseArgs& a, bool box_ok) {
@@ -1607,7 +1618,8 @@ static int choose_phase_kernel(int d, const PhaseArgs& a, bool box_ok) {
 }
 
 template <class ST>
-static void launch_phase_st(int id, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+static void launch_phase_st(int id, const PhaseArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                            bool lean) {
     const int work = a.nsub > a.k.ncells + 1 ? a.nsub : a.k.ncells + 1;
     const int blocks = (work + kBlock - 1) / kBlock;
     // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
@@ -1621,18 +1633,24 @@ static void launch_phase_st(int id, const PhaseArgs& a, hipStream_t s, hipEvent_
         return nb > 0 ? nb : 1;
     };
     switch (id) {
-        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false, 1, ST>), blocks); break;
-        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true, 1, ST>), blocks); break;
-        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 1, ST>), blocks); break;
-        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true, 1, ST>), blocks); break;
-        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 1, ST>), blocks); break;
-        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false, 1, ST>), blocks); break;
-        case kPkCompact2Lds2: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 2, ST>), cblocks(2)); break;
-        case kPkCompact2Lds4: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 4, ST>), cblocks(4)); break;
-        case kPkCompact3Small2: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 2, ST>), cblocks(2)); break;
-        case kPkCompact3Small4: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4, ST>), cblocks(4)); break;
-        case kPkCompact3Small8: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 8, ST>), cblocks(8)); break;
-        case kPkCompact2Lds8: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 8, ST>), cblocks(8)); break;
+        case kPkCompact2Small: CPM_LAUNCH((k_sweep_phase_compact<2, false, false, 1, ST, true>), blocks); break;
+        case kPkCompact2SmallFull: CPM_LAUNCH((k_sweep_phase_compact<2, false, true, 1, ST, true>), blocks); break;
+        case kPkCompact2Lds: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 1, ST, true>), blocks); break;
+        case kPkCompact2LdsFull: CPM_LAUNCH((k_sweep_phase_compact<2, true, true, 1, ST, true>), blocks); break;
+        case kPkCompact3Small: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 1, ST, true>), blocks); break;
+        case kPkCompact3Lds: CPM_LAUNCH((k_sweep_phase_compact<3, true, false, 1, ST, true>), blocks); break;
+        case kPkCompact2Lds2: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 2, ST, true>), cblocks(2)); break;
+        case kPkCompact2Lds4: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 4, ST, true>), cblocks(4)); break;
+        case kPkCompact3Small2: CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 2, ST, true>), cblocks(2)); break;
+        case kPkCompact3Small4:
+            if (lean) CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4, ST, false>), cblocks(4));
+            else CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 4, ST, true>), cblocks(4));
+            break;
+        case kPkCompact3Small8:
+            if (lean) CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 8, ST, false>), cblocks(8));
+            else CPM_LAUNCH((k_sweep_phase_compact<3, false, false, 8, ST, true>), cblocks(8));
+            break;
+        case kPkCompact2Lds8: CPM_LAUNCH((k_sweep_phase_compact<2, true, false, 8, ST, true>), cblocks(8)); break;
         case kPkPhase2Small: CPM_LAUNCH((k_sweep_phase<2, ST>), blocks); break;
         case kPkPhase2Lds: CPM_LAUNCH((k_sweep_phase_lds<2, ST>), blocks); break;
         case kPkPhase3Small: CPM_LAUNCH((k_sweep_phase<3, ST>), blocks); break;
@@ -1649,14 +1667,17 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
     const bool narrow = a.k.lat16 != nullptr;
     // the LDS box form reads the 32-bit lattice only
     const int id = choose_phase_kernel(d, a, !narrow);
-    if (kernel_id) *kernel_id = id + (narrow ? kPkNarrow : 0);
+    // without chemotaxis, act and persistence the 3D scalar kernels run lean
+    const bool lean = !a.k.chem && !a.k.act && !a.k.pers &&
+                      (id == kPkCompact3Small4 || id == kPkCompact3Small8);
+    if (kernel_id) *kernel_id = id + (narrow ? kPkNarrow : 0) + (lean ? kPkLean : 0);
     if (id == kPkBox3)
         hipExtLaunchKernelGGL(k_sweep_phase3_box, dim3((L / kBX) * (L / kBY) * (L / kBZ)), dim3(kBlock),
                               0, s, e0, e1, 0, a);
     else if (narrow)
-        launch_phase_st<uint16_t>(id, a, s, e0, e1);
+        launch_phase_st<uint16_t>(id, a, s, e0, e1, lean);
     else
-        launch_phase_st<uint32_t>(id, a, s, e0, e1);
+        launch_phase_st<uint32_t>(id, a, s, e0, e1, lean);
     return hipGetLastError();
 }
 
