@@ -332,7 +332,7 @@ def bench_gradient(a, log):
         "roofline": {"bound": "hbm", "kernel": "k_stats_pairs + k_stats_cells",
                      "achieved": out["backward_sites_per_s"] * 4 / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": out["backward_sites_per_s"] * 4 / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "note": "backward time includes the host round trip per replica"},
+                     "traffic": None, "note": "backward time includes the host side: one wait for all replicas' statistics and the per-replica energy and gradient from them"},
         "cpu_baseline": None,
     }
     line.update({k: v for k, v in out.items() if k != "gradient_rel_error"})

@@ -221,9 +221,9 @@ hipError_t launch_convert(uint32_t* wide, uint16_t* narrow, size_t sites, int id
 hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* anchor, int64_t* csum,
                        hipStream_t s);
 // pairs[20*20] and mom[20*5] must be zero on entry; ctype[n+1] is scratch.
-hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab, int ncells,
-                        unsigned long long* pairs, unsigned long long* mom, int32_t* ctype,
-                        hipStream_t s);
+hipError_t launch_stats(int d, const uint32_t* lat, const uint16_t* lat16, int idbits, int lgL,
+                        const int64_t* tab, int ncells, unsigned long long* pairs,
+                        unsigned long long* mom, int32_t* ctype, hipStream_t s);
 
 constexpr int kBlock = 256;  // threads per workgroup of the sweep kernels
 

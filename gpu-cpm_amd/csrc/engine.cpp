@@ -267,6 +267,7 @@ struct cpm_engine {
     gcpm::DevParams* d_prm = nullptr;
     unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation}
     unsigned long long* d_stats = nullptr;  // 400 pair counts + 100 moments
+    unsigned long long* h_stats = nullptr;  // their pinned host copy
     uint64_t* d_claims = nullptr;           // cell_sync claim keys [n+1]
     uint32_t sync_tag = 0;                  // cell_sync phase tag
     int32_t* d_ctype = nullptr;             // [n+1] cell types (stats scratch)
@@ -318,6 +319,7 @@ struct cpm_engine {
         (void)hipFree(d_prm);
         (void)hipFree(d_bar);
         (void)hipFree(d_stats);
+        (void)hipHostFree(h_stats);
         (void)hipFree(d_claims);
         (void)hipFree(d_ctype);
         (void)hipFree(d_thr);
@@ -1898,22 +1900,37 @@ int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks) {
 }
 
 int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments) {
-    if (int rc = check_handle(h)) return rc;
-    if (!h->pushed) return fail(CPM_ESTATE, "energy statistics exist only after push_to_gpu");
+    return cpm_energy_stats_many(&h, 1, pairs, moments);
+}
+
+int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments) {
+    if (!sims || n < 0) return fail(CPM_EINVAL, "null engine list");
     if (!pairs || !moments) return fail(CPM_EINVAL, "null output");
     const int np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5;
-    if (!h->d_stats) HIP_TRY(hipMalloc(&h->d_stats, sizeof(unsigned long long) * (np + nm)));
-    if (!h->d_ctype) HIP_TRY(hipMalloc(&h->d_ctype, sizeof(int32_t) * ((size_t)h->n + 1)));
-    if (int rc = h->ensure_wide()) return rc;
-    HIP_TRY(hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * (np + nm), h->stream));
-    HIP_TRY(gcpm::launch_stats(h->d, h->d_lat, h->lgL, h->d_tab[0], h->n, h->d_stats, h->d_stats + np,
-                               h->d_ctype, h->stream));
-    std::vector<unsigned long long> host(np + nm);
-    HIP_TRY(hipMemcpyAsync(host.data(), h->d_stats, sizeof(unsigned long long) * (np + nm),
-                           hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int i = 0; i < np; i++) pairs[i] = (int64_t)host[i];
-    for (int i = 0; i < nm; i++) moments[i] = (int64_t)host[np + i];
+    // Queue every engine's statistics on its own stream, then wait once.
+    for (int i = 0; i < n; i++) {
+        cpm_handle h = sims[i];
+        if (int rc = check_handle(h)) return rc;
+        if (!h->pushed) return fail(CPM_ESTATE, "energy statistics exist only after push_to_gpu");
+        if (!h->d_stats) HIP_TRY(hipMalloc(&h->d_stats, sizeof(unsigned long long) * (np + nm)));
+        if (!h->d_ctype) HIP_TRY(hipMalloc(&h->d_ctype, sizeof(int32_t) * ((size_t)h->n + 1)));
+        if (!h->h_stats) HIP_TRY(hipHostMalloc((void**)&h->h_stats, sizeof(unsigned long long) * (np + nm),
+                                               hipHostMallocDefault));
+        HIP_TRY(hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * (np + nm), h->stream));
+        // whichever lattice copy is current (no conversion)
+        const bool narrow = !h->wide_valid;
+        HIP_TRY(gcpm::launch_stats(h->d, h->d_lat, narrow ? h->d_lat16 : nullptr, narrow ? h->lat16_bits : 24,
+                                   h->lgL, h->d_tab[0], h->n, h->d_stats, h->d_stats + np, h->d_ctype,
+                                   h->stream));
+        HIP_TRY(hipMemcpyAsync(h->h_stats, h->d_stats, sizeof(unsigned long long) * (np + nm),
+                               hipMemcpyDeviceToHost, h->stream));
+    }
+    for (int i = 0; i < n; i++) {
+        cpm_handle h = sims[i];
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (int j = 0; j < np; j++) pairs[(size_t)i * np + j] = (int64_t)h->h_stats[j];
+        for (int j = 0; j < nm; j++) moments[(size_t)i * nm + j] = (int64_t)h->h_stats[np + j];
+    }
     return CPM_OK;
 }
 

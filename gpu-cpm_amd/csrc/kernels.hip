@@ -1475,9 +1475,11 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
 // ((dz, dy, dx) > 0: 4 offsets in 2D, 13 in 3D), counted by the type pair
 // (min, max) in an LDS histogram and added to pairs[20*20] once per block.
 // Each site also records its cell's type.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_stats_pairs(const uint32_t* __restrict__ lat, int lgL,
-                                                        unsigned long long* pairs, int32_t* ctype) {
+// ST: the 32-bit lattice (idbits 24) or its 16-bit form.
+template <int D, class ST>
+__global__ __launch_bounds__(kBlock) void k_stats_pairs(const ST* __restrict__ lat, int lgL,
+                                                        uint32_t idbits, unsigned long long* pairs,
+                                                        int32_t* ctype) {
     __shared__ unsigned int hist[kNTypes * kNTypes];
     for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x) hist[i] = 0;
     __syncthreads();
@@ -1486,8 +1488,8 @@ __global__ __launch_bounds__(kBlock) void k_stats_pairs(const uint32_t* __restri
     constexpr int NB = D == 3 ? 27 : 9;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nsites; i += gridDim.x * blockDim.x) {
         const uint32_t v = lat[i];
-        const uint32_t t = v >> 24;
-        if (v & kSpinMask) ctype[v & kSpinMask] = (int32_t)t;
+        const uint32_t t = v >> idbits, cell = v & ((1u << idbits) - 1u);
+        if (cell) ctype[cell] = (int32_t)t;
         const int x = (int)(i & Lm), y = (int)((i >> lgL) & Lm), z = D == 3 ? (int)(i >> (2 * lgL)) : 0;
 #pragma unroll
         for (int n = NB / 2 + 1; n < NB; n++) {  // offsets after the centre: the positive half
@@ -1495,7 +1497,7 @@ __global__ __launch_bounds__(kBlock) void k_stats_pairs(const uint32_t* __restri
             block_offset<D>(n, ox, oy, oz);
             const uint32_t vn = lat[site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL)];
             if (vn != v) {
-                const uint32_t tn = vn >> 24;
+                const uint32_t tn = vn >> idbits;
                 const uint32_t a = t < tn ? t : tn, b = t < tn ? tn : t;
                 atomicAdd(&hist[a * kNTypes + b], 1u);
             }
@@ -1531,15 +1533,22 @@ __global__ __launch_bounds__(kBlock) void k_stats_cells(const int64_t* __restric
         if (acc[i]) atomicAdd(&mom[i], acc[i]);
 }
 
-hipError_t launch_stats(int d, const uint32_t* lat, int lgL, const int64_t* tab, int ncells,
-                        unsigned long long* pairs, unsigned long long* mom, int32_t* ctype,
-                        hipStream_t s) {
+hipError_t launch_stats(int d, const uint32_t* lat, const uint16_t* lat16, int idbits, int lgL,
+                        const int64_t* tab, int ncells, unsigned long long* pairs,
+                        unsigned long long* mom, int32_t* ctype, hipStream_t s) {
     const uint32_t nsites = 1u << (d * lgL);
     const int blocks = (int)std::min<uint32_t>((nsites + kBlock - 1) / kBlock, 4096u);
-    if (d == 2)
-        hipLaunchKernelGGL(k_stats_pairs<2>, dim3(blocks), dim3(kBlock), 0, s, lat, lgL, pairs, ctype);
-    else
-        hipLaunchKernelGGL(k_stats_pairs<3>, dim3(blocks), dim3(kBlock), 0, s, lat, lgL, pairs, ctype);
+    const uint32_t ib = (uint32_t)idbits;
+    if (lat16) {
+        if (d == 2)
+            hipLaunchKernelGGL((k_stats_pairs<2, uint16_t>), dim3(blocks), dim3(kBlock), 0, s, lat16, lgL, ib, pairs, ctype);
+        else
+            hipLaunchKernelGGL((k_stats_pairs<3, uint16_t>), dim3(blocks), dim3(kBlock), 0, s, lat16, lgL, ib, pairs, ctype);
+    } else if (d == 2) {
+        hipLaunchKernelGGL((k_stats_pairs<2, uint32_t>), dim3(blocks), dim3(kBlock), 0, s, lat, lgL, ib, pairs, ctype);
+    } else {
+        hipLaunchKernelGGL((k_stats_pairs<3, uint32_t>), dim3(blocks), dim3(kBlock), 0, s, lat, lgL, ib, pairs, ctype);
+    }
     const int cb = std::max(1, std::min(1024, (ncells + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_stats_cells, dim3(cb), dim3(kBlock), 0, s, tab, ctype, ncells, mom);
     return hipGetLastError();

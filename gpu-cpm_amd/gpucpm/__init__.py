@@ -359,6 +359,20 @@ class Cpm:
         check(lib.cpm_set_mcs(self._h, _int(value, "mcs")))
 
 
+def energy_stats_many(sims):
+    """Cpm.energy_stats of several engines at once (each on its own stream,
+    one wait): pairs (n, 20, 20) and moments (n, 20, 5), int64."""
+    sims = list(sims)
+    n = len(sims)
+    pairs = np.zeros((n, 20, 20), np.int64)
+    mom = np.zeros((n, 20, 5), np.int64)
+    hs = (ctypes.c_void_p * max(n, 1))(*[s._h for s in sims])
+    P = ctypes.POINTER(ctypes.c_int64)
+    check(lib.cpm_energy_stats_many(ctypes.cast(hs, ctypes.POINTER(ctypes.c_void_p)), n,
+                                    pairs.ctypes.data_as(P), mom.ctypes.data_as(P)))
+    return pairs, mom
+
+
 class Batch:
     """Batched replicas: several pushed Cpm engines (same dimensionality; any
     sizes, parameters, temperatures and seeds) advanced together, by default

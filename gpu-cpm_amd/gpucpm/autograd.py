@@ -73,10 +73,14 @@ class CpmEnergy(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, theta, sims, layout):
+        from . import Cpm, energy_stats_many
         th = theta.detach().to(torch.float64).cpu().numpy()
         Hs, gs = [], []
-        for sim in sims:
-            pairs, mom = sim.energy_stats()
+        if all(isinstance(s, Cpm) for s in sims):  # engines: one batched call
+            stats = zip(*energy_stats_many(sims))
+        else:  # any object with energy_stats()
+            stats = (s.energy_stats() for s in sims)
+        for pairs, mom in stats:
             N, m = stats_to_arrays(pairs, mom, layout)
             H, g = energy_and_grad(th, N, m, layout)
             Hs.append(H)

@@ -246,6 +246,9 @@ int cpm_batch_destroy(cpm_batch_handle b);
  * k = 0: count, 1: sum A, 2: sum A^2, 3: sum P, 4: sum P^2.  H and dH/dtheta
  * for every theta follow in closed form (the backward pass of the energy). */
 int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments);
+/* Extension: cpm_energy_stats of n engines at once (each on its own stream,
+ * one wait at the end), into pairs[n * 400] and moments[n * 100]. */
+int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments);
 
 #ifdef __cplusplus
 }

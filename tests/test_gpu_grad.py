@@ -116,3 +116,29 @@ def test_contrastive_gradient_is_difference_of_means():
     nd = np.mean([s.energy_stats()[0][1, 2] for s in sims_d])
     nm = np.mean([s.energy_stats()[0][1, 2] for s in sims_m])
     assert g[k] == pytest.approx(nd - nm, rel=1e-6)
+
+
+def test_stats_many_equal_single_and_numpy():
+    """energy_stats_many (every engine's statistics queued, one wait) equals
+    energy_stats one by one and the numpy reduction, also read straight from
+    the 16-bit lattice a run leaves current."""
+    import gpucpm
+    from gpucpm import workloads as W
+    wls = [W.sorting2d(L=64, grid=5, edge=10, seed=60 + i) for i in range(3)] + \
+          [W.walled(W.sorting2d(L=128, grid=9, edge=10, seed=63))]
+    print("CASE stats many", [w.L for w in wls], flush=True)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.run(**run_kwargs(1, 1, 10), seed=30 + i)
+        sims.append(s)
+    pairs, mom = gpucpm.energy_stats_many(sims)  # before any pull: 16-bit sites current
+    for i, s in enumerate(sims):
+        p1, m1 = s.energy_stats()
+        np.testing.assert_array_equal(pairs[i], p1)
+        np.testing.assert_array_equal(mom[i], m1)
+        s.pull_from_gpu()
+        p_np, m_np = numpy_stats(s.get_state().view(np.uint32).copy(), 2)
+        np.testing.assert_array_equal(np.triu(pairs[i]), np.triu(p_np))
+        np.testing.assert_array_equal(mom[i], m_np)
