@@ -135,13 +135,18 @@ struct FormTuner {
     hipError_t settle() {
         if (pending.empty()) return hipSuccess;
         hipError_t err = hipEventSynchronize(pending.back().e1);
-        if (err != hipSuccess) return err;
         std::lock_guard<std::mutex> g(lock());
-        for (const Pending& p : pending) {
+        for (size_t i = 0; i < pending.size() && err == hipSuccess; i++) {
+            const Pending& p = pending[i];
             float ms = 0.0f;
-            if ((err = hipEventElapsedTime(&ms, p.e0, p.e1)) != hipSuccess) return err;
+            if ((err = hipEventElapsedTime(&ms, p.e0, p.e1)) != hipSuccess) break;
             p.e->ms[p.idx] += ms;
             p.e->n[p.idx]++;
+        }
+        if (err != hipSuccess) {  // drop this run's samples rather than retry them
+            pending.clear();
+            used = 0;
+            return err;
         }
         for (const Pending& p : pending) {
             Entry& e = *p.e;
@@ -1292,9 +1297,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                         if (ti >= 0) {
                             pa.form = tune->forms[ti];
                             HIP_TRY(h->tuner.events(&e0, &e1));
-                            h->tuner.pending.push_back({tune, ti, e0, e1});
                         }
                         HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, e0, e1, &h->last_kernel));
+                        if (ti >= 0) h->tuner.pending.push_back({tune, ti, e0, e1});
                         pa.form = form0;
                         if (h->timing) {
                             h->t_launches++;
@@ -1687,12 +1692,12 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                         pa.form = tune->forms[ti];
                         set_layout(layout_of(pa.form));
                         HIP_TRY(b->tuner.events(&e0, &e1));
-                        b->tuner.pending.push_back({tune, ti, e0, e1});
                         HIP_TRY(hipEventRecord(e0, b->stream));
                     }
                     HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
                     if (ti >= 0) {
                         HIP_TRY(hipEventRecord(e1, b->stream));
+                        b->tuner.pending.push_back({tune, ti, e0, e1});
                         pa.form = form;
                         set_layout(lay0);
                     }
