@@ -1374,27 +1374,59 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
     }
 }
 
-// Area and perimeter recount (kernels2d.cu:5-41): one thread per site.
+// Wave-aggregated updates of per-cell rows: the lanes of a wave are grouped
+// by cell (the first pending lane's cell, then the next, ...), each group's
+// contributions are summed across the wave and its first lane makes one
+// atomic.  Neighbouring sites mostly share a cell, so a wave of 64 sites
+// makes a handful of atomics instead of 64.  Every lane of the wave must call
+// it (the loop and the shuffles are wave-wide).
+__device__ __forceinline__ int64_t wave_sum(int64_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o));
+    return x;
+}
+
+// Area and perimeter recount (kernels2d.cu:5-41): one thread per site, the
+// per-cell adds aggregated per wave.
 template <int D>
 __global__ __launch_bounds__(256) void k_tally_area_perim(const uint32_t* __restrict__ lat, int lgL,
                                                           int64_t* tab, uint32_t* first) {
     const uint32_t nsites = 1u << (D * lgL);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsites) return;
-    const uint32_t c = lat[i] & kSpinMask;
-    if (!c) return;
-    const int Lm = (1 << lgL) - 1;
-    const int x = (int)(i & Lm), y = (int)((i >> lgL) & Lm), z = D == 3 ? (int)(i >> (2 * lgL)) : 0;
-    constexpr int NB = D == 3 ? 27 : 9;
-    int cnt = 0;
+    const uint32_t c = i < nsites ? lat[i] & kSpinMask : 0u;
+    int64_t add = 0;
+    if (c) {
+        const int Lm = (1 << lgL) - 1;
+        const int x = (int)(i & Lm), y = (int)((i >> lgL) & Lm), z = D == 3 ? (int)(i >> (2 * lgL)) : 0;
+        constexpr int NB = D == 3 ? 27 : 9;
+        int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < NB; k++) {
-        int ox, oy, oz;
-        block_offset<D>(k, ox, oy, oz);
-        cnt += (lat[site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL)] & kSpinMask) != c;
+        for (int k = 0; k < NB; k++) {
+            int ox, oy, oz;
+            block_offset<D>(k, ox, oy, oz);
+            cnt += (lat[site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL)] & kSpinMask) != c;
+        }
+        add = (int64_t)cnt * 4294967296LL + 1;
     }
-    atomic_add_i64(tab + c, (int64_t)cnt * 4294967296LL + 1);
-    atomicMin(first + c, i);
+    const int lane = (int)__lane_id();
+    uint64_t pending = __ballot(c != 0u);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lc = (uint32_t)__shfl((int)c, leader);
+        const bool mine = ((pending >> lane) & 1ull) && c == lc;
+        const int64_t sum = wave_sum(mine ? add : 0);
+        const uint32_t fmin = wave_min(mine ? i : 0xFFFFFFFFu);
+        if (lane == leader) {
+            atomic_add_i64(tab + lc, sum);
+            atomicMin(first + lc, fmin);
+        }
+        pending &= ~__ballot(mine);
+    }
 }
 
 __global__ void k_anchor(const uint32_t* __restrict__ first, int32_t* anchor, int ncells, int lgL, int d) {
@@ -1413,15 +1445,30 @@ __global__ __launch_bounds__(256) void k_tally_csum(const uint32_t* __restrict__
                                                     const int32_t* __restrict__ anchor, int64_t* csum) {
     const uint32_t nsites = 1u << (D * lgL);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsites) return;
-    const uint32_t c = lat[i] & kSpinMask;
-    if (!c) return;
+    const uint32_t c = i < nsites ? lat[i] & kSpinMask : 0u;
     const int L = 1 << lgL, Lm = L - 1;
     const int pos[3] = {(int)(i & Lm), (int)((i >> lgL) & Lm), D == 3 ? (int)(i >> (2 * lgL)) : 0};
+    int64_t w[3] = {0, 0, 0};
+    if (c) {
 #pragma unroll
-    for (int ax = 0; ax < D; ax++) {
-        const int v = pos[ax] - anchor[3 * c + ax];
-        atomic_add_i64(csum + 3 * c + ax, (int64_t)(((v + L / 2) & Lm) - L / 2));
+        for (int ax = 0; ax < D; ax++) {
+            const int v = pos[ax] - anchor[3 * c + ax];
+            w[ax] = (int64_t)(((v + L / 2) & Lm) - L / 2);
+        }
+    }
+    // per-wave sums by cell (see k_tally_area_perim)
+    const int lane = (int)__lane_id();
+    uint64_t pending = __ballot(c != 0u);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lc = (uint32_t)__shfl((int)c, leader);
+        const bool mine = ((pending >> lane) & 1ull) && c == lc;
+#pragma unroll
+        for (int ax = 0; ax < D; ax++) {
+            const int64_t sum = wave_sum(mine ? w[ax] : 0);
+            if (lane == leader) atomic_add_i64(csum + 3 * lc + ax, sum);
+        }
+        pending &= ~__ballot(mine);
     }
 }
 
