@@ -190,6 +190,7 @@ struct BatchPhaseArgs {
     int att0, natt;         // attempt indices of this launch
     int form;               // kForm* bits (compact, small, attempts per thread)
     int narrow;             // every replica on its 16-bit lattice (else all on 32-bit)
+    int refstream;          // every replica draws from its reference stream (not compacted)
 };
 
 // Host-side launchers (kernels.hip).

@@ -1552,7 +1552,8 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
         return e ? std::atoi(e) : dflt;
     };
     const int u1 = a->updates_per_checkerboard_switch;
-    const bool compact = u1 == 1 && env_int("GPUCPM_COMPACT", 1) != 0;
+    const bool refstream = reps[0].k.refstream != 0;  // (all replicas alike)
+    const bool compact = u1 == 1 && !refstream && env_int("GPUCPM_COMPACT", 1) != 0;
     bool small = env_int("GPUCPM_SMALL", b->d == 3) != 0;
     for (int i = 0; i < n; i++) small = small && reps[i].k.nt <= gcpm::kSmallTypes;
     // Attempts drawn per thread: 1, 4 or 8 (a batch is throughput bound, so
@@ -1658,6 +1659,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     set_layout(lay0);
     pa.form = form;
     pa.narrow = narrow ? 1 : 0;
+    pa.refstream = refstream ? 1 : 0;
     const gcpm::SweepConsts& k0 = reps[0].k;
     const int rounds = k0.rounds;
     pa.lstride = k0.lstride;
@@ -1667,6 +1669,9 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
         pa.m = (uint32_t)m;
         if (narrow)  // (a chunk end may have refreshed them)
             for (int i = 0; i < n; i++) b->sims[i]->wide_valid = false;
+        if (refstream && m % a->inner_iterations == 0)  // each chunk reseeds, as in cpm_run
+            for (int i = 0; i < n; i++)
+                HIP_TRY(gcpm::launch_ref_seed(b->d, reps[i].k, (uint32_t)b->sims[i]->seed, b->stream));
         for (int r = 0; r < rounds; r++) {
             for (int pc = 0; pc < k0.phases_per_round; pc++, g++) {
                 phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
@@ -1780,9 +1785,25 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     std::vector<gcpm::ReplicaDesc> reps(n);
     std::vector<long> need(n);
     for (int i = 0; i < n; i++) {
-        if (b->sims[i]->rng != 0)
-            return fail(CPM_ENOTIMPL, "the reference stream (cpm_set_rng) runs single engines only");
-        if (int rc = prepare_run(b->sims[i], a, &reps[i].k, &reps[i].lh)) return rc;
+        cpm_handle h = b->sims[i];
+        if (h->rng != b->sims[0]->rng)
+            return fail(CPM_EINVAL, "the engines of a batch must use the same random stream (cpm_set_rng)");
+        if (int rc = prepare_run(h, a, &reps[i].k, &reps[i].lh)) return rc;
+        if (h->rng == 1) {  // reference streams: one state per region, as in cpm_run
+            if (!reps[i].k.roles && a->positions_per_thread != 2 * a->positions_per_checkerboard)
+                return fail(CPM_ENOTIMPL,
+                            "the reference stream with shared = 1 needs positions_per_thread = 2 * "
+                            "positions_per_checkerboard");
+            const size_t nreg = (size_t)1 << (b->d * reps[i].k.lr);
+            if (nreg > h->rstate_cap) {
+                (void)hipFree(h->d_rstate);
+                h->d_rstate = nullptr;
+                HIP_TRY(hipMalloc(&h->d_rstate, nreg * sizeof(int32_t)));
+                h->rstate_cap = nreg;
+            }
+            reps[i].k.refstream = 1;
+            reps[i].k.rstate = h->d_rstate;
+        }
         if (reps[i].k.act && !reps[i].k.roles && reps[i].k.lc == 0)
             return fail(CPM_ENOTIMPL,
                         "batched replicas with the act term, shared roles and positions_per_checkerboard = 1");
@@ -1795,6 +1816,7 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
     if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations);
+    if (reps[0].k.refstream) return fail(CPM_ENOTIMPL, "the reference stream needs the per-phase batch form");
     if (reps[0].k.roles)
         return fail(CPM_ENOTIMPL, "the persistent batch form implements shared roles only (shared = 1)");
     if (a->cell_sync) return fail(CPM_ENOTIMPL, "cell_sync needs the per-phase batch form");

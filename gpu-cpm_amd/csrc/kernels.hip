@@ -1130,7 +1130,8 @@ hipError_t launch_ref_seed(int d, const SweepConsts& k, uint32_t seed32, hipStre
 
 // Batched replicas, one colour phase: block b works for replica rep_of[b]
 // on its same-colour sub-block (b - block0) * blockDim + t.
-template <int D, class ST>
+// REF: every replica draws from its reference stream (ref_attempts).
+template <int D, class ST, bool REF = false>
 __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
     const int r = a.rep_of[blockIdx.x];
     const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
@@ -1148,8 +1149,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
         const int by = (((gtid >> lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
-        sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T,
-                                     CellSync{a.sync_pass, rd.claims, a.sync_tag}, a.att0, a.natt);
+        const CellSync cy{a.sync_pass, rd.claims, a.sync_tag};
+        if constexpr (REF)
+            ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, T, cy, a.natt);
+        else
+            sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T, cy,
+                                         a.att0, a.natt);
     }
     if (a.sync_pass != 1) fold_tables<false>(T, k.ncells, gtid, gsz);
 }
@@ -1783,10 +1788,16 @@ static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hi
         return;
     }
 #undef CPM_BATCH
-    if (d == 2)
+    if (a.refstream) {
+        if (d == 2)
+            hipLaunchKernelGGL((k_batch_phase<2, T, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_batch_phase<3, T, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    } else if (d == 2) {
         hipLaunchKernelGGL((k_batch_phase<2, T>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else
+    } else {
         hipLaunchKernelGGL((k_batch_phase<3, T>), dim3(blocks), dim3(kBlock), 0, s, a);
+    }
 }
 
 hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {

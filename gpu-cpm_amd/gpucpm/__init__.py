@@ -327,8 +327,9 @@ class Cpm:
         stream (random.cu, kernels2d.cu:99): one MINSTD state per region of
         positions_per_thread sites, reseeded every inner_iterations chunk
         from the region's first site index + the low 32 bits of the seed, for
-        comparisons with the reference.  "minstd" runs single engines, and
-        with shared=1 needs positions_per_thread = 2 * positions_per_checkerboard."""
+        comparisons with the reference.  With shared=1 "minstd" needs
+        positions_per_thread = 2 * positions_per_checkerboard; a Batch's
+        engines must all use the same stream."""
         if name not in self._RNG:
             raise ValueError("rng must be 'philox' or 'minstd'")
         check(lib.cpm_set_rng(self._h, self._RNG[name]))

@@ -168,7 +168,8 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
  * positions_per_thread = 2c), seeded at every inner_iterations chunk from
  * the region's first site index + the low 32 bits of the seed, consumed in
  * the reference's draw order; acceptance u < ceil(exp(-E/T) (2^31 - 1)).
- * Single engines only (a batch returns CPM_ENOTIMPL). */
+ * A batch's engines must share the mode (CPM_EINVAL otherwise); the
+ * persistent batch form does not run it (CPM_ENOTIMPL). */
 /* Extension: online autotuning of the per-phase kernel form (no counterpart
  * in the reference, whose kernel is fixed at build time).  Every form
  * computes the same results, so while a workload (dimension, size, schedule,

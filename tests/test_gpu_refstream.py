@@ -70,9 +70,40 @@ def test_refstream_cell_sync_matches_oracle():
     assert_tables_equal(table, tab_o, 2)
 
 
+@pytest.mark.parametrize("d", [2, 3])
+def test_refstream_batch_equals_single_runs(d):
+    """A batch of reference-stream engines (each reseeded per chunk from its
+    own seed) equals each engine's single run and the oracle."""
+    import gpucpm
+    from gpucpm import workloads as W
+    if d == 2:
+        wls = [W.walled(w) if i % 2 else w
+               for i, w in enumerate(W.batched_sorting(nrep=4, seed=92, sizes=(32, 64, 128)))]
+        c, u, ppt, n, inner = 2, 1, 4, 12, 4
+    else:
+        wls = [W.plain3d(L=L, ncells=30, seed=93 + L) for L in (32, 64)]
+        c, u, ppt, n, inner = 1, 1, 2, 4, 2
+    _announce("refstream batch", d, [w.L for w in wls])
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(400 + i)
+        s.set_rng("minstd")
+        sims.append(s)
+    gpucpm.Batch(sims).run(**run_kwargs(c, u, n, inner, shared=0, positions_per_thread=ppt))
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.synchronize()
+        s.pull_from_gpu()
+        lat_o, tab_o = run_oracle(wl, c, u, 400 + i, n, inner=inner, shared=False, ppt=ppt, rng="minstd")
+        np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, d)
+
+
 def test_refstream_refusals_and_switching_back():
-    """shared=1 with positions_per_thread > 2c and batches are refused; the
-    counter stream is restored by set_rng("philox")."""
+    """shared=1 with positions_per_thread > 2c, batches mixing streams and
+    the persistent batch form are refused; the counter stream is restored by
+    set_rng("philox")."""
     import gpucpm
     from gpucpm import workloads as W
     wl = W.sorting2d(L=64, grid=5, edge=10, seed=91)
@@ -82,8 +113,14 @@ def test_refstream_refusals_and_switching_back():
     sim.set_rng("minstd")
     with pytest.raises(NotImplementedError):
         sim.run(**run_kwargs(1, 1, 2, shared=1, positions_per_thread=4))
+    other = wl.make_cpm()
+    other.push_to_gpu()
+    with pytest.raises(ValueError):
+        gpucpm.Batch([sim, other]).run(**run_kwargs(2, 1, 2, shared=0, positions_per_thread=4))
+    b = gpucpm.Batch([sim])
+    b.set_form(1)
     with pytest.raises(NotImplementedError):
-        gpucpm.Batch([sim]).run(**run_kwargs(2, 1, 2, shared=0, positions_per_thread=4))
+        b.run(**run_kwargs(1, 1, 2, shared=1, positions_per_thread=2))
     with pytest.raises(ValueError):
         sim.set_rng("mt19937")
     sim.set_rng("philox")
