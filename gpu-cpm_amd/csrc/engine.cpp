@@ -1930,6 +1930,41 @@ int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments) {
     return cpm_energy_stats_many(&h, 1, pairs, moments);
 }
 
+int cpm_energy_grad(cpm_handle h, int ntypes, const double* theta, double* H, double* dtheta) {
+    if (int rc = check_handle(h)) return rc;
+    if (ntypes < 1 || ntypes > CPM_NTYPES) return fail(CPM_EINVAL, "ntypes must be in [1, 20]");
+    if (!theta || !H || !dtheta) return fail(CPM_EINVAL, "null argument");
+    const int np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5;
+    std::vector<int64_t> pairs(np), mom(nm);
+    if (int rc = cpm_energy_stats_many(&h, 1, pairs.data(), mom.data())) return rc;
+    // H = sum_{a<=b} J_ab N_ab + sum_t lam_A (sum (A - A0)^2) + lam_P (sum (P - P0)^2),
+    // every term a polynomial in theta over the exact integer statistics.
+    const int nt = ntypes, nj = nt * (nt + 1) / 2;
+    double e = 0.0;
+    int j = 0;
+    for (int a = 0; a < nt; a++)
+        for (int b = a; b < nt; b++, j++) {
+            const double N = (double)pairs[a * CPM_NTYPES + b];
+            e += theta[j] * N;
+            dtheta[j] = N;
+        }
+    for (int t = 0; t < nt; t++) {
+        const double n = (double)mom[5 * t], SA = (double)mom[5 * t + 1], SA2 = (double)mom[5 * t + 2];
+        const double SP = (double)mom[5 * t + 3], SP2 = (double)mom[5 * t + 4];
+        const double la = theta[nj + t], a0 = theta[nj + nt + t];
+        const double lp = theta[nj + 2 * nt + t], p0 = theta[nj + 3 * nt + t];
+        const double qa = SA2 - 2.0 * a0 * SA + n * a0 * a0;  // sum (A - A0)^2
+        const double qp = SP2 - 2.0 * p0 * SP + n * p0 * p0;
+        e += la * qa + lp * qp;
+        dtheta[nj + t] = qa;
+        dtheta[nj + nt + t] = la * (2.0 * n * a0 - 2.0 * SA);
+        dtheta[nj + 2 * nt + t] = qp;
+        dtheta[nj + 3 * nt + t] = lp * (2.0 * n * p0 - 2.0 * SP);
+    }
+    *H = e;
+    return CPM_OK;
+}
+
 int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments) {
     if (!sims || n < 0) return fail(CPM_EINVAL, "null engine list");
     if (!pairs || !moments) return fail(CPM_EINVAL, "null output");

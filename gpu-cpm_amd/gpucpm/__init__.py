@@ -286,6 +286,19 @@ class Cpm:
         check(lib.cpm_energy_stats(self._h, pairs.ctypes.data_as(P), mom.ctypes.data_as(P)))
         return pairs, mom
 
+    def energy_grad(self, theta, ntypes):
+        """(H, dH/dtheta) in float64 on the current lattice, theta laid out as
+        gpucpm.autograd.ThetaLayout(ntypes) (the C entry cpm_energy_grad)."""
+        th = np.ascontiguousarray(np.asarray(theta, np.float64).reshape(-1))
+        nt = int(ntypes)
+        if th.size != nt * (nt + 1) // 2 + 4 * nt:
+            raise ValueError("theta has the wrong length for ntypes")
+        g = np.zeros_like(th)
+        H = ctypes.c_double()
+        D = ctypes.POINTER(ctypes.c_double)
+        check(lib.cpm_energy_grad(self._h, nt, th.ctypes.data_as(D), ctypes.byref(H), g.ctypes.data_as(D)))
+        return H.value, g
+
     def set_adhesion_real(self, cell_type, other_cell_type, adhesion, symmetric=True):
         """Adhesion as a multiple of 2^-8 (extension used by parameter fitting)."""
         check(lib.cpm_set_adhesion_real(self._h, _int(cell_type, "cell_type"),

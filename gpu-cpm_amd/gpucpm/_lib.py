@@ -81,6 +81,7 @@ SIGNATURES = [
     ("cpm_set_adhesion_real", _I, [_H, _I, _I, ctypes.c_double, _I]),
     ("cpm_energy_stats", _I, [_H, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("cpm_energy_stats_many", _I, [_P(_H), _I, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    ("cpm_energy_grad", _I, [_H, _I, _P(ctypes.c_double), _P(ctypes.c_double), _P(ctypes.c_double)]),
     ("cpm_set_launch", _I, [_H, _I, _I]),
     ("cpm_batch_set_blocks", _I, [_H, ctypes.c_double]),
     ("cpm_batch_set_form", _I, [_H, _I]),

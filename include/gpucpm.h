@@ -250,6 +250,14 @@ int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments);
 /* Extension: cpm_energy_stats of n engines at once (each on its own stream,
  * one wait at the end), into pairs[n * 400] and moments[n * 100]. */
 int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments);
+/* Extension (no counterpart in the reference): the backward pass in one call,
+ * H(sigma; theta) and dH/dtheta on the engine's current lattice for theta
+ * laid out, for ntypes types, as: J upper triangle (a <= b, row-major) |
+ * lambda_area[t] | target_area[t] | lambda_perimeter[t] | target_perimeter[t]
+ * (nt (nt + 1) / 2 + 4 nt doubles; dtheta alike).  H is the adhesion, area
+ * and perimeter energy (stickiness enters dH only); the derivatives are
+ * exact polynomials in theta over integer statistics, in double. */
+int cpm_energy_grad(cpm_handle h, int ntypes, const double* theta, double* H, double* dtheta);
 
 #ifdef __cplusplus
 }

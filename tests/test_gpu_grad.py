@@ -142,3 +142,32 @@ def test_stats_many_equal_single_and_numpy():
         p_np, m_np = numpy_stats(s.get_state().view(np.uint32).copy(), 2)
         np.testing.assert_array_equal(np.triu(pairs[i]), np.triu(p_np))
         np.testing.assert_array_equal(mom[i], m_np)
+
+
+def test_energy_grad_entry_equals_autograd_formula():
+    """cpm_energy_grad (H and dH/dtheta in one C call) equals the autograd
+    layer's closed forms on the same statistics, and its derivatives equal
+    central differences of its own H (H is quadratic in theta: exact up to
+    rounding)."""
+    from gpucpm import workloads as W
+    from gpucpm.autograd import ThetaLayout, energy_and_grad, stats_to_arrays
+    wl = W.sorting2d(L=128, grid=9, edge=10, seed=64)
+    print("CASE energy_grad", wl.L, flush=True)
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.run(**run_kwargs(1, 1, 20), seed=5)
+    layout = ThetaLayout(3)
+    p = wl.params()
+    th = layout.pack(p["J"], p["lam_area"], p["target_area"], p["lam_perim"], p["target_perim"],
+                     dtype=torch.float64).numpy()
+    H, g = sim.energy_grad(th, 3)
+    N, m = stats_to_arrays(*sim.energy_stats(), layout)
+    H2, g2 = energy_and_grad(th, N, m, layout)
+    np.testing.assert_allclose(H, H2, rtol=1e-14)
+    np.testing.assert_allclose(g, g2, rtol=1e-14)
+    h = 0.5
+    for i in range(th.size):
+        e = np.zeros_like(th)
+        e[i] = h
+        fd = (sim.energy_grad(th + e, 3)[0] - sim.energy_grad(th - e, 3)[0]) / (2 * h)
+        np.testing.assert_allclose(g[i], fd, rtol=1e-9, atol=1e-6)
