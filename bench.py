@@ -68,18 +68,30 @@ def cpu_baseline(wl, c, u, seconds):
     cores = len(os.sched_getaffinity(0))
     lat = wl.lattice.copy()
     tab = O.tally(lat, wl.d, wl.ncells)
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 0, 1, cores)  # thread pool start-up
     t = time.perf_counter()
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, 0, 2, cores)  # calibration
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 1, 2, cores)  # calibration
     per_mcs = max((time.perf_counter() - t) / 2, 1e-6)
     n = max(2, int(seconds / per_mcs))
     t = time.perf_counter()
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, 2, n, cores)
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 3, n, cores)
     dt = time.perf_counter() - t
     sites = wl.L ** wl.d
+    # The same sweep on one core, continuing from where the sample ended.
+    m = n + 3
+    t = time.perf_counter()
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, m, 1, 1)
+    per_mcs1 = max(time.perf_counter() - t, 1e-6)
+    n1 = max(1, int(seconds / 3 / per_mcs1))
+    t = time.perf_counter()
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, m + 1, n1, 1)
+    dt1 = time.perf_counter() - t
     return {"value": n * sites / dt, "unit": "site-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{n} MCS of the same {wl.L}^{wl.d} workload (MCS 2..{n + 1}) from its "
+            "sample": f"{n} MCS of the same {wl.L}^{wl.d} workload (MCS 3..{n + 2}) from its "
                       f"initial lattice, oracle/libcpm_oracle.so with OpenMP, {dt:.1f} s",
-            "mcs_per_s": n / dt}
+            "mcs_per_s": n / dt,
+            "single_core": {"value": n1 * sites / dt1, "mcs_per_s": n1 / dt1,
+                            "sample": f"{n1} MCS (MCS {m + 1}..{m + n1}), 1 thread, {dt1:.1f} s"}}
 
 
 def bench_batched(a, log):
