@@ -246,6 +246,24 @@ __device__ __forceinline__ uint32_t sub_block_id(const SweepConsts& k, int bx, i
     return (uint32_t)bx | ((uint32_t)by << lsb) | (D == 3 ? ((uint32_t)bz << (2 * lsb)) : 0u);
 }
 
+// The random words of attempt i of sub-block sb (x coordinate bx) in one
+// colour phase: word 0 picks the position in the sub-block, word 1 the
+// neighbour, word 2 the acceptance.  With one position per sub-block (lc = 0)
+// an attempt needs only words 1 and 2, so the same-colour sub-blocks 2j and
+// 2j + 1 along x (colour-grid column bx >> lstride) share one Philox block:
+// the even one's counter, words 1 and 2 to the even sub-block, words 3 and 0
+// to the odd one.
+__device__ __forceinline__ void attempt_words(const SweepConsts& k, uint32_t sb, int bx,
+                                              uint32_t mcs, uint32_t phase, uint32_t i,
+                                              uint32_t w[4]) {
+    const uint32_t odd = k.lc == 0 ? ((uint32_t)bx >> k.lstride) & 1u : 0u;
+    philox4x32_10(sb - (odd << k.lstride), mcs, phase, i, k.k0, k.k1, w);
+    if (odd) {
+        w[1] = w[3];
+        w[2] = w[0];
+    }
+}
+
 // Source direction (kernels2d.cu:216-225 drawn in one step): floor(w1 N /
 // 2^32) over the N neighbours in block order, skipping the centre.
 // In 2D the source is selected from the nine loaded values, so all loads
@@ -258,7 +276,7 @@ __device__ __forceinline__ void draw_attempt(const Lat& lat, const SweepConsts& 
                                              uint32_t phase, uint32_t i, Draw<D, Lat>& d) {
     const int lc = k.lc;
     const uint32_t cm = (1u << lc) - 1u;
-    philox4x32_10(sb, mcs, phase, i, k.k0, k.k1, d.w);
+    attempt_words(k, sb, bx, mcs, phase, i, d.w);
     d.tick = mcs;
     constexpr int N = D == 3 ? 26 : 8;
     constexpr int NB = N + 1;
@@ -932,12 +950,18 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
     const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
     using Lat = GlobalLat<D, false, ST>;
     const Lat lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
-    // Draws (kernels2d.cu:199-243 in the shared roles).
+    // Draws (kernels2d.cu:199-243 in the shared roles).  When two sub-blocks
+    // share a Philox block (attempt_words), rounds 2p and 2p + 1 of a thread
+    // take the pair of sub-blocks 2t and 2t + 1 of their 2 blockDim range, so
+    // the block is computed once.
+    const bool paired = ATT % 2 == 0 && k.lc == 0 && a.lh > 0 && a.lstride == k.lstride;
     bool active[ATT];
     QueuedAttempt<D, FULL, ST> e[ATT];
+    uint32_t wp[4];
 #pragma unroll
     for (int r = 0; r < ATT; r++) {
-        const int q = (a.lblock * ATT + r) * kBlock + (int)threadIdx.x;
+        const int q = paired ? (a.lblock * ATT + (r & ~1)) * kBlock + 2 * (int)threadIdx.x + (r & 1)
+                             : (a.lblock * ATT + r) * kBlock + (int)threadIdx.x;
         active[r] = false;
         if (q < a.nsub) {
             const int hm = (1 << a.lh) - 1;
@@ -946,7 +970,18 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
             const int bz = D == 3 ? ((q >> (2 * a.lh)) << a.lstride) + a.cz : 0;
             const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
             uint32_t w[4];
-            philox4x32_10(sb, a.mcs, a.phase, 0u, k.k0, k.k1, w);
+            if (!paired) {
+                attempt_words(k, sb, bx, a.mcs, a.phase, 0u, w);
+            } else if ((r & 1) == 0) {  // the even sub-block: its own counter
+                philox4x32_10(sb, a.mcs, a.phase, 0u, k.k0, k.k1, wp);
+                w[0] = wp[0];
+                w[1] = wp[1];
+                w[2] = wp[2];
+            } else {  // q and q - 1 are both below nsub (nsub is even)
+                w[0] = wp[0];
+                w[1] = wp[3];
+                w[2] = wp[0];
+            }
             const int lc = k.lc;
             const uint32_t cm = (1u << lc) - 1u;
             constexpr int N = D == 3 ? 26 : 8;

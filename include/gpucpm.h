@@ -161,7 +161,8 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
                        int64_t* csum);
 
 /* Extension: the random stream.  mode 0 (default): the counter stream, Philox
- * keyed by (sub-block, MCS, phase, attempt).  mode 1: the reference's stream
+ * keyed by (sub-block, MCS, phase, attempt); with positions_per_checkerboard
+ * = 1 two x-neighbouring same-colour sub-blocks share one Philox block.  mode 1: the reference's stream
  * (random.cu:5-28 seeded per thread at kernels2d.cu:99 / kernels3d.cu:86),
  * for comparisons with the reference: one MINSTD state per region of
  * positions_per_thread sites per axis (2c with shared = 1, which then needs

@@ -623,8 +623,19 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                         int px, py, pz, qx, qy, qz;
                         int o[3]; /* the source's offset from the target */
                         if (!refstream) {
-                            uint32_t ctr[4] = {sb, mcs, phase, (uint32_t)i};
+                            /* word 0: position, 1: neighbour, 2: acceptance.
+                             * With c = 1 only words 1 and 2 are needed: the
+                             * same-colour sub-blocks 2j and 2j+1 along x share
+                             * the even one's block, the odd one taking words 3
+                             * and 0 (the build's own stream layout; the
+                             * reference draws MINSTD per thread) */
+                            const int odd = lc == 0 ? (int)((qi % h) & 1) : 0;
+                            uint32_t ctr[4] = {sb - (uint32_t)(odd * S * K), mcs, phase, (uint32_t)i};
                             orc_philox4x32_10(ctr, key, w);
+                            if (odd) {
+                                w[1] = w[3];
+                                w[2] = w[0];
+                            }
                             /* the position drawn in the sub-block and a
                              * neighbour: shared roles (kernels2d.cu:204-226):
                              * target, source; non-shared (:227-243): source,

@@ -18,7 +18,9 @@
  *   - MINSTD generator (compat / known-answer only)       random.cu:5-12
  * and make it deterministic (the reference races on its cell tables):
  *   - counter-based Philox4x32-10 (Salmon et al., SC'11) keyed by
- *     (seed; sub-block, MCS, phase, attempt);
+ *     (seed; sub-block, MCS, phase, attempt): word 0 the position, 1 the
+ *     neighbour, 2 the acceptance; with c = 1 the same-colour sub-blocks
+ *     2j and 2j+1 along x share the even one's block (odd: words 3 and 0);
  *   - cell tables read from a phase-start snapshot, updates applied as
  *     commutative integer sums;
  *   - integer dH in a dyadic fixed-point scale and a host-built acceptance
