@@ -125,7 +125,8 @@ struct PhaseArgs {
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int nsub;               // active sub-blocks in this phase
     int box3;               // 3D, c = 1: stage each block's box in LDS
-    int sync_pass;          // 0 off; cell_sync: 1 claim, 2 commit; split phase: 3 decide, 4 apply
+    int sync_pass;          // 0 off; cell_sync: 1 claim, 2 commit; split phase: 3 decide, 4 apply;
+                            // both: 5 decide + claim, 6 apply + claim check
     uint64_t* claims;       // cell_sync: per-cell claim keys
     uint32_t sync_tag;      // cell_sync: phase tag (grows across runs)
     uint8_t* decide;        // split phase: one decision per sub-block of the phase

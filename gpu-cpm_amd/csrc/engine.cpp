@@ -1147,11 +1147,10 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
     // of the phase lie.  Each phase is then a decide launch (every attempt
     // against the phase-start lattice, decisions recorded) and an apply
     // launch (the accepted ones committed; a target's own neighbourhood
-    // holds no other target, so the deltas are those decided).
+    // holds no other target, so the deltas are those decided).  With
+    // cell_sync the decide launch also claims and the apply launch checks.
     const bool split = k.act && !k.roles && c == 1;
     if (split) {
-        if (a->cell_sync)
-            return fail(CPM_ENOTIMPL, "cell_sync with the act term, shared roles and positions_per_checkerboard = 1");
         plan = Plan{};
         plan.mode = Plan::kPhase;
         const size_t nsub = (size_t)1 << (lh * h->d);
@@ -1276,20 +1275,19 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                     for (int i = 0; i < nidx; i++) {
                         pa.att0 = a->cell_sync ? i : 0;
                         pa.natt = a->cell_sync ? 1 : u;
-                        if (split) {
-                            pa.decide = h->d_decide;
-                            pa.sync_pass = 3;
-                            HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, nullptr, nullptr));
-                            pa.sync_pass = 4;
-                        }
                         if (a->cell_sync) {
                             // the tag only grows: claims never need clearing
                             if (++h->sync_tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
                             pa.sync_tag = h->sync_tag;
                             pa.claims = h->d_claims;
-                            pa.sync_pass = 1;
+                        }
+                        // split: decide (with cell_sync also claim), then
+                        // apply; cell_sync alone: claim, then commit
+                        if (split || a->cell_sync) {
+                            pa.decide = h->d_decide;
+                            pa.sync_pass = split ? (a->cell_sync ? 5 : 3) : 1;
                             HIP_TRY(gcpm::launch_phase(h->d, pa, h->stream, nullptr, nullptr));
-                            pa.sync_pass = 2;
+                            pa.sync_pass = split ? (a->cell_sync ? 6 : 4) : 2;
                         }
                         hipEvent_t e0, e1;
                         if (int rc = h->next_events(&e0, &e1)) return rc;

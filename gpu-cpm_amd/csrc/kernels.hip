@@ -594,7 +594,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
     }
     bool accept;
-    if (cy.pass == 4) {
+    if (cy.pass == 4 || cy.pass == 6) {
         // split phase, apply: the decision made against the phase-start
         // lattice (the act term reads sites that other attempts now write)
         accept = cy.decide[cy.q] != 0;
@@ -604,14 +604,14 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         if (XT && D == 3 && k.pers) dh += pers_energy<D>(k, d.dir, vs, vt);
         accept = dh <= 0 || (dh < k.nthr && d.w[2] < thr[dh]);
     }
-    if (cy.pass == 3) {  // split phase, decide: record, write nothing else
+    if (cy.pass == 3 || cy.pass == 5) {  // split phase, decide: record (pass 5 also claims)
         if (accept) cy.decide[cy.q] = 1;
-        return;
+        if (cy.pass == 3) return;
     }
     if (!accept) return;
-    if (cy.pass == 1 || cy.pass == 2) {
+    if (cy.pass == 1 || cy.pass == 2 || cy.pass == 5 || cy.pass == 6) {
         const uint64_t key = ((uint64_t)cy.tag << 32) | (uint64_t)(0xFFFFFFFFu - sb);
-        if (cy.pass == 1) {
+        if (cy.pass == 1 || cy.pass == 5) {
             if (cs) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + cs), key);
             if (ct) atomicMax(reinterpret_cast<unsigned long long*>(cy.claims + ct), key);
             return;
@@ -714,7 +714,7 @@ __device__ __forceinline__ void ref_attempts(const Lat& lat, const SweepConsts& 
         d.tick = mcs;
         finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
     }
-    if (cy.pass != 1 && cy.pass != 3) k.rstate[rid] = st;
+    if (cy.pass != 1 && cy.pass != 3 && cy.pass != 5) k.rstate[rid] = st;
 }
 
 // The u copy attempts of sub-block (bx, by, bz) in one colour phase, in
@@ -822,10 +822,10 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
     const int gsz = gridDim.x * blockDim.x;
     const SweepConsts& k = a.k;
     const PhaseTables T = tables_of(k, a.g);
-    const bool fold = a.sync_pass != 1 && a.sync_pass != 3;
+    const bool fold = a.sync_pass != 1 && a.sync_pass != 3 && a.sync_pass != 5;
     const int64_t dv = fold && gtid <= k.ncells ? T.Dp[gtid] : 0;
     if (gtid < a.nsub) {
-        if (a.sync_pass == 3) a.decide[gtid] = 0;
+        if (a.sync_pass == 3 || a.sync_pass == 5) a.decide[gtid] = 0;
         const int hm = (1 << a.lh) - 1;
         const int bx = ((gtid & hm) << a.lstride) + a.cx;
         const int by = (((gtid >> a.lh) & hm) << a.lstride) + a.cy;

@@ -95,8 +95,9 @@ int cpm_set_chemotaxis(cpm_handle h, int type, double lambda);
  * its same-cell neighbours.  max_act in [0, 65535].  Runs with the term need
  * has_act.  With shared = 1 and positions_per_checkerboard = 1 (the term
  * then reads sites other attempts of the phase write) each phase is decided
- * against its start lattice, then applied: two launches per phase; not with
- * cell_sync or in batches (CPM_ENOTIMPL). */
+ * against its start lattice, then applied: two launches per phase (with
+ * cell_sync the decide launch also claims, the apply launch checks the
+ * claims); not in batches (CPM_ENOTIMPL). */
 int cpm_set_act(cpm_handle h, int type, int max_act, double lambda);
 /* Cpm::setPersistence (cpm.cu:112-115): the persistence term -lambda (d . p)
  * (kernels3d.cu:327-344, 3D only as in the reference), p the cell's preferred

@@ -523,9 +523,10 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
      * of the phase lie.  Each phase is then made in two passes: every
      * attempt is decided against the phase-start lattice (nothing written),
      * then the accepted ones are applied (a target's own neighbourhood holds
-     * no other target, so the applied deltas are those decided). */
+     * no other target, so the applied deltas are those decided).  With
+     * cell_sync the decide pass also claims and the apply pass checks the
+     * claims (kernels.cu:123-141 as a deterministic rule). */
     const int split = shared && act_on(p) && p->act != NULL && c < 2;
-    if (split && cell_sync) return -8;
     if (L < 2 || (L & (L - 1))) return -2;
     int K, S;
     if (orc_schedule(L, d, c, shared, ppt, &K, &S)) return -3;
@@ -729,7 +730,8 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                         }
                         if (split && pass == 0) {
                             dec[qi] = (uint8_t)accept;
-                            continue;
+                            /* with cell_sync the accepted ones also claim */
+                            if (!cell_sync) continue;
                         }
                         if (!accept) continue;
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;

@@ -104,7 +104,9 @@ void orc_init_pref_dirs(int ncells, uint64_t seed, double* pref);
  * sites from the target, where with shared roles and c = 1 other targets of
  * the same colour phase lie: such phases are made in two passes, every
  * attempt decided against the phase-start lattice, then the accepted ones
- * applied (not combined with cell_sync: -8). */
+ * applied.  With cell_sync the decide pass is also the claim pass (the
+ * accepted proposals claim their cells) and the apply pass commits the
+ * decided flips whose sub-block holds both claims. */
 #define ORC_ACT_MAX 65535
 int64_t orc_act_log2(int v);                          /* LG[v] */
 int64_t orc_act_geo_mean(int64_t sum_lg, int count);  /* G in 2^-16 units */

@@ -319,7 +319,8 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
     source neighbourhood reaches same-phase targets: each phase is decided
     against its start lattice and then applied (two launches), bit-exact
     against the oracle's two passes in 2D and 3D (also with the reference
-    stream); with cell_sync, or in a batch, it is refused, not raced."""
+    stream, and with cell_sync: the decide launch also claims, the apply
+    launch checks the claims); in a batch it is refused, not raced."""
     import gpucpm
     _, W = _gpu()
     monkeypatch.setenv("GPUCPM_NARROW", narrow)
@@ -329,18 +330,24 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
              (W.active(W.plain3d(L=32, ncells=60, seed=107), lam=(0.0, 200.0), act_max=(0, 30)), 6,
               "philox"),
              (W.active(W.sorting2d(L=64, grid=5, edge=10, seed=108)), 12, "minstd")]
-    for wl, n, rng in cases:
-        _announce("act c=1", narrow, rng, wl.name, wl.L, wl.d)
+    cases = [(wl, n, rng, False) for wl, n, rng in cases] + [
+        (W.active(W.sorting2d(L=128, grid=9, edge=10, seed=110)), 20, "philox", True),
+        (W.active(W.plain3d(L=32, ncells=60, seed=111), lam=(0.0, 200.0), act_max=(0, 30)), 5,
+         "philox", True),
+        (W.active(W.sorting2d(L=64, grid=5, edge=10, seed=112)), 12, "minstd", True)]
+    for wl, n, rng, cs in cases:
+        _announce("act c=1", narrow, rng, "cell_sync" if cs else "", wl.name, wl.L, wl.d)
         sim = wl.make_cpm()
         sim.push_to_gpu()
         sim.set_seed(113)
         sim.set_rng(rng)
-        sim.run(**run_kwargs(1, 1, n, n // 2 if rng == "minstd" else n, positions_per_thread=2))
+        sim.run(**run_kwargs(1, 1, n, n // 2 if rng == "minstd" else n, positions_per_thread=2,
+                             cell_sync=int(cs)))
         sim.synchronize()
         sim.pull_from_gpu()
         lat = sim.get_state().view(np.uint32).copy()
         lat_o, tab_o = run_oracle(wl, 1, 1, 113, n, inner=n // 2 if rng == "minstd" else None,
-                                  rng=rng)
+                                  rng=rng, cell_sync=cs)
         np.testing.assert_array_equal(lat, lat_o)
         assert_tables_equal(sim.get_cell_table(), tab_o, wl.d)
         np.testing.assert_array_equal(sim.get_act_state().view(np.uint32), tab_o.act)
@@ -349,8 +356,6 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
     _announce("act c=1 refusals", wl.L)
     sim = wl.make_cpm()
     sim.push_to_gpu()
-    with pytest.raises(NotImplementedError):
-        sim.run(**run_kwargs(1, 1, 1, cell_sync=1))
     with pytest.raises(NotImplementedError):
         gpucpm.Batch([sim]).run(**run_kwargs(1, 1, 1))
 

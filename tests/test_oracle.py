@@ -565,7 +565,7 @@ def test_act_sweep_keeps_tables_exact_and_stamps_copies():
     """An accepted copy sets act[target] = tick (kernels2d.cu:532-534): after
     one MCS at tick m every site that changed holds act m; the tables stay
     equal to a recount; with c = 1 (decided then applied per phase) likewise,
-    and the run is thread-count invariant; c = 1 with cell_sync is refused."""
+    also with cell_sync, and the run is thread-count invariant."""
     lat, act, p, _, _ = _act_setup(32, 2)
     before, act0 = lat.copy(), act.copy()
     tab = O.tally(lat, 2, 2)
@@ -577,17 +577,13 @@ def test_act_sweep_keeps_tables_exact_and_stamps_copies():
     area, perim = _numpy_recount(lat, 2, 2)
     np.testing.assert_array_equal(tab.area, area)
     np.testing.assert_array_equal(tab.perim, perim)
-    lat2, act2, p2, _, _ = _act_setup(32, 2)
-    tab2 = O.tally(lat2, 2, 2)
-    with pytest.raises(ValueError):
-        O.sweep(lat2, 2, tab2, p2, 1, 1, seed=4, mcs0=0, nmcs=1, cell_sync=True)
-    for d in (2, 3):
+    for d, cs in ((2, False), (3, False), (2, True), (3, True)):
         outs = []
         for nth in (1, 8):
             l1, a1, p1, _, _ = _act_setup(16, d, seed=3)
             before = l1.copy()
             t1 = O.tally(l1, d, 2)
-            O.sweep(l1, d, t1, p1, 1, 1, seed=6, mcs0=40, nmcs=6, nthreads=nth)
+            O.sweep(l1, d, t1, p1, 1, 1, seed=6, mcs0=40, nmcs=6, nthreads=nth, cell_sync=cs)
             area, perim = _numpy_recount(l1, d, 2)
             np.testing.assert_array_equal(t1.area, area)
             np.testing.assert_array_equal(t1.perim, perim)
@@ -608,12 +604,14 @@ def test_act_sweep_keeps_tables_exact_and_stamps_copies():
     np.testing.assert_array_equal(ac3, ab3)
 
 
-@pytest.mark.parametrize("d", [2, 3])
-def test_act_single_site_checkerboard_decides_on_phase_start(d):
+@pytest.mark.parametrize("d,cell_sync", [(2, False), (3, False), (2, True), (3, True)])
+def test_act_single_site_checkerboard_decides_on_phase_start(d, cell_sync):
     """Shared roles, c = 1, with an act term whose energy is exactly zero
     (lambda_act = 2^-10, actMax = 1: |m dG| < 2^31, so E = 0): the two-pass
     phases (decide on the phase-start lattice, then apply) accept exactly what
-    the act-free sweep accepts, and stamp act[t] = tick on every copy."""
+    the act-free sweep accepts, and stamp act[t] = tick on every copy.  With
+    cell_sync the decide pass also claims and the apply pass checks the
+    claims: the result is the act-free cell_sync sweep's."""
     lat, act, _, _, _ = _act_setup(16, d, seed=5)
     lat0 = lat.copy()
     la = np.zeros(20)
@@ -628,8 +626,8 @@ def test_act_single_site_checkerboard_decides_on_phase_start(d):
     act0 = act.copy()
     for m in range(4):
         before = lat.copy()
-        O.sweep(lat, d, ta, p_act, 1, 1, seed=21, mcs0=100 + m, nmcs=1)
-        O.sweep(lat0, d, tb, p_off, 1, 1, seed=21, mcs0=100 + m, nmcs=1)
+        O.sweep(lat, d, ta, p_act, 1, 1, seed=21, mcs0=100 + m, nmcs=1, cell_sync=cell_sync)
+        O.sweep(lat0, d, tb, p_off, 1, 1, seed=21, mcs0=100 + m, nmcs=1, cell_sync=cell_sync)
         np.testing.assert_array_equal(lat, lat0)
         assert (act[lat != before] == 100 + m).all()
     np.testing.assert_array_equal(ta.area, tb.area)
