@@ -163,6 +163,7 @@ struct PersistArgs {
 struct ReplicaDesc {
     SweepConsts k;
     uint64_t* claims;       // cell_sync claim keys of this replica
+    uint8_t* decide;        // split phases: one decision byte per sub-block
     uint32_t mcs0;          // this replica's MCS counter at the start
     int lh;                 // log2 (same-colour sub-blocks per axis)
     int block0, nblocks;    // its block group in the cooperative grid
@@ -186,7 +187,7 @@ struct BatchPhaseArgs {
     uint32_t phase;         // round * 2^d + colour
     uint32_t m;             // MCS offset within this run (replica MCS = mcs0 + m)
     int g;                  // phase counter since the last settle (parity)
-    int sync_pass;          // cell_sync: 0 off, 1 claim, 2 commit
+    int sync_pass;          // as PhaseArgs (0 off; 1/2 cell_sync; 3/4 split; 5/6 both)
     uint32_t sync_tag;      // cell_sync: tag (above every replica's earlier tags)
     int att0, natt;         // attempt indices of this launch
     int form;               // kForm* bits (compact, small, attempts per thread)

@@ -1522,9 +1522,11 @@ int cpm_get_mcs(cpm_handle h, uint64_t* mcs) {
 namespace {
 
 // Batched run, one launch per colour phase for all replicas (every replica
-// has the same c and u, hence the same phase sequence).
+// has the same c and u, hence the same phase sequence).  split: every phase
+// is a decide launch and an apply launch (as cpm_run with the act term,
+// shared roles and c = 1), on the uncompacted layout.
 int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::ReplicaDesc>& reps,
-                    long total) {
+                    long total, bool split) {
     const int n = (int)b->sims.size();
     uint32_t tag = 0;  // cell_sync: one tag for all replicas, above each one's earlier tags
     bool pers = false;
@@ -1551,7 +1553,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     };
     const int u1 = a->updates_per_checkerboard_switch;
     const bool refstream = reps[0].k.refstream != 0;  // (all replicas alike)
-    const bool compact = u1 == 1 && !refstream && env_int("GPUCPM_COMPACT", 1) != 0;
+    const bool compact = u1 == 1 && !refstream && !split && env_int("GPUCPM_COMPACT", 1) != 0;
     bool small = env_int("GPUCPM_SMALL", b->d == 3) != 0;
     for (int i = 0; i < n; i++) small = small && reps[i].k.nt <= gcpm::kSmallTypes;
     // Attempts drawn per thread: 1, 4 or 8 (a batch is throughput bound, so
@@ -1685,9 +1687,11 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                     if (a->cell_sync) {
                         if (++tag == 0) return fail(CPM_EINVAL, "cell_sync phase tag overflow");
                         pa.sync_tag = tag;
-                        pa.sync_pass = 1;
+                    }
+                    if (split || a->cell_sync) {
+                        pa.sync_pass = split ? (a->cell_sync ? 5 : 3) : 1;
                         HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
-                        pa.sync_pass = 2;
+                        pa.sync_pass = split ? (a->cell_sync ? 6 : 4) : 2;
                     }
                     const int ti = tune ? b->tuner.pick(*tune) : -1;
                     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1782,6 +1786,7 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     const int n = (int)b->sims.size();
     std::vector<gcpm::ReplicaDesc> reps(n);
     std::vector<long> need(n);
+    bool split = false;
     for (int i = 0; i < n; i++) {
         cpm_handle h = b->sims[i];
         if (h->rng != b->sims[0]->rng)
@@ -1802,18 +1807,38 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
             reps[i].k.refstream = 1;
             reps[i].k.rstate = h->d_rstate;
         }
-        if (reps[i].k.act && !reps[i].k.roles && reps[i].k.lc == 0)
-            return fail(CPM_ENOTIMPL,
-                        "batched replicas with the act term, shared roles and positions_per_checkerboard = 1");
+        reps[i].decide = nullptr;
+        if (reps[i].k.act && !reps[i].k.roles && reps[i].k.lc == 0) split = true;
         if (reps[i].k.K != reps[0].k.K || reps[i].k.S != reps[0].k.S)
             return fail(CPM_EINVAL, "batched replicas need the same colour schedule (K, S)");
         const long nsub = 1L << (reps[i].lh * b->d);
         const long w = (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
         need[i] = std::max(1L, (long)std::ceil(w * b->block_scale));
     }
+    if (split) {
+        // Act with shared roles and c = 1 in any replica: every replica's
+        // phases are decided, then applied (for a replica without the term
+        // the decisions are those of one launch: with shared roles and c = 1
+        // a target's neighbourhood holds no other target).
+        if (b->form != 0)
+            return fail(CPM_ENOTIMPL,
+                        "the act term with shared roles and positions_per_checkerboard = 1 needs the "
+                        "per-phase batch form");
+        for (int i = 0; i < n; i++) {
+            cpm_handle h = b->sims[i];
+            const size_t nsub = (size_t)1 << (reps[i].lh * b->d);
+            if (nsub > h->decide_cap) {
+                (void)hipFree(h->d_decide);
+                h->d_decide = nullptr;
+                HIP_TRY(hipMalloc(&h->d_decide, nsub));
+                h->decide_cap = nsub;
+            }
+            reps[i].decide = h->d_decide;
+        }
+    }
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
-    if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations);
+    if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations, split);
     if (reps[0].k.refstream) return fail(CPM_ENOTIMPL, "the reference stream needs the per-phase batch form");
     if (reps[0].k.roles)
         return fail(CPM_ENOTIMPL, "the persistent batch form implements shared roles only (shared = 1)");

@@ -1179,19 +1179,21 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
     const PhaseTables T = tables_of(k, a.g);
     const int lh = rd.lh;
     if (gtid < (1 << (D * lh))) {
+        if (a.sync_pass == 3 || a.sync_pass == 5) rd.decide[gtid] = 0;
         const int hm = (1 << lh) - 1;
         const int bx = ((gtid & hm) << a.lstride) + a.cx;
         const int by = (((gtid >> lh) & hm) << a.lstride) + a.cy;
         const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
-        const CellSync cy{a.sync_pass, rd.claims, a.sync_tag};
+        const CellSync cy{a.sync_pass, rd.claims, a.sync_tag, rd.decide, gtid};
         if constexpr (REF)
             ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, T, cy, a.natt);
         else
             sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T, cy,
                                          a.att0, a.natt);
     }
-    if (a.sync_pass != 1) fold_tables<false>(T, k.ncells, gtid, gsz);
+    if (a.sync_pass != 1 && a.sync_pass != 3 && a.sync_pass != 5)
+        fold_tables<false>(T, k.ncells, gtid, gsz);
 }
 
 // One 3D colour phase with positions_per_checkerboard = 1 (every sub-block

@@ -97,7 +97,7 @@ int cpm_set_chemotaxis(cpm_handle h, int type, double lambda);
  * then reads sites other attempts of the phase write) each phase is decided
  * against its start lattice, then applied: two launches per phase (with
  * cell_sync the decide launch also claims, the apply launch checks the
- * claims); not in batches (CPM_ENOTIMPL). */
+ * claims); in a batch only in the per-phase form (else CPM_ENOTIMPL). */
 int cpm_set_act(cpm_handle h, int type, int max_act, double lambda);
 /* Cpm::setPersistence (cpm.cu:112-115): the persistence term -lambda (d . p)
  * (kernels3d.cu:327-344, 3D only as in the reference), p the cell's preferred

@@ -194,3 +194,45 @@ def test_batch_attempts_per_thread(att, monkeypatch):
         lat_o, tab_o = run_oracle(wl, 1, 1, 300 + i, 12)
         np.testing.assert_array_equal(s.get_state().view(np.uint32), lat_o)
         assert_tables_equal(s.get_cell_table(), tab_o, 2)
+
+
+@pytest.mark.parametrize("cell_sync", [0, 1])
+def test_batch_act_single_site_checkerboard(cell_sync):
+    """The act term with shared roles and positions_per_checkerboard = 1 in a
+    batch: every phase is a decide launch and an apply launch for all
+    replicas (with cell_sync the decide launch also claims).  Replicas with
+    and without the term end bit-identical to their single runs and to the
+    oracle, act lattices included; the persistent batch form refuses."""
+    import gpucpm
+    from gpucpm import workloads as W
+    base = W.batched_sorting(nrep=4, seed=44, sizes=(32, 64))
+    wls = [W.active(w) if i % 2 == 0 else w for i, w in enumerate(base)]
+    print("CASE batch act c=1 cell_sync", cell_sync, [w.L for w in wls], flush=True)
+    kw = run_kwargs(1, 1, 16, 8, cell_sync=cell_sync)
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(200 + i)
+        sims.append(s)
+    batch = gpucpm.Batch(sims)
+    batch.run(**kw)
+    batch.synchronize()
+    for i, (wl, s) in enumerate(zip(wls, sims)):
+        s.pull_from_gpu()
+        lat = s.get_state().view(np.uint32).copy()
+        t = wl.make_cpm()
+        t.push_to_gpu()
+        t.run(**kw, seed=200 + i)
+        t.synchronize()
+        t.pull_from_gpu()
+        np.testing.assert_array_equal(lat, t.get_state().view(np.uint32))
+        lat_o, tab_o = run_oracle(wl, 1, 1, 200 + i, 16, cell_sync=bool(cell_sync))
+        np.testing.assert_array_equal(lat, lat_o)
+        assert_tables_equal(s.get_cell_table(), tab_o, 2)
+        if i % 2 == 0:
+            np.testing.assert_array_equal(s.get_act_state().view(np.uint32), tab_o.act)
+        assert np.count_nonzero(lat != wl.lattice) > 0
+    batch.set_form(1)
+    with pytest.raises(NotImplementedError):
+        batch.run(**kw)

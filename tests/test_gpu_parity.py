@@ -320,8 +320,7 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
     against its start lattice and then applied (two launches), bit-exact
     against the oracle's two passes in 2D and 3D (also with the reference
     stream, and with cell_sync: the decide launch also claims, the apply
-    launch checks the claims); in a batch it is refused, not raced."""
-    import gpucpm
+    launch checks the claims).  Batches: test_gpu_batch.py."""
     _, W = _gpu()
     monkeypatch.setenv("GPUCPM_NARROW", narrow)
     cases = [(W.active(W.sorting2d(L=128, grid=9, edge=10, seed=105)), 30, "philox"),
@@ -352,12 +351,6 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
         assert_tables_equal(sim.get_cell_table(), tab_o, wl.d)
         np.testing.assert_array_equal(sim.get_act_state().view(np.uint32), tab_o.act)
         assert np.count_nonzero(lat != wl.lattice) > 0
-    wl = W.active(W.sorting2d(L=64, grid=5, edge=10, seed=109))
-    _announce("act c=1 refusals", wl.L)
-    sim = wl.make_cpm()
-    sim.push_to_gpu()
-    with pytest.raises(NotImplementedError):
-        gpucpm.Batch([sim]).run(**run_kwargs(1, 1, 1))
 
 
 @pytest.mark.parametrize("small,compact,att", [("1", "1", "4"), ("0", "1", "1"), ("1", "0", "1")])
