@@ -196,8 +196,9 @@ def test_batch_attempts_per_thread(att, monkeypatch):
         assert_tables_equal(s.get_cell_table(), tab_o, 2)
 
 
-@pytest.mark.parametrize("cell_sync", [0, 1])
-def test_batch_act_single_site_checkerboard(cell_sync):
+@pytest.mark.parametrize("cell_sync,rng", [(0, "philox"), (1, "philox"), (0, "minstd"),
+                                           (1, "minstd")])
+def test_batch_act_single_site_checkerboard(cell_sync, rng):
     """The act term with shared roles and positions_per_checkerboard = 1 in a
     batch: every phase is a decide launch and an apply launch for all
     replicas (with cell_sync the decide launch also claims).  Replicas with
@@ -207,13 +208,14 @@ def test_batch_act_single_site_checkerboard(cell_sync):
     from gpucpm import workloads as W
     base = W.batched_sorting(nrep=4, seed=44, sizes=(32, 64))
     wls = [W.active(w) if i % 2 == 0 else w for i, w in enumerate(base)]
-    print("CASE batch act c=1 cell_sync", cell_sync, [w.L for w in wls], flush=True)
+    print("CASE batch act c=1 cell_sync", cell_sync, rng, [w.L for w in wls], flush=True)
     kw = run_kwargs(1, 1, 16, 8, cell_sync=cell_sync)
     sims = []
     for i, wl in enumerate(wls):
         s = wl.make_cpm()
         s.push_to_gpu()
         s.set_seed(200 + i)
+        s.set_rng(rng)
         sims.append(s)
     batch = gpucpm.Batch(sims)
     batch.run(**kw)
@@ -223,11 +225,13 @@ def test_batch_act_single_site_checkerboard(cell_sync):
         lat = s.get_state().view(np.uint32).copy()
         t = wl.make_cpm()
         t.push_to_gpu()
+        t.set_rng(rng)
         t.run(**kw, seed=200 + i)
         t.synchronize()
         t.pull_from_gpu()
         np.testing.assert_array_equal(lat, t.get_state().view(np.uint32))
-        lat_o, tab_o = run_oracle(wl, 1, 1, 200 + i, 16, cell_sync=bool(cell_sync))
+        lat_o, tab_o = run_oracle(wl, 1, 1, 200 + i, 16, cell_sync=bool(cell_sync), rng=rng,
+                                  inner=8 if rng == "minstd" else None)
         np.testing.assert_array_equal(lat, lat_o)
         assert_tables_equal(s.get_cell_table(), tab_o, 2)
         if i % 2 == 0:
