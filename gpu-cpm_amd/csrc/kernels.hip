@@ -1033,18 +1033,25 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
     }
     if (threadIdx.x == 0) nqueued = 0;
     __syncthreads();
-    // Queue the active attempts: one LDS atomic per wave and round.
+    // Queue the active attempts: one LDS atomic per wave reserves the slots
+    // of all its rounds.
+    uint64_t masks[ATT];
+    uint32_t total = 0;
 #pragma unroll
     for (int r = 0; r < ATT; r++) {
-        const uint64_t mask = __ballot(active[r]);
-        if (mask != 0) {
+        masks[r] = __ballot(active[r]);
+        total += (uint32_t)__popcll(masks[r]);
+    }
+    if (total != 0) {
+        uint32_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(&nqueued, total);
+        base = __shfl(base, 0);
+#pragma unroll
+        for (int r = 0; r < ATT; r++) {
             const uint32_t before = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(&nqueued, (uint32_t)__popcll(mask));
-            base = __shfl(base, leader);
+                (uint32_t)(masks[r] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)masks[r], 0u));
             if (active[r]) queue[base + before] = e[r];
+            base += (uint32_t)__popcll(masks[r]);
         }
     }
     __syncthreads();
