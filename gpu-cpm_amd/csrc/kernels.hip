@@ -51,9 +51,13 @@ __device__ __forceinline__ void philox4x32_10(uint32_t x0, uint32_t x1, uint32_t
 // fastest; the centre (k = (3^D - 1) / 2) is the site itself.
 template <int D>
 __device__ __forceinline__ void block_offset(int k, int& ox, int& oy, int& oz) {
-    ox = k % 3 - 1;
-    oy = (k / 3) % 3 - 1;
-    oz = D == 3 ? k / 9 - 1 : 0;
+    // k in [0, 27): k / 3 = (11 k) >> 5 and k / 9 = (57 k) >> 9 are exact
+    // there (checked for every k), and take 24-bit multiplies
+    const uint32_t u = (uint32_t)k;
+    const uint32_t d3 = (u * 11u) >> 5, d9 = (u * 57u) >> 9;
+    ox = (int)(u - 3u * d3) - 1;
+    oy = (int)(d3 - 3u * d9) - 1;
+    oz = D == 3 ? (int)d9 - 1 : 0;
 }
 
 template <int D>
