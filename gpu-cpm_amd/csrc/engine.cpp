@@ -1218,9 +1218,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                   (env_flag("GPUCPM_FULL", false) ? gcpm::kFormFull : 0);
         {
             // attempts drawn per thread by the compacted kernel (measured:
-            // 8 in 3D, 1 in 2D)
+            // 8 in 3D, 2 in 2D, where the pair shares one Philox block)
             const char* e = std::getenv("GPUCPM_ATT");
-            const int att = e ? std::atoi(e) : (h->d == 3 ? 8 : 1);
+            const int att = e ? std::atoi(e) : (h->d == 3 ? 8 : 2);
             pa.form |= gcpm::att_form(att);
         }
         pa.lstride = k.lstride;
