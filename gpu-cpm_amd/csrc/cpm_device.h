@@ -154,7 +154,7 @@ struct PersistArgs {
     uint32_t mcs0;          // first MCS counter
     int nmcs;               // MCS in this launch
     int lh;                 // log2 (same-colour sub-blocks per axis)
-    unsigned int* bar;      // grid barrier {arrivals, generation}
+    unsigned int* bar;      // grid barrier {arrivals, generation, abandoned-wait flag}
     int ltx, lty;           // log2 LDS tile edge (2D tiled kernel)
     int lntx;               // log2 tiles per row
 };
@@ -173,7 +173,8 @@ struct ReplicaDesc {
 struct BatchArgs {
     const ReplicaDesc* reps;
     const int* rep_of;      // block -> replica
-    unsigned int* bars;     // 2 words per replica
+    unsigned int* bars;     // 2 words per replica {arrivals, generation}
+    unsigned int* err;      // abandoned-wait flag of the launch
     int nmcs;               // MCS per replica in this launch
 };
 

@@ -58,6 +58,19 @@ struct Plan {
 };
 constexpr int kMaxThr = 1 << 22;
 
+// Per-phase kernel options (cpm_set_kernel_options); -1 keeps the default.
+// Every option computes the same results.
+struct KernelOpts {
+    int small = -1;    // at most four types: parameters from scalar loads (default in 3D)
+    int compact = -1;  // u = 1: queue the attempts whose sites differ (default on)
+    int full = -1;     // compacted queue carries the whole neighbourhood (default off)
+    int att = -1;      // attempts drawn per thread by the compacted kernel (1, 2, 4, 8)
+    int narrow = -1;   // 16-bit sites when they fit (default on)
+    int box3 = -1;     // 3D, c = 1: stage 32-bit boxes in LDS (default off)
+};
+
+int pick(int v, int dflt) { return v >= 0 ? v : dflt; }
+
 // Online autotuner of the per-phase kernel form (cpm_set_autotune).  Every
 // form computes the same results, so the tuner times a run's own launches:
 // while a workload key is unsettled each launch takes the candidate with the
@@ -270,7 +283,7 @@ struct cpm_engine {
     int64_t* d_csum = nullptr;
     uint32_t* d_first = nullptr;
     gcpm::DevParams* d_prm = nullptr;
-    unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation}
+    unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation, abandoned-wait flag}
     unsigned long long* d_stats = nullptr;  // 400 pair counts + 100 moments
     unsigned long long* h_stats = nullptr;  // their pinned host copy
     uint64_t* d_claims = nullptr;           // cell_sync claim keys [n+1]
@@ -294,6 +307,7 @@ struct cpm_engine {
     Plan last_plan;  // launch form of the last run
     int want_form = -1;  // cpm_set_launch: -1 auto, else Plan::Mode
     int want_tile = 0;   // cpm_set_launch: LDS tile edge (0 = heuristic)
+    KernelOpts opts;     // cpm_set_kernel_options
 
     int next_events(hipEvent_t* e0, hipEvent_t* e1) {
         *e0 = *e1 = nullptr;
@@ -371,10 +385,10 @@ struct cpm_engine {
     }
 
     // Cell bits of the 16-bit lattice for the current cells and types, or 0
-    // when the sites do not fit in 16 bits (or GPUCPM_NARROW=0).
+    // when the sites do not fit in 16 bits (or the kernel options turn the
+    // 16-bit form off).
     int narrow_bits() const {
-        if (const char* e = std::getenv("GPUCPM_NARROW"))
-            if (std::atoi(e) == 0) return 0;
+        if (opts.narrow == 0) return 0;
         int tb = 0;
         while ((1 << tb) < nt) tb++;
         const int bits = 16 - tb;
@@ -506,21 +520,16 @@ struct cpm_engine {
         return -1;
     }
 
-    // Launch form for a run with checkerboard edge c.  GPUCPM_MODE=phase |
-    // global | lds forces a form; GPUCPM_TILE sets the 2D tile edge.
+    // Launch form for a run with checkerboard edge c: one launch per phase
+    // unless cpm_set_launch asks for a persistent form (and tile edge).
     int plan(int c, Plan* p) {
-        const char* env = std::getenv("GPUCPM_MODE");
-        std::string want = env ? env : "phase";
-        if (want_form == Plan::kPhase) want = "phase";
-        if (want_form == Plan::kGlobal) want = "global";
-        if (want_form == Plan::kLds2d) want = "lds";
-        if (want == "phase") {
+        if (want_form < Plan::kGlobal) {
             p->mode = Plan::kPhase;
             return CPM_OK;
         }
-        if (d == 2 && want != "global") {
+        const bool want_lds = want_form == Plan::kLds2d;
+        if (d == 2 && want_lds) {
             int t = std::max(2 * c, std::min(64, L / 16));
-            if (const char* te = std::getenv("GPUCPM_TILE")) t = std::atoi(te);
             if (want_tile > 0) t = want_tile;
             if (t > L) t = L;
             if (is_pow2(t) && t >= 2 * c) {
@@ -537,16 +546,14 @@ struct cpm_engine {
                     return CPM_OK;
                 }
             }
-            if (want == "lds") return fail(CPM_EINVAL, "GPUCPM_MODE=lds: tiles do not fit co-resident");
+            return fail(CPM_EINVAL, "launch form 2: the LDS tiles do not fit co-resident");
         }
         if (d == 3) {
             // The persistent global-memory form is switched off in 3D: at
             // 256^3 (2048 co-resident blocks polling one barrier word) a run
             // did not finish within its time limit.  3D runs one launch per
             // colour phase.
-            if (want == "global") return fail(CPM_ENOTIMPL, "GPUCPM_MODE=global is disabled in 3D");
-            p->mode = Plan::kPhase;
-            return CPM_OK;
+            return fail(CPM_ENOTIMPL, "the persistent launch forms are disabled in 3D");
         }
         int maxb = 0;
         HIP_TRY(gcpm::persist_max_blocks(d, false, 0, &maxb));
@@ -664,7 +671,7 @@ struct cpm_batch {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     std::vector<hipEvent_t> ready;       // one per replica stream
-    unsigned int* d_bars = nullptr;      // 2 words per replica
+    unsigned int* d_bars = nullptr;      // 2 words per replica, then the abandoned-wait flag
     double block_scale = 1.0;            // cpm_batch_set_blocks: blocks per unit of work
     int form = 0;                        // cpm_batch_set_form: 0 per phase, 1 persistent
     gcpm::ReplicaDesc* d_reps = nullptr;
@@ -673,6 +680,7 @@ struct cpm_batch {
     int last_blocks = 0;
     bool autotune = false;  // cpm_batch_set_autotune
     FormTuner tuner;
+    KernelOpts opts;        // cpm_batch_set_kernel_options (small, compact, att)
 
     ~cpm_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -686,6 +694,23 @@ struct cpm_batch {
 };
 
 namespace {
+
+// After persistent launches: wait for them and read the abandoned-wait flag
+// (the last of nwords barrier words).  A set flag means a block of the
+// cooperative grid never became resident while the others waited; the
+// launch ended early, so the run's results are invalid.  The words are reset
+// for the next launch.
+int check_barrier(unsigned int* words, int nwords, hipStream_t s) {
+    unsigned int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, words + nwords - 1, sizeof(flag), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!flag) return CPM_OK;
+    HIP_TRY(hipMemsetAsync(words, 0, sizeof(unsigned int) * nwords, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return fail(CPM_EHIP,
+                "persistent launch: a grid barrier wait was abandoned (not every block of the "
+                "cooperative grid was resident); the run's results are invalid");
+}
 
 int check_handle(cpm_handle h) {
     if (!h) return fail(CPM_EINVAL, "null handle");
@@ -760,8 +785,8 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     if ((err = hipMalloc(&g->d_csum, rows * 3 * 8)) != hipSuccess) return bail(err, "hipMalloc(csum)");
     if ((err = hipMalloc(&g->d_first, rows * 4)) != hipSuccess) return bail(err, "hipMalloc(first)");
     if ((err = hipMalloc(&g->d_prm, sizeof(gcpm::DevParams))) != hipSuccess) return bail(err, "hipMalloc(params)");
-    if ((err = hipMalloc(&g->d_bar, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
-    if ((err = hipMemset(g->d_bar, 0, 2 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
+    if ((err = hipMalloc(&g->d_bar, 3 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
+    if ((err = hipMemset(g->d_bar, 0, 3 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
     if ((err = hipMalloc(&g->d_pref, rows * 3 * 4)) != hipSuccess) return bail(err, "hipMalloc(pref)");
     if (has_act) {  // cpm.cu:192-193,204-205
         g->has_act = true;
@@ -1189,10 +1214,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         pa.k = k;
         pa.lh = lh;
         pa.nsub = 1 << (lh * h->d);
-        // GPUCPM_BOX3=1 stages 3D boxes in LDS (c = 1).  Measured slower than
-        // global reads at 256^3 (61 vs 44 us per phase), so it is opt-in.
-        const char* bx = std::getenv("GPUCPM_BOX3");
-        pa.box3 = bx ? std::atoi(bx) : 0;
+        // box3 stages 3D boxes in LDS (c = 1).  Measured slower than global
+        // reads at 256^3 (61 vs 44 us per phase), so it is opt-in.
+        pa.box3 = pick(h->opts.box3, 0);
         // The per-phase kernels run on the 16-bit lattice when every site
         // fits (the LDS box form reads 32-bit sites only).
         const int bits = pa.box3 ? 0 : h->narrow_bits();
@@ -1205,24 +1229,16 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             if (int rc = h->ensure_wide()) return rc;
             h->narrow_valid = false;
         }
-        // Kernel form (measured fastest by default): GPUCPM_SMALL=1 reads
-        // parameters without LDS staging (default in 3D),
-        // GPUCPM_COMPACT=0 finishes every attempt in its own thread,
-        // GPUCPM_FULL=1 queues whole neighbourhoods.
-        auto env_flag = [](const char* name, bool dflt) {
-            const char* e = std::getenv(name);
-            return e ? std::atoi(e) != 0 : dflt;
-        };
-        pa.form = (env_flag("GPUCPM_SMALL", h->d == 3) ? gcpm::kFormSmall : 0) |
-                  (env_flag("GPUCPM_COMPACT", true) ? gcpm::kFormCompact : 0) |
-                  (env_flag("GPUCPM_FULL", false) ? gcpm::kFormFull : 0);
-        {
-            // attempts drawn per thread by the compacted kernel (measured:
-            // 8 in 3D, 2 in 2D, where the pair shares one Philox block)
-            const char* e = std::getenv("GPUCPM_ATT");
-            const int att = e ? std::atoi(e) : (h->d == 3 ? 8 : 2);
-            pa.form |= gcpm::att_form(att);
-        }
+        // Kernel form (defaults measured fastest): small reads parameters
+        // without LDS staging (default in 3D), compact = 0 finishes every
+        // attempt in its own thread, full queues whole neighbourhoods, att
+        // attempts are drawn per thread by the compacted kernel (8 in 3D, 2
+        // in 2D, where the pair shares one Philox block).
+        const KernelOpts& o = h->opts;
+        pa.form = (pick(o.small, h->d == 3) ? gcpm::kFormSmall : 0) |
+                  (pick(o.compact, 1) ? gcpm::kFormCompact : 0) |
+                  (pick(o.full, 0) ? gcpm::kFormFull : 0) |
+                  gcpm::att_form(pick(o.att, h->d == 3 ? 8 : 2));
         pa.lstride = k.lstride;
         // Autotuner: the candidate forms of this workload (all compute the
         // same results; the special launches -- cell_sync, split phases, the
@@ -1343,6 +1359,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                 h->t_updates += updates_per_mcs * a->inner_iterations;
             }
         }
+        if (int rc = check_barrier(h->d_bar, 3, h->stream)) return rc;
     }
     h->mcs += (uint64_t)total;
     return CPM_OK;
@@ -1469,6 +1486,17 @@ int cpm_set_launch(cpm_handle h, int form, int tile) {
     return CPM_OK;
 }
 
+int cpm_set_kernel_options(cpm_handle h, int small, int compact, int full, int attempts,
+                           int narrow, int box3) {
+    if (int rc = check_handle(h)) return rc;
+    for (int v : {small, compact, full, narrow, box3})
+        if (v < -1 || v > 1) return fail(CPM_EINVAL, "kernel options must be -1 (default), 0 or 1");
+    if (attempts != -1 && attempts != 1 && attempts != 2 && attempts != 4 && attempts != 8)
+        return fail(CPM_EINVAL, "attempts per thread must be -1 (default), 1, 2, 4 or 8");
+    h->opts = KernelOpts{small, compact, full, attempts, narrow, box3};
+    return CPM_OK;
+}
+
 int cpm_set_autotune(cpm_handle h, int on) {
     if (int rc = check_handle(h)) return rc;
     h->autotune = on != 0;
@@ -1544,21 +1572,17 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                 if (int rc = h->init_ring()) return rc;
         }
     }
-    // Kernel form, as for single runs: compacted when u = 1 (GPUCPM_COMPACT),
-    // parameters without staging when every replica has at most four types
-    // (GPUCPM_SMALL; default in 3D), GPUCPM_ATT attempts drawn per thread.
-    auto env_int = [](const char* name, int dflt) {
-        const char* e = std::getenv(name);
-        return e ? std::atoi(e) : dflt;
-    };
+    // Kernel form, as for single runs (cpm_batch_set_kernel_options):
+    // compacted when u = 1, parameters without staging when every replica
+    // has at most four types (default in 3D), att attempts drawn per thread.
     const int u1 = a->updates_per_checkerboard_switch;
     const bool refstream = reps[0].k.refstream != 0;  // (all replicas alike)
-    const bool compact = u1 == 1 && !refstream && !split && env_int("GPUCPM_COMPACT", 1) != 0;
-    bool small = env_int("GPUCPM_SMALL", b->d == 3) != 0;
+    const bool compact = u1 == 1 && !refstream && !split && pick(b->opts.compact, 1) != 0;
+    bool small = pick(b->opts.small, b->d == 3) != 0;
     for (int i = 0; i < n; i++) small = small && reps[i].k.nt <= gcpm::kSmallTypes;
     // Attempts drawn per thread: 1, 4 or 8 (a batch is throughput bound, so
     // fewer, fuller blocks stage the parameters fewer times).
-    int att = compact ? env_int("GPUCPM_ATT", b->d == 3 ? 8 : 4) : 1;
+    int att = compact ? pick(b->opts.att, b->d == 3 ? 8 : 4) : 1;
     att = att >= 8 ? 8 : att >= 4 ? 4 : 1;
     const int form_default = (compact ? gcpm::kFormCompact : 0) | (small ? gcpm::kFormSmall : 0) |
                              gcpm::att_form(att);
@@ -1767,8 +1791,8 @@ int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
     for (int i = 0; i < n; i++)
         if ((err = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming)) != hipSuccess)
             return bail(err, "hipEventCreate");
-    if ((err = hipMalloc(&b->d_bars, 2 * sizeof(unsigned int) * n)) != hipSuccess) return bail(err, "hipMalloc(barriers)");
-    if ((err = hipMemset(b->d_bars, 0, 2 * sizeof(unsigned int) * n)) != hipSuccess) return bail(err, "hipMemset(barriers)");
+    if ((err = hipMalloc(&b->d_bars, sizeof(unsigned int) * (2 * n + 1))) != hipSuccess) return bail(err, "hipMalloc(barriers)");
+    if ((err = hipMemset(b->d_bars, 0, sizeof(unsigned int) * (2 * n + 1))) != hipSuccess) return bail(err, "hipMemset(barriers)");
     // three block layouts at most (the autotuner's attempts-per-thread values)
     if ((err = hipMalloc(&b->d_reps, sizeof(gcpm::ReplicaDesc) * n * 3)) != hipSuccess)
         return bail(err, "hipMalloc(replicas)");
@@ -1895,6 +1919,7 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     ba.reps = b->d_reps;
     ba.rep_of = b->d_rep_of;
     ba.bars = b->d_bars;
+    ba.err = b->d_bars + 2 * n;
     ba.nmcs = a->inner_iterations;
     for (long ch = 0; ch < chunks; ch++) {
         for (int i = 0; i < n; i++)
@@ -1906,6 +1931,7 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         HIP_TRY(hipStreamSynchronize(b->stream));
         HIP_TRY(gcpm::launch_batch(b->d, ba, total, b->stream));
     }
+    if (int rc = check_barrier(b->d_bars, 2 * n + 1, b->stream)) return rc;
     // Each replica's later work waits for the batch.
     HIP_TRY(hipEventRecord(b->done, b->stream));
     for (int i = 0; i < n; i++) {
@@ -1918,6 +1944,18 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
 int cpm_batch_set_autotune(cpm_batch_handle b, int on) {
     if (!b) return fail(CPM_EINVAL, "null batch");
     b->autotune = on != 0;
+    return CPM_OK;
+}
+
+int cpm_batch_set_kernel_options(cpm_batch_handle b, int small, int compact, int attempts) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    for (int v : {small, compact})
+        if (v < -1 || v > 1) return fail(CPM_EINVAL, "kernel options must be -1 (default), 0 or 1");
+    if (attempts != -1 && attempts != 1 && attempts != 4 && attempts != 8)
+        return fail(CPM_EINVAL, "batched attempts per thread must be -1 (default), 1, 4 or 8");
+    b->opts.small = small;
+    b->opts.compact = compact;
+    b->opts.att = attempts;
     return CPM_OK;
 }
 

@@ -548,7 +548,11 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         if ((k.fixed_mask >> (vt >> k.idbits)) & 1u) return;  // a fixed target stays
         if ((k.fixed_mask >> (vs >> k.idbits)) & 1u) {        // a fixed source copies medium
             vs = 0;
-            if (vs == vt) return;
+            // Onto a medium target this is a medium -> medium attempt, which
+            // the reference evaluates and, if accepted, stamps act[target]
+            // (kernels2d.cu:532-534); nothing else changes, so without the
+            // act term it is skipped.
+            if (vs == vt && !(XT && k.act)) return;
         }
     }
 
@@ -1266,11 +1270,21 @@ __device__ __forceinline__ void wait_memory() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-__device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& gen,
-                                             unsigned int nblocks) {
+// A waiting block gives up after this many polls (seconds; a phase takes
+// microseconds): a block that never becomes resident, the one way the
+// barrier can stall, then ends the launch with the error word set instead of
+// hanging it.
+constexpr long kBarrierSpinCap = 1L << 22;
+
+// Returns false (the whole block) when the wait was abandoned; *err then
+// holds 1 and every other waiter leaves at its next poll.
+__device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int& gen,
+                                             unsigned int nblocks, unsigned int* err) {
+    __shared__ int abandoned;
     wait_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
+        int fail = 0;
         const unsigned int g = gen;
         const unsigned int prev =
             __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1279,13 +1293,23 @@ __device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int& ge
             wait_memory();
             __hip_atomic_store(&bar[1], g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g)
+            long polls = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (++polls > kBarrierSpinCap ||
+                    __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    fail = 1;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
+            }
         }
         wait_memory();
+        abandoned = fail;
     }
     gen++;
     __syncthreads();
+    return !abandoned;
 }
 
 // End of a persistent run: fold the last phase's deltas into the lagging
@@ -1306,8 +1330,8 @@ __device__ __forceinline__ void settle_in_kernel(const SweepConsts& k, int G, in
 // each phase and meet a group barrier between phases.
 template <int D>
 __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_t mcs0, int nmcs,
-                                                   int lh, unsigned int* bar, int lb, int nblocks,
-                                                   LdsParams& sp) {
+                                                   int lh, unsigned int* bar, unsigned int* err,
+                                                   int lb, int nblocks, LdsParams& sp) {
     const int gtid = lb * blockDim.x + threadIdx.x;
     const int gsz = nblocks * blockDim.x;
     unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1322,7 +1346,7 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
         const uint32_t mcs = mcs0 + (uint32_t)m;
         for (int r = 0; r < k.rounds; r++) {
             for (int col = 0; col < NCOL; col++, g++) {
-                if (g) grid_barrier(bar, gen, nblocks);
+                if (g && !grid_barrier(bar, gen, nblocks, err)) return;
                 int cx, cy, cz;
                 colour_of<D>(col, cx, cy, cz);
                 const PhaseTables T = tables_of(k, g);
@@ -1337,16 +1361,13 @@ __device__ __forceinline__ void persist_global_run(const SweepConsts& k, uint32_
             }
         }
     }
-    if (g) {
-        grid_barrier(bar, gen, nblocks);
-        settle_in_kernel(k, g, gtid, gsz);
-    }
+    if (g && grid_barrier(bar, gen, nblocks, err)) settle_in_kernel(k, g, gtid, gsz);
 }
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_persist_global(PersistArgs a) {
     __shared__ LdsParams sp;
-    persist_global_run<D>(a.k, a.mcs0, a.nmcs, a.lh, a.bar, blockIdx.x, gridDim.x, sp);
+    persist_global_run<D>(a.k, a.mcs0, a.nmcs, a.lh, a.bar, a.bar + 2, blockIdx.x, gridDim.x, sp);
 }
 
 // Batched replicas in one cooperative launch: block b works for replica
@@ -1357,7 +1378,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_global(BatchArgs a) {
     __shared__ LdsParams sp;
     const int r = a.rep_of[blockIdx.x];
     const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
-    persist_global_run<D>(rd.k, rd.mcs0, a.nmcs, rd.lh, a.bars + 2 * r, blockIdx.x - rd.block0,
+    persist_global_run<D>(rd.k, rd.mcs0, a.nmcs, rd.lh, a.bars + 2 * r, a.err, blockIdx.x - rd.block0,
                           rd.nblocks, sp);
 }
 
@@ -1396,7 +1417,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
         for (int r = 0; r < k.rounds; r++) {
             for (int col = 0; col < 4; col++, g++) {
                 if (g) {
-                    grid_barrier(a.bar, gen, gridDim.x);
+                    if (!grid_barrier(a.bar, gen, gridDim.x, a.bar + 2)) return;
                     for (int i = threadIdx.x; i < ring; i += blockDim.x) {
                         int lx, ly;
                         if (i < S) { lx = i; ly = 0; }
@@ -1421,10 +1442,7 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
             }
         }
     }
-    if (g) {
-        grid_barrier(a.bar, gen, gridDim.x);
-        settle_in_kernel(k, g, gtid, gsz);
-    }
+    if (g && grid_barrier(a.bar, gen, gridDim.x, a.bar + 2)) settle_in_kernel(k, g, gtid, gsz);
 }
 
 // Wave-aggregated updates of per-cell rows: the lanes of a wave are grouped

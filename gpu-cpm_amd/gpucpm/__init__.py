@@ -320,6 +320,17 @@ class Cpm:
         LDS tiles) and LDS tile edge (0 auto).  Results never depend on them."""
         check(lib.cpm_set_launch(self._h, _int(form, "form"), _int(tile, "tile")))
 
+    def set_kernel_options(self, small=None, compact=None, full=None, attempts=None,
+                           narrow=None, box3=None):
+        """Options of the per-phase kernels (None keeps the default): small
+        (scalar parameter loads, at most four types), compact (queue only
+        attempts whose sites differ, u = 1), full (queue whole
+        neighbourhoods), attempts (drawn per thread: 1, 2, 4, 8), narrow
+        (16-bit sites when they fit), box3 (3D, c = 1: LDS boxes).  Results
+        never depend on them."""
+        v = [-1 if x is None else int(x) for x in (small, compact, full, attempts, narrow, box3)]
+        check(lib.cpm_set_kernel_options(self._h, *v))
+
     def last_launch(self):
         """Launch form of the last run: dict(form, kernel, blocks, tile)."""
         f, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -427,6 +438,11 @@ class Batch:
         """0: one launch per colour phase for all replicas (default);
         1: one persistent cooperative launch per chunk (2D)."""
         check(lib.cpm_batch_set_form(self._h, _int(form, "form")))
+
+    def set_kernel_options(self, small=None, compact=None, attempts=None):
+        """Cpm.set_kernel_options for batched runs (attempts: 1, 4 or 8)."""
+        v = [-1 if x is None else int(x) for x in (small, compact, attempts)]
+        check(lib.cpm_batch_set_kernel_options(self._h, *v))
 
     def set_blocks(self, blocks_per_work):
         """Blocks per unit of work in each replica's group (default 1.0)."""

@@ -85,6 +85,8 @@ SIGNATURES = [
     ("cpm_set_launch", _I, [_H, _I, _I]),
     ("cpm_batch_set_blocks", _I, [_H, ctypes.c_double]),
     ("cpm_batch_set_form", _I, [_H, _I]),
+    ("cpm_set_kernel_options", _I, [_H, _I, _I, _I, _I, _I, _I]),
+    ("cpm_batch_set_kernel_options", _I, [_H, _I, _I, _I]),
 ]
 
 if not os.path.exists(LIBPATH):

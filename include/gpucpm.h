@@ -212,6 +212,17 @@ const char* cpm_last_kernel(cpm_handle h);
  * above) and the LDS tile edge (0 = automatic) for this engine's runs; the
  * autotuner sets these.  Results never depend on them. */
 int cpm_set_launch(cpm_handle h, int form, int tile);
+/* Extension (no counterpart in the reference, whose kernel variant is chosen
+ * by template flags, cpm.cu:13-45): options of the per-phase kernels for
+ * this engine's runs, each -1 (default), 0 or 1: small = read at most four
+ * types' parameters with scalar loads (default in 3D); compact = with u = 1
+ * queue only the attempts whose sites differ (default 1); full = the queue
+ * carries whole neighbourhoods (default 0); attempts = attempts drawn per
+ * thread by the compacted kernel, -1 or 1, 2, 4, 8 (default 2 in 2D, 8 in
+ * 3D); narrow = 16-bit sites when they fit (default 1); box3 = 3D with c = 1
+ * stages 32-bit boxes in LDS (default 0).  Results never depend on them. */
+int cpm_set_kernel_options(cpm_handle h, int small, int compact, int full, int attempts,
+                           int narrow, int box3);
 
 /* Extension (no counterpart in the reference, whose parameters are one
  * global __constant__ block, kernels.cu:48-53): batched replicas.  A batch
@@ -236,6 +247,10 @@ int cpm_batch_set_blocks(cpm_batch_handle b, double blocks_per_work);
 int cpm_batch_set_autotune(cpm_batch_handle b, int on);
 /* Batch launch form: 0 = one launch per colour phase, 1 = persistent. */
 int cpm_batch_set_form(cpm_batch_handle b, int form);
+/* Extension: cpm_set_kernel_options for batched runs (small, compact and
+ * attempts per thread: -1 or 1, 4, 8; default 4 in 2D, 8 in 3D).  Each
+ * replica's own narrow option still applies.  Results never depend on them. */
+int cpm_batch_set_kernel_options(cpm_batch_handle b, int small, int compact, int attempts);
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks);
 int cpm_batch_destroy(cpm_batch_handle b);
 

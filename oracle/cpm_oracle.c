@@ -708,11 +708,16 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                         }
                         if (vs == vt) continue; /* kernels2d.cu:276 */
                         /* fixed types (kernels2d.cu:281-290): a fixed target is
-                         * never overwritten; a fixed source copies medium */
+                         * never overwritten; a fixed source copies medium.
+                         * Onto a medium target that is a medium -> medium
+                         * attempt, which the reference evaluates, draws and,
+                         * if accepted, stamps act[target] (:532-534); it
+                         * changes nothing else, so without the act term it
+                         * is skipped */
                         if (p->fixed[vt >> 24]) continue;
                         if (p->fixed[vs >> 24]) {
                             vs = 0;
-                            if (vs == vt) continue;
+                            if (vs == vt && !q.act) continue;
                         }
                         int as, at;
                         int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
@@ -842,8 +847,8 @@ int orc_serial_sweep(int L, int d, int ncells, uint32_t* lat, int32_t* area, int
             if (vs == vt) continue;
             if (p->fixed[vt >> 24]) continue;
             if (p->fixed[vs >> 24]) {
-                vs = 0;
-                if (vs == vt) continue;
+                vs = 0; /* medium -> medium onto a medium target: see orc_sweep */
+                if (vs == vt && !q.act) continue;
             }
             int as, at;
             int64_t dh = delta_h_core(L, d, &nb, lat, area, perim, &q, tx, ty, tz, vs, vt, &as, &at);

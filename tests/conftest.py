@@ -19,3 +19,17 @@ def pytest_configure(config):
     if not os.path.exists(lib) and os.path.exists(hipcc):
         import subprocess
         subprocess.run([sys.executable, os.path.join(ROOT, "gpu-cpm_amd", "build.py"), "-q"], check=True)
+
+
+@pytest.fixture
+def engine_options():
+    """Set kernel options (small, compact, full, attempts, narrow, box3) and
+    the launch form (mode = phase | global | lds, tile) for the engines this
+    test makes through helpers.make_cpm; cleared afterwards."""
+    import helpers
+
+    def set_options(**kw):
+        helpers.ENGINE_OPTIONS.update(kw)
+
+    yield set_options
+    helpers.ENGINE_OPTIONS.clear()

@@ -5,6 +5,26 @@ import numpy as np
 
 import oracle as O
 
+# Kernel options and launch form applied to every engine the GPU tests make
+# through make_cpm (Cpm.set_kernel_options, Cpm.set_launch); set per test by
+# the engine_options fixture (tests/conftest.py).  None of them changes
+# results.
+ENGINE_OPTIONS = {}
+_FORMS = {"phase": 0, "global": 1, "lds": 2}
+
+
+def make_cpm(wl):
+    """wl.make_cpm() with the current test's kernel options and launch form."""
+    sim = wl.make_cpm()
+    opts = dict(ENGINE_OPTIONS)
+    mode, tile = opts.pop("mode", None), int(opts.pop("tile", 0))
+    opts = {k: int(v) for k, v in opts.items()}
+    if opts:
+        sim.set_kernel_options(**opts)
+    if mode is not None:
+        sim.set_launch(_FORMS[mode], tile)
+    return sim
+
 
 def oracle_params(wl) -> "O.Params":
     p = wl.params()

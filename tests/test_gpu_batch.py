@@ -173,12 +173,11 @@ def test_batch_site_width_mixed(form):
 
 
 @pytest.mark.parametrize("att", ["1", "4", "8"])
-def test_batch_attempts_per_thread(att, monkeypatch):
+def test_batch_attempts_per_thread(att):
     """The compacted batch kernel with 1, 4 or 8 attempts drawn per thread
     (each replica's block group covers its sub-blocks in chunks of att x 256)
     equals each replica's oracle run."""
     import gpucpm
-    monkeypatch.setenv("GPUCPM_ATT", att)
     wls = _small_batch()
     print("CASE batch att", att, [w.L for w in wls], flush=True)
     sims = []
@@ -187,7 +186,9 @@ def test_batch_attempts_per_thread(att, monkeypatch):
         s.push_to_gpu()
         s.set_seed(300 + i)
         sims.append(s)
-    gpucpm.Batch(sims).run(**run_kwargs(1, 1, 12))
+    batch = gpucpm.Batch(sims)
+    batch.set_kernel_options(attempts=int(att))
+    batch.run(**run_kwargs(1, 1, 12))
     for i, (wl, s) in enumerate(zip(wls, sims)):
         s.synchronize()
         s.pull_from_gpu()

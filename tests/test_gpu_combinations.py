@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import assert_tables_equal, run_kwargs, run_oracle
+from helpers import assert_tables_equal, run_kwargs, run_oracle, make_cpm
 
 pytestmark = pytest.mark.gpu
 
@@ -60,15 +60,15 @@ def _workload(case):
 
 
 @pytest.mark.parametrize("idx", range(len(CASES)))
-def test_combination(idx, monkeypatch):
+def test_combination(idx, engine_options):
     case = CASES[idx]
-    monkeypatch.setenv("GPUCPM_NARROW", case["narrow"])
+    engine_options(narrow=case["narrow"])
     wl = _workload(case)
     d, c, u = case["d"], case["c"], case["u"]
     n = 8 if d == 2 else 3
     inner = n // 2 if (case["rng"] == "minstd" or "pers" in case["terms"]) else n
     print("CASE combination", idx, case, flush=True)
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(case["seed"])
     sim.set_rng(case["rng"])

@@ -11,7 +11,7 @@ import dataclasses
 import numpy as np
 import pytest
 
-from helpers import assert_tables_equal, run_kwargs, run_oracle
+from helpers import assert_tables_equal, run_kwargs, run_oracle, make_cpm
 
 pytestmark = pytest.mark.gpu
 
@@ -44,7 +44,7 @@ def _retyped(wl, old, new):
 
 
 def _run(wl, c, u, seed, splits, between=None):
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(seed)
     for i, n in enumerate(splits):
@@ -109,7 +109,7 @@ def test_push_after_run_replaces_lattice():
     from gpucpm import workloads as W
     wl = W.sorting2d(L=128, grid=9, edge=10, seed=94)
     _announce("repush", wl.L)
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(3)
     sim.run(**run_kwargs(1, 1, 10))
@@ -129,7 +129,7 @@ def test_push_after_run_replaces_lattice():
     assert_tables_equal(sim.get_cell_table(), tab_o, 2)
 
 
-def test_energy_stats_equal_on_both_lattices(monkeypatch):
+def test_energy_stats_equal_on_both_lattices(engine_options):
     """The dH/dtheta statistics after a run on 16-bit sites equal those
     after the same run on 32-bit sites."""
     from gpucpm import workloads as W
@@ -137,7 +137,7 @@ def test_energy_stats_equal_on_both_lattices(monkeypatch):
     _announce("stats", wl.L)
     out = []
     for narrow in ("1", "0"):
-        monkeypatch.setenv("GPUCPM_NARROW", narrow)
+        engine_options(narrow=narrow)
         sim, lat, _ = _run(wl, 1, 1, 4, [10])
         assert ("unsigned short" in sim.last_launch()["kernel"]) == (narrow == "1")
         out.append((lat, sim.energy_stats()))

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import assert_tables_equal, run_kwargs, run_oracle
+from helpers import assert_tables_equal, run_kwargs, run_oracle, make_cpm
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +20,7 @@ def _gpu():
 
 def _run_gpu(wl, c, u, seed, nmcs, inner=None, splits=None):
     gpucpm, _ = _gpu()
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(seed)
     for n in (splits or [nmcs]):
@@ -39,7 +39,7 @@ def test_tally_after_push_matches_oracle():
     for wl in (W.plain2d(L=64, ncells=20, seed=5), W.sorting2d(L=128, grid=9, edge=10, seed=6),
                W.plain3d(L=32, ncells=40, seed=7)):
         _announce("tally", wl.name, wl.L, wl.d)
-        sim = wl.make_cpm()
+        sim = make_cpm(wl)
         sim.push_to_gpu()
         _, tab = run_oracle(wl, 1, 1, 0, 0)
         assert_tables_equal(sim.get_cell_table(), tab, wl.d)
@@ -132,11 +132,11 @@ def test_full_size_tables_equal_recount():
 
 
 @pytest.mark.parametrize("mode", ["phase", "global", "lds"])
-def test_launch_forms_agree_with_oracle(mode, monkeypatch):
+def test_launch_forms_agree_with_oracle(mode, engine_options):
     """The per-phase, persistent-global and persistent-LDS launch forms all
     reproduce the oracle in 2D; 3D runs per phase."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_MODE", mode)
+    engine_options(mode=mode)
     cases = [(W.plain2d(L=64, ncells=20, seed=21), 2, 4, 40), (W.sorting2d(L=128, grid=9, edge=10, seed=22), 1, 1, 30)]
     if mode == "phase":
         # the persistent global form is switched off in 3D (a 256^3 run
@@ -150,11 +150,28 @@ def test_launch_forms_agree_with_oracle(mode, monkeypatch):
         assert_tables_equal(table, tab_o, wl.d)
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])
-def test_lds_tile_size_does_not_change_results(tile, monkeypatch):
+@pytest.mark.skip(reason="the 3D persistent global form is switched off (CPM_ENOTIMPL): a 256^3 run "
+                         "of k_persist_global<3> (2048 blocks, the occupancy limit) stopped at a grid "
+                         "barrier in an event-timed pass and was killed at its time limit.  The barrier "
+                         "protocol is deadlock-free whenever every block is resident "
+                         "(tests/test_barrier_model.py), so a block of that grid was not resident; what "
+                         "kept it out is not established, so the case stays off the GPU")
+def test_persistent_global_3d_matches_oracle(engine_options):
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_MODE", "lds")
-    monkeypatch.setenv("GPUCPM_TILE", str(tile))
+    engine_options(mode="global")
+    wl = W.plain3d(L=32, ncells=60, seed=23)
+    _announce("mode global 3D", wl.L)
+    _, lat, table = _run_gpu(wl, 1, 1, seed=31, nmcs=8, inner=2)
+    lat_o, tab_o = run_oracle(wl, 1, 1, 31, 8)
+    np.testing.assert_array_equal(lat, lat_o)
+    assert_tables_equal(table, tab_o, 3)
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_lds_tile_size_does_not_change_results(tile, engine_options):
+    _, W = _gpu()
+    engine_options(mode="lds")
+    engine_options(tile=str(tile))
     wl = W.plain2d(L=128, ncells=40, seed=24)
     _announce("tile", tile)
     _, lat, table = _run_gpu(wl, 2, 3, seed=8, nmcs=25)
@@ -164,11 +181,11 @@ def test_lds_tile_size_does_not_change_results(tile, monkeypatch):
 
 
 @pytest.mark.parametrize("box", ["0", "1"])
-def test_3d_box_staging_matches_oracle(box, monkeypatch):
-    """3D with c = 1 reads global memory (GPUCPM_BOX3=0, default) or stages
+def test_3d_box_staging_matches_oracle(box, engine_options):
+    """3D with c = 1 reads global memory (box3 = 0, default) or stages
     16x16x8 boxes in LDS (1); both reproduce the oracle."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_BOX3", box)
+    engine_options(box3=box)
     wl = W.plain3d(L=64, ncells=300, seed=25)
     _announce("box3", box, wl.L)
     _, lat, table = _run_gpu(wl, 1, 1, seed=17, nmcs=6)
@@ -184,7 +201,7 @@ def test_cell_sync_matches_oracle():
     for wl, n in ((W.plain2d(L=64, ncells=20, seed=26), 30), (W.plain3d(L=32, ncells=60, seed=27), 6)):
         _announce("cell_sync", wl.name, wl.L)
         gpucpm, _ = _gpu()
-        sim = wl.make_cpm()
+        sim = make_cpm(wl)
         sim.push_to_gpu()
         sim.set_seed(41)
         sim.run(**run_kwargs(1, 1, n, cell_sync=1))
@@ -201,12 +218,12 @@ def test_cell_sync_matches_oracle():
 
 @pytest.mark.parametrize("mode,d,c,u", [("phase", 2, 1, 1), ("phase", 2, 2, 4), ("global", 2, 2, 3),
                                         ("lds", 2, 1, 1), ("phase", 3, 1, 1), ("phase", 3, 2, 8)])
-def test_fixed_wall_bit_exact(mode, d, c, u, monkeypatch):
+def test_fixed_wall_bit_exact(mode, d, c, u, engine_options):
     """A fixed cell type (Cpm::setFixed, kernels2d.cu:281-290) as a frame
     around the cells: bit-exact against the oracle in every launch form, and
     the wall is unchanged."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_MODE", mode)
+    engine_options(mode=mode)
     base = W.sorting2d(L=128, grid=9, edge=10, seed=28) if d == 2 else W.plain3d(L=32, ncells=60, seed=29)
     wl = W.walled(base)
     _announce("fixed", mode, wl.name, wl.L, c, u)
@@ -225,17 +242,17 @@ def test_fixed_wall_bit_exact(mode, d, c, u, monkeypatch):
     ("0", "1", "0", "1"), ("0", "1", "1", "1"), ("0", "1", "0", "2"), ("0", "1", "0", "4"),
     ("1", "1", "0", "2"), ("1", "1", "0", "4"), ("0", "1", "0", "8"), ("1", "1", "0", "8")])
 @pytest.mark.parametrize("narrow", ["1", "0"])
-def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, narrow, monkeypatch):
+def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, narrow, engine_options):
     """The per-phase kernel forms (parameters staged in LDS or read
     directly; attempts finished in place or queued and finished densely;
     16-bit or 32-bit sites) reproduce the oracle in 2D and 3D, with and
     without a fixed wall."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_NARROW", narrow)
-    monkeypatch.setenv("GPUCPM_SMALL", small)
-    monkeypatch.setenv("GPUCPM_COMPACT", compact)
-    monkeypatch.setenv("GPUCPM_FULL", full)
-    monkeypatch.setenv("GPUCPM_ATT", att)
+    engine_options(narrow=narrow)
+    engine_options(small=small)
+    engine_options(compact=compact)
+    engine_options(full=full)
+    engine_options(attempts=att)
     cases = [(W.sorting2d(L=128, grid=9, edge=10, seed=61), 1, 1, 30),
              (W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=62)), 1, 1, 30),
              (W.sorting2d(L=128, grid=9, edge=10, seed=63), 2, 4, 20),
@@ -251,14 +268,14 @@ def test_phase_kernel_forms_agree_with_oracle(small, compact, full, att, narrow,
 
 @pytest.mark.parametrize("small,compact,att", [("0", "1", "1"), ("1", "1", "4"), ("0", "0", "1"),
                                                ("1", "0", "1")])
-def test_chemotaxis_bit_exact(small, compact, att, monkeypatch):
+def test_chemotaxis_bit_exact(small, compact, att, engine_options):
     """The chemotaxis term (Cpm::setChemotaxis, kernels2d.cu:325-334,
     kernels3d.cu:314-325) with a smooth field, in 2D and 3D, every kernel
     form, bit-exact against the oracle."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_SMALL", small)
-    monkeypatch.setenv("GPUCPM_COMPACT", compact)
-    monkeypatch.setenv("GPUCPM_ATT", att)
+    engine_options(small=small)
+    engine_options(compact=compact)
+    engine_options(attempts=att)
     cases = [(W.chemotactic(W.sorting2d(L=128, grid=9, edge=10, seed=81)), 1, 1, 30),
              (W.chemotactic(W.sorting2d(L=64, grid=5, edge=10, seed=82)), 2, 4, 20),
              (W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=83))), 1, 1, 20),
@@ -278,7 +295,7 @@ def test_chemotaxis_needs_a_field():
     run (the reference would read an unallocated field)."""
     gpucpm, W = _gpu()
     wl = W.sorting2d(L=64, grid=5, edge=10, seed=85)
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.set_constraints(1, lambda_chemotaxis=2.0)
     sim.push_to_gpu()
     with pytest.raises(ValueError):
@@ -288,16 +305,16 @@ def test_chemotaxis_needs_a_field():
 @pytest.mark.parametrize("small,compact,att,mode", [("0", "1", "1", "phase"), ("1", "1", "4", "phase"),
                                                     ("0", "0", "1", "phase"), ("1", "0", "1", "phase"),
                                                     ("0", "1", "1", "global"), ("0", "1", "1", "lds")])
-def test_act_bit_exact(small, compact, att, mode, monkeypatch):
+def test_act_bit_exact(small, compact, att, mode, engine_options):
     """The act term (Cpm::setAct, kernels2d.cu:312-343,386-397,417-473,
     532-534) in 2D and 3D, every kernel and launch form, bit-exact against
     the oracle, act lattice included; combined with chemotaxis and a fixed
     wall."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_SMALL", small)
-    monkeypatch.setenv("GPUCPM_COMPACT", compact)
-    monkeypatch.setenv("GPUCPM_ATT", att)
-    monkeypatch.setenv("GPUCPM_MODE", mode)
+    engine_options(small=small)
+    engine_options(compact=compact)
+    engine_options(attempts=att)
+    engine_options(mode=mode)
     cases = [(W.active(W.sorting2d(L=128, grid=9, edge=10, seed=101)), 2, 1, 30),
              (W.active(W.sorting2d(L=64, grid=5, edge=10, seed=102)), 2, 4, 20),
              (W.active(W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=103)))), 4, 3, 20)]
@@ -314,7 +331,7 @@ def test_act_bit_exact(small, compact, att, mode, monkeypatch):
 
 
 @pytest.mark.parametrize("narrow", ["1", "0"])
-def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
+def test_act_single_site_checkerboard_bit_exact(narrow, engine_options):
     """With shared roles and positions_per_checkerboard = 1 the act term's
     source neighbourhood reaches same-phase targets: each phase is decided
     against its start lattice and then applied (two launches), bit-exact
@@ -322,7 +339,7 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
     stream, and with cell_sync: the decide launch also claims, the apply
     launch checks the claims).  Batches: test_gpu_batch.py."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_NARROW", narrow)
+    engine_options(narrow=narrow)
     cases = [(W.active(W.sorting2d(L=128, grid=9, edge=10, seed=105)), 30, "philox"),
              (W.active(W.chemotactic(W.walled(W.sorting2d(L=64, grid=5, edge=10, seed=106)))), 20,
               "philox"),
@@ -336,7 +353,7 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
         (W.active(W.sorting2d(L=64, grid=5, edge=10, seed=112)), 12, "minstd", True)]
     for wl, n, rng, cs in cases:
         _announce("act c=1", narrow, rng, "cell_sync" if cs else "", wl.name, wl.L, wl.d)
-        sim = wl.make_cpm()
+        sim = make_cpm(wl)
         sim.push_to_gpu()
         sim.set_seed(113)
         sim.set_rng(rng)
@@ -354,15 +371,15 @@ def test_act_single_site_checkerboard_bit_exact(narrow, monkeypatch):
 
 
 @pytest.mark.parametrize("small,compact,att", [("1", "1", "4"), ("0", "1", "1"), ("1", "0", "1")])
-def test_persistence_bit_exact(small, compact, att, monkeypatch):
+def test_persistence_bit_exact(small, compact, att, engine_options):
     """The 3D persistence term (Cpm::setPersistence, kernels3d.cu:327-344)
     with preferred directions updated from the centroid history after every
     inner_iterations chunk (kernels.cu:158-212, cpm.cu:487-514): lattice,
     tables and preferred directions bit-exact against the oracle."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_SMALL", small)
-    monkeypatch.setenv("GPUCPM_COMPACT", compact)
-    monkeypatch.setenv("GPUCPM_ATT", att)
+    engine_options(small=small)
+    engine_options(compact=compact)
+    engine_options(attempts=att)
     cases = [(W.persistent(W.plain3d(L=32, ncells=60, seed=121), lam=(0.0, 300.0), diffusion=(0.0, 0.7)),
               1, 1, 12, 2),
              (W.persistent(W.plain3d(L=32, ncells=40, seed=122), lam=(0.0, 150.0), diffusion=(0.0, 0.0),
@@ -391,14 +408,14 @@ def test_persistence_has_no_effect_in_2d():
 
 
 @pytest.mark.parametrize("compact,att", [("1", "1"), ("1", "4"), ("0", "1")])
-def test_nonshared_roles_bit_exact(compact, att, monkeypatch):
+def test_nonshared_roles_bit_exact(compact, att, engine_options):
     """run(shared=0), the reference's default (kernels2d.cu:227-243): the
     source drawn in the sub-block, the target its neighbour, K = ppt / c
     colours and groups that keep concurrent attempts apart; bit-exact against
     the oracle in 2D and 3D, with every Hamiltonian term."""
     _, W = _gpu()
-    monkeypatch.setenv("GPUCPM_COMPACT", compact)
-    monkeypatch.setenv("GPUCPM_ATT", att)
+    engine_options(compact=compact)
+    engine_options(attempts=att)
     cases = [(W.sorting2d(L=128, grid=9, edge=10, seed=141), 1, 2, 1, 20),
              (W.sorting2d(L=128, grid=9, edge=10, seed=142), 2, 4, 1, 10),
              (W.sorting2d(L=64, grid=5, edge=10, seed=143), 2, 4, 4, 10),
@@ -410,7 +427,7 @@ def test_nonshared_roles_bit_exact(compact, att, monkeypatch):
     for wl, c, ppt, u, n in cases:
         _announce("nonshared", compact, att, wl.name, wl.L, c, ppt, u)
         gpucpm, _ = _gpu()
-        sim = wl.make_cpm()
+        sim = make_cpm(wl)
         sim.push_to_gpu()
         sim.set_seed(151)
         sim.run(**run_kwargs(c, u, n, shared=0, positions_per_thread=ppt))
@@ -428,7 +445,7 @@ def test_nonshared_cell_sync_and_batch():
     from helpers import oracle_params
     wl = W.sorting2d(L=64, grid=5, edge=10, seed=152)
     _announce("nonshared cell_sync")
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(3)
     sim.run(**run_kwargs(1, 1, 10, cell_sync=1, shared=0, positions_per_thread=2))
@@ -464,7 +481,7 @@ def test_cell_sync_several_updates_matches_oracle(d, c, u, shared):
     from helpers import oracle_params
     wl = W.sorting2d(L=64, grid=5, edge=10, seed=161) if d == 2 else W.plain3d(L=32, ncells=60, seed=162)
     _announce("cell_sync u", d, c, u, shared)
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(17)
     n = 6 if d == 2 else 2

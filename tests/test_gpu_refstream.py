@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import assert_tables_equal, run_kwargs, run_oracle
+from helpers import assert_tables_equal, run_kwargs, run_oracle, make_cpm
 
 pytestmark = pytest.mark.gpu
 
@@ -17,7 +17,7 @@ def _announce(*a):
 
 
 def _run(wl, c, u, seed, nmcs, inner, shared, ppt, cell_sync=0):
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_seed(seed)
     sim.set_rng("minstd")
@@ -46,8 +46,8 @@ def _cases():
 
 @pytest.mark.parametrize("narrow", ["1", "0"])
 @pytest.mark.parametrize("idx", range(9))
-def test_refstream_matches_oracle(idx, narrow, monkeypatch):
-    monkeypatch.setenv("GPUCPM_NARROW", narrow)
+def test_refstream_matches_oracle(idx, narrow, engine_options):
+    engine_options(narrow=narrow)
     name, wl, c, u, shared, ppt, nmcs, inner = _cases()[idx]
     _announce("refstream", name, narrow, wl.L, wl.d, c, u, shared, ppt)
     sim, lat, table = _run(wl, c, u, 0x9E3779B9 + idx, nmcs, inner, shared, ppt)
@@ -108,7 +108,7 @@ def test_refstream_refusals_and_switching_back():
     from gpucpm import workloads as W
     wl = W.sorting2d(L=64, grid=5, edge=10, seed=91)
     _announce("refstream refusals", wl.L)
-    sim = wl.make_cpm()
+    sim = make_cpm(wl)
     sim.push_to_gpu()
     sim.set_rng("minstd")
     with pytest.raises(NotImplementedError):

@@ -395,6 +395,55 @@ def test_fixed_type_never_moves_and_copies_medium(serial):
             assert np.count_nonzero(wall_after != wall_before) > 0
 
 
+@pytest.mark.parametrize("stick,expect", [(0, "most"), (100, "most"), (-100, "none")])
+def test_fixed_source_onto_medium_stamps_act(stick, expect):
+    """kernels2d.cu:281-290,532-534: a copy from a fixed source onto a
+    medium target is a medium -> medium attempt that the reference still
+    evaluates and, when accepted, stamps act[target] = tick.  A fixed frame in
+    pure medium, no cells: that is the only possible move, the lattice never
+    changes, and only medium sites next to the frame get stamps.  Its energy
+    is the adhesion difference, -S(0, wall) per wall neighbour (stickiness of
+    medium to the wall): 0 or negative is always accepted, and with c = 1
+    every site is a target once per MCS with a uniform source, so a site with
+    w wall neighbours is stamped within k MCS with probability
+    1 - (1 - w/8)^k; +100 per neighbour at T = 10 is never accepted."""
+    L, k, mcs0 = 32, 3, 5
+    lat = np.zeros((L, L), np.uint32)
+    lat[0, :] = lat[:, 0] = (3 << 24) | 1
+    act = np.zeros((L, L), np.uint32)
+    fx = np.zeros(20)
+    fx[3] = 1
+    S = np.zeros((20, 20), np.int64)
+    S[0, 3] = stick
+    p = O.Params.make(10, S=S, fixed=fx, lam_act=[0, 5.0], act_max=[0, 40], act=act)
+    before = lat.copy()
+    tab = O.tally(lat, 2, 1)
+    O.sweep(lat, 2, tab, p, 1, 1, seed=17, mcs0=mcs0, nmcs=k)
+    np.testing.assert_array_equal(lat, before)
+    wall = (lat >> 24) == 3
+    near = np.zeros_like(wall)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= np.roll(np.roll(wall, dy, 0), dx, 1)
+    near &= ~wall
+    stamped = act != 0
+    assert not np.any(stamped & ~near)
+    assert np.all((act[stamped] >= mcs0) & (act[stamped] < mcs0 + k))
+    if expect == "none":
+        assert not np.any(stamped)
+        return
+    # straight runs of the frame: w = 3 wall neighbours
+    w = np.zeros((L, L), int)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            w += np.roll(np.roll(wall, dy, 0), dx, 1)
+    sel = near & (w == 3)
+    frac = stamped[sel].mean()
+    pexp = 1 - (5 / 8) ** k
+    sd = np.sqrt(pexp * (1 - pexp) / sel.sum())
+    assert abs(frac - pexp) < 4 * sd, (frac, pexp, sd)
+
+
 # --------------------------------------------------------- chemotaxis ------
 
 def _one_cell(L=16, d=2, t=1):
