@@ -224,10 +224,27 @@ hipError_t launch_convert(uint32_t* wide, uint16_t* narrow, size_t sites, int id
                           hipStream_t s);
 hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* anchor, int64_t* csum,
                        hipStream_t s);
-// pairs[20*20] and mom[20*5] must be zero on entry; ctype[n+1] is scratch.
-hipError_t launch_stats(int d, const uint32_t* lat, const uint16_t* lat16, int idbits, int lgL,
-                        const int64_t* tab, int ncells, unsigned long long* pairs,
-                        unsigned long long* mom, int32_t* ctype, hipStream_t s);
+// Energy statistics of one engine in a batched statistics launch: pair
+// counts over its lattice (16- or 32-bit sites) and per-type moments over
+// its settled cell table.  out[20*20 + 20*5] must be zero on entry.  Blocks
+// [pair0, cell0) of the launch count its pairs (stats_pair_blocks of them),
+// [cell0, cell0 + stats_cell_blocks) its cell moments.
+struct StatsDesc {
+    const void* lat;
+    int narrow;             // lat holds 16-bit sites (cell | type << idbits)
+    int lgL;
+    uint32_t idbits;
+    int nt;                 // types present (at most 4: packed counters)
+    const int64_t* tab;     // settled table rows (perimeter << 32 | area)
+    const int32_t* ctype;   // [n+1] type of each cell (set at push)
+    int ncells;
+    unsigned long long* out;
+    int pair0, cell0;
+};
+int stats_pair_blocks(int d, int lgL);
+int stats_cell_blocks(int ncells);
+hipError_t launch_stats_batch(int d, const StatsDesc* descs, const int* eng_of, int blocks,
+                              hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 
 constexpr int kBlock = 256;  // threads per workgroup of the sweep kernels
 

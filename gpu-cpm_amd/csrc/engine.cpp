@@ -284,11 +284,9 @@ struct cpm_engine {
     uint32_t* d_first = nullptr;
     gcpm::DevParams* d_prm = nullptr;
     unsigned int* d_bar = nullptr;  // grid barrier state {arrivals, generation, abandoned-wait flag}
-    unsigned long long* d_stats = nullptr;  // 400 pair counts + 100 moments
-    unsigned long long* h_stats = nullptr;  // their pinned host copy
     uint64_t* d_claims = nullptr;           // cell_sync claim keys [n+1]
     uint32_t sync_tag = 0;                  // cell_sync phase tag
-    int32_t* d_ctype = nullptr;             // [n+1] cell types (stats scratch)
+    int32_t* d_ctype = nullptr;             // [n+1] cell types at push (energy statistics)
     uint32_t* d_thr = nullptr;
     int thr_cap = 0;
     int nthr = 0;
@@ -337,8 +335,6 @@ struct cpm_engine {
         (void)hipFree(d_first);
         (void)hipFree(d_prm);
         (void)hipFree(d_bar);
-        (void)hipFree(d_stats);
-        (void)hipHostFree(h_stats);
         (void)hipFree(d_claims);
         (void)hipFree(d_ctype);
         (void)hipFree(d_thr);
@@ -788,6 +784,7 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     if ((err = hipMalloc(&g->d_bar, 3 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMalloc(barrier)");
     if ((err = hipMemset(g->d_bar, 0, 3 * sizeof(unsigned int))) != hipSuccess) return bail(err, "hipMemset(barrier)");
     if ((err = hipMalloc(&g->d_pref, rows * 3 * 4)) != hipSuccess) return bail(err, "hipMalloc(pref)");
+    if ((err = hipMalloc(&g->d_ctype, rows * 4)) != hipSuccess) return bail(err, "hipMalloc(ctype)");
     if (has_act) {  // cpm.cu:192-193,204-205
         g->has_act = true;
         if ((err = hipHostMalloc((void**)&g->h_act, g->sites * 4, hipHostMallocDefault)) != hipSuccess)
@@ -1016,6 +1013,7 @@ int cpm_push(cpm_handle h) {
     if (h->has_act)  // cpm.cu:548
         HIP_TRY(hipMemcpyAsync(h->d_act, h->h_act, h->sites * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(h->d_lat, h->h_lat, h->sites * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h->d_ctype, h->ctype.data(), rows * 4, hipMemcpyHostToDevice, s));
     h->wide_valid = true;
     h->narrow_valid = false;
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
@@ -1549,6 +1547,60 @@ int cpm_get_mcs(cpm_handle h, uint64_t* mcs) {
 
 namespace {
 
+// Device buffers and stream of the batched statistics launch
+// (cpm_energy_stats_many), shared by every engine of the process and kept
+// until it ends; calls are serialized.
+struct StatsRunner {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr, e0 = nullptr, e1 = nullptr;
+    unsigned long long* d_out = nullptr;  // [engines][500]
+    unsigned long long* h_out = nullptr;  // pinned copy
+    gcpm::StatsDesc* d_desc = nullptr;
+    int* d_map = nullptr;
+    size_t cap = 0, map_cap = 0;
+    double last_ms = 0.0;  // kernel time of the last call
+
+    int ensure(int n) {
+        if (!stream) {
+            HIP_TRY(hipStreamCreate(&stream));
+            HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            HIP_TRY(hipEventCreate(&e0));
+            HIP_TRY(hipEventCreate(&e1));
+        }
+        if ((size_t)n > cap) {
+            constexpr size_t nw = CPM_NTYPES * CPM_NTYPES + CPM_NTYPES * 5;
+            (void)hipFree(d_out);
+            (void)hipFree(d_desc);
+            if (h_out) (void)hipHostFree(h_out);
+            d_out = nullptr;
+            d_desc = nullptr;
+            h_out = nullptr;
+            cap = 0;
+            HIP_TRY(hipMalloc(&d_out, sizeof(unsigned long long) * nw * n));
+            HIP_TRY(hipMalloc(&d_desc, sizeof(gcpm::StatsDesc) * n));
+            HIP_TRY(hipHostMalloc((void**)&h_out, sizeof(unsigned long long) * nw * n, hipHostMallocDefault));
+            cap = (size_t)n;
+        }
+        return CPM_OK;
+    }
+    int ensure_map(size_t blocks) {
+        if (blocks > map_cap) {
+            (void)hipFree(d_map);
+            d_map = nullptr;
+            map_cap = 0;
+            HIP_TRY(hipMalloc(&d_map, sizeof(int) * blocks));
+            map_cap = blocks;
+        }
+        return CPM_OK;
+    }
+};
+
+StatsRunner& stats_runner() {
+    static StatsRunner* r = new StatsRunner();  // never destroyed: it may outlive the HIP runtime
+    return *r;
+}
+
 // Batched run, one launch per colour phase for all replicas (every replica
 // has the same c and u, hence the same phase sequence).  split: every phase
 // is a decide launch and an apply launch (as cpm_run with the act term,
@@ -2029,31 +2081,73 @@ int cpm_energy_grad(cpm_handle h, int ntypes, const double* theta, double* H, do
 int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments) {
     if (!sims || n < 0) return fail(CPM_EINVAL, "null engine list");
     if (!pairs || !moments) return fail(CPM_EINVAL, "null output");
-    const int np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5;
-    // Queue every engine's statistics on its own stream, then wait once.
+    constexpr int np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5, nw = np + nm;
     for (int i = 0; i < n; i++) {
-        cpm_handle h = sims[i];
-        if (int rc = check_handle(h)) return rc;
-        if (!h->pushed) return fail(CPM_ESTATE, "energy statistics exist only after push_to_gpu");
-        if (!h->d_stats) HIP_TRY(hipMalloc(&h->d_stats, sizeof(unsigned long long) * (np + nm)));
-        if (!h->d_ctype) HIP_TRY(hipMalloc(&h->d_ctype, sizeof(int32_t) * ((size_t)h->n + 1)));
-        if (!h->h_stats) HIP_TRY(hipHostMalloc((void**)&h->h_stats, sizeof(unsigned long long) * (np + nm),
-                                               hipHostMallocDefault));
-        HIP_TRY(hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * (np + nm), h->stream));
+        if (int rc = check_handle(sims[i])) return rc;
+        if (!sims[i]->pushed) return fail(CPM_ESTATE, "energy statistics exist only after push_to_gpu");
+    }
+    if (n == 0) return CPM_OK;
+    StatsRunner& S = stats_runner();
+    std::lock_guard<std::mutex> guard(S.mu);
+    if (int rc = S.ensure(n)) return rc;
+    double ms_total = 0.0;
+    for (int d = 2; d <= 3; d++) {
+        // one launch per dimensionality over its engines, each read from
         // whichever lattice copy is current (no conversion)
-        const bool narrow = !h->wide_valid;
-        HIP_TRY(gcpm::launch_stats(h->d, h->d_lat, narrow ? h->d_lat16 : nullptr, narrow ? h->lat16_bits : 24,
-                                   h->lgL, h->d_tab[0], h->n, h->d_stats, h->d_stats + np, h->d_ctype,
-                                   h->stream));
-        HIP_TRY(hipMemcpyAsync(h->h_stats, h->d_stats, sizeof(unsigned long long) * (np + nm),
-                               hipMemcpyDeviceToHost, h->stream));
+        std::vector<int> idx;
+        for (int i = 0; i < n; i++)
+            if (sims[i]->d == d) idx.push_back(i);
+        if (idx.empty()) continue;
+        std::vector<gcpm::StatsDesc> desc(idx.size());
+        std::vector<int> eng_of;
+        for (size_t j = 0; j < idx.size(); j++) {
+            cpm_handle h = sims[idx[j]];
+            const bool narrow = !h->wide_valid;
+            gcpm::StatsDesc& e = desc[j];
+            e.lat = narrow ? (const void*)h->d_lat16 : (const void*)h->d_lat;
+            e.narrow = narrow ? 1 : 0;
+            e.lgL = h->lgL;
+            e.idbits = narrow ? (uint32_t)h->lat16_bits : 24u;
+            e.nt = h->nt;
+            e.tab = h->d_tab[0];
+            e.ctype = h->d_ctype;
+            e.ncells = h->n;
+            e.out = S.d_out + (size_t)idx[j] * nw;
+            e.pair0 = (int)eng_of.size();
+            eng_of.insert(eng_of.end(), gcpm::stats_pair_blocks(d, h->lgL), (int)j);
+            e.cell0 = (int)eng_of.size();
+            eng_of.insert(eng_of.end(), gcpm::stats_cell_blocks(h->n), (int)j);
+            // the statistics read the lattice after the engine's queued work
+            HIP_TRY(hipEventRecord(S.ready, h->stream));
+            HIP_TRY(hipStreamWaitEvent(S.stream, S.ready, 0));
+        }
+        if (int rc = S.ensure_map(eng_of.size())) return rc;
+        HIP_TRY(hipMemsetAsync(S.d_out, 0, sizeof(unsigned long long) * nw * (size_t)n, S.stream));
+        HIP_TRY(hipMemcpyAsync(S.d_desc, desc.data(), sizeof(gcpm::StatsDesc) * desc.size(),
+                               hipMemcpyHostToDevice, S.stream));
+        HIP_TRY(hipMemcpyAsync(S.d_map, eng_of.data(), sizeof(int) * eng_of.size(), hipMemcpyHostToDevice,
+                               S.stream));
+        HIP_TRY(gcpm::launch_stats_batch(d, S.d_desc, S.d_map, (int)eng_of.size(), S.stream, S.e0, S.e1));
+        HIP_TRY(hipMemcpyAsync(S.h_out, S.d_out, sizeof(unsigned long long) * nw * (size_t)n,
+                               hipMemcpyDeviceToHost, S.stream));
+        HIP_TRY(hipStreamSynchronize(S.stream));  // (desc and eng_of are host temporaries)
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, S.e0, S.e1));
+        ms_total += ms;
+        for (int i : idx) {
+            for (int j = 0; j < np; j++) pairs[(size_t)i * np + j] = (int64_t)S.h_out[(size_t)i * nw + j];
+            for (int j = 0; j < nm; j++) moments[(size_t)i * nm + j] = (int64_t)S.h_out[(size_t)i * nw + np + j];
+        }
     }
-    for (int i = 0; i < n; i++) {
-        cpm_handle h = sims[i];
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        for (int j = 0; j < np; j++) pairs[(size_t)i * np + j] = (int64_t)h->h_stats[j];
-        for (int j = 0; j < nm; j++) moments[(size_t)i * nm + j] = (int64_t)h->h_stats[np + j];
-    }
+    S.last_ms = ms_total;
+    return CPM_OK;
+}
+
+int cpm_energy_stats_timing(double* kernel_ms) {
+    if (!kernel_ms) return fail(CPM_EINVAL, "null output");
+    StatsRunner& S = stats_runner();
+    std::lock_guard<std::mutex> guard(S.mu);
+    *kernel_ms = S.last_ms;
     return CPM_OK;
 }
 

@@ -1587,59 +1587,126 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
     del[c] = 0;
 }
 
-// Sufficient statistics of H(sigma; theta) (the backward pass of dH/dtheta).
-// Pair counts: every unordered unlike-cell neighbour pair is seen once from
-// the site whose offset to the other is lexicographically positive
-// ((dz, dy, dx) > 0: 4 offsets in 2D, 13 in 3D), counted by the type pair
-// (min, max) in an LDS histogram and added to pairs[20*20] once per block.
-// Each site also records its cell's type.
-// ST: the 32-bit lattice (idbits 24) or its 16-bit form.
-template <int D, class ST>
-__global__ __launch_bounds__(kBlock) void k_stats_pairs(const ST* __restrict__ lat, int lgL,
-                                                        uint32_t idbits, unsigned long long* pairs,
-                                                        int32_t* ctype) {
+// Sufficient statistics of H(sigma; theta) (the backward pass of dH/dtheta)
+// of many engines in one launch.  Block b serves engine r = eng_of[b]:
+// blocks [pair0, pair0 + npair) count the unordered unlike-cell neighbour
+// pairs by type pair; blocks [cell0, cell0 + ncell) sum the per-type cell
+// moments.  Each unordered pair is seen once, from the site whose offset to
+// the other is lexicographically positive ((dz, dy, dx) > 0: 4 offsets in
+// 2D, 13 in 3D).
+//
+// Pairs, at most four types present: every lane keeps the counts of the ten
+// type pairs (a <= b < 4) as 16-bit fields of three 64-bit words (at most
+// kStatsSites * 13 * 64 per wave, below 2^16), the words are summed across
+// the wave, and lane 0 adds the ten counts to the block's LDS counters;
+// the block then adds them to the engine's global counters once.  More
+// types: one LDS histogram add per unlike pair.
+// Cells: a cell's type is fixed (flips copy whole values), so it is taken
+// from the table made at push (ctype); per-type moments n, sum A, sum A^2,
+// sum P, sum P^2 over live cells are summed across the wave, then added to
+// the engine's counters (four types or fewer; else per-cell LDS adds).
+constexpr int kStatsSites = 8;  // sites per thread in the pair blocks
+
+__device__ __forceinline__ int pair_slot4(uint32_t a, uint32_t b) {
+    // (a, b), a <= b < 4, in row-major upper-triangle order: 0..9
+    return (int)(a * 4u - ((a * (a - 1u)) >> 1) + (b - a));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+template <int D>
+__device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
     __shared__ unsigned int hist[kNTypes * kNTypes];
+    constexpr int NB = D == 3 ? 27 : 9;
+    const int lgL = e.lgL, Lm = (1 << lgL) - 1;
+    const uint32_t nsites = 1u << (D * lgL);
+    const bool four = e.nt <= 4;
     for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    const uint32_t nsites = 1u << (D * lgL);
-    const int Lm = (1 << lgL) - 1;
-    constexpr int NB = D == 3 ? 27 : 9;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nsites; i += gridDim.x * blockDim.x) {
-        const uint32_t v = lat[i];
-        const uint32_t t = v >> idbits, cell = v & ((1u << idbits) - 1u);
-        if (cell) ctype[cell] = (int32_t)t;
+    uint64_t w[3] = {0, 0, 0};
+    const uint32_t base = (uint32_t)lb * (kBlock * kStatsSites);
+#pragma unroll 1
+    for (int k = 0; k < kStatsSites; k++) {
+        const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
+        if (i >= nsites) break;
+        auto site = [&](uint32_t j) -> uint32_t {
+            return e.narrow ? (uint32_t)static_cast<const uint16_t*>(e.lat)[j]
+                            : static_cast<const uint32_t*>(e.lat)[j];
+        };
+        const uint32_t v = site(i);
+        const uint32_t t = v >> e.idbits;
         const int x = (int)(i & Lm), y = (int)((i >> lgL) & Lm), z = D == 3 ? (int)(i >> (2 * lgL)) : 0;
 #pragma unroll
         for (int n = NB / 2 + 1; n < NB; n++) {  // offsets after the centre: the positive half
             int ox, oy, oz;
             block_offset<D>(n, ox, oy, oz);
-            const uint32_t vn = lat[site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL)];
+            const uint32_t vn = site(site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL));
             if (vn != v) {
-                const uint32_t tn = vn >> idbits;
+                const uint32_t tn = vn >> e.idbits;
                 const uint32_t a = t < tn ? t : tn, b = t < tn ? tn : t;
-                atomicAdd(&hist[a * kNTypes + b], 1u);
+                if (four) {
+                    const int sl = pair_slot4(a, b);
+                    const uint64_t inc = 1ull << (16 * (sl & 3));
+                    w[0] += sl < 4 ? inc : 0ull;
+                    w[1] += (sl >> 2) == 1 ? inc : 0ull;
+                    w[2] += sl >= 8 ? inc : 0ull;
+                } else {
+                    atomicAdd(&hist[a * kNTypes + b], 1u);
+                }
             }
+        }
+    }
+    if (four) {  // (uniform per block)
+#pragma unroll
+        for (int j = 0; j < 3; j++) w[j] = wave_sum_u64(w[j]);
+        if (__lane_id() == 0) {
+#pragma unroll
+            for (uint32_t a = 0; a < 4; a++)
+#pragma unroll
+                for (uint32_t b = a; b < 4; b++) {
+                    const int sl = pair_slot4(a, b);
+                    const uint32_t c = (uint32_t)(w[sl >> 2] >> (16 * (sl & 3))) & 0xFFFFu;
+                    if (c) atomicAdd(&hist[a * kNTypes + b], c);
+                }
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x)
-        if (hist[i]) atomicAdd(&pairs[i], (unsigned long long)hist[i]);
+        if (hist[i]) atomicAdd(&e.out[i], (unsigned long long)hist[i]);
 }
 
-// Per-type moments of the cell tables: n, sum A, sum A^2, sum P, sum P^2
-// over live cells (area > 0), mom[5 * type + k].
-__global__ __launch_bounds__(kBlock) void k_stats_cells(const int64_t* __restrict__ tab,
-                                                        const int32_t* __restrict__ ctype,
-                                                        int ncells, unsigned long long* mom) {
+__device__ __forceinline__ void stats_cells_block(const StatsDesc& e, int lb) {
     __shared__ unsigned long long acc[kNTypes * 5];
+    const bool four = e.nt <= 4;
     for (int i = threadIdx.x; i < kNTypes * 5; i += blockDim.x) acc[i] = 0;
     __syncthreads();
-    for (int c = 1 + blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += gridDim.x * blockDim.x) {
-        const int64_t row = tab[c];
-        const uint64_t A = (uint32_t)row;
-        if (!A) continue;
-        const uint64_t P = (uint64_t)(row >> 32);
-        unsigned long long* m = acc + 5 * ctype[c];
+    const int c = 1 + lb * kBlock + (int)threadIdx.x;
+    uint64_t A = 0, P = 0;
+    int t = 0;
+    if (c <= e.ncells) {
+        const int64_t row = e.tab[c];
+        A = (uint32_t)row;
+        P = (uint64_t)(row >> 32);
+        t = e.ctype[c];
+    }
+    if (four) {
+#pragma unroll
+        for (int ty = 0; ty < 4; ty++) {
+            const bool mine = A != 0 && t == ty;
+            const uint64_t v[5] = {mine ? 1ull : 0ull, mine ? A : 0ull, mine ? A * A : 0ull,
+                                   mine ? P : 0ull, mine ? P * P : 0ull};
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const uint64_t sum = wave_sum_u64(v[m]);
+                if (__lane_id() == 0 && sum) atomicAdd(&acc[5 * ty + m], (unsigned long long)sum);
+            }
+        }
+    } else if (A) {
+        unsigned long long* m = acc + 5 * t;
         atomicAdd(&m[0], 1ull);
         atomicAdd(&m[1], (unsigned long long)A);
         atomicAdd(&m[2], (unsigned long long)(A * A));
@@ -1648,27 +1715,34 @@ __global__ __launch_bounds__(kBlock) void k_stats_cells(const int64_t* __restric
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kNTypes * 5; i += blockDim.x)
-        if (acc[i]) atomicAdd(&mom[i], acc[i]);
+        if (acc[i]) atomicAdd(&e.out[kNTypes * kNTypes + i], acc[i]);
 }
 
-hipError_t launch_stats(int d, const uint32_t* lat, const uint16_t* lat16, int idbits, int lgL,
-                        const int64_t* tab, int ncells, unsigned long long* pairs,
-                        unsigned long long* mom, int32_t* ctype, hipStream_t s) {
-    const uint32_t nsites = 1u << (d * lgL);
-    const int blocks = (int)std::min<uint32_t>((nsites + kBlock - 1) / kBlock, 4096u);
-    const uint32_t ib = (uint32_t)idbits;
-    if (lat16) {
-        if (d == 2)
-            hipLaunchKernelGGL((k_stats_pairs<2, uint16_t>), dim3(blocks), dim3(kBlock), 0, s, lat16, lgL, ib, pairs, ctype);
-        else
-            hipLaunchKernelGGL((k_stats_pairs<3, uint16_t>), dim3(blocks), dim3(kBlock), 0, s, lat16, lgL, ib, pairs, ctype);
-    } else if (d == 2) {
-        hipLaunchKernelGGL((k_stats_pairs<2, uint32_t>), dim3(blocks), dim3(kBlock), 0, s, lat, lgL, ib, pairs, ctype);
-    } else {
-        hipLaunchKernelGGL((k_stats_pairs<3, uint32_t>), dim3(blocks), dim3(kBlock), 0, s, lat, lgL, ib, pairs, ctype);
-    }
-    const int cb = std::max(1, std::min(1024, (ncells + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_stats_cells, dim3(cb), dim3(kBlock), 0, s, tab, ctype, ncells, mom);
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_stats_batch(const StatsDesc* __restrict__ descs,
+                                                        const int* __restrict__ eng_of) {
+    const int r = eng_of[blockIdx.x];
+    const StatsDesc e = descs[r];  // uniform per block: scalar loads
+    const int b = (int)blockIdx.x;
+    if (b < e.cell0)
+        stats_pairs_block<D>(e, b - e.pair0);
+    else
+        stats_cells_block(e, b - e.cell0);
+}
+
+int stats_pair_blocks(int d, int lgL) {
+    const long nsites = 1L << (d * lgL);
+    return (int)((nsites + (long)kBlock * kStatsSites - 1) / ((long)kBlock * kStatsSites));
+}
+
+int stats_cell_blocks(int ncells) { return (ncells + kBlock - 1) / kBlock; }
+
+hipError_t launch_stats_batch(int d, const StatsDesc* descs, const int* eng_of, int blocks,
+                              hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (d == 2)
+        hipExtLaunchKernelGGL(k_stats_batch<2>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of);
+    else
+        hipExtLaunchKernelGGL(k_stats_batch<3>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of);
     return hipGetLastError();
 }
 

@@ -385,8 +385,8 @@ class Cpm:
 
 
 def energy_stats_many(sims):
-    """Cpm.energy_stats of several engines at once (each on its own stream,
-    one wait): pairs (n, 20, 20) and moments (n, 20, 5), int64."""
+    """Cpm.energy_stats of several engines at once (one statistics launch
+    for all of them): pairs (n, 20, 20) and moments (n, 20, 5), int64."""
     sims = list(sims)
     n = len(sims)
     pairs = np.zeros((n, 20, 20), np.int64)
@@ -396,6 +396,14 @@ def energy_stats_many(sims):
     check(lib.cpm_energy_stats_many(ctypes.cast(hs, ctypes.POINTER(ctypes.c_void_p)), n,
                                     pairs.ctypes.data_as(P), mom.ctypes.data_as(P)))
     return pairs, mom
+
+
+def energy_stats_kernel_ms():
+    """Kernel time (ms) of the last energy_stats / energy_stats_many call:
+    one statistics launch covers every engine of a call."""
+    v = ctypes.c_double()
+    check(lib.cpm_energy_stats_timing(ctypes.byref(v)))
+    return v.value
 
 
 class Batch:

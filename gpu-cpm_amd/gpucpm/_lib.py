@@ -87,6 +87,7 @@ SIGNATURES = [
     ("cpm_batch_set_form", _I, [_H, _I]),
     ("cpm_set_kernel_options", _I, [_H, _I, _I, _I, _I, _I, _I]),
     ("cpm_batch_set_kernel_options", _I, [_H, _I, _I, _I]),
+    ("cpm_energy_stats_timing", _I, [_P(ctypes.c_double)]),
 ]
 
 if not os.path.exists(LIBPATH):

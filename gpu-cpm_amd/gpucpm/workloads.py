@@ -10,6 +10,8 @@ Each builder returns a ``Workload``: the initial lattice (uint32, ids
                  area + perimeter constraints, differential adhesion.
 - ``plain3d``    256^3, 20000 cubes of 5^3 at random non-overlapping positions,
                  volume + surface constraints, 26-neighbourhood.
+- ``sorting_small_cells``  the reference sorting example's Hamiltonian
+                 (3x3 cells, area term only), which sorts visibly.
 - ``batched_sorting``  64 sorting replicas of mixed sizes and parameters.
 - ``walled``     any of these inside a one-site frame of a fixed cell type.
 - ``chemotactic`` any of these with a chemotaxis term and a smooth field.
@@ -145,6 +147,30 @@ def sorting2d(L=1024, grid=63, edge=10, seed=2, J_AB=11, temperature=10, ncells=
     cons.append(dict(cell_type=2, other_cell_type=2, adhesion=14))
     cons.append(dict(cell_type=1, other_cell_type=2, adhesion=int(J_AB)))
     return Workload("sorting2d", L, 2, n, int(temperature), lat, cons)
+
+
+def sorting_small_cells(L=256, seed=5, J_AB=11, temperature=10):
+    """The Hamiltonian of the reference's own sorting example (examples/
+    sorting.py), which sorts visibly: 3x3 cells on a 6-site grid inside a
+    disc of radius L/3, types 1/2 at random; lam_A = 1, A0 = 40, no
+    perimeter term; J_m1 = J_m2 = 16, J_11 = 14, J_22 = 2, J_12 = J_AB,
+    T = 10.  (The 10x10 squares of ``sorting2d`` under its perimeter
+    pressure keep their interfaces in place, so J_12 leaves no trace there.)"""
+    rng = np.random.default_rng(seed)
+    lat = np.zeros((L, L), np.uint32)
+    n = 0
+    for y in range(0, L - 1, 6):
+        for x in range(0, L - 1, 6):
+            if (x - L / 2) ** 2 + (y - L / 2) ** 2 < (L * L) / 9:
+                n += 1
+                t = rng.integers(1, 3)
+                lat[y:y + 3, x:x + 3] = (np.uint32(t) << np.uint32(24)) | np.uint32(n)
+    cons = [dict(cell_type=t, lambda_area=1, target_area=40) for t in (1, 2)]
+    cons += [dict(cell_type=t, other_cell_type=0, adhesion=16) for t in (1, 2)]
+    cons += [dict(cell_type=1, other_cell_type=1, adhesion=14),
+             dict(cell_type=2, other_cell_type=2, adhesion=2),
+             dict(cell_type=1, other_cell_type=2, adhesion=int(round(J_AB)))]
+    return Workload("sorting_small_cells", L, 2, n, int(temperature), lat, cons)
 
 
 def batched_sorting(nrep=64, seed=4, sizes=(128, 256, 512, 1024)):

@@ -264,9 +264,14 @@ int cpm_batch_destroy(cpm_batch_handle b);
  * k = 0: count, 1: sum A, 2: sum A^2, 3: sum P, 4: sum P^2.  H and dH/dtheta
  * for every theta follow in closed form (the backward pass of the energy). */
 int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments);
-/* Extension: cpm_energy_stats of n engines at once (each on its own stream,
- * one wait at the end), into pairs[n * 400] and moments[n * 100]. */
+/* Extension: cpm_energy_stats of n engines at once (one statistics launch
+ * for all of them, after each engine's queued work), into pairs[n * 400]
+ * and moments[n * 100]. */
 int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments);
+/* Extension: kernel time (ms, HIP events on the statistics stream) of the
+ * last cpm_energy_stats / cpm_energy_stats_many call: one launch per
+ * dimensionality covers every engine of the call. */
+int cpm_energy_stats_timing(double* kernel_ms);
 /* Extension (no counterpart in the reference): the backward pass in one call,
  * H(sigma; theta) and dH/dtheta on the engine's current lattice for theta
  * laid out, for ntypes types, as: J upper triangle (a <= b, row-major) |
