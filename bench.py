@@ -1,16 +1,22 @@
 """Benchmark: lattice site-updates/s of the 2D cell-sorting sweep on one GPU.
 
-Workload: 1024^2 lattice, 63x63 = 3969 cells of two types seeded as 10x10
-squares tiling the centre; area target 100 (lambda 25), perimeter target 40
-(lambda 2), J_mA = J_mB = 16, J_AA = 2, J_BB = 14, J_AB = 11, T = 10.
-A step is one ``run()`` of ``--mcs-per-step`` Monte Carlo steps (MCS); one
-MCS is L^2 copy attempts (site-updates).  The lattice and cell tables are
-resident in HBM before timing starts.
+Workload (the headline record): 1024^2 lattice, 63x63 = 3969 cells of two
+types seeded as 10x10 squares tiling the centre; area target 100 (lambda
+25), perimeter target 40 (lambda 2), J_mA = J_mB = 16, J_AA = 2, J_BB = 14,
+J_AB = 11, T = 10.  A step is one ``run()`` of ``--mcs-per-step`` Monte
+Carlo steps (MCS); one MCS is L^2 copy attempts (site-updates).  The
+lattice and cell tables are resident in HBM before timing starts.
 
 Prints one JSON line with the throughput, the HBM roofline of the sweep
 kernel (8 algorithmic bytes per site-update: one int32 cell-id read and one
 write) from HIP events on the engine's stream, and the CPU oracle (same
-sweep, same schedule, OpenMP over the host cores) timed on a bounded sample.
+sweep, same schedule, OpenMP over the host cores) timed on a bounded
+sample.  With the default workload the line also carries ``configs``, one
+record per other configuration, each with its own roofline and baseline:
+3D 256^3 (20,000 cells), 64 batched replicas over 2,000 MCS (against the same
+replicas run one after another and run concurrently on their own streams),
+and the dH/dtheta backward of 32 x 512^2 replicas after 2,000 MCS (with the
+gradient error against FP64 central differences and a contrastive fit).
 """
 from __future__ import annotations
 
@@ -43,7 +49,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="let the engine time its kernel forms during warmup and keep the fastest")
-    ap.add_argument("--no-grad", action="store_true", help="skip the dH/dtheta check")
+    ap.add_argument("--no-grad", action="store_true", help="skip the contrastive fit in the gradient record")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="sorting2d: only the headline record (no C3 / C4 / C5 sub-records)")
     ap.add_argument("--pmc-summary", nargs=6, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
                                                        "COPY_WRITE_DIR", "KERNEL", "OUT_JSON"),
                     help="summarize rocprofv3 FETCH_SIZE / WRITE_SIZE runs of KERNEL, scaled by the "
@@ -94,24 +102,87 @@ def cpu_baseline(wl, c, u, seconds):
                             "sample": f"{n1} MCS (MCS {m + 1}..{m + n1}), 1 thread, {dt1:.1f} s"}}
 
 
-def bench_batched(a, log):
-    """Batched replicas: 64 sorting lattices of mixed sizes and parameters in
-    one batch (one launch per colour phase for all of them) vs the same
-    replicas run one after another; both must end bit-identical."""
-    import hashlib
+def storage_of(kernel):
+    """Site storage of the timed kernel: the sweep runs on a lossless 16-bit
+    re-encoding of the int32 lattice when every site fits."""
+    if "unsigned short" in kernel:
+        return "uint16 sites (cell | type << b, b = 16 - type bits): lossless re-encoding of the int32 lattice"
+    return "uint32 sites (cell | type << 24)"
 
-    import numpy as np
+
+def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_cpu, autotune, log):
+    """Whole-run throughput of one engine (inputs resident before timing),
+    the sweep kernel's roofline from per-launch HIP events, and the CPU
+    oracle on a bounded sample of the same workload."""
+    sim = wl.make_cpm()
+    sim.push_to_gpu()
+    sim.set_seed(20240601)
+    sim.set_autotune(autotune)
+    # shared = 1: the reference's shared-memory schedule (K = 2 colours)
+    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
+              positions_per_thread=2 * c, positions_per_checkerboard=c,
+              updates_per_checkerboard_switch=u, updates_per_barrier=1,
+              iterations=mcs_per_step, inner_iterations=mcs_per_step, shared=1)
+    log(f"{name} L={wl.L} d={wl.d} cells={wl.ncells} c={c} u={u}: warmup {warmup} x {mcs_per_step} MCS")
+    for _ in range(warmup):
+        sim.run(**kw)
+    sim.synchronize()
+    log(f"{name} timed: {steps} x {mcs_per_step} MCS")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.run(**kw)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"{name} timed region {wall:.3f} s; kernel timing pass")
+    # Kernel durations of the same number of steps, bracketed per launch.
+    sim.timing_begin()
+    for _ in range(steps):
+        sim.run(**kw)
+    kernel_ms, launches, updates = sim.timing_end()
+    launch = sim.last_launch()
+    sites = wl.L ** wl.d
+    mcs = steps * mcs_per_step
+    avg_launch_s = kernel_ms / 1e3 / launches
+    bytes_per_launch = updates / launches * BYTES_PER_UPDATE
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel_token(launch['kernel'])}_{name}.json")
+    if os.path.exists(tfile) and c == 1 and u == 1:
+        traffic = json.load(open(tfile))["bytes_per_dispatch"]
+    out = {
+        "value": mcs * sites / wall,
+        "unit": "site-updates/s",
+        "ms_per_step": wall / steps * 1e3,
+        "mcs_per_s": mcs / wall,
+        "storage": storage_of(launch["kernel"]),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": launch["kernel"], "avg_launch_us": avg_launch_s * 1e6,
+                     "bytes_per_launch": bytes_per_launch, "launches": launches,
+                     "grid_blocks": launch["blocks"], "lds_tile": launch["tile"]},
+        "cpu_baseline": None,
+    }
+    if not no_cpu:
+        log(f"{name} CPU oracle sample (~{cpu_seconds:.0f} s)")
+        out["cpu_baseline"] = cpu_baseline(wl, c, u, cpu_seconds)
+    return out
+
+
+def measure_batched(nrep, mcs, log, autotune=False):
+    """Batched replicas (64 sorting lattices of mixed sizes and parameters)
+    in one batch, one launch per colour phase for all of them, against the
+    same replicas run one after another (each run waited for) and run
+    concurrently on their own engines' streams (one wait at the end); all
+    three must end bit-identical."""
+    import hashlib
 
     import gpucpm
     from gpucpm import workloads
 
-    wls = workloads.batched_sorting(nrep=a.replicas, seed=4)
-    c, u = a.checkerboard, a.updates
-    mcs = a.steps * a.mcs_per_step
-    # shared = 1: the reference's shared-memory schedule (K = 2 colours)
+    wls = workloads.batched_sorting(nrep=nrep, seed=4)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
-              positions_per_thread=2 * c, positions_per_checkerboard=c,
-              updates_per_checkerboard_switch=u, updates_per_barrier=1, shared=1)
+              positions_per_thread=2, positions_per_checkerboard=1,
+              updates_per_checkerboard_switch=1, updates_per_barrier=1, shared=1)
     sites = sum(w.L ** 2 for w in wls)
 
     def fresh():
@@ -130,63 +201,83 @@ def bench_batched(a, log):
             h.update(s.get_state().tobytes())
         return h.hexdigest()
 
-    log(f"batched: {len(wls)} replicas, {sites / 1e6:.1f} M sites, {mcs} MCS; batch")
-    sims = fresh()
-    batch = gpucpm.Batch(sims)
-    batch.set_autotune(a.autotune)  # (its choices carry over to the timed batch)
-    batch.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)  # warmup
+    log(f"batched: {len(wls)} replicas, {sites / 1e6:.1f} M sites, {mcs} MCS; warmup")
+    batch = gpucpm.Batch(fresh())
+    batch.set_autotune(autotune)  # (its choices carry over to the timed batch)
+    batch.run(**kw, iterations=100, inner_iterations=100)
     batch.synchronize()
     sims = fresh()
     batch = gpucpm.Batch(sims)
-    batch.set_autotune(a.autotune)
+    batch.set_autotune(autotune)
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        batch.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+    batch.run(**kw, iterations=mcs, inner_iterations=mcs)
     batch.synchronize()
     t_batch = time.perf_counter() - t0
     d_batch = digest(sims)
-    log(f"batch {t_batch:.3f} s; sequential")
+    log(f"batch {t_batch:.3f} s; kernel timing pass")
+    tsims = fresh()
+    tb = gpucpm.Batch(tsims)
+    tb.timing_begin()
+    tb.run(**kw, iterations=200, inner_iterations=200)
+    kernel_ms, launches, updates = tb.timing_end()
+    kernel = tb.last_kernel
+    log("sequential runs")
     sims = fresh()
     t0 = time.perf_counter()
     for s in sims:
-        for _ in range(a.steps):
-            s.run(**kw, iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step)
+        s.run(**kw, iterations=mcs, inner_iterations=mcs)
         s.synchronize()
     t_seq = time.perf_counter() - t0
     d_seq = digest(sims)
-    log(f"sequential {t_seq:.3f} s")
-    out = {
-        "metric": "lattice site-updates/sec (batched replicas)",
-        "value": mcs * sites / t_batch,
-        "unit": "site-updates/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": 1,
-        "ms_per_step": t_batch / a.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"{len(wls)} sorting replicas, L in {{128,256,512,1024}}, "
-                               "J_AB in [6,14], T in [5,20], L^2/250 cells",
-                   "replicas": len(wls), "sites": sites, "mcs": mcs,
-                   "positions_per_checkerboard": c, "updates_per_checkerboard_switch": u,
-                   "autotune": bool(a.autotune)},
-        "sequential_value": mcs * sites / t_seq,
+    log("concurrent runs on the engines' streams")
+    sims = fresh()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.run(**kw, iterations=mcs, inner_iterations=mcs)
+    for s in sims:
+        s.synchronize()
+    t_conc = time.perf_counter() - t0
+    d_conc = digest(sims)
+    avg_launch_s = kernel_ms / 1e3 / launches
+    bytes_per_launch = updates / launches * BYTES_PER_UPDATE
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    return {
+        "workload": f"{len(wls)} sorting replicas, L in {{128,256,512,1024}}, J_AB in [6,14], "
+                    "T in [5,20], L^2/250 cells, 2000-MCS config",
+        "value": mcs * sites / t_batch, "unit": "site-updates/s", "replicas": len(wls),
+        "sites": sites, "mcs": mcs, "ms_total": t_batch * 1e3,
+        "storage": storage_of(kernel),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
+                     "launches": launches},
+        "sequential": {"value": mcs * sites / t_seq, "ms_total": t_seq * 1e3,
+                       "note": "each replica's run() waited for before the next"},
+        "concurrent_streams": {"value": mcs * sites / t_conc, "ms_total": t_conc * 1e3,
+                               "note": "every replica's run() queued on its own stream, one wait"},
         "speedup_vs_sequential": t_seq / t_batch,
-        "bit_identical_to_sequential": d_batch == d_seq,
+        "speedup_vs_concurrent_streams": t_conc / t_batch,
+        "bit_identical": d_batch == d_seq == d_conc,
+        "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
 
 
 def _oracle_energy_fn(lats, wls, layout):
-    """FP64 H(theta) averaged over lattices, from the oracle's own reduction."""
+    """FP64 H(theta) averaged over lattices, from the oracle's own reduction;
+    also the seconds the oracle took to reduce the lattices."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
     import oracle as O
     stats = []
+    t0 = time.perf_counter()
     for lat, wl in zip(lats, wls):
         st = O.energy_stats(lat, 2, wl.ncells, O.Params.make(1.0))  # targets 0: raw moments
+        stats.append(st)
+    t_oracle = time.perf_counter() - t0
+    for st, lat in zip(stats, lats):
         spin, typ = lat & 0xFFFFFF, lat >> 24
-        ncell = [len(np.unique(spin[(typ == t) & (spin > 0)])) for t in range(layout.nt)]
-        stats.append((st, ncell))
+        st["ncell"] = [len(np.unique(spin[(typ == t) & (spin > 0)])) for t in range(layout.nt)]
 
     def H(th):
         nt, nj = layout.nt, layout.nj
@@ -194,83 +285,117 @@ def _oracle_energy_fn(lats, wls, layout):
         for k, (a, b) in enumerate(layout.pairs):
             J[a, b] = th[k]
         tot = 0.0
-        for st, ncell in stats:
+        for st in stats:
             tot += float(np.sum(J * st["pairs"]))
             for t in range(nt):
                 la, a0 = th[nj + t], th[nj + nt + t]
                 lp, p0 = th[nj + 2 * nt + t], th[nj + 3 * nt + t]
-                tot += la * (st["sa2"][t] - 2 * a0 * st["sa1"][t] + ncell[t] * a0 * a0)
-                tot += lp * (st["sp2"][t] - 2 * p0 * st["sp1"][t] + ncell[t] * p0 * p0)
+                n = st["ncell"][t]
+                tot += la * (st["sa2"][t] - 2 * a0 * st["sa1"][t] + n * a0 * a0)
+                tot += lp * (st["sp2"][t] - 2 * p0 * st["sp1"][t] + n * p0 * p0)
         return tot / len(stats)
-    return H
+    return H, t_oracle
 
 
-def gradient_check(nrep, L, mcs, log, J_AB=11, seed0=0, contrast=False):
+def measure_gradient(nrep, L, mcs, log, fit=True):
     """Replicas of the sorting lattice (C2 parameters) run forward mcs MCS as
-    one batch; then the backward statistics kernel gives dH/dtheta (FP32)
-    for J, lambda_A, A0, lambda_P, P0, checked against central differences
-    (step 1e-3) of the oracle's FP64 Hamiltonian on the same lattices."""
+    one batch; the batched statistics kernel then gives dH/dtheta (FP32) for
+    J, lambda_A, A0, lambda_P, P0, checked against central differences (step
+    1e-3) of the oracle's FP64 Hamiltonian on the same lattices.  The fit:
+    J_12 from 11 toward data sampled at J_12 = 8 (gpucpm.fit), on the
+    reference sorting example's Hamiltonian (with the C2 parameters the
+    interfaces do not move and J_12 leaves no trace in the samples)."""
     import numpy as np
     import torch
 
     import gpucpm
     from gpucpm import workloads
-    from gpucpm.autograd import ThetaLayout, contrastive_gradient, energy
+    from gpucpm.autograd import ThetaLayout, energy
 
     grid = (L - 112) // 10
     layout = ThetaLayout(3)
-
-    def make(J, seed):
-        wls = [workloads.sorting2d(L=L, grid=grid, edge=10, seed=seed + i, J_AB=J) for i in range(nrep)]
-        sims = []
-        for i, wl in enumerate(wls):
-            s = wl.make_cpm()
-            s.push_to_gpu()
-            s.set_seed(7000 + seed + i)
-            sims.append(s)
-        return wls, sims
-
+    wls = [workloads.sorting2d(L=L, grid=grid, edge=10, seed=i) for i in range(nrep)]
+    sims = []
+    for i, wl in enumerate(wls):
+        s = wl.make_cpm()
+        s.push_to_gpu()
+        s.set_seed(7000 + i)
+        sims.append(s)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4, positions_per_thread=2,
               positions_per_checkerboard=1, updates_per_checkerboard_switch=1, updates_per_barrier=1,
-              iterations=mcs, inner_iterations=mcs, shared=1)
-    wls, sims = make(J_AB, seed0)
+              shared=1)
     log(f"gradient: {nrep} x {L}^2 forward {mcs} MCS")
     t0 = time.perf_counter()
-    gpucpm.Batch(sims).run(**kw)
+    gpucpm.Batch(sims).run(**kw, iterations=mcs, inner_iterations=mcs)
     for s in sims:
         s.synchronize()
     t_fwd = time.perf_counter() - t0
     p = wls[0].params()
     theta = layout.pack(p["J"], p["lam_area"], p["target_area"], p["lam_perim"], p["target_perim"],
                         dtype=torch.float32).requires_grad_(True)
+    gpucpm.energy_stats_many(sims)  # warm
     t0 = time.perf_counter()
     H = energy(theta, sims, layout)
     H.backward()
     t_bwd = time.perf_counter() - t0
+    # kernel time: the mean of five more launches over the same lattices
+    kms = []
+    for _ in range(5):
+        gpucpm.energy_stats_many(sims)
+        kms.append(gpucpm.energy_stats_kernel_ms())
+    kernel_ms = sum(kms) / len(kms)
     g = theta.grad.double().numpy()
     lats = []
     for s in sims:
         s.pull_from_gpu()
         lats.append(s.get_state().view(np.uint32).copy())
-    H64 = _oracle_energy_fn(lats, wls, layout)
+    H64, t_oracle = _oracle_energy_fn(lats, wls, layout)
     th = theta.detach().double().numpy()
     h = 1e-3
     fd = np.array([(H64(th + h * e) - H64(th - h * e)) / (2 * h) for e in np.eye(th.size)])
     rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1.0)
+    nsites = nrep * L * L
+    achieved = nsites * 4 / (kernel_ms / 1e3) / 1e9
     out = {"gradient_rel_error": float(rel.max()),
            "energy_rel_error": abs(float(H.detach()) - H64(th)) / abs(H64(th)),
            "replicas": nrep, "L": L, "mcs": mcs, "forward_s": t_fwd,
-           "backward_s": t_bwd, "backward_sites_per_s": nrep * L * L / t_bwd}
-    if contrast:
-        log(f"contrastive: data sampled at J_AB = 8")
-        _, data = make(8, seed0 + 1000)
-        gpucpm.Batch(data).run(**kw)
-        for s in data:
-            s.synchronize()
-        gc = contrastive_gradient(theta, data, sims, layout).numpy()
-        k = layout.pairs.index((1, 2))
-        out["contrastive_grad_J_AB"] = float(gc[k])
-        out["contrastive_step_moves_J_AB_toward_8"] = bool(gc[k] > 0)  # theta -= lr * grad lowers J_AB
+           "forward_site_updates_per_s": nsites * mcs / t_fwd,
+           "backward_s": t_bwd, "backward_kernel_ms": kernel_ms,
+           "backward_kernel_ms_runs": kms,
+           "backward_sites_per_s": nsites / (kernel_ms / 1e3),
+           "roofline": {"bound": "hbm", "kernel": "k_stats_batch<2>", "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": None, "bytes_per_launch": nsites * 4,
+                        "avg_launch_us": kernel_ms * 1e3,
+                        "note": "4 algorithmic bytes per site (one int32 read); one launch covers "
+                                "all replicas' pair counts and cell moments"},
+           "cpu_baseline": {"value": nsites / t_oracle, "unit": "sites/s", "cores": 1, "kind": "port",
+                            "sample": f"the oracle's statistics of the same {nrep} lattices, one thread"}}
+    if fit:
+        from gpucpm.fit import fit_adhesion
+        log("contrastive fit of J_12: 16 x 128^2, 500 MCS per evaluation, data at J_12 = 8")
+        make = lambda seed: workloads.sorting_small_cells(L=128, seed=seed)
+
+        def replicas(J, lat0, rng0):
+            out_ = []
+            for r in range(16):
+                s = make(lat0 + r).make_cpm()
+                s.set_adhesion_real(1, 2, J)
+                s.push_to_gpu()
+                s.set_seed(rng0 + r)
+                out_.append(s)
+            return out_
+
+        data = replicas(8.0, 1000, 2000)
+        gpucpm.Batch(data).run(**kw, iterations=500, inner_iterations=500)
+        t0 = time.perf_counter()
+        res = fit_adhesion(data, lambda J: replicas(J, 3000, 4000), (1, 2), 11.0, 500, kw, steps=6)
+        out["fit"] = {"parameter": "J_12", "start": 11.0, "data_at": 8.0, "fitted": res.J,
+                      "tolerance": 0.5, "within_tolerance": abs(res.J - 8.0) <= 0.5,
+                      "evaluations": len(res.history), "seconds": time.perf_counter() - t0,
+                      "history": [[float(J), float(gr)] for J, gr in res.history],
+                      "workload": "16 replicas of the reference sorting example's Hamiltonian "
+                                  "(3x3 cells, area term), 128^2, 500 MCS per evaluation"}
     return out
 
 
@@ -328,27 +453,6 @@ def bench_interactive(a, log):
         "roofline": None,
         "cpu_baseline": None,
     }
-
-
-def bench_gradient(a, log):
-    out = gradient_check(32, 512, a.steps * a.mcs_per_step, log, contrast=True)
-    line = {
-        "metric": "gradient rel. error (dH/dtheta FP32 vs FP64 central differences)",
-        "value": out["gradient_rel_error"], "unit": "relative", "n_gpus": 1,
-        "steps": a.steps, "warmup": 0, "ms_per_step": out["forward_s"] / a.steps * 1e3,
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32 sweep, int64 statistics, fp32 gradient", "data": "synthetic",
-        "config": {"workload": "32 replicas of 512^2 two-type sorting (C2 parameters), forward then "
-                               "dH/dtheta for J (3x3), lambda_A, A0, lambda_P, P0; FD step 1e-3; "
-                               "contrastive step toward J_AB = 8"},
-        "roofline": {"bound": "hbm", "kernel": "k_stats_pairs + k_stats_cells",
-                     "achieved": out["backward_sites_per_s"] * 4 / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": out["backward_sites_per_s"] * 4 / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "note": "backward time includes the host side: one wait for all replicas' statistics and the per-replica energy and gradient from them"},
-        "cpu_baseline": None,
-    }
-    line.update({k: v for k, v in out.items() if k != "gradient_rel_error"})
-    print(json.dumps(line), flush=True)
 
 
 def kernel_token(name):
@@ -418,97 +522,80 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     log = lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True)
-    if a.workload == "batched":
-        return bench_batched(a, log)
     if a.workload == "interactive":
-        out = bench_interactive(a, log)
-        print(json.dumps(out), flush=True)
+        print(json.dumps(bench_interactive(a, log)), flush=True)
+        return 0
+    from gpucpm import workloads
+
+    if a.workload == "batched":
+        r = measure_batched(a.replicas, a.steps * a.mcs_per_step, log, a.autotune)
+        line = {"metric": "lattice site-updates/sec (batched replicas)", "n_gpus": 1, "steps": 1,
+                "warmup": 1, "ms_per_step": r["ms_total"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+                "config": {"workload": r["workload"]}}
+        line.update(r)
+        print(json.dumps(line), flush=True)
         return 0
     if a.workload == "gradient":
-        return bench_gradient(a, log)
-    import gpucpm
-    from gpucpm import workloads
+        r = measure_gradient(32, 512, a.steps * a.mcs_per_step, log, fit=not a.no_grad)
+        line = {"metric": "gradient rel. error (dH/dtheta FP32 vs FP64 central differences)",
+                "value": r["gradient_rel_error"], "unit": "relative", "n_gpus": 1, "steps": 1,
+                "warmup": 0, "ms_per_step": r["forward_s"] * 1e3, "higher_is_better": False,
+                "scaling": "weak", "vs_baseline": None,
+                "dtype": "int32 sweep, int64 statistics, fp32 gradient", "data": "synthetic",
+                "config": {"workload": "32 replicas of 512^2 two-type sorting (C2 parameters), "
+                                       "forward then dH/dtheta"}}
+        line.update(r)
+        print(json.dumps(line), flush=True)
+        return 0
 
     wl = {"sorting2d": workloads.sorting2d, "plain3d": workloads.plain3d,
           "plain2d": workloads.plain2d}[a.workload]()
     c, u = a.checkerboard, a.updates
-    sim = wl.make_cpm()
-    sim.push_to_gpu()
-    sim.set_seed(20240601)
-    sim.set_autotune(a.autotune)
-    # shared = 1: the reference's shared-memory schedule (K = 2 colours)
-    kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
-              positions_per_thread=2 * c, positions_per_checkerboard=c,
-              updates_per_checkerboard_switch=u, updates_per_barrier=1,
-              iterations=a.mcs_per_step, inner_iterations=a.mcs_per_step, shared=1)
-    log(f"{a.workload} L={wl.L} d={wl.d} cells={wl.ncells} c={c} u={u} "
-        f"mode={os.environ.get('GPUCPM_MODE', 'default')}: warmup {a.warmup} x {a.mcs_per_step} MCS")
-    for _ in range(a.warmup):
-        sim.run(**kw)
-    sim.synchronize()
-    log(f"timed: {a.steps} x {a.mcs_per_step} MCS")
-
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        sim.run(**kw)
-    sim.synchronize()
-    wall = time.perf_counter() - t0
-
-    log(f"timed region {wall:.3f} s; kernel timing pass")
-    # Kernel durations of the same number of steps, bracketed per launch.
-    sim.timing_begin()
-    for _ in range(a.steps):
-        sim.run(**kw)
-    kernel_ms, launches, updates = sim.timing_end()
-    launch = sim.last_launch()
-
-    sites = wl.L ** wl.d
-    mcs = a.steps * a.mcs_per_step
-    value = mcs * sites / wall
-    avg_launch_s = kernel_ms / 1e3 / launches
-    bytes_per_launch = updates / launches * BYTES_PER_UPDATE
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel_token(launch['kernel'])}_{a.workload}.json")
-    if os.path.exists(tfile) and c == 1 and u == 1:
-        traffic = json.load(open(tfile))["bytes_per_dispatch"]
+    r = measure_single(wl, a.workload, c, u, a.steps, a.warmup, a.mcs_per_step, a.cpu_seconds,
+                       a.no_cpu, a.autotune, log)
     workload_desc = {
         "sorting2d": "2D cell sorting 1024^2, 3969 cells (10x10 squares), two types, 10000-MCS config",
-        "plain3d": "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood",
+        "plain3d": "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood, 1000-MCS config",
         "plain2d": "plain 2D CPM 256^2, 100 cells",
     }[a.workload]
     out = {
         "metric": "lattice site-updates/sec",
-        "value": value,
+        "value": r["value"],
         "unit": "site-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
+        "storage": r["storage"],
         "data": "synthetic",
         "config": {"workload": workload_desc, "L": wl.L, "d": wl.d, "cells": wl.ncells,
                    "mcs_per_step": a.mcs_per_step, "positions_per_checkerboard": c,
                    "updates_per_checkerboard_switch": u, "seed": 20240601,
                    "autotune": bool(a.autotune)},
-        "mcs_per_s": mcs / wall,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": launch["kernel"], "avg_launch_us": avg_launch_s * 1e6,
-                     "bytes_per_launch": bytes_per_launch, "launches": launches,
-                     "grid_blocks": launch["blocks"], "lds_tile": launch["tile"]},
-        "cpu_baseline": None,
+        "mcs_per_s": r["mcs_per_s"],
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
     }
-    if not a.no_cpu:
-        log(f"CPU oracle sample (~{a.cpu_seconds:.0f} s)")
-        out["cpu_baseline"] = cpu_baseline(wl, c, u, a.cpu_seconds)
-    if a.workload == "sorting2d" and not a.no_grad:
-        g = gradient_check(4, 512, 200, log)
-        out["gradient_rel_error"] = g["gradient_rel_error"]
-        out["gradient_check"] = {k: g[k] for k in ("replicas", "L", "mcs", "energy_rel_error")}
+    if a.workload == "sorting2d" and not a.no_configs:
+        # The other configurations, each with its own roofline and baseline.
+        configs = {}
+        w3 = workloads.plain3d()
+        r3 = measure_single(w3, "plain3d", 1, 1, 5, 1, 20, 6.0, a.no_cpu, a.autotune, log)
+        r3["workload"] = "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood (1000-MCS config; 100 MCS timed)"
+        configs["C3_plain3d"] = r3
+        configs["C4_batched"] = measure_batched(64, 2000, log, a.autotune)
+        r5 = measure_gradient(32, 512, 2000, log, fit=not a.no_grad)
+        r5["workload"] = ("32 replicas of 512^2 two-type sorting (C2 parameters): forward 2000 MCS "
+                          "as one batch, then dH/dtheta for J (3x3), lambda_A, A0, lambda_P, P0 "
+                          "against FP64 central differences (step 1e-3)")
+        configs["C5_gradient"] = r5
+        out["configs"] = configs
+        out["gradient_rel_error"] = r5["gradient_rel_error"]
     print(json.dumps(out), flush=True)
 
 

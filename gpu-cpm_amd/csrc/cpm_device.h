@@ -208,7 +208,11 @@ hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, s
                           hipStream_t s);
 hipError_t batch_max_blocks(int d, int* max_blocks);
 hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s);
-hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s);
+// e0/e1 (or null) are recorded at the dispatch's start and end; name (or
+// null) receives the kernel's name as profilers show it.
+hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s,
+                              hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, char* name = nullptr,
+                              size_t name_len = 0);
 hipError_t launch_tally(int d, const uint32_t* lat, int lgL, int ncells, int64_t* tab,
                         uint32_t* first, int32_t* anchor, hipStream_t s);
 // Settle after G phases of a run (only the parity of G matters).
@@ -231,7 +235,7 @@ hipError_t launch_csum(int d, const uint32_t* lat, int lgL, const int32_t* ancho
 // [cell0, cell0 + stats_cell_blocks) its cell moments.
 struct StatsDesc {
     const void* lat;
-    int narrow;             // lat holds 16-bit sites (cell | type << idbits)
+    int narrow;             // lat holds 16-bit sites (cell | type << idbits); equal over a launch
     int lgL;
     uint32_t idbits;
     int nt;                 // types present (at most 4: packed counters)
@@ -243,7 +247,8 @@ struct StatsDesc {
 };
 int stats_pair_blocks(int d, int lgL);
 int stats_cell_blocks(int ncells);
-hipError_t launch_stats_batch(int d, const StatsDesc* descs, const int* eng_of, int blocks,
+// every engine of one launch has the same dimension and site width
+hipError_t launch_stats_batch(int d, bool narrow, const StatsDesc* descs, const int* eng_of, int blocks,
                               hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 
 constexpr int kBlock = 256;  // threads per workgroup of the sweep kernels

@@ -677,6 +677,24 @@ struct cpm_batch {
     bool autotune = false;  // cpm_batch_set_autotune
     FormTuner tuner;
     KernelOpts opts;        // cpm_batch_set_kernel_options (small, compact, att)
+    char last_kernel[96] = "";  // per-phase batch kernel of the last run
+    // Kernel timing (cpm_batch_timing_begin/end): one event pair per launch.
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    int64_t t_launches = 0, t_updates = 0;
+
+    int next_events(hipEvent_t* e0, hipEvent_t* e1) {
+        while (ev.size() < ev_used + 2) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        *e0 = ev[ev_used];
+        *e1 = ev[ev_used + 1];
+        ev_used += 2;
+        return CPM_OK;
+    }
 
     ~cpm_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -684,6 +702,7 @@ struct cpm_batch {
         (void)hipFree(d_reps);
         (void)hipFree(d_rep_of);
         for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1740,6 +1759,8 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     const int rounds = k0.rounds;
     pa.lstride = k0.lstride;
     const int u = a->updates_per_checkerboard_switch;
+    int64_t attempts_per_launch = 0;  // sub-blocks of one colour phase over all replicas
+    for (int i = 0; i < n; i++) attempts_per_launch += (int64_t)1 << (reps[i].lh * b->d);
     int g = 0;
     for (long m = 0; m < total; m++) {
         pa.m = (uint32_t)m;
@@ -1775,11 +1796,14 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                         pa.form = tune->forms[ti];
                         set_layout(layout_of(pa.form));
                         HIP_TRY(b->tuner.events(&e0, &e1));
-                        HIP_TRY(hipEventRecord(e0, b->stream));
+                    } else if (b->timing) {
+                        if (int rc = b->next_events(&e0, &e1)) return rc;
+                        b->t_launches++;
+                        b->t_updates += attempts_per_launch * pa.natt;
                     }
-                    HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream));
+                    HIP_TRY(gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream, e0, e1, b->last_kernel,
+                                                     sizeof(b->last_kernel)));
                     if (ti >= 0) {
-                        HIP_TRY(hipEventRecord(e1, b->stream));
                         b->tuner.pending.push_back({tune, ti, e0, e1});
                         pa.form = form;
                         set_layout(lay0);
@@ -2032,6 +2056,34 @@ int cpm_batch_sync(cpm_batch_handle b) {
     return CPM_OK;
 }
 
+int cpm_batch_timing_begin(cpm_batch_handle b) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    b->timing = true;
+    b->ev_used = 0;
+    b->t_launches = 0;
+    b->t_updates = 0;
+    return CPM_OK;
+}
+
+int cpm_batch_timing_end(cpm_batch_handle b, double* kernel_ms, int64_t* launches, int64_t* site_updates) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    double total = 0.0;
+    for (size_t i = 0; i + 1 < b->ev_used; i += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+        total += ms;
+    }
+    if (kernel_ms) *kernel_ms = total;
+    if (launches) *launches = b->t_launches;
+    if (site_updates) *site_updates = b->t_updates;
+    b->timing = false;
+    b->ev_used = 0;
+    return CPM_OK;
+}
+
+const char* cpm_batch_last_kernel(cpm_batch_handle b) { return b ? b->last_kernel : ""; }
+
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks) {
     if (!b) return fail(CPM_EINVAL, "null batch");
     if (replicas) *replicas = (int)b->sims.size();
@@ -2091,12 +2143,14 @@ int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t
     std::lock_guard<std::mutex> guard(S.mu);
     if (int rc = S.ensure(n)) return rc;
     double ms_total = 0.0;
-    for (int d = 2; d <= 3; d++) {
-        // one launch per dimensionality over its engines, each read from
-        // whichever lattice copy is current (no conversion)
+    for (int grp = 0; grp < 4; grp++) {
+        // one launch per dimensionality and site width over its engines, each
+        // read from whichever lattice copy is current (no conversion)
+        const int d = 2 + (grp >> 1);
+        const bool narrow_grp = grp & 1;
         std::vector<int> idx;
         for (int i = 0; i < n; i++)
-            if (sims[i]->d == d) idx.push_back(i);
+            if (sims[i]->d == d && !sims[i]->wide_valid == narrow_grp) idx.push_back(i);
         if (idx.empty()) continue;
         std::vector<gcpm::StatsDesc> desc(idx.size());
         std::vector<int> eng_of;
@@ -2127,7 +2181,8 @@ int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t
                                hipMemcpyHostToDevice, S.stream));
         HIP_TRY(hipMemcpyAsync(S.d_map, eng_of.data(), sizeof(int) * eng_of.size(), hipMemcpyHostToDevice,
                                S.stream));
-        HIP_TRY(gcpm::launch_stats_batch(d, S.d_desc, S.d_map, (int)eng_of.size(), S.stream, S.e0, S.e1));
+        HIP_TRY(gcpm::launch_stats_batch(d, narrow_grp, S.d_desc, S.d_map, (int)eng_of.size(), S.stream, S.e0,
+                                          S.e1));
         HIP_TRY(hipMemcpyAsync(S.h_out, S.d_out, sizeof(unsigned long long) * nw * (size_t)n,
                                hipMemcpyDeviceToHost, S.stream));
         HIP_TRY(hipStreamSynchronize(S.stream));  // (desc and eng_of are host temporaries)

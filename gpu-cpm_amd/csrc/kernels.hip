@@ -1595,21 +1595,28 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
 // the other is lexicographically positive ((dz, dy, dx) > 0: 4 offsets in
 // 2D, 13 in 3D).
 //
-// Pairs, at most four types present: every lane keeps the counts of the ten
-// type pairs (a <= b < 4) as 16-bit fields of three 64-bit words (at most
-// kStatsSites * 13 * 64 per wave, below 2^16), the words are summed across
-// the wave, and lane 0 adds the ten counts to the block's LDS counters;
+// Pairs, at most four types present: for each site every lane counts its
+// unlike positive-half neighbours by neighbour type in a byte histogram (one
+// byte per type of one 32-bit word) and adds it to the accumulator of its
+// own site's type: acc[t] byte tn counts the (site type t, neighbour type tn)
+// unlike pairs, at most 16 x 4 or 2 x 13 per lane.  The accumulators are
+// widened to 16-bit fields and summed across the wave, and lane 0 adds
+// M[a][b] + M[b][a] (M[a][a] on the diagonal) to the block's LDS counters;
 // the block then adds them to the engine's global counters once.  More
 // types: one LDS histogram add per unlike pair.
 // Cells: a cell's type is fixed (flips copy whole values), so it is taken
 // from the table made at push (ctype); per-type moments n, sum A, sum A^2,
 // sum P, sum P^2 over live cells are summed across the wave, then added to
 // the engine's counters (four types or fewer; else per-cell LDS adds).
-constexpr int kStatsSites = 8;  // sites per thread in the pair blocks
+// sites per thread in the pair blocks (their loads are all in flight at
+// once: 16 x 5 values in 2D, 2 x 14 in 3D)
+template <int D>
+constexpr int stats_sites() { return D == 3 ? 2 : 16; }
 
-__device__ __forceinline__ int pair_slot4(uint32_t a, uint32_t b) {
-    // (a, b), a <= b < 4, in row-major upper-triangle order: 0..9
-    return (int)(a * 4u - ((a * (a - 1u)) >> 1) + (b - a));
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
@@ -1618,60 +1625,154 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     return x;
 }
 
-template <int D>
+// Fold the lanes' byte accumulators acc[t] (byte tn: (site type t,
+// neighbour type tn) unlike pairs, at most 64 per lane) into the block's LDS
+// pair counters: bytes 0, 2 and bytes 1, 3 widened to 16-bit fields, summed
+// across the wave; lane 0 adds M[a][b] + M[b][a] (M[a][a] on the diagonal).
+__device__ __forceinline__ void stats_fold_four(const uint32_t acc[4], unsigned int* hist) {
+    uint32_t w[8];
+#pragma unroll
+    for (int ty = 0; ty < 4; ty++) {
+        w[2 * ty] = wave_sum_u32(acc[ty] & 0x00FF00FFu);
+        w[2 * ty + 1] = wave_sum_u32((acc[ty] >> 8) & 0x00FF00FFu);
+    }
+    if (__lane_id() == 0) {
+        auto M = [&](uint32_t t, uint32_t tn) {
+            return (w[2 * t + (tn & 1u)] >> (16u * (tn >> 1))) & 0xFFFFu;
+        };
+#pragma unroll
+        for (uint32_t a = 0; a < 4; a++)
+#pragma unroll
+            for (uint32_t b = a; b < 4; b++) {
+                const uint32_t c = a == b ? M(a, a) : M(a, b) + M(b, a);
+                if (c) atomicAdd(&hist[a * kNTypes + b], c);
+            }
+    }
+}
+
+// The pair counts of the four-type form for 2D 16-bit lattices with L >= 8,
+// read as chunks of 8 consecutive sites of a row: one 16-byte load of the
+// row, one of the row below, and three single sites at the chunk's ends
+// give every positive-half neighbour ((+1, 0), (-1, +1), (0, +1), (+1, +1)).
+// A thread takes two chunks (the 16 sites of stats_sites<2>()).
+__device__ __forceinline__ void stats_pairs_2d16(const StatsDesc& e, int lb, unsigned int* hist) {
+    const int lgL = e.lgL, Lm = (1 << lgL) - 1;
+    const uint32_t nchunks = 1u << (2 * lgL - 3);
+    const uint16_t* __restrict__ lat = static_cast<const uint16_t*>(e.lat);
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};
+    uint16_t r0[2][9], r1[2][10];  // row y: x0 .. x0 + 8; row y + 1: x0 - 1 .. x0 + 8
+    bool ok[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint32_t c = ((uint32_t)lb * 2u + (uint32_t)h) * kBlock + threadIdx.x;
+        ok[h] = c < nchunks;
+        const uint32_t cc = ok[h] ? c : 0u;
+        const int x0 = (int)((cc << 3) & (uint32_t)Lm), y = (int)((cc << 3) >> lgL);
+        const int y1 = (y + 1) & Lm;
+        const uint4 a = *reinterpret_cast<const uint4*>(lat + ((uint32_t)y << lgL) + x0);
+        const uint4 b = *reinterpret_cast<const uint4*>(lat + ((uint32_t)y1 << lgL) + x0);
+        const uint32_t av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            r0[h][2 * j] = (uint16_t)av[j];
+            r0[h][2 * j + 1] = (uint16_t)(av[j] >> 16);
+            r1[h][2 * j + 1] = (uint16_t)bv[j];
+            r1[h][2 * j + 2] = (uint16_t)(bv[j] >> 16);
+        }
+        r0[h][8] = lat[((uint32_t)y << lgL) + ((x0 + 8) & Lm)];
+        r1[h][0] = lat[((uint32_t)y1 << lgL) + ((x0 - 1) & Lm)];
+        r1[h][9] = lat[((uint32_t)y1 << lgL) + ((x0 + 8) & Lm)];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t v = r0[h][j];
+            const uint32_t nb[4] = {r0[h][j + 1], r1[h][j], r1[h][j + 1], r1[h][j + 2]};
+            uint32_t hh = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) hh += nb[q] != v ? 1u << ((nb[q] >> e.idbits) << 3) : 0u;
+            hh = ok[h] ? hh : 0u;
+            const uint32_t t = v >> e.idbits;
+#pragma unroll
+            for (uint32_t ty = 0; ty < 4; ty++) acc[ty] += t == ty ? hh : 0u;
+        }
+    }
+    stats_fold_four(acc, hist);
+}
+
+template <int D, class ST>
 __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
     __shared__ unsigned int hist[kNTypes * kNTypes];
     constexpr int NB = D == 3 ? 27 : 9;
+    constexpr int NH = NB / 2;
+    constexpr int kStatsSites = stats_sites<D>();
     const int lgL = e.lgL, Lm = (1 << lgL) - 1;
     const uint32_t nsites = 1u << (D * lgL);
     const bool four = e.nt <= 4;
     for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    uint64_t w[3] = {0, 0, 0};
+    if constexpr (D == 2 && sizeof(ST) == 2) {
+        if (four && lgL >= 3) {  // (uniform per block)
+            stats_pairs_2d16(e, lb, hist);
+            __syncthreads();
+            for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x)
+                if (hist[i]) atomicAdd(&e.out[i], (unsigned long long)hist[i]);
+            return;
+        }
+    }
     const uint32_t base = (uint32_t)lb * (kBlock * kStatsSites);
-#pragma unroll 1
+    const ST* __restrict__ lat = static_cast<const ST*>(e.lat);
+    auto site = [&](uint32_t j) -> uint32_t { return (uint32_t)lat[j]; };
+    // every load of the thread's sites and their positive-half neighbours is
+    // issued before any comparison (one memory round trip, not one per site)
+    uint32_t vs[kStatsSites], vn_[kStatsSites][NH];
+#pragma unroll
     for (int k = 0; k < kStatsSites; k++) {
         const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
-        if (i >= nsites) break;
-        auto site = [&](uint32_t j) -> uint32_t {
-            return e.narrow ? (uint32_t)static_cast<const uint16_t*>(e.lat)[j]
-                            : static_cast<const uint32_t*>(e.lat)[j];
-        };
-        const uint32_t v = site(i);
-        const uint32_t t = v >> e.idbits;
-        const int x = (int)(i & Lm), y = (int)((i >> lgL) & Lm), z = D == 3 ? (int)(i >> (2 * lgL)) : 0;
+        const uint32_t ii = i < nsites ? i : 0u;
+        const int x = (int)(ii & Lm), y = (int)((ii >> lgL) & Lm), z = D == 3 ? (int)(ii >> (2 * lgL)) : 0;
+        vs[k] = site(ii);
 #pragma unroll
         for (int n = NB / 2 + 1; n < NB; n++) {  // offsets after the centre: the positive half
             int ox, oy, oz;
             block_offset<D>(n, ox, oy, oz);
-            const uint32_t vn = site(site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL));
-            if (vn != v) {
-                const uint32_t tn = vn >> e.idbits;
-                const uint32_t a = t < tn ? t : tn, b = t < tn ? tn : t;
-                if (four) {
-                    const int sl = pair_slot4(a, b);
-                    const uint64_t inc = 1ull << (16 * (sl & 3));
-                    w[0] += sl < 4 ? inc : 0ull;
-                    w[1] += (sl >> 2) == 1 ? inc : 0ull;
-                    w[2] += sl >= 8 ? inc : 0ull;
-                } else {
-                    atomicAdd(&hist[a * kNTypes + b], 1u);
-                }
-            }
+            vn_[k][n - NB / 2 - 1] = site(site_index<D>((x + ox) & Lm, (y + oy) & Lm, (z + oz) & Lm, lgL));
         }
     }
     if (four) {  // (uniform per block)
+        uint32_t acc[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-        for (int j = 0; j < 3; j++) w[j] = wave_sum_u64(w[j]);
-        if (__lane_id() == 0) {
+        for (int k = 0; k < kStatsSites; k++) {
+            const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
+            const uint32_t v = vs[k];
+            uint32_t h = 0;
 #pragma unroll
-            for (uint32_t a = 0; a < 4; a++)
+            for (int q = 0; q < NH; q++) {
+                const uint32_t vn = vn_[k][q];
+                h += vn != v ? 1u << ((vn >> e.idbits) << 3) : 0u;
+            }
+            h = i < nsites ? h : 0u;
+            const uint32_t t = v >> e.idbits;
 #pragma unroll
-                for (uint32_t b = a; b < 4; b++) {
-                    const int sl = pair_slot4(a, b);
-                    const uint32_t c = (uint32_t)(w[sl >> 2] >> (16 * (sl & 3))) & 0xFFFFu;
-                    if (c) atomicAdd(&hist[a * kNTypes + b], c);
+            for (uint32_t ty = 0; ty < 4; ty++) acc[ty] += t == ty ? h : 0u;
+        }
+        stats_fold_four(acc, hist);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kStatsSites; k++) {
+            const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
+            if (i >= nsites) continue;
+            const uint32_t v = vs[k];
+            const uint32_t t = v >> e.idbits;
+#pragma unroll
+            for (int q = 0; q < NH; q++) {
+                const uint32_t vn = vn_[k][q];
+                if (vn != v) {
+                    const uint32_t tn = vn >> e.idbits;
+                    atomicAdd(&hist[min(t, tn) * kNTypes + max(t, tn)], 1u);
                 }
+            }
         }
     }
     __syncthreads();
@@ -1718,31 +1819,37 @@ __device__ __forceinline__ void stats_cells_block(const StatsDesc& e, int lb) {
         if (acc[i]) atomicAdd(&e.out[kNTypes * kNTypes + i], acc[i]);
 }
 
-template <int D>
+// ST: the site type of every engine of the launch (16-bit or 32-bit sites)
+template <int D, class ST>
 __global__ __launch_bounds__(kBlock) void k_stats_batch(const StatsDesc* __restrict__ descs,
                                                         const int* __restrict__ eng_of) {
     const int r = eng_of[blockIdx.x];
     const StatsDesc e = descs[r];  // uniform per block: scalar loads
     const int b = (int)blockIdx.x;
     if (b < e.cell0)
-        stats_pairs_block<D>(e, b - e.pair0);
+        stats_pairs_block<D, ST>(e, b - e.pair0);
     else
         stats_cells_block(e, b - e.cell0);
 }
 
 int stats_pair_blocks(int d, int lgL) {
     const long nsites = 1L << (d * lgL);
-    return (int)((nsites + (long)kBlock * kStatsSites - 1) / ((long)kBlock * kStatsSites));
+    const long per = (long)kBlock * (d == 3 ? stats_sites<3>() : stats_sites<2>());
+    return (int)((nsites + per - 1) / per);
 }
 
 int stats_cell_blocks(int ncells) { return (ncells + kBlock - 1) / kBlock; }
 
-hipError_t launch_stats_batch(int d, const StatsDesc* descs, const int* eng_of, int blocks,
+hipError_t launch_stats_batch(int d, bool narrow, const StatsDesc* descs, const int* eng_of, int blocks,
                               hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (d == 2)
-        hipExtLaunchKernelGGL(k_stats_batch<2>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of);
-    else
-        hipExtLaunchKernelGGL(k_stats_batch<3>, dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of);
+#define CPM_STATS(DD, ST) \
+    hipExtLaunchKernelGGL((k_stats_batch<DD, ST>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of)
+    if (d == 2) {
+        if (narrow) CPM_STATS(2, uint16_t); else CPM_STATS(2, uint32_t);
+    } else {
+        if (narrow) CPM_STATS(3, uint16_t); else CPM_STATS(3, uint32_t);
+    }
+#undef CPM_STATS
     return hipGetLastError();
 }
 
@@ -1905,13 +2012,18 @@ hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s) {
 }
 
 template <class T>
-static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
+static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s,
+                                  hipEvent_t e0, hipEvent_t e1, char* name, size_t name_len) {
     const bool compact = a.form & kFormCompact;
     const bool small = a.form & kFormSmall;
     const int att = form_att(a.form) >= 8 ? 8 : form_att(a.form) >= 4 ? 4 : 1;
-#define CPM_BATCH(DD, ST, AT) \
-    hipLaunchKernelGGL((k_batch_phase_compact<DD, ST, AT, T>), dim3(blocks), dim3(kBlock), 0, s, a)
+    const char* st = sizeof(T) == 2 ? "unsigned short" : "unsigned int";
+    // hipExtLaunchKernelGGL records e0/e1 at the dispatch's start and end
+    // (null events: plain launch)
+#define CPM_BATCH(DD, SG, AT) \
+    hipExtLaunchKernelGGL((k_batch_phase_compact<DD, SG, AT, T>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a)
     if (compact) {
+        if (name) snprintf(name, name_len, "k_batch_phase_compact<%d, %s, %d, %s>", d, small ? "false" : "true", att, st);
         if (d == 2) {
             if (small) {
                 if (att == 8) CPM_BATCH(2, false, 8); else if (att == 4) CPM_BATCH(2, false, 4); else CPM_BATCH(2, false, 1);
@@ -1928,23 +2040,25 @@ static void launch_batch_phase_st(int d, const BatchPhaseArgs& a, int blocks, hi
         return;
     }
 #undef CPM_BATCH
+    if (name) snprintf(name, name_len, "k_batch_phase<%d, %s%s>", d, st, a.refstream ? ", true" : "");
     if (a.refstream) {
         if (d == 2)
-            hipLaunchKernelGGL((k_batch_phase<2, T, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+            hipExtLaunchKernelGGL((k_batch_phase<2, T, true>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
         else
-            hipLaunchKernelGGL((k_batch_phase<3, T, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+            hipExtLaunchKernelGGL((k_batch_phase<3, T, true>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     } else if (d == 2) {
-        hipLaunchKernelGGL((k_batch_phase<2, T>), dim3(blocks), dim3(kBlock), 0, s, a);
+        hipExtLaunchKernelGGL((k_batch_phase<2, T>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     } else {
-        hipLaunchKernelGGL((k_batch_phase<3, T>), dim3(blocks), dim3(kBlock), 0, s, a);
+        hipExtLaunchKernelGGL((k_batch_phase<3, T>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, a);
     }
 }
 
-hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s) {
+hipError_t launch_batch_phase(int d, const BatchPhaseArgs& a, int blocks, hipStream_t s, hipEvent_t e0,
+                              hipEvent_t e1, char* name, size_t name_len) {
     if (a.narrow)
-        launch_batch_phase_st<uint16_t>(d, a, blocks, s);
+        launch_batch_phase_st<uint16_t>(d, a, blocks, s, e0, e1, name, name_len);
     else
-        launch_batch_phase_st<uint32_t>(d, a, blocks, s);
+        launch_batch_phase_st<uint32_t>(d, a, blocks, s, e0, e1, name, name_len);
     return hipGetLastError();
 }
 

@@ -456,6 +456,21 @@ class Batch:
         """Blocks per unit of work in each replica's group (default 1.0)."""
         check(lib.cpm_batch_set_blocks(self._h, float(blocks_per_work)))
 
+    def timing_begin(self):
+        """Start bracketing every batched launch with HIP events."""
+        check(lib.cpm_batch_timing_begin(self._h))
+
+    def timing_end(self):
+        """Synchronize; return (kernel_ms, launches, site_updates) since timing_begin."""
+        ms, n, upd = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib.cpm_batch_timing_end(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(upd)))
+        return ms.value, n.value, upd.value
+
+    @property
+    def last_kernel(self):
+        """Name of the per-phase batch kernel of the last run."""
+        return lib.cpm_batch_last_kernel(self._h).decode()
+
     @property
     def blocks(self):
         n, b = ctypes.c_int(), ctypes.c_int()

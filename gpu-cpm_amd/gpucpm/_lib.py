@@ -88,6 +88,9 @@ SIGNATURES = [
     ("cpm_set_kernel_options", _I, [_H, _I, _I, _I, _I, _I, _I]),
     ("cpm_batch_set_kernel_options", _I, [_H, _I, _I, _I]),
     ("cpm_energy_stats_timing", _I, [_P(ctypes.c_double)]),
+    ("cpm_batch_timing_begin", _I, [_H]),
+    ("cpm_batch_timing_end", _I, [_H, _P(ctypes.c_double), _P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    ("cpm_batch_last_kernel", ctypes.c_char_p, [_H]),
 ]
 
 if not os.path.exists(LIBPATH):

@@ -252,6 +252,13 @@ int cpm_batch_set_form(cpm_batch_handle b, int form);
  * replica's own narrow option still applies.  Results never depend on them. */
 int cpm_batch_set_kernel_options(cpm_batch_handle b, int small, int compact, int attempts);
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks);
+/* Extension: kernel timing of a batch's per-phase launches, as
+ * cpm_timing_begin / cpm_timing_end for one engine (site_updates counts the
+ * copy attempts of every replica). */
+int cpm_batch_timing_begin(cpm_batch_handle b);
+int cpm_batch_timing_end(cpm_batch_handle b, double* kernel_ms, int64_t* launches, int64_t* site_updates);
+/* Extension: name of the per-phase batch kernel of the last batched run ("" before one). */
+const char* cpm_batch_last_kernel(cpm_batch_handle b);
 int cpm_batch_destroy(cpm_batch_handle b);
 
 /* Extension (no counterpart in the reference): sufficient statistics of
@@ -270,7 +277,8 @@ int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments);
 int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments);
 /* Extension: kernel time (ms, HIP events on the statistics stream) of the
  * last cpm_energy_stats / cpm_energy_stats_many call: one launch per
- * dimensionality covers every engine of the call. */
+ * dimensionality and site width (16- or 32-bit) covers every engine of the
+ * call. */
 int cpm_energy_stats_timing(double* kernel_ms);
 /* Extension (no counterpart in the reference): the backward pass in one call,
  * H(sigma; theta) and dH/dtheta on the engine's current lattice for theta
