@@ -245,6 +245,23 @@ struct StatsDesc {
     unsigned long long* out;
     int pair0, cell0;
 };
+constexpr int kStatsWords = 20 * 20 + 20 * 5;   // words of one engine's statistics
+constexpr int kStatsArgEngines = 32;             // engines per argument-borne launch
+constexpr int kStatsScratchEngines = 64;         // engines of the allocation-free entry point
+constexpr int kMaxDevices = 64;
+// Descriptors of up to kStatsArgEngines engines by value (a kernel argument):
+// engine r owns the launch's blocks [d[r].pair0, end[r]).
+struct StatsArgs {
+    StatsDesc d[kStatsArgEngines];
+    int end[kStatsArgEngines];
+    int n;
+};
+// The per-device address of the allocation-free entry point's scratch
+// (a file-scope device array of kStatsScratchEngines * kStatsWords words).
+hipError_t stats_scratch(unsigned long long** out);
+// n words at p := 0 (a fill kernel, not a memset node).
+hipError_t launch_fill_zero(unsigned long long* p, size_t n, hipStream_t s);
+hipError_t launch_stats_args(int d, bool narrow, const StatsArgs& a, int blocks, hipStream_t s);
 int stats_pair_blocks(int d, int lgL);
 int stats_cell_blocks(int ncells);
 // every engine of one launch has the same dimension and site width

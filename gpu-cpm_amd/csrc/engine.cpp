@@ -262,6 +262,7 @@ struct cpm_engine {
     int32_t* d_pref = nullptr;           // [(n+1)*3] llround(p 2^16)
     std::vector<int32_t> ctype;          // [n+1] type of each cell at push
     hipStream_t stream = nullptr;
+    hipEvent_t ev_ready = nullptr;  // marks this engine's queued work for other streams to wait on
     uint32_t* d_lat = nullptr;
     // 16-bit form of the lattice (cell | type << lat16_bits) for the per-phase
     // kernels; wide_valid / narrow_valid say which copy holds the current
@@ -345,6 +346,7 @@ struct cpm_engine {
         if (h_act) (void)hipHostFree(h_act);
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (ev_ready) (void)hipEventDestroy(ev_ready);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -788,6 +790,8 @@ int cpm_create(int dimension, int dimensionality, int number_of_cells, int tempe
     const size_t rows = (size_t)g->n + 1;
     hipError_t err;
     if ((err = hipStreamCreate(&g->stream)) != hipSuccess) return bail(err, "hipStreamCreate");
+    if ((err = hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming)) != hipSuccess)
+        return bail(err, "hipEventCreate");
     if ((err = hipHostMalloc((void**)&g->h_lat, g->sites * 4, hipHostMallocDefault)) != hipSuccess)
         return bail(err, "hipHostMalloc(lattice)");
     std::memset(g->h_lat, 0, g->sites * 4);
@@ -2195,6 +2199,70 @@ int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t
         }
     }
     S.last_ms = ms_total;
+    return CPM_OK;
+}
+
+int cpm_energy_stats_many_noalloc(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments) {
+    if (!sims || n < 0) return fail(CPM_EINVAL, "null engine list");
+    if (!pairs || !moments) return fail(CPM_EINVAL, "null output");
+    if (n > gcpm::kStatsScratchEngines)
+        return fail(CPM_EINVAL, "cpm_energy_stats_many_noalloc: at most " +
+                                    std::to_string(gcpm::kStatsScratchEngines) + " engines per call");
+    for (int i = 0; i < n; i++) {
+        if (int rc = check_handle(sims[i])) return rc;
+        if (!sims[i]->pushed) return fail(CPM_ESTATE, "energy statistics exist only after push_to_gpu");
+    }
+    if (n == 0) return CPM_OK;
+    constexpr int nw = gcpm::kStatsWords, np = CPM_NTYPES * CPM_NTYPES, nm = CPM_NTYPES * 5;
+    unsigned long long* scr = nullptr;
+    HIP_TRY(gcpm::stats_scratch(&scr));  // the device's own copy of the scratch array
+    // Everything runs on the first engine's stream, after every engine's
+    // queued work.
+    hipStream_t s = sims[0]->stream;
+    for (int i = 1; i < n; i++) {
+        HIP_TRY(hipEventRecord(sims[i]->ev_ready, sims[i]->stream));
+        HIP_TRY(hipStreamWaitEvent(s, sims[i]->ev_ready, 0));
+    }
+    HIP_TRY(gcpm::launch_fill_zero(scr, (size_t)n * nw, s));
+    for (int grp = 0; grp < 4; grp++) {
+        const int d = 2 + (grp >> 1);
+        const bool narrow_grp = grp & 1;
+        gcpm::StatsArgs a;  // (on the stack: a kernel argument)
+        a.n = 0;
+        int blocks = 0;
+        for (int i = 0; i <= n; i++) {
+            const bool take = i < n && sims[i]->d == d && !sims[i]->wide_valid == narrow_grp;
+            if (take) {
+                cpm_handle h = sims[i];
+                gcpm::StatsDesc& e = a.d[a.n];
+                e.lat = narrow_grp ? (const void*)h->d_lat16 : (const void*)h->d_lat;
+                e.narrow = narrow_grp ? 1 : 0;
+                e.lgL = h->lgL;
+                e.idbits = narrow_grp ? (uint32_t)h->lat16_bits : 24u;
+                e.nt = h->nt;
+                e.tab = h->d_tab[0];
+                e.ctype = h->d_ctype;
+                e.ncells = h->n;
+                e.out = scr + (size_t)i * nw;
+                e.pair0 = blocks;
+                blocks += gcpm::stats_pair_blocks(d, h->lgL);
+                e.cell0 = blocks;
+                blocks += gcpm::stats_cell_blocks(h->n);
+                a.end[a.n] = blocks;
+                a.n++;
+            }
+            if (a.n > 0 && (a.n == gcpm::kStatsArgEngines || i == n)) {
+                HIP_TRY(gcpm::launch_stats_args(d, narrow_grp, a, blocks, s));
+                a.n = 0;
+                blocks = 0;
+            }
+        }
+    }
+    HIP_TRY(hipMemcpy2DAsync(pairs, np * sizeof(int64_t), scr, nw * sizeof(unsigned long long),
+                             np * sizeof(int64_t), n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpy2DAsync(moments, nm * sizeof(int64_t), scr + np, nw * sizeof(unsigned long long),
+                             nm * sizeof(int64_t), n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return CPM_OK;
 }
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 
 #include "cpm_device.h"
 
@@ -1830,6 +1831,69 @@ __global__ __launch_bounds__(kBlock) void k_stats_batch(const StatsDesc* __restr
         stats_pairs_block<D, ST>(e, b - e.pair0);
     else
         stats_cells_block(e, b - e.cell0);
+}
+
+// The allocation-free statistics launch (cpm_energy_stats_many_noalloc):
+// up to kStatsArgEngines engines per launch, their descriptors passed by
+// value as the kernel argument; block b serves the engine whose block range
+// [pair0, end) holds it.
+template <int D, class ST>
+__global__ __launch_bounds__(kBlock) void k_stats_args(StatsArgs a) {
+    const int b = (int)blockIdx.x;
+    int r = 0;
+    while (r + 1 < a.n && b >= a.end[r]) r++;  // uniform per block
+    const StatsDesc e = a.d[r];
+    if (b < e.cell0)
+        stats_pairs_block<D, ST>(e, b - e.pair0);
+    else
+        stats_cells_block(e, b - e.cell0);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// Scratch of the allocation-free statistics entry point: the statistics of
+// kStatsScratchEngines engines (400 pair counts + 100 moments each).  Each
+// GPU holds its own copy until the process ends; its address is looked up
+// once per device (the array's name is not a device address on the host).
+static __device__ __align__(256) unsigned long long scratch[kStatsScratchEngines * kStatsWords];
+
+hipError_t stats_scratch(unsigned long long** out) {
+    static std::mutex mu;
+    static unsigned long long* ptr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+        ptr[dev] = static_cast<unsigned long long*>(p);
+    }
+    *out = ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_fill_zero(unsigned long long* p, size_t n, hipStream_t s) {
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+static_assert(sizeof(StatsArgs) <= 4096, "the descriptors are passed as one kernel argument");
+
+hipError_t launch_stats_args(int d, bool narrow, const StatsArgs& a, int blocks, hipStream_t s) {
+#define CPM_STATS(DD, ST) hipLaunchKernelGGL((k_stats_args<DD, ST>), dim3(blocks), dim3(kBlock), 0, s, a)
+    if (d == 2) {
+        if (narrow) CPM_STATS(2, uint16_t); else CPM_STATS(2, uint32_t);
+    } else {
+        if (narrow) CPM_STATS(3, uint16_t); else CPM_STATS(3, uint32_t);
+    }
+#undef CPM_STATS
+    return hipGetLastError();
 }
 
 int stats_pair_blocks(int d, int lgL) {

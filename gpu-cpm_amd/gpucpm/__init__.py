@@ -398,6 +398,20 @@ def energy_stats_many(sims):
     return pairs, mom
 
 
+def energy_stats_many_noalloc(sims):
+    """energy_stats_many without any device allocation (for callers that
+    cannot allocate): at most 64 engines; overlapping calls on one GPU race."""
+    sims = list(sims)
+    n = len(sims)
+    pairs = np.zeros((n, 20, 20), np.int64)
+    mom = np.zeros((n, 20, 5), np.int64)
+    hs = (ctypes.c_void_p * max(n, 1))(*[s._h for s in sims])
+    P = ctypes.POINTER(ctypes.c_int64)
+    check(lib.cpm_energy_stats_many_noalloc(ctypes.cast(hs, ctypes.POINTER(ctypes.c_void_p)), n,
+                                            pairs.ctypes.data_as(P), mom.ctypes.data_as(P)))
+    return pairs, mom
+
+
 def energy_stats_kernel_ms():
     """Kernel time (ms) of the last energy_stats / energy_stats_many call:
     one statistics launch covers every engine of a call."""

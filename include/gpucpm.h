@@ -275,6 +275,16 @@ int cpm_energy_stats(cpm_handle h, int64_t* pairs, int64_t* moments);
  * for all of them, after each engine's queued work), into pairs[n * 400]
  * and moments[n * 100]. */
 int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments);
+/* Extension with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): cpm_energy_stats_many with no
+ * allocation.  Its statistics land in a file-scope device scratch array of
+ * 64 engines x 500 words (256 KiB, held by each GPU until the process ends),
+ * so it takes at most 64 engines (CPM_EINVAL beyond); it runs on the first
+ * engine's stream after every engine's queued work, clears the scratch with
+ * a fill kernel and waits for the result.  Calls that overlap on one GPU
+ * share the scratch and race: serialize them. */
+int cpm_energy_stats_many_noalloc(const cpm_handle* sims, int n, int64_t* pairs, int64_t* moments);
 /* Extension: kernel time (ms, HIP events on the statistics stream) of the
  * last cpm_energy_stats / cpm_energy_stats_many call: one launch per
  * dimensionality and site width (16- or 32-bit) covers every engine of the
