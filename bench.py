@@ -399,13 +399,13 @@ def measure_gradient(nrep, L, mcs, log, fit=True):
     return out
 
 
-def bench_interactive(a, log):
-    """The notebook pattern (examples/gpucpm_sorting_example.ipynb: one
+def measure_interactive(frames, log, no_cpu=False):
+    """The notebook pattern (examples/gpucpm_sorting_example.ipynb:316: one
     run(iterations=ticks, inner_iterations=1) per frame, then a readback):
     frames per second on the 256^2 sorting lattice with one MCS and one
-    pull_from_gpu per frame, the default shared=0 roles of the notebook
-    (positions_per_thread 4, c 2)."""
-    import gpucpm
+    pull_from_gpu per frame, the notebook's shared=0 roles (positions_per_
+    thread 4, c 2: 64 colour phases of 1024 attempts per MCS).  The engine
+    runs such small phases in one workgroup with the lattice in LDS."""
     from gpucpm import workloads
     wl = workloads.sorting2d(L=256, grid=20, edge=10, seed=2)
     sim = wl.make_cpm()
@@ -415,8 +415,7 @@ def bench_interactive(a, log):
               positions_per_thread=4, positions_per_checkerboard=2,
               updates_per_checkerboard_switch=1, updates_per_barrier=1,
               iterations=1, inner_iterations=1, shared=0, partial_dispatch=1)
-    frames = a.steps * a.mcs_per_step
-    for _ in range(a.warmup * 10):
+    for _ in range(50):
         sim.run(**kw)
         sim.pull_from_gpu()
     log(f"interactive: {frames} frames of 1 MCS + readback at 256^2")
@@ -426,33 +425,48 @@ def bench_interactive(a, log):
         sim.pull_from_gpu()
     wall = time.perf_counter() - t0
     sim.timing_begin()
-    for _ in range(a.steps):
+    nt = min(frames, 200)
+    for _ in range(nt):
         sim.run(**kw)
     kernel_ms, launches, updates = sim.timing_end()
     launch = sim.last_launch()
-    return {
-        "metric": "frames/s (1 MCS + lattice readback per frame)",
-        "value": frames / wall,
-        "unit": "frames/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "interactive notebook pattern: 2D sorting 256^2, 400 cells, "
-                               "run(iterations=1) + pull per frame, shared=0, ppt 4, c 2",
-                   "frames": frames},
-        "ms_per_frame": wall / frames * 1e3,
-        "kernel_ms_per_mcs": kernel_ms / a.steps,
-        "launches_per_mcs": launches / a.steps,
-        "kernel": launch["kernel"],
-        "roofline": None,
-        "cpu_baseline": None,
-    }
+    sites = wl.L ** 2
+    out = {"workload": "interactive notebook pattern: 2D sorting 256^2, 400 cells, run(iterations=1) + "
+                       "pull_from_gpu per frame, shared=0, positions_per_thread 4, c 2",
+           "value": frames / wall, "unit": "frames/s", "frames": frames,
+           "ms_per_frame": wall / frames * 1e3, "kernel_ms_per_mcs": kernel_ms / nt,
+           "launches_per_mcs": launches / nt, "kernel": launch["kernel"], "launch_form": launch["form"],
+           "site_updates_per_s_kernel": sites * nt / (kernel_ms / 1e3),
+           "cpu_baseline": None}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        p = wl.params()
+        prm = O.Params.make(p["temperature"], J=p["J"], S=p["S"], lam_area=p["lam_area"],
+                            target_area=p["target_area"], lam_perim=p["lam_perim"],
+                            target_perim=p["target_perim"])
+        lat = wl.lattice.copy()
+        tab = O.tally(lat, 2, wl.ncells)
+        cores = len(os.sched_getaffinity(0))
+        n = 200
+        t0 = time.perf_counter()
+        for m in range(n):
+            O.sweep(lat, 2, tab, prm, 2, 1, 1, m, 1, cores, shared=False, ppt=4)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "frames/s (MCS/s, no readback)", "cores": cores,
+                               "kind": "port", "sample": f"{n} one-MCS sweeps of the same lattice and "
+                                                        "schedule, oracle/libcpm_oracle.so with OpenMP"}
+    return out
+
+
+def bench_interactive(a, log):
+    r = measure_interactive(a.steps * a.mcs_per_step, log, a.no_cpu)
+    line = {"metric": "frames/s (1 MCS + lattice readback per frame)", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": r["ms_per_frame"] * a.mcs_per_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic", "config": {"workload": r["workload"]}, "roofline": None}
+    line.update(r)
+    return line
 
 
 def kernel_token(name):
@@ -594,6 +608,7 @@ def main():
                           "as one batch, then dH/dtheta for J (3x3), lambda_A, A0, lambda_P, P0 "
                           "against FP64 central differences (step 1e-3)")
         configs["C5_gradient"] = r5
+        configs["interactive"] = measure_interactive(2000, log, a.no_cpu)
         out["configs"] = configs
         out["gradient_rel_error"] = r5["gradient_rel_error"]
     print(json.dumps(out), flush=True)

@@ -159,6 +159,14 @@ struct PersistArgs {
     int lntx;               // log2 tiles per row
 };
 
+// A whole inner_iterations chunk in one workgroup with the lattice in LDS.
+struct SmallArgs {
+    SweepConsts k;
+    uint32_t mcs0;          // first MCS counter
+    int nmcs;               // MCS in this launch
+    int lh;                 // log2 (same-colour sub-blocks per axis)
+};
+
 // One replica of a batched run.
 struct ReplicaDesc {
     SweepConsts k;
@@ -206,6 +214,10 @@ const char* phase_kernel_name(int id);
 hipError_t persist_max_blocks(int d, bool lds2d, size_t lds_bytes, int* max_blocks);
 hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, size_t lds_bytes,
                           hipStream_t s);
+// The single-workgroup form: dynamic LDS bytes it needs, and its launch
+// (one block; e0/e1 or null are recorded at the dispatch's start and end).
+size_t small_lds_bytes(int d, int lgL, int ncells, bool narrow);
+hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 hipError_t batch_max_blocks(int d, int* max_blocks);
 hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s);
 // e0/e1 (or null) are recorded at the dispatch's start and end; name (or

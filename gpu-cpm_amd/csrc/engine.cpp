@@ -51,12 +51,17 @@ constexpr int kMaxScaleLog2 = 8;
 
 // How one run is launched.  All three forms give identical results.
 struct Plan {
-    enum Mode { kPhase, kGlobal, kLds2d } mode = kPhase;
+    enum Mode { kPhase, kGlobal, kLds2d, kSmall } mode = kPhase;
     int blocks = 0;        // cooperative grid size
     int ltile = 0;         // log2 tile edge (kLds2d)
     size_t lds_bytes = 0;  // dynamic LDS per block (kLds2d)
 };
 constexpr int kMaxThr = 1 << 22;
+// The single-workgroup form: dynamic LDS it may use (of the 160 KiB of a
+// gfx950 workgroup, leaving room for its static LDS), and the largest number
+// of attempts per phase for which it is chosen automatically.
+constexpr size_t kSmallLdsBudget = 158 * 1024;
+constexpr long kSmallAutoAttempts = 4096;
 
 // Per-phase kernel options (cpm_set_kernel_options); -1 keeps the default.
 // Every option computes the same results.
@@ -91,6 +96,7 @@ struct FormTuner {
         Entry* e;
         int idx;
         hipEvent_t e0, e1;
+        double scale = 1.0;  // the sample is the elapsed time divided by this (e.g. MCS of a run)
     };
     static std::map<uint64_t, Entry>& table() {
         static std::map<uint64_t, Entry> t;
@@ -153,7 +159,7 @@ struct FormTuner {
             const Pending& p = pending[i];
             float ms = 0.0f;
             if ((err = hipEventElapsedTime(&ms, p.e0, p.e1)) != hipSuccess) break;
-            p.e->ms[p.idx] += ms;
+            p.e->ms[p.idx] += ms / p.scale;
             p.e->n[p.idx]++;
         }
         if (err != hipSuccess) {  // drop this run's samples rather than retry them
@@ -294,6 +300,7 @@ struct cpm_engine {
     int scale = 1;
     int nt = 1;  // types present in the lattice at the last push
     int last_kernel = -1;  // per-phase kernel of the last run (gcpm::phase_kernel_name)
+    bool small_narrow = false;  // the last single-workgroup run used 16-bit sites
     bool autotune = false;  // cpm_set_autotune
     FormTuner tuner;
 
@@ -521,7 +528,7 @@ struct cpm_engine {
     // Launch form for a run with checkerboard edge c: one launch per phase
     // unless cpm_set_launch asks for a persistent form (and tile edge).
     int plan(int c, Plan* p) {
-        if (want_form < Plan::kGlobal) {
+        if (want_form < Plan::kGlobal || want_form == Plan::kSmall) {
             p->mode = Plan::kPhase;
             return CPM_OK;
         }
@@ -1227,8 +1234,104 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         k.refstream = 1;
         k.rstate = h->d_rstate;
     }
+    FormTuner::Entry* run_tune = nullptr;  // run-level autotuning of the launch form
+    int run_ti = -1;
+    hipEvent_t run_e0 = nullptr, run_e1 = nullptr;
+    auto finish_run_tune = [&]() -> int {
+        if (!run_tune) return CPM_OK;
+        HIP_TRY(hipEventRecord(run_e1, h->stream));
+        FormTuner::Pending pd{run_tune, run_ti, run_e0, run_e1};
+        pd.scale = (double)total;
+        h->tuner.pending.push_back(pd);
+        HIP_TRY(h->tuner.settle());
+        return CPM_OK;
+    };
+    // The single-workgroup form (lattice, tables and phases in one
+    // workgroup's LDS): forced by launch form 3, chosen automatically when a
+    // phase has at most kSmallAutoAttempts attempts (a whole chunk is then
+    // cheaper on one CU than a launch per phase); not with cell_sync, the
+    // reference stream or the act term.
+    {
+        const int bits_s = h->narrow_bits();
+        const size_t bytes = gcpm::small_lds_bytes(h->d, h->lgL, h->n, bits_s != 0);
+        const bool fits = bytes <= kSmallLdsBudget;
+        const bool ok = !a->cell_sync && !k.refstream && !k.act && fits;
+        const long attempts = ((long)1 << (lh * h->d)) * u;
+        if (h->want_form == Plan::kSmall && !ok)
+            return fail(CPM_EINVAL,
+                        "launch form 3 (single workgroup) needs a lattice whose sites and tables fit in "
+                        "one workgroup's LDS, and no cell_sync, reference stream or act term");
+        bool use = ok && (h->want_form == Plan::kSmall || (h->want_form == -1 && attempts <= kSmallAutoAttempts));
+        // Autotuner: where the form is eligible, whole runs alternate
+        // between it and per-phase launches (each run timed per MCS) until
+        // both have their samples; then the faster is kept for the workload.
+        if (ok && h->autotune && h->want_form == -1 && !h->timing) {
+            const uint64_t key = (uint64_t)3 << 62 | (uint64_t)h->d | (uint64_t)h->lgL << 2 |
+                                 (uint64_t)k.lc << 8 | (uint64_t)u << 12 | (uint64_t)k.roles << 24 |
+                                 (uint64_t)h->nt << 25 | (uint64_t)k.chem << 30 | (uint64_t)k.pers << 31 |
+                                 (uint64_t)k.S << 32 | (uint64_t)(h->narrow_bits() != 0) << 40 |
+                                 (uint64_t)a->inner_iterations << 41;
+            run_tune = &h->tuner.entry(key, {0, 1});  // 0 per phase, 1 single workgroup
+            if (run_tune->best >= 0) {
+                use = run_tune->best == 1;
+                run_tune = nullptr;
+            } else {
+                run_ti = h->tuner.pick(*run_tune);
+                if (run_ti < 0) {
+                    run_tune = nullptr;  // (all samples pending)
+                } else {
+                    use = run_tune->forms[run_ti] == 1;
+                    HIP_TRY(h->tuner.events(&run_e0, &run_e1));
+                    HIP_TRY(hipEventRecord(run_e0, h->stream));
+                }
+            }
+        }
+        if (use) {
+            plan = Plan{};
+            plan.mode = Plan::kSmall;
+            plan.blocks = 1;
+        }
+    }
     h->last_plan = plan;
     const int64_t updates_per_mcs = (int64_t)k.rounds * k.phases_per_round * ((int64_t)1 << (lh * h->d)) * u;
+
+    if (plan.mode == Plan::kSmall) {
+        gcpm::SmallArgs sa{};
+        sa.k = k;
+        sa.lh = lh;
+        sa.nmcs = a->inner_iterations;
+        const int bits = h->narrow_bits();
+        if (bits) {
+            if (int rc = h->ensure_narrow(bits)) return rc;
+            sa.k.lat16 = h->d_lat16;
+            sa.k.idbits = (uint32_t)bits;
+            sa.k.idmask = (1u << bits) - 1u;
+        } else {
+            if (int rc = h->ensure_wide()) return rc;
+            h->narrow_valid = false;
+        }
+        const bool pers = k.pers != 0;
+        if (pers && !h->ring_valid)
+            if (int rc = h->init_ring()) return rc;
+        for (long ch = 0; ch < chunks; ch++) {
+            sa.mcs0 = (uint32_t)(h->mcs + (uint64_t)ch * a->inner_iterations);
+            hipEvent_t e0, e1;
+            if (int rc = h->next_events(&e0, &e1)) return rc;
+            HIP_TRY(gcpm::launch_small(h->d, bits != 0, sa, h->stream, e0, e1));
+            if (bits) h->wide_valid = false;
+            if (h->timing) {
+                h->t_launches++;
+                h->t_updates += updates_per_mcs * a->inner_iterations;
+            }
+            // end of a chunk (one reference launch): the tables are settled;
+            // update the preferred directions from the centroid history
+            if (pers)
+                if (int rc = h->update_pref_dirs()) return rc;
+        }
+        h->small_narrow = bits != 0;
+        h->mcs += (uint64_t)total;
+        return finish_run_tune();
+    }
 
     if (plan.mode == Plan::kPhase) {
         gcpm::PhaseArgs pa{};
@@ -1355,6 +1458,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         // Fold the last phase's deltas into the lagging base: both bases
         // then hold the current state and both delta buffers are zero.
         HIP_TRY(gcpm::launch_settle(h->d_tab, h->d_del, g, h->n, h->stream));
+        if (int rc = finish_run_tune()) return rc;
         HIP_TRY(h->tuner.settle());
     } else {
         if (int rc = h->ensure_wide()) return rc;
@@ -1494,13 +1598,16 @@ const char* cpm_last_kernel(cpm_handle h) {
     switch (h->last_plan.mode) {
         case Plan::kGlobal: return h->d == 3 ? "k_persist_global<3>" : "k_persist_global<2>";
         case Plan::kLds2d: return "k_persist_lds2d";
+        case Plan::kSmall:
+            return h->d == 3 ? (h->small_narrow ? "k_small_lds<3, unsigned short>" : "k_small_lds<3, unsigned int>")
+                             : (h->small_narrow ? "k_small_lds<2, unsigned short>" : "k_small_lds<2, unsigned int>");
         default: return gcpm::phase_kernel_name(h->last_kernel);
     }
 }
 
 int cpm_set_launch(cpm_handle h, int form, int tile) {
     if (int rc = check_handle(h)) return rc;
-    if (form < -1 || form > 2) return fail(CPM_EINVAL, "form must be -1 (auto), 0, 1 or 2");
+    if (form < -1 || form > 3) return fail(CPM_EINVAL, "form must be -1 (auto), 0, 1, 2 or 3");
     if (tile < 0 || (tile && !is_pow2(tile))) return fail(CPM_EINVAL, "tile must be 0 or a power of two");
     h->want_form = form;
     h->want_tile = tile;

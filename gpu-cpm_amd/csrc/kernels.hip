@@ -219,7 +219,7 @@ struct LdsLat3 {
 struct PhaseTables {
     const int64_t* R;   // tab[g & 1]: the phase-start snapshot
     int64_t* W;         // tab[(g + 1) & 1]: becomes the state after phase g
-    int64_t* Dc;        // del[g & 1]: deltas of this phase
+    int64_t* Dc;        // del[g & 1]: deltas of this phase (null: W alone takes them)
     int64_t* Dp;        // del[(g + 1) & 1]: deltas of phase g - 1, folded into W
 };
 
@@ -635,12 +635,12 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     if (cs) {
         const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
         atomic_add_i64(T.W + cs, delta);
-        atomic_add_i64(T.Dc + cs, delta);
+        if (T.Dc) atomic_add_i64(T.Dc + cs, delta);
     }
     if (ct) {
         const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
         atomic_add_i64(T.W + ct, delta);
-        atomic_add_i64(T.Dc + ct, delta);
+        if (T.Dc) atomic_add_i64(T.Dc + ct, delta);
     }
 }
 
@@ -1444,6 +1444,215 @@ __global__ __launch_bounds__(kBlock) void k_persist_lds2d(PersistArgs a) {
         }
     }
     if (g && grid_barrier(a.bar, gen, gridDim.x, a.bar + 2)) settle_in_kernel(k, g, gtid, gsz);
+}
+
+// A whole lattice resident in one workgroup's LDS (the single-workgroup
+// form): sites in row-major order with periodic wrap.
+template <int D, class T>
+struct LdsWholeLat {
+    using Site = T;
+    T* s;
+    int lgL, Lm;
+    struct Pos {
+        int x, y, z;
+        uint32_t i;
+    };
+    __device__ __forceinline__ Pos at(int x, int y, int z) const {
+        return {x, y, z, site_index<D>(x, y, z, lgL)};
+    }
+    __device__ __forceinline__ uint32_t self(const Pos& q) const { return s[q.i]; }
+    __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int dz) const {
+        return s[site_index<D>((q.x + dx) & Lm, (q.y + dy) & Lm, (q.z + dz) & Lm, lgL)];
+    }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { s[q.i] = (T)v; }
+    __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.i; }
+    __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const {
+        constexpr int NB = D == 3 ? 27 : 9;
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            int ox, oy, oz;
+            block_offset<D>(n, ox, oy, oz);
+            nv[n] = nb(q, ox, oy, oz);
+        }
+    }
+};
+
+// Active sub-block offsets of phase pc of a round (as the host's
+// phase_offsets): colour (cbX outermost, kernels2d.cu:179-180), then group.
+template <int D>
+__device__ __forceinline__ void phase_offsets_dev(const SweepConsts& k, int pc, int& ox, int& oy, int& oz) {
+    const int K = k.K, S = k.S;
+    const int ngrp = D == 3 ? S * S * S : S * S;
+    const int col = pc / ngrp, grp = pc % ngrp;
+    const int cx = D == 3 ? col / (K * K) : col / K;
+    const int cy = D == 3 ? (col / K) % K : col % K;
+    const int cz = D == 3 ? col % K : 0;
+    const int gx = D == 3 ? grp / (S * S) : grp / S;
+    const int gy = D == 3 ? (grp / S) % S : grp % S;
+    const int gz = D == 3 ? grp % S : 0;
+    ox = gx * K + cx;
+    oy = gy * K + cy;
+    oz = gz * K + cz;
+}
+
+// The single-workgroup form: one block of kSmallThreads threads runs a whole
+// inner_iterations chunk of a lattice that fits in LDS (lattice, the
+// snapshot table rows R and the phase deltas Dl), with __syncthreads in
+// place of kernel boundaries.  Each phase: the attempts of the active
+// sub-blocks (u = 1: drawn, the ones whose sites differ queued in LDS and
+// finished densely; u > 1: one thread per sub-block, its u attempts in
+// order), then the deltas are folded into R.  Same draws, roles, snapshot
+// rule and integer arithmetic as the per-phase kernels, so the results are
+// identical.  At the end the lattice and both table buffers are written back
+// (the run's state is then settled).
+constexpr int kSmallThreads = 1024;
+
+template <int D, class ST, class PV>
+__device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* lat_s, int64_t* R,
+                                          int64_t* Dl, QueuedAttempt<D, false, ST>* queue,
+                                          unsigned int* nqueued) {
+    const SweepConsts& k = a.k;
+    const int tid = threadIdx.x;
+    const int nsub = 1 << (D * a.lh);
+    const int hm = (1 << a.lh) - 1;
+    const int Lm = (1 << k.lgL) - 1;
+    using Lat = LdsWholeLat<D, ST>;
+    const Lat lat{lat_s, k.lgL, Lm};
+    const PhaseTables T{R, Dl, nullptr, nullptr};
+    const uint32_t* __restrict__ thr = k.thr;
+    for (int m = 0; m < a.nmcs; m++) {
+        const uint32_t mcs = a.mcs0 + (uint32_t)m;
+        for (int r = 0; r < k.rounds; r++) {
+            for (int pc = 0; pc < k.phases_per_round; pc++) {
+                int ox, oy, oz;
+                phase_offsets_dev<D>(k, pc, ox, oy, oz);
+                const uint32_t phase = (uint32_t)(r * k.phases_per_round + pc);
+                for (int q0 = 0; q0 < nsub; q0 += kSmallThreads) {
+                    const int q = q0 + tid;
+                    int bx = 0, by = 0, bz = 0;
+                    if (q < nsub) {
+                        bx = ((q & hm) << k.lstride) + ox;
+                        by = (((q >> a.lh) & hm) << k.lstride) + oy;
+                        bz = D == 3 ? ((q >> (2 * a.lh)) << k.lstride) + oz : 0;
+                    }
+                    if (k.u > 1) {
+                        if (q < nsub) sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, mcs, phase, T);
+                        continue;
+                    }
+                    // u = 1: draw, queue the attempts whose sites differ
+                    bool active = false;
+                    QueuedAttempt<D, false, ST> e;
+                    if (q < nsub) {
+                        Draw<D, Lat> d;
+                        draw_attempt<D>(lat, k, sub_block_id<D>(k, bx, by, bz), bx, by, bz, mcs, phase, 0u, d);
+                        const uint32_t vt = d.nv[(D == 3 ? 27 : 9) / 2];
+                        active = d.vs != vt;  // kernels2d.cu:276
+                        e.site = d.t.i;
+                        e.vs = d.vs;
+                        e.w2 = d.w[2];
+                        e.aux = (uint32_t)d.dir;
+                        e.set_vt(vt);
+                    }
+                    if (tid == 0) *nqueued = 0;
+                    __syncthreads();
+                    const uint64_t mask = __ballot(active);
+                    if (mask) {
+                        uint32_t base = 0;
+                        if (__lane_id() == 0) base = atomicAdd(nqueued, (uint32_t)__popcll(mask));
+                        base = __shfl(base, 0);
+                        const uint32_t before = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                        if (active) queue[base + before] = e;
+                    }
+                    __syncthreads();
+                    const int nq = (int)*nqueued;
+                    for (int j = tid; j < nq; j += kSmallThreads) {
+                        const QueuedAttempt<D, false, ST>& qa = queue[j];
+                        const int lgL = k.lgL;
+                        Draw<D, Lat> d;
+                        d.t = typename Lat::Pos{(int)(qa.site & Lm), (int)((qa.site >> lgL) & Lm),
+                                                D == 3 ? (int)(qa.site >> (2 * lgL)) : 0, qa.site};
+                        d.w[0] = d.w[1] = d.w[3] = 0u;
+                        d.w[2] = qa.w2;
+                        lat.block(d.t, d.nv);
+                        d.vs = qa.vs;
+                        d.dir = qa.dir();
+                        d.tick = mcs;
+                        finish_attempt<D, false>(lat, k, P, thr, T, CellSync{0, nullptr, 0}, 0u, d);
+                    }
+                    __syncthreads();  // (the queue is reused by the next round)
+                }
+                __syncthreads();
+                // fold the phase's deltas into the snapshot
+                for (int c = tid; c <= k.ncells; c += kSmallThreads) {
+                    const int64_t v = Dl[c];
+                    if (v) {
+                        R[c] += v;
+                        Dl[c] = 0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+template <int D, class ST>
+__global__ __launch_bounds__(kSmallThreads) void k_small_lds(SmallArgs a) {
+    extern __shared__ __align__(16) unsigned char small_smem[];
+    const SweepConsts& k = a.k;
+    const uint32_t sites = 1u << (D * k.lgL);
+    const int rows = k.ncells + 1;
+    int64_t* R = reinterpret_cast<int64_t*>(small_smem);
+    int64_t* Dl = R + rows;
+    auto* queue = reinterpret_cast<QueuedAttempt<D, false, ST>*>(Dl + rows);
+    ST* lat_s = reinterpret_cast<ST*>(queue + kSmallThreads);
+    __shared__ unsigned int nqueued;
+    const ST* __restrict__ g = lat_base<ST>(k);
+    for (uint32_t i = threadIdx.x; i < sites; i += kSmallThreads) lat_s[i] = g[i];
+    for (int c = threadIdx.x; c < rows; c += kSmallThreads) {
+        R[c] = k.tab[0][c];  // settled: both buffers equal, deltas zero
+        Dl[c] = 0;
+    }
+    __syncthreads();
+    if (k.nt <= kSmallTypes) {
+        SmallParams P;
+        P.load(k.prm);
+        small_run<D, ST>(a, P, lat_s, R, Dl, queue, &nqueued);
+    } else {
+        const LdsParamView P{k.prm->w, k.nt, k.nt * k.nt};  // (read from global memory)
+        small_run<D, ST>(a, P, lat_s, R, Dl, queue, &nqueued);
+    }
+    ST* gw = lat_base<ST>(k);
+    for (uint32_t i = threadIdx.x; i < sites; i += kSmallThreads) gw[i] = lat_s[i];
+    for (int c = threadIdx.x; c < rows; c += kSmallThreads) {
+        k.tab[0][c] = R[c];
+        k.tab[1][c] = R[c];
+    }
+}
+
+size_t small_lds_bytes(int d, int lgL, int ncells, bool narrow) {
+    const size_t sites = (size_t)1 << (d * lgL);
+    const size_t entry = narrow ? sizeof(QueuedAttempt<2, false, uint16_t>) : sizeof(QueuedAttempt<2, false, uint32_t>);
+    const size_t entry3 = narrow ? sizeof(QueuedAttempt<3, false, uint16_t>) : sizeof(QueuedAttempt<3, false, uint32_t>);
+    return 16 * ((size_t)ncells + 1) + (d == 3 ? entry3 : entry) * kSmallThreads + sites * (narrow ? 2 : 4);
+}
+
+hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    const size_t bytes = small_lds_bytes(d, a.k.lgL, a.k.ncells, narrow);
+    const void* f = d == 2 ? (narrow ? (const void*)&k_small_lds<2, uint16_t> : (const void*)&k_small_lds<2, uint32_t>)
+                           : (narrow ? (const void*)&k_small_lds<3, uint16_t> : (const void*)&k_small_lds<3, uint32_t>);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+#define CPM_SMALL(DD, ST) \
+    hipExtLaunchKernelGGL((k_small_lds<DD, ST>), dim3(1), dim3(kSmallThreads), (unsigned)bytes, s, e0, e1, 0, a)
+    if (d == 2) {
+        if (narrow) CPM_SMALL(2, uint16_t); else CPM_SMALL(2, uint32_t);
+    } else {
+        if (narrow) CPM_SMALL(3, uint16_t); else CPM_SMALL(3, uint32_t);
+    }
+#undef CPM_SMALL
+    return hipGetLastError();
 }
 
 // Wave-aggregated updates of per-cell rows: the lanes of a wave are grouped

@@ -10,17 +10,21 @@ import oracle as O
 # the engine_options fixture (tests/conftest.py).  None of them changes
 # results.
 ENGINE_OPTIONS = {}
-_FORMS = {"phase": 0, "global": 1, "lds": 2}
+_FORMS = {"auto": -1, "phase": 0, "global": 1, "lds": 2, "small": 3}
 
 
 def make_cpm(wl):
-    """wl.make_cpm() with the current test's kernel options and launch form."""
+    """wl.make_cpm() with the current test's kernel options and launch form.
+    Kernel options select among the per-phase kernels, so a test that sets
+    them without a launch form runs one launch per phase."""
     sim = wl.make_cpm()
     opts = dict(ENGINE_OPTIONS)
     mode, tile = opts.pop("mode", None), int(opts.pop("tile", 0))
     opts = {k: int(v) for k, v in opts.items()}
     if opts:
         sim.set_kernel_options(**opts)
+        if mode is None:
+            mode = "phase"
     if mode is not None:
         sim.set_launch(_FORMS[mode], tile)
     return sim

@@ -45,11 +45,13 @@ def test_tally_after_push_matches_oracle():
         assert_tables_equal(sim.get_cell_table(), tab, wl.d)
 
 
+@pytest.mark.parametrize("mode", ["phase", "small"])
 @pytest.mark.parametrize("c,u", [(1, 1), (2, 1), (2, 4), (4, 3), (4, 16)])
-def test_sorting2d_small_bit_exact(c, u):
+def test_sorting2d_small_bit_exact(c, u, mode, engine_options):
     _, W = _gpu()
+    engine_options(mode=mode)
     wl = W.sorting2d(L=128, grid=9, edge=10, seed=11)
-    _announce("sorting2d", wl.L, c, u)
+    _announce("sorting2d", wl.L, c, u, mode)
     nmcs = 30
     _, lat, table = _run_gpu(wl, c, u, seed=1234, nmcs=nmcs)
     lat_o, tab_o = run_oracle(wl, c, u, 1234, nmcs)
