@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="let the engine time its kernel forms during warmup and keep the fastest")
     ap.add_argument("--no-grad", action="store_true", help="skip the contrastive fit in the gradient record")
+    ap.add_argument("--launch", type=int, default=-1,
+                    help="launch form (-1 auto, 0 per phase, 3 one workgroup, 4 resident 2D tiles)")
     ap.add_argument("--no-configs", action="store_true",
                     help="sorting2d: only the headline record (no C3 / C4 / C5 sub-records)")
     ap.add_argument("--pmc-summary", nargs=6, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
@@ -110,7 +112,8 @@ def storage_of(kernel):
     return "uint32 sites (cell | type << 24)"
 
 
-def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_cpu, autotune, log):
+def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_cpu, autotune, log,
+                   launch=-1):
     """Whole-run throughput of one engine (inputs resident before timing),
     the sweep kernel's roofline from per-launch HIP events, and the CPU
     oracle on a bounded sample of the same workload."""
@@ -118,6 +121,7 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     sim.push_to_gpu()
     sim.set_seed(20240601)
     sim.set_autotune(autotune)
+    sim.set_launch(launch)
     # shared = 1: the reference's shared-memory schedule (K = 2 colours)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
               positions_per_thread=2 * c, positions_per_checkerboard=c,
@@ -567,7 +571,7 @@ def main():
           "plain2d": workloads.plain2d}[a.workload]()
     c, u = a.checkerboard, a.updates
     r = measure_single(wl, a.workload, c, u, a.steps, a.warmup, a.mcs_per_step, a.cpu_seconds,
-                       a.no_cpu, a.autotune, log)
+                       a.no_cpu, a.autotune, log, a.launch)
     workload_desc = {
         "sorting2d": "2D cell sorting 1024^2, 3969 cells (10x10 squares), two types, 10000-MCS config",
         "plain3d": "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood, 1000-MCS config",

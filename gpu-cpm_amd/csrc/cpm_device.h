@@ -167,6 +167,20 @@ struct SmallArgs {
     int lh;                 // log2 (same-colour sub-blocks per axis)
 };
 
+// The resident form (2D, shared roles, c = 1, u = 1): one cooperative block
+// per tile of 2^lte x 2^lte sites, 2^lnt tiles per axis, for one chunk.
+constexpr int kResFlagStride = 16;  // words between tile counters (one per 64 bytes)
+struct ResidentArgs {
+    SweepConsts k;
+    uint32_t mcs0;
+    int nmcs;
+    int lte, lnt;
+    unsigned int* flags;     // [tiles * kResFlagStride] phase counters, 0 on entry
+    unsigned int* err;       // abandoned-wait flag, 0 on entry
+    int32_t* bbox;           // [4 (n+1)] per-cell box of the chunk's sites (the guard)
+    const int32_t* anchor;   // [3 (n+1)] cell anchors
+};
+
 // One replica of a batched run.
 struct ReplicaDesc {
     SweepConsts k;
@@ -218,6 +232,13 @@ hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, s
 // (one block; e0/e1 or null are recorded at the dispatch's start and end).
 size_t small_lds_bytes(int d, int lgL, int ncells, bool narrow);
 hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+size_t resident_lds_bytes(int lte, bool narrow);
+hipError_t resident_max_blocks(bool narrow, int lte, int* max_blocks);
+hipError_t launch_resident(bool narrow, const ResidentArgs& a, int blocks, hipStream_t s);
+// begin: bbox := the boxes of the current lattice's cells; end: *bad |= 1 if
+// a cell's box (widened by the halo) spans more than two tiles per axis.
+hipError_t launch_resident_guard(bool begin, bool narrow, const SweepConsts& k, const int32_t* anchor,
+                                 int32_t* bbox, int te, unsigned int* bad, hipStream_t s);
 hipError_t batch_max_blocks(int d, int* max_blocks);
 hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s);
 // e0/e1 (or null) are recorded at the dispatch's start and end; name (or

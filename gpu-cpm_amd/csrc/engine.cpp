@@ -51,7 +51,7 @@ constexpr int kMaxScaleLog2 = 8;
 
 // How one run is launched.  All three forms give identical results.
 struct Plan {
-    enum Mode { kPhase, kGlobal, kLds2d, kSmall } mode = kPhase;
+    enum Mode { kPhase, kGlobal, kLds2d, kSmall, kResident } mode = kPhase;
     int blocks = 0;        // cooperative grid size
     int ltile = 0;         // log2 tile edge (kLds2d)
     size_t lds_bytes = 0;  // dynamic LDS per block (kLds2d)
@@ -301,6 +301,16 @@ struct cpm_engine {
     int nt = 1;  // types present in the lattice at the last push
     int last_kernel = -1;  // per-phase kernel of the last run (gcpm::phase_kernel_name)
     bool small_narrow = false;  // the last single-workgroup run used 16-bit sites
+    // Resident form (2D): tile counters + abandoned-wait flag + guard flag,
+    // per-cell boxes, and the copy of lattice and table a failed chunk is
+    // restored from.
+    unsigned int* d_res_flags = nullptr;
+    size_t res_flags_cap = 0;
+    int32_t* d_bbox = nullptr;
+    void* d_backup_lat = nullptr;
+    int64_t* d_backup_tab = nullptr;
+    int64_t res_fallbacks = 0;      // chunks redone with one launch per phase: the guard
+    int64_t res_wait_fallbacks = 0; // ... or an abandoned neighbour wait
     bool autotune = false;  // cpm_set_autotune
     FormTuner tuner;
 
@@ -354,6 +364,10 @@ struct cpm_engine {
         if (h_lat) (void)hipHostFree(h_lat);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (ev_ready) (void)hipEventDestroy(ev_ready);
+        (void)hipFree(d_res_flags);
+        (void)hipFree(d_bbox);
+        (void)hipFree(d_backup_lat);
+        (void)hipFree(d_backup_tab);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -528,7 +542,7 @@ struct cpm_engine {
     // Launch form for a run with checkerboard edge c: one launch per phase
     // unless cpm_set_launch asks for a persistent form (and tile edge).
     int plan(int c, Plan* p) {
-        if (want_form < Plan::kGlobal || want_form == Plan::kSmall) {
+        if (want_form < Plan::kGlobal || want_form >= Plan::kSmall) {
             p->mode = Plan::kPhase;
             return CPM_OK;
         }
@@ -1292,8 +1306,120 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             plan.blocks = 1;
         }
     }
+    // The resident form (2D, shared roles, c = 1, u = 1): launch form 4.
+    int res_lte = 0;
+    if (h->want_form == Plan::kResident) {
+        const bool ok = h->d == 2 && !k.roles && k.lc == 0 && u == 1 && !a->cell_sync && !k.refstream &&
+                        !k.act && h->L >= 64;
+        if (!ok)
+            return fail(CPM_EINVAL,
+                        "launch form 4 (resident tiles) needs 2D, shared = 1, positions_per_checkerboard = 1, "
+                        "updates_per_checkerboard_switch = 1, L >= 64, and no cell_sync, reference stream or "
+                        "act term");
+        // tiles of 32 or 64 sites: up to 256 of them at L <= 1024, and wide
+        // enough that cells a few tens of sites across keep within two
+        res_lte = std::min(6, std::max(5, h->lgL - 4));
+        int maxb = 0;
+        HIP_TRY(gcpm::resident_max_blocks(h->narrow_bits() != 0, res_lte, &maxb));
+        const int tiles = 1 << (2 * (h->lgL - res_lte));
+        if (tiles > maxb) return fail(CPM_EINVAL, "launch form 4: the tiles do not fit co-resident");
+        plan = Plan{};
+        plan.mode = Plan::kResident;
+        plan.blocks = tiles;
+        plan.ltile = res_lte;
+    }
     h->last_plan = plan;
     const int64_t updates_per_mcs = (int64_t)k.rounds * k.phases_per_round * ((int64_t)1 << (lh * h->d)) * u;
+
+    if (plan.mode == Plan::kResident) {
+        const int bits = h->narrow_bits();
+        gcpm::ResidentArgs ra{};
+        ra.k = k;
+        if (bits) {
+            if (int rc = h->ensure_narrow(bits)) return rc;
+            ra.k.lat16 = h->d_lat16;
+            ra.k.idbits = (uint32_t)bits;
+            ra.k.idmask = (1u << bits) - 1u;
+        } else {
+            if (int rc = h->ensure_wide()) return rc;
+            h->narrow_valid = false;
+        }
+        const int tiles = plan.blocks;
+        const size_t nflag = (size_t)tiles * gcpm::kResFlagStride + 2;  // counters, err, guard
+        const size_t rows = (size_t)h->n + 1;
+        const size_t lat_bytes = h->sites * (bits ? 2 : 4);
+        if (nflag > h->res_flags_cap) {
+            (void)hipFree(h->d_res_flags);
+            h->d_res_flags = nullptr;
+            h->res_flags_cap = 0;
+            HIP_TRY(hipMalloc(&h->d_res_flags, sizeof(unsigned int) * (nflag + 1)));
+            h->res_flags_cap = nflag;
+        }
+        if (!h->d_bbox) HIP_TRY(hipMalloc(&h->d_bbox, sizeof(int32_t) * 4 * rows));
+        if (!h->d_backup_lat) HIP_TRY(hipMalloc(&h->d_backup_lat, h->sites * 4));
+        if (!h->d_backup_tab) HIP_TRY(hipMalloc(&h->d_backup_tab, sizeof(int64_t) * rows));
+        unsigned int* err = h->d_res_flags + (size_t)tiles * gcpm::kResFlagStride;
+        unsigned int* bad = err + 1;
+        ra.lte = res_lte;
+        ra.lnt = h->lgL - res_lte;
+        ra.nmcs = a->inner_iterations;
+        ra.flags = h->d_res_flags;
+        ra.err = err;
+        ra.bbox = h->d_bbox;
+        ra.anchor = h->d_anchor;
+        void* cur_lat = bits ? (void*)h->d_lat16 : (void*)h->d_lat;
+        for (long ch = 0; ch < chunks; ch++) {
+            ra.mcs0 = (uint32_t)(h->mcs + (uint64_t)ch * a->inner_iterations);
+            // the state a failed chunk is redone from
+            HIP_TRY(hipMemcpyAsync(h->d_backup_lat, cur_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(h->d_backup_tab, h->d_tab[0], sizeof(int64_t) * rows, hipMemcpyDeviceToDevice,
+                                   h->stream));
+            HIP_TRY(gcpm::launch_fill_zero(reinterpret_cast<unsigned long long*>(h->d_res_flags), (nflag + 1) / 2,
+                                           h->stream));
+            HIP_TRY(gcpm::launch_resident_guard(true, bits != 0, ra.k, h->d_anchor, h->d_bbox, 1 << res_lte, bad,
+                                                h->stream));
+            hipEvent_t e0, e1;
+            if (int rc = h->next_events(&e0, &e1)) return rc;
+            if (e0) HIP_TRY(hipEventRecord(e0, h->stream));
+            HIP_TRY(gcpm::launch_resident(bits != 0, ra, tiles, h->stream));
+            if (e1) HIP_TRY(hipEventRecord(e1, h->stream));
+            HIP_TRY(gcpm::launch_resident_guard(false, bits != 0, ra.k, h->d_anchor, h->d_bbox, 1 << res_lte, bad,
+                                                h->stream));
+            unsigned int fl[2] = {0, 0};
+            HIP_TRY(hipMemcpyAsync(fl, err, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            if (bits) h->wide_valid = false;
+            if (fl[0] || fl[1]) {
+                // A cell spread over more than two tiles per axis (or a wait
+                // was abandoned): the chunk's result does not stand.  Restore
+                // it and run the rest of the run with one launch per phase.
+                HIP_TRY(hipMemcpyAsync(cur_lat, h->d_backup_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
+                for (int i = 0; i < 2; i++)
+                    HIP_TRY(hipMemcpyAsync(h->d_tab[i], h->d_backup_tab, sizeof(int64_t) * rows,
+                                           hipMemcpyDeviceToDevice, h->stream));
+                HIP_TRY(hipStreamSynchronize(h->stream));
+                if (fl[0])
+                    h->res_wait_fallbacks++;
+                else
+                    h->res_fallbacks++;
+                h->mcs += (uint64_t)ch * a->inner_iterations;
+                cpm_run_args rest = *a;
+                rest.iterations = (int)((chunks - ch) * a->inner_iterations);
+                const int saved = h->want_form;
+                h->want_form = Plan::kPhase;
+                const int rc = cpm_run(h, &rest);
+                h->want_form = saved;
+                return rc;
+            }
+            if (h->timing) {
+                h->t_launches++;
+                h->t_updates += updates_per_mcs * a->inner_iterations;
+            }
+        }
+        h->small_narrow = bits != 0;
+        h->mcs += (uint64_t)total;
+        return CPM_OK;
+    }
 
     if (plan.mode == Plan::kSmall) {
         gcpm::SmallArgs sa{};
@@ -1598,6 +1724,8 @@ const char* cpm_last_kernel(cpm_handle h) {
     switch (h->last_plan.mode) {
         case Plan::kGlobal: return h->d == 3 ? "k_persist_global<3>" : "k_persist_global<2>";
         case Plan::kLds2d: return "k_persist_lds2d";
+        case Plan::kResident:
+            return h->small_narrow ? "k_resident2d<unsigned short>" : "k_resident2d<unsigned int>";
         case Plan::kSmall:
             return h->d == 3 ? (h->small_narrow ? "k_small_lds<3, unsigned short>" : "k_small_lds<3, unsigned int>")
                              : (h->small_narrow ? "k_small_lds<2, unsigned short>" : "k_small_lds<2, unsigned int>");
@@ -1607,7 +1735,7 @@ const char* cpm_last_kernel(cpm_handle h) {
 
 int cpm_set_launch(cpm_handle h, int form, int tile) {
     if (int rc = check_handle(h)) return rc;
-    if (form < -1 || form > 3) return fail(CPM_EINVAL, "form must be -1 (auto), 0, 1, 2 or 3");
+    if (form < -1 || form > 4) return fail(CPM_EINVAL, "form must be -1 (auto), 0, 1, 2, 3 or 4");
     if (tile < 0 || (tile && !is_pow2(tile))) return fail(CPM_EINVAL, "tile must be 0 or a power of two");
     h->want_form = form;
     h->want_tile = tile;
@@ -1622,6 +1750,13 @@ int cpm_set_kernel_options(cpm_handle h, int small, int compact, int full, int a
     if (attempts != -1 && attempts != 1 && attempts != 2 && attempts != 4 && attempts != 8)
         return fail(CPM_EINVAL, "attempts per thread must be -1 (default), 1, 2, 4 or 8");
     h->opts = KernelOpts{small, compact, full, attempts, narrow, box3};
+    return CPM_OK;
+}
+
+int cpm_resident_fallbacks(cpm_handle h, int64_t* guard, int64_t* waits) {
+    if (int rc = check_handle(h)) return rc;
+    if (guard) *guard = h->res_fallbacks;
+    if (waits) *waits = h->res_wait_fallbacks;
     return CPM_OK;
 }
 

@@ -10,6 +10,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -215,12 +216,25 @@ struct LdsLat3 {
     __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.gi; }
 };
 
+// The table deltas of one block's flips in one colour phase (the resident
+// form adds them to the other table buffer in the next phase).  At most two
+// entries per attempt, so kDeferCap covers a tile's attempts of one phase.
+constexpr int kDeferCap = 2048;
+struct DeferList {
+    unsigned int n;
+    uint32_t cell[kDeferCap];
+    int64_t delta[kDeferCap];
+};
+
 // Cell-table buffers of colour phase g (see SweepConsts).
 struct PhaseTables {
     const int64_t* R;   // tab[g & 1]: the phase-start snapshot
     int64_t* W;         // tab[(g + 1) & 1]: becomes the state after phase g
     int64_t* Dc;        // del[g & 1]: deltas of this phase (null: W alone takes them)
     int64_t* Dp;        // del[(g + 1) & 1]: deltas of phase g - 1, folded into W
+    DeferList* defer = nullptr;      // resident form: this block's deltas, also kept here
+    int32_t* bbox = nullptr;         // resident form: per cell min/max x, min/max y reached
+    const int32_t* anchor = nullptr; // (coordinates relative to the cell's anchor, wrapped)
 };
 
 __device__ __forceinline__ PhaseTables tables_of(const SweepConsts& k, int g) {
@@ -636,11 +650,31 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
         atomic_add_i64(T.W + cs, delta);
         if (T.Dc) atomic_add_i64(T.Dc + cs, delta);
+        if (T.defer) {
+            const unsigned int i = atomicAdd(&T.defer->n, 1u);
+            T.defer->cell[i] = cs;
+            T.defer->delta[i] = delta;
+        }
+        if (T.bbox) {  // the gaining cell reaches the target site
+            const int L = 1 << k.lgL, Lm = L - 1;
+            const uint32_t gi = lat.gindex(d.t);
+            const int rx = (((int)(gi & Lm) - T.anchor[3 * cs] + L / 2) & Lm) - L / 2;
+            const int ry = (((int)((gi >> k.lgL) & Lm) - T.anchor[3 * cs + 1] + L / 2) & Lm) - L / 2;
+            atomicMin(T.bbox + 4 * cs, rx);
+            atomicMax(T.bbox + 4 * cs + 1, rx);
+            atomicMin(T.bbox + 4 * cs + 2, ry);
+            atomicMax(T.bbox + 4 * cs + 3, ry);
+        }
     }
     if (ct) {
         const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
         atomic_add_i64(T.W + ct, delta);
         if (T.Dc) atomic_add_i64(T.Dc + ct, delta);
+        if (T.defer) {
+            const unsigned int i = atomicAdd(&T.defer->n, 1u);
+            T.defer->cell[i] = ct;
+            T.defer->delta[i] = delta;
+        }
     }
 }
 
@@ -1652,6 +1686,347 @@ hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, h
         if (narrow) CPM_SMALL(3, uint16_t); else CPM_SMALL(3, uint32_t);
     }
 #undef CPM_SMALL
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The resident form (2D, shared roles, c = 1, u = 1): one block per tile of
+// TE x TE sites, the tile and a one-site halo resident in LDS for a whole
+// inner_iterations chunk, and between colour phases a tile waits only for its
+// eight neighbours (per-tile phase counters; kernel boundaries and grid
+// barriers are gone).
+//
+// Why neighbours suffice.  In phase p a tile's attempts read its tile, the
+// halo (the neighbours' border sites) and the snapshot rows R = tab[p & 1]
+// of the cells they meet, and write the tile, W = tab[(p + 1) & 1] and, for
+// the deltas of phase p - 1, W as well (each tile keeps its own deltas of a
+// phase in an LDS list and adds them to the other buffer in the next phase:
+// W then holds S(p-2) + D(p-1) + D(p) = S(p), with no global fold).  A tile
+// starts phase p only when its neighbours have finished phase p - 1, so
+// neighbouring tiles never run different phases at once.  If every tile that
+// reads or writes a cell's rows is a neighbour of every other such tile,
+// each read of R sees exactly the state after phase p - 1 and no write of
+// phase p + 1 can overtake it.  That holds when every cell's sites over the
+// chunk, widened by one site (the halo), span at most two tiles per axis:
+// the guard.  Gaining flips widen a per-cell box (bbox); a check after the
+// launch tells the host whether the chunk's result stands or must be redone
+// with one launch per phase (from a copy made before it).
+//
+// Same-colour targets are two sites apart, so no attempt of a phase reads a
+// site another attempt of that phase writes: the halo a tile loads at a
+// phase start is exact for that phase even while neighbours write their own
+// targets.  Border sites are written through to global memory with
+// device-coherent stores; every wave waits for its stores and atomics before
+// the tile's counter is raised.
+constexpr int kResThreads = 256;
+
+template <class ST>
+struct TileLat {
+    using Site = ST;
+    ST* s;        // tile + halo in LDS, row stride S
+    ST* g;        // the global lattice
+    int S, TE, x0, y0, lgL;
+    struct Pos {
+        int li;       // LDS index
+        int lx, ly;   // 1 .. TE inside the tile
+        uint32_t gi;  // global site index
+    };
+    __device__ __forceinline__ Pos at(int x, int y, int) const {
+        const int lx = x - x0 + 1, ly = y - y0 + 1;
+        return {ly * S + lx, lx, ly, (uint32_t)x | ((uint32_t)y << lgL)};
+    }
+    __device__ __forceinline__ uint32_t self(const Pos& q) const { return s[q.li]; }
+    __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int) const {
+        return s[q.li + dy * S + dx];
+    }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const {
+        s[q.li] = (ST)v;
+        if (q.lx == 1 || q.ly == 1 || q.lx == TE || q.ly == TE) st<true>(g + q.gi, (ST)v);
+    }
+    __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const { lds_block<2>(*this, q, nv); }
+    __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.gi; }
+};
+
+// Wait until the eight neighbour tiles' counters reach p (lanes 0..7 of wave
+// 0 poll one each, bounded).  Returns false (the whole block) when a wait was
+// abandoned; *err is then set and every other waiter gives up too.
+__device__ __forceinline__ bool resident_wait(const ResidentArgs& a, int tx, int ty, unsigned int p) {
+    __shared__ int abandoned;
+    if (threadIdx.x == 0) abandoned = 0;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x < 4 ? threadIdx.x : threadIdx.x + 1;  // the 3 x 3 block without its centre
+        const int ntm = (1 << a.lnt) - 1;
+        const int nx = (tx + (k % 3) - 1) & ntm, ny = (ty + (k / 3) - 1) & ntm;
+        const unsigned int* f = a.flags + kResFlagStride * (nx | (ny << a.lnt));
+        long polls = 0;
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < p) {
+            if (++polls > kBarrierSpinCap ||
+                __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                abandoned = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    return !abandoned;
+}
+
+template <class ST, class PV>
+__device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ST* tile,
+                                             QueuedAttempt<2, false, ST>* queue, unsigned int* nqueued,
+                                             DeferList* dl) {
+    const SweepConsts& k = a.k;
+    const int TE = 1 << a.lte, S = TE + 2;
+    const int b = (int)blockIdx.x;
+    const int tx = b & ((1 << a.lnt) - 1), ty = b >> a.lnt;
+    const int x0 = tx << a.lte, y0 = ty << a.lte;
+    const int Lm = (1 << k.lgL) - 1;
+    ST* glat = lat_base<ST>(k);
+    const TileLat<ST> lat{tile, glat, S, TE, x0, y0, k.lgL};
+    unsigned int* myflag = a.flags + kResFlagStride * b;
+    // the tile and its halo
+    for (int i = threadIdx.x; i < S * S; i += kResThreads) {
+        const int ly = i / S, lx = i - ly * S;
+        tile[i] = glat[(uint32_t)((x0 + lx - 1) & Lm) | ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
+    }
+    if (threadIdx.x == 0) dl->n = 0;
+    __syncthreads();
+    const int ring = 4 * S - 4;
+    const int half = TE / 2;                 // same-colour sites per tile row
+    const int npairs = half * half / 2;      // x-neighbouring pairs of them (one Philox block each)
+    const unsigned int G = 4u * (unsigned int)a.nmcs;
+    for (unsigned int p = 0; p < G; p++) {
+        const uint32_t mcs = a.mcs0 + p / 4;
+        const int col = (int)(p & 3u);
+        const PhaseTables T{tables_of(k, (int)p).R, tables_of(k, (int)p).W, nullptr, nullptr, dl,
+                            a.bbox, a.anchor};
+        if (p > 0) {
+            if (!resident_wait(a, tx, ty, p)) return;
+            // halo after the neighbours' phase p - 1
+            for (int i = threadIdx.x; i < ring; i += kResThreads) {
+                int lx, ly;
+                if (i < S) { lx = i; ly = 0; }
+                else if (i < 2 * S) { lx = i - S; ly = S - 1; }
+                else if (i < 2 * S + TE) { lx = 0; ly = i - 2 * S + 1; }
+                else { lx = S - 1; ly = i - 2 * S - TE + 1; }
+                tile[ly * S + lx] = ld<true>(glat + ((uint32_t)((x0 + lx - 1) & Lm) |
+                                                     ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)));
+            }
+            // this tile's deltas of phase p - 1 into the other buffer (W of phase p)
+            const unsigned int nd = dl->n;
+            for (unsigned int i = threadIdx.x; i < nd; i += kResThreads) atomic_add_i64(T.W + dl->cell[i], dl->delta[i]);
+            __syncthreads();
+            if (threadIdx.x == 0) dl->n = 0;
+        }
+        if (threadIdx.x == 0) *nqueued = 0;
+        __syncthreads();
+        // draws: each thread takes x-neighbouring pairs of same-colour sites
+        int cx, cy, cz;
+        colour_of<2>(col, cx, cy, cz);
+        constexpr int PPT = 2;  // pairs per thread per round
+        QueuedAttempt<2, false, ST> e[2 * PPT];
+        bool act_[2 * PPT];
+#pragma unroll
+        for (int r = 0; r < PPT; r++) {
+            const int pi = (int)threadIdx.x + r * kResThreads;
+            act_[2 * r] = act_[2 * r + 1] = false;
+            if (pi < npairs) {
+                const int j = pi / (half / 2), jc = pi - j * (half / 2);
+                const int ly = 2 * j + cy + 1;
+                const int bx = x0 + 4 * jc + cx, by = y0 + 2 * j + cy;  // the even site of the pair
+                uint32_t wp[4];
+                philox4x32_10((uint32_t)bx | ((uint32_t)by << k.lgL), mcs, (uint32_t)col, 0u, k.k0, k.k1, wp);
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const uint32_t w1 = h ? wp[3] : wp[1], w2 = h ? wp[0] : wp[2];
+                    int dir = (int)(((uint64_t)w1 * 8ull) >> 32);
+                    dir += dir >= 4;
+                    int ox, oy, oz;
+                    block_offset<2>(dir, ox, oy, oz);
+                    const int li = ly * S + (4 * jc + cx + 2 * h + 1);
+                    const uint32_t vt = tile[li], vs = tile[li + oy * S + ox];
+                    act_[2 * r + h] = vs != vt;  // kernels2d.cu:276
+                    e[2 * r + h].site = (uint32_t)li;
+                    e[2 * r + h].vs = vs;
+                    e[2 * r + h].w2 = w2;
+                    e[2 * r + h].aux = (uint32_t)dir;
+                    e[2 * r + h].set_vt(vt);
+                }
+            }
+        }
+        uint64_t masks[2 * PPT];
+        uint32_t total = 0;
+#pragma unroll
+        for (int r = 0; r < 2 * PPT; r++) {
+            masks[r] = __ballot(act_[r]);
+            total += (uint32_t)__popcll(masks[r]);
+        }
+        if (total != 0) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(nqueued, total);
+            base = __shfl(base, 0);
+#pragma unroll
+            for (int r = 0; r < 2 * PPT; r++) {
+                const uint32_t before = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(masks[r] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)masks[r], 0u));
+                if (act_[r]) queue[base + before] = e[r];
+                base += (uint32_t)__popcll(masks[r]);
+            }
+        }
+        __syncthreads();
+        const int nq = (int)*nqueued;
+        for (int j = threadIdx.x; j < nq; j += kResThreads) {
+            const QueuedAttempt<2, false, ST>& q = queue[j];
+            const int li = (int)q.site, ly = li / S, lx = li - ly * S;
+            Draw<2, TileLat<ST>> d;
+            d.t = typename TileLat<ST>::Pos{li, lx, ly,
+                                            (uint32_t)(x0 + lx - 1) | ((uint32_t)(y0 + ly - 1) << k.lgL)};
+            d.w[0] = d.w[1] = d.w[3] = 0u;
+            d.w[2] = q.w2;
+            lat.block(d.t, d.nv);
+            d.vs = q.vs;
+            d.dir = q.dir();
+            d.tick = mcs;
+            finish_attempt<2, true>(lat, k, P, k.thr, T, CellSync{0, nullptr, 0}, 0u, d);
+        }
+        // every store and atomic of this tile's phase completes, then the
+        // tile's counter says so
+        wait_memory();
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(myflag, p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the last phase's deltas into the lagging buffer, once the neighbours
+    // have finished reading it
+    if (!resident_wait(a, tx, ty, G)) return;
+    int64_t* lag = (G & 1u) ? k.tab[0] : k.tab[1];  // tab[(G + 1) & 1]
+    const unsigned int nd = dl->n;
+    for (unsigned int i = threadIdx.x; i < nd; i += kResThreads) atomic_add_i64(lag + dl->cell[i], dl->delta[i]);
+    // the tile's sites (border sites are already there)
+    for (int i = threadIdx.x; i < TE * TE; i += kResThreads) {
+        const int ly = i >> a.lte, lx = i & (TE - 1);
+        glat[(uint32_t)(x0 + lx) | ((uint32_t)(y0 + ly) << k.lgL)] = tile[(ly + 1) * S + lx + 1];
+    }
+}
+
+template <class ST>
+__global__ __launch_bounds__(kResThreads) void k_resident2d(ResidentArgs a) {
+    extern __shared__ __align__(16) unsigned char res_smem[];
+    __shared__ QueuedAttempt<2, false, ST> queue[2 * 2 * kResThreads];
+    __shared__ DeferList dl;
+    __shared__ unsigned int nqueued;
+    ST* tile = reinterpret_cast<ST*>(res_smem);
+    if (a.k.nt <= kSmallTypes) {
+        SmallParams P;
+        P.load(a.k.prm);
+        resident_run<ST>(a, P, tile, queue, &nqueued, &dl);
+    } else {
+        const LdsParamView P{a.k.prm->w, a.k.nt, a.k.nt * a.k.nt};  // (read from global memory)
+        resident_run<ST>(a, P, tile, queue, &nqueued, &dl);
+    }
+}
+
+// Per-cell boxes of the lattice's sites (relative to the anchors, wrapped):
+// bbox must hold (INT_MAX, INT_MIN, INT_MAX, INT_MIN) per cell on entry.
+template <class ST>
+__global__ __launch_bounds__(256) void k_bbox2d(const ST* __restrict__ lat, int lgL, uint32_t idmask,
+                                                const int32_t* __restrict__ anchor, int32_t* bbox) {
+    const uint32_t nsites = 1u << (2 * lgL);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = i < nsites ? (uint32_t)lat[i] & idmask : 0u;
+    const int L = 1 << lgL, Lm = L - 1;
+    int rx = 0, ry = 0;
+    if (c) {
+        rx = (((int)(i & Lm) - anchor[3 * c] + L / 2) & Lm) - L / 2;
+        ry = (((int)(i >> lgL) - anchor[3 * c + 1] + L / 2) & Lm) - L / 2;
+    }
+    const int lane = (int)__lane_id();
+    uint64_t pending = __ballot(c != 0u);
+    while (pending) {  // lanes grouped by cell, one atomic per group and bound
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lc = (uint32_t)__shfl((int)c, leader);
+        const bool mine = ((pending >> lane) & 1ull) && c == lc;
+        int mnx = mine ? rx : INT_MAX, mxx = mine ? rx : INT_MIN;
+        int mny = mine ? ry : INT_MAX, mxy = mine ? ry : INT_MIN;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mnx = min(mnx, __shfl_xor(mnx, o));
+            mxx = max(mxx, __shfl_xor(mxx, o));
+            mny = min(mny, __shfl_xor(mny, o));
+            mxy = max(mxy, __shfl_xor(mxy, o));
+        }
+        if (lane == leader) {
+            atomicMin(bbox + 4 * lc, mnx);
+            atomicMax(bbox + 4 * lc + 1, mxx);
+            atomicMin(bbox + 4 * lc + 2, mny);
+            atomicMax(bbox + 4 * lc + 3, mxy);
+        }
+        pending &= ~__ballot(mine);
+    }
+}
+
+__global__ void k_bbox_init(int32_t* bbox, int rows) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= rows) return;
+    bbox[4 * c] = INT_MAX;
+    bbox[4 * c + 1] = INT_MIN;
+    bbox[4 * c + 2] = INT_MAX;
+    bbox[4 * c + 3] = INT_MIN;
+}
+
+// The guard: a cell whose box, widened by one site, spans more than two
+// tiles per axis (width + 2 > TE + 1) sets *bad.
+__global__ void k_bbox_check(const int32_t* __restrict__ bbox, int rows, int te, unsigned int* bad) {
+    const int c = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= rows) return;
+    const int32_t* q = bbox + 4 * c;
+    if (q[1] < q[0]) return;  // no site
+    if ((q[1] - q[0] + 1) + 2 > te + 1 || (q[3] - q[2] + 1) + 2 > te + 1) atomicOr(bad, 1u);
+}
+
+size_t resident_lds_bytes(int lte, bool narrow) {
+    const size_t S = ((size_t)1 << lte) + 2;
+    return S * S * (narrow ? 2 : 4);
+}
+
+hipError_t resident_max_blocks(bool narrow, int lte, int* max_blocks) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const void* f = narrow ? (const void*)&k_resident2d<uint16_t> : (const void*)&k_resident2d<uint32_t>;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kResThreads,
+                                                          resident_lds_bytes(lte, narrow))) != hipSuccess)
+        return e;
+    *max_blocks = per_cu * cus;
+    return hipSuccess;
+}
+
+hipError_t launch_resident(bool narrow, const ResidentArgs& a, int blocks, hipStream_t s) {
+    ResidentArgs copy = a;
+    void* args[] = {&copy};
+    const void* f = narrow ? (const void*)&k_resident2d<uint16_t> : (const void*)&k_resident2d<uint32_t>;
+    return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(kResThreads), args,
+                                      (unsigned int)resident_lds_bytes(a.lte, narrow), s);
+}
+
+hipError_t launch_resident_guard(bool begin, bool narrow, const SweepConsts& k, const int32_t* anchor,
+                                 int32_t* bbox, int te, unsigned int* bad, hipStream_t s) {
+    const int rows = k.ncells + 1;
+    if (begin) {
+        hipLaunchKernelGGL(k_bbox_init, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows);
+        const uint32_t nsites = 1u << (2 * k.lgL);
+        if (narrow)
+            hipLaunchKernelGGL(k_bbox2d<uint16_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat16, k.lgL,
+                               k.idmask, anchor, bbox);
+        else
+            hipLaunchKernelGGL(k_bbox2d<uint32_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat, k.lgL,
+                               k.idmask, anchor, bbox);
+    } else {
+        hipLaunchKernelGGL(k_bbox_check, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows, te, bad);
+    }
     return hipGetLastError();
 }
 

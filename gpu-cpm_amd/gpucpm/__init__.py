@@ -317,7 +317,9 @@ class Cpm:
 
     def set_launch(self, form=-1, tile=0):
         """Launch form (-1 auto, 0 per phase, 1 persistent global, 2 persistent
-        LDS tiles) and LDS tile edge (0 auto).  Results never depend on them."""
+        LDS tiles, 3 one workgroup with the lattice in LDS, 4 resident 2D
+        tiles with neighbour-only waits) and LDS tile edge (0 auto).  Results
+        never depend on them."""
         check(lib.cpm_set_launch(self._h, _int(form, "form"), _int(tile, "tile")))
 
     def set_kernel_options(self, small=None, compact=None, full=None, attempts=None,
@@ -330,6 +332,15 @@ class Cpm:
         never depend on them."""
         v = [-1 if x is None else int(x) for x in (small, compact, full, attempts, narrow, box3)]
         check(lib.cpm_set_kernel_options(self._h, *v))
+
+    @property
+    def resident_fallbacks(self):
+        """(guard, waits): chunks of resident-form runs (launch form 4) redone
+        with one launch per phase because a cell spread over more than two
+        tiles per axis, or because a neighbour wait gave up."""
+        g, w = ctypes.c_int64(), ctypes.c_int64()
+        check(lib.cpm_resident_fallbacks(self._h, ctypes.byref(g), ctypes.byref(w)))
+        return g.value, w.value
 
     def last_launch(self):
         """Launch form of the last run: dict(form, kernel, blocks, tile)."""

@@ -209,9 +209,16 @@ int cpm_last_launch(cpm_handle h, int* form, int* blocks, int* tile);
 const char* cpm_last_kernel(cpm_handle h);
 
 /* Extension: choose the launch form (-1 = automatic, else a form number as
- * above) and the LDS tile edge (0 = automatic) for this engine's runs; the
- * autotuner sets these.  Results never depend on them. */
+ * above, 3 = one workgroup with the lattice in LDS, 4 = resident 2D tiles
+ * that wait only on their neighbours between colour phases) and the LDS tile
+ * edge (0 = automatic) for this engine's runs; the autotuner sets these.
+ * Results never depend on them. */
 int cpm_set_launch(cpm_handle h, int form, int tile);
+/* Extension: chunks of resident-form runs (launch form 4) that were redone
+ * with one launch per phase: *guard counts those where a cell spread over
+ * more than two tiles per axis (the form's exactness condition), *waits
+ * those where a neighbour wait gave up (either pointer may be NULL). */
+int cpm_resident_fallbacks(cpm_handle h, int64_t* guard, int64_t* waits);
 /* Extension (no counterpart in the reference, whose kernel variant is chosen
  * by template flags, cpm.cu:13-45): options of the per-phase kernels for
  * this engine's runs, each -1 (default), 0 or 1: small = read at most four

@@ -10,7 +10,7 @@ import oracle as O
 # the engine_options fixture (tests/conftest.py).  None of them changes
 # results.
 ENGINE_OPTIONS = {}
-_FORMS = {"auto": -1, "phase": 0, "global": 1, "lds": 2, "small": 3}
+_FORMS = {"auto": -1, "phase": 0, "global": 1, "lds": 2, "small": 3, "resident": 4}
 
 
 def make_cpm(wl):

@@ -1,0 +1,114 @@
+"""The resident form (launch form 4: 2D tiles resident in LDS for a chunk,
+each waiting only on its eight neighbours between colour phases) against
+the oracle, bit-exact; the guard (a cell spread over more than two tiles
+per axis) sends a chunk back to one launch per phase, still exact."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import assert_tables_equal, make_cpm, run_kwargs, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(*a):
+    print("CASE", *a, flush=True)
+
+
+def _six_types(L, seed):
+    from gpucpm import workloads as W
+    rng = np.random.default_rng(seed)
+    lat = np.zeros((L, L), np.uint32)
+    n = 0
+    for y in range(0, L, 6):
+        for x in range(0, L, 6):
+            n += 1
+            lat[y:y + 4, x:x + 4] = (np.uint32(rng.integers(1, 6)) << np.uint32(24)) | np.uint32(n)
+    cons = [dict(cell_type=t, lambda_area=2, target_area=16) for t in range(1, 6)]
+    cons += [dict(cell_type=t, other_cell_type=0, adhesion=8 + t) for t in range(1, 6)]
+    cons += [dict(cell_type=1, other_cell_type=t, adhesion=3 * t) for t in range(1, 6)]
+    return W.Workload("six_types", L, 2, n, 10, lat, cons)
+
+
+def _run(wl, seed, n, inner, narrow=1):
+    from helpers import ENGINE_OPTIONS
+    ENGINE_OPTIONS.update(mode="resident", narrow=narrow)
+    try:
+        sim = make_cpm(wl)
+    finally:
+        ENGINE_OPTIONS.clear()
+    sim.push_to_gpu()
+    sim.set_seed(seed)
+    sim.run(**run_kwargs(1, 1, n, inner))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    return sim
+
+
+def _cases():
+    from gpucpm import workloads as W
+    return [  # name, workload, nmcs, inner, narrow
+        ("sorting1024", W.sorting2d(seed=401), 40, 20, 1),
+        ("sorting512", W.sorting2d(L=512, grid=40, edge=10, seed=402), 30, 10, 1),
+        ("plain256", W.plain2d(seed=403), 60, 30, 1),
+        ("small-cells256", W.sorting_small_cells(L=256, seed=404), 40, 40, 1),
+        ("wall-chem256", W.chemotactic(W.walled(W.sorting2d(L=256, grid=20, edge=10, seed=405)),
+                                       lam=(0.0, 20.0, 5.0, 0.0)), 30, 15, 1),
+        ("wide512", W.sorting2d(L=512, grid=40, edge=10, seed=406), 20, 20, 0),
+        ("six-types256", _six_types(256, 407), 30, 30, 1),
+    ]
+
+
+@pytest.mark.parametrize("idx", range(7))
+def test_resident_matches_oracle(idx):
+    name, wl, n, inner, narrow = _cases()[idx]
+    _announce("resident", name, wl.L, n, inner, narrow)
+    sim = _run(wl, 500 + idx, n, inner, narrow)
+    launch = sim.last_launch()
+    lat_o, tab_o = run_oracle(wl, 1, 1, 500 + idx, n)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+    guard, waits = sim.resident_fallbacks
+    assert waits == 0
+    # the frame is one cell around the whole lattice: the guard sends it
+    # back; elsewhere the form must hold
+    if name != "wall-chem256":
+        assert guard == 0 and launch["form"] == 4, (guard, launch)
+        assert "k_resident2d" in launch["kernel"]
+
+
+def test_guard_sends_a_spread_cell_back_to_per_phase_launches():
+    """One cell of 70 x 70 sites at 256^2 (tiles of 32): it spans three tiles,
+    so the chunk is restored and run with one launch per phase, exactly."""
+    from gpucpm import workloads as W
+    wl = W.plain2d(seed=408)
+    lat = wl.lattice.copy()
+    lat[100:170, 60:130] = (1 << 24) | 1
+    wl.lattice = lat
+    _announce("guard", wl.L)
+    sim = _run(wl, 509, 20, 10)
+    assert sim.resident_fallbacks[0] >= 1 and sim.resident_fallbacks[1] == 0
+    lat_o, tab_o = run_oracle(wl, 1, 1, 509, 20)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+
+
+def test_resident_split_runs_equal_one_run():
+    from gpucpm import workloads as W
+    wl = W.sorting2d(L=512, grid=40, edge=10, seed=410)
+    _announce("resident split", wl.L)
+    a = _run(wl, 7, 24, 24).get_state().copy()
+    from helpers import ENGINE_OPTIONS
+    ENGINE_OPTIONS.update(mode="resident")
+    try:
+        sim = make_cpm(wl)
+    finally:
+        ENGINE_OPTIONS.clear()
+    sim.push_to_gpu()
+    sim.set_seed(7)
+    for n, inner in ((8, 4), (16, 16)):
+        sim.run(**run_kwargs(1, 1, n, inner))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    np.testing.assert_array_equal(sim.get_state(), a)
