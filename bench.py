@@ -146,6 +146,8 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     launch = sim.last_launch()
     sites = wl.L ** wl.d
     mcs = steps * mcs_per_step
+    log("batch forms on a launch-bound batch")
+    forms_small = _batch_forms_small(kw)
     avg_launch_s = kernel_ms / 1e3 / launches
     bytes_per_launch = updates / launches * BYTES_PER_UPDATE
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -225,6 +227,16 @@ def measure_batched(nrep, mcs, log, autotune=False):
     tb.run(**kw, iterations=200, inner_iterations=200)
     kernel_ms, launches, updates = tb.timing_end()
     kernel = tb.last_kernel
+    log("graph replay")
+    gsims = fresh()
+    gb = gpucpm.Batch(gsims)
+    gb.set_form(2)
+    t0 = time.perf_counter()
+    gb.run(**kw, iterations=mcs, inner_iterations=mcs)  # (the capture is inside the timed run)
+    gb.synchronize()
+    t_graph = time.perf_counter() - t0
+    d_graph = digest(gsims)
+    graph_info = gb.graph_info()
     log("sequential runs")
     sims = fresh()
     t0 = time.perf_counter()
@@ -242,6 +254,8 @@ def measure_batched(nrep, mcs, log, autotune=False):
         s.synchronize()
     t_conc = time.perf_counter() - t0
     d_conc = digest(sims)
+    log("batch forms on a launch-bound batch")
+    forms_small = _batch_forms_small(kw)
     avg_launch_s = kernel_ms / 1e3 / launches
     bytes_per_launch = updates / launches * BYTES_PER_UPDATE
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -259,11 +273,43 @@ def measure_batched(nrep, mcs, log, autotune=False):
                        "note": "each replica's run() waited for before the next"},
         "concurrent_streams": {"value": mcs * sites / t_conc, "ms_total": t_conc * 1e3,
                                "note": "every replica's run() queued on its own stream, one wait"},
+        "graph_replay": {"value": mcs * sites / t_graph, "ms_total": t_graph * 1e3,
+                         "form_used": graph_info[0], "captures": graph_info[1], "replays": graph_info[2],
+                         "note": f"batch form 2: the per-phase launches of {mcs // max(graph_info[2], 1)} MCS "
+                                 f"captured once (inside the timed run) and replayed {graph_info[2]} times"},
+        "forms_small_batch": forms_small,
         "speedup_vs_sequential": t_seq / t_batch,
         "speedup_vs_concurrent_streams": t_conc / t_batch,
-        "bit_identical": d_batch == d_seq == d_conc,
+        "bit_identical": d_batch == d_seq == d_conc == d_graph,
         "cpu_baseline": None,
     }
+
+
+def _batch_forms_small(kw, nrep=16, L=64, mcs=1280):
+    """MCS/s of a launch-bound batch (16 sorting replicas of 64^2) in each
+    batch form: per-phase launches, one persistent launch per run, hip graph
+    replay (captured in an untimed warm-up run of the same shape)."""
+    import gpucpm
+    from gpucpm import workloads
+    wls = workloads.batched_sorting(nrep=nrep, seed=4, sizes=(L,))
+    out = {"workload": f"{nrep} sorting replicas of {L}^2, {mcs} MCS per run"}
+    for name, form in (("per_phase", 0), ("persistent", 1), ("graph_replay", 2)):
+        sims = []
+        for i, wl in enumerate(wls):
+            s = wl.make_cpm()
+            s.push_to_gpu()
+            s.set_seed(9 + i)
+            sims.append(s)
+        b = gpucpm.Batch(sims)
+        b.set_form(form)
+        b.run(**kw, iterations=64, inner_iterations=64)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.run(**kw, iterations=mcs, inner_iterations=mcs)
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"mcs_per_s": mcs / dt, "us_per_mcs": dt / mcs * 1e6}
+    return out
 
 
 def _oracle_energy_fn(lats, wls, layout):

@@ -208,7 +208,8 @@ struct BatchPhaseArgs {
     int cx, cy, cz;         // as PhaseArgs
     int lstride;
     uint32_t phase;         // round * 2^d + colour
-    uint32_t m;             // MCS offset within this run (replica MCS = mcs0 + m)
+    uint32_t m;             // MCS offset within this run (replica MCS = mcs0 + m + *mbase)
+    const uint32_t* mbase;  // graph replay: device MCS offset of the replayed block of MCS (or null)
     int g;                  // phase counter since the last settle (parity)
     int sync_pass;          // as PhaseArgs (0 off; 1/2 cell_sync; 3/4 split; 5/6 both)
     uint32_t sync_tag;      // cell_sync: tag (above every replica's earlier tags)
@@ -239,6 +240,9 @@ hipError_t launch_resident(bool narrow, const ResidentArgs& a, int blocks, hipSt
 // a cell's box (widened by the halo) spans more than two tiles per axis.
 hipError_t launch_resident_guard(bool begin, bool narrow, const SweepConsts& k, const int32_t* anchor,
                                  int32_t* bbox, int te, unsigned int* bad, hipStream_t s);
+// *p = v / *p += v (one thread: nodes of a captured graph, no memset nodes)
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+hipError_t launch_add_u32(uint32_t* p, uint32_t v, hipStream_t s);
 hipError_t batch_max_blocks(int d, int* max_blocks);
 hipError_t launch_batch(int d, const BatchArgs& a, int blocks, hipStream_t s);
 // e0/e1 (or null) are recorded at the dispatch's start and end; name (or

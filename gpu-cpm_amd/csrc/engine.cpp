@@ -62,6 +62,8 @@ constexpr int kMaxThr = 1 << 22;
 // of attempts per phase for which it is chosen automatically.
 constexpr size_t kSmallLdsBudget = 158 * 1024;
 constexpr long kSmallAutoAttempts = 4096;
+// Batch graph replay: the largest number of MCS captured in one graph.
+constexpr long kGraphMcs = 64;
 
 // Per-phase kernel options (cpm_set_kernel_options); -1 keeps the default.
 // Every option computes the same results.
@@ -692,7 +694,7 @@ struct cpm_batch {
     std::vector<hipEvent_t> ready;       // one per replica stream
     unsigned int* d_bars = nullptr;      // 2 words per replica, then the abandoned-wait flag
     double block_scale = 1.0;            // cpm_batch_set_blocks: blocks per unit of work
-    int form = 0;                        // cpm_batch_set_form: 0 per phase, 1 persistent
+    int form = 0;                        // cpm_batch_set_form: -1 choose, 0 per phase, 1 persistent, 2 graph
     gcpm::ReplicaDesc* d_reps = nullptr;
     int* d_rep_of = nullptr;
     int rep_of_cap = 0;
@@ -701,6 +703,13 @@ struct cpm_batch {
     FormTuner tuner;
     KernelOpts opts;        // cpm_batch_set_kernel_options (small, compact, att)
     char last_kernel[96] = "";  // per-phase batch kernel of the last run
+    // Graph replay (form 2): the instantiated graph of G MCS, its shape key
+    // and the device MCS offset its kernels add.
+    hipGraphExec_t gexec = nullptr;
+    uint64_t gkey = 0;
+    uint32_t* d_mbase = nullptr;
+    int64_t graph_captures = 0, graph_replays = 0;
+    int last_form = 0;  // the form the last run used
     // Kernel timing (cpm_batch_timing_begin/end): one event pair per launch.
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -726,6 +735,8 @@ struct cpm_batch {
         (void)hipFree(d_rep_of);
         for (hipEvent_t e : ready) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        (void)hipFree(d_mbase);
         if (done) (void)hipEventDestroy(done);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1871,7 +1882,7 @@ StatsRunner& stats_runner() {
 // is a decide launch and an apply launch (as cpm_run with the act term,
 // shared roles and c = 1), on the uncompacted layout.
 int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::ReplicaDesc>& reps,
-                    long total, bool split) {
+                    long total, bool split, bool graph) {
     const int n = (int)b->sims.size();
     uint32_t tag = 0;  // cell_sync: one tag for all replicas, above each one's earlier tags
     bool pers = false;
@@ -2008,7 +2019,86 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
     int64_t attempts_per_launch = 0;  // sub-blocks of one colour phase over all replicas
     for (int i = 0; i < n; i++) attempts_per_launch += (int64_t)1 << (reps[i].lh * b->d);
     int g = 0;
-    for (long m = 0; m < total; m++) {
+    // Graph replay (batch form 2): the launches of G MCS (G the largest
+    // divisor of the run's MCS up to kGraphMcs) are captured once as a hip
+    // graph, with the MCS offset read from a device counter that a last node
+    // advances by G, and the graph is replayed total / G times (and kept for
+    // later runs of the same shape).  Kernel nodes only: no memset nodes.
+    // Not with cell_sync (its tags grow per launch), the reference stream
+    // (reseeds per chunk), persistence (host updates per chunk), timing or
+    // per-launch tuning.
+    // The graph's phases keep the table parity g (which starts at 0), so the
+    // phases of G MCS must be even in number.
+    const long P = (long)rounds * k0.phases_per_round;
+    long G = 0;
+    for (long gg = std::min<long>(total, kGraphMcs); gg >= 1 && graph; gg--)
+        if (total % gg == 0 && (gg * P) % 2 == 0) {
+            G = gg;
+            break;
+        }
+    const bool use_graph = G > 0 && !tune && !b->timing && !a->cell_sync && !refstream && !pers;
+    if (use_graph) {
+        uint64_t key = (uint64_t)G | (uint64_t)lay0 << 8 | (uint64_t)form << 12 | (uint64_t)narrow << 20 |
+                       (uint64_t)split << 21 | (uint64_t)u << 22 | (uint64_t)rounds << 32 |
+                       (uint64_t)k0.phases_per_round << 44 | (uint64_t)k0.lstride << 56;
+        key = key * 1000003u + (uint64_t)nblocks;
+        key = key * 1000003u + (uint64_t)(uintptr_t)pa.reps;
+        key = key * 1000003u + (uint64_t)(uintptr_t)pa.rep_of;
+        if (!b->gexec || b->gkey != key) {
+            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = hipSuccess;
+            pa.mbase = b->d_mbase;
+            int gl = 0;
+            for (long m = 0; m < G && ce == hipSuccess; m++) {
+                pa.m = (uint32_t)m;
+                for (int r = 0; r < rounds && ce == hipSuccess; r++) {
+                    for (int pc = 0; pc < k0.phases_per_round && ce == hipSuccess; pc++, gl++) {
+                        phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
+                        pa.phase = (uint32_t)(r * k0.phases_per_round + pc);
+                        pa.g = gl & 1;
+                        pa.att0 = 0;
+                        pa.natt = u;
+                        pa.sync_pass = 0;
+                        if (split) {
+                            pa.sync_pass = 3;
+                            ce = gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream);
+                            pa.sync_pass = 4;
+                        }
+                        if (ce == hipSuccess)
+                            ce = gcpm::launch_batch_phase(b->d, pa, nblocks, b->stream, nullptr, nullptr,
+                                                          b->last_kernel, sizeof(b->last_kernel));
+                    }
+                }
+            }
+            if (ce == hipSuccess) ce = gcpm::launch_add_u32(b->d_mbase, (uint32_t)G, b->stream);
+            hipGraph_t gr = nullptr;
+            const hipError_t ee = hipStreamEndCapture(b->stream, &gr);
+            pa.mbase = nullptr;
+            if (ce != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return hip_fail(ce, "capturing the batch graph");
+            }
+            HIP_TRY(ee);
+            const hipError_t ie = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ie != hipSuccess) {
+                b->gexec = nullptr;
+                return hip_fail(ie, "hipGraphInstantiate (batch graph)");
+            }
+            b->gkey = key;
+            b->graph_captures++;
+        }
+        HIP_TRY(gcpm::launch_set_u32(b->d_mbase, 0u, b->stream));
+        for (long rep = 0; rep < total / G; rep++) HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+        b->graph_replays += total / G;
+        if (narrow)
+            for (int i = 0; i < n; i++) b->sims[i]->wide_valid = false;
+        // (an even number of phases per graph: g ends at 0)
+    }
+    b->last_form = use_graph ? 2 : 0;
+    for (long m = 0; m < (use_graph ? 0 : total); m++) {
         pa.m = (uint32_t)m;
         if (narrow)  // (a chunk end may have refreshed them)
             for (int i = 0; i < n; i++) b->sims[i]->wide_valid = false;
@@ -2118,6 +2208,7 @@ int cpm_batch_create(const cpm_handle* sims, int n, cpm_batch_handle* out) {
     // three block layouts at most (the autotuner's attempts-per-thread values)
     if ((err = hipMalloc(&b->d_reps, sizeof(gcpm::ReplicaDesc) * n * 3)) != hipSuccess)
         return bail(err, "hipMalloc(replicas)");
+    if ((err = hipMalloc(&b->d_mbase, sizeof(uint32_t))) != hipSuccess) return bail(err, "hipMalloc(mcs offset)");
     *out = b;
     return CPM_OK;
 }
@@ -2161,12 +2252,68 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         const long w = (std::max(nsub, (long)b->sims[i]->n + 1) + gcpm::kBlock - 1) / gcpm::kBlock;
         need[i] = std::max(1L, (long)std::ceil(w * b->block_scale));
     }
+    // Batch form: 0 per-phase launches, 1 one persistent cooperative launch
+    // per chunk, 2 per-phase launches captured once and replayed as a hip
+    // graph; -1 chooses.  The persistent form needs 2D, shared roles and none
+    // of the reference stream, cell_sync, persistence or the split act path.
+    bool pers_any = false;
+    for (int i = 0; i < n; i++) pers_any = pers_any || reps[i].k.pers;
+    const bool persistent_ok = b->d == 2 && !reps[0].k.refstream && !reps[0].k.roles && !a->cell_sync &&
+                               !pers_any && !split;
+    int form = b->form;
+    FormTuner::Entry* run_tune = nullptr;  // run-level choice among the forms
+    int run_ti = -1;
+    hipEvent_t run_e0 = nullptr, run_e1 = nullptr;
+    const long total_mcs = (a->iterations / a->inner_iterations) * a->inner_iterations;
+    if (form == -1) {
+        // Per-phase launches unless the autotuner finds another form faster
+        // (graph replay measured no faster than direct per-phase launches on
+        // MI355X: equal on 64 replicas of 128^2-1024^2, 20-40% slower on
+        // small launch-bound batches; the persistent form wins on a few tiny
+        // replicas).
+        form = 0;
+        if (b->autotune && !b->timing && total_mcs > 0) {
+            // Whole runs alternate between the candidates (each timed per
+            // MCS on the batch stream) until each has its samples; then the
+            // fastest is kept for the workload.
+            std::vector<int> cands = {0, 2};
+            if (persistent_ok) cands.push_back(1);
+            uint64_t key = (uint64_t)1 << 63 | (uint64_t)b->d | (uint64_t)n << 2 | (uint64_t)reps[0].k.lc << 20 |
+                           (uint64_t)reps[0].k.roles << 24 | (uint64_t)a->updates_per_checkerboard_switch << 25 |
+                           (uint64_t)persistent_ok << 33 | (uint64_t)std::min(total_mcs, 1L << 20) << 34;
+            for (int i = 0; i < n; i++) key = key * 1000003u + (uint64_t)reps[i].lh;
+            key = key * 1000003u + (uint64_t)a->inner_iterations;
+            run_tune = &b->tuner.entry(key, cands);
+            if (run_tune->best >= 0) {
+                form = run_tune->best;
+                run_tune = nullptr;
+            } else if ((run_ti = b->tuner.pick(*run_tune)) < 0) {
+                run_tune = nullptr;
+            } else {
+                form = run_tune->forms[run_ti];
+                HIP_TRY(b->tuner.events(&run_e0, &run_e1));
+            }
+        }
+    }
+    // A run-level sample counts when the run used its candidate without
+    // capturing a graph (a capture's host time would be charged to it).
+    auto finish_run_tune = [&](int used_form, int64_t captures0) -> int {
+        if (!run_tune) return CPM_OK;
+        HIP_TRY(hipEventRecord(run_e1, b->stream));
+        if (used_form == run_tune->forms[run_ti] && b->graph_captures == captures0) {
+            FormTuner::Pending pd{run_tune, run_ti, run_e0, run_e1};
+            pd.scale = (double)total_mcs;
+            b->tuner.pending.push_back(pd);
+        }
+        HIP_TRY(b->tuner.settle());
+        return CPM_OK;
+    };
     if (split) {
         // Act with shared roles and c = 1 in any replica: every replica's
         // phases are decided, then applied (for a replica without the term
         // the decisions are those of one launch: with shared roles and c = 1
         // a target's neighbourhood holds no other target).
-        if (b->form != 0)
+        if (form == 1)
             return fail(CPM_ENOTIMPL,
                         "the act term with shared roles and positions_per_checkerboard = 1 needs the "
                         "per-phase batch form");
@@ -2184,7 +2331,12 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
     }
     const long chunks = a->iterations / a->inner_iterations;
     if (chunks == 0) return CPM_OK;
-    if (b->form == 0) return batch_run_phase(b, a, reps, chunks * a->inner_iterations, split);
+    if (form != 1) {
+        const int64_t captures0 = b->graph_captures;
+        if (run_tune) HIP_TRY(hipEventRecord(run_e0, b->stream));
+        if (int rc = batch_run_phase(b, a, reps, total_mcs, split, form == 2)) return rc;
+        return finish_run_tune(b->last_form, captures0);
+    }
     if (reps[0].k.refstream) return fail(CPM_ENOTIMPL, "the reference stream needs the per-phase batch form");
     if (reps[0].k.roles)
         return fail(CPM_ENOTIMPL, "the persistent batch form implements shared roles only (shared = 1)");
@@ -2251,8 +2403,11 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         HIP_TRY(hipMemcpyAsync(b->d_reps, reps.data(), sizeof(gcpm::ReplicaDesc) * n, hipMemcpyHostToDevice,
                                b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
+        if (run_tune && ch == 0) HIP_TRY(hipEventRecord(run_e0, b->stream));
         HIP_TRY(gcpm::launch_batch(b->d, ba, total, b->stream));
     }
+    b->last_form = 1;
+    if (int rc = finish_run_tune(1, b->graph_captures)) return rc;
     if (int rc = check_barrier(b->d_bars, 2 * n + 1, b->stream)) return rc;
     // Each replica's later work waits for the batch.
     HIP_TRY(hipEventRecord(b->done, b->stream));
@@ -2283,7 +2438,8 @@ int cpm_batch_set_kernel_options(cpm_batch_handle b, int small, int compact, int
 
 int cpm_batch_set_form(cpm_batch_handle b, int form) {
     if (!b) return fail(CPM_EINVAL, "null batch");
-    if (form != 0 && form != 1) return fail(CPM_EINVAL, "form must be 0 (per phase) or 1 (persistent)");
+    if (form < -1 || form > 2)
+        return fail(CPM_EINVAL, "form must be -1 (choose), 0 (per phase), 1 (persistent) or 2 (graph replay)");
     b->form = form;
     return CPM_OK;
 }
@@ -2329,6 +2485,14 @@ int cpm_batch_timing_end(cpm_batch_handle b, double* kernel_ms, int64_t* launche
 }
 
 const char* cpm_batch_last_kernel(cpm_batch_handle b) { return b ? b->last_kernel : ""; }
+
+int cpm_batch_graph_info(cpm_batch_handle b, int* last_form, int64_t* captures, int64_t* replays) {
+    if (!b) return fail(CPM_EINVAL, "null batch");
+    if (last_form) *last_form = b->last_form;
+    if (captures) *captures = b->graph_captures;
+    if (replays) *replays = b->graph_replays;
+    return CPM_OK;
+}
 
 int cpm_batch_info(cpm_batch_handle b, int* replicas, int* blocks) {
     if (!b) return fail(CPM_EINVAL, "null batch");

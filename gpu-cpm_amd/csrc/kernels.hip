@@ -1172,8 +1172,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase_compact(BatchPhaseArgs a
     const int r = a.rep_of[blockIdx.x];
     const ReplicaDesc rd = a.reps[r];  // uniform per block: scalar loads
     const int lb = (int)blockIdx.x - rd.block0;
+    const uint32_t m = a.m + (a.mbase ? *a.mbase : 0u);
     const CompactPhase cp{1 << (D * rd.lh), rd.lh, a.lstride, a.cx, a.cy, a.cz, a.g, a.phase,
-                          rd.mcs0 + a.m, a.sync_pass, rd.claims, a.sync_tag, lb,
+                          rd.mcs0 + m, a.sync_pass, rd.claims, a.sync_tag, lb,
                           lb * (int)blockDim.x + (int)threadIdx.x, rd.nblocks * (int)blockDim.x};
     compact_phase<D, STAGE, false, ATT, ST>(rd.k, cp);
 }
@@ -1236,10 +1237,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
         const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
         const CellSync cy{a.sync_pass, rd.claims, a.sync_tag, rd.decide, gtid};
+        const uint32_t m = a.m + (a.mbase ? *a.mbase : 0u);
         if constexpr (REF)
-            ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, T, cy, a.natt);
+            ref_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + m, T, cy, a.natt);
         else
-            sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + a.m, a.phase, T, cy,
+            sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, rd.mcs0 + m, a.phase, T, cy,
                                          a.att0, a.natt);
     }
     if (a.sync_pass != 1 && a.sync_pass != 3 && a.sync_pass != 5)
@@ -2634,6 +2636,19 @@ hipError_t launch_phase(int d, const PhaseArgs& a, hipStream_t s, hipEvent_t e0,
         launch_phase_st<uint16_t>(id, a, s, e0, e1, lean);
     else
         launch_phase_st<uint32_t>(id, a, s, e0, e1, lean);
+    return hipGetLastError();
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_add_u32(uint32_t* p, uint32_t v) { *p += v; }
+
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(1), 0, s, p, v);
     return hipGetLastError();
 }
 

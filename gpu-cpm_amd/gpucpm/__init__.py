@@ -469,8 +469,17 @@ class Batch:
 
     def set_form(self, form):
         """0: one launch per colour phase for all replicas (default);
-        1: one persistent cooperative launch per chunk (2D)."""
+        1: one persistent cooperative launch per chunk (2D);
+        2: the per-phase launches of up to 64 MCS captured as a hip graph and
+        replayed; -1: choose (form 0, or with set_autotune the fastest of 0,
+        1 and 2 by whole-run timings)."""
         check(lib.cpm_batch_set_form(self._h, _int(form, "form")))
+
+    def graph_info(self):
+        """(form the last run used, graph captures, graph replays)."""
+        f, c, r = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib.cpm_batch_graph_info(self._h, ctypes.byref(f), ctypes.byref(c), ctypes.byref(r)))
+        return f.value, c.value, r.value
 
     def set_kernel_options(self, small=None, compact=None, attempts=None):
         """Cpm.set_kernel_options for batched runs (attempts: 1, 4 or 8)."""

@@ -91,6 +91,8 @@ SIGNATURES = [
     ("cpm_batch_timing_begin", _I, [_H]),
     ("cpm_batch_timing_end", _I, [_H, _P(ctypes.c_double), _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("cpm_batch_last_kernel", ctypes.c_char_p, [_H]),
+    ("cpm_batch_graph_info", _I, [_H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_int64)]),
     ("cpm_energy_stats_many_noalloc", _I, [_P(_H), _I, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("cpm_resident_fallbacks", _I, [_H, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
 ]

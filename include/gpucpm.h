@@ -252,8 +252,18 @@ int cpm_batch_set_blocks(cpm_batch_handle b, double blocks_per_work);
 /* Extension: the autotuner of cpm_set_autotune for batched runs (attempts
  * drawn per thread 1 / 4 / 8, staged or scalar parameters). */
 int cpm_batch_set_autotune(cpm_batch_handle b, int on);
-/* Batch launch form: 0 = one launch per colour phase, 1 = persistent. */
+/* Batch launch form: 0 = one launch per colour phase, 1 = persistent.
+ * Extension: 2 = the per-phase launches of up to 64 MCS captured once as a
+ * hip graph (a device MCS counter advanced inside it) and replayed; the graph
+ * is kept while runs keep the same shape; not with cell_sync, the reference
+ * stream, persistence or kernel timing, where form 0 runs instead.  -1 =
+ * choose: form 0, or with cpm_batch_set_autotune whole runs timed per MCS
+ * across forms 0, 2 and (where it applies) 1, the fastest kept.  Results
+ * never depend on the form. */
 int cpm_batch_set_form(cpm_batch_handle b, int form);
+/* Extension: the form the last batched run used (0, 1 or 2), and the graph
+ * captures and graph replays of this batch so far. */
+int cpm_batch_graph_info(cpm_batch_handle b, int* last_form, int64_t* captures, int64_t* replays);
 /* Extension: cpm_set_kernel_options for batched runs (small, compact and
  * attempts per thread: -1 or 1, 4, 8; default 4 in 2D, 8 in 3D).  Each
  * replica's own narrow option still applies.  Results never depend on them. */
