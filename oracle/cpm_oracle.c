@@ -493,14 +493,31 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                       const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                       uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
-                      int64_t* trace_n);
+                      int64_t* trace_n, int live);
 
 int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                  const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt) {
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
-                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL);
+                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL, 0);
+}
+
+/* Live-table checkerboard: the same schedule, stream and decisions as
+ * orc_sweep_ex, except that each phase's attempts are made one at a time in
+ * a per-phase pseudo-random order (qi -> (a qi + b) mod nq, a odd, from the
+ * phase's own Philox draw), every attempt reading the area and perimeter
+ * tables as the earlier attempts of the phase left them.  It is the other
+ * end of what the reference's concurrent phase can produce (kernels2d.cu:
+ * 292-302 read cellArea / cellPerimeter while other threads' atomicAdds at
+ * :552-569 land): orc_sweep reads every table entry before any write of the
+ * phase, this reads every write that precedes it in one sequential order.
+ * Single-threaded; used only for statistical comparisons. */
+int orc_sweep_live(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                   const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                   uint64_t seed, uint32_t mcs0, int nmcs, int shared, int ppt) {
+    return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs, 1, 0,
+                      shared, ppt, 0, NULL, 0, NULL, 1);
 }
 
 int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
@@ -509,14 +526,14 @@ int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, 
                         int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n) {
     if (shared && ppt != 2 * c) return -9; /* the reference's shared mode with ppt > 2c loops regions per thread */
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
-                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n);
+                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n, 0);
 }
 
 static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                       const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                       uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
-                      int64_t* trace_n) {
+                      int64_t* trace_n, int live) {
     if (d != 2 && d != 3) return -1;
     /* Act with shared roles and c = 1: an attempt's act term reads the
      * source's neighbourhood, two sites from its target, where other targets
@@ -527,6 +544,8 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
      * cell_sync the decide pass also claims and the apply pass checks the
      * claims (kernels.cu:123-141 as a deterministic rule). */
     const int split = shared && act_on(p) && p->act != NULL && c < 2;
+    if (live && (split || cell_sync || refstream)) return -10; /* (not needed for the comparisons) */
+    if (live) nthreads = 1;
     if (L < 2 || (L & (L - 1))) return -2;
     int K, S;
     if (orc_schedule(L, d, c, shared, ppt, &K, &S)) return -3;
@@ -595,8 +614,20 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                 int gy = d == 3 ? (grp / S) % S : grp % S;
                 int gz = d == 3 ? grp % S : 0;
                 uint32_t phase = (uint32_t)(r * ncol * ngrp + pc);
-                memcpy(areaR, area, sizeof(int32_t) * (size_t)(ncells + 1));
-                memcpy(perimR, perim, sizeof(int32_t) * (size_t)(ncells + 1));
+                /* the phase-start snapshot the attempts read (live: the tables themselves) */
+                const int32_t* aR = live ? area : areaR;
+                const int32_t* pR = live ? perim : perimR;
+                uint64_t perm_a = 1, perm_b = 0;
+                if (live) {
+                    uint32_t pw[4];
+                    const uint32_t pctr[4] = {0xA11CE5u, mcs, phase, 0x11FEu};
+                    orc_philox4x32_10(pctr, key, pw);
+                    perm_a = ((uint64_t)pw[0] << 1) | 1u;
+                    perm_b = pw[1];
+                } else {
+                    memcpy(areaR, area, sizeof(int32_t) * (size_t)(ncells + 1));
+                    memcpy(perimR, perim, sizeof(int32_t) * (size_t)(ncells + 1));
+                }
                 /* cell_sync (kernels.cu:123-141, as a deterministic rule), per
                  * attempt index i: pass 0 claims the cells of every accepted
                  * proposal i for the lowest sub-block index; pass 1 recomputes
@@ -609,7 +640,9 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
-                for (int64_t qi = 0; qi < nq; qi++) {
+                for (int64_t qj = 0; qj < nq; qj++) {
+                    /* (nq is a power of two: an odd multiplier permutes) */
+                    const int64_t qi = live ? (int64_t)((perm_a * (uint64_t)qj + perm_b) & (uint64_t)(nq - 1)) : qj;
                     /* sub-block of region (a S + g) at colour offset c */
                     int bx = ((int)(qi % h) * S + gx) * K + cx;
                     int by = ((int)((qi / h) % h) * S + gy) * K + cy;
@@ -720,7 +753,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                             if (vs == vt && !q.act) continue;
                         }
                         int as, at;
-                        int64_t dh = delta_h_core(L, d, &nb, lat, areaR, perimR, &q, tx, ty,
+                        int64_t dh = delta_h_core(L, d, &nb, lat, aR, pR, &q, tx, ty,
                                                   tz, vs, vt, &as, &at);
                         int accept;
                         if (split && pass == 1) {

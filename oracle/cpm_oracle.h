@@ -187,6 +187,15 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt);
 
+/* Live-table checkerboard (statistics only): orc_sweep_ex's schedule,
+ * stream and decisions, each phase's attempts made one at a time in a
+ * per-phase pseudo-random order, every attempt reading the area / perimeter
+ * tables as the earlier attempts of its phase left them.  No cell_sync,
+ * reference stream or split act path (-10). */
+int orc_sweep_live(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                   const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                   uint64_t seed, uint32_t mcs0, int nmcs, int shared, int ppt);
+
 /* orc_sweep_ex with the reference's random stream in place of the counter
  * stream (one MINSTD state per region of R sites, a reference thread; see
  * cpm_oracle.c), one call = one reference launch: the states are seeded from

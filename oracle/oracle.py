@@ -129,6 +129,8 @@ def lib():
         L.orc_sweep_sync.restype = ctypes.c_int
         L.orc_sweep_ex.argtypes = L.orc_sweep.argtypes + [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_sweep_ex.restype = ctypes.c_int
+        L.orc_sweep_live.argtypes = L.orc_sweep.argtypes[:-1] + [ctypes.c_int, ctypes.c_int]
+        L.orc_sweep_live.restype = ctypes.c_int
         L.orc_sweep_refstream.argtypes = L.orc_sweep_ex.argtypes + [
             P(ctypes.c_int64), ctypes.c_int64, P(ctypes.c_int64)]
         L.orc_sweep_refstream.restype = ctypes.c_int
@@ -344,6 +346,23 @@ def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, se
     if rc:
         raise ValueError(f"orc_sweep failed with code {rc}")
     return trace
+
+
+def sweep_live(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
+               mcs0: int, nmcs: int, shared: bool = True, ppt: int = None):
+    """The live-table checkerboard (orc_sweep_live): sweep()'s schedule and
+    stream, each phase's attempts one at a time in a per-phase random order
+    against the live area / perimeter tables."""
+    assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
+    lat32 = lat.view(np.uint32)
+    rc = lib().orc_sweep_live(lat.shape[0], d, tables.ncells, _ptr(lat32, ctypes.c_uint32),
+                              _ptr(tables.area, ctypes.c_int32), _ptr(tables.perim, ctypes.c_int32),
+                              _ptr(tables.anchor, ctypes.c_int32), _ptr(tables.csum, ctypes.c_int64),
+                              ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
+                              int(mcs0) & 0xFFFFFFFF, int(nmcs), int(bool(shared)),
+                              int(ppt if ppt is not None else 2 * c))
+    if rc:
+        raise ValueError(f"orc_sweep_live failed with code {rc}")
 
 
 def ref_threshold_table(temperature: float, scale: int, cap: int = 1 << 16) -> np.ndarray:

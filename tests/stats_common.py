@@ -1,17 +1,25 @@
-"""Observables for statistical comparisons between CPM dynamics."""
+"""Observables and the seed-spread comparison for statistical tests of CPM
+dynamics.
+
+The reference's colour phase is concurrent (kernels2d.cu:292-302 read
+cellArea / cellPerimeter while other threads' atomicAdds at :552-569 land),
+so its outcome depends on the interleaving.  Two executions bound what it
+can produce: every table read before any write of the phase (the engine's
+phase-start snapshot, oracle.sweep) and every read after the writes that
+precede it in one sequential order (oracle.sweep_live).  The tests compare
+the engine with the live-read execution and with the serial random-sequential
+CPM (oracle.serial_sweep), with tolerances taken from the spread over seeds
+alone: a difference passes when Welch's t statistic is within the two-sided
+critical value at ALPHA per comparison.
+"""
 from __future__ import annotations
 
 import numpy as np
+from scipy import stats as sps
 
 import oracle as O
 
-Z_TOL = 4.0  # seed-spread term: Z_TOL * sqrt(se_a^2 + se_b^2)
-# Relative floors for the systematic differences between the parallel
-# checkerboard dynamics (phase-start table snapshot, every site a target
-# once per MCS) and serial random-sequential dynamics.  Measured at 256^2
-# over 8 seeds: energy/site within 0.2%, mean area +0.4%, area std +8%,
-# sorting index -5% (the checkerboard sorts slightly faster per MCS).
-REL_TOL = [0.01, 0.01, 0.15, 0.10]
+ALPHA = 1e-4  # per-comparison false-alarm rate of compare()
 
 
 def sorting_lattice(L=64, seed=0):
@@ -32,31 +40,49 @@ def sorting_lattice(L=64, seed=0):
     return lat, n, p, J
 
 
-def observables(lat, n, p, area):
-    """(energy per site, mean cell area, std of cell areas, sorting index)
-    where the sorting index is the heterotypic share of cell-cell contacts."""
-    st = O.energy_stats(lat, 2, n, p)
-    pr = st["pairs"]
-    H = O.hamiltonian(lat, 2, n, p) / lat.size
+NAMES = ["adhesion/site", "mean area", "area std", "sorting index"]
+AREA_STD = 2
+
+
+def observables(lat, n, p, area, J):
+    """(adhesion energy per site, mean cell area, std of cell areas, sorting
+    index), the sorting index being the heterotypic share of cell-cell
+    contacts.  The adhesion energy is sum_{a<=b} J_ab N_ab over the unordered
+    unlike-cell neighbour pairs N_ab (the area term of H is a function of the
+    area moments, which the second and third entries carry)."""
+    pr = O.energy_stats(lat, 2, n, p)["pairs"]  # upper triangle, a <= b
+    adh = float(np.sum(np.triu(np.asarray(J, float)[:3, :3] * pr[:3, :3]))) / lat.size
     a = area[1:][area[1:] > 0]
     sort = pr[1, 2] / max(1, pr[1, 1] + pr[1, 2] + pr[2, 2])
-    return np.array([H, a.mean(), a.std(), sort])
+    return np.array([adh, a.mean(), a.std(), sort])
 
 
-NAMES = ["energy/site", "mean area", "area std", "sorting index"]
-
-
-def compare(a, b):
-    """a, b: (seeds, times, observables).  An observable agrees when
-    |mean_a - mean_b| <= max(Z_TOL * sqrt(se_a^2 + se_b^2), REL_TOL * |mean_b|).
-    Returns (ok, report rows (time, name, mean_a, mean_b, z, ratio)) where
-    ratio = |difference| / allowed."""
+def compare(a, b, names=None, skip=()):
+    """a, b: (seeds, times, observables).  Welch's t for every time and
+    observable (not in skip); ok when every |t| <= the two-sided critical
+    value at ALPHA with Welch-Satterthwaite degrees of freedom.  Returns
+    (ok, rows (time, name, mean_a, mean_b, t, critical))."""
+    names = names or NAMES
+    na, nb = a.shape[0], b.shape[0]
     ma, mb = a.mean(0), b.mean(0)
-    se = np.sqrt(a.var(0, ddof=1) / a.shape[0] + b.var(0, ddof=1) / b.shape[0])
-    allowed = np.maximum(Z_TOL * se, np.array(REL_TOL)[None, :] * np.abs(mb))
-    diff = np.abs(ma - mb)
-    z = diff / np.maximum(se, 1e-12)
-    ratio = diff / np.maximum(allowed, 1e-12)
-    report = [(t, NAMES[k], ma[t, k], mb[t, k], z[t, k], ratio[t, k])
-              for t in range(a.shape[1]) for k in range(a.shape[2])]
-    return bool(np.all(ratio <= 1.0)), report
+    va, vb = a.var(0, ddof=1) / na, b.var(0, ddof=1) / nb
+    se = np.sqrt(va + vb)
+    t = (ma - mb) / np.maximum(se, 1e-300)
+    df = (va + vb) ** 2 / np.maximum(va ** 2 / (na - 1) + vb ** 2 / (nb - 1), 1e-300)
+    crit = sps.t.ppf(1 - ALPHA / 2, df)
+    rows, ok = [], True
+    for i in range(a.shape[1]):
+        for k in range(a.shape[2]):
+            if k in skip:
+                continue
+            rows.append((i, names[k], ma[i, k], mb[i, k], t[i, k], crit[i, k]))
+            ok = ok and abs(t[i, k]) <= crit[i, k]
+    return ok, rows
+
+
+def welch_t(a, b):
+    """Welch's t of mean(a) - mean(b) and its critical value at ALPHA."""
+    na, nb = len(a), len(b)
+    va, vb = np.var(a, ddof=1) / na, np.var(b, ddof=1) / nb
+    df = (va + vb) ** 2 / (va ** 2 / (na - 1) + vb ** 2 / (nb - 1))
+    return (np.mean(a) - np.mean(b)) / np.sqrt(va + vb), sps.t.ppf(1 - ALPHA / 2, df)
