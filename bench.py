@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--no-grad", action="store_true", help="skip the contrastive fit in the gradient record")
     ap.add_argument("--launch", type=int, default=-1,
                     help="launch form (-1 auto, 0 per phase, 3 one workgroup, 4 resident 2D tiles)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batched: only the batch runs (no sequential, concurrent, graph or form records), "
+                         "for profiling the batched kernel on the C4 shape alone")
     ap.add_argument("--no-configs", action="store_true",
                     help="sorting2d: only the headline record (no C3 / C4 / C5 sub-records)")
     ap.add_argument("--pmc-summary", nargs=6, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
@@ -174,7 +177,7 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     return out
 
 
-def measure_batched(nrep, mcs, log, autotune=False):
+def measure_batched(nrep, mcs, log, autotune=False, batch_only=False):
     """Batched replicas (64 sorting lattices of mixed sizes and parameters)
     in one batch, one launch per colour phase for all of them, against the
     same replicas run one after another (each run waited for) and run
@@ -227,6 +230,16 @@ def measure_batched(nrep, mcs, log, autotune=False):
     tb.run(**kw, iterations=200, inner_iterations=200)
     kernel_ms, launches, updates = tb.timing_end()
     kernel = tb.last_kernel
+    if batch_only:
+        avg_launch_s = kernel_ms / 1e3 / launches
+        bytes_per_launch = updates / launches * BYTES_PER_UPDATE
+        achieved = bytes_per_launch / avg_launch_s / 1e9
+        return {"workload": f"{len(wls)} sorting replicas (batch runs only)",
+                "value": mcs * sites / t_batch, "unit": "site-updates/s", "ms_total": t_batch * 1e3,
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                             "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
+                             "launches": launches}}
     log("graph replay")
     gsims = fresh()
     gb = gpucpm.Batch(gsims)
@@ -388,9 +401,9 @@ def measure_gradient(nrep, L, mcs, log, fit=True):
     H = energy(theta, sims, layout)
     H.backward()
     t_bwd = time.perf_counter() - t0
-    # kernel time: the mean of five more launches over the same lattices
+    # kernel time: the mean of twenty more launches over the same lattices
     kms = []
-    for _ in range(5):
+    for _ in range(20):
         gpucpm.energy_stats_many(sims)
         kms.append(gpucpm.energy_stats_kernel_ms())
     kernel_ms = sum(kms) / len(kms)
@@ -411,7 +424,7 @@ def measure_gradient(nrep, L, mcs, log, fit=True):
            "replicas": nrep, "L": L, "mcs": mcs, "forward_s": t_fwd,
            "forward_site_updates_per_s": nsites * mcs / t_fwd,
            "backward_s": t_bwd, "backward_kernel_ms": kernel_ms,
-           "backward_kernel_ms_runs": kms,
+           "backward_kernel_ms_runs": kms, "backward_kernel_ms_median": sorted(kms)[len(kms) // 2],
            "backward_sites_per_s": nsites / (kernel_ms / 1e3),
            "roofline": {"bound": "hbm", "kernel": "k_stats_batch<2>", "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -592,7 +605,7 @@ def main():
     from gpucpm import workloads
 
     if a.workload == "batched":
-        r = measure_batched(a.replicas, a.steps * a.mcs_per_step, log, a.autotune)
+        r = measure_batched(a.replicas, a.steps * a.mcs_per_step, log, a.autotune, a.batch_only)
         line = {"metric": "lattice site-updates/sec (batched replicas)", "n_gpus": 1, "steps": 1,
                 "warmup": 1, "ms_per_step": r["ms_total"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",

@@ -298,11 +298,12 @@ struct StatsArgs {
 hipError_t stats_scratch(unsigned long long** out);
 // n words at p := 0 (a fill kernel, not a memset node).
 hipError_t launch_fill_zero(unsigned long long* p, size_t n, hipStream_t s);
-hipError_t launch_stats_args(int d, bool narrow, const StatsArgs& a, int blocks, hipStream_t s);
+// t3: every engine of the launch is 2D, 16-bit, L >= 8, at most three types
+hipError_t launch_stats_args(int d, bool narrow, bool t3, const StatsArgs& a, int blocks, hipStream_t s);
 int stats_pair_blocks(int d, int lgL);
 int stats_cell_blocks(int ncells);
 // every engine of one launch has the same dimension and site width
-hipError_t launch_stats_batch(int d, bool narrow, const StatsDesc* descs, const int* eng_of, int blocks,
+hipError_t launch_stats_batch(int d, bool narrow, bool t3, const StatsDesc* descs, const int* eng_of, int blocks,
                               hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 
 constexpr int kBlock = 256;  // threads per workgroup of the sweep kernels

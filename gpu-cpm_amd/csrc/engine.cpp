@@ -2591,8 +2591,10 @@ int cpm_energy_stats_many(const cpm_handle* sims, int n, int64_t* pairs, int64_t
                                hipMemcpyHostToDevice, S.stream));
         HIP_TRY(hipMemcpyAsync(S.d_map, eng_of.data(), sizeof(int) * eng_of.size(), hipMemcpyHostToDevice,
                                S.stream));
-        HIP_TRY(gcpm::launch_stats_batch(d, narrow_grp, S.d_desc, S.d_map, (int)eng_of.size(), S.stream, S.e0,
-                                          S.e1));
+        bool t3 = narrow_grp && d == 2;  // the three-type chunk form applies to every engine
+        for (const gcpm::StatsDesc& e : desc) t3 = t3 && e.nt <= 3 && e.lgL >= 3;
+        HIP_TRY(gcpm::launch_stats_batch(d, narrow_grp, t3, S.d_desc, S.d_map, (int)eng_of.size(), S.stream,
+                                          S.e0, S.e1));
         HIP_TRY(hipMemcpyAsync(S.h_out, S.d_out, sizeof(unsigned long long) * nw * (size_t)n,
                                hipMemcpyDeviceToHost, S.stream));
         HIP_TRY(hipStreamSynchronize(S.stream));  // (desc and eng_of are host temporaries)
@@ -2658,7 +2660,9 @@ int cpm_energy_stats_many_noalloc(const cpm_handle* sims, int n, int64_t* pairs,
                 a.n++;
             }
             if (a.n > 0 && (a.n == gcpm::kStatsArgEngines || i == n)) {
-                HIP_TRY(gcpm::launch_stats_args(d, narrow_grp, a, blocks, s));
+                bool t3 = narrow_grp && d == 2;  // (as in cpm_energy_stats_many)
+                for (int j = 0; j < a.n; j++) t3 = t3 && a.d[j].nt <= 3 && a.d[j].lgL >= 3;
+                HIP_TRY(gcpm::launch_stats_args(d, narrow_grp, t3, a, blocks, s));
                 a.n = 0;
                 blocks = 0;
             }

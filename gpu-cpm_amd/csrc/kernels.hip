@@ -2195,10 +2195,10 @@ __global__ void k_settle(int64_t* tab, int64_t* del, int ncells) {
 // from the table made at push (ctype); per-type moments n, sum A, sum A^2,
 // sum P, sum P^2 over live cells are summed across the wave, then added to
 // the engine's counters (four types or fewer; else per-cell LDS adds).
-// sites per thread in the pair blocks (their loads are all in flight at
-// once: 16 x 5 values in 2D, 2 x 14 in 3D)
+// sites per thread in the pair blocks (in groups whose loads are all in
+// flight at once: 16 x 5 values in 2D, 2 x 14 in 3D)
 template <int D>
-constexpr int stats_sites() { return D == 3 ? 2 : 16; }
+constexpr int stats_sites() { return D == 3 ? 2 : 32; }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
 #pragma unroll
@@ -2237,58 +2237,166 @@ __device__ __forceinline__ void stats_fold_four(const uint32_t acc[4], unsigned 
     }
 }
 
-// The pair counts of the four-type form for 2D 16-bit lattices with L >= 8,
-// read as chunks of 8 consecutive sites of a row: one 16-byte load of the
-// row, one of the row below, and three single sites at the chunk's ends
-// give every positive-half neighbour ((+1, 0), (-1, +1), (0, +1), (+1, +1)).
-// A thread takes two chunks (the 16 sites of stats_sites<2>()).
-__device__ __forceinline__ void stats_pairs_2d16(const StatsDesc& e, int lb, unsigned int* hist) {
-    const int lgL = e.lgL, Lm = (1 << lgL) - 1;
-    const uint32_t nchunks = 1u << (2 * lgL - 3);
-    const uint16_t* __restrict__ lat = static_cast<const uint16_t*>(e.lat);
-    uint32_t acc[4] = {0u, 0u, 0u, 0u};
-    uint16_t r0[2][9], r1[2][10];  // row y: x0 .. x0 + 8; row y + 1: x0 - 1 .. x0 + 8
-    bool ok[2];
+// Sum of x over the wave, valid in every lane: DPP butterflies within each
+// row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror leave every lane of a row holding the row's sum), then the
+// four row sums read from lanes 15, 31, 47 and 63.
+__device__ __forceinline__ uint32_t wave_sum_dpp(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 15) + (uint32_t)__builtin_amdgcn_readlane((int)x, 31) +
+           (uint32_t)__builtin_amdgcn_readlane((int)x, 47) + (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// One chunk of 8 consecutive sites of a 2D 16-bit lattice (L >= 8) with
+// every positive-half neighbour ((+1, 0), (-1, +1), (0, +1), (+1, +1)): one
+// 16-byte load of the row, one of the row below, and three single sites at
+// the chunk's ends.  Chunk c starts at site 8 c.
+struct Chunk2d16 {
+    uint4 a, b;          // row y: x0 .. x0 + 7; row y + 1: x0 .. x0 + 7
+    uint32_t a8, bm, b8;  // row y: x0 + 8; row y + 1: x0 - 1, x0 + 8
+};
+
+__device__ __forceinline__ Chunk2d16 load_chunk_2d16(const uint16_t* __restrict__ lat, int lgL, uint32_t c) {
+    const int Lm = (1 << lgL) - 1;
+    const int x0 = (int)((c << 3) & (uint32_t)Lm), y = (int)((c << 3) >> lgL);
+    const int y1 = (y + 1) & Lm;
+    Chunk2d16 k;
+    k.a = *reinterpret_cast<const uint4*>(lat + ((uint32_t)y << lgL) + x0);
+    k.b = *reinterpret_cast<const uint4*>(lat + ((uint32_t)y1 << lgL) + x0);
+    k.a8 = lat[((uint32_t)y << lgL) + ((x0 + 8) & Lm)];
+    k.bm = lat[((uint32_t)y1 << lgL) + ((x0 - 1) & Lm)];
+    k.b8 = lat[((uint32_t)y1 << lgL) + ((x0 + 8) & Lm)];
+    return k;
+}
+
+// Unpack a chunk: r0[0..8] = row y, x0 .. x0 + 8; r1[0..9] = row y + 1,
+// x0 - 1 .. x0 + 8.
+__device__ __forceinline__ void unpack_chunk_2d16(const Chunk2d16& k, uint32_t r0[9], uint32_t r1[10]) {
+    const uint32_t av[4] = {k.a.x, k.a.y, k.a.z, k.a.w}, bv[4] = {k.b.x, k.b.y, k.b.z, k.b.w};
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const uint32_t c = ((uint32_t)lb * 2u + (uint32_t)h) * kBlock + threadIdx.x;
-        ok[h] = c < nchunks;
-        const uint32_t cc = ok[h] ? c : 0u;
-        const int x0 = (int)((cc << 3) & (uint32_t)Lm), y = (int)((cc << 3) >> lgL);
-        const int y1 = (y + 1) & Lm;
-        const uint4 a = *reinterpret_cast<const uint4*>(lat + ((uint32_t)y << lgL) + x0);
-        const uint4 b = *reinterpret_cast<const uint4*>(lat + ((uint32_t)y1 << lgL) + x0);
-        const uint32_t av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            r0[h][2 * j] = (uint16_t)av[j];
-            r0[h][2 * j + 1] = (uint16_t)(av[j] >> 16);
-            r1[h][2 * j + 1] = (uint16_t)bv[j];
-            r1[h][2 * j + 2] = (uint16_t)(bv[j] >> 16);
-        }
-        r0[h][8] = lat[((uint32_t)y << lgL) + ((x0 + 8) & Lm)];
-        r1[h][0] = lat[((uint32_t)y1 << lgL) + ((x0 - 1) & Lm)];
-        r1[h][9] = lat[((uint32_t)y1 << lgL) + ((x0 + 8) & Lm)];
+    for (int j = 0; j < 4; j++) {
+        r0[2 * j] = av[j] & 0xFFFFu;
+        r0[2 * j + 1] = av[j] >> 16;
+        r1[2 * j + 1] = bv[j] & 0xFFFFu;
+        r1[2 * j + 2] = bv[j] >> 16;
     }
+    r0[8] = k.a8;
+    r1[0] = k.bm;
+    r1[9] = k.b8;
+}
+
+// The pair counts of the three-type form (every site's type 0, 1 or 2) for
+// 2D 16-bit lattices with L >= 8: a thread takes stats_sites<2>() / 8
+// chunks.  Per site, hh = sum over its unlike positive-half neighbours of
+// 1 << 7 tn (three 7-bit slots, at most 4 each); the chunk's sum of
+// hh << 21 t keeps the nine (t, tn) counts in 7-bit fields f = 3 t + tn of
+// one 64-bit word (at most 32 each per chunk).  Two chunks are added (at
+// most 64), then split into the even and odd fields, which have 14 bits of
+// room apart, and accumulated.  At the end the fields are widened to 16-bit
+// halves of five words, summed across the wave, and one lane adds the six
+// unordered type-pair counts (M[a][b] + M[b][a], M[a][a] on the diagonal)
+// to the block's LDS counters.
+__device__ __forceinline__ void stats_pairs_2d16_t3(const StatsDesc& e, int lb, unsigned int* hist) {
+    constexpr int NCH = stats_sites<2>() / 8;
+    static_assert(NCH % 2 == 0, "chunks are added in pairs");
+    constexpr uint64_t kEven = 0x7Full | 0x7Full << 14 | 0x7Full << 28 | 0x7Full << 42 | 0x7Full << 56;
+    const int lgL = e.lgL;
+    const uint32_t nchunks = 1u << (2 * lgL - 3);
+    const uint32_t idb = e.idbits;
+    const uint16_t* __restrict__ lat = static_cast<const uint16_t*>(e.lat);
+    uint64_t E = 0, O = 0, two = 0;
+    auto chunk_of = [&](int h) { return ((uint32_t)lb * NCH + (uint32_t)h) * kBlock + threadIdx.x; };
+    Chunk2d16 k = load_chunk_2d16(lat, lgL, chunk_of(0) < nchunks ? chunk_of(0) : 0u);
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < NCH; h++) {
+        const bool ok = chunk_of(h) < nchunks;
+        Chunk2d16 kn = k;
+        if (h + 1 < NCH) kn = load_chunk_2d16(lat, lgL, chunk_of(h + 1) < nchunks ? chunk_of(h + 1) : 0u);
+        uint32_t r0[9], r1[10];
+        unpack_chunk_2d16(k, r0, r1);
+        uint32_t o0[9], o1[10];  // 1 << 7 t of each neighbour
+#pragma unroll
+        for (int j = 1; j < 9; j++) o0[j] = 1u << ((r0[j] >> idb) * 7u);
+#pragma unroll
+        for (int j = 0; j < 10; j++) o1[j] = 1u << ((r1[j] >> idb) * 7u);
+        uint64_t acc = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const uint32_t v = r0[h][j];
-            const uint32_t nb[4] = {r0[h][j + 1], r1[h][j], r1[h][j + 1], r1[h][j + 2]};
+            const uint32_t v = r0[j];
+            const uint32_t hh = (r0[j + 1] != v ? o0[j + 1] : 0u) + (r1[j] != v ? o1[j] : 0u) +
+                                (r1[j + 1] != v ? o1[j + 1] : 0u) + (r1[j + 2] != v ? o1[j + 2] : 0u);
+            acc += (uint64_t)hh << ((v >> idb) * 21u);
+        }
+        two += ok ? acc : 0ull;
+        if (h & 1) {
+            E += two & kEven;
+            O += (two >> 7) & kEven;
+            two = 0;
+        }
+        k = kn;
+    }
+    // fields f = 3 t + tn: even f = 2 i in E at 14 i, odd f = 2 i + 1 in O at 14 i
+    auto fe = [&](int i) { return (uint32_t)(E >> (14 * i)) & 0x3FFFu; };
+    auto fo = [&](int i) { return (uint32_t)(O >> (14 * i)) & 0x3FFFu; };
+    uint32_t w[5] = {fe(0) | fo(0) << 16, fe(1) | fo(1) << 16, fe(2) | fo(2) << 16, fe(3) | fo(3) << 16, fe(4)};
+#pragma unroll
+    for (int q = 0; q < 5; q++) w[q] = wave_sum_dpp(w[q]);
+    if (__lane_id() == 0) {
+        auto f = [&](int i) { return (w[i >> 1] >> (16 * (i & 1))) & 0xFFFFu; };
+        const uint32_t c00 = f(0), c01 = f(1) + f(3), c02 = f(2) + f(6), c11 = f(4), c12 = f(5) + f(7),
+                       c22 = f(8);
+        if (c00) atomicAdd(&hist[0 * kNTypes + 0], c00);
+        if (c01) atomicAdd(&hist[0 * kNTypes + 1], c01);
+        if (c02) atomicAdd(&hist[0 * kNTypes + 2], c02);
+        if (c11) atomicAdd(&hist[1 * kNTypes + 1], c11);
+        if (c12) atomicAdd(&hist[1 * kNTypes + 2], c12);
+        if (c22) atomicAdd(&hist[2 * kNTypes + 2], c22);
+    }
+}
+
+// The pair counts of the four-type form for 2D 16-bit lattices with L >= 8,
+// read as chunks (Chunk2d16); a thread takes stats_sites<2>() / 8 chunks,
+// the next chunk's loads issued before the current one is counted.
+__device__ __forceinline__ void stats_pairs_2d16(const StatsDesc& e, int lb, unsigned int* hist) {
+    constexpr int NCH = stats_sites<2>() / 8;
+    const int lgL = e.lgL;
+    const uint32_t nchunks = 1u << (2 * lgL - 3);
+    const uint32_t idb = e.idbits;
+    const uint16_t* __restrict__ lat = static_cast<const uint16_t*>(e.lat);
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};  // bytes: at most 4 x 32 per lane
+    auto chunk_of = [&](int h) { return ((uint32_t)lb * NCH + (uint32_t)h) * kBlock + threadIdx.x; };
+    Chunk2d16 k = load_chunk_2d16(lat, lgL, chunk_of(0) < nchunks ? chunk_of(0) : 0u);
+#pragma unroll
+    for (int h = 0; h < NCH; h++) {
+        const bool ok = chunk_of(h) < nchunks;
+        Chunk2d16 kn = k;
+        if (h + 1 < NCH) kn = load_chunk_2d16(lat, lgL, chunk_of(h + 1) < nchunks ? chunk_of(h + 1) : 0u);
+        uint32_t r0[9], r1[10];
+        unpack_chunk_2d16(k, r0, r1);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t v = r0[j];
+            const uint32_t nb[4] = {r0[j + 1], r1[j], r1[j + 1], r1[j + 2]};
             uint32_t hh = 0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) hh += nb[q] != v ? 1u << ((nb[q] >> e.idbits) << 3) : 0u;
-            hh = ok[h] ? hh : 0u;
-            const uint32_t t = v >> e.idbits;
+            for (int q = 0; q < 4; q++) hh += nb[q] != v ? 1u << ((nb[q] >> idb) << 3) : 0u;
+            hh = ok ? hh : 0u;
+            const uint32_t t = v >> idb;
 #pragma unroll
             for (uint32_t ty = 0; ty < 4; ty++) acc[ty] += t == ty ? hh : 0u;
         }
+        k = kn;
     }
     stats_fold_four(acc, hist);
 }
 
-template <int D, class ST>
+// T3: every engine of the launch is 2D with 16-bit sites, L >= 8 and at
+// most three types (the three-type chunk form, a kernel of its own so that
+// its register budget is not the other forms').
+template <int D, class ST, bool T3>
 __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
     __shared__ unsigned int hist[kNTypes * kNTypes];
     constexpr int NB = D == 3 ? 27 : 9;
@@ -2299,6 +2407,14 @@ __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
     const bool four = e.nt <= 4;
     for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x) hist[i] = 0;
     __syncthreads();
+    if constexpr (T3) {
+        static_assert(D == 2 && sizeof(ST) == 2, "the three-type chunk form is 2D, 16-bit");
+        stats_pairs_2d16_t3(e, lb, hist);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x)
+            if (hist[i]) atomicAdd(&e.out[i], (unsigned long long)hist[i]);
+        return;
+    }
     if constexpr (D == 2 && sizeof(ST) == 2) {
         if (four && lgL >= 3) {  // (uniform per block)
             stats_pairs_2d16(e, lb, hist);
@@ -2308,14 +2424,17 @@ __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
             return;
         }
     }
-    const uint32_t base = (uint32_t)lb * (kBlock * kStatsSites);
     const ST* __restrict__ lat = static_cast<const ST*>(e.lat);
     auto site = [&](uint32_t j) -> uint32_t { return (uint32_t)lat[j]; };
-    // every load of the thread's sites and their positive-half neighbours is
+    constexpr int G = kStatsSites < 8 ? kStatsSites : 8;  // sites per group
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};  // bytes: at most 13 x 2 (3D) or 4 x 32 (2D) per lane
+    for (int g0 = 0; g0 < kStatsSites; g0 += G) {
+    const uint32_t base = (uint32_t)lb * (kBlock * kStatsSites) + (uint32_t)g0 * kBlock;
+    // every load of a group's sites and their positive-half neighbours is
     // issued before any comparison (one memory round trip, not one per site)
-    uint32_t vs[kStatsSites], vn_[kStatsSites][NH];
+    uint32_t vs[G], vn_[G][NH];
 #pragma unroll
-    for (int k = 0; k < kStatsSites; k++) {
+    for (int k = 0; k < G; k++) {
         const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
         const uint32_t ii = i < nsites ? i : 0u;
         const int x = (int)(ii & Lm), y = (int)((ii >> lgL) & Lm), z = D == 3 ? (int)(ii >> (2 * lgL)) : 0;
@@ -2328,9 +2447,8 @@ __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
         }
     }
     if (four) {  // (uniform per block)
-        uint32_t acc[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-        for (int k = 0; k < kStatsSites; k++) {
+        for (int k = 0; k < G; k++) {
             const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
             const uint32_t v = vs[k];
             uint32_t h = 0;
@@ -2344,10 +2462,9 @@ __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
 #pragma unroll
             for (uint32_t ty = 0; ty < 4; ty++) acc[ty] += t == ty ? h : 0u;
         }
-        stats_fold_four(acc, hist);
     } else {
 #pragma unroll
-        for (int k = 0; k < kStatsSites; k++) {
+        for (int k = 0; k < G; k++) {
             const uint32_t i = base + (uint32_t)k * kBlock + threadIdx.x;
             if (i >= nsites) continue;
             const uint32_t v = vs[k];
@@ -2362,44 +2479,67 @@ __device__ __forceinline__ void stats_pairs_block(const StatsDesc& e, int lb) {
             }
         }
     }
+    }
+    if (four) stats_fold_four(acc, hist);
     __syncthreads();
     for (int i = threadIdx.x; i < kNTypes * kNTypes; i += blockDim.x)
         if (hist[i]) atomicAdd(&e.out[i], (unsigned long long)hist[i]);
 }
+
+// cells per thread in the cell blocks (summed in registers before the wave
+// sums)
+constexpr int kStatsCells = 8;
 
 __device__ __forceinline__ void stats_cells_block(const StatsDesc& e, int lb) {
     __shared__ unsigned long long acc[kNTypes * 5];
     const bool four = e.nt <= 4;
     for (int i = threadIdx.x; i < kNTypes * 5; i += blockDim.x) acc[i] = 0;
     __syncthreads();
-    const int c = 1 + lb * kBlock + (int)threadIdx.x;
-    uint64_t A = 0, P = 0;
-    int t = 0;
-    if (c <= e.ncells) {
-        const int64_t row = e.tab[c];
-        A = (uint32_t)row;
-        P = (uint64_t)(row >> 32);
-        t = e.ctype[c];
-    }
     if (four) {
+        // the thread's cells (area, perimeter and type; area 0 for none)
+        uint32_t A[kStatsCells], P[kStatsCells];
+        int T[kStatsCells];
 #pragma unroll
-        for (int ty = 0; ty < 4; ty++) {
-            const bool mine = A != 0 && t == ty;
-            const uint64_t v[5] = {mine ? 1ull : 0ull, mine ? A : 0ull, mine ? A * A : 0ull,
-                                   mine ? P : 0ull, mine ? P * P : 0ull};
+        for (int k = 0; k < kStatsCells; k++) {
+            const int c = 1 + (lb * kStatsCells + k) * kBlock + (int)threadIdx.x;
+            const bool in = c <= e.ncells;
+            const int64_t row = in ? e.tab[c] : 0;
+            A[k] = (uint32_t)row;
+            P[k] = (uint32_t)(row >> 32);
+            T[k] = in ? e.ctype[c] : 0;
+        }
+        for (int ty = 0; ty < e.nt; ty++) {  // (uniform: no cell has another type)
+            uint64_t v[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < kStatsCells; k++) {
+                const bool mine = A[k] != 0 && T[k] == ty;
+                const uint64_t a = mine ? A[k] : 0u, p = mine ? P[k] : 0u;
+                v[0] += mine ? 1u : 0u;
+                v[1] += a;
+                v[2] += a * a;
+                v[3] += p;
+                v[4] += p * p;
+            }
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 const uint64_t sum = wave_sum_u64(v[m]);
                 if (__lane_id() == 0 && sum) atomicAdd(&acc[5 * ty + m], (unsigned long long)sum);
             }
         }
-    } else if (A) {
-        unsigned long long* m = acc + 5 * t;
-        atomicAdd(&m[0], 1ull);
-        atomicAdd(&m[1], (unsigned long long)A);
-        atomicAdd(&m[2], (unsigned long long)(A * A));
-        atomicAdd(&m[3], (unsigned long long)P);
-        atomicAdd(&m[4], (unsigned long long)(P * P));
+    } else {
+        for (int k = 0; k < kStatsCells; k++) {
+            const int c = 1 + (lb * kStatsCells + k) * kBlock + (int)threadIdx.x;
+            if (c > e.ncells) break;
+            const int64_t row = e.tab[c];
+            const uint64_t A = (uint32_t)row, P = (uint64_t)(row >> 32);
+            if (!A) continue;
+            unsigned long long* m = acc + 5 * e.ctype[c];
+            atomicAdd(&m[0], 1ull);
+            atomicAdd(&m[1], (unsigned long long)A);
+            atomicAdd(&m[2], (unsigned long long)(A * A));
+            atomicAdd(&m[3], (unsigned long long)P);
+            atomicAdd(&m[4], (unsigned long long)(P * P));
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kNTypes * 5; i += blockDim.x)
@@ -2407,14 +2547,14 @@ __device__ __forceinline__ void stats_cells_block(const StatsDesc& e, int lb) {
 }
 
 // ST: the site type of every engine of the launch (16-bit or 32-bit sites)
-template <int D, class ST>
+template <int D, class ST, bool T3>
 __global__ __launch_bounds__(kBlock) void k_stats_batch(const StatsDesc* __restrict__ descs,
                                                         const int* __restrict__ eng_of) {
     const int r = eng_of[blockIdx.x];
     const StatsDesc e = descs[r];  // uniform per block: scalar loads
     const int b = (int)blockIdx.x;
     if (b < e.cell0)
-        stats_pairs_block<D, ST>(e, b - e.pair0);
+        stats_pairs_block<D, ST, T3>(e, b - e.pair0);
     else
         stats_cells_block(e, b - e.cell0);
 }
@@ -2423,14 +2563,14 @@ __global__ __launch_bounds__(kBlock) void k_stats_batch(const StatsDesc* __restr
 // up to kStatsArgEngines engines per launch, their descriptors passed by
 // value as the kernel argument; block b serves the engine whose block range
 // [pair0, end) holds it.
-template <int D, class ST>
+template <int D, class ST, bool T3>
 __global__ __launch_bounds__(kBlock) void k_stats_args(StatsArgs a) {
     const int b = (int)blockIdx.x;
     int r = 0;
     while (r + 1 < a.n && b >= a.end[r]) r++;  // uniform per block
     const StatsDesc e = a.d[r];
     if (b < e.cell0)
-        stats_pairs_block<D, ST>(e, b - e.pair0);
+        stats_pairs_block<D, ST, T3>(e, b - e.pair0);
     else
         stats_cells_block(e, b - e.cell0);
 }
@@ -2471,12 +2611,14 @@ hipError_t launch_fill_zero(unsigned long long* p, size_t n, hipStream_t s) {
 
 static_assert(sizeof(StatsArgs) <= 4096, "the descriptors are passed as one kernel argument");
 
-hipError_t launch_stats_args(int d, bool narrow, const StatsArgs& a, int blocks, hipStream_t s) {
-#define CPM_STATS(DD, ST) hipLaunchKernelGGL((k_stats_args<DD, ST>), dim3(blocks), dim3(kBlock), 0, s, a)
+hipError_t launch_stats_args(int d, bool narrow, bool t3, const StatsArgs& a, int blocks, hipStream_t s) {
+#define CPM_STATS(DD, ST, T3) hipLaunchKernelGGL((k_stats_args<DD, ST, T3>), dim3(blocks), dim3(kBlock), 0, s, a)
     if (d == 2) {
-        if (narrow) CPM_STATS(2, uint16_t); else CPM_STATS(2, uint32_t);
+        if (narrow && t3) CPM_STATS(2, uint16_t, true);
+        else if (narrow) CPM_STATS(2, uint16_t, false);
+        else CPM_STATS(2, uint32_t, false);
     } else {
-        if (narrow) CPM_STATS(3, uint16_t); else CPM_STATS(3, uint32_t);
+        if (narrow) CPM_STATS(3, uint16_t, false); else CPM_STATS(3, uint32_t, false);
     }
 #undef CPM_STATS
     return hipGetLastError();
@@ -2488,16 +2630,18 @@ int stats_pair_blocks(int d, int lgL) {
     return (int)((nsites + per - 1) / per);
 }
 
-int stats_cell_blocks(int ncells) { return (ncells + kBlock - 1) / kBlock; }
+int stats_cell_blocks(int ncells) { return (ncells + kBlock * kStatsCells - 1) / (kBlock * kStatsCells); }
 
-hipError_t launch_stats_batch(int d, bool narrow, const StatsDesc* descs, const int* eng_of, int blocks,
+hipError_t launch_stats_batch(int d, bool narrow, bool t3, const StatsDesc* descs, const int* eng_of, int blocks,
                               hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-#define CPM_STATS(DD, ST) \
-    hipExtLaunchKernelGGL((k_stats_batch<DD, ST>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of)
+#define CPM_STATS(DD, ST, T3) \
+    hipExtLaunchKernelGGL((k_stats_batch<DD, ST, T3>), dim3(blocks), dim3(kBlock), 0, s, e0, e1, 0, descs, eng_of)
     if (d == 2) {
-        if (narrow) CPM_STATS(2, uint16_t); else CPM_STATS(2, uint32_t);
+        if (narrow && t3) CPM_STATS(2, uint16_t, true);
+        else if (narrow) CPM_STATS(2, uint16_t, false);
+        else CPM_STATS(2, uint32_t, false);
     } else {
-        if (narrow) CPM_STATS(3, uint16_t); else CPM_STATS(3, uint32_t);
+        if (narrow) CPM_STATS(3, uint16_t, false); else CPM_STATS(3, uint32_t, false);
     }
 #undef CPM_STATS
     return hipGetLastError();
