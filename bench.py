@@ -149,15 +149,10 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     launch = sim.last_launch()
     sites = wl.L ** wl.d
     mcs = steps * mcs_per_step
-    log("batch forms on a launch-bound batch")
-    forms_small = _batch_forms_small(kw)
     avg_launch_s = kernel_ms / 1e3 / launches
     bytes_per_launch = updates / launches * BYTES_PER_UPDATE
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel_token(launch['kernel'])}_{name}.json")
-    if os.path.exists(tfile) and c == 1 and u == 1:
-        traffic = json.load(open(tfile))["bytes_per_dispatch"]
+    traffic = _profiled_traffic(launch["kernel"], name) if c == 1 and u == 1 else None
     out = {
         "value": mcs * sites / wall,
         "unit": "site-updates/s",
@@ -279,8 +274,8 @@ def measure_batched(nrep, mcs, log, autotune=False, batch_only=False):
         "sites": sites, "mcs": mcs, "ms_total": t_batch * 1e3,
         "storage": storage_of(kernel),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
-                     "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _profiled_traffic(kernel, "batched"),
+                     "kernel": kernel, "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
                      "launches": launches},
         "sequential": {"value": mcs * sites / t_seq, "ms_total": t_seq * 1e3,
                        "note": "each replica's run() waited for before the next"},
@@ -426,9 +421,10 @@ def measure_gradient(nrep, L, mcs, log, fit=True):
            "backward_s": t_bwd, "backward_kernel_ms": kernel_ms,
            "backward_kernel_ms_runs": kms, "backward_kernel_ms_median": sorted(kms)[len(kms) // 2],
            "backward_sites_per_s": nsites / (kernel_ms / 1e3),
-           "roofline": {"bound": "hbm", "kernel": "k_stats_batch<2>", "achieved": achieved,
+           "roofline": {"bound": "hbm", "kernel": "k_stats_batch<2, unsigned short, true>", "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": None, "bytes_per_launch": nsites * 4,
+                        "traffic": _profiled_traffic("k_stats_batch<2, unsigned short, true>", "gradient"),
+                        "bytes_per_launch": nsites * 4,
                         "avg_launch_us": kernel_ms * 1e3,
                         "note": "4 algorithmic bytes per site (one int32 read); one launch covers "
                                 "all replicas' pair counts and cell moments"},
@@ -530,6 +526,13 @@ def bench_interactive(a, log):
             "data": "synthetic", "config": {"workload": r["workload"]}, "roofline": None}
     line.update(r)
     return line
+
+
+def _profiled_traffic(kernel, workload):
+    """HBM bytes per dispatch of kernel on workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel_token(kernel)}_{workload}.json")
+    return json.load(open(tfile))["bytes_per_dispatch"] if os.path.exists(tfile) else None
 
 
 def kernel_token(name):
