@@ -110,6 +110,10 @@ struct SweepConsts {
     int refstream;
     int32_t* rstate;
     int lk, lr;             // log2 K, log2 (regions per axis)
+    // Attempt counters (cpm_set_counters), or null: attempts whose source and
+    // target differ, attempts evaluated (after the fixed-type rules),
+    // proposals accepted, flips committed.
+    unsigned long long* counters;
 };
 
 // One colour phase per launch.

@@ -311,6 +311,13 @@ struct cpm_engine {
     int32_t* d_bbox = nullptr;
     void* d_backup_lat = nullptr;
     int64_t* d_backup_tab = nullptr;
+    // attempt counters (cpm_set_counters): differing, evaluated, accepted,
+    // committed, then a copy of them that a redone resident chunk restores;
+    // attempts drawn are L^d per MCS counted (ctr_mcs_acc, plus the MCS since
+    // ctr_mcs0 while on)
+    unsigned long long* d_counters = nullptr;
+    bool counters_on = false;
+    uint64_t ctr_mcs0 = 0, ctr_mcs_acc = 0;
     int64_t res_fallbacks = 0;      // chunks redone with one launch per phase: the guard
     int64_t res_wait_fallbacks = 0; // ... or an abandoned neighbour wait
     bool autotune = false;  // cpm_set_autotune
@@ -368,6 +375,7 @@ struct cpm_engine {
         if (ev_ready) (void)hipEventDestroy(ev_ready);
         (void)hipFree(d_res_flags);
         (void)hipFree(d_bbox);
+        (void)hipFree(d_counters);
         (void)hipFree(d_backup_lat);
         (void)hipFree(d_backup_tab);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1138,6 +1146,7 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     gcpm::SweepConsts k{};
     k.lat = h->d_lat;
     k.lat16 = nullptr;  // set by the caller when it runs on the 16-bit lattice
+    k.counters = h->counters_on ? h->d_counters : nullptr;
     k.idbits = 24;
     k.idmask = gcpm::kSpinMask;
     for (int i = 0; i < 2; i++) k.tab[i] = h->d_tab[i];
@@ -1385,6 +1394,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             HIP_TRY(hipMemcpyAsync(h->d_backup_lat, cur_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
             HIP_TRY(hipMemcpyAsync(h->d_backup_tab, h->d_tab[0], sizeof(int64_t) * rows, hipMemcpyDeviceToDevice,
                                    h->stream));
+            if (h->counters_on)
+                HIP_TRY(hipMemcpyAsync(h->d_counters + 4, h->d_counters, 4 * sizeof(unsigned long long),
+                                       hipMemcpyDeviceToDevice, h->stream));
             HIP_TRY(gcpm::launch_fill_zero(reinterpret_cast<unsigned long long*>(h->d_res_flags), (nflag + 1) / 2,
                                            h->stream));
             HIP_TRY(gcpm::launch_resident_guard(true, bits != 0, ra.k, h->d_anchor, h->d_bbox, 1 << res_lte, bad,
@@ -1407,6 +1419,9 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                 HIP_TRY(hipMemcpyAsync(cur_lat, h->d_backup_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
                 for (int i = 0; i < 2; i++)
                     HIP_TRY(hipMemcpyAsync(h->d_tab[i], h->d_backup_tab, sizeof(int64_t) * rows,
+                                           hipMemcpyDeviceToDevice, h->stream));
+                if (h->counters_on)  // the chunk is counted when it is redone
+                    HIP_TRY(hipMemcpyAsync(h->d_counters, h->d_counters + 4, 4 * sizeof(unsigned long long),
                                            hipMemcpyDeviceToDevice, h->stream));
                 HIP_TRY(hipStreamSynchronize(h->stream));
                 if (fl[0])
@@ -1768,6 +1783,38 @@ int cpm_resident_fallbacks(cpm_handle h, int64_t* guard, int64_t* waits) {
     if (int rc = check_handle(h)) return rc;
     if (guard) *guard = h->res_fallbacks;
     if (waits) *waits = h->res_wait_fallbacks;
+    return CPM_OK;
+}
+
+int cpm_set_counters(cpm_handle h, int on) {
+    if (int rc = check_handle(h)) return rc;
+    if (on && !h->d_counters) HIP_TRY(hipMalloc(&h->d_counters, 8 * sizeof(unsigned long long)));
+    if (on) {
+        HIP_TRY(hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), h->stream));
+        h->ctr_mcs_acc = 0;
+        h->ctr_mcs0 = h->mcs;
+    } else if (h->counters_on) {
+        h->ctr_mcs_acc += h->mcs - h->ctr_mcs0;
+    }
+    h->counters_on = on != 0;
+    return CPM_OK;
+}
+
+int cpm_get_counters(cpm_handle h, int64_t out[5], int reset) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(CPM_EINVAL, "null output");
+    if (!h->d_counters) return fail(CPM_ESTATE, "counters were never switched on (cpm_set_counters)");
+    unsigned long long c[4];
+    HIP_TRY(hipMemcpyAsync(c, h->d_counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const uint64_t mcs = h->ctr_mcs_acc + (h->counters_on ? h->mcs - h->ctr_mcs0 : 0);
+    out[0] = (int64_t)(mcs * (uint64_t)h->sites);
+    for (int i = 0; i < 4; i++) out[1 + i] = (int64_t)c[i];
+    if (reset) {
+        HIP_TRY(hipMemsetAsync(h->d_counters, 0, sizeof(c), h->stream));
+        h->ctr_mcs_acc = 0;
+        h->ctr_mcs0 = h->mcs;
+    }
     return CPM_OK;
 }
 

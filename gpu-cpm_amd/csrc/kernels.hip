@@ -546,6 +546,15 @@ __device__ __forceinline__ int64_t pers_energy(const SweepConsts& k, int dir, ui
     return (m * g + ((int64_t)1 << 23)) >> 24;
 }
 
+// Attempt counters (cpm_set_counters; the reference allocates analytics[3]
+// for evaluated / accepted / committed and leaves its increments commented
+// out, kernels2d.cu:292,526,531): the lanes that reach one point together
+// add their number with one atomic per wave.
+__device__ __forceinline__ void count_attempts(unsigned long long* c, int slot) {
+    const unsigned long long m = __ballot(1);
+    if (__lane_id() == (unsigned int)(__ffsll(m) - 1)) atomicAdd(c + slot, (unsigned long long)__popcll(m));
+}
+
 // XT = false leaves out the chemotaxis, act and persistence terms (the
 // caller launches it only when none is on): fewer registers in 3D.
 template <int D, bool CO, bool XT = true, class Lat, class PV>
@@ -559,6 +568,10 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     const uint32_t vt = d.nv[CENTRE];
     uint32_t vs = d.vs;
     if (vs == vt) return;  // kernels2d.cu:276
+    // counted once per attempt: in its only pass, or in the commit / apply
+    // pass of a claimed or split phase
+    unsigned long long* const ctr = (cy.pass == 0 || cy.pass == 2 || cy.pass == 4 || cy.pass == 6) ? k.counters : nullptr;
+    if (ctr) count_attempts(ctr, 0);
     if (k.fixed_mask) {    // fixed types (kernels2d.cu:281-290)
         if ((k.fixed_mask >> (vt >> k.idbits)) & 1u) return;  // a fixed target stays
         if ((k.fixed_mask >> (vs >> k.idbits)) & 1u) {        // a fixed source copies medium
@@ -571,6 +584,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         }
     }
 
+    if (ctr) count_attempts(ctr, 1);  // kernels2d.cu:292
     const int ts = (int)(vs >> k.idbits), tt = (int)(vt >> k.idbits);
     const uint32_t cs = vs & k.idmask, ct = vt & k.idmask;
     // Issue the snapshot-table loads first: their latency overlaps the
@@ -632,6 +646,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         if (cy.pass == 3) return;
     }
     if (!accept) return;
+    if (ctr) count_attempts(ctr, 2);  // kernels2d.cu:526
     if (cy.pass == 1 || cy.pass == 2 || cy.pass == 5 || cy.pass == 6) {
         const uint64_t key = ((uint64_t)cy.tag << 32) | (uint64_t)(0xFFFFFFFFu - sb);
         if (cy.pass == 1 || cy.pass == 5) {
@@ -642,6 +657,7 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
         if ((cs && cy.claims[cs] != key) || (ct && cy.claims[ct] != key)) return;
     }
 
+    if (ctr) count_attempts(ctr, 3);  // kernels2d.cu:531
     lat.put(d.t, vs);  // kernels2d.cu:548
     if (XT && k.act) st<CO>(k.act_lat + lat.gindex(d.t), d.tick);  // kernels2d.cu:532-534
     // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums are

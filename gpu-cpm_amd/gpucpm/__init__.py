@@ -354,6 +354,19 @@ class Cpm:
         keep the fastest (all forms compute the same results)."""
         check(lib.cpm_set_autotune(self._h, int(bool(on))))
 
+    def set_counters(self, on=True):
+        """Count attempts in the sweep (the reference's analytics block,
+        cpm.cu:215, kernels2d.cu:292,526,531); switching on zeroes the counts."""
+        check(lib.cpm_set_counters(self._h, int(bool(on))))
+
+    def counters(self, reset=False):
+        """dict of attempts drawn, differing (source != target), evaluated
+        (after the fixed-type rules), accepted and committed since the
+        counters were switched on or last reset."""
+        out = (ctypes.c_int64 * 5)()
+        check(lib.cpm_get_counters(self._h, out, int(bool(reset))))
+        return dict(zip(("attempts", "differing", "evaluated", "accepted", "committed"), out))
+
     _RNG = {"philox": 0, "minstd": 1}
 
     def set_rng(self, name):

@@ -55,6 +55,8 @@ SIGNATURES = [
     ("cpm_get_cell_table", _I, [_H, _P(ctypes.c_int32), _P(ctypes.c_int32), _P(ctypes.c_int32),
                                 _P(ctypes.c_int64)]),
     ("cpm_set_autotune", _I, [_H, _I]),
+    ("cpm_set_counters", _I, [_H, _I]),
+    ("cpm_get_counters", _I, [_H, ctypes.POINTER(ctypes.c_int64), _I]),
     ("cpm_batch_set_autotune", _I, [_H, _I]),
     ("cpm_set_rng", _I, [_H, _I]),
     ("cpm_get_rng", _I, [_H, _P(_I)]),

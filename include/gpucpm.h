@@ -180,6 +180,20 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
  * kept for later runs of that workload (the choices are shared by the
  * engines of one process).  Off by default. */
 int cpm_set_autotune(cpm_handle h, int on);
+/* Attempt counters, the reference's analytics block (cpm.cu:215,559: three
+ * device counters allocated and zeroed; their increments at
+ * kernels2d.cu:292,526,531 and kernels3d.cu:263,548,553 are commented out)
+ * made live behind a flag.  cpm_set_counters(h, 1) allocates and zeroes
+ * them (0 switches counting off; the counts stay readable).
+ * cpm_get_counters fills out[5]: attempts drawn (L^d per MCS since the
+ * counters were switched on or reset), attempts whose source and target
+ * differ, attempts evaluated (after the fixed-type rules; analytics[0]),
+ * proposals accepted (analytics[1]), flips committed (after cell_sync's
+ * claim check; analytics[2]); reset != 0 zeroes them.  Each attempt counts
+ * once, also in claimed or split phases; batched runs count per engine.
+ * Results never depend on the flag. */
+int cpm_set_counters(cpm_handle h, int on);
+int cpm_get_counters(cpm_handle h, int64_t out[5], int reset);
 int cpm_set_rng(cpm_handle h, int mode);
 int cpm_get_rng(cpm_handle h, int* mode);
 

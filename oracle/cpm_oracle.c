@@ -493,14 +493,14 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                       const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                       uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
-                      int64_t* trace_n, int live);
+                      int64_t* trace_n, int live, int64_t* counts);
 
 int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                  const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt) {
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
-                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL, 0);
+                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL, 0, NULL);
 }
 
 /* Live-table checkerboard: the same schedule, stream and decisions as
@@ -517,7 +517,20 @@ int orc_sweep_live(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                    const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                    uint64_t seed, uint32_t mcs0, int nmcs, int shared, int ppt) {
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs, 1, 0,
-                      shared, ppt, 0, NULL, 0, NULL, 1);
+                      shared, ppt, 0, NULL, 0, NULL, 1, NULL);
+}
+
+/* orc_sweep_ex that also counts, once per attempt (in the commit / apply pass
+ * of a claimed or split phase), into counts[4]: attempts whose source and
+ * target differ, attempts evaluated after the fixed-type rules, proposals
+ * accepted, flips committed (the reference's analytics block, whose
+ * increments at kernels2d.cu:292,526,531 are commented out). */
+int orc_sweep_counts(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                     const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                     uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
+                     int ppt, int64_t* counts) {
+    return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs, nthreads,
+                      cell_sync, shared, ppt, 0, NULL, 0, NULL, 0, counts);
 }
 
 int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
@@ -526,14 +539,14 @@ int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, 
                         int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n) {
     if (shared && ppt != 2 * c) return -9; /* the reference's shared mode with ppt > 2c loops regions per thread */
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
-                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n, 0);
+                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n, 0, NULL);
 }
 
 static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                       const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                       uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
-                      int64_t* trace_n, int live) {
+                      int64_t* trace_n, int live, int64_t* counts) {
     if (d != 2 && d != 3) return -1;
     /* Act with shared roles and c = 1: an attempt's act term reads the
      * source's neighbourhood, two sites from its target, where other targets
@@ -740,6 +753,13 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                             }
                         }
                         if (vs == vt) continue; /* kernels2d.cu:276 */
+                        const int counting = counts != NULL && pass == 1;
+                        if (counting) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            counts[0]++;
+                        }
                         /* fixed types (kernels2d.cu:281-290): a fixed target is
                          * never overwritten; a fixed source copies medium.
                          * Onto a medium target that is a medium -> medium
@@ -751,6 +771,12 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                         if (p->fixed[vs >> 24]) {
                             vs = 0;
                             if (vs == vt && !q.act) continue;
+                        }
+                        if (counting) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            counts[1]++;
                         }
                         int as, at;
                         int64_t dh = delta_h_core(L, d, &nb, lat, aR, pR, &q, tx, ty,
@@ -772,6 +798,12 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                             if (!cell_sync) continue;
                         }
                         if (!accept) continue;
+                        if (counting) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            counts[2]++;
+                        }
                         uint32_t cs = vs & 0xFFFFFFu, ct = vt & 0xFFFFFFu;
                         if (cell_sync && pass == 0) {
                             if (cs) claim_min(owner, cs, sb);
@@ -779,6 +811,12 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                             continue;
                         }
                         if (cell_sync && ((cs && owner[cs] != sb) || (ct && owner[ct] != sb))) continue;
+                        if (counting) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                            counts[3]++;
+                        }
                         lat[ti] = vs; /* kernels2d.cu:548 */
                         if (q.act) q.act_lat[ti] = mcs; /* kernels2d.cu:532-534 */
                         int pos[3] = {tx, ty, tz};

@@ -187,6 +187,14 @@ int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt);
 
+/* orc_sweep_ex counting, once per attempt, into counts[4]: attempts whose
+ * source and target differ, evaluated (after the fixed-type rules),
+ * accepted, committed. */
+int orc_sweep_counts(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
+                     const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
+                     uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
+                     int ppt, int64_t* counts);
+
 /* Live-table checkerboard (statistics only): orc_sweep_ex's schedule,
  * stream and decisions, each phase's attempts made one at a time in a
  * per-phase pseudo-random order, every attempt reading the area / perimeter

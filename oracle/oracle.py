@@ -129,6 +129,8 @@ def lib():
         L.orc_sweep_sync.restype = ctypes.c_int
         L.orc_sweep_ex.argtypes = L.orc_sweep.argtypes + [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_sweep_ex.restype = ctypes.c_int
+        L.orc_sweep_counts.argtypes = L.orc_sweep_ex.argtypes + [i64p]
+        L.orc_sweep_counts.restype = ctypes.c_int
         L.orc_sweep_live.argtypes = L.orc_sweep.argtypes[:-1] + [ctypes.c_int, ctypes.c_int]
         L.orc_sweep_live.restype = ctypes.c_int
         L.orc_sweep_refstream.argtypes = L.orc_sweep_ex.argtypes + [
@@ -316,13 +318,15 @@ def schedule(L: int, d: int, c: int, shared: bool, ppt: int):
 
 def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
           mcs0: int, nmcs: int, nthreads: int = 1, cell_sync: bool = False, shared: bool = True,
-          ppt: int = None, rng: str = "philox", trace_cap: int = 0):
+          ppt: int = None, rng: str = "philox", trace_cap: int = 0, counts: np.ndarray = None):
     """Run nmcs MCS in place on lat (uint32 view) and tables.  shared selects
     the roles of the run() kwarg of that name; ppt is positions_per_thread
     (used without shared roles).  rng = "minstd" uses the reference's
     per-thread MINSTD stream (one call = one reference launch: the streams
     are seeded at entry); with trace_cap > 0 the attempts' (region, target,
-    source, acceptance draw or -1) rows are returned."""
+    source, acceptance draw or -1) rows are returned.  counts (int64[4],
+    added to): attempts with differing source and target, evaluated,
+    accepted, committed (counter stream only)."""
     assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
     lat32 = lat.view(np.uint32)
     L = lat.shape[0]
@@ -332,7 +336,11 @@ def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, se
             ctypes.byref(p), int(c), int(u), int(seed) & (2**64 - 1),
             int(mcs0) & 0xFFFFFFFF, int(nmcs), int(nthreads), int(bool(cell_sync)),
             int(bool(shared)), int(ppt if ppt is not None else 2 * c))
-    if rng == "philox":
+    if rng == "philox" and counts is not None:
+        assert counts.dtype == np.int64 and counts.size >= 4
+        rc = lib().orc_sweep_counts(*args, _ptr(counts, ctypes.c_int64))
+        trace = None
+    elif rng == "philox":
         rc = lib().orc_sweep_ex(*args)
         trace = None
     elif rng == "minstd":

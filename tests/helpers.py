@@ -40,12 +40,13 @@ def oracle_params(wl) -> "O.Params":
 
 
 def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=None, shared=True,
-               ppt=None, rng="philox", cell_sync=False):
+               ppt=None, rng="philox", cell_sync=False, counts=None):
     """Tally + sweep in the oracle; returns (lattice, Tables).  With a 3D
     persistence term the run is cut into chunks of ``inner`` MCS with a
     preferred-direction update after each (tables.pref holds the result);
     with rng="minstd" each chunk is one reference launch (its streams are
-    seeded at the chunk's start)."""
+    seeded at the chunk's start).  counts (int64[4]) collects the attempt
+    counters (counter stream only)."""
     lat = np.ascontiguousarray(wl.lattice if lattice is None else lattice).astype(np.uint32).copy()
     tab = O.tally(lat, wl.d, wl.ncells)
     p = oracle_params(wl)
@@ -65,7 +66,7 @@ def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=Non
         inner = inner or nmcs
         for m in range(0, nmcs, inner):
             O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0 + m, inner, nthreads, shared=shared, ppt=ppt,
-                    rng=rng, cell_sync=cell_sync)
+                    rng=rng, cell_sync=cell_sync, counts=counts)
             ps.update(tab, ctype, pw["pers_diff"])
         tab.pref = ps.pref
     elif nmcs and rng == "minstd":
@@ -75,7 +76,7 @@ def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=Non
                     rng=rng, cell_sync=cell_sync)
     elif nmcs:
         O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0, nmcs, nthreads, shared=shared, ppt=ppt,
-                cell_sync=cell_sync)
+                cell_sync=cell_sync, counts=counts)
     if act is not None:
         tab.act = act
     return lat, tab

@@ -923,3 +923,51 @@ def test_nonshared_sweep_is_thread_invariant_and_exact():
         area, perim = _numpy_recount(a, 2, n)
         np.testing.assert_array_equal(ta.area, area)
         np.testing.assert_array_equal(ta.perim, perim)
+
+
+def _counted_sweep(lat, d, n, p, c, u, seed, nmcs, nthreads=4, **kw):
+    tab = O.tally(lat, d, n)
+    counts = np.zeros(4, np.int64)
+    O.sweep(lat, d, tab, p, c, u, seed, 0, nmcs, nthreads, counts=counts, **kw)
+    return counts
+
+
+def test_attempt_counters_closed_forms():
+    """The oracle's attempt counters: with c = u = 1 every site is a target
+    exactly once per MCS and a committed flip changes its target, so after
+    one MCS the committed count is the number of changed sites; the counts
+    are nested (committed <= accepted <= evaluated <= differing <= L^d per
+    MCS), equal where nothing can refuse a step (committed = accepted without
+    cell_sync; evaluated = differing without fixed types), and independent of
+    the thread count."""
+    from stats_common import sorting_lattice
+    lat0, n, p, _ = sorting_lattice(L=64, seed=3)
+    lat = lat0.copy()
+    c1 = _counted_sweep(lat, 2, n, p, 1, 1, 11, 1)
+    assert c1[3] == np.count_nonzero(lat != lat0) > 0
+    assert c1[0] == c1[1] and c1[2] == c1[3]
+    for nthreads in (1, 8):
+        lat = lat0.copy()
+        c10 = _counted_sweep(lat, 2, n, p, 2, 4, 12, 10, nthreads=nthreads)
+        if nthreads == 1:
+            ref = c10
+        np.testing.assert_array_equal(c10, ref)
+    assert 64 * 64 * 10 >= ref[0] >= ref[1] >= ref[2] >= ref[3] > 0
+
+
+def test_attempt_counters_with_cell_sync_and_fixed_types():
+    """cell_sync refuses accepted proposals whose cells another sub-block
+    claimed (committed < accepted on a busy lattice); a fixed wall makes
+    attempts that never reach the energy (evaluated < differing)."""
+    from stats_common import sorting_lattice
+    lat0, n, p, _ = sorting_lattice(L=64, seed=4)
+    lat = lat0.copy()
+    cs = _counted_sweep(lat, 2, n, p, 2, 1, 13, 10, cell_sync=True)
+    assert cs[3] < cs[2] <= cs[1] == cs[0]
+    wall = lat0.copy()
+    wall[0, :] = wall[:, 0] = (3 << 24) | (n + 1)
+    pw = O.Params.make(10, J=[[0, 16, 16, 5], [16, 14, 11, 5], [16, 11, 2, 5], [5, 5, 5, 0]],
+                       lam_area=[0, 1, 1, 0], target_area=[0, 40, 40, 0], fixed=[0, 0, 0, 1])
+    cw = _counted_sweep(wall, 2, n + 1, pw, 1, 1, 14, 10)
+    assert cw[1] < cw[0] and cw[2] == cw[3]
+    assert np.all(wall[0, :] == (3 << 24) | (n + 1))
