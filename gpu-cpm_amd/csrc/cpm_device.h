@@ -169,6 +169,7 @@ struct SmallArgs {
     uint32_t mcs0;          // first MCS counter
     int nmcs;               // MCS in this launch
     int lh;                 // log2 (same-colour sub-blocks per axis)
+    int thr_lds;            // 1: the acceptance thresholds are staged in LDS too
 };
 
 // The resident form (2D, shared roles, c = 1, u = 1): one cooperative block

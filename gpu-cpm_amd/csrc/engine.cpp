@@ -1462,6 +1462,8 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             if (int rc = h->ensure_wide()) return rc;
             h->narrow_valid = false;
         }
+        // the thresholds go to LDS too when they fit beside the lattice
+        sa.thr_lds = gcpm::small_lds_bytes(h->d, h->lgL, h->n, bits != 0) + 4 * (size_t)k.nthr <= kSmallLdsBudget;
         const bool pers = k.pers != 0;
         if (pers && !h->ring_valid)
             if (int rc = h->init_ring()) return rc;

@@ -778,7 +778,7 @@ __device__ __forceinline__ void ref_attempts(const Lat& lat, const SweepConsts& 
 
 // The u copy attempts of sub-block (bx, by, bz) in one colour phase, in
 // order.
-template <int D, bool CO, class Lat, class PV>
+template <int D, bool CO, bool XT = true, class Lat, class PV>
 __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepConsts& k,
                                                    const PV& P, const uint32_t* __restrict__ thr,
                                                    int bx, int by, int bz, uint32_t mcs,
@@ -792,13 +792,13 @@ __device__ __forceinline__ void sub_block_attempts(const Lat& lat, const SweepCo
         // loop-invariant values alive (they would spill scalar registers).
         Draw<D, Lat> d;
         draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, (uint32_t)att0, d);
-        finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
+        finish_attempt<D, CO, XT>(lat, k, P, thr, T, cy, sb, d);
         return;
     }
     for (int i = att0; i < att0 + natt; i++) {
         Draw<D, Lat> d;
         draw_attempt<D>(lat, k, sb, bx, by, bz, mcs, phase, (uint32_t)i, d);
-        finish_attempt<D, CO>(lat, k, P, thr, T, cy, sb, d);
+        finish_attempt<D, CO, XT>(lat, k, P, thr, T, cy, sb, d);
     }
 }
 
@@ -1559,10 +1559,10 @@ __device__ __forceinline__ void phase_offsets_dev(const SweepConsts& k, int pc, 
 // (the run's state is then settled).
 constexpr int kSmallThreads = 1024;
 
-template <int D, class ST, class PV>
+template <int D, class ST, bool XT, class PV>
 __device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* lat_s, int64_t* R,
                                           int64_t* Dl, QueuedAttempt<D, false, ST>* queue,
-                                          unsigned int* nqueued) {
+                                          unsigned int* nqueued, const uint32_t* __restrict__ thr) {
     const SweepConsts& k = a.k;
     const int tid = threadIdx.x;
     const int nsub = 1 << (D * a.lh);
@@ -1571,7 +1571,6 @@ __device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* l
     using Lat = LdsWholeLat<D, ST>;
     const Lat lat{lat_s, k.lgL, Lm};
     const PhaseTables T{R, Dl, nullptr, nullptr};
-    const uint32_t* __restrict__ thr = k.thr;
     for (int m = 0; m < a.nmcs; m++) {
         const uint32_t mcs = a.mcs0 + (uint32_t)m;
         for (int r = 0; r < k.rounds; r++) {
@@ -1588,7 +1587,7 @@ __device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* l
                         bz = D == 3 ? ((q >> (2 * a.lh)) << k.lstride) + oz : 0;
                     }
                     if (k.u > 1) {
-                        if (q < nsub) sub_block_attempts<D, false>(lat, k, P, thr, bx, by, bz, mcs, phase, T);
+                        if (q < nsub) sub_block_attempts<D, false, XT>(lat, k, P, thr, bx, by, bz, mcs, phase, T);
                         continue;
                     }
                     // u = 1: draw, queue the attempts whose sites differ
@@ -1630,7 +1629,7 @@ __device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* l
                         d.vs = qa.vs;
                         d.dir = qa.dir();
                         d.tick = mcs;
-                        finish_attempt<D, false>(lat, k, P, thr, T, CellSync{0, nullptr, 0}, 0u, d);
+                        finish_attempt<D, false, XT>(lat, k, P, thr, T, CellSync{0, nullptr, 0}, 0u, d);
                     }
                     __syncthreads();  // (the queue is reused by the next round)
                 }
@@ -1649,7 +1648,9 @@ __device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* l
     }
 }
 
-template <int D, class ST>
+// XT = false: without the chemotaxis, act and persistence code (fewer
+// registers, no scalar spills; the host picks it when those terms are off).
+template <int D, class ST, bool XT>
 __global__ __launch_bounds__(kSmallThreads) void k_small_lds(SmallArgs a) {
     extern __shared__ __align__(16) unsigned char small_smem[];
     const SweepConsts& k = a.k;
@@ -1659,9 +1660,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_lds(SmallArgs a) {
     int64_t* Dl = R + rows;
     auto* queue = reinterpret_cast<QueuedAttempt<D, false, ST>*>(Dl + rows);
     ST* lat_s = reinterpret_cast<ST*>(queue + kSmallThreads);
+    // the thresholds after the lattice (an even number of sites: 4-byte aligned)
+    uint32_t* thr_s = reinterpret_cast<uint32_t*>(lat_s + sites);
     __shared__ unsigned int nqueued;
     const ST* __restrict__ g = lat_base<ST>(k);
     for (uint32_t i = threadIdx.x; i < sites; i += kSmallThreads) lat_s[i] = g[i];
+    if (a.thr_lds)
+        for (int i = threadIdx.x; i < k.nthr; i += kSmallThreads) thr_s[i] = k.thr[i];
+    const uint32_t* thr = a.thr_lds ? thr_s : k.thr;
     for (int c = threadIdx.x; c < rows; c += kSmallThreads) {
         R[c] = k.tab[0][c];  // settled: both buffers equal, deltas zero
         Dl[c] = 0;
@@ -1670,10 +1676,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_lds(SmallArgs a) {
     if (k.nt <= kSmallTypes) {
         SmallParams P;
         P.load(k.prm);
-        small_run<D, ST>(a, P, lat_s, R, Dl, queue, &nqueued);
+        small_run<D, ST, XT>(a, P, lat_s, R, Dl, queue, &nqueued, thr);
     } else {
         const LdsParamView P{k.prm->w, k.nt, k.nt * k.nt};  // (read from global memory)
-        small_run<D, ST>(a, P, lat_s, R, Dl, queue, &nqueued);
+        small_run<D, ST, XT>(a, P, lat_s, R, Dl, queue, &nqueued, thr);
     }
     ST* gw = lat_base<ST>(k);
     for (uint32_t i = threadIdx.x; i < sites; i += kSmallThreads) gw[i] = lat_s[i];
@@ -1691,17 +1697,28 @@ size_t small_lds_bytes(int d, int lgL, int ncells, bool narrow) {
 }
 
 hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    const size_t bytes = small_lds_bytes(d, a.k.lgL, a.k.ncells, narrow);
-    const void* f = d == 2 ? (narrow ? (const void*)&k_small_lds<2, uint16_t> : (const void*)&k_small_lds<2, uint32_t>)
-                           : (narrow ? (const void*)&k_small_lds<3, uint16_t> : (const void*)&k_small_lds<3, uint32_t>);
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-#define CPM_SMALL(DD, ST) \
-    hipExtLaunchKernelGGL((k_small_lds<DD, ST>), dim3(1), dim3(kSmallThreads), (unsigned)bytes, s, e0, e1, 0, a)
+    const size_t bytes = small_lds_bytes(d, a.k.lgL, a.k.ncells, narrow) + (a.thr_lds ? 4 * (size_t)a.k.nthr : 0);
+    const bool xt = a.k.chem || a.k.act || a.k.pers;
+#define CPM_SMALL(DD, ST, XT)                                                                                        \
+    do {                                                                                                             \
+        hipError_t e = hipFuncSetAttribute((const void*)&k_small_lds<DD, ST, XT>,                                    \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                 \
+        if (e != hipSuccess) return e;                                                                               \
+        hipExtLaunchKernelGGL((k_small_lds<DD, ST, XT>), dim3(1), dim3(kSmallThreads), (unsigned)bytes, s, e0, e1,   \
+                              0, a);                                                                                 \
+    } while (0)
     if (d == 2) {
-        if (narrow) CPM_SMALL(2, uint16_t); else CPM_SMALL(2, uint32_t);
+        if (narrow) {
+            if (xt) CPM_SMALL(2, uint16_t, true); else CPM_SMALL(2, uint16_t, false);
+        } else {
+            if (xt) CPM_SMALL(2, uint32_t, true); else CPM_SMALL(2, uint32_t, false);
+        }
     } else {
-        if (narrow) CPM_SMALL(3, uint16_t); else CPM_SMALL(3, uint32_t);
+        if (narrow) {
+            if (xt) CPM_SMALL(3, uint16_t, true); else CPM_SMALL(3, uint16_t, false);
+        } else {
+            if (xt) CPM_SMALL(3, uint32_t, true); else CPM_SMALL(3, uint32_t, false);
+        }
     }
 #undef CPM_SMALL
     return hipGetLastError();
