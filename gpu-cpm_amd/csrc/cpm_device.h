@@ -109,7 +109,11 @@ struct SweepConsts {
     // lr) | rx]; thr then holds the 31-bit acceptance table.
     int refstream;
     int32_t* rstate;
-    int lk, lr;             // log2 K, log2 (regions per axis)
+    int lk, lr;             // log2 K, log2 (reference threads per axis)
+    // The reference's shared mode with positions_per_thread > 2c: each
+    // reference thread owns 2^lg regions of 2c sites per axis, threads in
+    // blocks of 2^ld (threads_per_block) per axis (0, 0 otherwise).
+    int lg, ld;
     // Attempt counters (cpm_set_counters), or null: attempts whose source and
     // target differ, attempts evaluated (after the fixed-type rules),
     // proposals accepted, flips committed.
@@ -136,6 +140,7 @@ struct PhaseArgs {
     uint8_t* decide;        // split phase: one decision per sub-block of the phase
     int form;               // kForm* bits: which per-phase kernel runs
     int att0, natt;         // attempt indices att0 .. att0 + natt - 1 of each sub-block
+    int gsx, gsy, gsz;      // reference stream with lg > 0: the phase's block slice per axis
 };
 
 // Per-phase kernel forms (results never depend on them).
@@ -222,6 +227,7 @@ struct BatchPhaseArgs {
     int form;               // kForm* bits (compact, small, attempts per thread)
     int narrow;             // every replica on its 16-bit lattice (else all on 32-bit)
     int refstream;          // every replica draws from its reference stream (not compacted)
+    int gsx, gsy, gsz;      // as PhaseArgs
 };
 
 // Host-side launchers (kernels.hip).

@@ -1093,8 +1093,11 @@ namespace {
 
 // Active sub-block offsets of phase pc of a round: colour (cbX outermost,
 // kernels2d.cu:179-180) then group, offset = group K + colour per axis.
+// With block slices (k.lg > 0, shared roles: S = 1) phase pc of a round is
+// slice pc / K^d (phase_slice), colour pc mod K^d.
 void phase_offsets(int d, const gcpm::SweepConsts& k, int pc, int* ox, int* oy, int* oz) {
     const int K = k.K, S = k.S;
+    if (k.lg) pc %= d == 3 ? K * K * K : K * K;
     const int ngrp = d == 3 ? S * S * S : S * S;
     const int col = pc / ngrp, grp = pc % ngrp;
     const int cx = d == 3 ? col / (K * K) : col / K;
@@ -1106,6 +1109,16 @@ void phase_offsets(int d, const gcpm::SweepConsts& k, int pc, int* ox, int* oy, 
     *ox = gx * K + cx;
     *oy = gy * K + cy;
     *oz = gz * K + cz;
+}
+
+void phase_slice(int d, const gcpm::SweepConsts& k, int pc, int* gx, int* gy, int* gz) {
+    *gx = *gy = *gz = 0;
+    if (!k.lg) return;
+    const int K = k.K, G = 1 << k.lg;
+    const int gs = pc / (d == 3 ? K * K * K : K * K);  // x outermost (kernels2d.cu:92-93)
+    *gx = d == 3 ? gs / (G * G) : gs / G;
+    *gy = d == 3 ? (gs / G) % G : gs % G;
+    *gz = d == 3 ? gs % G : 0;
 }
 
 // Validate run arguments against engine h and fill the sweep constants
@@ -1135,6 +1148,23 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     if (!shared) {
         while (S * R - c < 3) S *= 2;
         if (S > h->L / R) S = h->L / R;
+    }
+    // The reference stream with shared roles and positions_per_thread > 2c
+    // keeps the reference's thread layout (kernels2d.cu:72-86,92-106,136-146;
+    // cpm.cu:363): L / ppt threads per axis in blocks of threads_per_block,
+    // each owning ppt / 2c regions of 2c sites per axis, visited one block
+    // slice at a time; its stream is the thread's.
+    int lg = 0, ld = 0;
+    const bool gblocks = h->rng == 1 && shared && a->positions_per_thread > R;
+    if (gblocks) {
+        const int ppt = a->positions_per_thread, tpb = a->threads_per_block;
+        if (ppt % R || h->L % ppt || (h->L / ppt) % tpb)
+            return fail(CPM_EINVAL,
+                        "the reference stream with shared = 1 and positions_per_thread > "
+                        "2 * positions_per_checkerboard needs 2c to divide positions_per_thread and "
+                        "threads_per_block * positions_per_thread to divide dimension (the reference's grid)");
+        lg = ilog2(ppt / R);
+        ld = ilog2(tpb);
     }
     const long cd = (long)c * c * (h->d == 3 ? c : 1);
     const int u = a->updates_per_checkerboard_switch;
@@ -1194,13 +1224,15 @@ int prepare_run(cpm_handle h, const cpm_run_args* a, gcpm::SweepConsts* kout, in
     k.K = K;
     k.S = S;
     k.phases_per_round = 1;
-    for (int i = 0; i < h->d; i++) k.phases_per_round *= K * S;
+    for (int i = 0; i < h->d; i++) k.phases_per_round *= K * S << lg;  // (block slices x colours)
     k.refstream = 0;
     k.rstate = nullptr;
     k.lk = ilog2(K);
-    k.lr = h->lgL - ilog2(R);
+    k.lg = lg;
+    k.ld = ld;
+    k.lr = h->lgL - ilog2(R) - lg;
     *kout = k;
-    *lh_out = h->lgL - k.lc - k.lstride;
+    *lh_out = h->lgL - k.lc - k.lstride - lg;
     return CPM_OK;
 }
 
@@ -1249,13 +1281,8 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         }
     }
     if (h->rng == 1) {
-        // The reference stream: one launch per colour phase (each region's
-        // state is carried between phases in d_rstate).
-        if (!k.roles && a->positions_per_thread != 2 * c)
-            return fail(CPM_ENOTIMPL,
-                        "the reference stream with shared = 1 needs positions_per_thread = 2 * "
-                        "positions_per_checkerboard (with more, the reference's threads loop over "
-                        "several regions)");
+        // The reference stream: one launch per colour phase (each reference
+        // thread's state is carried between phases in d_rstate).
         plan = Plan{};
         plan.mode = Plan::kPhase;
         const size_t nreg = (size_t)1 << (h->d * k.lr);
@@ -1561,6 +1588,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             for (int r = 0; r < k.rounds; r++) {
                 for (int pc = 0; pc < k.phases_per_round; pc++, g++) {
                     phase_offsets(h->d, k, pc, &pa.cx, &pa.cy, &pa.cz);
+                    phase_slice(h->d, k, pc, &pa.gsx, &pa.gsy, &pa.gsz);
                     pa.phase = (uint32_t)(r * k.phases_per_round + pc);
                     pa.g = g;
                     // cell_sync: per attempt index, a claim launch and a commit
@@ -2105,6 +2133,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
                 for (int r = 0; r < rounds && ce == hipSuccess; r++) {
                     for (int pc = 0; pc < k0.phases_per_round && ce == hipSuccess; pc++, gl++) {
                         phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
+                        phase_slice(b->d, k0, pc, &pa.gsx, &pa.gsy, &pa.gsz);
                         pa.phase = (uint32_t)(r * k0.phases_per_round + pc);
                         pa.g = gl & 1;
                         pa.att0 = 0;
@@ -2157,6 +2186,7 @@ int batch_run_phase(cpm_batch* b, const cpm_run_args* a, std::vector<gcpm::Repli
         for (int r = 0; r < rounds; r++) {
             for (int pc = 0; pc < k0.phases_per_round; pc++, g++) {
                 phase_offsets(b->d, k0, pc, &pa.cx, &pa.cy, &pa.cz);
+                phase_slice(b->d, k0, pc, &pa.gsx, &pa.gsy, &pa.gsz);
                 pa.phase = (uint32_t)(r * k0.phases_per_round + pc);
                 pa.g = g;
                 // as cpm_run: cell_sync makes a claim and a commit launch per
@@ -2278,11 +2308,7 @@ int cpm_batch_run(cpm_batch_handle b, const cpm_run_args* a) {
         if (h->rng != b->sims[0]->rng)
             return fail(CPM_EINVAL, "the engines of a batch must use the same random stream (cpm_set_rng)");
         if (int rc = prepare_run(h, a, &reps[i].k, &reps[i].lh)) return rc;
-        if (h->rng == 1) {  // reference streams: one state per region, as in cpm_run
-            if (!reps[i].k.roles && a->positions_per_thread != 2 * a->positions_per_checkerboard)
-                return fail(CPM_ENOTIMPL,
-                            "the reference stream with shared = 1 needs positions_per_thread = 2 * "
-                            "positions_per_checkerboard");
+        if (h->rng == 1) {  // reference streams: one state per reference thread, as in cpm_run
             const size_t nreg = (size_t)1 << (b->d * reps[i].k.lr);
             if (nreg > h->rstate_cap) {
                 (void)hipFree(h->d_rstate);

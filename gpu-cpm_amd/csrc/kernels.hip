@@ -721,8 +721,13 @@ __device__ __forceinline__ void ref_attempts(const Lat& lat, const SweepConsts& 
     constexpr int NB = D == 3 ? 27 : 9;
     const uint32_t sb = sub_block_id<D>(k, bx, by, bz);
     if (natt < 0) natt = k.u;
-    const uint32_t rid = (uint32_t)(bx >> k.lk) | ((uint32_t)(by >> k.lk) << k.lr) |
-                         (D == 3 ? (uint32_t)(bz >> k.lk) << (2 * k.lr) : 0u);
+    // the reference thread whose region holds the sub-block (with lg = 0 the
+    // region itself; see ref_region)
+    auto thread_of = [&](int b) {
+        const int r = b >> k.lk;
+        return (uint32_t)(((r >> (k.lg + k.ld)) << k.ld) | (r & ((1 << k.ld) - 1)));
+    };
+    const uint32_t rid = thread_of(bx) | (thread_of(by) << k.lr) | (D == 3 ? thread_of(bz) << (2 * k.lr) : 0u);
     int32_t st = k.rstate[rid];
     const int c = 1 << k.lc, Lm = (1 << k.lgL) - 1;
     for (int i = 0; i < natt; i++) {
@@ -866,9 +871,15 @@ __device__ __forceinline__ void colour_of(int col, int& cx, int& cy, int& cz) {
     cz = D == 3 ? col & 1 : 0;
 }
 
-// One colour phase.  Thread q owns same-colour sub-block q and makes its u
-// copy attempts in order; sub-blocks of one colour are two sub-blocks apart,
-// so no attempt reads a site another thread writes in this phase.
+// The reference's shared mode with positions_per_thread > 2c
+// (kernels2d.cu:72-86,92-106,136-146): blocks of 2^ld threads per axis,
+// each thread owning 2^lg regions of 2c sites per axis, visited one block
+// slice g at a time, so thread j of a phase in slice g works in region
+// ((j >> ld) << (lg + ld)) | (g << ld) | (j mod 2^ld).
+__device__ __forceinline__ int ref_region(const SweepConsts& k, int j, int g) {
+    return ((j >> k.ld) << (k.lg + k.ld)) | (g << k.ld) | (j & ((1 << k.ld) - 1));
+}
+
 // One colour phase.  Thread q owns same-colour sub-block q and makes its u
 // copy attempts in order; sub-blocks of one colour are two sub-blocks apart,
 // so no attempt reads a site another thread writes in this phase.  The
@@ -886,9 +897,15 @@ __device__ __forceinline__ void sweep_phase_body(const PhaseArgs& a, const PV& P
     if (gtid < a.nsub) {
         if (a.sync_pass == 3 || a.sync_pass == 5) a.decide[gtid] = 0;
         const int hm = (1 << a.lh) - 1;
-        const int bx = ((gtid & hm) << a.lstride) + a.cx;
-        const int by = (((gtid >> a.lh) & hm) << a.lstride) + a.cy;
-        const int bz = D == 3 ? ((gtid >> (2 * a.lh)) << a.lstride) + a.cz : 0;
+        int jx = gtid & hm, jy = (gtid >> a.lh) & hm, jz = D == 3 ? gtid >> (2 * a.lh) : 0;
+        if (REF && k.lg) {
+            jx = ref_region(k, jx, a.gsx);
+            jy = ref_region(k, jy, a.gsy);
+            if (D == 3) jz = ref_region(k, jz, a.gsz);
+        }
+        const int bx = (jx << a.lstride) + a.cx;
+        const int by = (jy << a.lstride) + a.cy;
+        const int bz = D == 3 ? (jz << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
         const CellSync cy{a.sync_pass, a.claims, a.sync_tag, a.decide, gtid};
         if constexpr (REF)
@@ -1225,8 +1242,9 @@ __global__ void k_ref_seed(int32_t* st, uint32_t n, int lr, int lR, int lgL, int
 
 hipError_t launch_ref_seed(int d, const SweepConsts& k, uint32_t seed32, hipStream_t s) {
     const uint32_t n = 1u << (d * k.lr);
+    // a reference thread's first site: its index times positions_per_thread
     hipLaunchKernelGGL(k_ref_seed, dim3((n + 255) / 256), dim3(256), 0, s, k.rstate, n, k.lr,
-                       k.lk + k.lc, k.lgL, d, seed32);
+                       k.lk + k.lc + k.lg, k.lgL, d, seed32);
     return hipGetLastError();
 }
 
@@ -1248,9 +1266,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_phase(BatchPhaseArgs a) {
     if (gtid < (1 << (D * lh))) {
         if (a.sync_pass == 3 || a.sync_pass == 5) rd.decide[gtid] = 0;
         const int hm = (1 << lh) - 1;
-        const int bx = ((gtid & hm) << a.lstride) + a.cx;
-        const int by = (((gtid >> lh) & hm) << a.lstride) + a.cy;
-        const int bz = D == 3 ? ((gtid >> (2 * lh)) << a.lstride) + a.cz : 0;
+        int jx = gtid & hm, jy = (gtid >> lh) & hm, jz = D == 3 ? gtid >> (2 * lh) : 0;
+        if (REF && k.lg) {  // (see ref_region)
+            jx = ref_region(k, jx, a.gsx);
+            jy = ref_region(k, jy, a.gsy);
+            if (D == 3) jz = ref_region(k, jz, a.gsz);
+        }
+        const int bx = (jx << a.lstride) + a.cx;
+        const int by = (jy << a.lstride) + a.cy;
+        const int bz = D == 3 ? (jz << a.lstride) + a.cz : 0;
         const GlobalLat<D, false, ST> lat{lat_base<ST>(k), k.lgL, (1 << k.lgL) - 1};
         const CellSync cy{a.sync_pass, rd.claims, a.sync_tag, rd.decide, gtid};
         const uint32_t m = a.m + (a.mbase ? *a.mbase : 0u);

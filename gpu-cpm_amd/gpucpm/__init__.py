@@ -372,12 +372,14 @@ class Cpm:
     def set_rng(self, name):
         """Random stream: "philox" (default; counter-based, keyed by sub-block,
         MCS, phase and attempt) or "minstd", the reference's per-thread
-        stream (random.cu, kernels2d.cu:99): one MINSTD state per region of
-        positions_per_thread sites, reseeded every inner_iterations chunk
-        from the region's first site index + the low 32 bits of the seed, for
-        comparisons with the reference.  With shared=1 "minstd" needs
-        positions_per_thread = 2 * positions_per_checkerboard; a Batch's
-        engines must all use the same stream."""
+        stream (random.cu, kernels2d.cu:99): one MINSTD state per reference
+        thread of positions_per_thread sites, reseeded every inner_iterations
+        chunk from the thread's first site index + the low 32 bits of the
+        seed, for comparisons with the reference.  With shared=1 and
+        positions_per_thread > 2 * positions_per_checkerboard the threads sit
+        in blocks of threads_per_block per axis and visit their regions one
+        block slice at a time, as the reference's do; a Batch's engines must
+        all use the same stream."""
         if name not in self._RNG:
             raise ValueError("rng must be 'philox' or 'minstd'")
         check(lib.cpm_set_rng(self._h, self._RNG[name]))

@@ -165,11 +165,16 @@ int cpm_get_cell_table(cpm_handle h, int32_t* area, int32_t* perim, int32_t* anc
  * keyed by (sub-block, MCS, phase, attempt); with positions_per_checkerboard
  * = 1 two x-neighbouring same-colour sub-blocks share one Philox block.  mode 1: the reference's stream
  * (random.cu:5-28 seeded per thread at kernels2d.cu:99 / kernels3d.cu:86),
- * for comparisons with the reference: one MINSTD state per region of
- * positions_per_thread sites per axis (2c with shared = 1, which then needs
- * positions_per_thread = 2c), seeded at every inner_iterations chunk from
- * the region's first site index + the low 32 bits of the seed, consumed in
- * the reference's draw order; acceptance u < ceil(exp(-E/T) (2^31 - 1)).
+ * for comparisons with the reference: one MINSTD state per reference
+ * thread of positions_per_thread sites per axis, seeded at every
+ * inner_iterations chunk from the thread's first site index + the low 32
+ * bits of the seed, consumed in the reference's draw order; acceptance
+ * u < ceil(exp(-E/T) (2^31 - 1)).  With shared = 1 and positions_per_thread
+ * > 2c the reference's layout is kept (kernels2d.cu:72-86,92-106,136-146):
+ * threads in blocks of threads_per_block per axis, each thread's
+ * positions_per_thread / 2c regions of 2c sites per axis visited one block
+ * slice at a time; threads_per_block x positions_per_thread must divide
+ * dimension (CPM_EINVAL otherwise).
  * A batch's engines must share the mode (CPM_EINVAL otherwise); the
  * persistent batch form does not run it (CPM_ENOTIMPL). */
 /* Extension: online autotuning of the per-phase kernel form (no counterpart

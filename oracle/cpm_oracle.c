@@ -493,14 +493,14 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                       const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                       uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
-                      int64_t* trace_n, int live, int64_t* counts);
+                      int64_t* trace_n, int live, int64_t* counts, int tpb);
 
 int orc_sweep_ex(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                  const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                  uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                  int ppt) {
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
-                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL, 0, NULL);
+                      nthreads, cell_sync, shared, ppt, 0, NULL, 0, NULL, 0, NULL, 1);
 }
 
 /* Live-table checkerboard: the same schedule, stream and decisions as
@@ -517,7 +517,7 @@ int orc_sweep_live(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
                    const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                    uint64_t seed, uint32_t mcs0, int nmcs, int shared, int ppt) {
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs, 1, 0,
-                      shared, ppt, 0, NULL, 0, NULL, 1, NULL);
+                      shared, ppt, 0, NULL, 0, NULL, 1, NULL, 1);
 }
 
 /* orc_sweep_ex that also counts, once per attempt (in the commit / apply pass
@@ -530,23 +530,30 @@ int orc_sweep_counts(int L, int d, int ncells, uint32_t* lat, int32_t* area, int
                      uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync, int shared,
                      int ppt, int64_t* counts) {
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs, nthreads,
-                      cell_sync, shared, ppt, 0, NULL, 0, NULL, 0, counts);
+                      cell_sync, shared, ppt, 0, NULL, 0, NULL, 0, counts, 1);
 }
 
 int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                         const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                         uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
-                        int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n) {
-    if (shared && ppt != 2 * c) return -9; /* the reference's shared mode with ppt > 2c loops regions per thread */
+                        int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n,
+                        int tpb) {
+    /* the reference's shared mode with ppt > 2c: global blocks of tpb threads
+     * per axis, each thread looping over ppt / 2c regions (kernels2d.cu:72-86,
+     * 92-106, cpm.cu:363) */
+    if (shared && ppt != 2 * c &&
+        (ppt < 2 * c || ppt % (2 * c) || tpb < 1 || L % ppt || (L / ppt) % tpb))
+        return -9;
     return sweep_impl(L, d, ncells, lat, area, perim, anchor, csum, p, c, u, seed, mcs0, nmcs,
-                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n, 0, NULL);
+                      trace ? 1 : nthreads, cell_sync, shared, ppt, 1, trace, trace_cap, trace_n, 0, NULL,
+                      tpb);
 }
 
 static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                       const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                       uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
                       int shared, int ppt, int refstream, int64_t* trace, int64_t trace_cap,
-                      int64_t* trace_n, int live, int64_t* counts) {
+                      int64_t* trace_n, int live, int64_t* counts, int tpb) {
     if (d != 2 && d != 3) return -1;
     /* Act with shared roles and c = 1: an attempt's act term reads the
      * source's neighbourhood, two sites from its target, where other targets
@@ -581,8 +588,23 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
     make_nbhd(d, &nb);
     const int N = nb.n;
     const int lc = log2i(c);
-    /* reference stream: one state per region, seeded per call */
-    const int nreg = L / R;
+    /* The reference's shared mode with ppt > 2c (reference stream): the
+     * lattice is cut into blocks of tpb threads per axis, each thread owning
+     * G = ppt / 2c regions of 2c sites per axis; a block's threads work on one
+     * slice of G^d at a time (blockx, blocky(, blockz), x outermost), so thread
+     * j of the colour phase in slice g works in region
+     * (j / tpb) G tpb + g tpb + j mod tpb (kernels2d.cu:72-86,92-106;
+     * kernels3d.cu:65-108).  One state per thread, seeded from its first
+     * site j ppt (kernels2d.cu:70-71,99). */
+    const int gb = refstream && shared && ppt > R;
+    const int G = gb ? ppt / R : 1;
+    const int tb = gb ? tpb : 1;
+    const int T = gb ? L / ppt : 0;  /* threads per axis */
+    const int ngb = d == 3 ? G * G * G : G * G;
+    /* reference stream: one state per thread (a region, or with gb a thread
+     * of ppt sites per axis), seeded per call */
+    const int nreg = gb ? T : L / R;
+    const int sreg = gb ? ppt : R;
     int32_t* rstate = NULL;
     if (refstream) {
         int64_t nr = d == 3 ? (int64_t)nreg * nreg * nreg : (int64_t)nreg * nreg;
@@ -590,14 +612,14 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
         for (int64_t ri = 0; ri < nr; ri++) {
             int rx = (int)(ri % nreg), ry = (int)((ri / nreg) % nreg);
             int rz = d == 3 ? (int)(ri / ((int64_t)nreg * nreg)) : 0;
-            uint32_t base = (uint32_t)site(L, rx * R, ry * R, rz * R); /* kernels2d.cu:99 */
+            uint32_t base = (uint32_t)site(L, rx * sreg, ry * sreg, rz * sreg); /* kernels2d.cu:99 */
             rstate[ri] = ref_next((int32_t)(base + (uint32_t)seed));
         }
         if (trace_n) *trace_n = 0;
     }
     const uint32_t cm = (uint32_t)c - 1;
     const int nsb = L / c;       /* sub-blocks per axis */
-    const int h = L / R / S;     /* same-phase sub-blocks per axis */
+    const int h = gb ? T : L / R / S;  /* same-phase sub-blocks per axis */
     const int64_t nq = d == 3 ? (int64_t)h * h * h : (int64_t)h * h;
     const int rounds = cd / u;   /* kernels2d.cu:95 */
     const int ncol = d == 3 ? K * K * K : K * K;   /* colours: K^d (kernels2d.cu:91) */
@@ -617,6 +639,10 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
     for (int m = 0; m < nmcs; m++) {
         uint32_t mcs = mcs0 + (uint32_t)m;
         for (int r = 0; r < rounds; r++) {
+            for (int gs = 0; gs < ngb; gs++) {  /* the global-block slice (one unless gb) */
+            const int gsx = d == 3 ? gs / (G * G) : gs / G;
+            const int gsy = d == 3 ? (gs / G) % G : gs % G;
+            const int gsz = d == 3 ? gs % G : 0;
             for (int pc = 0; pc < ncol * ngrp; pc++) {
                 /* colour (cbX outermost, kernels2d.cu:179-180), then group */
                 int col = pc / ngrp, grp = pc % ngrp;
@@ -626,7 +652,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                 int gx = d == 3 ? grp / (S * S) : grp / S;
                 int gy = d == 3 ? (grp / S) % S : grp % S;
                 int gz = d == 3 ? grp % S : 0;
-                uint32_t phase = (uint32_t)(r * ncol * ngrp + pc);
+                uint32_t phase = (uint32_t)((r * ngb + gs) * ncol * ngrp + pc);
                 /* the phase-start snapshot the attempts read (live: the tables themselves) */
                 const int32_t* aR = live ? area : areaR;
                 const int32_t* pR = live ? perim : perimR;
@@ -660,9 +686,17 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                     int bx = ((int)(qi % h) * S + gx) * K + cx;
                     int by = ((int)((qi / h) % h) * S + gy) * K + cy;
                     int bz = d == 3 ? ((int)(qi / ((int64_t)h * h)) * S + gz) * K + cz : 0;
+                    int64_t rid = d == 3 ? ((int64_t)(bz / K) * nreg + by / K) * nreg + bx / K
+                                         : (int64_t)(by / K) * nreg + bx / K;
+                    if (gb) {  /* thread j of slice gs: its region, and its own state */
+                        const int jx = (int)(qi % h), jy = (int)((qi / h) % h);
+                        const int jz = d == 3 ? (int)(qi / ((int64_t)h * h)) : 0;
+                        bx = ((jx / tb) * G * tb + gsx * tb + jx % tb) * K + cx;
+                        by = ((jy / tb) * G * tb + gsy * tb + jy % tb) * K + cy;
+                        bz = d == 3 ? ((jz / tb) * G * tb + gsz * tb + jz % tb) * K + cz : 0;
+                        rid = qi;
+                    }
                     uint32_t sb = (uint32_t)(((int64_t)bz * nsb + by) * nsb + bx);
-                    const int64_t rid = d == 3 ? ((int64_t)(bz / K) * nreg + by / K) * nreg + bx / K
-                                               : (int64_t)(by / K) * nreg + bx / K;
                     int32_t rs = refstream ? rstate[rid] : 0;
                     if (split && pass == 0) dec[qi] = 0;
                     for (int i = i0; i < (cell_sync ? i0 + 1 : u); i++) {
@@ -860,6 +894,7 @@ static int sweep_impl(int L, int d, int ncells, uint32_t* lat, int32_t* area, in
                 }
                 }
             }
+            }  /* gs */
         }
     }
     free(areaR);

@@ -208,13 +208,16 @@ int orc_sweep_live(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32
  * stream (one MINSTD state per region of R sites, a reference thread; see
  * cpm_oracle.c), one call = one reference launch: the states are seeded from
  * the regions' first-site indices + (int32) seed at entry.  Acceptance:
- * orc_ref_threshold_table.  shared = 1 needs ppt = 2c (-9 otherwise).  With
+ * orc_ref_threshold_table.  shared = 1 with ppt > 2c: the reference's global
+ * blocks of tpb threads per axis, one state per thread of ppt sites (-9 when
+ * 2c does not divide ppt or tpb ppt does not divide L).  With
  * trace != NULL the run is serial and every attempt made appends (region,
  * target site, source site, acceptance draw or -1) to trace[4 * trace_cap]. */
 int orc_sweep_refstream(int L, int d, int ncells, uint32_t* lat, int32_t* area, int32_t* perim,
                         const int32_t* anchor, int64_t* csum, const orc_params* p, int c, int u,
                         uint64_t seed, uint32_t mcs0, int nmcs, int nthreads, int cell_sync,
-                        int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n);
+                        int shared, int ppt, int64_t* trace, int64_t trace_cap, int64_t* trace_n,
+                        int tpb);
 int orc_ref_threshold_table(double temperature, int scale, uint32_t* out, int cap);
 
 /* Serial random-sequential CPM with live tables (statistical comparisons

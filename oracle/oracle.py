@@ -134,7 +134,7 @@ def lib():
         L.orc_sweep_live.argtypes = L.orc_sweep.argtypes[:-1] + [ctypes.c_int, ctypes.c_int]
         L.orc_sweep_live.restype = ctypes.c_int
         L.orc_sweep_refstream.argtypes = L.orc_sweep_ex.argtypes + [
-            P(ctypes.c_int64), ctypes.c_int64, P(ctypes.c_int64)]
+            P(ctypes.c_int64), ctypes.c_int64, P(ctypes.c_int64), ctypes.c_int]
         L.orc_sweep_refstream.restype = ctypes.c_int
         L.orc_ref_threshold_table.argtypes = [ctypes.c_double, ctypes.c_int, u32p, ctypes.c_int]
         L.orc_ref_threshold_table.restype = ctypes.c_int
@@ -318,13 +318,15 @@ def schedule(L: int, d: int, c: int, shared: bool, ppt: int):
 
 def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, seed: int,
           mcs0: int, nmcs: int, nthreads: int = 1, cell_sync: bool = False, shared: bool = True,
-          ppt: int = None, rng: str = "philox", trace_cap: int = 0, counts: np.ndarray = None):
+          ppt: int = None, rng: str = "philox", trace_cap: int = 0, counts: np.ndarray = None,
+          tpb: int = 4):
     """Run nmcs MCS in place on lat (uint32 view) and tables.  shared selects
     the roles of the run() kwarg of that name; ppt is positions_per_thread
     (used without shared roles).  rng = "minstd" uses the reference's
     per-thread MINSTD stream (one call = one reference launch: the streams
     are seeded at entry); with trace_cap > 0 the attempts' (region, target,
-    source, acceptance draw or -1) rows are returned.  counts (int64[4],
+    source, acceptance draw or -1) rows are returned (tpb: threads_per_block,
+    which places the regions with shared roles and ppt > 2c).  counts (int64[4],
     added to): attempts with differing source and target, evaluated,
     accepted, committed (counter stream only)."""
     assert lat.dtype in (np.uint32, np.int32) and lat.flags["C_CONTIGUOUS"]
@@ -347,7 +349,7 @@ def sweep(lat: np.ndarray, d: int, tables: Tables, p: Params, c: int, u: int, se
         trace = np.zeros((max(trace_cap, 1), 4), np.int64)
         n = ctypes.c_int64(0)
         rc = lib().orc_sweep_refstream(*args, _ptr(trace, ctypes.c_int64) if trace_cap else None,
-                                       int(trace_cap), ctypes.byref(n))
+                                       int(trace_cap), ctypes.byref(n), int(tpb))
         trace = trace[:n.value] if trace_cap else None
     else:
         raise ValueError("rng must be 'philox' or 'minstd'")

@@ -40,13 +40,14 @@ def oracle_params(wl) -> "O.Params":
 
 
 def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=None, shared=True,
-               ppt=None, rng="philox", cell_sync=False, counts=None):
+               ppt=None, rng="philox", cell_sync=False, counts=None, tpb=4):
     """Tally + sweep in the oracle; returns (lattice, Tables).  With a 3D
     persistence term the run is cut into chunks of ``inner`` MCS with a
     preferred-direction update after each (tables.pref holds the result);
     with rng="minstd" each chunk is one reference launch (its streams are
-    seeded at the chunk's start).  counts (int64[4]) collects the attempt
-    counters (counter stream only)."""
+    seeded at the chunk's start; tpb is threads_per_block, which places the
+    regions with shared roles and ppt > 2c).  counts (int64[4]) collects the
+    attempt counters (counter stream only)."""
     lat = np.ascontiguousarray(wl.lattice if lattice is None else lattice).astype(np.uint32).copy()
     tab = O.tally(lat, wl.d, wl.ncells)
     p = oracle_params(wl)
@@ -66,14 +67,14 @@ def run_oracle(wl, c, u, seed, nmcs, mcs0=0, nthreads=8, lattice=None, inner=Non
         inner = inner or nmcs
         for m in range(0, nmcs, inner):
             O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0 + m, inner, nthreads, shared=shared, ppt=ppt,
-                    rng=rng, cell_sync=cell_sync, counts=counts)
+                    rng=rng, cell_sync=cell_sync, counts=counts, tpb=tpb)
             ps.update(tab, ctype, pw["pers_diff"])
         tab.pref = ps.pref
     elif nmcs and rng == "minstd":
         inner = inner or nmcs
         for m in range(0, nmcs, inner):
             O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0 + m, inner, nthreads, shared=shared, ppt=ppt,
-                    rng=rng, cell_sync=cell_sync)
+                    rng=rng, cell_sync=cell_sync, tpb=tpb)
     elif nmcs:
         O.sweep(lat, wl.d, tab, p, c, u, seed, mcs0, nmcs, nthreads, shared=shared, ppt=ppt,
                 cell_sync=cell_sync, counts=counts)

@@ -35,15 +35,20 @@ def int32(v: int) -> int:
     return v - (1 << 32) if v >= 1 << 31 else v
 
 
-def reference_thread_streams(lat, d, c, u, ppt, shared, seed32, nmcs, fixed):
+def reference_thread_streams(lat, d, c, u, ppt, shared, seed32, nmcs, fixed, tpb=4):
     """{thread index: [(target, source, acceptance draw or -1), ...]} of one
-    launch of nmcs MCS, for a lattice that no attempt changes."""
+    launch of nmcs MCS, for a lattice that no attempt changes.  Threads are
+    laid out as the reference lays them out: L / ppt per axis in blocks of
+    tpb (threads_per_block, cpm.cu:363); with shared roles each thread's ppt
+    sites are G = ppt / 2c regions of 2c sites, visited one block slice at a
+    time (kernels2d.cu:72-86,92-106,136-146; kernels3d.cu:65-108)."""
     L = lat.shape[0]
     flat = lat.reshape(-1)
     R = 2 * c if shared else ppt
     K = R // c
+    G = ppt // R  # global blocks (1 without shared roles)
     mult = c ** d // u
-    nt = L // R
+    nt = L // ppt
 
     def index(x, y, z):  # kernels.cu:20-42
         return (z * L + y) * L + x if d == 3 else y * L + x
@@ -52,50 +57,57 @@ def reference_thread_streams(lat, d, c, u, ppt, shared, seed32, nmcs, fixed):
         seed = next_random(seed)
         return (x + seed % 3 - 1) & (L - 1), seed
 
+    def region_base(t, g):  # kernels2d.cu:136-146
+        return (t // tpb) * G * tpb * R + (t % tpb) * R + g * tpb * R
+
     out = {}
     for t in range(nt ** d):
         tx, ty, tz = t % nt, (t // nt) % nt, (t // (nt * nt) if d == 3 else 0)
-        bx, by, bz = tx * R, ty * R, tz * R
-        seed = next_random(int32(index(bx, by, bz) + seed32))
+        seed = next_random(int32(index(tx * ppt, ty * ppt, tz * ppt) + seed32))
         ev = []
         for _ in range(nmcs):
             for _ in range(mult):
-                for cx in range(K):
-                    for cy in range(K):
-                        for cz in (range(K) if d == 3 else (0,)):
-                            for _ in range(u):
-                                seed = next_random(seed)
-                                a = seed % c
-                                seed = next_random(seed)
-                                b = seed % c
-                                cc = 0
-                                if d == 3:
-                                    seed = next_random(seed)
-                                    cc = seed % c
-                                px, py, pz = bx + cx * c + a, by + cy * c + b, bz + cz * c + cc
-                                while True:
-                                    qx, seed = neighbour(px, seed)
-                                    qy, seed = neighbour(py, seed)
-                                    qz = 0
-                                    if d == 3:
-                                        qz, seed = neighbour(pz, seed)
-                                    if d == 2:
-                                        rej = qx == px and qy == py
-                                    elif shared:
-                                        rej = qx == px and qy == py and qz == pz
-                                    else:  # kernels3d.cu:208 compares z with y
-                                        rej = qx == px and qy == py and qz == py
-                                    if not rej:
-                                        break
-                                if shared:  # drawn site = target
-                                    tgt, src = index(px, py, pz), index(qx, qy, qz)
-                                else:       # drawn site = source
-                                    tgt, src = index(qx, qy, qz), index(px, py, pz)
-                                draw = -1
-                                if flat[src] != flat[tgt] and not fixed[int(flat[tgt]) >> 24]:
-                                    seed = next_random(seed)
-                                    draw = seed
-                                ev.append((tgt, src, draw))
+                for gx in range(G):
+                    for gy in range(G):
+                        for gz in (range(G) if d == 3 else (0,)):
+                            bx, by = region_base(tx, gx), region_base(ty, gy)
+                            bz = region_base(tz, gz) if d == 3 else 0
+                            for cx in range(K):
+                                for cy in range(K):
+                                    for cz in (range(K) if d == 3 else (0,)):
+                                        for _ in range(u):
+                                            seed = next_random(seed)
+                                            a = seed % c
+                                            seed = next_random(seed)
+                                            b = seed % c
+                                            cc = 0
+                                            if d == 3:
+                                                seed = next_random(seed)
+                                                cc = seed % c
+                                            px, py, pz = bx + cx * c + a, by + cy * c + b, bz + cz * c + cc
+                                            while True:
+                                                qx, seed = neighbour(px, seed)
+                                                qy, seed = neighbour(py, seed)
+                                                qz = 0
+                                                if d == 3:
+                                                    qz, seed = neighbour(pz, seed)
+                                                if d == 2:
+                                                    rej = qx == px and qy == py
+                                                elif shared:
+                                                    rej = qx == px and qy == py and qz == pz
+                                                else:  # kernels3d.cu:208 compares z with y
+                                                    rej = qx == px and qy == py and qz == py
+                                                if not rej:
+                                                    break
+                                            if shared:  # drawn site = target
+                                                tgt, src = index(px, py, pz), index(qx, qy, qz)
+                                            else:       # drawn site = source
+                                                tgt, src = index(qx, qy, qz), index(px, py, pz)
+                                            draw = -1
+                                            if flat[src] != flat[tgt] and not fixed[int(flat[tgt]) >> 24]:
+                                                seed = next_random(seed)
+                                                draw = seed
+                                            ev.append((tgt, src, draw))
         out[t] = ev
     return out
 
@@ -117,20 +129,28 @@ def _distinct_lattice(L, d, types=(1, 2)):
     return ((t << np.uint32(24)) | ids).reshape((L,) * d), n
 
 
-CASES = [  # d, L, c, u, shared, ppt
-    (2, 16, 2, 1, False, 4),
-    (2, 16, 2, 3, False, 8),
-    (2, 16, 1, 1, True, 2),
-    (2, 16, 2, 4, True, 4),
-    (3, 8, 1, 1, False, 2),
-    (3, 8, 2, 3, False, 4),
-    (3, 8, 1, 1, True, 2),
+CASES = [  # d, L, c, u, shared, ppt, threads_per_block
+    (2, 16, 2, 1, False, 4, 4),
+    (2, 16, 2, 3, False, 8, 4),
+    (2, 16, 1, 1, True, 2, 4),
+    (2, 16, 2, 4, True, 4, 4),
+    (3, 8, 1, 1, False, 2, 4),
+    (3, 8, 2, 3, False, 4, 4),
+    (3, 8, 1, 1, True, 2, 4),
+    # shared roles with ppt > 2c: each thread loops over ppt / 2c regions per
+    # axis, placed by threads_per_block
+    (2, 16, 1, 1, True, 4, 1),
+    (2, 16, 1, 1, True, 4, 2),
+    (2, 32, 2, 2, True, 8, 2),
+    (2, 32, 2, 4, True, 8, 4),
+    (3, 8, 1, 1, True, 4, 2),
+    (3, 16, 1, 1, True, 8, 1),
 ]
 
 
-@pytest.mark.parametrize("d,L,c,u,shared,ppt", CASES)
+@pytest.mark.parametrize("d,L,c,u,shared,ppt,tpb", CASES)
 @pytest.mark.parametrize("lattice", ["uniform", "distinct", "distinct_fixed"])
-def test_oracle_refstream_equals_thread_major_restatement(d, L, c, u, shared, ppt, lattice):
+def test_oracle_refstream_equals_thread_major_restatement(d, L, c, u, shared, ppt, tpb, lattice):
     seed = 0x7FFF1234 if lattice == "uniform" else -19
     if lattice == "uniform":
         lat = np.full((L,) * d, (1 << 24) | 1, np.uint32)
@@ -141,10 +161,10 @@ def test_oracle_refstream_equals_thread_major_restatement(d, L, c, u, shared, pp
         p, fixed = _frozen_params(2 if lattice == "distinct_fixed" else None)
     tab = O.tally(lat, d, ncells)
     nmcs = 2
-    ref = reference_thread_streams(lat, d, c, u, ppt, shared, int32(seed), nmcs, fixed)
+    ref = reference_thread_streams(lat, d, c, u, ppt, shared, int32(seed), nmcs, fixed, tpb)
     nev = sum(len(v) for v in ref.values())
     trace = O.sweep(lat, d, tab, p, c, u, seed & 0xFFFFFFFF, 0, nmcs, shared=shared, ppt=ppt,
-                    rng="minstd", trace_cap=nev + 8)
+                    rng="minstd", trace_cap=nev + 8, tpb=tpb)
     assert len(trace) == nev
     got = {}
     for rid, t, s, w in trace.tolist():
