@@ -67,7 +67,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(wl, c, u, seconds):
+def cpu_baseline(wl, c, u, seconds, shared=True, ppt=None):
     """Time the CPU oracle (OpenMP over all host cores) on a bounded sample of
     the same workload, from the same initial lattice."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -81,23 +81,24 @@ def cpu_baseline(wl, c, u, seconds):
     cores = len(os.sched_getaffinity(0))
     lat = wl.lattice.copy()
     tab = O.tally(lat, wl.d, wl.ncells)
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, 0, 1, cores)  # thread pool start-up
+    sk = dict(shared=shared, ppt=ppt)
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 0, 1, cores, **sk)  # thread pool start-up
     t = time.perf_counter()
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, 1, 2, cores)  # calibration
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 1, 2, cores, **sk)  # calibration
     per_mcs = max((time.perf_counter() - t) / 2, 1e-6)
     n = max(2, int(seconds / per_mcs))
     t = time.perf_counter()
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, 3, n, cores)
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, 3, n, cores, **sk)
     dt = time.perf_counter() - t
     sites = wl.L ** wl.d
     # The same sweep on one core, continuing from where the sample ended.
     m = n + 3
     t = time.perf_counter()
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, m, 1, 1)
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, m, 1, 1, **sk)
     per_mcs1 = max(time.perf_counter() - t, 1e-6)
     n1 = max(1, int(seconds / 3 / per_mcs1))
     t = time.perf_counter()
-    O.sweep(lat, wl.d, tab, prm, c, u, 1, m + 1, n1, 1)
+    O.sweep(lat, wl.d, tab, prm, c, u, 1, m + 1, n1, 1, **sk)
     dt1 = time.perf_counter() - t
     return {"value": n * sites / dt, "unit": "site-updates/s", "cores": cores, "kind": "port",
             "sample": f"{n} MCS of the same {wl.L}^{wl.d} workload (MCS 3..{n + 2}) from its "
@@ -116,7 +117,7 @@ def storage_of(kernel):
 
 
 def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_cpu, autotune, log,
-                   launch=-1):
+                   launch=-1, shared=1, ppt=None):
     """Whole-run throughput of one engine (inputs resident before timing),
     the sweep kernel's roofline from per-launch HIP events, and the CPU
     oracle on a bounded sample of the same workload."""
@@ -125,11 +126,12 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     sim.set_seed(20240601)
     sim.set_autotune(autotune)
     sim.set_launch(launch)
-    # shared = 1: the reference's shared-memory schedule (K = 2 colours)
+    # shared = 1: the reference's shared-memory schedule (K = 2 colours);
+    # shared = 0: K = ppt / c colours (the reference's default)
     kw = dict(cell_sync=0, block_sync=0, global_sync=1, threads_per_block=4,
-              positions_per_thread=2 * c, positions_per_checkerboard=c,
+              positions_per_thread=ppt or 2 * c, positions_per_checkerboard=c,
               updates_per_checkerboard_switch=u, updates_per_barrier=1,
-              iterations=mcs_per_step, inner_iterations=mcs_per_step, shared=1)
+              iterations=mcs_per_step, inner_iterations=mcs_per_step, shared=shared)
     log(f"{name} L={wl.L} d={wl.d} cells={wl.ncells} c={c} u={u}: warmup {warmup} x {mcs_per_step} MCS")
     for _ in range(warmup):
         sim.run(**kw)
@@ -152,7 +154,7 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     avg_launch_s = kernel_ms / 1e3 / launches
     bytes_per_launch = updates / launches * BYTES_PER_UPDATE
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = _profiled_traffic(launch["kernel"], name) if c == 1 and u == 1 else None
+    traffic = _profiled_traffic(launch["kernel"], name) if c == 1 and u == 1 and shared else None
     out = {
         "value": mcs * sites / wall,
         "unit": "site-updates/s",
@@ -168,7 +170,7 @@ def measure_single(wl, name, c, u, steps, warmup, mcs_per_step, cpu_seconds, no_
     }
     if not no_cpu:
         log(f"{name} CPU oracle sample (~{cpu_seconds:.0f} s)")
-        out["cpu_baseline"] = cpu_baseline(wl, c, u, cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(wl, c, u, cpu_seconds, shared=bool(shared), ppt=ppt)
     return out
 
 
@@ -668,6 +670,14 @@ def main():
         r3 = measure_single(w3, "plain3d", 1, 1, 5, 1, 20, 6.0, a.no_cpu, a.autotune, log)
         r3["workload"] = "3D CPM 256^3, 20000 cubes 5^3, 26-neighbourhood (1000-MCS config; 100 MCS timed)"
         configs["C3_plain3d"] = r3
+        # the C2 lattice under the reference's own sorting.py schedule
+        # (examples/sorting.py:55-63: shared = 0, positions_per_thread 4, c 2:
+        # 64 colour phases per MCS)
+        rs = measure_single(workloads.sorting2d(), "sorting2d", 2, 1, 5, 1, 50, 6.0, a.no_cpu, a.autotune,
+                            log, shared=0, ppt=4)
+        rs["workload"] = ("2D cell sorting 1024^2, 3969 cells, the reference sorting.py schedule: shared=0, "
+                          "positions_per_thread 4, positions_per_checkerboard 2 (64 colour phases per MCS)")
+        configs["C2_sorting_py_schedule"] = rs
         configs["C4_batched"] = measure_batched(64, 2000, log, a.autotune)
         r5 = measure_gradient(32, 512, 2000, log, fit=not a.no_grad)
         r5["workload"] = ("32 replicas of 512^2 two-type sorting (C2 parameters): forward 2000 MCS "
