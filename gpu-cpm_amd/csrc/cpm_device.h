@@ -178,17 +178,30 @@ struct SmallArgs {
 };
 
 // The resident form (2D, shared roles, c = 1, u = 1): one cooperative block
-// per tile of 2^lte x 2^lte sites, 2^lnt tiles per axis, for one chunk.
-constexpr int kResFlagStride = 16;  // words between tile counters (one per 64 bytes)
+// per tile of 2^lte x 2^lte sites, 2^lnt tiles per axis, for one chunk.  A
+// tile keeps its sites, a one-site halo and the rows of the cells it holds
+// in LDS; neighbouring tiles exchange border sites and the deltas of the
+// cells both hold through tagged mailbox words.
+constexpr int kResSlots = 254;   // cells one tile holds at most (slots 254, 255: medium, none)
+constexpr int kResDirCap = 64;   // cells one tile shares with one neighbour at most
+__host__ __device__ inline int res_mail_stride(int te) { return te + kResDirCap; }
+// Why a chunk failed the guard (bits of ResidentArgs::bad): a cell's widened
+// box spans more than two tiles per axis; a tile holds more than kResSlots
+// cells; it shares more than kResDirCap with one neighbour; an attempt met a
+// cell its tile does not hold.
+constexpr unsigned int kResGuardWide = 1, kResGuardSlots = 2, kResGuardShared = 4, kResGuardMiss = 8;
+// Sites a cell's box is widened by (beyond the halo) for its growth within a
+// chunk: a quarter tile.
+__host__ __device__ inline int res_margin(int te) { return te / 4; }
 struct ResidentArgs {
     SweepConsts k;
     uint32_t mcs0;
     int nmcs;
     int lte, lnt;
-    unsigned int* flags;     // [tiles * kResFlagStride] phase counters, 0 on entry
-    unsigned int* err;       // abandoned-wait flag, 0 on entry
-    int32_t* bbox;           // [4 (n+1)] per-cell box of the chunk's sites (the guard)
-    const int32_t* anchor;   // [3 (n+1)] cell anchors
+    unsigned long long* mail;  // [tiles][2][8][res_mail_stride(2^lte)] tagged words, 0 on entry
+    const uint32_t* holders;   // [n + 1] the tiles holding each cell (0: none)
+    unsigned int* err;         // abandoned-wait flag, 0 on entry
+    unsigned int* bad;         // guard flag: a cell outside its tiles, too many cells
 };
 
 // One replica of a batched run.
@@ -244,13 +257,20 @@ hipError_t launch_persist(int d, bool lds2d, const PersistArgs& a, int blocks, s
 // (one block; e0/e1 or null are recorded at the dispatch's start and end).
 size_t small_lds_bytes(int d, int lgL, int ncells, bool narrow);
 hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
-size_t resident_lds_bytes(int lte, bool narrow);
-hipError_t resident_max_blocks(bool narrow, int lte, int* max_blocks);
-hipError_t launch_resident(bool narrow, const ResidentArgs& a, int blocks, hipStream_t s);
-// begin: bbox := the boxes of the current lattice's cells; end: *bad |= 1 if
-// a cell's box (widened by the halo) spans more than two tiles per axis.
-hipError_t launch_resident_guard(bool begin, bool narrow, const SweepConsts& k, const int32_t* anchor,
-                                 int32_t* bbox, int te, unsigned int* bad, hipStream_t s);
+// The resident form: dynamic LDS bytes (the cell map), co-resident block
+// limit and launch; var selects the kernel: 0 the lean attempt path (at
+// most four types, no fixed types, chemotaxis or counters, a threshold
+// table of at most kResThrLean entries), 1 the general one, 2 the general
+// one with the chemotaxis term.
+constexpr int kResThrLean = 1024;
+size_t resident_lds_bytes(int ncells);
+hipError_t resident_max_blocks(bool narrow, int lte, int var, int ncells, int* max_blocks);
+hipError_t launch_resident(bool narrow, int var, const ResidentArgs& a, int blocks, hipStream_t s);
+// Before a chunk: the boxes of the current lattice's cells into bbox, and
+// the holder word of each cell; *bad |= 1 if a cell's widened box spans
+// more than two tiles per axis.
+hipError_t launch_resident_guard(bool narrow, const SweepConsts& k, const int32_t* anchor, int32_t* bbox,
+                                 int lte, int lnt, uint32_t* holders, unsigned int* bad, hipStream_t s);
 // *p = v / *p += v (one thread: nodes of a captured graph, no memset nodes)
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 hipError_t launch_add_u32(uint32_t* p, uint32_t v, hipStream_t s);

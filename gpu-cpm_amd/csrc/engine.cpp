@@ -64,6 +64,9 @@ constexpr size_t kSmallLdsBudget = 158 * 1024;
 constexpr long kSmallAutoAttempts = 4096;
 // Batch graph replay: the largest number of MCS captured in one graph.
 constexpr long kGraphMcs = 64;
+// Resident form: the shortest launch (MCS) the adaptive launch length goes
+// down to after guard failures.
+constexpr int64_t kResMinMcs = 4;
 
 // Per-phase kernel options (cpm_set_kernel_options); -1 keeps the default.
 // Every option computes the same results.
@@ -302,12 +305,14 @@ struct cpm_engine {
     int scale = 1;
     int nt = 1;  // types present in the lattice at the last push
     int last_kernel = -1;  // per-phase kernel of the last run (gcpm::phase_kernel_name)
-    bool small_narrow = false;  // the last single-workgroup run used 16-bit sites
-    // Resident form (2D): tile counters + abandoned-wait flag + guard flag,
-    // per-cell boxes, and the copy of lattice and table a failed chunk is
-    // restored from.
+    bool small_narrow = false;  // the last single-workgroup or resident run used 16-bit sites
+    int res_var = 0;            // the last resident run's kernel variant (gcpm::launch_resident)
+    // Resident form (2D): abandoned-wait flag, guard flag and the cells'
+    // holder words; the tiles' mailboxes; per-cell boxes; and the copy of
+    // lattice and table a failed chunk is restored from.
     unsigned int* d_res_flags = nullptr;
-    size_t res_flags_cap = 0;
+    unsigned long long* d_res_mail = nullptr;
+    size_t res_mail_cap = 0;
     int32_t* d_bbox = nullptr;
     void* d_backup_lat = nullptr;
     int64_t* d_backup_tab = nullptr;
@@ -318,7 +323,9 @@ struct cpm_engine {
     unsigned long long* d_counters = nullptr;
     bool counters_on = false;
     uint64_t ctr_mcs0 = 0, ctr_mcs_acc = 0;
-    int64_t res_fallbacks = 0;      // chunks redone with one launch per phase: the guard
+    int64_t res_mcs = 0;            // resident form: MCS per launch after a guard failure (0: the chunk)
+    int res_streak = 0;             // launches in a row that held since K last changed
+    int64_t res_fallbacks = 0;      // launches redone with one launch per phase: the guard
     int64_t res_wait_fallbacks = 0; // ... or an abandoned neighbour wait
     bool autotune = false;  // cpm_set_autotune
     FormTuner tuner;
@@ -374,6 +381,7 @@ struct cpm_engine {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (ev_ready) (void)hipEventDestroy(ev_ready);
         (void)hipFree(d_res_flags);
+        (void)hipFree(d_res_mail);
         (void)hipFree(d_bbox);
         (void)hipFree(d_counters);
         (void)hipFree(d_backup_lat);
@@ -1354,20 +1362,26 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         }
     }
     // The resident form (2D, shared roles, c = 1, u = 1): launch form 4.
-    int res_lte = 0;
+    int res_lte = 0, res_var = 0;
     if (h->want_form == Plan::kResident) {
         const bool ok = h->d == 2 && !k.roles && k.lc == 0 && u == 1 && !a->cell_sync && !k.refstream &&
-                        !k.act && h->L >= 64;
+                        !k.act && h->L >= 128;
         if (!ok)
             return fail(CPM_EINVAL,
                         "launch form 4 (resident tiles) needs 2D, shared = 1, positions_per_checkerboard = 1, "
-                        "updates_per_checkerboard_switch = 1, L >= 64, and no cell_sync, reference stream or "
+                        "updates_per_checkerboard_switch = 1, L >= 128, and no cell_sync, reference stream or "
                         "act term");
-        // tiles of 32 or 64 sites: up to 256 of them at L <= 1024, and wide
-        // enough that cells a few tens of sites across keep within two
+        // tiles of 32 or 64 sites, at least four per axis: up to 256 of
+        // them at L <= 1024, and wide enough that cells a few tens of sites
+        // across keep within two
         res_lte = std::min(6, std::max(5, h->lgL - 4));
+        res_var = k.chem ? 2
+                         : (k.nt <= gcpm::kSmallTypes && !k.fixed_mask && !h->counters_on &&
+                            k.nthr <= gcpm::kResThrLean)
+                               ? 0
+                               : 1;
         int maxb = 0;
-        HIP_TRY(gcpm::resident_max_blocks(h->narrow_bits() != 0, res_lte, &maxb));
+        HIP_TRY(gcpm::resident_max_blocks(h->narrow_bits() != 0, res_lte, res_var, h->n, &maxb));
         const int tiles = 1 << (2 * (h->lgL - res_lte));
         if (tiles > maxb) return fail(CPM_EINVAL, "launch form 4: the tiles do not fit co-resident");
         plan = Plan{};
@@ -1392,85 +1406,115 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             h->narrow_valid = false;
         }
         const int tiles = plan.blocks;
-        const size_t nflag = (size_t)tiles * gcpm::kResFlagStride + 2;  // counters, err, guard
         const size_t rows = (size_t)h->n + 1;
         const size_t lat_bytes = h->sites * (bits ? 2 : 4);
-        if (nflag > h->res_flags_cap) {
-            (void)hipFree(h->d_res_flags);
-            h->d_res_flags = nullptr;
-            h->res_flags_cap = 0;
-            HIP_TRY(hipMalloc(&h->d_res_flags, sizeof(unsigned int) * (nflag + 1)));
-            h->res_flags_cap = nflag;
+        const size_t nmail = (size_t)tiles * 16 * gcpm::res_mail_stride(1 << res_lte);
+        if (nmail > h->res_mail_cap) {
+            (void)hipFree(h->d_res_mail);
+            h->d_res_mail = nullptr;
+            h->res_mail_cap = 0;
+            HIP_TRY(hipMalloc(&h->d_res_mail, sizeof(unsigned long long) * nmail));
+            h->res_mail_cap = nmail;
         }
+        if (!h->d_res_flags) HIP_TRY(hipMalloc(&h->d_res_flags, sizeof(unsigned int) * (2 + rows)));
         if (!h->d_bbox) HIP_TRY(hipMalloc(&h->d_bbox, sizeof(int32_t) * 4 * rows));
         if (!h->d_backup_lat) HIP_TRY(hipMalloc(&h->d_backup_lat, h->sites * 4));
         if (!h->d_backup_tab) HIP_TRY(hipMalloc(&h->d_backup_tab, sizeof(int64_t) * rows));
-        unsigned int* err = h->d_res_flags + (size_t)tiles * gcpm::kResFlagStride;
+        unsigned int* err = h->d_res_flags;
         unsigned int* bad = err + 1;
         ra.lte = res_lte;
         ra.lnt = h->lgL - res_lte;
         ra.nmcs = a->inner_iterations;
-        ra.flags = h->d_res_flags;
+        ra.mail = h->d_res_mail;
+        ra.holders = h->d_res_flags + 2;
         ra.err = err;
-        ra.bbox = h->d_bbox;
-        ra.anchor = h->d_anchor;
+        ra.bad = bad;
         void* cur_lat = bits ? (void*)h->d_lat16 : (void*)h->d_lat;
-        for (long ch = 0; ch < chunks; ch++) {
-            ra.mcs0 = (uint32_t)(h->mcs + (uint64_t)ch * a->inner_iterations);
-            // the state a failed chunk is redone from
+        // Launches of up to K MCS (the results do not depend on how a run is
+        // split).  K starts at the run's inner_iterations; a launch that
+        // fails its guard (a cell met outside the tiles holding it, or wider
+        // than two tiles) is redone with one launch per phase and halves K
+        // for this engine; four launches in a row that hold double it again.
+        const uint64_t base = h->mcs;
+        const int64_t kmax = a->inner_iterations;
+        int64_t K = h->res_mcs > 0 ? std::min<int64_t>(h->res_mcs, kmax) : kmax;
+        uint64_t done = 0;
+        while (done < (uint64_t)total) {
+            const int64_t n = std::min<int64_t>(K, (int64_t)((uint64_t)total - done));
+            ra.mcs0 = (uint32_t)(base + done);
+            ra.nmcs = (int)n;
+            // the state a failed launch is redone from
             HIP_TRY(hipMemcpyAsync(h->d_backup_lat, cur_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
             HIP_TRY(hipMemcpyAsync(h->d_backup_tab, h->d_tab[0], sizeof(int64_t) * rows, hipMemcpyDeviceToDevice,
                                    h->stream));
             if (h->counters_on)
                 HIP_TRY(hipMemcpyAsync(h->d_counters + 4, h->d_counters, 4 * sizeof(unsigned long long),
                                        hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(gcpm::launch_fill_zero(reinterpret_cast<unsigned long long*>(h->d_res_flags), (nflag + 1) / 2,
-                                           h->stream));
-            HIP_TRY(gcpm::launch_resident_guard(true, bits != 0, ra.k, h->d_anchor, h->d_bbox, 1 << res_lte, bad,
-                                                h->stream));
+            HIP_TRY(gcpm::launch_fill_zero(h->d_res_mail, nmail, h->stream));
+            HIP_TRY(gcpm::launch_fill_zero(reinterpret_cast<unsigned long long*>(h->d_res_flags), 1, h->stream));
+            HIP_TRY(gcpm::launch_resident_guard(bits != 0, ra.k, h->d_anchor, h->d_bbox, ra.lte, ra.lnt,
+                                                h->d_res_flags + 2, bad, h->stream));
             hipEvent_t e0, e1;
             if (int rc = h->next_events(&e0, &e1)) return rc;
             if (e0) HIP_TRY(hipEventRecord(e0, h->stream));
-            HIP_TRY(gcpm::launch_resident(bits != 0, ra, tiles, h->stream));
+            HIP_TRY(gcpm::launch_resident(bits != 0, res_var, ra, tiles, h->stream));
             if (e1) HIP_TRY(hipEventRecord(e1, h->stream));
-            HIP_TRY(gcpm::launch_resident_guard(false, bits != 0, ra.k, h->d_anchor, h->d_bbox, 1 << res_lte, bad,
-                                                h->stream));
             unsigned int fl[2] = {0, 0};
             HIP_TRY(hipMemcpyAsync(fl, err, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
             if (bits) h->wide_valid = false;
             if (fl[0] || fl[1]) {
-                // A cell spread over more than two tiles per axis (or a wait
-                // was abandoned): the chunk's result does not stand.  Restore
-                // it and run the rest of the run with one launch per phase.
+                // The launch's result does not stand: restore its start and
+                // make its MCS with one launch per phase.
                 HIP_TRY(hipMemcpyAsync(cur_lat, h->d_backup_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
                 for (int i = 0; i < 2; i++)
                     HIP_TRY(hipMemcpyAsync(h->d_tab[i], h->d_backup_tab, sizeof(int64_t) * rows,
                                            hipMemcpyDeviceToDevice, h->stream));
-                if (h->counters_on)  // the chunk is counted when it is redone
+                if (h->counters_on)  // the MCS are counted when they are redone
                     HIP_TRY(hipMemcpyAsync(h->d_counters, h->d_counters + 4, 4 * sizeof(unsigned long long),
                                            hipMemcpyDeviceToDevice, h->stream));
                 HIP_TRY(hipStreamSynchronize(h->stream));
-                if (fl[0])
-                    h->res_wait_fallbacks++;
-                else
+                if (fl[1]) {
                     h->res_fallbacks++;
-                h->mcs += (uint64_t)ch * a->inner_iterations;
-                cpm_run_args rest = *a;
-                rest.iterations = (int)((chunks - ch) * a->inner_iterations);
+                    K = std::max<int64_t>(kResMinMcs, n / 2);
+                    h->res_mcs = K;
+                    h->res_streak = 0;
+                } else {
+                    h->res_wait_fallbacks++;
+                }
+                h->mcs = base + done;
+                cpm_run_args part = *a;
+                part.iterations = (int)n;
+                part.inner_iterations = (int)n;
                 const int saved = h->want_form;
                 h->want_form = Plan::kPhase;
-                const int rc = cpm_run(h, &rest);
+                const int rc = cpm_run(h, &part);
                 h->want_form = saved;
-                return rc;
+                if (rc) return rc;
+                // (the per-phase launches leave the 32-bit or the 16-bit copy
+                // current, as any run; the next launch converts as needed)
+                if (bits) {
+                    if (int rc2 = h->ensure_narrow(bits)) return rc2;
+                } else {
+                    if (int rc2 = h->ensure_wide()) return rc2;
+                }
+            } else {
+                h->last_plan = plan;
+                if (h->timing) {
+                    h->t_launches++;
+                    h->t_updates += updates_per_mcs * n;
+                }
+                if (K < kmax && ++h->res_streak >= 4) {
+                    K = std::min<int64_t>(2 * K, kmax);
+                    h->res_mcs = K;
+                    h->res_streak = 0;
+                }
             }
-            if (h->timing) {
-                h->t_launches++;
-                h->t_updates += updates_per_mcs * a->inner_iterations;
-            }
+            done += (uint64_t)n;
         }
         h->small_narrow = bits != 0;
-        h->mcs += (uint64_t)total;
+        h->res_var = res_var;
+        h->mcs = base + (uint64_t)total;
         return CPM_OK;
     }
 
@@ -1780,8 +1824,18 @@ const char* cpm_last_kernel(cpm_handle h) {
     switch (h->last_plan.mode) {
         case Plan::kGlobal: return h->d == 3 ? "k_persist_global<3>" : "k_persist_global<2>";
         case Plan::kLds2d: return "k_persist_lds2d";
-        case Plan::kResident:
-            return h->small_narrow ? "k_resident2d<unsigned short>" : "k_resident2d<unsigned int>";
+        case Plan::kResident: {
+            static const char* const names[2][2][3] = {
+                {{"k_resident2d<unsigned int, 5, 0>", "k_resident2d<unsigned int, 5, 1>",
+                  "k_resident2d<unsigned int, 5, 2>"},
+                 {"k_resident2d<unsigned int, 6, 0>", "k_resident2d<unsigned int, 6, 1>",
+                  "k_resident2d<unsigned int, 6, 2>"}},
+                {{"k_resident2d<unsigned short, 5, 0>", "k_resident2d<unsigned short, 5, 1>",
+                  "k_resident2d<unsigned short, 5, 2>"},
+                 {"k_resident2d<unsigned short, 6, 0>", "k_resident2d<unsigned short, 6, 1>",
+                  "k_resident2d<unsigned short, 6, 2>"}}};
+            return names[h->small_narrow][h->last_plan.ltile == 6][h->res_var];
+        }
         case Plan::kSmall:
             return h->d == 3 ? (h->small_narrow ? "k_small_lds<3, unsigned short>" : "k_small_lds<3, unsigned int>")
                              : (h->small_narrow ? "k_small_lds<2, unsigned short>" : "k_small_lds<2, unsigned int>");

@@ -216,25 +216,19 @@ struct LdsLat3 {
     __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.gi; }
 };
 
-// The table deltas of one block's flips in one colour phase (the resident
-// form adds them to the other table buffer in the next phase).  At most two
-// entries per attempt, so kDeferCap covers a tile's attempts of one phase.
-constexpr int kDeferCap = 2048;
-struct DeferList {
-    unsigned int n;
-    uint32_t cell[kDeferCap];
-    int64_t delta[kDeferCap];
-};
-
-// Cell-table buffers of colour phase g (see SweepConsts).
+// Cell-table buffers of colour phase g (see SweepConsts).  row() reads a
+// cell's phase-start row, add() commits an accepted flip's delta.
 struct PhaseTables {
     const int64_t* R;   // tab[g & 1]: the phase-start snapshot
     int64_t* W;         // tab[(g + 1) & 1]: becomes the state after phase g
     int64_t* Dc;        // del[g & 1]: deltas of this phase (null: W alone takes them)
     int64_t* Dp;        // del[(g + 1) & 1]: deltas of phase g - 1, folded into W
-    DeferList* defer = nullptr;      // resident form: this block's deltas, also kept here
-    int32_t* bbox = nullptr;         // resident form: per cell min/max x, min/max y reached
-    const int32_t* anchor = nullptr; // (coordinates relative to the cell's anchor, wrapped)
+    template <bool CO>
+    __device__ __forceinline__ int64_t row(uint32_t c) const { return ld<CO>(R + c); }
+    __device__ __forceinline__ void add(uint32_t c, int64_t delta) const {
+        atomic_add_i64(W + c, delta);
+        if (Dc) atomic_add_i64(Dc + c, delta);
+    }
 };
 
 __device__ __forceinline__ PhaseTables tables_of(const SweepConsts& k, int g) {
@@ -546,6 +540,67 @@ __device__ __forceinline__ int64_t pers_energy(const SweepConsts& k, int dir, ui
     return (m * g + ((int64_t)1 << 23)) >> 24;
 }
 
+// Adhesion over the 3^D block nv (centre included; kernels2d.cu:359-402,477)
+// of a copy of source value vs (type ts) onto target value vt (type tt), and
+// the numbers of neighbours equal to the source (as) and to the target (at).
+template <int D, class PV>
+__device__ __forceinline__ int32_t adhesion_energy(const SweepConsts& k, const PV& P, const uint32_t* nv,
+                                                   uint32_t vs, uint32_t vt, int ts, int tt, int& as, int& at) {
+    constexpr int NB = D == 3 ? 27 : 9;
+    constexpr int CENTRE = NB / 2;
+    int32_t adh = 0;
+    as = 0;
+    at = 0;
+    if (PV::kAlwaysHist || P.hist()) {
+        uint32_t hist = 0;
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            if (n == CENTRE) continue;
+            const uint32_t vn = nv[n];
+            hist += 1u << ((vn >> k.idbits) * 5u);
+            as += vn == vs;
+            at += vn == vt;
+        }
+        adh = P.adhesion(ts, tt, hist, as, at);
+    } else {
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            if (n == CENTRE) continue;
+            const uint32_t vn = nv[n];
+            const int tn = (int)(vn >> k.idbits);
+            const int32_t js = P.J(ts, tn), jt = P.JS(tt, tn);
+            const bool eqs = vn == vs, eqt = vn == vt;
+            adh += (eqs ? 0 : js) - (eqt ? 0 : jt);
+            as += eqs;
+            at += eqt;
+        }
+    }
+    return adh;
+}
+
+// Perimeter and area terms (kernels2d.cu:479-520) of a copy from
+// cell cs (type ts, phase-start row row_s) onto cell ct (tt, row_t); as and
+// at as adhesion_energy gives them.  Medium (cell 0) has no terms.
+template <int D, class PV>
+__device__ __forceinline__ int64_t row_energy(const PV& P, uint32_t cs, uint32_t ct, int ts, int tt,
+                                              int64_t row_s, int64_t row_t, int as, int at) {
+    constexpr int N = D == 3 ? 26 : 8;
+    int64_t dh = 0;
+    if (cs) {
+        const int64_t A = (int64_t)(uint32_t)row_s, Pm = row_s >> 32;
+        const int64_t dP = N - 2 * as;
+        dh += (int64_t)P.la(ts) * (1 + 2 * A - 2 * (int64_t)P.a0(ts));
+        dh += (int64_t)P.lp(ts) * dP * (dP + 2 * (Pm - P.p0(ts)));
+    }
+    if (ct) {
+        const int64_t A = (int64_t)(uint32_t)row_t, Pm = row_t >> 32;
+        const int64_t dP = 2 * at - N;
+        dh += (int64_t)P.la(tt) * (1 - 2 * A + 2 * (int64_t)P.a0(tt));
+        dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
+    }
+    return dh;
+}
+
 // Attempt counters (cpm_set_counters; the reference allocates analytics[3]
 // for evaluated / accepted / committed and leaves its increments commented
 // out, kernels2d.cu:292,526,531): the lanes that reach one point together
@@ -557,10 +612,10 @@ __device__ __forceinline__ void count_attempts(unsigned long long* c, int slot) 
 
 // XT = false leaves out the chemotaxis, act and persistence terms (the
 // caller launches it only when none is on): fewer registers in 3D.
-template <int D, bool CO, bool XT = true, class Lat, class PV>
+template <int D, bool CO, bool XT = true, class Lat, class PV, class TB>
 __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts& k, const PV& P,
                                                const uint32_t* __restrict__ thr,
-                                               const PhaseTables& T, const CellSync& cy,
+                                               const TB& T, const CellSync& cy,
                                                uint32_t sb, const Draw<D, Lat>& d) {
     constexpr int N = D == 3 ? 26 : 8;
     constexpr int NB = N + 1;
@@ -589,47 +644,11 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     const uint32_t cs = vs & k.idmask, ct = vt & k.idmask;
     // Issue the snapshot-table loads first: their latency overlaps the
     // neighbourhood sums.
-    const int64_t row_s = cs ? ld<CO>(T.R + cs) : 0;
-    const int64_t row_t = ct ? ld<CO>(T.R + ct) : 0;
-    int32_t adh = 0;
-    int as = 0, at = 0;
-    if (PV::kAlwaysHist || P.hist()) {
-        uint32_t hist = 0;
-#pragma unroll
-        for (int n = 0; n < NB; n++) {
-            if (n == CENTRE) continue;
-            const uint32_t vn = d.nv[n];
-            hist += 1u << ((vn >> k.idbits) * 5u);
-            as += vn == vs;
-            at += vn == vt;
-        }
-        adh = P.adhesion(ts, tt, hist, as, at);
-    } else {
-#pragma unroll
-        for (int n = 0; n < NB; n++) {
-            if (n == CENTRE) continue;
-            const uint32_t vn = d.nv[n];
-            const int tn = (int)(vn >> k.idbits);
-            const int32_t js = P.J(ts, tn), jt = P.JS(tt, tn);
-            const bool eqs = vn == vs, eqt = vn == vt;
-            adh += (eqs ? 0 : js) - (eqt ? 0 : jt);
-            as += eqs;
-            at += eqt;
-        }
-    }
-    int64_t dh = adh;
-    if (cs) {
-        const int64_t A = (int64_t)(uint32_t)row_s, Pm = row_s >> 32;
-        const int64_t dP = N - 2 * as;
-        dh += (int64_t)P.la(ts) * (1 + 2 * A - 2 * (int64_t)P.a0(ts));
-        dh += (int64_t)P.lp(ts) * dP * (dP + 2 * (Pm - P.p0(ts)));
-    }
-    if (ct) {
-        const int64_t A = (int64_t)(uint32_t)row_t, Pm = row_t >> 32;
-        const int64_t dP = 2 * at - N;
-        dh += (int64_t)P.la(tt) * (1 - 2 * A + 2 * (int64_t)P.a0(tt));
-        dh += (int64_t)P.lp(tt) * dP * (dP + 2 * (Pm - P.p0(tt)));
-    }
+    const int64_t row_s = cs ? T.template row<CO>(cs) : 0;
+    const int64_t row_t = ct ? T.template row<CO>(ct) : 0;
+    int as, at;
+    int64_t dh = adhesion_energy<D>(k, P, d.nv, vs, vt, ts, tt, as, at);
+    dh += row_energy<D>(P, cs, ct, ts, tt, row_s, row_t, as, at);
     bool accept;
     if (cy.pass == 4 || cy.pass == 6) {
         // split phase, apply: the decision made against the phase-start
@@ -662,36 +681,8 @@ __device__ __forceinline__ void finish_attempt(const Lat& lat, const SweepConsts
     if (XT && k.act) st<CO>(k.act_lat + lat.gindex(d.t), d.tick);  // kernels2d.cu:532-534
     // Area and perimeter deltas (kernels2d.cu:552-569); centroid sums are
     // rebuilt from the lattice when read.
-    if (cs) {
-        const int64_t delta = (int64_t)(N - 2 * as) * 4294967296LL + 1;
-        atomic_add_i64(T.W + cs, delta);
-        if (T.Dc) atomic_add_i64(T.Dc + cs, delta);
-        if (T.defer) {
-            const unsigned int i = atomicAdd(&T.defer->n, 1u);
-            T.defer->cell[i] = cs;
-            T.defer->delta[i] = delta;
-        }
-        if (T.bbox) {  // the gaining cell reaches the target site
-            const int L = 1 << k.lgL, Lm = L - 1;
-            const uint32_t gi = lat.gindex(d.t);
-            const int rx = (((int)(gi & Lm) - T.anchor[3 * cs] + L / 2) & Lm) - L / 2;
-            const int ry = (((int)((gi >> k.lgL) & Lm) - T.anchor[3 * cs + 1] + L / 2) & Lm) - L / 2;
-            atomicMin(T.bbox + 4 * cs, rx);
-            atomicMax(T.bbox + 4 * cs + 1, rx);
-            atomicMin(T.bbox + 4 * cs + 2, ry);
-            atomicMax(T.bbox + 4 * cs + 3, ry);
-        }
-    }
-    if (ct) {
-        const int64_t delta = (int64_t)(2 * at - N) * 4294967296LL - 1;
-        atomic_add_i64(T.W + ct, delta);
-        if (T.Dc) atomic_add_i64(T.Dc + ct, delta);
-        if (T.defer) {
-            const unsigned int i = atomicAdd(&T.defer->n, 1u);
-            T.defer->cell[i] = ct;
-            T.defer->delta[i] = delta;
-        }
-    }
+    if (cs) T.add(cs, (int64_t)(N - 2 * as) * 4294967296LL + 1);
+    if (ct) T.add(ct, (int64_t)(2 * at - N) * 4294967296LL - 1);
 }
 
 // MINSTD (random.cu:5-12): 16807 x mod (2^31 - 1), in double as there.
@@ -1750,240 +1741,686 @@ hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, h
 
 // ---------------------------------------------------------------------------
 // The resident form (2D, shared roles, c = 1, u = 1): one block per tile of
-// TE x TE sites, the tile and a one-site halo resident in LDS for a whole
-// inner_iterations chunk, and between colour phases a tile waits only for its
-// eight neighbours (per-tile phase counters; kernel boundaries and grid
-// barriers are gone).
+// TE x TE sites for a whole inner_iterations chunk.  The tile, a one-site
+// halo and the table rows of every cell the tile may meet live in LDS;
+// between colour phases a tile hears only from its eight neighbours, through
+// mailbox words that carry their own phase tag.
 //
-// Why neighbours suffice.  In phase p a tile's attempts read its tile, the
-// halo (the neighbours' border sites) and the snapshot rows R = tab[p & 1]
-// of the cells they meet, and write the tile, W = tab[(p + 1) & 1] and, for
-// the deltas of phase p - 1, W as well (each tile keeps its own deltas of a
-// phase in an LDS list and adds them to the other buffer in the next phase:
-// W then holds S(p-2) + D(p-1) + D(p) = S(p), with no global fold).  A tile
-// starts phase p only when its neighbours have finished phase p - 1, so
-// neighbouring tiles never run different phases at once.  If every tile that
-// reads or writes a cell's rows is a neighbour of every other such tile,
-// each read of R sees exactly the state after phase p - 1 and no write of
-// phase p + 1 can overtake it.  That holds when every cell's sites over the
-// chunk, widened by one site (the halo), span at most two tiles per axis:
-// the guard.  Gaining flips widen a per-cell box (bbox); a check after the
-// launch tells the host whether the chunk's result stands or must be redone
-// with one launch per phase (from a copy made before it).
+// Holding cells.  Before the launch every cell gets the block of tiles its
+// box, widened by one site (the halo) and by a margin for its growth within
+// the launch (up to a quarter tile, less where a wide cell has no room),
+// overlaps: one tile, two side by side or a 2 x 2 block (a cell wider than
+// that fails the guard and the MCS run per phase).  Each tile holds the
+// rows of its cells in LDS, slots in increasing cell order.  Every attempt
+// whose cells a tile does not hold would read a row it cannot know: it sets
+// the guard flag and the chunk is redone per phase from a copy.
 //
-// Same-colour targets are two sites apart, so no attempt of a phase reads a
-// site another attempt of that phase writes: the halo a tile loads at a
-// phase start is exact for that phase even while neighbours write their own
-// targets.  Border sites are written through to global memory with
-// device-coherent stores; every wave waits for its stores and atomics before
-// the tile's counter is raised.
-constexpr int kResThreads = 256;
+// Why the rows stay exact.  A flip changes the rows of its two cells, and
+// only tiles holding them make such flips.  The holders of a cell are
+// neighbours of one another, so after every phase each tile sends each
+// neighbour the deltas of the cells both hold (in increasing cell order,
+// which both sides know) and adds the deltas it receives: every holder
+// then has the row after the phase, which is what every attempt of the next
+// phase must read.
+//
+// Why the halo stays exact.  Same-colour targets are two sites apart, so no
+// attempt of a phase reads a site another attempt of it writes; the halo a
+// tile needs in phase p is its neighbours' border after phase p - 1, which
+// they send with the deltas.
+//
+// Mailboxes.  Tile b's packet to neighbour d after phase p sits in
+// mail[b][p & 1][d]: its border sites facing d (two 16-bit sites or one
+// 32-bit site per word), then one word per shared cell (perimeter delta
+// << 12 plus area delta).  Each 64-bit word carries tag p + 1 in its high
+// half, so a reader polls the words themselves: no fence, flag or second
+// round trip.  A tile writes mail[b][p & 1] again only after its neighbours
+// have sent phase p + 1, which they do after reading phase p.
+//
+// Order within a phase: the Philox draws and the attempts whose 3 x 3
+// block lies inside the tile are made while the neighbours' packets are on
+// their way; then the halo and the deltas arrive, the remaining attempts
+// are drawn, all accepted flips are committed, the tile's packets are sent
+// and its own deltas join the rows.
+constexpr uint8_t kResMed = 0xFE;   // the medium's slot (row 0, never changed)
+constexpr uint8_t kResNone = 0xFF;  // a cell the tile does not hold
 
-template <class ST>
-struct TileLat {
+template <int LTE>
+struct ResGeom {
+    static constexpr int TE = 1 << LTE;
+    static constexpr int S = TE + 2;        // LDS row stride (tile + halo)
+    static constexpr int NT = TE * TE / 8;  // threads: two same-colour sites (one Philox block) each
+    static constexpr int NQ = TE * TE / 4;  // attempts per tile and phase
+};
+
+// Neighbour direction d = 0..7: the 3 x 3 block of tiles without its centre,
+// dx fastest; the opposite direction is 7 - d.
+__device__ __forceinline__ void res_dir(int d, int& dx, int& dy) {
+    const int k = d < 4 ? d : d + 1;
+    dx = k % 3 - 1;
+    dy = k / 3 - 1;
+}
+
+// Tile (tx, ty) holds the cell of holder word hw (res_holder_word).
+__device__ __forceinline__ bool res_holds(uint32_t hw, int tx, int ty, int ntm) {
+    if (!(hw >> 31)) return false;
+    const int tx0 = (int)(hw & 0xFFFu), ty0 = (int)((hw >> 12) & 0xFFFu);
+    return ((tx - tx0) & ntm) <= (int)((hw >> 24) & 1u) && ((ty - ty0) & ntm) <= (int)((hw >> 25) & 1u);
+}
+
+// The tile's lattice in LDS (sites and halo, row stride S).
+template <class ST, int LTE>
+struct ResLat {
     using Site = ST;
-    ST* s;        // tile + halo in LDS, row stride S
-    ST* g;        // the global lattice
-    int S, TE, x0, y0, lgL;
+    ST* s;
+    int x0, y0, lgL;
     struct Pos {
         int li;       // LDS index
         int lx, ly;   // 1 .. TE inside the tile
         uint32_t gi;  // global site index
     };
-    __device__ __forceinline__ Pos at(int x, int y, int) const {
-        const int lx = x - x0 + 1, ly = y - y0 + 1;
-        return {ly * S + lx, lx, ly, (uint32_t)x | ((uint32_t)y << lgL)};
-    }
     __device__ __forceinline__ uint32_t self(const Pos& q) const { return s[q.li]; }
     __device__ __forceinline__ uint32_t nb(const Pos& q, int dx, int dy, int) const {
-        return s[q.li + dy * S + dx];
+        return s[q.li + dy * ResGeom<LTE>::S + dx];
     }
-    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const {
-        s[q.li] = (ST)v;
-        if (q.lx == 1 || q.ly == 1 || q.lx == TE || q.ly == TE) st<true>(g + q.gi, (ST)v);
-    }
+    __device__ __forceinline__ void put(const Pos& q, uint32_t v) const { s[q.li] = (ST)v; }
     __device__ __forceinline__ void block(const Pos& q, uint32_t* nv) const { lds_block<2>(*this, q, nv); }
     __device__ __forceinline__ uint32_t gindex(const Pos& q) const { return q.gi; }
 };
 
-// Wait until the eight neighbour tiles' counters reach p (lanes 0..7 of wave
-// 0 poll one each, bounded).  Returns false (the whole block) when a wait was
-// abandoned; *err is then set and every other waiter gives up too.
-__device__ __forceinline__ bool resident_wait(const ResidentArgs& a, int tx, int ty, unsigned int p) {
-    __shared__ int abandoned;
-    if (threadIdx.x == 0) abandoned = 0;
-    __syncthreads();
-    if (threadIdx.x < 8) {
-        const int k = threadIdx.x < 4 ? threadIdx.x : threadIdx.x + 1;  // the 3 x 3 block without its centre
-        const int ntm = (1 << a.lnt) - 1;
-        const int nx = (tx + (k % 3) - 1) & ntm, ny = (ty + (k / 3) - 1) & ntm;
-        const unsigned int* f = a.flags + kResFlagStride * (nx | (ny << a.lnt));
-        long polls = 0;
-        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < p) {
-            if (++polls > kBarrierSpinCap ||
-                __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                abandoned = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+// The rows of the cells a tile holds: R the phase-start rows, Dl this
+// phase's deltas, by slot.  Meeting a cell the tile does not hold sets *miss.
+struct TileTables {
+    const uint8_t* map;  // cell -> slot, kResNone: not held
+    const int64_t* R;
+    int64_t* Dl;
+    unsigned int* miss;
+    template <bool CO>
+    __device__ __forceinline__ int64_t row(uint32_t c) const {
+        const uint32_t s = map[c];
+        if (s == kResNone) {
+            *miss = 1u;
+            return 0;
         }
+        return R[s];
     }
-    __syncthreads();
-    return !abandoned;
+    __device__ __forceinline__ void add(uint32_t c, int64_t delta) const {
+        const uint32_t s = map[c];
+        if (s != kResNone) atomicAdd(reinterpret_cast<unsigned long long*>(Dl + s), (unsigned long long)delta);
+    }
+};
+
+// A queued attempt of the resident form (32 bytes of LDS).
+constexpr uint32_t kResRing = 0x80000000u;  // li flag: the 3 x 3 block reaches the halo
+struct ResEntry {
+    uint32_t li;    // target (LDS index), | kResRing
+    uint32_t vs, vt, w2;
+    uint32_t meta;  // dir | on << 4 | as << 8 | at << 16
+    uint32_t pad;
+    int64_t e;      // the terms that need no rows
+};
+
+// The lean attempt path's parameters (at most four types) as LDS rows.
+struct ResParams4 {
+    int4 J[4];   // J[a][0..3]
+    int4 JS[4];  // JS[a][0..3]
+    int4 T[4];   // lambda_area, target area, lambda_perimeter, target perimeter of type a
+};
+
+template <class ST, int LTE>
+struct ResShared {
+    using G = ResGeom<LTE>;
+    ST tile[G::S * G::S];
+    int64_t R[256];
+    int64_t D[2][256];           // deltas of the phases of each parity, by slot
+    uint32_t cell[kResSlots];
+    uint8_t nbm[kResSlots];      // bit d: neighbour d holds the cell too
+    uint8_t dl[8][kResDirCap];   // slots shared with neighbour d, increasing cell order
+    int dn[8];
+    int woff[9];                 // first word of each packet in the block's word order
+    uint32_t thr[kThrLds];
+    int32_t prm[kParamWords];    // more than four types: the parameters, staged
+    ResParams4 p4;               // the lean path's parameters
+    int scan[G::NT];
+    ResEntry queue[G::NQ];       // this phase's attempts whose source and target differ, and the ring's
+    uint4 draws[G::NT];          // the Philox block of each pair of same-colour sites, for the next phase
+    unsigned int nq[2];          // queue length of the phases of each parity
+    unsigned int miss, quit;
+};
+
+// Border site q (along the edge) facing (dx, dy), LDS coordinates 1 .. TE.
+template <int LTE>
+__device__ __forceinline__ void res_border(int dx, int dy, int q, int& lx, int& ly) {
+    constexpr int TE = ResGeom<LTE>::TE;
+    lx = dx > 0 ? TE : dx < 0 ? 1 : 1 + q;
+    ly = dy > 0 ? TE : dy < 0 ? 1 : 1 + q;
 }
 
-template <class ST, class PV>
-__device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ST* tile,
-                                             QueuedAttempt<2, false, ST>* queue, unsigned int* nqueued,
-                                             DeferList* dl) {
+// Mailbox words a thread reads and writes each phase: at most WPT of them.
+template <class ST, int LTE>
+struct ResWords {
+    static constexpr int TE = ResGeom<LTE>::TE;
+    static constexpr int SPW = 4 / (int)sizeof(ST);  // sites per word
+    static constexpr int EDGE = TE / SPW;             // site words of an edge packet
+    static constexpr int MAXW = 4 * (EDGE + kResDirCap) + 4 * (1 + kResDirCap);
+    static constexpr int WPT = (MAXW + ResGeom<LTE>::NT - 1) / ResGeom<LTE>::NT;
+    // A word's LDS side: sites (index of the first, stride to the next,
+    // count) or a delta (slot).
+    static __device__ __forceinline__ uint32_t sites(int li, int stride, int n) {
+        return (uint32_t)li | ((uint32_t)stride << 16) | ((uint32_t)n << 28);
+    }
+    static __device__ __forceinline__ uint32_t delta(int slot) { return (uint32_t)slot | 0x80000000u; }
+};
+
+// One queued attempt of the resident form, in two parts: res_prepare()
+// applies the fixed-type rules and evaluates the terms that need no cell
+// rows (adhesion, chemotaxis); res_commit() adds the area and perimeter
+// terms from the phase-start rows, accepts and commits.  The split lets a
+// tile prepare its interior attempts while its neighbours' packets travel.
+// On entry vs and vt hold the drawn source and target values, which differ.
+template <class ST, int LTE, bool XT, class PV>
+__device__ __forceinline__ void res_prepare(const ResLat<ST, LTE>& lat, const SweepConsts& k, const PV& P,
+                                            int x0, int y0, ResEntry& q) {
+    constexpr int S = ResGeom<LTE>::S;
+    const uint32_t li = q.li & ~kResRing, vt = q.vt;
+    const int dir = (int)(q.meta & 0xFu);
+    uint32_t vs = q.vs;
+    unsigned long long* const ctr = k.counters;
+    if (ctr) count_attempts(ctr, 0);
+    if (k.fixed_mask) {  // fixed types (kernels2d.cu:281-290); the act term is off in this form
+        if ((k.fixed_mask >> (vt >> k.idbits)) & 1u) return;
+        if ((k.fixed_mask >> (vs >> k.idbits)) & 1u) {
+            vs = 0;
+            if (vs == vt) return;
+        }
+    }
+    if (ctr) count_attempts(ctr, 1);  // kernels2d.cu:292
+    const int ly = (int)li / S, lx = (int)li - ly * S;
+    const typename ResLat<ST, LTE>::Pos t{(int)li, lx, ly,
+                                          (uint32_t)(x0 + lx - 1) | ((uint32_t)(y0 + ly - 1) << lat.lgL)};
+    uint32_t nv[9];
+    lat.block(t, nv);
+    const int ts = (int)(vs >> k.idbits), tt = (int)(vt >> k.idbits);
+    int as, at;
+    int64_t e = adhesion_energy<2>(k, P, nv, vs, vt, ts, tt, as, at);
+    if (XT && k.chem) e += chem_energy<2>(k, lat, t, dir, ts, tt);
+    q.vs = vs;
+    q.e = e;
+    q.meta = (uint32_t)dir | (1u << 4) | ((uint32_t)as << 8) | ((uint32_t)at << 16);
+}
+
+template <class ST, int LTE, class PV>
+__device__ __forceinline__ void res_commit(ST* tile, const SweepConsts& k, const PV& P, const TileTables& T,
+                                           const uint32_t* thr_s, bool thr_lds, const ResEntry& q) {
+    const uint32_t meta = q.meta;
+    if (!((meta >> 4) & 1u)) return;
+    const int as = (int)((meta >> 8) & 0xFFu), at = (int)(meta >> 16);
+    const uint32_t vs = q.vs, vt = q.vt;
+    const int ts = (int)(vs >> k.idbits), tt = (int)(vt >> k.idbits);
+    const uint32_t cs = vs & k.idmask, ct = vt & k.idmask;
+    const int64_t row_s = cs ? T.row<false>(cs) : 0;
+    const int64_t row_t = ct ? T.row<false>(ct) : 0;
+    const int64_t dh = q.e + row_energy<2>(P, cs, ct, ts, tt, row_s, row_t, as, at);
+    bool accept = dh <= 0;
+    if (!accept && dh < k.nthr) {
+        // (a device-coherent load for the long table keeps the two loads
+        // apart, so the LDS one stays an LDS instruction)
+        const uint32_t th = thr_lds ? thr_s[dh] : ld<true>(k.thr + dh);
+        accept = q.w2 < th;
+    }
+    if (!accept) return;
+    unsigned long long* const ctr = k.counters;
+    if (ctr) {
+        count_attempts(ctr, 2);  // kernels2d.cu:526
+        count_attempts(ctr, 3);  // kernels2d.cu:531
+    }
+    tile[q.li & ~kResRing] = (ST)vs;  // kernels2d.cu:548
+    // area and perimeter deltas (kernels2d.cu:552-569)
+    constexpr int N = 8;
+    if (cs) T.add(cs, (int64_t)(N - 2 * as) * 4294967296LL + 1);
+    if (ct) T.add(ct, (int64_t)(2 * at - N) * 4294967296LL - 1);
+}
+
+// The lean attempt path of the resident form: at most four types, no fixed
+// types, no chemotaxis, no counters, a threshold table that fits LDS.  The
+// parameters are LDS rows (one 16-byte load each), the neighbour counts are
+// packed in one word and both cells' terms are computed without branches,
+// which shortens the dependent chain each phase waits on.
+__device__ __forceinline__ int32_t pick4(int i, const int4& v) { return pick4(i, v.x, v.y, v.z, v.w); }
+
+template <class ST, int LTE>
+__device__ __forceinline__ void res_prepare_lean(const ST* tile, const ResParams4& pr, uint32_t idbits,
+                                                 ResEntry& q) {
+    constexpr int S = ResGeom<LTE>::S;
+    const int li = (int)(q.li & ~kResRing);
+    const uint32_t vs = q.vs, vt = q.vt;
+    const int ts = (int)(vs >> idbits), tt = (int)(vt >> idbits);
+    const int4 jr = pr.J[ts], jsr = pr.JS[tt];
+    const uint32_t nv[8] = {tile[li - S - 1], tile[li - S], tile[li - S + 1], tile[li - 1],
+                            tile[li + 1],     tile[li + S - 1], tile[li + S], tile[li + S + 1]};
+    uint32_t hist = 0;
+    int as = 0, at = 0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        hist += 1u << ((nv[n] >> idbits) * 5u);
+        as += nv[n] == vs;
+        at += nv[n] == vt;
+    }
+    // adhesion (kernels2d.cu:359-402,477) from the counts per type
+    int32_t adh = (jr.x - jsr.x) * (int32_t)(hist & 31u) + (jr.y - jsr.y) * (int32_t)((hist >> 5) & 31u) +
+                  (jr.z - jsr.z) * (int32_t)((hist >> 10) & 31u) + (jr.w - jsr.w) * (int32_t)((hist >> 15) & 31u);
+    adh += at * pick4(tt, jsr) - as * pick4(ts, jr);
+    q.e = adh;
+    q.meta = (q.meta & 0xFu) | (1u << 4) | ((uint32_t)as << 8) | ((uint32_t)at << 16);
+}
+
+// Perimeter and area terms of one cell (kernels2d.cu:479-520): sign +1 for
+// the gaining source cell, -1 for the losing target cell.
+__device__ __forceinline__ int64_t res_cell_terms(const int4& tp, int64_t row, int64_t dP, int sgn) {
+    const int64_t A = (int64_t)(uint32_t)row, Pm = row >> 32;
+    return (int64_t)tp.x * (1 + sgn * (2 * A - 2 * (int64_t)tp.y)) +
+           (int64_t)tp.z * dP * (dP + 2 * (Pm - tp.w));
+}
+
+template <class ST, int LTE>
+__device__ __forceinline__ void res_commit_lean(ST* tile, const uint8_t* map, const int64_t* R, int64_t* Dl,
+                                                const uint32_t* thr, int nthr, const ResParams4& pr,
+                                                uint32_t idbits, uint32_t idmask, const ResEntry& q,
+                                                bool& miss) {
+    const uint32_t meta = q.meta;
+    if (!((meta >> 4) & 1u)) return;
+    constexpr int N = 8;
+    const int as = (int)((meta >> 8) & 0xFFu), at = (int)(meta >> 16);
+    const uint32_t vs = q.vs, vt = q.vt;
+    const int ts = (int)(vs >> idbits), tt = (int)(vt >> idbits);
+    const uint32_t cs = vs & idmask, ct = vt & idmask;
+    const uint32_t ss = map[cs], st_ = map[ct];  // (the medium has slot kResMed, row 0)
+    const int4 ps = pr.T[ts], pt = pr.T[tt];
+    const int64_t rs = R[ss], rt = R[st_];
+    const int64_t dPs = N - 2 * as, dPt = 2 * at - N;
+    int64_t dh = q.e;
+    dh += cs ? res_cell_terms(ps, rs, dPs, 1) : 0;
+    dh += ct ? res_cell_terms(pt, rt, dPt, -1) : 0;
+    const int64_t ix = dh < 0 ? 0 : (dh >= nthr ? nthr - 1 : dh);
+    const uint32_t th = thr[ix];
+    const bool accept = dh <= 0 || (dh < nthr && q.w2 < th);
+    miss |= ss == kResNone || st_ == kResNone;
+    if (!accept || ss == kResNone || st_ == kResNone) return;
+    tile[q.li & ~kResRing] = (ST)vs;  // kernels2d.cu:548
+    // area and perimeter deltas (kernels2d.cu:552-569)
+    if (cs) atomicAdd(reinterpret_cast<unsigned long long*>(Dl + ss), (unsigned long long)(dPs * 4294967296LL + 1));
+    if (ct) atomicAdd(reinterpret_cast<unsigned long long*>(Dl + st_), (unsigned long long)(dPt * 4294967296LL - 1));
+}
+
+// The Philox block of pair q (same-colour row q / (TE / 4), x-neighbouring
+// pair q mod (TE / 4)) of a tile with origin (x0, y0) in phase p of a
+// chunk starting at MCS mcs0: the counter of the pair's even site, as
+// attempt_words draws it.
+template <int LTE>
+__device__ __forceinline__ uint4 res_pair_draws(const SweepConsts& k, int x0, int y0, uint32_t mcs0,
+                                                unsigned int p, int q) {
+    constexpr int TE = ResGeom<LTE>::TE;
+    const int col = (int)(p & 3u);
+    int cx, cy, cz;
+    colour_of<2>(col, cx, cy, cz);
+    const int j = q / (TE / 4), jc = q % (TE / 4);
+    const int bx = x0 + 4 * jc + cx, by = y0 + 2 * j + cy;
+    uint32_t wp[4];
+    philox4x32_10((uint32_t)bx | ((uint32_t)by << k.lgL), mcs0 + (p >> 2), (uint32_t)col, 0u, k.k0, k.k1, wp);
+    return make_uint4(wp[0], wp[1], wp[2], wp[3]);
+}
+
+template <class ST, int LTE, bool LEAN, bool XT, class PV>
+__device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ResShared<ST, LTE>& sh,
+                                             uint8_t* map) {
+    using G = ResGeom<LTE>;
+    using WD = ResWords<ST, LTE>;
+    constexpr int TE = G::TE, S = G::S, NT = G::NT, SPW = WD::SPW, WPT = WD::WPT;
     const SweepConsts& k = a.k;
-    const int TE = 1 << a.lte, S = TE + 2;
+    const int tid = (int)threadIdx.x;
     const int b = (int)blockIdx.x;
-    const int tx = b & ((1 << a.lnt) - 1), ty = b >> a.lnt;
-    const int x0 = tx << a.lte, y0 = ty << a.lte;
+    const int ntm = (1 << a.lnt) - 1;
+    const int tx = b & ntm, ty = b >> a.lnt;
+    const int x0 = tx << LTE, y0 = ty << LTE;
     const int Lm = (1 << k.lgL) - 1;
     ST* glat = lat_base<ST>(k);
-    const TileLat<ST> lat{tile, glat, S, TE, x0, y0, k.lgL};
-    unsigned int* myflag = a.flags + kResFlagStride * b;
-    // the tile and its halo
-    for (int i = threadIdx.x; i < S * S; i += kResThreads) {
+    if (*a.bad != 0u) return;  // a cell wider than two tiles (set before the launch)
+    for (int i = tid; i < S * S; i += NT) {
         const int ly = i / S, lx = i - ly * S;
-        tile[i] = glat[(uint32_t)((x0 + lx - 1) & Lm) | ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
+        sh.tile[i] = glat[(uint32_t)((x0 + lx - 1) & Lm) | ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
     }
-    if (threadIdx.x == 0) dl->n = 0;
+    const bool thr_lds = k.nthr <= kThrLds;
+    if (thr_lds)
+        for (int i = tid; i < k.nthr; i += NT) sh.thr[i] = k.thr[i];
+    for (int c = tid; c <= k.ncells; c += NT) map[c] = c ? kResNone : kResMed;
+    if (tid < 2) sh.D[tid][kResMed] = 0;
+    if (tid == 0) sh.R[kResMed] = 0;
+    if (LEAN && tid < 48) {
+        const int f = tid / 16, a = (tid / 4) & 3, c = tid & 3;  // J, JS rows; then the terms by type
+        int32_t* dst = reinterpret_cast<int32_t*>(&sh.p4);
+        const int32_t* s4 = k.prm->s4;
+        dst[tid] = f < 2 ? s4[tid] : s4[2 * kSmallTypes * kSmallTypes + c * kSmallTypes + a];
+    }
+    // slots in increasing cell order: thread t counts the cells of its range
+    // it holds, a block scan gives each range its first slot
+    const int per = (k.ncells + NT - 1) / NT;
+    const int c0 = 1 + tid * per, c1 = min(c0 + per, k.ncells + 1);
+    int cnt = 0;
+    for (int c = c0; c < c1; c++) cnt += res_holds(a.holders[c], tx, ty, ntm);
+    sh.scan[tid] = cnt;
+    if (tid == 0) {
+        sh.miss = 0;
+        sh.quit = 0;
+    }
     __syncthreads();
-    const int ring = 4 * S - 4;
-    const int half = TE / 2;                 // same-colour sites per tile row
-    const int npairs = half * half / 2;      // x-neighbouring pairs of them (one Philox block each)
-    const unsigned int G = 4u * (unsigned int)a.nmcs;
-    for (unsigned int p = 0; p < G; p++) {
-        const uint32_t mcs = a.mcs0 + p / 4;
-        const int col = (int)(p & 3u);
-        const PhaseTables T{tables_of(k, (int)p).R, tables_of(k, (int)p).W, nullptr, nullptr, dl,
-                            a.bbox, a.anchor};
-        if (p > 0) {
-            if (!resident_wait(a, tx, ty, p)) return;
-            // halo after the neighbours' phase p - 1
-            for (int i = threadIdx.x; i < ring; i += kResThreads) {
-                int lx, ly;
-                if (i < S) { lx = i; ly = 0; }
-                else if (i < 2 * S) { lx = i - S; ly = S - 1; }
-                else if (i < 2 * S + TE) { lx = 0; ly = i - 2 * S + 1; }
-                else { lx = S - 1; ly = i - 2 * S - TE + 1; }
-                tile[ly * S + lx] = ld<true>(glat + ((uint32_t)((x0 + lx - 1) & Lm) |
-                                                     ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)));
-            }
-            // this tile's deltas of phase p - 1 into the other buffer (W of phase p)
-            const unsigned int nd = dl->n;
-            for (unsigned int i = threadIdx.x; i < nd; i += kResThreads) atomic_add_i64(T.W + dl->cell[i], dl->delta[i]);
-            __syncthreads();
-            if (threadIdx.x == 0) dl->n = 0;
-        }
-        if (threadIdx.x == 0) *nqueued = 0;
+    for (int off = 1; off < NT; off <<= 1) {
+        const int v = tid >= off ? sh.scan[tid - off] : 0;
         __syncthreads();
-        // draws: each thread takes x-neighbouring pairs of same-colour sites
-        int cx, cy, cz;
-        colour_of<2>(col, cx, cy, cz);
-        constexpr int PPT = 2;  // pairs per thread per round
-        QueuedAttempt<2, false, ST> e[2 * PPT];
-        bool act_[2 * PPT];
+        sh.scan[tid] += v;
+        __syncthreads();
+    }
+    const int nslots = sh.scan[NT - 1];
+    if (nslots > kResSlots) {  // more cells than slots: the guard
+        if (tid == 0) atomicOr(a.bad, kResGuardSlots);
+        return;
+    }
+    int slot = sh.scan[tid] - cnt;
+    for (int c = c0; c < c1; c++) {
+        const uint32_t hw = a.holders[c];
+        if (!res_holds(hw, tx, ty, ntm)) continue;
+        uint32_t m = 0;
 #pragma unroll
-        for (int r = 0; r < PPT; r++) {
-            const int pi = (int)threadIdx.x + r * kResThreads;
-            act_[2 * r] = act_[2 * r + 1] = false;
-            if (pi < npairs) {
-                const int j = pi / (half / 2), jc = pi - j * (half / 2);
-                const int ly = 2 * j + cy + 1;
-                const int bx = x0 + 4 * jc + cx, by = y0 + 2 * j + cy;  // the even site of the pair
-                uint32_t wp[4];
-                philox4x32_10((uint32_t)bx | ((uint32_t)by << k.lgL), mcs, (uint32_t)col, 0u, k.k0, k.k1, wp);
+        for (int d = 0; d < 8; d++) {
+            int dx, dy;
+            res_dir(d, dx, dy);
+            m |= (uint32_t)res_holds(hw, (tx + dx) & ntm, (ty + dy) & ntm, ntm) << d;
+        }
+        map[c] = (uint8_t)slot;
+        sh.cell[slot] = (uint32_t)c;
+        sh.nbm[slot] = (uint8_t)m;
+        sh.R[slot] = k.tab[0][c];
+        sh.D[0][slot] = 0;
+        sh.D[1][slot] = 0;
+        slot++;
+    }
+    __syncthreads();
+    if (tid < 8) {
+        int n = 0;
+        for (int s = 0; s < nslots; s++)
+            if ((sh.nbm[s] >> tid) & 1u) {
+                if (n < kResDirCap) sh.dl[tid][n] = (uint8_t)s;
+                n++;
+            }
+        sh.dn[tid] = n;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int o = 0;
+        unsigned int over = 0;
+        for (int d = 0; d < 8; d++) {
+            int dx, dy;
+            res_dir(d, dx, dy);
+            sh.woff[d] = o;
+            over |= sh.dn[d] > kResDirCap;
+            o += (dx != 0 && dy != 0 ? 1 : WD::EDGE) + sh.dn[d];
+        }
+        sh.woff[8] = o;
+        sh.quit = over;
+    }
+    __syncthreads();
+    if (sh.quit) {  // more shared cells than a packet holds: the guard
+        if (tid == 0) atomicOr(a.bad, kResGuardShared);
+        return;
+    }
+    // This thread's mailbox words: word offsets in the neighbour's packet
+    // (read) and in this tile's own (written), without the parity, and the
+    // LDS side of each.
+    const uint32_t mstride = (uint32_t)res_mail_stride(TE);
+    uint32_t wsrc[WPT], wdst[WPT], win[WPT], wout[WPT];
+    uint32_t need = 0;
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const uint32_t w1 = h ? wp[3] : wp[1], w2 = h ? wp[0] : wp[2];
-                    int dir = (int)(((uint64_t)w1 * 8ull) >> 32);
-                    dir += dir >= 4;
+    for (int j = 0; j < WPT; j++) {
+        const int w = tid + j * NT;
+        wsrc[j] = wdst[j] = win[j] = wout[j] = 0;
+        if (w < sh.woff[8]) {
+            int d = 0;
+            while (d < 7 && w >= sh.woff[d + 1]) d++;
+            int dx, dy;
+            res_dir(d, dx, dy);
+            const int i = w - sh.woff[d];
+            const int nb = ((tx + dx) & ntm) | (((ty + dy) & ntm) << a.lnt);
+            wsrc[j] = ((uint32_t)nb * 16u + (uint32_t)(7 - d)) * mstride + (uint32_t)i;
+            wdst[j] = ((uint32_t)b * 16u + (uint32_t)d) * mstride + (uint32_t)i;
+            const bool corner = dx != 0 && dy != 0;
+            if (i < (corner ? 1 : WD::EDGE)) {
+                const int n = corner ? 1 : SPW, stride = dx == 0 ? 1 : S;
+                int lx, ly;
+                res_border<LTE>(-dx, -dy, i * SPW, lx, ly);  // the neighbour's border: this tile's halo
+                win[j] = WD::sites((ly + dy * TE) * S + lx + dx * TE, stride, n);
+                res_border<LTE>(dx, dy, i * SPW, lx, ly);  // this tile's border facing d
+                wout[j] = WD::sites(ly * S + lx, stride, n);
+            } else {
+                win[j] = wout[j] = WD::delta(sh.dl[d][i - (corner ? 1 : WD::EDGE)]);
+            }
+            need |= 1u << j;
+        }
+    }
+    const ResLat<ST, LTE> lat{sh.tile, x0, y0, k.lgL};
+    if (tid == 0) sh.nq[0] = sh.nq[1] = 0;
+    __syncthreads();
+    // this thread's pair: same-colour row j, x-neighbouring pair jc
+    const int j = tid / (TE / 4), jc = tid % (TE / 4);
+    const unsigned int Gp = 4u * (unsigned int)a.nmcs;
+    sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, tid);
+    for (unsigned int p = 0;; p++) {
+        const bool work = p < Gp;  // p == Gp: receive the last phase's deltas only
+        // The neighbours' packets of phase p - 1 (halo sites and deltas):
+        // their loads are issued first and checked after this phase's own
+        // interior work.
+        const uint32_t par_in = ((p - 1) & 1u) * 8u * mstride;
+        unsigned long long v[WPT];
+        uint32_t got = p > 0 ? 0u : need;
+#pragma unroll
+        for (int jj = 0; jj < WPT; jj++)
+            if (p > 0 && (need >> jj & 1u))
+                v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned int* const nq = &sh.nq[p & 1u];
+        if (work) {
+            const int col = (int)(p & 3u);
+            int cx, cy, cz;
+            colour_of<2>(col, cx, cy, cz);
+            const uint4 dw = sh.draws[tid];  // (made in the previous phase)
+            const uint32_t wp[4] = {dw.x, dw.y, dw.z, dw.w};
+            const int ly = 2 * j + cy + 1;
+            ResEntry e[2];
+            bool act[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                int dir = (int)(((uint64_t)(h ? wp[3] : wp[1]) * 8ull) >> 32);
+                dir += dir >= 4;
+                const int lx = 4 * jc + cx + 2 * h + 1;
+                const bool ring = lx == 1 || lx == TE || ly == 1 || ly == TE;
+                e[h].li = (uint32_t)(ly * S + lx) | (ring ? kResRing : 0u);
+                e[h].w2 = h ? wp[0] : wp[2];
+                e[h].meta = (uint32_t)dir;
+                if (ring) {
+                    act[h] = true;  // queued as it is: its source may lie in the halo
+                } else {
                     int ox, oy, oz;
                     block_offset<2>(dir, ox, oy, oz);
-                    const int li = ly * S + (4 * jc + cx + 2 * h + 1);
-                    const uint32_t vt = tile[li], vs = tile[li + oy * S + ox];
-                    act_[2 * r + h] = vs != vt;  // kernels2d.cu:276
-                    e[2 * r + h].site = (uint32_t)li;
-                    e[2 * r + h].vs = vs;
-                    e[2 * r + h].w2 = w2;
-                    e[2 * r + h].aux = (uint32_t)dir;
-                    e[2 * r + h].set_vt(vt);
+                    const int li = ly * S + lx;
+                    e[h].vt = sh.tile[li];
+                    e[h].vs = sh.tile[li + oy * S + ox];
+                    act[h] = e[h].vs != e[h].vt;  // kernels2d.cu:276
+                }
+            }
+            // queue: one LDS atomic per wave
+            const uint64_t m0 = __ballot(act[0]), m1 = __ballot(act[1]);
+            const uint32_t total = (uint32_t)(__popcll(m0) + __popcll(m1));
+            if (total != 0) {
+                uint32_t base = 0;
+                if (__lane_id() == 0) base = atomicAdd(nq, total);
+                base = __shfl(base, 0);
+                const uint32_t b0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u));
+                const uint32_t b1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+                if (act[0]) sh.queue[base + b0] = e[0];
+                if (act[1]) sh.queue[base + (uint32_t)__popcll(m0) + b1] = e[1];
+            }
+        }
+        __syncthreads();
+        const int nqp = (int)*nq;
+        if (work) {
+            if (tid == 0) sh.nq[(p + 1) & 1u] = 0;  // (last read in phase p - 1)
+            // the previous phase's deltas are sent and in the rows
+            for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
+            // prepare the interior attempts while the packets travel
+            for (int qi = tid; qi < nqp; qi += NT) {
+                ResEntry& qe = sh.queue[qi];
+                if (qe.li & kResRing) continue;
+                if constexpr (LEAN)
+                    res_prepare_lean<ST, LTE>(sh.tile, sh.p4, k.idbits, qe);
+                else
+                    res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+            }
+        }
+        if (p > 0) {
+            long polls = 0;
+            for (;;) {
+#pragma unroll
+                for (int jj = 0; jj < WPT; jj++)
+                    if (((need & ~got) >> jj & 1u) && (uint32_t)(v[jj] >> 32) == p) got |= 1u << jj;
+                if (got == need) break;
+                if ((++polls & 63) == 0 &&
+                    (__hip_atomic_load(a.bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                     __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u))
+                    break;
+                if (polls > kBarrierSpinCap) {
+                    __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int jj = 0; jj < WPT; jj++)
+                    if ((need & ~got) >> jj & 1u)
+                        v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (got != need) {
+                sh.quit = 1u;
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < WPT; jj++) {
+                    if (!(need >> jj & 1u)) continue;
+                    const uint32_t x = win[jj], pl = (uint32_t)v[jj];
+                    if (x >> 31) {
+                        const int32_t pk = (int32_t)pl;
+                        const int32_t dA = ((pk & 0xFFF) ^ 0x800) - 0x800;
+                        const int64_t dP = (int64_t)((pk - dA) >> 12);
+                        atomicAdd(reinterpret_cast<unsigned long long*>(&sh.R[x & 0xFFu]),
+                                  (unsigned long long)(dP * 4294967296LL + dA));
+                    } else {
+                        const int li = (int)(x & 0xFFFFu), stride = (int)((x >> 16) & 0xFFFu);
+                        sh.tile[li] = (ST)(SPW == 2 ? pl & 0xFFFFu : pl);
+                        if (SPW == 2 && (x >> 28) > 1u) sh.tile[li + stride] = (ST)(pl >> 16);
+                    }
                 }
             }
         }
-        uint64_t masks[2 * PPT];
-        uint32_t total = 0;
-#pragma unroll
-        for (int r = 0; r < 2 * PPT; r++) {
-            masks[r] = __ballot(act_[r]);
-            total += (uint32_t)__popcll(masks[r]);
+        __syncthreads();
+        if (sh.quit) return;
+        if (!work) break;
+        // the ring's attempts (their blocks reach the halo), then every
+        // queued attempt's rows, acceptance and commit
+        // The upper half of the threads, idle here unless the queue is long,
+        // makes the next phase's draws: two pairs each.
+        if (tid >= NT / 2 && p + 1 < Gp) {
+            sh.draws[tid - NT / 2] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid - NT / 2);
+            sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid);
         }
-        if (total != 0) {
-            uint32_t base = 0;
-            if (__lane_id() == 0) base = atomicAdd(nqueued, total);
-            base = __shfl(base, 0);
-#pragma unroll
-            for (int r = 0; r < 2 * PPT; r++) {
-                const uint32_t before = __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t)(masks[r] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)masks[r], 0u));
-                if (act_[r]) queue[base + before] = e[r];
-                base += (uint32_t)__popcll(masks[r]);
+        const TileTables T{map, sh.R, sh.D[p & 1u], &sh.miss};
+        bool miss = false;
+        for (int qi = tid; qi < nqp; qi += NT) {
+            ResEntry& qe = sh.queue[qi];
+            if (qe.li & kResRing) {
+                const uint32_t li = qe.li & ~kResRing;
+                int ox, oy, oz;
+                block_offset<2>((int)(qe.meta & 0xFu), ox, oy, oz);
+                qe.vt = sh.tile[li];
+                qe.vs = sh.tile[(int)li + oy * S + ox];
+                qe.li = li;
+                if (qe.vs == qe.vt) continue;  // kernels2d.cu:276
+                if constexpr (LEAN)
+                    res_prepare_lean<ST, LTE>(sh.tile, sh.p4, k.idbits, qe);
+                else
+                    res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
             }
+            if constexpr (LEAN)
+                res_commit_lean<ST, LTE>(sh.tile, map, sh.R, sh.D[p & 1u], sh.thr, k.nthr, sh.p4, k.idbits,
+                                         k.idmask, qe, miss);
+            else
+                res_commit<ST, LTE>(sh.tile, k, P, T, sh.thr, thr_lds, qe);
         }
+        if (miss) sh.miss = 1u;
         __syncthreads();
-        const int nq = (int)*nqueued;
-        for (int j = threadIdx.x; j < nq; j += kResThreads) {
-            const QueuedAttempt<2, false, ST>& q = queue[j];
-            const int li = (int)q.site, ly = li / S, lx = li - ly * S;
-            Draw<2, TileLat<ST>> d;
-            d.t = typename TileLat<ST>::Pos{li, lx, ly,
-                                            (uint32_t)(x0 + lx - 1) | ((uint32_t)(y0 + ly - 1) << k.lgL)};
-            d.w[0] = d.w[1] = d.w[3] = 0u;
-            d.w[2] = q.w2;
-            lat.block(d.t, d.nv);
-            d.vs = q.vs;
-            d.dir = q.dir();
-            d.tick = mcs;
-            finish_attempt<2, true>(lat, k, P, k.thr, T, CellSync{0, nullptr, 0}, 0u, d);
+        if (sh.miss) {  // a cell this tile does not hold: the guard
+            if (tid == 0) atomicOr(a.bad, kResGuardMiss);
+            return;
         }
-        // every store and atomic of this tile's phase completes, then the
-        // tile's counter says so
-        wait_memory();
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(myflag, p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // this tile's packets of phase p, tag p + 1
+        const uint32_t par_out = (p & 1u) * 8u * mstride;
+#pragma unroll
+        for (int jj = 0; jj < WPT; jj++) {
+            if (!(need >> jj & 1u)) continue;
+            const uint32_t x = wout[jj];
+            uint32_t pl;
+            if (x >> 31) {
+                const int64_t dv = sh.D[p & 1u][x & 0xFFu];
+                const int32_t dA = (int32_t)(uint32_t)(uint64_t)dv;
+                const int32_t dP = (int32_t)((dv - dA) >> 32);
+                pl = (uint32_t)(dP * 4096 + dA);
+            } else {
+                const int li = (int)(x & 0xFFFFu), stride = (int)((x >> 16) & 0xFFFu);
+                pl = (uint32_t)sh.tile[li];
+                if (SPW == 2 && (x >> 28) > 1u) pl |= (uint32_t)sh.tile[li + stride] << 16;
+            }
+            __hip_atomic_store(a.mail + wdst[jj] + par_out, ((unsigned long long)(p + 1) << 32) | pl,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the tile's own deltas join the rows (the neighbours' deltas of
+        // this phase arrive with atomics too, possibly at the same time)
+        for (int s = tid; s < nslots; s += NT) {
+            const int64_t dv = sh.D[p & 1u][s];
+            if (dv != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&sh.R[s]), (unsigned long long)dv);
+        }
     }
-    // the last phase's deltas into the lagging buffer, once the neighbours
-    // have finished reading it
-    if (!resident_wait(a, tx, ty, G)) return;
-    int64_t* lag = (G & 1u) ? k.tab[0] : k.tab[1];  // tab[(G + 1) & 1]
-    const unsigned int nd = dl->n;
-    for (unsigned int i = threadIdx.x; i < nd; i += kResThreads) atomic_add_i64(lag + dl->cell[i], dl->delta[i]);
-    // the tile's sites (border sites are already there)
-    for (int i = threadIdx.x; i < TE * TE; i += kResThreads) {
-        const int ly = i >> a.lte, lx = i & (TE - 1);
-        glat[(uint32_t)(x0 + lx) | ((uint32_t)(y0 + ly) << k.lgL)] = tile[(ly + 1) * S + lx + 1];
+    // the tile's sites and the rows of its cells (every holder has the same)
+    for (int i = tid; i < TE * TE; i += NT) {
+        const int ly = i >> LTE, lx = i & (TE - 1);
+        glat[(uint32_t)(x0 + lx) | ((uint32_t)(y0 + ly) << k.lgL)] = sh.tile[(ly + 1) * S + lx + 1];
+    }
+    for (int s = tid; s < nslots; s += NT) {
+        k.tab[0][sh.cell[s]] = sh.R[s];
+        k.tab[1][sh.cell[s]] = sh.R[s];
     }
 }
 
-template <class ST>
-__global__ __launch_bounds__(kResThreads) void k_resident2d(ResidentArgs a) {
-    extern __shared__ __align__(16) unsigned char res_smem[];
-    __shared__ QueuedAttempt<2, false, ST> queue[2 * 2 * kResThreads];
-    __shared__ DeferList dl;
-    __shared__ unsigned int nqueued;
-    ST* tile = reinterpret_cast<ST*>(res_smem);
-    if (a.k.nt <= kSmallTypes) {
+// VAR 0: the lean attempt path; 1: the general one; 2: the general one
+// with the chemotaxis term.
+template <class ST, int LTE, int VAR>
+__global__ __launch_bounds__(ResGeom<LTE>::NT) void k_resident2d(ResidentArgs a) {
+    extern __shared__ __align__(16) unsigned char res_map[];
+    __shared__ ResShared<ST, LTE> sh;
+    if (VAR == 0 || a.k.nt <= kSmallTypes) {
         SmallParams P;
         P.load(a.k.prm);
-        resident_run<ST>(a, P, tile, queue, &nqueued, &dl);
+        resident_run<ST, LTE, VAR == 0, VAR == 2>(a, P, sh, res_map);
     } else {
-        const LdsParamView P{a.k.prm->w, a.k.nt, a.k.nt * a.k.nt};  // (read from global memory)
-        resident_run<ST>(a, P, tile, queue, &nqueued, &dl);
+        const int npar = param_words(a.k.nt);
+        for (int i = (int)threadIdx.x; i < npar; i += ResGeom<LTE>::NT) sh.prm[i] = a.k.prm->w[i];
+        __syncthreads();
+        const LdsParamView P{sh.prm, a.k.nt, a.k.nt * a.k.nt};
+        resident_run<ST, LTE, false, VAR == 2>(a, P, sh, res_map);
     }
 }
 
@@ -2035,57 +2472,94 @@ __global__ void k_bbox_init(int32_t* bbox, int rows) {
     bbox[4 * c + 3] = INT_MIN;
 }
 
-// The guard: a cell whose box, widened by one site, spans more than two
-// tiles per axis (width + 2 > TE + 1) sets *bad.
-__global__ void k_bbox_check(const int32_t* __restrict__ bbox, int rows, int te, unsigned int* bad) {
-    const int c = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+// Holder words: the tiles whose sites or halo a cell's box overlaps, the box
+// widened per axis by the largest margin up to res_margin that keeps it
+// within two tiles (res_holds).  A cell whose box with its halo alone spans
+// more than two tiles on an axis sets *bad; a cell with no site is held by no
+// tile.
+__device__ __forceinline__ bool res_axis(int lo, int hi, int lte, int mmax, int& t0, int& wide) {
+    for (int m = mmax; m >= 0; m--) {
+        const int a = (lo - 1 - m) >> lte, b = (hi + 1 + m) >> lte;  // (floor)
+        if (b - a <= 1) {
+            t0 = a;
+            wide = b - a;
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ void k_res_holders(const int32_t* __restrict__ bbox, const int32_t* __restrict__ anchor, int rows,
+                              int lte, int lnt, uint32_t* holders, unsigned int* bad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= rows) return;
     const int32_t* q = bbox + 4 * c;
-    if (q[1] < q[0]) return;  // no site
-    if ((q[1] - q[0] + 1) + 2 > te + 1 || (q[3] - q[2] + 1) + 2 > te + 1) atomicOr(bad, 1u);
+    if (c == 0 || q[1] < q[0]) {
+        holders[c] = 0u;
+        return;
+    }
+    const int mmax = res_margin(1 << lte);
+    int tx0, wx, ty0, wy;
+    if (!res_axis(anchor[3 * c] + q[0], anchor[3 * c] + q[1], lte, mmax, tx0, wx) ||
+        !res_axis(anchor[3 * c + 1] + q[2], anchor[3 * c + 1] + q[3], lte, mmax, ty0, wy)) {
+        atomicOr(bad, kResGuardWide);
+        holders[c] = 0u;
+        return;
+    }
+    const int ntm = (1 << lnt) - 1;
+    holders[c] = (1u << 31) | (uint32_t)(tx0 & ntm) | ((uint32_t)(ty0 & ntm) << 12) | ((uint32_t)wx << 24) |
+                 ((uint32_t)wy << 25);
 }
 
-size_t resident_lds_bytes(int lte, bool narrow) {
-    const size_t S = ((size_t)1 << lte) + 2;
-    return S * S * (narrow ? 2 : 4);
+template <class ST, int LTE>
+static const void* resident_kernel(int var) {
+    return var == 0 ? (const void*)&k_resident2d<ST, LTE, 0>
+                    : var == 1 ? (const void*)&k_resident2d<ST, LTE, 1> : (const void*)&k_resident2d<ST, LTE, 2>;
 }
 
-hipError_t resident_max_blocks(bool narrow, int lte, int* max_blocks) {
+static const void* resident_kernel(bool narrow, int lte, int var) {
+    if (narrow) return lte == 6 ? resident_kernel<uint16_t, 6>(var) : resident_kernel<uint16_t, 5>(var);
+    return lte == 6 ? resident_kernel<uint32_t, 6>(var) : resident_kernel<uint32_t, 5>(var);
+}
+
+size_t resident_lds_bytes(int ncells) { return ((size_t)ncells + 1 + 15) & ~(size_t)15; }
+
+int resident_threads(int lte) { return lte == 6 ? ResGeom<6>::NT : ResGeom<5>::NT; }
+
+hipError_t resident_max_blocks(bool narrow, int lte, int var, int ncells, int* max_blocks) {
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    const void* f = narrow ? (const void*)&k_resident2d<uint16_t> : (const void*)&k_resident2d<uint32_t>;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kResThreads,
-                                                          resident_lds_bytes(lte, narrow))) != hipSuccess)
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, resident_kernel(narrow, lte, var),
+                                                          resident_threads(lte), resident_lds_bytes(ncells))) !=
+        hipSuccess)
         return e;
     *max_blocks = per_cu * cus;
     return hipSuccess;
 }
 
-hipError_t launch_resident(bool narrow, const ResidentArgs& a, int blocks, hipStream_t s) {
+hipError_t launch_resident(bool narrow, int var, const ResidentArgs& a, int blocks, hipStream_t s) {
     ResidentArgs copy = a;
     void* args[] = {&copy};
-    const void* f = narrow ? (const void*)&k_resident2d<uint16_t> : (const void*)&k_resident2d<uint32_t>;
-    return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(kResThreads), args,
-                                      (unsigned int)resident_lds_bytes(a.lte, narrow), s);
+    return hipLaunchCooperativeKernel(resident_kernel(narrow, a.lte, var), dim3(blocks),
+                                      dim3(resident_threads(a.lte)), args,
+                                      (unsigned int)resident_lds_bytes(a.k.ncells), s);
 }
 
-hipError_t launch_resident_guard(bool begin, bool narrow, const SweepConsts& k, const int32_t* anchor,
-                                 int32_t* bbox, int te, unsigned int* bad, hipStream_t s) {
+hipError_t launch_resident_guard(bool narrow, const SweepConsts& k, const int32_t* anchor, int32_t* bbox,
+                                 int lte, int lnt, uint32_t* holders, unsigned int* bad, hipStream_t s) {
     const int rows = k.ncells + 1;
-    if (begin) {
-        hipLaunchKernelGGL(k_bbox_init, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows);
-        const uint32_t nsites = 1u << (2 * k.lgL);
-        if (narrow)
-            hipLaunchKernelGGL(k_bbox2d<uint16_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat16, k.lgL,
-                               k.idmask, anchor, bbox);
-        else
-            hipLaunchKernelGGL(k_bbox2d<uint32_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat, k.lgL,
-                               k.idmask, anchor, bbox);
-    } else {
-        hipLaunchKernelGGL(k_bbox_check, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows, te, bad);
-    }
+    hipLaunchKernelGGL(k_bbox_init, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows);
+    const uint32_t nsites = 1u << (2 * k.lgL);
+    if (narrow)
+        hipLaunchKernelGGL(k_bbox2d<uint16_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat16, k.lgL,
+                           k.idmask, anchor, bbox);
+    else
+        hipLaunchKernelGGL(k_bbox2d<uint32_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat, k.lgL,
+                           k.idmask, anchor, bbox);
+    hipLaunchKernelGGL(k_res_holders, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, anchor, rows, lte, lnt,
+                       holders, bad);
     return hipGetLastError();
 }
 

@@ -1,7 +1,9 @@
-"""The resident form (launch form 4: 2D tiles resident in LDS for a chunk,
-each waiting only on its eight neighbours between colour phases) against
-the oracle, bit-exact; the guard (a cell spread over more than two tiles
-per axis) sends a chunk back to one launch per phase, still exact."""
+"""The resident form (launch form 4: 2D tiles resident in LDS for a launch,
+holding the rows of their cells and exchanging border sites and shared
+cells' deltas with their eight neighbours between colour phases) against
+the oracle, bit-exact; the guard (a cell wider than two tiles, or met by a
+tile not holding it) sends a launch back to one launch per phase, still
+exact, and shortens the following launches."""
 from __future__ import annotations
 
 import numpy as np
@@ -52,11 +54,13 @@ def _cases():
         ("sorting1024", W.sorting2d(seed=401), 40, 20, 1),
         ("sorting512", W.sorting2d(L=512, grid=40, edge=10, seed=402), 30, 10, 1),
         ("plain256", W.plain2d(seed=403), 60, 30, 1),
-        ("small-cells256", W.sorting_small_cells(L=256, seed=404), 40, 40, 1),
+        # (cells that grow or drift by a few sites per ten MCS: launches of
+        # 10 MCS stay within the tiles that hold them)
+        ("small-cells256", W.sorting_small_cells(L=256, seed=404), 40, 10, 1),
         ("wall-chem256", W.chemotactic(W.walled(W.sorting2d(L=256, grid=20, edge=10, seed=405)),
                                        lam=(0.0, 20.0, 5.0, 0.0)), 30, 15, 1),
         ("wide512", W.sorting2d(L=512, grid=40, edge=10, seed=406), 20, 20, 0),
-        ("six-types256", _six_types(256, 407), 30, 30, 1),
+        ("six-types256", _six_types(256, 407), 30, 10, 1),
     ]
 
 
@@ -92,6 +96,30 @@ def test_guard_sends_a_spread_cell_back_to_per_phase_launches():
     lat_o, tab_o = run_oracle(wl, 1, 1, 509, 20)
     np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
     assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+
+
+def test_launch_length_adapts_to_moving_cells():
+    """Cells of the sorting example grow from 3 x 3 to ~40 sites and drift:
+    one launch of 40 MCS meets cells outside the tiles holding them, so it is
+    redone per phase and the next launches are shorter; every result stays
+    bit-exact, and later runs of the engine keep to launches that hold."""
+    from gpucpm import workloads as W
+    wl = W.sorting_small_cells(L=256, seed=404)
+    _announce("adaptive launches", wl.L)
+    sim = _run(wl, 503, 40, 40)
+    guard, waits = sim.resident_fallbacks
+    assert guard >= 1 and waits == 0
+    lat_o, tab_o = run_oracle(wl, 1, 1, 503, 40)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+    # a second run of 40 MCS starts with the shortened launches
+    sim.run(**run_kwargs(1, 1, 40, 40))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    lat_o2, tab_o2 = run_oracle(wl, 1, 1, 503, 80)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o2)
+    assert_tables_equal(sim.get_cell_table(), tab_o2, 2)
+    assert sim.last_launch()["form"] == 4
 
 
 def test_resident_split_runs_equal_one_run():
