@@ -325,6 +325,7 @@ struct cpm_engine {
     uint64_t ctr_mcs0 = 0, ctr_mcs_acc = 0;
     int64_t res_mcs = 0;            // resident form: MCS per launch after a guard failure (0: the chunk)
     int res_streak = 0;             // launches in a row that held since K last changed
+    bool res_auto_off = false;      // the automatic choice no longer takes the form (until a push)
     int64_t res_fallbacks = 0;      // launches redone with one launch per phase: the guard
     int64_t res_wait_fallbacks = 0; // ... or an abandoned neighbour wait
     bool autotune = false;  // cpm_set_autotune
@@ -1087,6 +1088,9 @@ int cpm_push(cpm_handle h) {
     HIP_TRY(hipMemcpyAsync(h->d_ctype, h->ctype.data(), rows * 4, hipMemcpyHostToDevice, s));
     h->wide_valid = true;
     h->narrow_valid = false;
+    h->res_auto_off = false;  // a new lattice: the resident form's launches start at full length
+    h->res_mcs = 0;
+    h->res_streak = 0;
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
     for (int i = 0; i < 2; i++) HIP_TRY(hipMemsetAsync(h->d_del[i], 0, rows * 8, s));
     HIP_TRY(hipMemsetAsync(h->d_first, 0xFF, rows * 4, s));
@@ -1361,12 +1365,17 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             plan.blocks = 1;
         }
     }
-    // The resident form (2D, shared roles, c = 1, u = 1): launch form 4.
+    // The resident form (2D, shared roles, c = 1, u = 1): launch form 4,
+    // and the automatic choice for such runs on lattices of 256^2 sites and
+    // more (unless the autotuner decides, or a guard failure that shorter
+    // launches cannot cure switched it off for the lattice pushed).
     int res_lte = 0, res_var = 0;
-    if (h->want_form == Plan::kResident) {
-        const bool ok = h->d == 2 && !k.roles && k.lc == 0 && u == 1 && !a->cell_sync && !k.refstream &&
+    const bool res_ok = h->d == 2 && !k.roles && k.lc == 0 && u == 1 && !a->cell_sync && !k.refstream &&
                         !k.act && h->L >= 128;
-        if (!ok)
+    const bool res_auto = h->want_form == -1 && plan.mode == Plan::kPhase && res_ok && h->L >= 256 &&
+                          !h->autotune && !h->res_auto_off;
+    if (h->want_form == Plan::kResident || res_auto) {
+        if (!res_ok)
             return fail(CPM_EINVAL,
                         "launch form 4 (resident tiles) needs 2D, shared = 1, positions_per_checkerboard = 1, "
                         "updates_per_checkerboard_switch = 1, L >= 128, and no cell_sync, reference stream or "
@@ -1383,11 +1392,13 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         int maxb = 0;
         HIP_TRY(gcpm::resident_max_blocks(h->narrow_bits() != 0, res_lte, res_var, h->n, &maxb));
         const int tiles = 1 << (2 * (h->lgL - res_lte));
-        if (tiles > maxb) return fail(CPM_EINVAL, "launch form 4: the tiles do not fit co-resident");
-        plan = Plan{};
-        plan.mode = Plan::kResident;
-        plan.blocks = tiles;
-        plan.ltile = res_lte;
+        if (tiles > maxb && !res_auto) return fail(CPM_EINVAL, "launch form 4: the tiles do not fit co-resident");
+        if (tiles <= maxb) {
+            plan = Plan{};
+            plan.mode = Plan::kResident;
+            plan.blocks = tiles;
+            plan.ltile = res_lte;
+        }
     }
     h->last_plan = plan;
     const int64_t updates_per_mcs = (int64_t)k.rounds * k.phases_per_round * ((int64_t)1 << (lh * h->d)) * u;
@@ -1476,6 +1487,12 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                 HIP_TRY(hipStreamSynchronize(h->stream));
                 if (fl[1]) {
                     h->res_fallbacks++;
+                    // a cell too wide, too many cells for a tile, or misses
+                    // even in the shortest launches: the automatic choice
+                    // stops taking the form for this lattice
+                    if (h->want_form == -1 &&
+                        ((fl[1] & ~gcpm::kResGuardMiss) != 0 || n <= kResMinMcs))
+                        h->res_auto_off = true;
                     K = std::max<int64_t>(kResMinMcs, n / 2);
                     h->res_mcs = K;
                     h->res_streak = 0;

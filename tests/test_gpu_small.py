@@ -94,13 +94,17 @@ def test_interactive_frames_choose_small_form_and_match_oracle():
     assert_tables_equal(sim.get_cell_table(), tab_o, 2)
 
 
-def test_auto_keeps_per_phase_launches_for_large_phases():
+def test_auto_spreads_large_phases_over_resident_tiles():
     """256^2 with c = 1 has 16384 attempts per phase: one CU would be slower
-    than a launch per phase, so the automatic choice keeps per-phase launches."""
+    than many, so the automatic choice runs the phases on resident tiles
+    (launch form 4); with c = 2 and u = 4, which that form does not take,
+    it launches once per phase."""
     from gpucpm import workloads as W
     wl = W.plain2d()
     _announce("auto", wl.L)
     sim = wl.make_cpm()
     sim.push_to_gpu()
     sim.run(**run_kwargs(1, 1, 2))
+    assert sim.last_launch()["form"] == 4
+    sim.run(**run_kwargs(2, 4, 2))
     assert sim.last_launch()["form"] == 0
