@@ -1787,7 +1787,10 @@ constexpr uint8_t kResNone = 0xFF;  // a cell the tile does not hold
 template <int LTE>
 struct ResGeom {
     static constexpr int TE = 1 << LTE;
-    static constexpr int S = TE + 2;        // LDS row stride (tile + halo)
+    // LDS row stride: the tile and its halo need TE + 2; one more makes it
+    // odd, so that same-colour rows (two apart) of 16-bit sites start on
+    // banks of opposite parity
+    static constexpr int S = TE + 3;
     static constexpr int NT = TE * TE / 8;  // threads: two same-colour sites (one Philox block) each
     static constexpr int NQ = TE * TE / 4;  // attempts per tile and phase
 };
@@ -1866,6 +1869,16 @@ struct ResParams4 {
     int4 T[4];   // lambda_area, target area, lambda_perimeter, target perimeter of type a
 };
 
+// The queue in LDS, one array per field: consecutive lanes touch
+// consecutive words (a 32-byte record per entry would put eight lanes on
+// one bank).
+template <int NQ>
+struct ResQueue {
+    uint32_t li[NQ], vs[NQ], vt[NQ], w2[NQ], meta[NQ];
+    int64_t e[NQ];
+    __device__ __forceinline__ ResEntry get(int i) const { return {li[i], vs[i], vt[i], w2[i], meta[i], 0u, e[i]}; }
+};
+
 template <class ST, int LTE>
 struct ResShared {
     using G = ResGeom<LTE>;
@@ -1881,7 +1894,7 @@ struct ResShared {
     int32_t prm[kParamWords];    // more than four types: the parameters, staged
     ResParams4 p4;               // the lean path's parameters
     int scan[G::NT];
-    ResEntry queue[G::NQ];       // this phase's attempts whose source and target differ, and the ring's
+    ResQueue<G::NQ> queue;       // this phase's attempts whose source and target differ, and the ring's
     uint4 draws[G::NT];          // the Philox block of each pair of same-colour sites, for the next phase
     unsigned int nq[2];          // queue length of the phases of each parity
     unsigned int miss, quit;
@@ -2264,8 +2277,16 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u));
                 const uint32_t b1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
-                if (act[0]) sh.queue[base + b0] = e[0];
-                if (act[1]) sh.queue[base + (uint32_t)__popcll(m0) + b1] = e[1];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (!act[h]) continue;
+                    const uint32_t i = base + (h ? (uint32_t)__popcll(m0) + b1 : b0);
+                    sh.queue.li[i] = e[h].li;
+                    sh.queue.vs[i] = e[h].vs;
+                    sh.queue.vt[i] = e[h].vt;
+                    sh.queue.w2[i] = e[h].w2;
+                    sh.queue.meta[i] = e[h].meta;
+                }
             }
         }
         __syncthreads();
@@ -2276,12 +2297,15 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
             // prepare the interior attempts while the packets travel
             for (int qi = tid; qi < nqp; qi += NT) {
-                ResEntry& qe = sh.queue[qi];
-                if (qe.li & kResRing) continue;
+                if (sh.queue.li[qi] & kResRing) continue;
+                ResEntry qe = sh.queue.get(qi);
                 if constexpr (LEAN)
                     res_prepare_lean<ST, LTE>(sh.tile, sh.p4, k.idbits, qe);
                 else
                     res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+                sh.queue.vs[qi] = qe.vs;
+                sh.queue.meta[qi] = qe.meta;
+                sh.queue.e[qi] = qe.e;
             }
         }
         if (p > 0) {
@@ -2341,7 +2365,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         const TileTables T{map, sh.R, sh.D[p & 1u], &sh.miss};
         bool miss = false;
         for (int qi = tid; qi < nqp; qi += NT) {
-            ResEntry& qe = sh.queue[qi];
+            ResEntry qe = sh.queue.get(qi);
             if (qe.li & kResRing) {
                 const uint32_t li = qe.li & ~kResRing;
                 int ox, oy, oz;
