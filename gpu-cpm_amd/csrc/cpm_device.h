@@ -177,8 +177,8 @@ struct SmallArgs {
     int thr_lds;            // 1: the acceptance thresholds are staged in LDS too
 };
 
-// The resident form (2D, shared roles, c = 1, u = 1): one cooperative block
-// per tile of 2^lte x 2^lte sites, 2^lnt tiles per axis, for one chunk.  A
+// The resident form (2D, u = 1): one cooperative block per tile of 2^lte x
+// 2^lte sites, 2^lnt tiles per axis, for one launch.  A
 // tile keeps its sites, a one-site halo and the rows of the cells it holds
 // in LDS; neighbouring tiles exchange border sites and the deltas of the
 // cells both hold through tagged mailbox words.
@@ -198,6 +198,7 @@ struct ResidentArgs {
     uint32_t mcs0;
     int nmcs;
     int lte, lnt;
+    int gen;                   // 0: c = 1 with shared roles (pairs of sites); 1: any schedule with u = 1
     unsigned long long* mail;  // [tiles][2][8][res_mail_stride(2^lte)] tagged words, 0 on entry
     const uint32_t* holders;   // [n + 1] the tiles holding each cell (0: none)
     unsigned int* err;         // abandoned-wait flag, 0 on entry

@@ -1365,21 +1365,20 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             plan.blocks = 1;
         }
     }
-    // The resident form (2D, shared roles, c = 1, u = 1): launch form 4,
-    // and the automatic choice for such runs on lattices of 256^2 sites and
-    // more (unless the autotuner decides, or a guard failure that shorter
-    // launches cannot cure switched it off for the lattice pushed).
+    // The resident form (2D, u = 1): launch form 4, and the automatic choice
+    // for such runs on lattices of 256^2 sites and more whose phases are too
+    // large for one workgroup (unless the autotuner decides, or a guard
+    // failure that shorter launches cannot cure switched it off for the
+    // lattice pushed).
     int res_lte = 0, res_var = 0;
-    const bool res_ok = h->d == 2 && !k.roles && k.lc == 0 && u == 1 && !a->cell_sync && !k.refstream &&
-                        !k.act && h->L >= 128;
+    const bool res_ok = h->d == 2 && u == 1 && !a->cell_sync && !k.refstream && !k.act && h->L >= 128;
     const bool res_auto = h->want_form == -1 && plan.mode == Plan::kPhase && res_ok && h->L >= 256 &&
                           !h->autotune && !h->res_auto_off;
     if (h->want_form == Plan::kResident || res_auto) {
         if (!res_ok)
             return fail(CPM_EINVAL,
-                        "launch form 4 (resident tiles) needs 2D, shared = 1, positions_per_checkerboard = 1, "
-                        "updates_per_checkerboard_switch = 1, L >= 128, and no cell_sync, reference stream or "
-                        "act term");
+                        "launch form 4 (resident tiles) needs 2D, updates_per_checkerboard_switch = 1, L >= 128, "
+                        "and no cell_sync, reference stream or act term");
         // tiles of 32 or 64 sites, at least four per axis: up to 256 of
         // them at L <= 1024, and wide enough that cells a few tens of sites
         // across keep within two
@@ -1435,6 +1434,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         unsigned int* bad = err + 1;
         ra.lte = res_lte;
         ra.lnt = h->lgL - res_lte;
+        ra.gen = !(k.lc == 0 && !k.roles);
         ra.nmcs = a->inner_iterations;
         ra.mail = h->d_res_mail;
         ra.holders = h->d_res_flags + 2;

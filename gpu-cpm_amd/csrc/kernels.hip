@@ -2222,8 +2222,8 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     __syncthreads();
     // this thread's pair: same-colour row j, x-neighbouring pair jc
     const int j = tid / (TE / 4), jc = tid % (TE / 4);
-    const unsigned int Gp = 4u * (unsigned int)a.nmcs;
-    sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, tid);
+    const unsigned int Gp = (unsigned int)(a.nmcs * k.rounds * k.phases_per_round);
+    if (!a.gen) sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, tid);
     for (unsigned int p = 0;; p++) {
         const bool work = p < Gp;  // p == Gp: receive the last phase's deltas only
         // The neighbours' packets of phase p - 1 (halo sites and deltas):
@@ -2237,36 +2237,8 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             if (p > 0 && (need >> jj & 1u))
                 v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned int* const nq = &sh.nq[p & 1u];
-        if (work) {
-            const int col = (int)(p & 3u);
-            int cx, cy, cz;
-            colour_of<2>(col, cx, cy, cz);
-            const uint4 dw = sh.draws[tid];  // (made in the previous phase)
-            const uint32_t wp[4] = {dw.x, dw.y, dw.z, dw.w};
-            const int ly = 2 * j + cy + 1;
-            ResEntry e[2];
-            bool act[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                int dir = (int)(((uint64_t)(h ? wp[3] : wp[1]) * 8ull) >> 32);
-                dir += dir >= 4;
-                const int lx = 4 * jc + cx + 2 * h + 1;
-                const bool ring = lx == 1 || lx == TE || ly == 1 || ly == TE;
-                e[h].li = (uint32_t)(ly * S + lx) | (ring ? kResRing : 0u);
-                e[h].w2 = h ? wp[0] : wp[2];
-                e[h].meta = (uint32_t)dir;
-                if (ring) {
-                    act[h] = true;  // queued as it is: its source may lie in the halo
-                } else {
-                    int ox, oy, oz;
-                    block_offset<2>(dir, ox, oy, oz);
-                    const int li = ly * S + lx;
-                    e[h].vt = sh.tile[li];
-                    e[h].vs = sh.tile[li + oy * S + ox];
-                    act[h] = e[h].vs != e[h].vt;  // kernels2d.cu:276
-                }
-            }
-            // queue: one LDS atomic per wave
+        // Queue one or two attempts per lane: one LDS atomic per wave.
+        auto enqueue = [&](const ResEntry* e, const bool* act) {
             const uint64_t m0 = __ballot(act[0]), m1 = __ballot(act[1]);
             const uint32_t total = (uint32_t)(__popcll(m0) + __popcll(m1));
             if (total != 0) {
@@ -2287,6 +2259,87 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                     sh.queue.w2[i] = e[h].w2;
                     sh.queue.meta[i] = e[h].meta;
                 }
+            }
+        };
+        // An attempt on target (lx, ly) of the tile with source direction
+        // dir: the ring's (its 3 x 3 block reaches the halo) are queued as
+        // they are, the others when their sites differ.
+        auto stage = [&](int lx, int ly, int dir, uint32_t w2, ResEntry& e, bool& act) {
+            const bool ring = lx == 1 || lx == TE || ly == 1 || ly == TE;
+            const int li = ly * S + lx;
+            e.li = (uint32_t)li | (ring ? kResRing : 0u);
+            e.w2 = w2;
+            e.meta = (uint32_t)dir;
+            if (ring) {
+                act = true;
+            } else {
+                int ox, oy, oz;
+                block_offset<2>(dir, ox, oy, oz);
+                e.vt = sh.tile[li];
+                e.vs = sh.tile[li + oy * S + ox];
+                act = e.vs != e.vt;  // kernels2d.cu:276
+            }
+        };
+        if (work && !a.gen) {
+            // c = 1, shared roles: each thread takes an x-neighbouring pair of
+            // same-colour sites, whose Philox block was made in the previous
+            // phase
+            const int col = (int)(p & 3u);
+            int cx, cy, cz;
+            colour_of<2>(col, cx, cy, cz);
+            const uint4 dw = sh.draws[tid];
+            const uint32_t wp[4] = {dw.x, dw.y, dw.z, dw.w};
+            const int ly = 2 * j + cy + 1;
+            ResEntry e[2];
+            bool act[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                int dir = (int)(((uint64_t)(h ? wp[3] : wp[1]) * 8ull) >> 32);
+                dir += dir >= 4;
+                stage(4 * jc + cx + 2 * h + 1, ly, dir, h ? wp[0] : wp[2], e[h], act[h]);
+            }
+            enqueue(e, act);
+        } else if (work) {
+            // Any schedule with u = 1 (kernels2d.cu:72-106,197-243): phase p
+            // is phase pc of round r of an MCS, and the tile makes every
+            // attempt of it whose target lies in the tile (with non-shared
+            // roles the target may be a sub-block's neighbour site; the
+            // attempt then belongs to the target's tile, which holds all the
+            // sites the attempt reads).  Candidate sub-blocks: the active ones
+            // within c sites of the tile.
+            const uint32_t per_mcs = (uint32_t)(k.rounds * k.phases_per_round);
+            const uint32_t mcs = a.mcs0 + p / per_mcs;
+            const uint32_t phase = p % per_mcs;  // round * phases_per_round + pc
+            int ox, oy, oz;
+            phase_offsets_dev<2>(k, (int)(phase % (uint32_t)k.phases_per_round), ox, oy, oz);
+            const int lc = k.lc, c = 1 << lc, cm = c - 1, sbs = c << k.lstride;
+            const int fx = x0 - c + (((ox * c - (x0 - c)) % sbs) + sbs) % sbs;
+            const int fy = y0 - c + (((oy * c - (y0 - c)) % sbs) + sbs) % sbs;
+            const int ncx = (x0 + TE - fx) / sbs + 1, ncy = (y0 + TE - fy) / sbs + 1;
+            for (int c0 = 0; c0 < ncx * ncy; c0 += NT) {
+                const int ci = c0 + tid;
+                ResEntry e[2];
+                bool act[2] = {false, false};
+                if (ci < ncx * ncy) {
+                    const int bx = ((fx + (ci % ncx) * sbs) & Lm) >> lc;
+                    const int by = ((fy + (ci / ncx) * sbs) & Lm) >> lc;
+                    uint32_t w[4];
+                    attempt_words(k, sub_block_id<2>(k, bx, by, 0), bx, mcs, phase, 0u, w);
+                    int px = (bx << lc) + (int)(w[0] & (uint32_t)cm);
+                    int py = (by << lc) + (int)((w[0] >> lc) & (uint32_t)cm);
+                    int dir = (int)(((uint64_t)w[1] * 8ull) >> 32);
+                    dir += dir >= 4;
+                    if (k.roles) {  // the drawn site is the source, its neighbour the target
+                        int ddx, ddy, ddz;
+                        block_offset<2>(dir, ddx, ddy, ddz);
+                        px = (px + ddx) & Lm;
+                        py = (py + ddy) & Lm;
+                        dir = 8 - dir;
+                    }
+                    const int lx = ((px - x0) & Lm) + 1, ly = ((py - y0) & Lm) + 1;
+                    if (lx <= TE && ly <= TE) stage(lx, ly, dir, w[2], e[0], act[0]);
+                }
+                enqueue(e, act);
             }
         }
         __syncthreads();
@@ -2358,7 +2411,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // queued attempt's rows, acceptance and commit
         // The upper half of the threads, idle here unless the queue is long,
         // makes the next phase's draws: two pairs each.
-        if (tid >= NT / 2 && p + 1 < Gp) {
+        if (!a.gen && tid >= NT / 2 && p + 1 < Gp) {
             sh.draws[tid - NT / 2] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid - NT / 2);
             sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid);
         }

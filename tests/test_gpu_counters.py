@@ -27,10 +27,11 @@ def _cases():
         ("2d-act-split", W.active(W.sorting2d(L=64, grid=5, edge=10, seed=607)), 1, 1, 1, 0, 0, 10),
         ("2d-small", W.sorting2d(L=128, grid=9, edge=10, seed=608), 2, 1, 0, 0, 3, 8),
         ("2d-resident", W.sorting2d(L=256, grid=20, edge=10, seed=609), 1, 1, 1, 0, 4, 12),
+        ("2d-resident-ns-c2", W.sorting2d(L=256, grid=20, edge=10, seed=610), 2, 1, 0, 0, 4, 10),
     ]
 
 
-@pytest.mark.parametrize("idx", range(9))
+@pytest.mark.parametrize("idx", range(10))
 def test_counters_equal_oracle(idx):
     name, wl, c, u, shared, sync, form, n = _cases()[idx]
     print("CASE counters", name, flush=True)

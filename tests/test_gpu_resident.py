@@ -82,6 +82,49 @@ def test_resident_matches_oracle(idx):
         assert "k_resident2d" in launch["kernel"]
 
 
+def _general_cases():
+    from gpucpm import workloads as W
+    return [  # name, workload, c, shared, positions_per_thread, nmcs, inner, narrow
+        # the reference sorting.py schedule (non-shared roles, c 2, ppt 4)
+        ("sortingpy1024", W.sorting2d(seed=421), 2, 0, 4, 8, 8, 1),
+        ("sortingpy512", W.sorting2d(L=512, grid=40, edge=10, seed=422), 2, 0, 4, 20, 20, 1),
+        ("ns-c1-256", W.sorting2d(L=256, grid=20, edge=10, seed=423), 1, 0, 2, 20, 10, 1),
+        ("ns-c4-256", W.plain2d(seed=424), 4, 0, 8, 20, 20, 1),
+        ("shared-c2-256", W.sorting2d(L=256, grid=20, edge=10, seed=425), 2, 1, 4, 20, 20, 1),
+        ("chem-ns-c2-256", W.chemotactic(W.sorting2d(L=256, grid=20, edge=10, seed=426), lam=(0.0, 20.0, 5.0)),
+         2, 0, 4, 20, 20, 1),
+        ("six-types-ns-256", _six_types(256, 427), 2, 0, 4, 20, 10, 1),
+        ("wide-ns-512", W.sorting2d(L=512, grid=40, edge=10, seed=428), 2, 0, 4, 10, 10, 0),
+    ]
+
+
+@pytest.mark.parametrize("idx", range(8))
+def test_resident_general_schedules_match_oracle(idx):
+    """Any schedule with u = 1: each tile makes the attempts whose target it
+    holds (with non-shared roles the target may be a sub-block's neighbour
+    site in another tile), bit-exact against the oracle."""
+    from helpers import ENGINE_OPTIONS
+    name, wl, c, shared, ppt, n, inner, narrow = _general_cases()[idx]
+    _announce("resident general", name, wl.L, c, shared, ppt, n, inner)
+    ENGINE_OPTIONS.update(mode="resident", narrow=narrow)
+    try:
+        sim = make_cpm(wl)
+    finally:
+        ENGINE_OPTIONS.clear()
+    sim.push_to_gpu()
+    sim.set_seed(530 + idx)
+    sim.run(**run_kwargs(c, 1, n, inner, shared=shared, positions_per_thread=ppt))
+    sim.synchronize()
+    sim.pull_from_gpu()
+    launch = sim.last_launch()
+    lat_o, tab_o = run_oracle(wl, c, 1, 530 + idx, n, shared=bool(shared), ppt=ppt)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+    guard, waits = sim.resident_fallbacks
+    assert waits == 0
+    assert guard == 0 and launch["form"] == 4, (guard, launch)
+
+
 def test_guard_sends_a_spread_cell_back_to_per_phase_launches():
     """One cell of 70 x 70 sites at 256^2 (tiles of 32): it spans three tiles,
     so the chunk is restored and run with one launch per phase, exactly."""
