@@ -2082,6 +2082,59 @@ __device__ __forceinline__ uint4 res_pair_draws(const SweepConsts& k, int x0, in
     return make_uint4(wp[0], wp[1], wp[2], wp[3]);
 }
 
+// The schedule of phase p of a launch, any schedule with u = 1 (every count
+// a power of two, so shifts and masks): its MCS and Philox phase index
+// (round * phases_per_round + phase in the round, as phase_offsets_dev and
+// the per-phase kernels count them), and for a tile at (x0, y0) the active
+// sub-blocks whose targets may lie in it, origins fx + ix * sbs, fy + iy *
+// sbs (sites) for ix < ncx, iy < ncy; candidate ci = iy * ncx + ix.
+struct ResSched {
+    uint32_t mcs, phase;
+    int fx, fy, ncx, ncy, lsbs;
+    uint32_t rcx;  // ceil(2^20 / ncx): iy = (ci * rcx) >> 20 for ci < 2^10
+};
+
+template <int LTE>
+__device__ __forceinline__ ResSched res_sched(const SweepConsts& k, uint32_t mcs0, unsigned int p, int x0,
+                                              int y0) {
+    constexpr int TE = 1 << LTE;
+    const int lK = __builtin_ctz((unsigned)k.K), lS = __builtin_ctz((unsigned)k.S);
+    const int lppr = 2 * (lK + lS), lper = __builtin_ctz((unsigned)k.rounds) + lppr;
+    ResSched r;
+    r.mcs = mcs0 + (p >> lper);
+    r.phase = p & ((1u << lper) - 1u);
+    const uint32_t pc = r.phase & ((1u << lppr) - 1u);
+    const uint32_t col = pc >> (2 * lS), grp = pc & ((1u << (2 * lS)) - 1u);
+    const int ox = ((int)(grp >> lS) << lK) + (int)(col >> lK);
+    const int oy = ((int)(grp & (uint32_t)(k.S - 1)) << lK) + (int)(col & (uint32_t)(k.K - 1));
+    const int lc = k.lc, c = 1 << lc;
+    r.lsbs = lc + k.lstride;
+    const int sbm = (1 << r.lsbs) - 1;
+    r.fx = x0 - c + ((ox * c - x0 + c) & sbm);
+    r.fy = y0 - c + ((oy * c - y0 + c) & sbm);
+    r.ncx = max(0, ((x0 + TE - r.fx) >> r.lsbs) + 1);
+    r.ncy = max(0, ((y0 + TE - r.fy) >> r.lsbs) + 1);
+    r.rcx = r.ncx ? ((1u << 20) + (uint32_t)r.ncx - 1u) / (uint32_t)r.ncx : 0u;
+    return r;
+}
+
+// Candidate ci of a phase: its sub-block and the Philox words of its
+// attempt (attempt_words).
+__device__ __forceinline__ void res_candidate(const SweepConsts& k, const ResSched& r, int ci, int Lm, int& bx,
+                                              int& by) {
+    const int iy = (int)(((uint32_t)ci * r.rcx) >> 20), ix = ci - iy * r.ncx;
+    bx = ((r.fx + (ix << r.lsbs)) & Lm) >> k.lc;
+    by = ((r.fy + (iy << r.lsbs)) & Lm) >> k.lc;
+}
+
+__device__ __forceinline__ uint4 res_candidate_draws(const SweepConsts& k, const ResSched& r, int ci, int Lm) {
+    int bx, by;
+    res_candidate(k, r, ci, Lm, bx, by);
+    uint32_t w[4];
+    attempt_words(k, sub_block_id<2>(k, bx, by, 0), bx, r.mcs, r.phase, 0u, w);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 template <class ST, int LTE, bool LEAN, bool XT, class PV>
 __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ResShared<ST, LTE>& sh,
                                              uint8_t* map) {
@@ -2223,7 +2276,12 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     // this thread's pair: same-colour row j, x-neighbouring pair jc
     const int j = tid / (TE / 4), jc = tid % (TE / 4);
     const unsigned int Gp = (unsigned int)(a.nmcs * k.rounds * k.phases_per_round);
-    if (!a.gen) sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, tid);
+    if (!a.gen) {
+        sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, tid);
+    } else {
+        const ResSched r = res_sched<LTE>(k, a.mcs0, 0u, x0, y0);
+        if (tid < r.ncx * r.ncy) sh.draws[tid] = res_candidate_draws(k, r, tid, Lm);
+    }
     for (unsigned int p = 0;; p++) {
         const bool work = p < Gp;  // p == Gp: receive the last phase's deltas only
         // The neighbours' packets of phase p - 1 (halo sites and deltas):
@@ -2300,34 +2358,27 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             }
             enqueue(e, act);
         } else if (work) {
-            // Any schedule with u = 1 (kernels2d.cu:72-106,197-243): phase p
-            // is phase pc of round r of an MCS, and the tile makes every
-            // attempt of it whose target lies in the tile (with non-shared
-            // roles the target may be a sub-block's neighbour site; the
-            // attempt then belongs to the target's tile, which holds all the
-            // sites the attempt reads).  Candidate sub-blocks: the active ones
-            // within c sites of the tile.
-            const uint32_t per_mcs = (uint32_t)(k.rounds * k.phases_per_round);
-            const uint32_t mcs = a.mcs0 + p / per_mcs;
-            const uint32_t phase = p % per_mcs;  // round * phases_per_round + pc
-            int ox, oy, oz;
-            phase_offsets_dev<2>(k, (int)(phase % (uint32_t)k.phases_per_round), ox, oy, oz);
-            const int lc = k.lc, c = 1 << lc, cm = c - 1, sbs = c << k.lstride;
-            const int fx = x0 - c + (((ox * c - (x0 - c)) % sbs) + sbs) % sbs;
-            const int fy = y0 - c + (((oy * c - (y0 - c)) % sbs) + sbs) % sbs;
-            const int ncx = (x0 + TE - fx) / sbs + 1, ncy = (y0 + TE - fy) / sbs + 1;
-            for (int c0 = 0; c0 < ncx * ncy; c0 += NT) {
+            // Any schedule with u = 1 (kernels2d.cu:72-106,197-243): the tile
+            // makes every attempt of the phase whose target lies in the tile
+            // (with non-shared roles the target may be a sub-block's
+            // neighbour site; the attempt then belongs to the target's tile,
+            // which holds all the sites the attempt reads).  Candidates: the
+            // active sub-blocks within c sites of the tile; their Philox
+            // words were made in the previous phase.
+            const ResSched r = res_sched<LTE>(k, a.mcs0, p, x0, y0);
+            const int lc = k.lc, cm = (1 << lc) - 1;
+            const int ncand = r.ncx * r.ncy;
+            for (int c0 = 0; c0 < ncand; c0 += NT) {
                 const int ci = c0 + tid;
                 ResEntry e[2];
                 bool act[2] = {false, false};
-                if (ci < ncx * ncy) {
-                    const int bx = ((fx + (ci % ncx) * sbs) & Lm) >> lc;
-                    const int by = ((fy + (ci / ncx) * sbs) & Lm) >> lc;
-                    uint32_t w[4];
-                    attempt_words(k, sub_block_id<2>(k, bx, by, 0), bx, mcs, phase, 0u, w);
-                    int px = (bx << lc) + (int)(w[0] & (uint32_t)cm);
-                    int py = (by << lc) + (int)((w[0] >> lc) & (uint32_t)cm);
-                    int dir = (int)(((uint64_t)w[1] * 8ull) >> 32);
+                if (ci < ncand) {
+                    int bx, by;
+                    res_candidate(k, r, ci, Lm, bx, by);
+                    const uint4 w = ci < NT ? sh.draws[ci] : res_candidate_draws(k, r, ci, Lm);
+                    int px = (bx << lc) + (int)(w.x & (uint32_t)cm);
+                    int py = (by << lc) + (int)((w.x >> lc) & (uint32_t)cm);
+                    int dir = (int)(((uint64_t)w.y * 8ull) >> 32);
                     dir += dir >= 4;
                     if (k.roles) {  // the drawn site is the source, its neighbour the target
                         int ddx, ddy, ddz;
@@ -2337,7 +2388,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                         dir = 8 - dir;
                     }
                     const int lx = ((px - x0) & Lm) + 1, ly = ((py - y0) & Lm) + 1;
-                    if (lx <= TE && ly <= TE) stage(lx, ly, dir, w[2], e[0], act[0]);
+                    if (lx <= TE && ly <= TE) stage(lx, ly, dir, w.z, e[0], act[0]);
                 }
                 enqueue(e, act);
             }
@@ -2411,9 +2462,15 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // queued attempt's rows, acceptance and commit
         // The upper half of the threads, idle here unless the queue is long,
         // makes the next phase's draws: two pairs each.
-        if (!a.gen && tid >= NT / 2 && p + 1 < Gp) {
-            sh.draws[tid - NT / 2] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid - NT / 2);
-            sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid);
+        if (tid >= NT / 2 && p + 1 < Gp) {
+            if (!a.gen) {
+                sh.draws[tid - NT / 2] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid - NT / 2);
+                sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid);
+            } else {
+                const ResSched r = res_sched<LTE>(k, a.mcs0, p + 1, x0, y0);
+                const int n = min(r.ncx * r.ncy, NT);
+                for (int ci = tid - NT / 2; ci < n; ci += NT / 2) sh.draws[ci] = res_candidate_draws(k, r, ci, Lm);
+            }
         }
         const TileTables T{map, sh.R, sh.D[p & 1u], &sh.miss};
         bool miss = false;
