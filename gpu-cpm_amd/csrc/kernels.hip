@@ -2284,16 +2284,11 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     }
     for (unsigned int p = 0;; p++) {
         const bool work = p < Gp;  // p == Gp: receive the last phase's deltas only
-        // The neighbours' packets of phase p - 1 (halo sites and deltas):
-        // their loads are issued first and checked after this phase's own
-        // interior work.
+        // (the neighbours' packets of phase p - 1, halo sites and deltas, are
+        // read from this parity)
         const uint32_t par_in = ((p - 1) & 1u) * 8u * mstride;
         unsigned long long v[WPT];
         uint32_t got = p > 0 ? 0u : need;
-#pragma unroll
-        for (int jj = 0; jj < WPT; jj++)
-            if (p > 0 && (need >> jj & 1u))
-                v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned int* const nq = &sh.nq[p & 1u];
         // Queue one or two attempts per lane: one LDS atomic per wave.
         auto enqueue = [&](const ResEntry* e, const bool* act) {
@@ -2395,6 +2390,14 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         }
         __syncthreads();
         const int nqp = (int)*nq;
+        // The neighbours' packets of phase p - 1 were sent about when this
+        // tile sent its own: loads issued now, while the queue is prepared,
+        // usually find them (issued at the phase start they come back stale
+        // and cost a second round trip).
+#pragma unroll
+        for (int jj = 0; jj < WPT; jj++)
+            if (p > 0 && (need >> jj & 1u))
+                v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (work) {
             if (tid == 0) sh.nq[(p + 1) & 1u] = 0;  // (last read in phase p - 1)
             // the previous phase's deltas are sent and in the rows
