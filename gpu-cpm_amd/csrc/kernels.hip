@@ -1791,9 +1791,15 @@ struct ResGeom {
     // odd, so that same-colour rows (two apart) of 16-bit sites start on
     // banks of opposite parity
     static constexpr int S = TE + 3;
-    static constexpr int NT = TE * TE / 8;  // threads: two same-colour sites (one Philox block) each
     static constexpr int NQ = TE * TE / 4;  // attempts per tile and phase
 };
+
+// Threads per tile of kernel variant VAR: one attempt each on the lean path
+// (at most 128 VGPRs: 16 waves per CU), two on the general one.
+template <int LTE, int VAR>
+constexpr int res_threads() {
+    return (1 << (2 * LTE)) / (VAR == 0 ? 4 : 8);
+}
 
 // Neighbour direction d = 0..7: the 3 x 3 block of tiles without its centre,
 // dx fastest; the opposite direction is 7 - d.
@@ -1879,7 +1885,7 @@ struct ResQueue {
     __device__ __forceinline__ ResEntry get(int i) const { return {li[i], vs[i], vt[i], w2[i], meta[i], 0u, e[i]}; }
 };
 
-template <class ST, int LTE>
+template <class ST, int LTE, int NT>
 struct ResShared {
     using G = ResGeom<LTE>;
     ST tile[G::S * G::S];
@@ -1893,9 +1899,9 @@ struct ResShared {
     uint32_t thr[kThrLds];
     int32_t prm[kParamWords];    // more than four types: the parameters, staged
     ResParams4 p4;               // the lean path's parameters
-    int scan[G::NT];
+    int scan[NT];
     ResQueue<G::NQ> queue;       // this phase's attempts whose source and target differ, and the ring's
-    uint4 draws[G::NT];          // the Philox block of each pair of same-colour sites, for the next phase
+    uint4 draws[NT];             // the next phase's Philox blocks: of each pair of same-colour sites, or candidate
     unsigned int nq[2];          // queue length of the phases of each parity
     unsigned int miss, quit;
 };
@@ -1909,13 +1915,13 @@ __device__ __forceinline__ void res_border(int dx, int dy, int q, int& lx, int& 
 }
 
 // Mailbox words a thread reads and writes each phase: at most WPT of them.
-template <class ST, int LTE>
+template <class ST, int LTE, int NT>
 struct ResWords {
     static constexpr int TE = ResGeom<LTE>::TE;
     static constexpr int SPW = 4 / (int)sizeof(ST);  // sites per word
     static constexpr int EDGE = TE / SPW;             // site words of an edge packet
     static constexpr int MAXW = 4 * (EDGE + kResDirCap) + 4 * (1 + kResDirCap);
-    static constexpr int WPT = (MAXW + ResGeom<LTE>::NT - 1) / ResGeom<LTE>::NT;
+    static constexpr int WPT = (MAXW + NT - 1) / NT;
     // A word's LDS side: sites (index of the first, stride to the next,
     // count) or a delta (slot).
     static __device__ __forceinline__ uint32_t sites(int li, int stride, int n) {
@@ -2135,12 +2141,14 @@ __device__ __forceinline__ uint4 res_candidate_draws(const SweepConsts& k, const
     return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-template <class ST, int LTE, bool LEAN, bool XT, class PV>
-__device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ResShared<ST, LTE>& sh,
+template <class ST, int LTE, int NT, bool LEAN, bool XT, class PV>
+__device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ResShared<ST, LTE, NT>& sh,
                                              uint8_t* map) {
     using G = ResGeom<LTE>;
-    using WD = ResWords<ST, LTE>;
-    constexpr int TE = G::TE, S = G::S, NT = G::NT, SPW = WD::SPW, WPT = WD::WPT;
+    using WD = ResWords<ST, LTE, NT>;
+    constexpr int TE = G::TE, S = G::S, SPW = WD::SPW, WPT = WD::WPT;
+    constexpr int APT = G::NQ / NT;          // attempts per thread and phase (c = 1, shared roles)
+    constexpr int NPAIR = G::NQ / 2;         // pairs of same-colour sites sharing a Philox block
     const SweepConsts& k = a.k;
     const int tid = (int)threadIdx.x;
     const int b = (int)blockIdx.x;
@@ -2274,10 +2282,14 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     if (tid == 0) sh.nq[0] = sh.nq[1] = 0;
     __syncthreads();
     // this thread's pair: same-colour row j, x-neighbouring pair jc
-    const int j = tid / (TE / 4), jc = tid % (TE / 4);
+    // this thread's pair of same-colour sites (c = 1, shared roles): row j of
+    // the colour, x-neighbouring pair jc; with one attempt per thread, the
+    // pair's site h
+    const int pair = APT == 2 ? tid : tid >> 1, hs = APT == 2 ? 0 : tid & 1;
+    const int j = pair / (TE / 4), jc = pair % (TE / 4);
     const unsigned int Gp = (unsigned int)(a.nmcs * k.rounds * k.phases_per_round);
     if (!a.gen) {
-        sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, tid);
+        for (int q = tid; q < NPAIR; q += NT) sh.draws[q] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, q);
     } else {
         const ResSched r = res_sched<LTE>(k, a.mcs0, 0u, x0, y0);
         if (tid < r.ncx * r.ncy) sh.draws[tid] = res_candidate_draws(k, r, tid, Lm);
@@ -2340,16 +2352,17 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             const int col = (int)(p & 3u);
             int cx, cy, cz;
             colour_of<2>(col, cx, cy, cz);
-            const uint4 dw = sh.draws[tid];
+            const uint4 dw = sh.draws[pair];
             const uint32_t wp[4] = {dw.x, dw.y, dw.z, dw.w};
             const int ly = 2 * j + cy + 1;
             ResEntry e[2];
-            bool act[2];
+            bool act[2] = {false, false};
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int i = 0; i < APT; i++) {
+                const int h = APT == 2 ? i : hs;
                 int dir = (int)(((uint64_t)(h ? wp[3] : wp[1]) * 8ull) >> 32);
                 dir += dir >= 4;
-                stage(4 * jc + cx + 2 * h + 1, ly, dir, h ? wp[0] : wp[2], e[h], act[h]);
+                stage(4 * jc + cx + 2 * h + 1, ly, dir, h ? wp[0] : wp[2], e[i], act[i]);
             }
             enqueue(e, act);
         } else if (work) {
@@ -2464,11 +2477,12 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // the ring's attempts (their blocks reach the halo), then every
         // queued attempt's rows, acceptance and commit
         // The upper half of the threads, idle here unless the queue is long,
-        // makes the next phase's draws: two pairs each.
+        // makes the next phase's draws: two pairs of sites each (one with a
+        // thread per attempt), or the candidates.
         if (tid >= NT / 2 && p + 1 < Gp) {
             if (!a.gen) {
-                sh.draws[tid - NT / 2] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid - NT / 2);
-                sh.draws[tid] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, tid);
+                for (int q = tid - NT / 2; q < NPAIR; q += NT / 2)
+                    sh.draws[q] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, q);
             } else {
                 const ResSched r = res_sched<LTE>(k, a.mcs0, p + 1, x0, y0);
                 const int n = min(r.ncx * r.ncy, NT);
@@ -2545,19 +2559,20 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
 // VAR 0: the lean attempt path; 1: the general one; 2: the general one
 // with the chemotaxis term.
 template <class ST, int LTE, int VAR>
-__global__ __launch_bounds__(ResGeom<LTE>::NT) void k_resident2d(ResidentArgs a) {
+__global__ __launch_bounds__((res_threads<LTE, VAR>())) void k_resident2d(ResidentArgs a) {
+    constexpr int NT = res_threads<LTE, VAR>();
     extern __shared__ __align__(16) unsigned char res_map[];
-    __shared__ ResShared<ST, LTE> sh;
+    __shared__ ResShared<ST, LTE, NT> sh;
     if (VAR == 0 || a.k.nt <= kSmallTypes) {
         SmallParams P;
         P.load(a.k.prm);
-        resident_run<ST, LTE, VAR == 0, VAR == 2>(a, P, sh, res_map);
+        resident_run<ST, LTE, NT, VAR == 0, VAR == 2>(a, P, sh, res_map);
     } else {
         const int npar = param_words(a.k.nt);
-        for (int i = (int)threadIdx.x; i < npar; i += ResGeom<LTE>::NT) sh.prm[i] = a.k.prm->w[i];
+        for (int i = (int)threadIdx.x; i < npar; i += NT) sh.prm[i] = a.k.prm->w[i];
         __syncthreads();
         const LdsParamView P{sh.prm, a.k.nt, a.k.nt * a.k.nt};
-        resident_run<ST, LTE, false, VAR == 2>(a, P, sh, res_map);
+        resident_run<ST, LTE, NT, false, VAR == 2>(a, P, sh, res_map);
     }
 }
 
@@ -2661,7 +2676,10 @@ static const void* resident_kernel(bool narrow, int lte, int var) {
 
 size_t resident_lds_bytes(int ncells) { return ((size_t)ncells + 1 + 15) & ~(size_t)15; }
 
-int resident_threads(int lte) { return lte == 6 ? ResGeom<6>::NT : ResGeom<5>::NT; }
+int resident_threads(int lte, int var) {
+    if (var == 0) return lte == 6 ? res_threads<6, 0>() : res_threads<5, 0>();
+    return lte == 6 ? res_threads<6, 1>() : res_threads<5, 1>();
+}
 
 hipError_t resident_max_blocks(bool narrow, int lte, int var, int ncells, int* max_blocks) {
     int dev = 0, cus = 0, per_cu = 0;
@@ -2669,7 +2687,7 @@ hipError_t resident_max_blocks(bool narrow, int lte, int var, int ncells, int* m
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
     if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, resident_kernel(narrow, lte, var),
-                                                          resident_threads(lte), resident_lds_bytes(ncells))) !=
+                                                          resident_threads(lte, var), resident_lds_bytes(ncells))) !=
         hipSuccess)
         return e;
     *max_blocks = per_cu * cus;
@@ -2680,7 +2698,7 @@ hipError_t launch_resident(bool narrow, int var, const ResidentArgs& a, int bloc
     ResidentArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchCooperativeKernel(resident_kernel(narrow, a.lte, var), dim3(blocks),
-                                      dim3(resident_threads(a.lte)), args,
+                                      dim3(resident_threads(a.lte, var)), args,
                                       (unsigned int)resident_lds_bytes(a.k.ncells), s);
 }
 
