@@ -326,6 +326,7 @@ struct cpm_engine {
     int64_t res_mcs = 0;            // resident form: MCS per launch after a guard failure (0: the chunk)
     int res_streak = 0;             // launches in a row that held since K last changed
     bool res_auto_off = false;      // the automatic choice no longer takes the form (until a push)
+    bool res_small_tiles = false;   // tiles of 32^2 where 64^2 held too many cells (until a push)
     int64_t res_fallbacks = 0;      // launches redone with one launch per phase: the guard
     int64_t res_wait_fallbacks = 0; // ... or an abandoned neighbour wait
     bool autotune = false;  // cpm_set_autotune
@@ -1089,6 +1090,7 @@ int cpm_push(cpm_handle h) {
     h->wide_valid = true;
     h->narrow_valid = false;
     h->res_auto_off = false;  // a new lattice: the resident form's launches start at full length
+    h->res_small_tiles = false;
     h->res_mcs = 0;
     h->res_streak = 0;
     HIP_TRY(hipMemsetAsync(h->d_tab[0], 0, rows * 8, s));
@@ -1379,10 +1381,15 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             return fail(CPM_EINVAL,
                         "launch form 4 (resident tiles) needs 2D, updates_per_checkerboard_switch = 1, L >= 128, "
                         "and no cell_sync, reference stream or act term");
-        // tiles of 32 or 64 sites, at least four per axis: up to 256 of
-        // them at L <= 1024, and wide enough that cells a few tens of sites
-        // across keep within two
-        res_lte = std::min(6, std::max(5, h->lgL - 4));
+        // tiles of 64 sites from 256^2 up (a phase takes the same time on
+        // 16 tiles as on 256: each tile's chain of draws, prepare, commit
+        // and exchange sets it, and 64^2 tiles run 1024 threads; measured
+        // 7-17% faster than 32^2 tiles at 256^2 to 1024^2), 32 at 128^2:
+        // at least four per axis.  set_launch(4, 32 or 64) picks the edge.
+        res_lte = h->lgL >= 8 && !h->res_small_tiles ? 6 : 5;
+        if (h->want_form == Plan::kResident && (h->want_tile == 32 || h->want_tile == 64) &&
+            h->lgL - (h->want_tile == 64 ? 6 : 5) >= 2)
+            res_lte = h->want_tile == 64 ? 6 : 5;
         res_var = k.chem ? 2
                          : (k.nt <= gcpm::kSmallTypes && !k.fixed_mask && !h->counters_on &&
                             k.nthr <= gcpm::kResThrLean)
@@ -1415,17 +1422,22 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             if (int rc = h->ensure_wide()) return rc;
             h->narrow_valid = false;
         }
-        const int tiles = plan.blocks;
+        int tiles = plan.blocks;
         const size_t rows = (size_t)h->n + 1;
         const size_t lat_bytes = h->sites * (bits ? 2 : 4);
-        const size_t nmail = (size_t)tiles * 16 * gcpm::res_mail_stride(1 << res_lte);
-        if (nmail > h->res_mail_cap) {
-            (void)hipFree(h->d_res_mail);
-            h->d_res_mail = nullptr;
-            h->res_mail_cap = 0;
-            HIP_TRY(hipMalloc(&h->d_res_mail, sizeof(unsigned long long) * nmail));
-            h->res_mail_cap = nmail;
-        }
+        size_t nmail = 0;
+        auto mail_for = [&]() -> int {
+            nmail = (size_t)tiles * 16 * gcpm::res_mail_stride(1 << res_lte);
+            if (nmail > h->res_mail_cap) {
+                (void)hipFree(h->d_res_mail);
+                h->d_res_mail = nullptr;
+                h->res_mail_cap = 0;
+                HIP_TRY(hipMalloc(&h->d_res_mail, sizeof(unsigned long long) * nmail));
+                h->res_mail_cap = nmail;
+            }
+            return CPM_OK;
+        };
+        if (int rc = mail_for()) return rc;
         if (!h->d_res_flags) HIP_TRY(hipMalloc(&h->d_res_flags, sizeof(unsigned int) * (2 + rows)));
         if (!h->d_bbox) HIP_TRY(hipMalloc(&h->d_bbox, sizeof(int32_t) * 4 * rows));
         if (!h->d_backup_lat) HIP_TRY(hipMalloc(&h->d_backup_lat, h->sites * 4));
@@ -1436,7 +1448,6 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         ra.lnt = h->lgL - res_lte;
         ra.gen = !(k.lc == 0 && !k.roles);
         ra.nmcs = a->inner_iterations;
-        ra.mail = h->d_res_mail;
         ra.holders = h->d_res_flags + 2;
         ra.err = err;
         ra.bad = bad;
@@ -1454,6 +1465,7 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             const int64_t n = std::min<int64_t>(K, (int64_t)((uint64_t)total - done));
             ra.mcs0 = (uint32_t)(base + done);
             ra.nmcs = (int)n;
+            ra.mail = h->d_res_mail;
             // the state a failed launch is redone from
             HIP_TRY(hipMemcpyAsync(h->d_backup_lat, cur_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
             HIP_TRY(hipMemcpyAsync(h->d_backup_tab, h->d_tab[0], sizeof(int64_t) * rows, hipMemcpyDeviceToDevice,
@@ -1487,15 +1499,40 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
                 HIP_TRY(hipStreamSynchronize(h->stream));
                 if (fl[1]) {
                     h->res_fallbacks++;
-                    // a cell too wide, too many cells for a tile, or misses
-                    // even in the shortest launches: the automatic choice
-                    // stops taking the form for this lattice
-                    if (h->want_form == -1 &&
-                        ((fl[1] & ~gcpm::kResGuardMiss) != 0 || n <= kResMinMcs))
-                        h->res_auto_off = true;
-                    K = std::max<int64_t>(kResMinMcs, n / 2);
-                    h->res_mcs = K;
-                    h->res_streak = 0;
+                    // Too many cells near a 64^2 tile for its slots, or
+                    // shared with one neighbour, and none wider than two
+                    // tiles: tiles of 32^2 meet a quarter as many cells and
+                    // half as much border, so the rest of the run (and the
+                    // lattice's later runs) take those, unless an edge was
+                    // asked for or they do not fit co-resident.
+                    int alt = 0;
+                    if (res_lte == 6 && h->want_tile == 0 && !(fl[1] & gcpm::kResGuardWide) &&
+                        (fl[1] & (gcpm::kResGuardSlots | gcpm::kResGuardShared)) != 0) {
+                        int maxb = 0;
+                        HIP_TRY(gcpm::resident_max_blocks(bits != 0, 5, res_var, h->n, &maxb));
+                        if ((1 << (2 * (h->lgL - 5))) <= maxb) alt = 5;
+                    }
+                    if (alt) {
+                        h->res_small_tiles = true;
+                        res_lte = alt;
+                        tiles = 1 << (2 * (h->lgL - res_lte));
+                        plan.blocks = tiles;
+                        plan.ltile = res_lte;
+                        ra.lte = res_lte;
+                        ra.lnt = h->lgL - res_lte;
+                        if (int rc = mail_for()) return rc;
+                    } else {
+                        // a cell too wide, too many cells for a tile, or
+                        // misses even in the shortest launches: the
+                        // automatic choice stops taking the form for this
+                        // lattice
+                        if (h->want_form == -1 &&
+                            ((fl[1] & ~gcpm::kResGuardMiss) != 0 || n <= kResMinMcs))
+                            h->res_auto_off = true;
+                        K = std::max<int64_t>(kResMinMcs, n / 2);
+                        h->res_mcs = K;
+                        h->res_streak = 0;
+                    }
                 } else {
                     h->res_wait_fallbacks++;
                 }

@@ -33,9 +33,9 @@ def _six_types(L, seed):
     return W.Workload("six_types", L, 2, n, 10, lat, cons)
 
 
-def _run(wl, seed, n, inner, narrow=1):
+def _run(wl, seed, n, inner, narrow=1, tile=0):
     from helpers import ENGINE_OPTIONS
-    ENGINE_OPTIONS.update(mode="resident", narrow=narrow)
+    ENGINE_OPTIONS.update(mode="resident", narrow=narrow, tile=tile)
     try:
         sim = make_cpm(wl)
     finally:
@@ -50,25 +50,26 @@ def _run(wl, seed, n, inner, narrow=1):
 
 def _cases():
     from gpucpm import workloads as W
-    return [  # name, workload, nmcs, inner, narrow
-        ("sorting1024", W.sorting2d(seed=401), 40, 20, 1),
-        ("sorting512", W.sorting2d(L=512, grid=40, edge=10, seed=402), 30, 10, 1),
-        ("plain256", W.plain2d(seed=403), 60, 30, 1),
+    return [  # name, workload, nmcs, inner, narrow, tile edge (0: the engine's, 64 from 256^2)
+        ("sorting1024", W.sorting2d(seed=401), 40, 20, 1, 0),
+        ("sorting512", W.sorting2d(L=512, grid=40, edge=10, seed=402), 30, 10, 1, 32),
+        ("plain256", W.plain2d(seed=403), 60, 30, 1, 0),
         # (cells that grow or drift by a few sites per ten MCS: launches of
         # 10 MCS stay within the tiles that hold them)
-        ("small-cells256", W.sorting_small_cells(L=256, seed=404), 40, 10, 1),
+        ("small-cells256", W.sorting_small_cells(L=256, seed=404), 40, 10, 1, 32),
         ("wall-chem256", W.chemotactic(W.walled(W.sorting2d(L=256, grid=20, edge=10, seed=405)),
-                                       lam=(0.0, 20.0, 5.0, 0.0)), 30, 15, 1),
-        ("wide512", W.sorting2d(L=512, grid=40, edge=10, seed=406), 20, 20, 0),
-        ("six-types256", _six_types(256, 407), 30, 10, 1),
+                                       lam=(0.0, 20.0, 5.0, 0.0)), 30, 15, 1, 32),
+        ("wide512", W.sorting2d(L=512, grid=40, edge=10, seed=406), 20, 20, 0, 0),
+        ("six-types256", _six_types(256, 407), 30, 10, 1, 32),
+        ("plain128", W.plain2d(L=128, ncells=60, seed=411), 30, 10, 1, 0),
     ]
 
 
-@pytest.mark.parametrize("idx", range(7))
+@pytest.mark.parametrize("idx", range(8))
 def test_resident_matches_oracle(idx):
-    name, wl, n, inner, narrow = _cases()[idx]
-    _announce("resident", name, wl.L, n, inner, narrow)
-    sim = _run(wl, 500 + idx, n, inner, narrow)
+    name, wl, n, inner, narrow, tile = _cases()[idx]
+    _announce("resident", name, wl.L, n, inner, narrow, tile)
+    sim = _run(wl, 500 + idx, n, inner, narrow, tile)
     launch = sim.last_launch()
     lat_o, tab_o = run_oracle(wl, 1, 1, 500 + idx, n)
     np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
@@ -80,33 +81,36 @@ def test_resident_matches_oracle(idx):
     if name != "wall-chem256":
         assert guard == 0 and launch["form"] == 4, (guard, launch)
         assert "k_resident2d" in launch["kernel"]
+        assert launch["tile"] == (tile or (64 if wl.L >= 256 else 32)), launch
 
 
 def _general_cases():
     from gpucpm import workloads as W
-    return [  # name, workload, c, shared, positions_per_thread, nmcs, inner, narrow
+    return [  # name, workload, c, shared, positions_per_thread, nmcs, inner, narrow, tile edge
         # the reference sorting.py schedule (non-shared roles, c 2, ppt 4)
-        ("sortingpy1024", W.sorting2d(seed=421), 2, 0, 4, 8, 8, 1),
-        ("sortingpy512", W.sorting2d(L=512, grid=40, edge=10, seed=422), 2, 0, 4, 20, 20, 1),
-        ("ns-c1-256", W.sorting2d(L=256, grid=20, edge=10, seed=423), 1, 0, 2, 20, 10, 1),
-        ("ns-c4-256", W.plain2d(seed=424), 4, 0, 8, 20, 20, 1),
-        ("shared-c2-256", W.sorting2d(L=256, grid=20, edge=10, seed=425), 2, 1, 4, 20, 20, 1),
+        ("sortingpy1024", W.sorting2d(seed=421), 2, 0, 4, 8, 8, 1, 0),
+        ("sortingpy512", W.sorting2d(L=512, grid=40, edge=10, seed=422), 2, 0, 4, 20, 20, 1, 32),
+        ("ns-c1-256", W.sorting2d(L=256, grid=20, edge=10, seed=423), 1, 0, 2, 20, 10, 1, 32),
+        ("ns-c4-256", W.plain2d(seed=424), 4, 0, 8, 20, 20, 1, 0),
+        ("shared-c2-256", W.sorting2d(L=256, grid=20, edge=10, seed=425), 2, 1, 4, 20, 20, 1, 32),
         ("chem-ns-c2-256", W.chemotactic(W.sorting2d(L=256, grid=20, edge=10, seed=426), lam=(0.0, 20.0, 5.0)),
-         2, 0, 4, 20, 20, 1),
-        ("six-types-ns-256", _six_types(256, 427), 2, 0, 4, 20, 10, 1),
-        ("wide-ns-512", W.sorting2d(L=512, grid=40, edge=10, seed=428), 2, 0, 4, 10, 10, 0),
+         2, 0, 4, 20, 20, 1, 0),
+        ("six-types-ns-256", _six_types(256, 427), 2, 0, 4, 20, 10, 1, 32),
+        ("wide-ns-512", W.sorting2d(L=512, grid=40, edge=10, seed=428), 2, 0, 4, 10, 10, 0, 0),
+        ("shared-c2-256-t64", W.sorting2d(L=256, grid=20, edge=10, seed=429), 2, 1, 4, 20, 20, 1, 64),
+        ("ns-c4-256-t32", W.plain2d(seed=430), 4, 0, 8, 20, 20, 1, 32),
     ]
 
 
-@pytest.mark.parametrize("idx", range(8))
+@pytest.mark.parametrize("idx", range(10))
 def test_resident_general_schedules_match_oracle(idx):
     """Any schedule with u = 1: each tile makes the attempts whose target it
     holds (with non-shared roles the target may be a sub-block's neighbour
     site in another tile), bit-exact against the oracle."""
     from helpers import ENGINE_OPTIONS
-    name, wl, c, shared, ppt, n, inner, narrow = _general_cases()[idx]
-    _announce("resident general", name, wl.L, c, shared, ppt, n, inner)
-    ENGINE_OPTIONS.update(mode="resident", narrow=narrow)
+    name, wl, c, shared, ppt, n, inner, narrow, tile = _general_cases()[idx]
+    _announce("resident general", name, wl.L, c, shared, ppt, n, inner, tile)
+    ENGINE_OPTIONS.update(mode="resident", narrow=narrow, tile=tile)
     try:
         sim = make_cpm(wl)
     finally:
@@ -123,6 +127,7 @@ def test_resident_general_schedules_match_oracle(idx):
     guard, waits = sim.resident_fallbacks
     assert waits == 0
     assert guard == 0 and launch["form"] == 4, (guard, launch)
+    assert launch["tile"] == (tile or 64), launch
 
 
 def test_guard_sends_a_spread_cell_back_to_per_phase_launches():
@@ -134,11 +139,29 @@ def test_guard_sends_a_spread_cell_back_to_per_phase_launches():
     lat[100:170, 60:130] = (1 << 24) | 1
     wl.lattice = lat
     _announce("guard", wl.L)
-    sim = _run(wl, 509, 20, 10)
+    sim = _run(wl, 509, 20, 10, tile=32)
     assert sim.resident_fallbacks[0] >= 1 and sim.resident_fallbacks[1] == 0
     lat_o, tab_o = run_oracle(wl, 1, 1, 509, 20)
     np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
     assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+
+
+def test_dense_small_cells_move_to_smaller_tiles():
+    """1849 cells of 16 sites at 256^2: a 64^2 tile with its margins meets
+    more cells than its slots hold, so the first launch is redone per phase
+    and the rest of the run (and the next run) takes 32^2 tiles, exactly."""
+    wl = _six_types(256, 412)
+    _announce("retile", wl.L)
+    sim = _run(wl, 513, 30, 10)
+    guard, waits = sim.resident_fallbacks
+    assert guard == 1 and waits == 0, (guard, waits)
+    assert sim.last_launch()["form"] == 4 and sim.last_launch()["tile"] == 32
+    lat_o, tab_o = run_oracle(wl, 1, 1, 513, 30)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+    sim.run(**run_kwargs(1, 1, 10, 10))
+    sim.synchronize()
+    assert sim.resident_fallbacks[0] == 1 and sim.last_launch()["tile"] == 32
 
 
 def test_launch_length_adapts_to_moving_cells():
@@ -149,7 +172,7 @@ def test_launch_length_adapts_to_moving_cells():
     from gpucpm import workloads as W
     wl = W.sorting_small_cells(L=256, seed=404)
     _announce("adaptive launches", wl.L)
-    sim = _run(wl, 503, 40, 40)
+    sim = _run(wl, 503, 40, 40, tile=32)
     guard, waits = sim.resident_fallbacks
     assert guard >= 1 and waits == 0
     lat_o, tab_o = run_oracle(wl, 1, 1, 503, 40)
@@ -159,6 +182,7 @@ def test_launch_length_adapts_to_moving_cells():
     sim.run(**run_kwargs(1, 1, 40, 40))
     sim.synchronize()
     sim.pull_from_gpu()
+    assert sim.last_launch()["tile"] == 32
     lat_o2, tab_o2 = run_oracle(wl, 1, 1, 503, 80)
     np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o2)
     assert_tables_equal(sim.get_cell_table(), tab_o2, 2)
