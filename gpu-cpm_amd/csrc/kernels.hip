@@ -2006,16 +2006,33 @@ __device__ __forceinline__ void res_commit(ST* tile, const SweepConsts& k, const
 // which shortens the dependent chain each phase waits on.
 __device__ __forceinline__ int32_t pick4(int i, const int4& v) { return pick4(i, v.x, v.y, v.z, v.w); }
 
+// The lean path's meta word: dir | on << 4 | as << 8 | at << 12 | the
+// source's slot << 16 | the target's slot << 24 (slots as the tile's map
+// gives them, fixed for a launch, so the prepare looks them up).
 template <class ST, int LTE>
-__device__ __forceinline__ void res_prepare_lean(const ST* tile, const ResParams4& pr, uint32_t idbits,
-                                                 ResEntry& q) {
+__device__ __forceinline__ void res_prepare_lean(const ST* tile, const uint8_t* map, const ResParams4& pr,
+                                                 uint32_t idbits, uint32_t idmask, ResEntry& q) {
     constexpr int S = ResGeom<LTE>::S;
     const int li = (int)(q.li & ~kResRing);
     const uint32_t vs = q.vs, vt = q.vt;
     const int ts = (int)(vs >> idbits), tt = (int)(vt >> idbits);
     const int4 jr = pr.J[ts], jsr = pr.JS[tt];
-    const uint32_t nv[8] = {tile[li - S - 1], tile[li - S], tile[li - S + 1], tile[li - 1],
-                            tile[li + 1],     tile[li + S - 1], tile[li + S], tile[li + S + 1]};
+    int32_t j0 = jr.x, j1 = jr.y, j2 = jr.z, j3 = jr.w, k0 = jsr.x, k1 = jsr.y, k2 = jsr.z, k3 = jsr.w;
+    // (the diagonal entries as loads of their own: a select among a row's
+    // four loaded words compiles to branches)
+    int32_t jss = reinterpret_cast<const int32_t*>(&pr.J[ts])[ts];
+    int32_t jtt = reinterpret_cast<const int32_t*>(&pr.JS[tt])[tt];
+    uint32_t ss = map[vs & idmask], st_ = map[vt & idmask];
+    uint32_t nv[8] = {tile[li - S - 1], tile[li - S], tile[li - S + 1], tile[li - 1],
+                      tile[li + 1],     tile[li + S - 1], tile[li + S], tile[li + S + 1]};
+    // One wait for every load above: the statement takes and gives back
+    // each value, so nothing computed from them starts before all have
+    // arrived (the scheduler otherwise interleaves arithmetic with the
+    // loads and waits three times).
+    asm volatile(""
+                 : "+v"(nv[0]), "+v"(nv[1]), "+v"(nv[2]), "+v"(nv[3]), "+v"(nv[4]), "+v"(nv[5]), "+v"(nv[6]),
+                   "+v"(nv[7]), "+v"(ss), "+v"(st_), "+v"(jss), "+v"(jtt), "+v"(j0), "+v"(j1), "+v"(j2), "+v"(j3),
+                   "+v"(k0), "+v"(k1), "+v"(k2), "+v"(k3));
     uint32_t hist = 0;
     int as = 0, at = 0;
 #pragma unroll
@@ -2025,11 +2042,11 @@ __device__ __forceinline__ void res_prepare_lean(const ST* tile, const ResParams
         at += nv[n] == vt;
     }
     // adhesion (kernels2d.cu:359-402,477) from the counts per type
-    int32_t adh = (jr.x - jsr.x) * (int32_t)(hist & 31u) + (jr.y - jsr.y) * (int32_t)((hist >> 5) & 31u) +
-                  (jr.z - jsr.z) * (int32_t)((hist >> 10) & 31u) + (jr.w - jsr.w) * (int32_t)((hist >> 15) & 31u);
-    adh += at * pick4(tt, jsr) - as * pick4(ts, jr);
+    int32_t adh = (j0 - k0) * (int32_t)(hist & 31u) + (j1 - k1) * (int32_t)((hist >> 5) & 31u) +
+                  (j2 - k2) * (int32_t)((hist >> 10) & 31u) + (j3 - k3) * (int32_t)((hist >> 15) & 31u);
+    adh += at * jtt - as * jss;
     q.e = adh;
-    q.meta = (q.meta & 0xFu) | (1u << 4) | ((uint32_t)as << 8) | ((uint32_t)at << 16);
+    q.meta = (q.meta & 0xFu) | (1u << 4) | ((uint32_t)as << 8) | ((uint32_t)at << 12) | (ss << 16) | (st_ << 24);
 }
 
 // Perimeter and area terms of one cell (kernels2d.cu:479-520): sign +1 for
@@ -2041,18 +2058,17 @@ __device__ __forceinline__ int64_t res_cell_terms(const int4& tp, int64_t row, i
 }
 
 template <class ST, int LTE>
-__device__ __forceinline__ void res_commit_lean(ST* tile, const uint8_t* map, const int64_t* R, int64_t* Dl,
-                                                const uint32_t* thr, int nthr, const ResParams4& pr,
-                                                uint32_t idbits, uint32_t idmask, const ResEntry& q,
-                                                bool& miss) {
+__device__ __forceinline__ void res_commit_lean(ST* tile, const int64_t* R, int64_t* Dl, const uint32_t* thr,
+                                                int nthr, const ResParams4& pr, uint32_t idbits, uint32_t idmask,
+                                                const ResEntry& q, bool& miss) {
     const uint32_t meta = q.meta;
     if (!((meta >> 4) & 1u)) return;
     constexpr int N = 8;
-    const int as = (int)((meta >> 8) & 0xFFu), at = (int)(meta >> 16);
+    const int as = (int)((meta >> 8) & 0xFu), at = (int)((meta >> 12) & 0xFu);
     const uint32_t vs = q.vs, vt = q.vt;
     const int ts = (int)(vs >> idbits), tt = (int)(vt >> idbits);
     const uint32_t cs = vs & idmask, ct = vt & idmask;
-    const uint32_t ss = map[cs], st_ = map[ct];  // (the medium has slot kResMed, row 0)
+    const uint32_t ss = (meta >> 16) & 0xFFu, st_ = meta >> 24;  // (the medium has slot kResMed, row 0)
     const int4 ps = pr.T[ts], pt = pr.T[tt];
     const int64_t rs = R[ss], rt = R[st_];
     const int64_t dPs = N - 2 * as, dPt = 2 * at - N;
@@ -2417,10 +2433,12 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
             // prepare the interior attempts while the packets travel
             for (int qi = tid; qi < nqp; qi += NT) {
-                if (sh.queue.li[qi] & kResRing) continue;
+                // (every field in one round of loads, the ring test after)
                 ResEntry qe = sh.queue.get(qi);
+                asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.meta));
+                if (qe.li & kResRing) continue;
                 if constexpr (LEAN)
-                    res_prepare_lean<ST, LTE>(sh.tile, sh.p4, k.idbits, qe);
+                    res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
                 else
                     res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
                 sh.queue.vs[qi] = qe.vs;
@@ -2493,6 +2511,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         bool miss = false;
         for (int qi = tid; qi < nqp; qi += NT) {
             ResEntry qe = sh.queue.get(qi);
+            if constexpr (LEAN) asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.w2), "v"(qe.meta), "v"(qe.e));
             if (qe.li & kResRing) {
                 const uint32_t li = qe.li & ~kResRing;
                 int ox, oy, oz;
@@ -2502,12 +2521,12 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                 qe.li = li;
                 if (qe.vs == qe.vt) continue;  // kernels2d.cu:276
                 if constexpr (LEAN)
-                    res_prepare_lean<ST, LTE>(sh.tile, sh.p4, k.idbits, qe);
+                    res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
                 else
                     res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
             }
             if constexpr (LEAN)
-                res_commit_lean<ST, LTE>(sh.tile, map, sh.R, sh.D[p & 1u], sh.thr, k.nthr, sh.p4, k.idbits,
+                res_commit_lean<ST, LTE>(sh.tile, sh.R, sh.D[p & 1u], sh.thr, k.nthr, sh.p4, k.idbits,
                                          k.idmask, qe, miss);
             else
                 res_commit<ST, LTE>(sh.tile, k, P, T, sh.thr, thr_lds, qe);
@@ -2558,8 +2577,11 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
 
 // VAR 0: the lean attempt path; 1: the general one; 2: the general one
 // with the chemotaxis term.
+// (one workgroup per CU: the compiler may schedule for four waves per SIMD
+// rather than the most it could fit)
 template <class ST, int LTE, int VAR>
-__global__ __launch_bounds__((res_threads<LTE, VAR>())) void k_resident2d(ResidentArgs a) {
+__global__ __launch_bounds__((res_threads<LTE, VAR>())) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_resident2d(
+    ResidentArgs a) {
     constexpr int NT = res_threads<LTE, VAR>();
     extern __shared__ __align__(16) unsigned char res_map[];
     __shared__ ResShared<ST, LTE, NT> sh;
