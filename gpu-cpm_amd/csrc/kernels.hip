@@ -1902,8 +1902,11 @@ struct ResShared {
     int scan[NT];
     ResQueue<G::NQ> queue;       // this phase's attempts whose source and target differ, and the ring's
     uint4 draws[NT];             // the next phase's Philox blocks: of each pair of same-colour sites, or candidate
-    unsigned int nq[2];          // queue length of the phases of each parity
-    unsigned int miss, quit;
+    // Per phase parity: the queue length, and what stops the launch (1: an
+    // attempt met a cell the tile does not hold; 2: a wait gave up), raised
+    // in the phase before and read with the length after the first barrier.
+    uint2 ctl[2];
+    unsigned int quit;           // the set-up's guard
 };
 
 // Border site q (along the edge) facing (dx, dy), LDS coordinates 1 .. TE.
@@ -2197,10 +2200,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     int cnt = 0;
     for (int c = c0; c < c1; c++) cnt += res_holds(a.holders[c], tx, ty, ntm);
     sh.scan[tid] = cnt;
-    if (tid == 0) {
-        sh.miss = 0;
-        sh.quit = 0;
-    }
+    if (tid == 0) sh.quit = 0;
     __syncthreads();
     for (int off = 1; off < NT; off <<= 1) {
         const int v = tid >= off ? sh.scan[tid - off] : 0;
@@ -2295,7 +2295,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         }
     }
     const ResLat<ST, LTE> lat{sh.tile, x0, y0, k.lgL};
-    if (tid == 0) sh.nq[0] = sh.nq[1] = 0;
+    if (tid == 0) sh.ctl[0] = sh.ctl[1] = make_uint2(0u, 0u);
     __syncthreads();
     // this thread's pair: same-colour row j, x-neighbouring pair jc
     // this thread's pair of same-colour sites (c = 1, shared roles): row j of
@@ -2317,7 +2317,8 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         const uint32_t par_in = ((p - 1) & 1u) * 8u * mstride;
         unsigned long long v[WPT];
         uint32_t got = p > 0 ? 0u : need;
-        unsigned int* const nq = &sh.nq[p & 1u];
+        unsigned int* const nq = &sh.ctl[p & 1u].x;
+        unsigned int* const stop = &sh.ctl[(p + 1) & 1u].y;  // (raised in this phase)
         // Queue one or two attempts per lane: one LDS atomic per wave.
         auto enqueue = [&](const ResEntry* e, const bool* act) {
             const uint64_t m0 = __ballot(act[0]), m1 = __ballot(act[1]);
@@ -2418,7 +2419,16 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             }
         }
         __syncthreads();
-        const int nqp = (int)*nq;
+        // this phase's queue length, and whether the previous phase stopped
+        // the launch (its result is then discarded by the host, so the
+        // commit and publish that followed the stop did no harm: every
+        // index they used stays in bounds)
+        const uint2 cq = sh.ctl[p & 1u];
+        if (cq.y) {
+            if (tid == 0 && (cq.y & 1u)) atomicOr(a.bad, kResGuardMiss);
+            return;
+        }
+        const int nqp = (int)cq.x;
         // The neighbours' packets of phase p - 1 were sent about when this
         // tile sent its own: loads issued now, while the queue is prepared,
         // usually find them (issued at the phase start they come back stale
@@ -2428,7 +2438,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             if (p > 0 && (need >> jj & 1u))
                 v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (work) {
-            if (tid == 0) sh.nq[(p + 1) & 1u] = 0;  // (last read in phase p - 1)
+            if (tid == 0) sh.ctl[(p + 1) & 1u].x = 0;  // (last read in phase p - 1)
             // the previous phase's deltas are sent and in the rows
             for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
             // prepare the interior attempts while the packets travel
@@ -2469,7 +2479,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                                                   __HIP_MEMORY_SCOPE_AGENT);
             }
             if (got != need) {
-                sh.quit = 1u;
+                *stop = 2u;
             } else {
 #pragma unroll
                 for (int jj = 0; jj < WPT; jj++) {
@@ -2490,7 +2500,6 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             }
         }
         __syncthreads();
-        if (sh.quit) return;
         if (!work) break;
         // the ring's attempts (their blocks reach the halo), then every
         // queued attempt's rows, acceptance and commit
@@ -2507,7 +2516,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                 for (int ci = tid - NT / 2; ci < n; ci += NT / 2) sh.draws[ci] = res_candidate_draws(k, r, ci, Lm);
             }
         }
-        const TileTables T{map, sh.R, sh.D[p & 1u], &sh.miss};
+        const TileTables T{map, sh.R, sh.D[p & 1u], stop};
         bool miss = false;
         for (int qi = tid; qi < nqp; qi += NT) {
             ResEntry qe = sh.queue.get(qi);
@@ -2531,12 +2540,8 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             else
                 res_commit<ST, LTE>(sh.tile, k, P, T, sh.thr, thr_lds, qe);
         }
-        if (miss) sh.miss = 1u;
+        if (miss) *stop = 1u;
         __syncthreads();
-        if (sh.miss) {  // a cell this tile does not hold: the guard
-            if (tid == 0) atomicOr(a.bad, kResGuardMiss);
-            return;
-        }
         // this tile's packets of phase p, tag p + 1
         const uint32_t par_out = (p & 1u) * 8u * mstride;
 #pragma unroll
@@ -2564,6 +2569,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             if (dv != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&sh.R[s]), (unsigned long long)dv);
         }
     }
+    if (sh.ctl[(Gp + 1) & 1u].y) return;  // the last wait gave up
     // the tile's sites and the rows of its cells (every holder has the same)
     for (int i = tid; i < TE * TE; i += NT) {
         const int ly = i >> LTE, lx = i & (TE - 1);
