@@ -983,6 +983,16 @@ struct CompactPhase {
     int gtid, gsz;  // thread index and count in the group (fold)
 };
 
+// n queue slots for the calling wave (n the same on every active lane): the
+// first active lane reserves them with one LDS atomic, and the base comes
+// back to every lane through a scalar register (a cross-lane shuffle would
+// cost a further LDS round trip).
+__device__ __forceinline__ uint32_t wave_reserve(unsigned int* ctr, uint32_t n) {
+    uint32_t base = 0;
+    if (__lane_id() == (unsigned int)__builtin_amdgcn_readfirstlane((int)__lane_id())) base = atomicAdd(ctr, n);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+}
+
 // Thread t of block b draws the attempts of same-colour sub-blocks
 // (b ATT + r) blockDim + t for r < ATT (consecutive threads, consecutive
 // sub-blocks), so the block's queue collects up to ATT blockDim attempts.
@@ -1110,9 +1120,7 @@ __device__ __forceinline__ void compact_phase(const SweepConsts& k, const Compac
         total += (uint32_t)__popcll(masks[r]);
     }
     if (total != 0) {
-        uint32_t base = 0;
-        if (__lane_id() == 0) base = atomicAdd(&nqueued, total);
-        base = __shfl(base, 0);
+        uint32_t base = wave_reserve(&nqueued, total);
 #pragma unroll
         for (int r = 0; r < ATT; r++) {
             const uint32_t before = __builtin_amdgcn_mbcnt_hi(
@@ -1623,9 +1631,7 @@ __device__ __forceinline__ void small_run(const SmallArgs& a, const PV& P, ST* l
                     __syncthreads();
                     const uint64_t mask = __ballot(active);
                     if (mask) {
-                        uint32_t base = 0;
-                        if (__lane_id() == 0) base = atomicAdd(nqueued, (uint32_t)__popcll(mask));
-                        base = __shfl(base, 0);
+                        const uint32_t base = wave_reserve(nqueued, (uint32_t)__popcll(mask));
                         const uint32_t before = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
                         if (active) queue[base + before] = e;
@@ -2324,9 +2330,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             const uint64_t m0 = __ballot(act[0]), m1 = __ballot(act[1]);
             const uint32_t total = (uint32_t)(__popcll(m0) + __popcll(m1));
             if (total != 0) {
-                uint32_t base = 0;
-                if (__lane_id() == 0) base = atomicAdd(nq, total);
-                base = __shfl(base, 0);
+                const uint32_t base = wave_reserve(nq, total);
                 const uint32_t b0 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32),
                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u));
                 const uint32_t b1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
