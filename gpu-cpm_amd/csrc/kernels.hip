@@ -2610,41 +2610,30 @@ __global__ __launch_bounds__((res_threads<LTE, VAR>())) __attribute__((amdgpu_wa
 
 // Per-cell boxes of the lattice's sites (relative to the anchors, wrapped):
 // bbox must hold (INT_MAX, INT_MIN, INT_MAX, INT_MIN) per cell on entry.
+// A wave reads 64 sites of one row (L >= 64); each run of equal sites in
+// it adds its first and last x and its row to the cell's box, from the
+// run's first lane.  (A run whose relative x wraps belongs to a cell at
+// least half the lattice wide: its box comes out wide either way.)
 template <class ST>
 __global__ __launch_bounds__(256) void k_bbox2d(const ST* __restrict__ lat, int lgL, uint32_t idmask,
                                                 const int32_t* __restrict__ anchor, int32_t* bbox) {
-    const uint32_t nsites = 1u << (2 * lgL);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = i < nsites ? (uint32_t)lat[i] & idmask : 0u;
-    const int L = 1 << lgL, Lm = L - 1;
-    int rx = 0, ry = 0;
-    if (c) {
-        rx = (((int)(i & Lm) - anchor[3 * c] + L / 2) & Lm) - L / 2;
-        ry = (((int)(i >> lgL) - anchor[3 * c + 1] + L / 2) & Lm) - L / 2;
-    }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // (the grid covers the lattice exactly)
+    const uint32_t c = (uint32_t)lat[i] & idmask;
     const int lane = (int)__lane_id();
-    uint64_t pending = __ballot(c != 0u);
-    while (pending) {  // lanes grouped by cell, one atomic per group and bound
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lc = (uint32_t)__shfl((int)c, leader);
-        const bool mine = ((pending >> lane) & 1ull) && c == lc;
-        int mnx = mine ? rx : INT_MAX, mxx = mine ? rx : INT_MIN;
-        int mny = mine ? ry : INT_MAX, mxy = mine ? ry : INT_MIN;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            mnx = min(mnx, __shfl_xor(mnx, o));
-            mxx = max(mxx, __shfl_xor(mxx, o));
-            mny = min(mny, __shfl_xor(mny, o));
-            mxy = max(mxy, __shfl_xor(mxy, o));
-        }
-        if (lane == leader) {
-            atomicMin(bbox + 4 * lc, mnx);
-            atomicMax(bbox + 4 * lc + 1, mxx);
-            atomicMin(bbox + 4 * lc + 2, mny);
-            atomicMax(bbox + 4 * lc + 3, mxy);
-        }
-        pending &= ~__ballot(mine);
-    }
+    const uint32_t cl = (uint32_t)__shfl_up((int)c, 1), cr = (uint32_t)__shfl_down((int)c, 1);
+    const bool first = lane == 0 || cl != c, last = lane == 63 || cr != c;
+    const uint64_t lasts = __ballot(last);
+    if (!first || c == 0u) return;
+    const int end = lane + __ffsll((unsigned long long)(lasts >> lane)) - 1;  // this run's last lane
+    const int L = 1 << lgL, Lm = L - 1;
+    const int x0 = (int)(i & Lm), y = (int)(i >> lgL);
+    const int ax = anchor[3 * c], ay = anchor[3 * c + 1];
+    const int r0 = ((x0 - ax + L / 2) & Lm) - L / 2, r1 = ((x0 + end - lane - ax + L / 2) & Lm) - L / 2;
+    const int ry = ((y - ay + L / 2) & Lm) - L / 2;
+    atomicMin(bbox + 4 * c, min(r0, r1));
+    atomicMax(bbox + 4 * c + 1, max(r0, r1));
+    atomicMin(bbox + 4 * c + 2, ry);
+    atomicMax(bbox + 4 * c + 3, ry);
 }
 
 __global__ void k_bbox_init(int32_t* bbox, int rows) {
@@ -2738,6 +2727,7 @@ hipError_t launch_resident_guard(bool narrow, const SweepConsts& k, const int32_
                                  int lte, int lnt, uint32_t* holders, unsigned int* bad, hipStream_t s) {
     const int rows = k.ncells + 1;
     hipLaunchKernelGGL(k_bbox_init, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows);
+    if (k.lgL < 7) return hipErrorInvalidValue;  // (the resident form's lattices: 128^2 and up)
     const uint32_t nsites = 1u << (2 * k.lgL);
     if (narrow)
         hipLaunchKernelGGL(k_bbox2d<uint16_t>, dim3((nsites + 255) / 256), dim3(256), 0, s, k.lat16, k.lgL,
