@@ -270,6 +270,22 @@ hipError_t launch_resident(bool narrow, int var, const ResidentArgs& a, int bloc
 // Before a chunk: the boxes of the current lattice's cells into bbox, and
 // the holder word of each cell; *bad |= 1 if a cell's widened box spans
 // more than two tiles per axis.
+// The copies and fills before a resident launch (one kernel).
+struct ResPrep {
+    const void* lat;  // the current lattice (16- or 32-bit), lat_bytes a multiple of 16
+    void* lat_backup;
+    size_t lat_bytes;
+    unsigned long long* mail;  // nmail words zeroed (nmail even)
+    size_t nmail;
+    const int64_t* tab;  // rows copied to tab_backup; bbox rows set empty
+    int64_t* tab_backup;
+    int32_t* bbox;
+    int rows;
+    unsigned int* flags;             // the two error words, zeroed
+    unsigned long long* counters;    // if set: counters[4..7] = counters[0..3]
+};
+hipError_t launch_res_prep(const ResPrep& q, hipStream_t s);
+// Boxes (set empty beforehand), holder words and the box guard of a resident launch.
 hipError_t launch_resident_guard(bool narrow, const SweepConsts& k, const int32_t* anchor, int32_t* bbox,
                                  int lte, int lnt, uint32_t* holders, unsigned int* bad, hipStream_t s);
 // *p = v / *p += v (one thread: nodes of a captured graph, no memset nodes)

@@ -1466,15 +1466,21 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
             ra.mcs0 = (uint32_t)(base + done);
             ra.nmcs = (int)n;
             ra.mail = h->d_res_mail;
-            // the state a failed launch is redone from
-            HIP_TRY(hipMemcpyAsync(h->d_backup_lat, cur_lat, lat_bytes, hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(hipMemcpyAsync(h->d_backup_tab, h->d_tab[0], sizeof(int64_t) * rows, hipMemcpyDeviceToDevice,
-                                   h->stream));
-            if (h->counters_on)
-                HIP_TRY(hipMemcpyAsync(h->d_counters + 4, h->d_counters, 4 * sizeof(unsigned long long),
-                                       hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(gcpm::launch_fill_zero(h->d_res_mail, nmail, h->stream));
-            HIP_TRY(gcpm::launch_fill_zero(reinterpret_cast<unsigned long long*>(h->d_res_flags), 1, h->stream));
+            // the state a failed launch is redone from, cleared mailboxes,
+            // error words and boxes (one kernel)
+            gcpm::ResPrep rp{};
+            rp.lat = cur_lat;
+            rp.lat_backup = h->d_backup_lat;
+            rp.lat_bytes = lat_bytes;
+            rp.mail = h->d_res_mail;
+            rp.nmail = nmail;
+            rp.tab = h->d_tab[0];
+            rp.tab_backup = h->d_backup_tab;
+            rp.bbox = h->d_bbox;
+            rp.rows = (int)rows;
+            rp.flags = h->d_res_flags;
+            rp.counters = h->counters_on ? h->d_counters : nullptr;
+            HIP_TRY(gcpm::launch_res_prep(rp, h->stream));
             HIP_TRY(gcpm::launch_resident_guard(bits != 0, ra.k, h->d_anchor, h->d_bbox, ra.lte, ra.lnt,
                                                 h->d_res_flags + 2, bad, h->stream));
             hipEvent_t e0, e1;

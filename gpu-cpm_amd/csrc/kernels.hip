@@ -2636,15 +2636,6 @@ __global__ __launch_bounds__(256) void k_bbox2d(const ST* __restrict__ lat, int 
     atomicMax(bbox + 4 * c + 3, ry);
 }
 
-__global__ void k_bbox_init(int32_t* bbox, int rows) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= rows) return;
-    bbox[4 * c] = INT_MAX;
-    bbox[4 * c + 1] = INT_MIN;
-    bbox[4 * c + 2] = INT_MAX;
-    bbox[4 * c + 3] = INT_MIN;
-}
-
 // Holder words: the tiles whose sites or halo a cell's box overlaps, the box
 // widened per axis by the largest margin up to res_margin that keeps it
 // within two tiles (res_holds).  A cell whose box with its halo alone spans
@@ -2723,10 +2714,40 @@ hipError_t launch_resident(bool narrow, int var, const ResidentArgs& a, int bloc
                                       (unsigned int)resident_lds_bytes(a.k.ncells), s);
 }
 
+// Everything a resident launch needs before its guard, in one kernel (the
+// host issues one launch instead of six copies and fills): the lattice and
+// the rows backed up, the counters' copy, the mailboxes and the error words
+// zeroed, the boxes set empty.  Unit i of the grid: a 16-byte piece of the
+// lattice copy, then of the mailboxes, then one row (copy and box), then the
+// error words and counters.
+__global__ __launch_bounds__(256) void k_res_prep(ResPrep q) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t u1 = q.lat_bytes / 16, u2 = q.nmail / 2, u3 = (size_t)q.rows;
+    if (i < u1) {
+        reinterpret_cast<uint4*>(q.lat_backup)[i] = reinterpret_cast<const uint4*>(q.lat)[i];
+    } else if (i < u1 + u2) {
+        reinterpret_cast<uint4*>(q.mail)[i - u1] = make_uint4(0u, 0u, 0u, 0u);
+    } else if (i < u1 + u2 + u3) {
+        const size_t c = i - u1 - u2;
+        q.tab_backup[c] = q.tab[c];
+        reinterpret_cast<int4*>(q.bbox)[c] = make_int4(INT_MAX, INT_MIN, INT_MAX, INT_MIN);
+    } else if (i == u1 + u2 + u3) {
+        q.flags[0] = q.flags[1] = 0u;
+        if (q.counters)
+            for (int j = 0; j < 4; j++) q.counters[4 + j] = q.counters[j];
+    }
+}
+
+hipError_t launch_res_prep(const ResPrep& q, hipStream_t s) {
+    if (q.lat_bytes % 16 != 0 || q.nmail % 2 != 0) return hipErrorInvalidValue;
+    const size_t units = q.lat_bytes / 16 + q.nmail / 2 + (size_t)q.rows + 1;
+    hipLaunchKernelGGL(k_res_prep, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s, q);
+    return hipGetLastError();
+}
+
 hipError_t launch_resident_guard(bool narrow, const SweepConsts& k, const int32_t* anchor, int32_t* bbox,
                                  int lte, int lnt, uint32_t* holders, unsigned int* bad, hipStream_t s) {
     const int rows = k.ncells + 1;
-    hipLaunchKernelGGL(k_bbox_init, dim3((rows + 255) / 256), dim3(256), 0, s, bbox, rows);
     if (k.lgL < 7) return hipErrorInvalidValue;  // (the resident form's lattices: 128^2 and up)
     const uint32_t nsites = 1u << (2 * k.lgL);
     if (narrow)
