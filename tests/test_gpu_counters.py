@@ -89,3 +89,28 @@ def test_counters_per_engine_in_a_batch_and_reset():
     s.run(**run_kwargs(1, 1, 3))
     s.synchronize()
     assert s.counters()["differing"] == after["differing"]
+
+
+def test_counters_survive_redone_resident_launches():
+    """A resident launch that fails its guard is redone per phase: the
+    counters go back to their state before the launch and count the redone
+    MCS once, so the totals still equal the oracle's."""
+    from gpucpm import workloads as W
+    wl = W.sorting_small_cells(L=256, seed=404)
+    print("CASE counters redone launches", wl.L, flush=True)
+    sim = wl.make_cpm()
+    sim.set_launch(4, 32)
+    sim.push_to_gpu()
+    sim.set_seed(503)
+    sim.set_counters(True)
+    sim.run(**run_kwargs(1, 1, 40, 40))
+    sim.synchronize()
+    got = sim.counters()
+    assert sim.resident_fallbacks[0] >= 1
+    sim.pull_from_gpu()
+    counts = np.zeros(4, np.int64)
+    lat_o, tab_o = run_oracle(wl, 1, 1, 503, 40, counts=counts)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+    assert got["attempts"] == wl.L ** 2 * 40
+    assert [got[k] for k in ("differing", "evaluated", "accepted", "committed")] == counts.tolist()
