@@ -230,8 +230,8 @@ const char* cpm_last_kernel(cpm_handle h);
 /* Extension: choose the launch form (-1 = automatic, else a form number as
  * above, 3 = one workgroup with the lattice in LDS, 4 = resident 2D tiles
  * that wait only on their neighbours between colour phases) and the LDS tile
- * edge (0 = automatic) for this engine's runs; the autotuner sets these.
- * Results never depend on them. */
+ * edge (0 = automatic; form 4 takes 32 or 64) for this engine's runs; the
+ * autotuner sets these.  Results never depend on them. */
 int cpm_set_launch(cpm_handle h, int form, int tile);
 /* Extension: chunks of resident-form runs (launch form 4) that were redone
  * with one launch per phase: *guard counts those where a cell spread over
