@@ -2322,6 +2322,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // read from this parity)
         const uint32_t par_in = ((p - 1) & 1u) * 8u * mstride;
         unsigned long long v[WPT];
+        ResEntry own{};  // this thread's first attempt of the phase (queue entry tid)
         uint32_t got = p > 0 ? 0u : need;
         unsigned int* const nq = &sh.ctl[p & 1u].x;
         unsigned int* const stop = &sh.ctl[(p + 1) & 1u].y;  // (raised in this phase)
@@ -2445,19 +2446,24 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             if (tid == 0) sh.ctl[(p + 1) & 1u].x = 0;  // (last read in phase p - 1)
             // the previous phase's deltas are sent and in the rows
             for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
-            // prepare the interior attempts while the packets travel
+            // prepare the interior attempts while the packets travel; a
+            // thread's first attempt stays in its registers for the commit
             for (int qi = tid; qi < nqp; qi += NT) {
                 // (every field in one round of loads, the ring test after)
                 ResEntry qe = sh.queue.get(qi);
-                asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.meta));
-                if (qe.li & kResRing) continue;
-                if constexpr (LEAN)
-                    res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
-                else
-                    res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
-                sh.queue.vs[qi] = qe.vs;
-                sh.queue.meta[qi] = qe.meta;
-                sh.queue.e[qi] = qe.e;
+                asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.w2), "v"(qe.meta));
+                if (!(qe.li & kResRing)) {
+                    if constexpr (LEAN)
+                        res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
+                    else
+                        res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+                    if (qi != tid) {
+                        sh.queue.vs[qi] = qe.vs;
+                        sh.queue.meta[qi] = qe.meta;
+                        sh.queue.e[qi] = qe.e;
+                    }
+                }
+                if (qi == tid) own = qe;
             }
         }
         if (p > 0) {
@@ -2523,8 +2529,12 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         const TileTables T{map, sh.R, sh.D[p & 1u], stop};
         bool miss = false;
         for (int qi = tid; qi < nqp; qi += NT) {
-            ResEntry qe = sh.queue.get(qi);
-            if constexpr (LEAN) asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.w2), "v"(qe.meta), "v"(qe.e));
+            ResEntry qe = own;
+            if (qi != tid) {
+                qe = sh.queue.get(qi);
+                if constexpr (LEAN)
+                    asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.w2), "v"(qe.meta), "v"(qe.e));
+            }
             if (qe.li & kResRing) {
                 const uint32_t li = qe.li & ~kResRing;
                 int ox, oy, oz;
