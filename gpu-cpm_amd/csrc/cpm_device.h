@@ -261,8 +261,9 @@ hipError_t launch_small(int d, bool narrow, const SmallArgs& a, hipStream_t s, h
 // The resident form: dynamic LDS bytes (the cell map), co-resident block
 // limit and launch; var selects the kernel: 0 the lean attempt path (at
 // most four types, no fixed types, chemotaxis or counters, a threshold
-// table of at most kResThrLean entries), 1 the general one, 2 the general
-// one with the chemotaxis term.
+// table of at most kResThrLean entries) for c = 1 with shared roles, 3 the
+// lean attempt path for any other schedule, 1 the general one, 2 the
+// general one with the chemotaxis term.
 constexpr int kResThrLean = 1024;
 size_t resident_lds_bytes(int ncells);
 hipError_t resident_max_blocks(bool narrow, int lte, int var, int ncells, int* max_blocks);

@@ -1390,10 +1390,12 @@ int cpm_run(cpm_handle h, const cpm_run_args* a) {
         if (h->want_form == Plan::kResident && (h->want_tile == 32 || h->want_tile == 64) &&
             h->lgL - (h->want_tile == 64 ? 6 : 5) >= 2)
             res_lte = h->want_tile == 64 ? 6 : 5;
+        // (the lean path: variant 0 for c = 1 with shared roles, 3 for the
+        // other schedules)
         res_var = k.chem ? 2
                          : (k.nt <= gcpm::kSmallTypes && !k.fixed_mask && !h->counters_on &&
                             k.nthr <= gcpm::kResThrLean)
-                               ? 0
+                               ? (k.lc == 0 && !k.roles ? 0 : 3)
                                : 1;
         int maxb = 0;
         HIP_TRY(gcpm::resident_max_blocks(h->narrow_bits() != 0, res_lte, res_var, h->n, &maxb));
@@ -1885,15 +1887,15 @@ const char* cpm_last_kernel(cpm_handle h) {
         case Plan::kGlobal: return h->d == 3 ? "k_persist_global<3>" : "k_persist_global<2>";
         case Plan::kLds2d: return "k_persist_lds2d";
         case Plan::kResident: {
-            static const char* const names[2][2][3] = {
+            static const char* const names[2][2][4] = {
                 {{"k_resident2d<unsigned int, 5, 0>", "k_resident2d<unsigned int, 5, 1>",
-                  "k_resident2d<unsigned int, 5, 2>"},
+                  "k_resident2d<unsigned int, 5, 2>", "k_resident2d<unsigned int, 5, 3>"},
                  {"k_resident2d<unsigned int, 6, 0>", "k_resident2d<unsigned int, 6, 1>",
-                  "k_resident2d<unsigned int, 6, 2>"}},
+                  "k_resident2d<unsigned int, 6, 2>", "k_resident2d<unsigned int, 6, 3>"}},
                 {{"k_resident2d<unsigned short, 5, 0>", "k_resident2d<unsigned short, 5, 1>",
-                  "k_resident2d<unsigned short, 5, 2>"},
+                  "k_resident2d<unsigned short, 5, 2>", "k_resident2d<unsigned short, 5, 3>"},
                  {"k_resident2d<unsigned short, 6, 0>", "k_resident2d<unsigned short, 6, 1>",
-                  "k_resident2d<unsigned short, 6, 2>"}}};
+                  "k_resident2d<unsigned short, 6, 2>", "k_resident2d<unsigned short, 6, 3>"}}};
             return names[h->small_narrow][h->last_plan.ltile == 6][h->res_var];
         }
         case Plan::kSmall:

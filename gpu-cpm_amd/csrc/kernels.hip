@@ -1801,10 +1801,11 @@ struct ResGeom {
 };
 
 // Threads per tile of kernel variant VAR: one attempt each on the lean path
-// (at most 128 VGPRs: 16 waves per CU), two on the general one.
+// (variants 0 and 3; at most 128 VGPRs: 16 waves per CU), two on the
+// general one.
 template <int LTE, int VAR>
 constexpr int res_threads() {
-    return (1 << (2 * LTE)) / (VAR == 0 ? 4 : 8);
+    return (1 << (2 * LTE)) / (VAR == 0 || VAR == 3 ? 4 : 8);
 }
 
 // Neighbour direction d = 0..7: the 3 x 3 block of tiles without its centre,
@@ -2166,7 +2167,9 @@ __device__ __forceinline__ uint4 res_candidate_draws(const SweepConsts& k, const
     return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-template <class ST, int LTE, int NT, bool LEAN, bool XT, class PV>
+// GEN: 0 for c = 1 with shared roles (pairs of same-colour sites), 1 for any
+// other schedule (candidate sub-blocks), -1 to take it from the arguments.
+template <class ST, int LTE, int NT, bool LEAN, bool XT, int GEN, class PV>
 __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P, ResShared<ST, LTE, NT>& sh,
                                              uint8_t* map) {
     using G = ResGeom<LTE>;
@@ -2175,6 +2178,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     constexpr int APT = G::NQ / NT;          // attempts per thread and phase (c = 1, shared roles)
     constexpr int NPAIR = G::NQ / 2;         // pairs of same-colour sites sharing a Philox block
     const SweepConsts& k = a.k;
+    const bool gen = GEN < 0 ? a.gen != 0 : GEN == 1;
     const int tid = (int)threadIdx.x;
     const int b = (int)blockIdx.x;
     const int ntm = (1 << a.lnt) - 1;
@@ -2310,12 +2314,15 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     const int pair = APT == 2 ? tid : tid >> 1, hs = APT == 2 ? 0 : tid & 1;
     const int j = pair / (TE / 4), jc = pair % (TE / 4);
     const unsigned int Gp = (unsigned int)(a.nmcs * k.rounds * k.phases_per_round);
-    if (!a.gen) {
+    if (!gen) {
         for (int q = tid; q < NPAIR; q += NT) sh.draws[q] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, q);
     } else {
         const ResSched r = res_sched<LTE>(k, a.mcs0, 0u, x0, y0);
         if (tid < r.ncx * r.ncy) sh.draws[tid] = res_candidate_draws(k, r, tid, Lm);
     }
+    // (with one attempt per thread, thread t draws with pair t / 2, which
+    // another wave wrote)
+    __syncthreads();
     for (unsigned int p = 0;; p++) {
         const bool work = p < Gp;  // p == Gp: receive the last phase's deltas only
         // (the neighbours' packets of phase p - 1, halo sites and deltas, are
@@ -2367,7 +2374,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                 act = e.vs != e.vt;  // kernels2d.cu:276
             }
         };
-        if (work && !a.gen) {
+        if (work && !gen) {
             // c = 1, shared roles: each thread takes an x-neighbouring pair of
             // same-colour sites, whose Philox block was made in the previous
             // phase
@@ -2517,7 +2524,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // makes the next phase's draws: two pairs of sites each (one with a
         // thread per attempt), or the candidates.
         if (tid >= NT / 2 && p + 1 < Gp) {
-            if (!a.gen) {
+            if (!gen) {
                 for (int q = tid - NT / 2; q < NPAIR; q += NT / 2)
                     sh.draws[q] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, p + 1, q);
             } else {
@@ -2595,8 +2602,9 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     }
 }
 
-// VAR 0: the lean attempt path; 1: the general one; 2: the general one
-// with the chemotaxis term.
+// VAR 0: the lean attempt path for c = 1 with shared roles; 3: the lean
+// attempt path for any other schedule; 1: the general one; 2: the general
+// one with the chemotaxis term.
 // (one workgroup per CU: the compiler may schedule for four waves per SIMD
 // rather than the most it could fit)
 template <class ST, int LTE, int VAR>
@@ -2605,16 +2613,17 @@ __global__ __launch_bounds__((res_threads<LTE, VAR>())) __attribute__((amdgpu_wa
     constexpr int NT = res_threads<LTE, VAR>();
     extern __shared__ __align__(16) unsigned char res_map[];
     __shared__ ResShared<ST, LTE, NT> sh;
-    if (VAR == 0 || a.k.nt <= kSmallTypes) {
+    constexpr bool LEAN = VAR == 0 || VAR == 3;
+    if (LEAN || a.k.nt <= kSmallTypes) {
         SmallParams P;
         P.load(a.k.prm);
-        resident_run<ST, LTE, NT, VAR == 0, VAR == 2>(a, P, sh, res_map);
+        resident_run<ST, LTE, NT, LEAN, VAR == 2, VAR == 0 ? 0 : VAR == 3 ? 1 : -1>(a, P, sh, res_map);
     } else {
         const int npar = param_words(a.k.nt);
         for (int i = (int)threadIdx.x; i < npar; i += NT) sh.prm[i] = a.k.prm->w[i];
         __syncthreads();
         const LdsParamView P{sh.prm, a.k.nt, a.k.nt * a.k.nt};
-        resident_run<ST, LTE, NT, false, VAR == 2>(a, P, sh, res_map);
+        resident_run<ST, LTE, NT, false, VAR == 2, -1>(a, P, sh, res_map);
     }
 }
 
@@ -2687,8 +2696,12 @@ __global__ void k_res_holders(const int32_t* __restrict__ bbox, const int32_t* _
 
 template <class ST, int LTE>
 static const void* resident_kernel(int var) {
-    return var == 0 ? (const void*)&k_resident2d<ST, LTE, 0>
-                    : var == 1 ? (const void*)&k_resident2d<ST, LTE, 1> : (const void*)&k_resident2d<ST, LTE, 2>;
+    switch (var) {
+        case 0: return (const void*)&k_resident2d<ST, LTE, 0>;
+        case 1: return (const void*)&k_resident2d<ST, LTE, 1>;
+        case 2: return (const void*)&k_resident2d<ST, LTE, 2>;
+        default: return (const void*)&k_resident2d<ST, LTE, 3>;
+    }
 }
 
 static const void* resident_kernel(bool narrow, int lte, int var) {
@@ -2699,7 +2712,7 @@ static const void* resident_kernel(bool narrow, int lte, int var) {
 size_t resident_lds_bytes(int ncells) { return ((size_t)ncells + 1 + 15) & ~(size_t)15; }
 
 int resident_threads(int lte, int var) {
-    if (var == 0) return lte == 6 ? res_threads<6, 0>() : res_threads<5, 0>();
+    if (var == 0 || var == 3) return lte == 6 ? res_threads<6, 0>() : res_threads<5, 0>();
     return lte == 6 ? res_threads<6, 1>() : res_threads<5, 1>();
 }
 

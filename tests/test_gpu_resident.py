@@ -130,6 +130,24 @@ def test_resident_general_schedules_match_oracle(idx):
     assert launch["tile"] == (tile or 64), launch
 
 
+@pytest.mark.parametrize("case", ["plain128", "sorting256"])
+def test_many_short_launches_match_oracle(case):
+    """Launches of one MCS each: every launch's first phase draws with the
+    Philox blocks made at its set-up (with one attempt per thread, thread t
+    reads the block of pair t / 2, which another wave wrote), bit-exact."""
+    from gpucpm import workloads as W
+    wl = W.plain2d(L=128, ncells=60, seed=411) if case == "plain128" else W.sorting2d(L=256, grid=20, edge=10,
+                                                                                       seed=412)
+    _announce("short launches", case, wl.L)
+    n = 12
+    sim = _run(wl, 515, n, 1)
+    assert sim.last_launch()["form"] == 4
+    lat_o, tab_o = run_oracle(wl, 1, 1, 515, n)
+    np.testing.assert_array_equal(sim.get_state().view(np.uint32), lat_o)
+    assert_tables_equal(sim.get_cell_table(), tab_o, 2)
+    assert sim.resident_fallbacks == (0, 0)
+
+
 def test_guard_sends_a_spread_cell_back_to_per_phase_launches():
     """One cell of 70 x 70 sites at 256^2 (tiles of 32): it spans three tiles,
     so the chunk is restored and run with one launch per phase, exactly."""
