@@ -1896,6 +1896,10 @@ template <class ST, int LTE, int NT>
 struct ResShared {
     using G = ResGeom<LTE>;
     ST tile[G::S * G::S];
+    // 1 where this phase's packets changed a halo site (the interior stays
+    // 0), so a ring attempt prepared before they arrived is known to stand
+    // (lean path, c = 1 with shared roles).
+    uint8_t hdirty[G::S * G::S];
     int64_t R[256];
     int64_t D[2][256];           // deltas of the phases of each parity, by slot
     uint32_t cell[kResSlots];
@@ -2179,6 +2183,9 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     constexpr int NPAIR = G::NQ / 2;         // pairs of same-colour sites sharing a Philox block
     const SweepConsts& k = a.k;
     const bool gen = GEN < 0 ? a.gen != 0 : GEN == 1;
+    // The ring's attempts prepared before the packets arrive (lean path, c = 1
+    // with shared roles; measured slower for the other schedules)
+    constexpr bool EARLY = LEAN && GEN == 0;
     const int tid = (int)threadIdx.x;
     const int b = (int)blockIdx.x;
     const int ntm = (1 << a.lnt) - 1;
@@ -2190,6 +2197,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
     for (int i = tid; i < S * S; i += NT) {
         const int ly = i / S, lx = i - ly * S;
         sh.tile[i] = glat[(uint32_t)((x0 + lx - 1) & Lm) | ((uint32_t)((y0 + ly - 1) & Lm) << k.lgL)];
+        sh.hdirty[i] = 0;
     }
     const bool thr_lds = k.nthr <= kThrLds;
     if (thr_lds)
@@ -2302,6 +2310,18 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                 win[j] = wout[j] = WD::delta(sh.dl[d][i - (corner ? 1 : WD::EDGE)]);
             }
             need |= 1u << j;
+        }
+    }
+    // the halo sites of each word as last received (loaded at the start)
+    uint32_t prev[WPT];
+#pragma unroll
+    for (int jj = 0; jj < WPT; jj++) {
+        prev[jj] = 0;
+        const uint32_t x = win[jj];
+        if (EARLY && (need >> jj & 1u) && !(x >> 31)) {
+            const int li = (int)(x & 0xFFFFu), stride = (int)((x >> 16) & 0xFFFu);
+            prev[jj] = (uint32_t)sh.tile[li];
+            if (SPW == 2 && (x >> 28) > 1u) prev[jj] |= (uint32_t)sh.tile[li + stride] << 16;
         }
     }
     const ResLat<ST, LTE> lat{sh.tile, x0, y0, k.lgL};
@@ -2459,13 +2479,30 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                 // (every field in one round of loads, the ring test after)
                 ResEntry qe = sh.queue.get(qi);
                 asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.w2), "v"(qe.meta));
-                if (!(qe.li & kResRing)) {
-                    if constexpr (LEAN)
-                        res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
-                    else
-                        res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+                const bool ring = (qe.li & kResRing) != 0;
+                if (EARLY || !ring) {
+                    bool same = false;
+                    if (ring) {
+                        // the ring's attempts too, on the halo as it stands
+                        // (the commit redoes one whose 3 x 3 block holds a
+                        // halo site that the packets change)
+                        const int li = (int)(qe.li & ~kResRing);
+                        int ox, oy, oz;
+                        block_offset<2>((int)(qe.meta & 0xFu), ox, oy, oz);
+                        qe.vt = sh.tile[li];
+                        qe.vs = sh.tile[li + oy * S + ox];
+                        same = qe.vs == qe.vt;  // (nothing to commit unless redone)
+                        if (same) qe.meta &= 0xFu;
+                    }
+                    if (!same) {
+                        if constexpr (LEAN)
+                            res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
+                        else
+                            res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+                    }
                     if (qi != tid) {
                         sh.queue.vs[qi] = qe.vs;
+                        sh.queue.vt[qi] = qe.vt;
                         sh.queue.meta[qi] = qe.meta;
                         sh.queue.e[qi] = qe.e;
                     }
@@ -2512,6 +2549,12 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
                         const int li = (int)(x & 0xFFFFu), stride = (int)((x >> 16) & 0xFFFu);
                         sh.tile[li] = (ST)(SPW == 2 ? pl & 0xFFFFu : pl);
                         if (SPW == 2 && (x >> 28) > 1u) sh.tile[li + stride] = (ST)(pl >> 16);
+                        if constexpr (EARLY) {
+                            const uint32_t ch = pl ^ prev[jj];
+                            sh.hdirty[li] = (uint8_t)((SPW == 2 ? ch & 0xFFFFu : ch) != 0u);
+                            if (SPW == 2 && (x >> 28) > 1u) sh.hdirty[li + stride] = (uint8_t)((ch >> 16) != 0u);
+                            prev[jj] = pl;
+                        }
                     }
                 }
             }
@@ -2544,16 +2587,23 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             }
             if (qe.li & kResRing) {
                 const uint32_t li = qe.li & ~kResRing;
-                int ox, oy, oz;
-                block_offset<2>((int)(qe.meta & 0xFu), ox, oy, oz);
-                qe.vt = sh.tile[li];
-                qe.vs = sh.tile[(int)li + oy * S + ox];
+                bool redo = true;
+                if constexpr (EARLY) {  // prepared before the packets: redone if its halo changed
+                    const uint8_t* hd = sh.hdirty + li;
+                    redo = (hd[-S - 1] | hd[-S] | hd[-S + 1] | hd[-1] | hd[1] | hd[S - 1] | hd[S] | hd[S + 1]) != 0;
+                }
                 qe.li = li;
-                if (qe.vs == qe.vt) continue;  // kernels2d.cu:276
-                if constexpr (LEAN)
-                    res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
-                else
-                    res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+                if (redo) {
+                    int ox, oy, oz;
+                    block_offset<2>((int)(qe.meta & 0xFu), ox, oy, oz);
+                    qe.vt = sh.tile[li];
+                    qe.vs = sh.tile[(int)li + oy * S + ox];
+                    if (qe.vs == qe.vt) continue;  // kernels2d.cu:276
+                    if constexpr (LEAN)
+                        res_prepare_lean<ST, LTE>(sh.tile, map, sh.p4, k.idbits, k.idmask, qe);
+                    else
+                        res_prepare<ST, LTE, XT>(lat, k, P, x0, y0, qe);
+                }
             }
             if constexpr (LEAN)
                 res_commit_lean<ST, LTE>(sh.tile, sh.R, sh.D[p & 1u], sh.thr, k.nthr, sh.p4, k.idbits,
