@@ -1912,7 +1912,7 @@ struct ResShared {
     ResParams4 p4;               // the lean path's parameters
     int scan[NT];
     ResQueue<G::NQ> queue;       // this phase's attempts whose source and target differ, and the ring's
-    uint4 draws[NT];             // the next phase's Philox blocks: of each pair of same-colour sites, or candidate
+    uint4 draws[NT];             // the next phase's Philox blocks of each pair of same-colour sites, or staged candidates
     // Per phase parity: the queue length, and what stops the launch (1: an
     // attempt met a cell the tile does not hold; 2: a wait gave up), raised
     // in the phase before and read with the length after the first barrier.
@@ -2127,7 +2127,6 @@ __device__ __forceinline__ uint4 res_pair_draws(const SweepConsts& k, int x0, in
 struct ResSched {
     uint32_t mcs, phase;
     int fx, fy, ncx, ncy, lsbs;
-    uint32_t rcx;  // ceil(2^20 / ncx): iy = (ci * rcx) >> 20 for ci < 2^10
 };
 
 template <int LTE>
@@ -2150,25 +2149,39 @@ __device__ __forceinline__ ResSched res_sched(const SweepConsts& k, uint32_t mcs
     r.fy = y0 - c + ((oy * c - y0 + c) & sbm);
     r.ncx = max(0, ((x0 + TE - r.fx) >> r.lsbs) + 1);
     r.ncy = max(0, ((y0 + TE - r.fy) >> r.lsbs) + 1);
-    r.rcx = r.ncx ? ((1u << 20) + (uint32_t)r.ncx - 1u) / (uint32_t)r.ncx : 0u;
     return r;
 }
 
-// Candidate ci of a phase: its sub-block and the Philox words of its
-// attempt (attempt_words).
-__device__ __forceinline__ void res_candidate(const SweepConsts& k, const ResSched& r, int ci, int Lm, int& bx,
-                                              int& by) {
-    const int iy = (int)(((uint32_t)ci * r.rcx) >> 20), ix = ci - iy * r.ncx;
-    bx = ((r.fx + (ix << r.lsbs)) & Lm) >> k.lc;
-    by = ((r.fy + (iy << r.lsbs)) & Lm) >> k.lc;
-}
-
-__device__ __forceinline__ uint4 res_candidate_draws(const SweepConsts& k, const ResSched& r, int ci, int Lm) {
-    int bx, by;
-    res_candidate(k, r, ci, Lm, bx, by);
+// Candidate ci of a phase (ci < ncx ncy) as the tile at (x0, y0) stages it:
+// its sub-block's attempt (attempt_words: position, direction and roles as
+// the per-phase kernels draw them) and, when the target lies in the tile at
+// (lx, ly) (1 .. TE), (1 << 31 | dir << 16 | ly << 8 | lx, w2); (0, 0)
+// otherwise.
+template <int LTE>
+__device__ __forceinline__ uint2 res_candidate_target(const SweepConsts& k, const ResSched& r, int ci, int Lm,
+                                                      int x0, int y0) {
+    constexpr int TE = 1 << LTE;
+    const int iy = ci / r.ncx, ix = ci - iy * r.ncx;
+    const int bx = ((r.fx + (ix << r.lsbs)) & Lm) >> k.lc;
+    const int by = ((r.fy + (iy << r.lsbs)) & Lm) >> k.lc;
     uint32_t w[4];
     attempt_words(k, sub_block_id<2>(k, bx, by, 0), bx, r.mcs, r.phase, 0u, w);
-    return make_uint4(w[0], w[1], w[2], w[3]);
+    const int lc = k.lc, cm = (1 << lc) - 1;
+    int px = (bx << lc) + (int)(w[0] & (uint32_t)cm);
+    int py = (by << lc) + (int)((w[0] >> lc) & (uint32_t)cm);
+    int dir = (int)(((uint64_t)w[1] * 8ull) >> 32);
+    dir += dir >= 4;
+    if (k.roles) {  // the drawn site is the source, its neighbour the target
+        int ddx, ddy, ddz;
+        block_offset<2>(dir, ddx, ddy, ddz);
+        px = (px + ddx) & Lm;
+        py = (py + ddy) & Lm;
+        dir = 8 - dir;
+    }
+    const int lx = ((px - x0) & Lm) + 1, ly = ((py - y0) & Lm) + 1;
+    return lx <= TE && ly <= TE
+               ? make_uint2((1u << 31) | ((uint32_t)dir << 16) | ((uint32_t)ly << 8) | (uint32_t)lx, w[2])
+               : make_uint2(0u, 0u);
 }
 
 // GEN: 0 for c = 1 with shared roles (pairs of same-colour sites), 1 for any
@@ -2338,7 +2351,10 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         for (int q = tid; q < NPAIR; q += NT) sh.draws[q] = res_pair_draws<LTE>(k, x0, y0, a.mcs0, 0u, q);
     } else {
         const ResSched r = res_sched<LTE>(k, a.mcs0, 0u, x0, y0);
-        if (tid < r.ncx * r.ncy) sh.draws[tid] = res_candidate_draws(k, r, tid, Lm);
+        if (tid < r.ncx * r.ncy) {
+            const uint2 t = res_candidate_target<LTE>(k, r, tid, Lm, x0, y0);
+            sh.draws[tid] = make_uint4(t.x, t.y, 0u, 0u);
+        }
     }
     // (with one attempt per thread, thread t draws with pair t / 2, which
     // another wave wrote)
@@ -2420,32 +2436,25 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             // (with non-shared roles the target may be a sub-block's
             // neighbour site; the attempt then belongs to the target's tile,
             // which holds all the sites the attempt reads).  Candidates: the
-            // active sub-blocks within c sites of the tile; their Philox
-            // words were made in the previous phase.
+            // active sub-blocks within c sites of the tile; the first NT
+            // were staged in the previous phase (target and direction).
             const ResSched r = res_sched<LTE>(k, a.mcs0, p, x0, y0);
-            const int lc = k.lc, cm = (1 << lc) - 1;
             const int ncand = r.ncx * r.ncy;
             for (int c0 = 0; c0 < ncand; c0 += NT) {
                 const int ci = c0 + tid;
                 ResEntry e[2];
                 bool act[2] = {false, false};
                 if (ci < ncand) {
-                    int bx, by;
-                    res_candidate(k, r, ci, Lm, bx, by);
-                    const uint4 w = ci < NT ? sh.draws[ci] : res_candidate_draws(k, r, ci, Lm);
-                    int px = (bx << lc) + (int)(w.x & (uint32_t)cm);
-                    int py = (by << lc) + (int)((w.x >> lc) & (uint32_t)cm);
-                    int dir = (int)(((uint64_t)w.y * 8ull) >> 32);
-                    dir += dir >= 4;
-                    if (k.roles) {  // the drawn site is the source, its neighbour the target
-                        int ddx, ddy, ddz;
-                        block_offset<2>(dir, ddx, ddy, ddz);
-                        px = (px + ddx) & Lm;
-                        py = (py + ddy) & Lm;
-                        dir = 8 - dir;
+                    uint2 t;
+                    if (ci < NT) {
+                        const uint4 d = sh.draws[ci];
+                        t = make_uint2(d.x, d.y);
+                    } else {
+                        t = res_candidate_target<LTE>(k, r, ci, Lm, x0, y0);
                     }
-                    const int lx = ((px - x0) & Lm) + 1, ly = ((py - y0) & Lm) + 1;
-                    if (lx <= TE && ly <= TE) stage(lx, ly, dir, w.z, e[0], act[0]);
+                    if (t.x >> 31)
+                        stage((int)(t.x & 0xFFu), (int)((t.x >> 8) & 0xFFu), (int)((t.x >> 16) & 0xFu), t.y, e[0],
+                              act[0]);
                 }
                 enqueue(e, act);
             }
@@ -2565,7 +2574,7 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // queued attempt's rows, acceptance and commit
         // The upper half of the threads, idle here unless the queue is long,
         // makes the next phase's draws: two pairs of sites each (one with a
-        // thread per attempt), or the candidates.
+        // thread per attempt), or the candidates' staged attempts.
         if (tid >= NT / 2 && p + 1 < Gp) {
             if (!gen) {
                 for (int q = tid - NT / 2; q < NPAIR; q += NT / 2)
@@ -2573,7 +2582,10 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             } else {
                 const ResSched r = res_sched<LTE>(k, a.mcs0, p + 1, x0, y0);
                 const int n = min(r.ncx * r.ncy, NT);
-                for (int ci = tid - NT / 2; ci < n; ci += NT / 2) sh.draws[ci] = res_candidate_draws(k, r, ci, Lm);
+                for (int ci = tid - NT / 2; ci < n; ci += NT / 2) {
+                    const uint2 t = res_candidate_target<LTE>(k, r, ci, Lm, x0, y0);
+                    sh.draws[ci] = make_uint4(t.x, t.y, 0u, 0u);
+                }
             }
         }
         const TileTables T{map, sh.R, sh.D[p & 1u], stop};
