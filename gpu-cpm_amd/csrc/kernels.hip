@@ -2465,6 +2465,9 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         // commit and publish that followed the stop did no harm: every
         // index they used stays in bounds)
         const uint2 cq = sh.ctl[p & 1u];
+        // this thread's first queue entry, read with the length (entry tid
+        // lies within the queue's storage; it is used only below the length)
+        const ResEntry q0 = sh.queue.get(tid);
         if (cq.y) {
             if (tid == 0 && (cq.y & 1u)) atomicOr(a.bad, kResGuardMiss);
             return;
@@ -2486,7 +2489,8 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
             // thread's first attempt stays in its registers for the commit
             for (int qi = tid; qi < nqp; qi += NT) {
                 // (every field in one round of loads, the ring test after)
-                ResEntry qe = sh.queue.get(qi);
+                ResEntry qe = q0;
+                if (qi != tid) qe = sh.queue.get(qi);
                 asm volatile("" ::"v"(qe.li), "v"(qe.vs), "v"(qe.vt), "v"(qe.w2), "v"(qe.meta));
                 const bool ring = (qe.li & kResRing) != 0;
                 if (EARLY || !ring) {
