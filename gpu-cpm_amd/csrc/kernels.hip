@@ -2481,10 +2481,26 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         for (int jj = 0; jj < WPT; jj++)
             if (p > 0 && (need >> jj & 1u))
                 v[jj] = __hip_atomic_load(a.mail + wsrc[jj] + par_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gen) {
+            // the previous phase's own deltas (sent with its packets) join
+            // the rows, and their buffer is cleared for the next phase, by
+            // the highest threads (idle here unless the queue is long); the
+            // rows are next read after the second barrier.  (With c = 1 and
+            // shared roles they join right after the publish: measured the
+            // faster there, this the faster for the other schedules.)
+            for (int s = NT - 1 - tid; s < nslots; s += NT) {
+                const int64_t dv = sh.D[(p + 1) & 1u][s];
+                if (dv != 0) {
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&sh.R[s]), (unsigned long long)dv);
+                    sh.D[(p + 1) & 1u][s] = 0;
+                }
+            }
+        }
         if (work) {
             if (tid == 0) sh.ctl[(p + 1) & 1u].x = 0;  // (last read in phase p - 1)
             // the previous phase's deltas are sent and in the rows
-            for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
+            if (!gen)
+                for (int s = tid; s < nslots; s += NT) sh.D[(p + 1) & 1u][s] = 0;
             // prepare the interior attempts while the packets travel; a
             // thread's first attempt stays in its registers for the commit
             for (int qi = tid; qi < nqp; qi += NT) {
@@ -2651,10 +2667,11 @@ __device__ __forceinline__ void resident_run(const ResidentArgs& a, const PV& P,
         }
         // the tile's own deltas join the rows (the neighbours' deltas of
         // this phase arrive with atomics too, possibly at the same time)
-        for (int s = tid; s < nslots; s += NT) {
-            const int64_t dv = sh.D[p & 1u][s];
-            if (dv != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&sh.R[s]), (unsigned long long)dv);
-        }
+        if (!gen)
+            for (int s = tid; s < nslots; s += NT) {
+                const int64_t dv = sh.D[p & 1u][s];
+                if (dv != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&sh.R[s]), (unsigned long long)dv);
+            }
     }
     if (sh.ctl[(Gp + 1) & 1u].y) return;  // the last wait gave up
     // the tile's sites and the rows of its cells (every holder has the same)
